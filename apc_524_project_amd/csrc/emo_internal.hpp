@@ -1,0 +1,185 @@
+// emo_internal.hpp -- context layout and kernel entry points of libemopic.
+//
+// One emo_ctx owns: the padded field arrays on the device, the particle
+// store, per-cell scratch for the deterministic deposit, and a HIP stream on
+// which every kernel of the context is launched.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/emo_api.h"
+
+namespace emo {
+
+// reference constants (src/globals.hpp:12,16,20)
+constexpr double kUnitAcc = 0.58668774;
+constexpr double kUnitRhoJ = 4.80320427e-13;
+constexpr double kUnitVth = 0.002232;
+constexpr double kPi = 3.14159265358979323846;
+
+// Geometry of the padded grid, passed by value to every field kernel.
+struct Geo {
+  int n[3];       // physical cells
+  int nt[3];      // n+3 padded points (src/grid/grid.cpp:25-27)
+  int nc[3];      // n+2 cell-id extents nx_,ny_,nz_ (src/grid/grid.cpp:22-24)
+  double x0[3];
+  double d[3];    // Lxyz/nCell
+  double id[3];   // 1.0/d
+  long long sx, sy;  // strides of i and j in the padded arrays (k stride 1)
+  long long ntot;
+  long long ncell;
+};
+
+__host__ __device__ inline long long gidx(const Geo &g, int i, int j, int k) {
+  return ((long long)i * g.nt[1] + j) * g.nt[2] + k;
+}
+
+struct SpeciesParams {
+  double q[EMO_NSPEC];
+  double m[EMO_NSPEC];
+  int test[EMO_NSPEC];
+};
+
+// Pulse scalars evaluated on the host for one boundary plane fill.
+struct PlaneFill {
+  double *f;      // field array (member pointer)
+  int dim;        // plane normal
+  int idx;        // plane index along dim
+  double base;    // E0/B0 component
+  double wave;    // pulse value added on top
+  double waist;   // <=0: uniform
+  double off[3];  // staggering offsets of this field (Grid::getGridPhys)
+};
+
+// FP64 particle store (structure of arrays).
+struct ParticlesF64 {
+  double *x[3] = {nullptr, nullptr, nullptr};
+  double *v[3] = {nullptr, nullptr, nullptr};
+  double *gam = nullptr;
+  double *fld[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int32_t *sp = nullptr;
+  int64_t *id = nullptr;
+};
+
+}  // namespace emo
+
+struct emo_ctx {
+  emo_config cfg;
+  emo::Geo geo;
+  emo::SpeciesParams species;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // fields, FP64. Bfix/Btfix are the fixed allocations that the reference's
+  // Grid::fieldPtr_ keeps (src/grid/grid.cpp:321-342); B/Bt are the member
+  // pointers that evolveFields swaps (src/grid/oGrid.cpp:18-20).
+  double *E[3] = {};
+  double *Bfix[3] = {};
+  double *Btfix[3] = {};
+  double *B[3] = {};
+  double *Bt[3] = {};
+  double *J[3] = {};
+  double *rho = nullptr;
+
+  // particles
+  int64_t np = 0;
+  int64_t cap = 0;
+  emo::ParticlesF64 p, palt;
+  uint32_t *key = nullptr, *key_alt = nullptr;
+  uint32_t *perm = nullptr, *perm_alt = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int sort_bits = 0;
+  int32_t *cbeg = nullptr, *cend = nullptr;  // per cell [beg,end) after the sort
+  double *cj = nullptr;                      // 12 per cell
+  double *cr = nullptr;                      // 8 per cell
+  bool sorted = false;
+
+  // reductions / flags
+  double *red = nullptr;   // scratch for FP64 reductions
+  int64_t red_len = 0;
+  unsigned long long *dflag = nullptr;  // [0] error flags, [1] max bits, ...
+  unsigned long long *hflag = nullptr;  // pinned host mirror
+
+  // timing
+  bool timing = false;
+  struct Rec {
+    hipEvent_t a, b;
+  };
+  std::map<std::string, std::vector<Rec>> pending;
+  std::map<std::string, std::pair<double, int64_t>> totals;
+};
+
+namespace emo {
+
+// error flag bits written by kernels into dflag[0]
+constexpr unsigned long long kFlagOutOfGrid = 1ull;
+constexpr unsigned long long kFlagNonFinite = 2ull;
+
+// ---- plain FP64 particle kernels (plain_kernels.hip) ----
+void launch_gather_f64(emo_ctx *c);
+void launch_push_f64(emo_ctx *c, double dt);
+void launch_particle_bc_f64(emo_ctx *c);
+int sort_f64(emo_ctx *c);
+void launch_deposit_f64(emo_ctx *c, int rho);
+void launch_cell_ids(emo_ctx *c, int32_t *out);
+size_t sort_tmp_bytes_for(int64_t n, int bits);
+
+// ---- field kernels (field_kernels.hip) ----
+void launch_fold_f64(emo_ctx *c);
+void launch_evolve_f64(emo_ctx *c, double dt);
+void launch_plane_copy_f64(emo_ctx *c, double *const *fields, int nf, int d, int src, int dst,
+                           int add);
+void launch_plane_fill_f64(emo_ctx *c, const PlaneFill &pf);
+void launch_const_fill_f64(emo_ctx *c, double *f, double v, long long n);
+void launch_jacobi_f64(emo_ctx *c, const double *src, double *dst, const double *R, double ax,
+                       double ay, double az, double af, double sourceMult);
+void launch_residual_f64(emo_ctx *c, const double *u1, const double *u2, double *partials,
+                         int nblocks);
+int residual_blocks(emo_ctx *c);
+void launch_phi_to_e_f64(emo_ctx *c, const double *phi);
+void launch_a_to_b_f64(emo_ctx *c, const double *Ax, const double *Ay, const double *Az);
+void launch_diag_particles_f64(emo_ctx *c, double *partials, int nblocks);
+void launch_diag_fields_f64(emo_ctx *c, double *partials, int nblocks);
+void launch_reduce_partials(emo_ctx *c, const double *partials, int nblocks, int ncomp,
+                            double *out);
+
+// ---- FP32 chosen form (chosen.hip) ----
+int fp32_create(emo_ctx *c);
+void fp32_destroy(emo_ctx *c);
+int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const double *gamma,
+              const int32_t *species, const int64_t *id);
+int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species,
+                       int64_t *id);
+int fp32_get_cells(emo_ctx *c, int32_t *cells);
+int fp32_get_field(emo_ctx *c, int fid, double *out);
+int fp32_set_field(emo_ctx *c, int fid, const double *in);
+int fp32_init_constant(emo_ctx *c);
+int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps, double *resid);
+int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt);
+int fp32_diag(emo_ctx *c, double *out8);
+
+// timing helpers (emo_api.cpp)
+struct TimedLaunch {
+  emo_ctx *c;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedLaunch(emo_ctx *ctx, const char *nm);
+  ~TimedLaunch();
+};
+
+}  // namespace emo
+
+#define EMO_HIP_CHECK_RET(ctx, expr)                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);             \
+      return EMO_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)

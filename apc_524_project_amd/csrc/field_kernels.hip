@@ -1,0 +1,483 @@
+// field_kernels.hip -- FP64 grid kernels for gfx950: Yee E/B stencils, ghost
+// plane fold/refresh, LightBC plane fill, Jacobi sweeps, phiToE/AToB and the
+// FP64 diagnostic reductions.
+//
+// All stencils read the padded (n+3)^3 arrays with z fastest, so a wavefront
+// covers 64 consecutive k and every load coalesces.  Operation order follows
+// the reference expressions exactly (built with -ffp-contract=off).
+#include "emo_internal.hpp"
+
+namespace emo {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(long long n, int block = kBlock) {
+  long long g = (n + block - 1) / block;
+  return (unsigned)(g > 0 ? g : 1);
+}
+
+// decode a linear index over the physical box [1..n]^3 (k fastest)
+__device__ inline void phys_ijk(const Geo &g, long long t, int *i, int *j, int *k) {
+  *k = 1 + (int)(t % g.n[2]);
+  long long r = t / g.n[2];
+  *j = 1 + (int)(r % g.n[1]);
+  *i = 1 + (int)(r / g.n[1]);
+}
+
+// ---------------------------------------------------------------------------
+// Fold, Grid::executeBC(-2) -> BC_F_Periodic::completeBC(-2)
+// (src/boundaries/b_fields/bc_f_periodic.cpp:40-46): add the upper ghost
+// plane n_d+1 of (J_j, J_k, rho) into plane 1 over the physical range of the
+// other two dims (src/grid/spookyGrid.cpp:54-72, 286-425).
+// ---------------------------------------------------------------------------
+struct PlaneArgs {
+  Geo g;
+  double *f[6];
+  int nf;
+  int d, src, dst, add;
+};
+
+__global__ void __launch_bounds__(kBlock) k_plane_copy(PlaneArgs a) {
+  const Geo &g = a.g;
+  int d = a.d;
+  int u = d == 0 ? 1 : 0;
+  int w = d == 2 ? 1 : 2;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long np = (long long)g.n[u] * g.n[w];
+  if (t >= np) return;
+  int pu = 1 + (int)(t / g.n[w]);
+  int pw = 1 + (int)(t % g.n[w]);
+  int is[3], id[3];
+  is[d] = a.src;
+  id[d] = a.dst;
+  is[u] = pu;
+  id[u] = pu;
+  is[w] = pw;
+  id[w] = pw;
+  long long ps = gidx(g, is[0], is[1], is[2]);
+  long long pd = gidx(g, id[0], id[1], id[2]);
+  for (int f = 0; f < a.nf; f++) {
+    double val = a.f[f][ps];
+    if (a.add) a.f[f][pd] += val;
+    else a.f[f][pd] = val;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Yee update, Grid::evolveFields (src/grid/oGrid.cpp:23-57): E on the
+// physical points from B_tm1 and J; then B from B_tm1 and the new E.  Two
+// kernels: the B kernel must see every new E, including the stale ghost
+// planes that are only refreshed afterwards.
+// ---------------------------------------------------------------------------
+struct YeeArgs {
+  Geo g;
+  double *E[3];
+  double *B[3];
+  const double *Bt[3];
+  const double *J[3];
+  double dt;
+};
+
+__global__ void __launch_bounds__(kBlock) k_evolve_E(YeeArgs a) {
+  const Geo &g = a.g;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
+  int i, j, k;
+  phys_ijk(g, t, &i, &j, &k);
+  long long c = gidx(g, i, j, k);
+  const long long sx = g.sx, sy = g.sy;
+  const double dt = a.dt, dx_ = g.d[0], dy_ = g.d[1], dz_ = g.d[2];
+  const double *Btx = a.Bt[0], *Bty = a.Bt[1], *Btz = a.Bt[2];
+  a.E[0][c] = a.E[0][c] - dt * ((Bty[c] - Bty[c - 1]) / dz_ - (Btz[c] - Btz[c - sy]) / dy_ +
+                                4 * kPi * a.J[0][c]);
+  a.E[1][c] = a.E[1][c] - dt * ((Btz[c] - Btz[c - sx]) / dx_ - (Btx[c] - Btx[c - 1]) / dz_ +
+                                4 * kPi * a.J[1][c]);
+  a.E[2][c] = a.E[2][c] - dt * ((Btx[c] - Btx[c - sy]) / dy_ - (Bty[c] - Bty[c - sx]) / dx_ +
+                                4 * kPi * a.J[2][c]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_evolve_B(YeeArgs a) {
+  const Geo &g = a.g;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
+  int i, j, k;
+  phys_ijk(g, t, &i, &j, &k);
+  long long c = gidx(g, i, j, k);
+  const long long sx = g.sx, sy = g.sy;
+  const double dt = a.dt, dx_ = g.d[0], dy_ = g.d[1], dz_ = g.d[2];
+  const double *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+  a.B[0][c] = a.Bt[0][c] + dt * ((Ey[c + 1] - Ey[c]) / dz_ - (Ez[c + sy] - Ez[c]) / dy_);
+  a.B[1][c] = a.Bt[1][c] + dt * ((Ez[c + sx] - Ez[c]) / dx_ - (Ex[c + 1] - Ex[c]) / dz_);
+  a.B[2][c] = a.Bt[2][c] + dt * ((Ex[c + sy] - Ex[c]) / dy_ - (Ey[c + sx] - Ey[c]) / dx_);
+}
+
+// ---------------------------------------------------------------------------
+// LightBC plane fill: Grid::setFieldInPlane over the full (n+3)^2 plane
+// (src/grid/oGrid.cpp:446-476) with the value of LightBC::applyBCs
+// (src/grid/lightBC.cpp:165-198).  With waist>0 the wave part is shaped by a
+// transverse Gaussian (an extension; the reference is plane-uniform).
+// ---------------------------------------------------------------------------
+struct FillArgs {
+  Geo g;
+  PlaneFill pf;
+  double Lc[3];  // box centre per dim
+};
+
+__global__ void __launch_bounds__(kBlock) k_plane_fill(FillArgs a) {
+  const Geo &g = a.g;
+  const PlaneFill &pf = a.pf;
+  int d = pf.dim;
+  int u = d == 0 ? 1 : 0;
+  int w = d == 2 ? 1 : 2;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nt[u] * g.nt[w]) return;
+  int pu = (int)(t / g.nt[w]);
+  int pw = (int)(t % g.nt[w]);
+  int ii[3];
+  ii[d] = pf.idx;
+  ii[u] = pu;
+  ii[w] = pw;
+  double v;
+  if (pf.waist > 0) {
+    double ya = g.x0[u] + g.d[u] * (pu - 1) + pf.off[u] - a.Lc[u];
+    double yb = g.x0[w] + g.d[w] * (pw - 1) + pf.off[w] - a.Lc[w];
+    v = pf.base + pf.wave * exp(-(ya * ya + yb * yb) / (pf.waist * pf.waist));
+  } else {
+    v = pf.base + pf.wave;
+  }
+  pf.f[gidx(g, ii[0], ii[1], ii[2])] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_const_fill(double *f, double v, long long n) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) f[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi sweep, Poisson_Solver::run_poisson_solver_ (src/poisson/poisson.cpp:
+// 223-240) over the physical points.
+// ---------------------------------------------------------------------------
+struct JacobiArgs {
+  Geo g;
+  const double *src;
+  double *dst;
+  const double *R;
+  double ax, ay, az, af, sourceMult;
+};
+
+__global__ void __launch_bounds__(kBlock) k_jacobi(JacobiArgs a) {
+  const Geo &g = a.g;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
+  int i, j, k;
+  phys_ijk(g, t, &i, &j, &k);
+  long long c = gidx(g, i, j, k);
+  const double *u = a.src;
+  a.dst[c] = a.ax * (u[c - g.sx] + u[c + g.sx]) + a.ay * (u[c - g.sy] + u[c + g.sy]) +
+             a.az * (u[c - 1] + u[c + 1]) - a.af * a.R[c] * a.sourceMult;
+}
+
+// max|u2-u1| (:250-260) and sum (u2-u1)^2 per block: partials[2*b], [2*b+1].
+// The max is exact in any order; the L2 partials are reduced in a fixed
+// order by launch_reduce_partials.
+__global__ void __launch_bounds__(kBlock) k_residual(Geo g, const double *u1, const double *u2,
+                                                     double *partials) {
+  __shared__ double smax[kBlock];
+  __shared__ double ssum[kBlock];
+  long long nphys = (long long)g.n[0] * g.n[1] * g.n[2];
+  double mx = 0.0, sm = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nphys;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    phys_ijk(g, t, &i, &j, &k);
+    long long c = gidx(g, i, j, k);
+    double dd = u2[c] - u1[c];
+    double ad = fabs(dd);
+    mx = ad > mx ? ad : mx;
+    sm += dd * dd;
+  }
+  smax[threadIdx.x] = mx;
+  ssum[threadIdx.x] = sm;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+      ssum[threadIdx.x] += ssum[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = smax[0];
+    partials[2 * blockIdx.x + 1] = ssum[0];
+  }
+}
+
+// phiToE, src/poisson/convertFields.cpp:20-67
+__global__ void __launch_bounds__(kBlock) k_phi_to_e(Geo g, const double *phi, double *Ex, double *Ey,
+                                                     double *Ez) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
+  int i, j, k;
+  phys_ijk(g, t, &i, &j, &k);
+  long long p = gidx(g, i, j, k);
+  Ex[p] = -(phi[p + g.sx] - phi[p]) / g.d[0];
+  Ey[p] = -(phi[p + g.sy] - phi[p]) / g.d[1];
+  Ez[p] = -(phi[p + 1] - phi[p]) / g.d[2];
+}
+
+// AToB, src/poisson/convertFields.cpp:83-153 (not a curl: Q10)
+__global__ void __launch_bounds__(kBlock) k_a_to_b(Geo g, const double *Ax, const double *Ay,
+                                                   const double *Az, double *Bx, double *By,
+                                                   double *Bz) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
+  int i, j, k;
+  phys_ijk(g, t, &i, &j, &k);
+  long long p = gidx(g, i, j, k);
+  // Bx: Ak=Az (hk=dz), Aj=Ay (hj=dy); nextValk=Ak[j+1], nextValj=Ak[k+1]
+  Bx[p] = (Az[p + g.sy] - Az[p]) / g.d[1] + (Az[p + 1] - Ay[p]) / g.d[2];
+  // By: Ak=Ax (hk=dx), Aj=Az (hj=dz); nextValk=Ak[k+1], nextValj=Ak[i+1]
+  By[p] = (Ax[p + 1] - Ax[p]) / g.d[2] + (Ax[p + g.sx] - Az[p]) / g.d[0];
+  // Bz: Ak=Ay (hk=dy), Aj=Ax (hj=dx); nextValk=Ak[i+1], nextValj=Ak[j+1]
+  Bz[p] = (Ay[p + g.sx] - Ay[p]) / g.d[0] + (Ay[p + g.sy] - Ax[p]) / g.d[1];
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics: OutputBoxQuantities::output (src/IO/output.cpp:76-118).
+// Per-block partial sums in a fixed grid-stride order, then a fixed-order
+// final sum: deterministic run to run (not the reference's sequential
+// order, so equal to it within FP64 rounding).
+// ---------------------------------------------------------------------------
+struct DiagPArgs {
+  const double *v[3];
+  const int32_t *sp;
+  SpeciesParams s;
+  long long np;
+  double *partials;  // 5 per block
+};
+
+template <int NC>
+__device__ void block_reduce_store(double (&vals)[NC], double *out) {
+  __shared__ double sh[NC][kBlock];
+#pragma unroll
+  for (int q = 0; q < NC; q++) sh[q][threadIdx.x] = vals[q];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int q = 0; q < NC; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NC; q++) out[NC * blockIdx.x + q] = sh[q][0];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_diag_particles(DiagPArgs a) {
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.np;
+       i += (long long)gridDim.x * blockDim.x) {
+    int sp = a.sp[i];
+    double m = a.s.m[sp];
+    double vx = a.v[0][i], vy = a.v[1][i], vz = a.v[2][i];
+    acc[0] += m * vx;
+    acc[1] += m * vy;
+    acc[2] += m * vz;
+    acc[3] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    acc[4] += a.s.q[sp];
+  }
+  block_reduce_store<5>(acc, a.partials);
+}
+
+struct DiagFArgs {
+  Geo g;
+  const double *E[3];
+  const double *B[3];
+  double *partials;  // 6 per block
+};
+
+#define EMO_SQR(x) (x * x)
+__global__ void __launch_bounds__(kBlock) k_diag_fields(DiagFArgs a) {
+  const Geo &g = a.g;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  double vol = g.d[0] * g.d[1] * g.d[2];
+  long long nphys = (long long)g.n[0] * g.n[1] * g.n[2];
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nphys;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    phys_ijk(g, t, &i, &j, &k);
+    long long p = gidx(g, i, j, k);
+    acc[0] += 0.5 * vol * EMO_SQR(0.5 * (a.E[0][p] + a.E[0][p + g.sx]));
+    acc[1] += 0.5 * vol * EMO_SQR(0.5 * (a.E[1][p] + a.E[1][p + g.sy]));
+    acc[2] += 0.5 * vol * EMO_SQR(0.5 * (a.E[2][p] + a.E[2][p + 1]));
+    acc[3] += 0.5 * vol * EMO_SQR(0.5 * (a.B[0][p] + a.B[0][p + g.sx]));
+    acc[4] += 0.5 * vol * EMO_SQR(0.5 * (a.B[1][p] + a.B[1][p + g.sy]));
+    acc[5] += 0.5 * vol * EMO_SQR(0.5 * (a.B[2][p] + a.B[2][p + 1]));
+  }
+  block_reduce_store<6>(acc, a.partials);
+}
+#undef EMO_SQR
+
+// one block: out[q] = sum over blocks b (ascending, fixed tree) of partials[ncomp*b+q]
+__global__ void __launch_bounds__(kBlock) k_reduce_partials(const double *partials, int nblocks,
+                                                            int ncomp, double *out) {
+  __shared__ double sh[kBlock];
+  for (int q = 0; q < ncomp; q++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partials[ncomp * b + q];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sh[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+void launch_fold_f64(emo_ctx *c) {
+  const Geo &g = c->geo;
+  for (int d = 0; d < 3; d++) {
+    if (c->cfg.fields_bound[2 * d] != EMO_BC_PERIODIC) continue;
+    PlaneArgs a;
+    a.g = g;
+    a.f[0] = d == 0 ? c->J[1] : c->J[0];
+    a.f[1] = d == 2 ? c->J[1] : c->J[2];
+    a.f[2] = c->rho;
+    a.nf = 3;
+    a.d = d;
+    a.src = g.n[d] + 1;
+    a.dst = 1;
+    a.add = 1;
+    int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
+    hipLaunchKernelGGL(k_plane_copy, dim3(grid_for((long long)g.n[u] * g.n[w])), dim3(kBlock), 0,
+                       c->stream, a);
+  }
+}
+
+void launch_evolve_f64(emo_ctx *c, double dt) {
+  // swap the member pointers (src/grid/oGrid.cpp:18-20)
+  for (int q = 0; q < 3; q++) std::swap(c->B[q], c->Bt[q]);
+  YeeArgs a;
+  a.g = c->geo;
+  for (int q = 0; q < 3; q++) {
+    a.E[q] = c->E[q];
+    a.B[q] = c->B[q];
+    a.Bt[q] = c->Bt[q];
+    a.J[q] = c->J[q];
+  }
+  a.dt = dt;
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  {
+    TimedLaunch t(c, "yee_E_f64");
+    hipLaunchKernelGGL(k_evolve_E, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+  }
+  TimedLaunch t(c, "yee_B_f64");
+  hipLaunchKernelGGL(k_evolve_B, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+}
+
+void launch_plane_copy_f64(emo_ctx *c, double *const *fields, int nf, int d, int src, int dst,
+                           int add) {
+  PlaneArgs a;
+  a.g = c->geo;
+  for (int q = 0; q < nf; q++) a.f[q] = fields[q];
+  a.nf = nf;
+  a.d = d;
+  a.src = src;
+  a.dst = dst;
+  a.add = add;
+  int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
+  hipLaunchKernelGGL(k_plane_copy, dim3(grid_for((long long)c->geo.n[u] * c->geo.n[w])),
+                     dim3(kBlock), 0, c->stream, a);
+}
+
+void launch_plane_fill_f64(emo_ctx *c, const PlaneFill &pf) {
+  FillArgs a;
+  a.g = c->geo;
+  a.pf = pf;
+  for (int q = 0; q < 3; q++) a.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
+  int d = pf.dim;
+  int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
+  hipLaunchKernelGGL(k_plane_fill, dim3(grid_for((long long)c->geo.nt[u] * c->geo.nt[w])),
+                     dim3(kBlock), 0, c->stream, a);
+}
+
+void launch_const_fill_f64(emo_ctx *c, double *f, double v, long long n) {
+  hipLaunchKernelGGL(k_const_fill, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, f, v, n);
+}
+
+void launch_jacobi_f64(emo_ctx *c, const double *src, double *dst, const double *R, double ax,
+                       double ay, double az, double af, double sourceMult) {
+  JacobiArgs a;
+  a.g = c->geo;
+  a.src = src;
+  a.dst = dst;
+  a.R = R;
+  a.ax = ax;
+  a.ay = ay;
+  a.az = az;
+  a.af = af;
+  a.sourceMult = sourceMult;
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  TimedLaunch t(c, "jacobi_f64");
+  hipLaunchKernelGGL(k_jacobi, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+}
+
+int residual_blocks(emo_ctx *c) {
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  long long b = (nphys + kBlock - 1) / kBlock;
+  return (int)(b < 1024 ? (b > 0 ? b : 1) : 1024);
+}
+
+void launch_residual_f64(emo_ctx *c, const double *u1, const double *u2, double *partials,
+                         int nblocks) {
+  hipLaunchKernelGGL(k_residual, dim3(nblocks), dim3(kBlock), 0, c->stream, c->geo, u1, u2,
+                     partials);
+}
+
+void launch_phi_to_e_f64(emo_ctx *c, const double *phi) {
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  hipLaunchKernelGGL(k_phi_to_e, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, c->geo, phi,
+                     c->E[0], c->E[1], c->E[2]);
+}
+
+void launch_a_to_b_f64(emo_ctx *c, const double *Ax, const double *Ay, const double *Az) {
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  hipLaunchKernelGGL(k_a_to_b, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, c->geo, Ax, Ay,
+                     Az, c->B[0], c->B[1], c->B[2]);
+}
+
+void launch_diag_particles_f64(emo_ctx *c, double *partials, int nblocks) {
+  DiagPArgs a;
+  for (int q = 0; q < 3; q++) a.v[q] = c->p.v[q];
+  a.sp = c->p.sp;
+  a.s = c->species;
+  a.np = c->np;
+  a.partials = partials;
+  hipLaunchKernelGGL(k_diag_particles, dim3(nblocks), dim3(kBlock), 0, c->stream, a);
+}
+
+void launch_diag_fields_f64(emo_ctx *c, double *partials, int nblocks) {
+  DiagFArgs a;
+  a.g = c->geo;
+  for (int q = 0; q < 3; q++) {
+    a.E[q] = c->E[q];
+    a.B[q] = c->Bfix[q];  // Q23: the diagnostics read Grid::fieldPtr_
+  }
+  a.partials = partials;
+  hipLaunchKernelGGL(k_diag_fields, dim3(nblocks), dim3(kBlock), 0, c->stream, a);
+}
+
+void launch_reduce_partials(emo_ctx *c, const double *partials, int nblocks, int ncomp,
+                            double *out) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, partials, nblocks,
+                     ncomp, out);
+}
+
+}  // namespace emo

@@ -1,0 +1,242 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-steps/s of EMOOPIC's per-timestep PIC loop on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--mode fp64|fp32|both]
+
+Workload c2 (64^3 periodic, 32 particles/cell/species, 16.8M particles,
+relativistic Boris, electron sigma 0.1c with a 0.01 cos(2 pi x/64)
+density perturbation), synthetic seeded input, inputs resident in HBM
+before the timed region.  A step is one full iteration of the loop (push,
+particle BCs, sort, deposit, fold, Yee, ghost refresh, gather); value is
+particles x steps / wall time of K steps.
+
+Prints one JSON line.  The roofline entry is for the step's dominant
+kernel, timed with HIP events on the library stream during the timed
+steps; cpu_baseline times the reference's own serial step loop
+(oracle/_ref, built from the reference sources) on one host core over a
+bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s (push+gather+deposit) FP32 vs FP64; adjoint/forward time"
+
+
+def algorithmic_bytes(kernel: str, np_: int, ncell: int, relativity: int) -> float | None:
+    """Bytes a kernel must move per launch at minimum (each operand read or
+    written once).  Per-particle and per-cell figures are documented in
+    DESIGN.md."""
+    ng = ncell  # physical cells ~ padded points for the grid kernels
+    table = {
+        # FP64 plain form
+        "gather_f64": 72 * np_ + 72 * ncell,
+        "push_f64": (164 if relativity else 148) * np_,
+        "particle_bc_f64": 24 * np_,
+        "sort_keys": 32 * np_,
+        "sort_permute": 236 * np_,
+        "deposit_cells_f64": 52 * np_ + 168 * ncell,
+        "deposit_grid_f64": 160 * ncell + 32 * ng,
+        "yee_E_f64": 96 * ncell,
+        "yee_B_f64": 72 * ncell,
+        # FP32 chosen form
+        "pgd_f32": 56 * np_ + 36 * ncell + 48 * ncell,
+        "sort_f32_scatter": 56 * np_,
+        "jsum_f32": 48 * ncell + 12 * ncell,
+        "yee_E_f32": 48 * ncell,
+        "yee_B_f32": 36 * ncell,
+    }
+    return table.get(kernel)
+
+
+KERNELS = {
+    "fp64": ["gather_f64", "push_f64", "particle_bc_f64", "sort_keys", "sort_radix", "sort_permute",
+             "deposit_cells_f64", "deposit_grid_f64", "yee_E_f64", "yee_B_f64"],
+    "fp32": ["pgd_f32", "sort_f32_count", "sort_f32_scan", "sort_f32_scatter", "jsum_f32",
+             "yee_E_f32", "yee_B_f32"],
+}
+
+
+def run_gpu(mode: str, steps: int, warmup: int, scale: float):
+    import apc_524_project_amd as emo
+    from apc_524_project_amd import FP32_CHOSEN, FP64_PLAIN
+
+    prec = FP64_PLAIN if mode == "fp64" else FP32_CHOSEN
+    w = emo.make_workload("c2", precision=prec, scale=scale)
+    t0 = time.perf_counter()
+    ctx = emo.Context(w.cfg)
+    ctx.load_maxwellian(w.loader)
+    ctx.init_constant()
+    ctx.synchronize()
+    t_load = time.perf_counter() - t0
+    print(f"[bench] {mode}: loaded {w.nparticles} particles in {t_load:.1f}s", file=sys.stderr, flush=True)
+    t = 0.0
+    ctx.step(warmup, t, w.dt)
+    t += warmup * w.dt
+    ctx.synchronize()
+    ctx.timing(True)
+    ctx.timing_reset()
+    ctx.synchronize()
+    a = time.perf_counter()
+    ctx.step(steps, t, w.dt)
+    ctx.synchronize()
+    el = time.perf_counter() - a
+    ctx.timing(False)
+    ktimes = {}
+    for k in KERNELS[mode]:
+        ms, n = ctx.kernel_time(k)
+        if n:
+            ktimes[k] = (ms, n)
+    rate = w.nparticles * steps / el
+    print(f"[bench] {mode}: {steps} steps in {el:.3f}s -> {rate:.4g} particle-steps/s", file=sys.stderr,
+          flush=True)
+    ctx.close()
+    return dict(rate=rate, elapsed=el, ms_per_step=1e3 * el / steps, ktimes=ktimes, workload=w)
+
+
+def roofline_for(res):
+    w = res["workload"]
+    kt = res["ktimes"]
+    if not kt:
+        return None
+    dom = max(kt, key=lambda k: kt[k][0])
+    ms, n = kt[dom]
+    avg_s = ms / n / 1e3
+    b = algorithmic_bytes(dom, w.nparticles, w.ncells, int(w.cfg.relativity))
+    if b is None:
+        return {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None, "avg_ms": ms / n}
+    ach = b / avg_s / 1e9
+    return {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_ms": round(ms / n, 5),
+            "algorithmic_bytes": int(b),
+            "kernel_ms_per_step": {k: round(v[0] / v[1], 5) for k, v in sorted(kt.items())}}
+
+
+def cpu_baseline(sample_cells: int = 32, sample_steps: int = 10):
+    """The reference's serial loop (oracle/_ref) on one core, c2 physics on a
+    sample_cells^3 sub-box at the same density (32 ppc/species)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as oc
+
+    n = sample_cells
+    wpe2 = 4 * math.pi * oc.UNIT_RHOJ * oc.UNIT_ACC
+    case = oc.Case(n=(n, n, n), L=(float(n),) * 3, nparticles=2 * 32 * n ** 3, relativity=1,
+                   temp=((0.1 / oc.UNIT_VTH) ** 2, (0.1 / oc.UNIT_VTH) ** 2), dens_phys=0.01 / wpe2,
+                   zero_avg=0)
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    dt = 0.5 / math.sqrt(3.0)
+    if oc.ref_available():
+        r = oc.Reference(case, nstep_sort=100)
+        r.initialize()
+        el = r.time_steps(sample_steps, dt)
+        kind = "reference"
+    else:
+        o = oc.Oracle(case)
+        o.load()
+        o.initialize()
+        a = time.perf_counter()
+        t = 0.0
+        for _ in range(sample_steps):
+            o.step(t, dt)
+            t += dt
+        el = time.perf_counter() - a
+        kind = "port"
+    cpu = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": case.nparticles * sample_steps / el, "unit": "particle-steps/s", "cores": 1,
+            "kind": kind,
+            "sample": f"c2 physics on a {n}^3 sub-box (32 ppc/species, {case.nparticles} particles), "
+                      f"{sample_steps} steps of the reference serial loop (nstep_sort=100), "
+                      f"1 thread pinned; host CPU: {cpu}; {el:.2f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", default="auto", choices=["auto", "fp64", "fp32", "both"])
+    ap.add_argument("--scale", type=float, default=1.0, help="particle-count scale (1 = full c2)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cells", type=int, default=32)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the multi-GPU decomposition is a design, "
+              "not part of this build (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import apc_524_project_amd as emo
+
+    have_fp32 = True
+    try:
+        w = emo.make_workload("c2", precision=emo.FP32_CHOSEN, scale=0.001)
+        c = emo.Context(w.cfg)
+        c.close()
+    except emo.EmoError:
+        have_fp32 = False
+    mode = args.mode
+    if mode == "auto":
+        mode = "both" if have_fp32 else "fp64"
+    results = {}
+    for m in (["fp64", "fp32"] if mode == "both" else [mode]):
+        results[m] = run_gpu(m, args.steps, args.warmup, args.scale)
+    head = "fp32" if "fp32" in results else "fp64"
+    r = results[head]
+    w = r["workload"]
+    out = {
+        "metric": METRIC,
+        "value": round(r["rate"], 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "c2: 64^3 periodic Yee grid, dx=1, dt=0.5 CFL, 32 ppc/species "
+                        "(16,777,216 particles), relativistic Boris, sigma_e=0.1c, "
+                        "0.01 cos(2 pi x/64) electron density perturbation, seed -2",
+            "form": "FP32 chosen form" if head == "fp32" else "FP64 plain form",
+            "particles": w.nparticles,
+            "cells": w.ncells,
+            "scale": args.scale,
+        },
+        "roofline": roofline_for(r),
+    }
+    for m, rr in results.items():
+        out[f"{m}_particle_steps_per_s"] = round(rr["rate"], 1)
+        out[f"{m}_yee_cell_updates_per_s"] = round(rr["workload"].ncells * args.steps / rr["elapsed"], 1)
+    if "fp32" in results and "fp64" in results:
+        out["fp32_over_fp64"] = round(results["fp32"]["rate"] / results["fp64"]["rate"], 3)
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_cells, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * emo_api.h -- C ABI of libemopic, the MI355X (gfx950) implementation of
+ * EMOOPIC's per-timestep particle-in-cell loop.
+ *
+ * Plain pointers and sizes only.  A context owns its device memory and one
+ * HIP stream; every call is stream-ordered and returns 0 on success or a
+ * negative EMO_ERR_* code, with a message from emo_last_error().  Host arrays
+ * passed in are copied during the call and never retained.
+ *
+ * The entry points stand in for the methods that the reference's driver
+ * loop calls (DenSto/APC_524_Project src/main.cpp:221-287); each declaration
+ * names the reference method it replaces.
+ *
+ * Array conventions (the reference's, src/grid/grid.cpp:161-181):
+ *   fields are padded (nx+3)*(ny+3)*(nz+3) doubles, z fastest; physical
+ *   cells are indices 1..n; field ids follow src/grid/grid.cpp:45-60.
+ *   particle vectors x3, v3 are [n][3] (Particle::x/v, src/particles/particle.hpp:10-11).
+ */
+#ifndef EMO_API_H
+#define EMO_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMO_NSPEC 32 /* Input_Info_t NSPEC, src/IO/input.hpp:10 */
+#define EMO_NWAVE 32 /* Input_Info_t NWAVE, src/IO/input.hpp:12 */
+
+/* return codes */
+#define EMO_OK 0
+#define EMO_ERR_CONFIG (-1)      /* invalid configuration or argument */
+#define EMO_ERR_ALLOC (-2)       /* device or host allocation failed */
+#define EMO_ERR_HIP (-3)         /* a HIP runtime call failed */
+#define EMO_ERR_NUMERIC (-4)     /* NaN/Inf, Poisson non-convergence when required */
+#define EMO_ERR_INVARIANT (-5)   /* particle outside the padded grid, net charge, ... */
+#define EMO_ERR_UNSUPPORTED (-6) /* entry point not available in this precision mode */
+
+/* boundary kinds (Input_Info_t parts_bound / fields_bound strings) */
+#define EMO_BC_PERIODIC 0
+#define EMO_BC_EXTERNAL 1   /* fields: "external" (LightBC / PoissonBC) */
+#define EMO_BC_REFLECTING 1 /* particles: "reflecting" */
+
+/* precision modes */
+#define EMO_FP64_PLAIN 0  /* FP64 throughout, reference pass structure, bit-exact to the oracle */
+#define EMO_FP32_CHOSEN 1 /* FP32 cell+offset SoA, fused gather/push/deposit */
+
+/* field ids (src/grid/grid.cpp:45-60; src/poisson/poisson.cpp:18-26) */
+enum {
+  EMO_EX = 0, EMO_EY, EMO_EZ, EMO_BX, EMO_BY, EMO_BZ, EMO_JX, EMO_JY, EMO_JZ, EMO_RHO,
+  EMO_BX_TM1, EMO_BY_TM1, EMO_BZ_TM1,
+  EMO_PHI = 13, EMO_AX, EMO_AY, EMO_AZ
+};
+
+/* Configuration: the subset of Input_Info_t (src/IO/input.hpp:15-126) the
+ * step needs, after Input::ProcessInfo (q, m are super-particle scaled). */
+typedef struct emo_config {
+  int32_t nCell[3];
+  double xyz0[3];
+  double Lxyz[3];
+  int32_t relativity;          /* 0 Boris, 1 relativistic (Vay) Boris */
+  int32_t precision;           /* EMO_FP64_PLAIN / EMO_FP32_CHOSEN */
+  int32_t fields_bound[6];     /* x-,x+,y-,y+,z-,z+ */
+  int32_t parts_bound[6];
+  int32_t nspecies;
+  double charge[EMO_NSPEC];    /* super-particle charge per species */
+  double mass[EMO_NSPEC];      /* super-particle mass per species */
+  int32_t isTestParticle[EMO_NSPEC];
+  double E0[3], B0[3];
+  int32_t nwaves;
+  int32_t inSide[EMO_NWAVE];
+  int32_t inPolE[EMO_NWAVE];
+  double peakamps[EMO_NWAVE], omegas[EMO_NWAVE], phases[EMO_NWAVE];
+  double invWidths[EMO_NWAVE], delays[EMO_NWAVE];
+  double waist;                /* 0: plane-uniform pulse (reference); >0: transverse Gaussian waist */
+  double bound_phi[6], bound_Ax[6], bound_Ay[6], bound_Az[6];
+  int64_t max_particles;       /* capacity of the particle store */
+} emo_config;
+
+typedef struct emo_ctx emo_ctx;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Builds the Grid/Domain/Particle_Handler state (src/main.cpp:115-150). */
+int emo_create(const emo_config *cfg, emo_ctx **out);
+void emo_destroy(emo_ctx *ctx);
+const char *emo_last_error(const emo_ctx *ctx);
+/* library version string */
+const char *emo_version(void);
+
+/* ---- particles ---------------------------------------------------------- */
+/* Replaces Particle_Handler::Load's result (src/particles/particle_handler.cpp:34-122):
+ * copies n particles in; gamma may be NULL for Boris. */
+int emo_load_particles(emo_ctx *ctx, int64_t n, const double *x3, const double *v3,
+                       const double *gamma, const int32_t *species, const int64_t *id);
+/* Particle_Handler::Load with the reference's ran2 + Box-Muller stream
+ * (src/utils/RNG.cpp:23-141) after Input::ProcessInfo (src/IO/check.cpp:53-111).
+ * Writes the super-particle charge/mass into cfg_inout. */
+typedef struct emo_loader {
+  int64_t nparticles_tot;
+  int32_t nspecies;
+  double mass_ratio[EMO_NSPEC], charge_ratio[EMO_NSPEC], dens_frac[EMO_NSPEC], temp[EMO_NSPEC];
+  int32_t isTestParticle[EMO_NSPEC];
+  double dens_phys;
+  int64_t seed;                /* the reference uses -1 */
+  int32_t zero_avg;            /* -1: reference rule; 0/1 force */
+  int32_t poisson_init;        /* fields_init == "poisson" */
+  double pert_eps[EMO_NSPEC];  /* extension: x -= (eps/k) sin(k x) displacement loader */
+  double pert_k[EMO_NSPEC];
+  double drift[EMO_NSPEC][3];  /* extension: drift velocity added after sampling */
+} emo_loader;
+int emo_process_info(const emo_loader *ld, emo_config *cfg_inout);
+int emo_load_maxwellian(emo_ctx *ctx, const emo_loader *ld);
+int64_t emo_num_particles(const emo_ctx *ctx);
+/* copy out in the current array order; any pointer may be NULL */
+int emo_get_particles(emo_ctx *ctx, double *x3, double *v3, double *gamma, int32_t *species,
+                      int64_t *id, double *field6);
+/* per-particle cell ids (Grid::getCellID, src/grid/oGrid.cpp:208-222) in array order */
+int emo_get_cell_ids(emo_ctx *ctx, int32_t *cells);
+
+/* ---- fields ------------------------------------------------------------- */
+int64_t emo_field_points(const emo_ctx *ctx);
+/* B/B_tm1 ids address the current member arrays (src/grid/oGrid.cpp:18-20) */
+int emo_get_field(emo_ctx *ctx, int fid, double *padded);
+int emo_set_field(emo_ctx *ctx, int fid, const double *padded);
+
+/* ---- initialisation ----------------------------------------------------- */
+/* Grid::InitializeFields constant branch (src/grid/grid.cpp:554-574) */
+int emo_init_constant(emo_ctx *ctx);
+/* src/main.cpp:160-174 on the Poisson path: deposit, executeBC(-2),
+ * Poisson_Solver::InitializeFields (src/poisson/poisson.cpp:143-170).
+ * iter_max mirrors ITER_MAX (src/poisson/poisson.cpp:14; the loop runs up to
+ * iter_max+1 sweeps); tol < 0 selects the reference tolerance rule.
+ * sweeps_out[4] (phi, Ax, Ay, Az) and resid_out[8] (FP64 max|du| and L2 of
+ * the last sweep per solve) may be NULL. */
+int emo_init_poisson(emo_ctx *ctx, int32_t iter_max, double tol, int32_t *sweeps_out,
+                     double *resid_out);
+
+/* ---- the step's phases (FP64 plain form; EMO_ERR_UNSUPPORTED in FP32) --- */
+int emo_sort(emo_ctx *ctx);                  /* Particle_Handler::SortParticles, particle_handler.cpp:180-183 */
+int emo_push(emo_ctx *ctx, double dt);       /* Particle_Handler::Push, particle_handler.cpp:124-128 */
+int emo_particle_bc(emo_ctx *ctx);           /* executeParticleBoundaryConditions + clearGhosts, :271-295 */
+int emo_deposit(emo_ctx *ctx, int32_t rho);  /* Particle_Handler::depositRhoJ, :185-239 */
+int emo_fold(emo_ctx *ctx);                  /* Grid::executeBC(-2), src/grid/grid.cpp:577-585 */
+int emo_evolve_fields(emo_ctx *ctx, double dt); /* Grid::evolveFields, src/grid/oGrid.cpp:15-60 */
+int emo_refresh(emo_ctx *ctx, double t, double dt); /* Grid::executeBC(-1) incl. LightBC::applyBCs */
+int emo_interpolate(emo_ctx *ctx);           /* Particle_Handler::InterpolateEB, :138-173 */
+
+/* ---- whole steps -------------------------------------------------------- */
+/* nsteps iterations of src/main.cpp:231-275 starting at time t0 */
+int emo_step(emo_ctx *ctx, int32_t nsteps, double t0, double dt);
+
+/* ---- diagnostics -------------------------------------------------------- */
+/* OutputBoxQuantities::output sums (src/IO/output.cpp:76-118) in FP64:
+ * out8 = {pMx, pMy, pMz, pEn, B_en, E_en, E_tot, total charge} */
+int emo_diag(emo_ctx *ctx, double *out8);
+
+/* ---- stream / timing ---------------------------------------------------- */
+int emo_synchronize(emo_ctx *ctx);
+/* Record HIP events around every launch of the step's dominant kernels on
+ * the context stream; emo_kernel_time returns the summed milliseconds and
+ * launch count per kernel name since the last reset. */
+int emo_timing_enable(emo_ctx *ctx, int32_t on);
+int emo_kernel_time(emo_ctx *ctx, const char *name, double *ms_total, int64_t *launches);
+int emo_timing_reset(emo_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -7,6 +7,7 @@
  * Each function cites the reference file:line it restates.  Reference paths
  * are relative to the DenSto/APC_524_Project checkout.
  */
+#define _GNU_SOURCE
 #include "emo_oracle.h"
 
 #include <float.h>
@@ -139,6 +140,10 @@ int eo_get_field(const eo_state *s, int fid, double *out) {
   return 0;
 }
 
+void eo_set_b_alias(eo_state *s, int odd) {
+  for (int c = 0; c < 3; c++) s->fBfix[c] = odd ? s->f[FBTX + c] : s->f[FBX + c];
+}
+
 int eo_set_field(eo_state *s, int fid, const double *in) {
   if (fid < 0 || fid >= NFIELD) return -1;
   memcpy(s->f[fid], in, s->ntot * sizeof(double));
@@ -207,8 +212,12 @@ double eo_rng_gaussian(eo_rng *r, double mu, double sigma) {
     u1 = eo_rng_uniform(r);
     u2 = eo_rng_uniform(r);
   } while (u1 <= epsilon);
-  r->z0 = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-  r->z1 = sqrt(-2.0 * log(u1)) * sin(two_pi * u2);
+  /* g++ -O3 (the reference's build, src/Makefile:5) merges the cos and sin
+   * of one argument into a single glibc sincos call; do so explicitly */
+  double sn, cs;
+  sincos(two_pi * u2, &sn, &cs);
+  r->z0 = sqrt(-2.0 * log(u1)) * cs;
+  r->z1 = sqrt(-2.0 * log(u1)) * sn;
   r->generate = !r->generate;
   return r->z0 * sigma + mu;
 }

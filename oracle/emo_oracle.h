@@ -82,6 +82,10 @@ int64_t eo_field_points(const eo_state *s);
 int eo_get_field(const eo_state *s, int fid, double *out);
 int eo_set_field(eo_state *s, int fid, const double *in);
 
+/* Q23 aliasing: odd != 0 makes Grid::fieldPtr_[B] alias the member B_tm1
+ * (the state after an odd number of evolveFields calls), 0 the member B. */
+void eo_set_b_alias(eo_state *s, int odd);
+
 /* ---- loader: Particle_Handler::Load + Input::ProcessInfo + zeroAvg ------ */
 typedef struct {
   int64_t nparticles_tot;
@@ -125,10 +129,11 @@ int32_t eo_cell_id(const eo_state *s, double x, double y, double z);
 
 /* Poisson initialisation: zeroes E,B,B_tm1,phi,A (Q8), solves phi and A with
  * the reference's Jacobi loop, phiToE, AToB, executeBC(-1).  sweeps_out[4]
- * gets the number of sweeps run per solve.  max_sweeps: the loop runs while
- * not converged && iternum < max_sweeps-1 ... i.e. at most max_sweeps sweeps
- * (reference: 10001).  tol < 0: use the reference tolerance rule. */
-void eo_poisson_init(eo_state *s, int32_t max_sweeps, double tol, int32_t *sweeps_out,
+ * gets the number of sweeps run per solve.  iter_max mirrors ITER_MAX
+ * (src/poisson/poisson.cpp:14): the loop runs while not converged and
+ * iternum < iter_max, i.e. at most iter_max+1 sweeps (reference: 10000).
+ * tol < 0 selects the reference tolerance rule (poisson.cpp:151,157). */
+void eo_poisson_init(eo_state *s, int32_t iter_max, double tol, int32_t *sweeps_out,
                      double t, double dt);
 /* constant initialisation: Grid::InitializeFields (src/grid/grid.cpp:554-574) */
 void eo_const_init(eo_state *s);

@@ -227,6 +227,7 @@ def oracle_lib():
         lib.eo_cell_id.argtypes = [vp, C.c_double, C.c_double, C.c_double]
         lib.eo_poisson_init.argtypes = [vp, C.c_int32, C.c_double, vp, C.c_double, C.c_double]
         lib.eo_diag.argtypes = [vp, vp]
+        lib.eo_set_b_alias.argtypes = [vp, C.c_int]
         lib.eo_gaussian_pulses.restype = C.c_double
         lib.eo_gaussian_pulses.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_double]
         _oracle = lib
@@ -365,6 +366,10 @@ class Oracle:
 
     def push(self, dt):
         self.lib.eo_push(self.h, dt)
+
+    def set_b_alias(self, odd: bool):
+        """Q23: fieldPtr_[B] aliases the member B_tm1 after an odd number of swaps."""
+        self.lib.eo_set_b_alias(self.h, 1 if odd else 0)
 
     def particle_bc(self):
         self.lib.eo_particle_bc(self.h)

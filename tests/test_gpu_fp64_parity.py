@@ -1,0 +1,231 @@
+"""FP64 plain form on the GPU vs the CPU oracle: bit-exact particles, cell
+indices and fields, phase by phase and over whole runs.
+
+Every GPU call goes through the C ABI (libemopic.so via ctypes).  The oracle
+runs the same seeded input on the CPU.
+"""
+import numpy as np
+import pytest
+
+from apc_524_project_amd._lib import Context
+from emo_case import emo_config, emo_loader
+from oracle_ctypes import (BTX, BTY, BTZ, BX, BY, BZ, EX, EY, EZ, EXTERNAL, JX, JY, JZ,
+                           PERIODIC, REFLECTING, RHO, Case, Oracle, Wave, cell_ids_from_x)
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = [EX, EY, EZ, BX, BY, BZ, JX, JY, JZ, RHO, BTX, BTY, BTZ]
+
+
+def by_id(d):
+    o = np.argsort(d["id"], kind="stable")
+    return {k: (v[o] if v is not None else None) for k, v in d.items()}
+
+
+def pair(case, constant=True):
+    """A context and an oracle loaded with the same particles."""
+    o = Oracle(case)
+    assert o.load() == 0
+    ctx = Context(emo_config(case, q=o.q, m=o.m))
+    p = o.particles()
+    ctx.load_particles(p.x, p.v, p.gamma, p.species, p.id)
+    return ctx, o
+
+
+def assert_particles_equal(ctx, o, fields=True):
+    g = by_id(ctx.particles())
+    p = o.particles().by_id()
+    assert np.array_equal(g["id"], p.id)
+    assert np.array_equal(g["x"], p.x), np.abs(g["x"] - p.x).max()
+    assert np.array_equal(g["v"], p.v), np.abs(g["v"] - p.v).max()
+    if o.case.relativity:
+        assert np.array_equal(g["gamma"], p.gamma)
+    if fields:
+        assert np.array_equal(g["field"], p.field), np.abs(g["field"] - p.field).max()
+
+
+def assert_fields_equal(ctx, o, fids=FIELDS):
+    for f in fids:
+        a, b = ctx.field(f), o.field(f)
+        assert np.array_equal(a, b), (f, np.abs(a - b).max())
+
+
+THERMAL = Case(n=(6, 5, 7), L=(6.0, 2.5, 3.5), nparticles=1680, temp=(5000.0, 5000.0),
+               dens_phys=2.824e9, E0=(0.01, -0.02, 0.03), B0=(0.05, 0.02, -0.04))
+
+
+@pytest.mark.parametrize("relativity", [0, 1])
+def test_phase_by_phase_bit_exact(relativity):
+    case = Case(**{**THERMAL.__dict__, "relativity": relativity})
+    ctx, o = pair(case)
+    dt = case.dt_cfl()
+    ctx.init_constant()
+    o.initialize()
+    assert_particles_equal(ctx, o)
+    ctx.push(dt)
+    o.push(dt)
+    assert_particles_equal(ctx, o)
+    ctx.particle_bc()
+    o.particle_bc()
+    assert_particles_equal(ctx, o)
+    ctx.sort()
+    o.sort()
+    # the stable sort yields the same array order, not just the same multiset
+    g, p = ctx.particles(), o.particles()
+    assert np.array_equal(g["id"], p.id)
+    assert np.array_equal(ctx.cell_ids(), cell_ids_from_x(case, p.x))
+    ctx.deposit(True)
+    o.deposit(True)
+    assert_fields_equal(ctx, o, [JX, JY, JZ, RHO])
+    ctx.fold()
+    o.fold()
+    assert_fields_equal(ctx, o, [JX, JY, JZ, RHO])
+    ctx.evolve(dt)
+    o.evolve(dt)
+    assert_fields_equal(ctx, o)
+    ctx.refresh(0.0, dt)
+    o.refresh(0.0, dt)
+    assert_fields_equal(ctx, o)
+    ctx.interpolate()
+    o.gather()
+    assert_particles_equal(ctx, o)
+
+
+@pytest.mark.parametrize("relativity", [0, 1])
+def test_multi_step_bit_exact(relativity):
+    case = Case(**{**THERMAL.__dict__, "relativity": relativity})
+    ctx, o = pair(case)
+    dt = case.dt_cfl()
+    ctx.init_constant()
+    o.initialize()
+    t = 0.0
+    for _ in range(12):
+        o.step(t, dt)
+        t += dt
+    ctx.step(12, 0.0, dt)
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
+    assert np.array_equal(by_id(dict(id=ctx.particles()["id"], c=ctx.cell_ids()))["c"],
+                          cell_ids_from_x(case, o.particles().by_id().x))
+
+
+def test_poisson_init_and_steps_bit_exact():
+    # C1-shaped (periodic, Boris, Poisson init with a fixed sweep count), reduced
+    case = Case(n=(8, 8, 8), L=(8.0, 8.0, 8.0), nparticles=2 * 8 * 512, temp=(501.8, 501.8),
+                dens_phys=2.824e9, poisson=True)
+    o = Oracle(case)
+    assert o.load() == 0
+    ctx = Context(emo_config(case, q=o.q, m=o.m))
+    p = o.particles()
+    ctx.load_particles(p.x, p.v, p.gamma, p.species, p.id)
+    sw, res = ctx.init_poisson(iter_max=49, tol=0.0)
+    osw = o.initialize(iter_max=49, tol=0.0)
+    assert list(sw) == list(osw) == [50, 50, 50, 50]
+    assert_fields_equal(ctx, o)
+    assert_particles_equal(ctx, o)
+    dt = case.dt_cfl()
+    t = 0.0
+    for _ in range(10):
+        o.step(t, dt)
+        t += dt
+    ctx.step(10, 0.0, dt)
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
+
+
+def test_poisson_convergence_rule_matches():
+    # the reference's tolerance loop; bound_phi sets conv_phi_ (poisson.cpp:56-60)
+    case = Case(n=(5, 4, 6), L=(5.0, 4.0, 6.0), nparticles=480, temp=(501.8, 501.8),
+                dens_phys=2.824e9, poisson=True, bound_phi=(0.0, 3e-2, 0.0, 0.0, 0.0, 0.0))
+    o = Oracle(case)
+    assert o.load() == 0
+    ctx = Context(emo_config(case, q=o.q, m=o.m))
+    p = o.particles()
+    ctx.load_particles(p.x, p.v, p.gamma, p.species, p.id)
+    sw, _ = ctx.init_poisson(iter_max=10000, tol=-1.0)
+    osw = o.initialize(iter_max=10000, tol=-1.0)
+    assert list(sw) == list(osw)
+    assert_fields_equal(ctx, o)
+
+
+def test_external_lightbc_reflecting_bit_exact():
+    # C3-shaped: x external + reflecting, y/z periodic, y-polarised pulse from x-
+    w = Wave(side=-1, pol=2, amp=0.3, omega=0.39, phase=30.0, inv_width=0.05, delay=5.0)
+    w2 = Wave(side=1, pol=3, amp=0.1, omega=0.7, phase=-10.0, inv_width=0.1, delay=2.0)
+    case = Case(n=(10, 4, 5), L=(10.0, 4.0, 5.0), nparticles=1600, temp=(5000.0, 5000.0),
+                dens_phys=4.355e9, relativity=1, waves=[w, w2],
+                fbc=(EXTERNAL, EXTERNAL, PERIODIC, PERIODIC, PERIODIC, PERIODIC),
+                pbc=(REFLECTING, REFLECTING, PERIODIC, PERIODIC, PERIODIC, PERIODIC),
+                B0=(0.0, 0.0, 0.01))
+    ctx, o = pair(case)
+    dt = case.dt_cfl()
+    ctx.init_constant()
+    o.initialize()
+    t = 0.0
+    for _ in range(15):
+        o.step(t, dt)
+        t += dt
+    ctx.step(15, 0.0, dt)
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
+
+
+def test_waist_extension_close():
+    w = Wave(side=-1, pol=2, amp=0.3, omega=0.39, phase=0.0, inv_width=0.05, delay=5.0)
+    case = Case(n=(10, 8, 8), L=(10.0, 8.0, 8.0), nparticles=1024, temp=(5000.0, 5000.0),
+                dens_phys=4.355e9, relativity=1, waves=[w], waist=3.0,
+                fbc=(EXTERNAL, EXTERNAL, PERIODIC, PERIODIC, PERIODIC, PERIODIC),
+                pbc=(REFLECTING, REFLECTING, PERIODIC, PERIODIC, PERIODIC, PERIODIC))
+    ctx, o = pair(case)
+    dt = case.dt_cfl()
+    ctx.init_constant()
+    o.initialize()
+    t = 0.0
+    for _ in range(10):
+        o.step(t, dt)
+        t += dt
+    ctx.step(10, 0.0, dt)
+    # device exp() vs glibc exp() in the transverse profile: tolerance 1e-13 rel
+    for f in (EY, BZ):
+        a, b = ctx.field(f), o.field(f)
+        assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(b)
+
+
+def test_loader_matches_oracle():
+    case = Case(n=(8, 8, 8), L=(8.0, 8.0, 8.0), nparticles=3000, temp=(501.8, 501.8),
+                dens_phys=2.824e9, poisson=True, seed=-7)
+    o = Oracle(case)
+    assert o.load() == 0
+    ctx = Context(emo_config(case, q=o.q, m=o.m))
+    ctx.load_maxwellian(emo_loader(case))
+    g = ctx.particles(fields=False)
+    p = o.particles()
+    for k in ("x", "v", "gamma"):
+        assert np.array_equal(g[k], getattr(p, k))
+    assert np.array_equal(g["species"], p.species)
+
+
+def test_diag_matches_oracle():
+    case = Case(**{**THERMAL.__dict__, "relativity": 1})
+    ctx, o = pair(case)
+    dt = case.dt_cfl()
+    ctx.init_constant()
+    o.initialize()
+    ctx.step(3, 0.0, dt)
+    t = 0.0
+    for _ in range(3):
+        o.step(t, dt)
+        t += dt
+    a, b = ctx.diag(), o.diag()
+    np.testing.assert_allclose(a[:7], b[:7], rtol=1e-13, atol=0)
+
+
+def test_out_of_grid_particle_is_an_error():
+    case = Case(n=(4, 4, 4), L=(4.0, 4.0, 4.0), nparticles=2)
+    ctx = Context(emo_config(case, q=[-1.0, 1.0], m=[1.0, 1836.0]))
+    x = np.array([[1.5, 1.5, 1.5], [40.0, 1.0, 1.0]])
+    ctx.load_particles(x, np.zeros((2, 3)), np.ones(2), np.array([0, 1]))
+    from apc_524_project_amd._lib import EmoError
+    with pytest.raises(EmoError) as e:
+        ctx.init_constant()
+    assert e.value.code == -5
