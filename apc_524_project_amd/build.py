@@ -19,6 +19,9 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libemopic.so")
+HOST = os.path.join(PKG, "host")
+RUN_MAIN = os.path.join(PKG, "emo_run_main")
+CXX = os.environ.get("CXX", "g++")
 
 ARCH = os.environ.get("EMO_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -35,7 +38,8 @@ SOURCES = [
     ("field_kernels.hip", ["-ffp-contract=off"]),
     ("emo_api.cpp", ["-ffp-contract=off"]),
     ("loader.cpp", ["-ffp-contract=off"]),
-    ("chosen.cpp", []),
+    ("boundary.cpp", ["-ffp-contract=off"]),
+    ("chosen.hip", ["-ffp-contract=fast"]),
 ]
 
 
@@ -72,7 +76,24 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}")
     shutil.move(tmp, LIB)
+    build_host()
     return LIB
+
+
+def build_host() -> str:
+    """The C++ facade (host/emopic.cpp) and the driver that runs the
+    reference's main loop through it (host/run_main.cpp), linked against
+    libemopic.so; host code only, so the host compiler builds it."""
+    tmp = RUN_MAIN + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter",
+           "-I", os.path.join(ROOT, "include"),
+           os.path.join(HOST, "emopic.cpp"), os.path.join(HOST, "run_main.cpp"),
+           "-L", PKG, "-lemopic", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host facade build failed:\n{r.stdout}")
+    shutil.move(tmp, RUN_MAIN)
+    return RUN_MAIN
 
 
 if __name__ == "__main__":

@@ -123,122 +123,6 @@ void free_particles(ParticlesF64 &p) {
   p = ParticlesF64();
 }
 
-// ---- LightBC (src/grid/lightBC.cpp:10-198) --------------------------------
-double gaussian_pulses(int nw, const double *amp, const double *omg, const double *pha,
-                       const double *del, const double *dwt, double t, double dt) {
-  // GaussianPulses, src/utils/evaluateGaussianPulse.cpp:7-30
-  double val = 0.0;
-  for (int i = 0; i < nw; i++) {
-    double tmp = amp[i] * std::cos(omg[i] * (t + dt) + pha[i] * kPi / 180.0);
-    val += tmp * std::exp(-std::pow((t - del[i]) * dwt[i], 2));
-  }
-  return val;
-}
-
-// plane index of Grid::setFieldAlongEdge (src/grid/oGrid.cpp:291-431)
-int edge_plane_index(const Geo &g, int fid, int dim, bool upper) {
-  if (!upper) return 1;
-  int n_ = g.nc[dim];
-  int comp = fid % 3;
-  bool is_e = fid < 3;
-  if (is_e) return comp == dim ? n_ - 1 - 1 : n_ - 1;
-  return comp == dim ? n_ - 1 : n_ - 1 - 1;
-}
-
-void light_bc(emo_ctx *c, int side, double t, double dt) {
-  const emo_config &cf = c->cfg;
-  int sign = (side & 1) ? 1 : -1;
-  int dim = side / 2;
-  int sidev = sign * (dim + 1);
-  int nw[3] = {0, 0, 0};
-  double amp[3][EMO_NWAVE], omg[3][EMO_NWAVE], pha[3][EMO_NWAVE], del[3][EMO_NWAVE], dwt[3][EMO_NWAVE];
-  for (int i = 0; i < cf.nwaves; i++) {
-    if (cf.inSide[i] == sidev) {
-      int pid = cf.inPolE[i] - 1;
-      int k = nw[pid]++;
-      amp[pid][k] = cf.peakamps[i];
-      omg[pid][k] = cf.omegas[i];
-      pha[pid][k] = cf.phases[i];
-      del[pid][k] = cf.delays[i];
-      dwt[pid][k] = cf.invWidths[i];
-    }
-  }
-  static const int EpolT[3][3] = {{1, 2, 0}, {2, 0, 1}, {0, 1, 2}};
-  static const int BpolT[3][3] = {{2, 1, 0}, {0, 2, 1}, {1, 0, 2}};
-  double dx = c->geo.d[dim];
-  double bPhase = (dx + sign * dt) / 2;
-  for (int i = 0; i < 3; i++) {
-    int eid = EpolT[dim][i], bid = BpolT[dim][i];
-    double ewave = 0.0, bwave = 0.0;
-    bool has = nw[eid] > 0 && eid != dim;
-    if (has) {
-      ewave = gaussian_pulses(nw[eid], amp[eid], omg[eid], pha[eid], del[eid], dwt[eid], t, 0.0);
-      int bsign = sign * (int)std::pow(-1, i + 1);
-      bwave = bsign * gaussian_pulses(nw[eid], amp[eid], omg[eid], pha[eid], del[eid], dwt[eid], t, bPhase);
-    }
-    for (int pass = 0; pass < 2; pass++) {
-      PlaneFill pf;
-      int fid = pass == 0 ? eid : 3 + bid;
-      pf.f = pass == 0 ? c->E[eid] : c->B[bid];  // member arrays
-      pf.dim = dim;
-      pf.idx = edge_plane_index(c->geo, fid, dim, sign > 0);
-      pf.base = pass == 0 ? cf.E0[eid] : cf.B0[bid];
-      pf.wave = pass == 0 ? ewave : bwave;
-      pf.waist = has ? cf.waist : 0.0;
-      int comp = fid % 3;
-      for (int q = 0; q < 3; q++) {
-        if (fid < 3) pf.off[q] = q == comp ? 0.5 * c->geo.d[q] : 0.0;
-        else pf.off[q] = q == comp ? 0.0 : 0.5 * c->geo.d[q];
-      }
-      if (!has) {
-        pf.wave = 0.0;
-        pf.waist = 0.0;
-      }
-      launch_plane_fill_f64(c, pf);
-    }
-  }
-}
-
-// Grid::executeBC(-1): sides in order; periodic copies E and, through the
-// stale fieldPtr_ allocations, B (Q23); external sides run LightBC.
-void refresh_fields(emo_ctx *c, double t, double dt) {
-  for (int side = 0; side < 6; side++) {
-    int d = side / 2, right = side & 1;
-    if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
-      int src = right ? 1 : c->geo.n[d];
-      int dst = right ? c->geo.n[d] + 1 : 0;
-      double *fs[6] = {c->E[0], c->E[1], c->E[2], c->Bfix[0], c->Bfix[1], c->Bfix[2]};
-      launch_plane_copy_f64(c, fs, 6, d, src, dst, 0);
-    } else {
-      light_bc(c, side, t, dt);
-    }
-  }
-}
-
-// executeBC(potentialID) for one potential array (periodic copy or the
-// PoissonBC constant plane, src/poisson/poissonBC.cpp:67-80)
-void potential_bc(emo_ctx *c, double *u, const double *bound) {
-  for (int side = 0; side < 6; side++) {
-    int d = side / 2, right = side & 1;
-    if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
-      int src = right ? 1 : c->geo.n[d];
-      int dst = right ? c->geo.n[d] + 1 : 0;
-      double *fs[1] = {u};
-      launch_plane_copy_f64(c, fs, 1, d, src, dst, 0);
-    } else {
-      PlaneFill pf;
-      pf.f = u;
-      pf.dim = d;
-      pf.idx = right ? 1 + c->geo.n[d] : 1;
-      pf.base = bound[side];
-      pf.wave = 0.0;
-      pf.waist = 0.0;
-      pf.off[0] = pf.off[1] = pf.off[2] = 0.0;
-      launch_plane_fill_f64(c, pf);
-    }
-  }
-}
-
 int require_plain(emo_ctx *c, const char *what) {
   if (c->cfg.precision != EMO_FP64_PLAIN)
     return set_err(c, EMO_ERR_UNSUPPORTED, std::string(what) + " is a separate phase only in the FP64 plain form");
@@ -552,7 +436,7 @@ int emo_sort(emo_ctx *c) {
   if (!c) return EMO_ERR_CONFIG;
   int rc = require_plain(c, "emo_sort");
   if (rc) return rc;
-  rc = sort_f64(c);
+  rc = sort_f64(c, 1);
   if (rc) return rc;
   return last_launch(c);
 }
@@ -580,7 +464,7 @@ int emo_deposit(emo_ctx *c, int32_t rho) {
   int rc = require_plain(c, "emo_deposit");
   if (rc) return rc;
   if (!c->sorted) {
-    rc = sort_f64(c);
+    rc = sort_f64(c, 1);
     if (rc) return rc;
   }
   launch_deposit_f64(c, rho ? 1 : 0);
@@ -607,7 +491,8 @@ int emo_refresh(emo_ctx *c, double t, double dt) {
   if (!c) return EMO_ERR_CONFIG;
   int rc = require_plain(c, "emo_refresh");
   if (rc) return rc;
-  refresh_fields(c, t, dt);
+  FieldSet<double> v = f64_view(c);
+  refresh_fields<double>(c, v, t, dt);
   return last_launch(c);
 }
 
@@ -628,7 +513,7 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
   const Geo &g = c->geo;
   size_t nt = (size_t)g.ntot;
   // deposit + fold (src/main.cpp:160-166)
-  int rc = sort_f64(c);
+  int rc = sort_f64(c, 1);
   if (rc) return rc;
   launch_deposit_f64(c, 1);
   launch_fold_f64(c);
@@ -682,7 +567,7 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
   for (int s = 0; s < 4 && !rc; s++) {
     double *u1 = u[2 * s], *u2 = u[2 * s + 1];
     double tl = s == 0 ? tphi : tA;
-    potential_bc(c, u1, bounds[s]);
+    potential_bc<double>(c, u1, bounds[s]);
     int iternum = -1;
     bool converged = false;
     do {
@@ -690,7 +575,7 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
       const double *src = (iternum % 2 == 0) ? u1 : u2;
       double *dst = (iternum % 2 == 0) ? u2 : u1;
       launch_jacobi_f64(c, src, dst, srcs[s], ax, ay, az, af, sourceMult);
-      potential_bc(c, dst, bounds[s]);
+      potential_bc<double>(c, dst, bounds[s]);
       launch_residual_f64(c, u1, u2, partials, nb);
       // per-block max|du| and sum du^2, combined on the host in block order
       hipError_t e = hipMemcpyAsync(hpart.data(), partials, 2 * nb * sizeof(double),
@@ -714,7 +599,8 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
   }
   if (!rc) {
     launch_a_to_b_f64(c, u[2], u[4], u[6]);
-    refresh_fields(c, 0.0, 0.0);  // executeBC(-1) at time_phys = dt_phys = 0
+    FieldSet<double> v = f64_view(c);
+    refresh_fields<double>(c, v, 0.0, 0.0);  // executeBC(-1) at time_phys = dt_phys = 0
     launch_gather_f64(c);         // src/main.cpp:179
   }
   hipStreamSynchronize(c->stream);
@@ -737,12 +623,15 @@ int emo_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
   for (int s = 0; s < nsteps; s++) {
     launch_push_f64(c, dt);
     launch_particle_bc_f64(c);
-    int rc = sort_f64(c);
+    int rc = sort_f64(c, 0);  // the gather at the end of the step refills the fields
     if (rc) return rc;
     launch_deposit_f64(c, 1);
     launch_fold_f64(c);
     launch_evolve_f64(c, dt);
-    refresh_fields(c, t, dt);
+    {
+      FieldSet<double> v = f64_view(c);
+      refresh_fields<double>(c, v, t, dt);
+    }
     launch_gather_f64(c);
     t += dt;
   }

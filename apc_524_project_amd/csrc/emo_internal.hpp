@@ -47,7 +47,7 @@ struct SpeciesParams {
 
 // Pulse scalars evaluated on the host for one boundary plane fill.
 struct PlaneFill {
-  double *f;      // field array (member pointer)
+  void *f;        // field array (member pointer), double or float
   int dim;        // plane normal
   int idx;        // plane index along dim
   double base;    // E0/B0 component
@@ -55,6 +55,21 @@ struct PlaneFill {
   double waist;   // <=0: uniform
   double off[3];  // staggering offsets of this field (Grid::getGridPhys)
 };
+
+// One precision's grid arrays.  Bfix are the fixed allocations that the
+// reference's Grid::fieldPtr_ keeps (src/grid/grid.cpp:321-342); B/Bt are
+// the member pointers that Grid::evolveFields swaps (src/grid/oGrid.cpp:18-20).
+template <typename T>
+struct FieldSet {
+  T *E[3] = {nullptr, nullptr, nullptr};
+  T *B[3] = {nullptr, nullptr, nullptr};
+  T *Bt[3] = {nullptr, nullptr, nullptr};
+  T *Bfix[3] = {nullptr, nullptr, nullptr};
+  T *J[3] = {nullptr, nullptr, nullptr};
+  T *rho = nullptr;
+};
+
+struct ChosenState;  // FP32 chosen form (chosen.hip)
 
 // FP64 particle store (structure of arrays).
 struct ParticlesF64 {
@@ -106,6 +121,9 @@ struct emo_ctx {
   unsigned long long *dflag = nullptr;  // [0] error flags, [1] max bits, ...
   unsigned long long *hflag = nullptr;  // pinned host mirror
 
+  // FP32 chosen form
+  emo::ChosenState *ch = nullptr;
+
   // timing
   bool timing = false;
   struct Rec {
@@ -125,7 +143,9 @@ constexpr unsigned long long kFlagNonFinite = 2ull;
 void launch_gather_f64(emo_ctx *c);
 void launch_push_f64(emo_ctx *c, double dt);
 void launch_particle_bc_f64(emo_ctx *c);
-int sort_f64(emo_ctx *c);
+// with_fields = 0 leaves the per-particle gathered fields behind (valid
+// only when a gather follows before the next push)
+int sort_f64(emo_ctx *c, int with_fields);
 void launch_deposit_f64(emo_ctx *c, int rho);
 void launch_cell_ids(emo_ctx *c, int32_t *out);
 size_t sort_tmp_bytes_for(int64_t n, int bits);
@@ -148,6 +168,30 @@ void launch_diag_particles_f64(emo_ctx *c, double *partials, int nblocks);
 void launch_diag_fields_f64(emo_ctx *c, double *partials, int nblocks);
 void launch_reduce_partials(emo_ctx *c, const double *partials, int nblocks, int ncomp,
                             double *out);
+
+// ---- precision-generic grid kernels (field_kernels.hip), T = double or float ----
+template <typename T>
+void fk_plane_copy(emo_ctx *c, T *const *fields, int nf, int d, int src, int dst, int add);
+template <typename T>
+void fk_plane_fill(emo_ctx *c, const PlaneFill &pf);
+template <typename T>
+void fk_const_fill(emo_ctx *c, T *f, double v, long long n);
+template <typename T>
+void fk_fold(emo_ctx *c, FieldSet<T> &fs);
+// swaps fs.B <-> fs.Bt, then the E and B stencils
+template <typename T>
+void fk_evolve(emo_ctx *c, FieldSet<T> &fs, double dt);
+template <typename T>
+void fk_diag_fields(emo_ctx *c, const FieldSet<T> &fs, double *partials, int nblocks);
+
+// ---- boundary host logic (boundary.cpp) ----
+// Grid::executeBC(-1): periodic ghost copies (through Bfix for B) and LightBC
+template <typename T>
+void refresh_fields(emo_ctx *c, FieldSet<T> &fs, double t, double dt);
+// executeBC(potential) for a Jacobi array: periodic copy / PoissonBC plane
+template <typename T>
+void potential_bc(emo_ctx *c, T *u, const double *bound);
+FieldSet<double> f64_view(emo_ctx *c);
 
 // ---- FP32 chosen form (chosen.hip) ----
 int fp32_create(emo_ctx *c);

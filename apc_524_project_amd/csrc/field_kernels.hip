@@ -31,14 +31,16 @@ __device__ inline void phys_ijk(const Geo &g, long long t, int *i, int *j, int *
 // plane n_d+1 of (J_j, J_k, rho) into plane 1 over the physical range of the
 // other two dims (src/grid/spookyGrid.cpp:54-72, 286-425).
 // ---------------------------------------------------------------------------
+template <typename T>
 struct PlaneArgs {
   Geo g;
-  double *f[6];
+  T *f[6];
   int nf;
   int d, src, dst, add;
 };
 
-__global__ void __launch_bounds__(kBlock) k_plane_copy(PlaneArgs a) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_plane_copy(PlaneArgs<T> a) {
   const Geo &g = a.g;
   int d = a.d;
   int u = d == 0 ? 1 : 0;
@@ -58,7 +60,7 @@ __global__ void __launch_bounds__(kBlock) k_plane_copy(PlaneArgs a) {
   long long ps = gidx(g, is[0], is[1], is[2]);
   long long pd = gidx(g, id[0], id[1], id[2]);
   for (int f = 0; f < a.nf; f++) {
-    double val = a.f[f][ps];
+    T val = a.f[f][ps];
     if (a.add) a.f[f][pd] += val;
     else a.f[f][pd] = val;
   }
@@ -70,16 +72,18 @@ __global__ void __launch_bounds__(kBlock) k_plane_copy(PlaneArgs a) {
 // kernels: the B kernel must see every new E, including the stale ghost
 // planes that are only refreshed afterwards.
 // ---------------------------------------------------------------------------
+template <typename T>
 struct YeeArgs {
   Geo g;
-  double *E[3];
-  double *B[3];
-  const double *Bt[3];
-  const double *J[3];
-  double dt;
+  T *E[3];
+  T *B[3];
+  const T *Bt[3];
+  const T *J[3];
+  T dt;
 };
 
-__global__ void __launch_bounds__(kBlock) k_evolve_E(YeeArgs a) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_evolve_E(YeeArgs<T> a) {
   const Geo &g = a.g;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
@@ -87,17 +91,19 @@ __global__ void __launch_bounds__(kBlock) k_evolve_E(YeeArgs a) {
   phys_ijk(g, t, &i, &j, &k);
   long long c = gidx(g, i, j, k);
   const long long sx = g.sx, sy = g.sy;
-  const double dt = a.dt, dx_ = g.d[0], dy_ = g.d[1], dz_ = g.d[2];
-  const double *Btx = a.Bt[0], *Bty = a.Bt[1], *Btz = a.Bt[2];
+  const T dt = a.dt, dx_ = (T)g.d[0], dy_ = (T)g.d[1], dz_ = (T)g.d[2];
+  const T fpi = (T)(4 * kPi);
+  const T *Btx = a.Bt[0], *Bty = a.Bt[1], *Btz = a.Bt[2];
   a.E[0][c] = a.E[0][c] - dt * ((Bty[c] - Bty[c - 1]) / dz_ - (Btz[c] - Btz[c - sy]) / dy_ +
-                                4 * kPi * a.J[0][c]);
+                                fpi * a.J[0][c]);
   a.E[1][c] = a.E[1][c] - dt * ((Btz[c] - Btz[c - sx]) / dx_ - (Btx[c] - Btx[c - 1]) / dz_ +
-                                4 * kPi * a.J[1][c]);
+                                fpi * a.J[1][c]);
   a.E[2][c] = a.E[2][c] - dt * ((Btx[c] - Btx[c - sy]) / dy_ - (Bty[c] - Bty[c - sx]) / dx_ +
-                                4 * kPi * a.J[2][c]);
+                                fpi * a.J[2][c]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_evolve_B(YeeArgs a) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_evolve_B(YeeArgs<T> a) {
   const Geo &g = a.g;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
@@ -105,8 +111,8 @@ __global__ void __launch_bounds__(kBlock) k_evolve_B(YeeArgs a) {
   phys_ijk(g, t, &i, &j, &k);
   long long c = gidx(g, i, j, k);
   const long long sx = g.sx, sy = g.sy;
-  const double dt = a.dt, dx_ = g.d[0], dy_ = g.d[1], dz_ = g.d[2];
-  const double *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+  const T dt = a.dt, dx_ = (T)g.d[0], dy_ = (T)g.d[1], dz_ = (T)g.d[2];
+  const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
   a.B[0][c] = a.Bt[0][c] + dt * ((Ey[c + 1] - Ey[c]) / dz_ - (Ez[c + sy] - Ez[c]) / dy_);
   a.B[1][c] = a.Bt[1][c] + dt * ((Ez[c + sx] - Ez[c]) / dx_ - (Ex[c + 1] - Ex[c]) / dz_);
   a.B[2][c] = a.Bt[2][c] + dt * ((Ex[c + sy] - Ex[c]) / dy_ - (Ey[c + sx] - Ey[c]) / dx_);
@@ -124,6 +130,7 @@ struct FillArgs {
   double Lc[3];  // box centre per dim
 };
 
+template <typename T>
 __global__ void __launch_bounds__(kBlock) k_plane_fill(FillArgs a) {
   const Geo &g = a.g;
   const PlaneFill &pf = a.pf;
@@ -146,10 +153,11 @@ __global__ void __launch_bounds__(kBlock) k_plane_fill(FillArgs a) {
   } else {
     v = pf.base + pf.wave;
   }
-  pf.f[gidx(g, ii[0], ii[1], ii[2])] = v;
+  ((T *)pf.f)[gidx(g, ii[0], ii[1], ii[2])] = (T)v;
 }
 
-__global__ void __launch_bounds__(kBlock) k_const_fill(double *f, double v, long long n) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_const_fill(T *f, T v, long long n) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) f[t] = v;
 }
@@ -292,15 +300,17 @@ __global__ void __launch_bounds__(kBlock) k_diag_particles(DiagPArgs a) {
   block_reduce_store<5>(acc, a.partials);
 }
 
+template <typename T>
 struct DiagFArgs {
   Geo g;
-  const double *E[3];
-  const double *B[3];
+  const T *E[3];
+  const T *B[3];
   double *partials;  // 6 per block
 };
 
 #define EMO_SQR(x) (x * x)
-__global__ void __launch_bounds__(kBlock) k_diag_fields(DiagFArgs a) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_diag_fields(DiagFArgs<T> a) {
   const Geo &g = a.g;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   double vol = g.d[0] * g.d[1] * g.d[2];
@@ -310,12 +320,12 @@ __global__ void __launch_bounds__(kBlock) k_diag_fields(DiagFArgs a) {
     int i, j, k;
     phys_ijk(g, t, &i, &j, &k);
     long long p = gidx(g, i, j, k);
-    acc[0] += 0.5 * vol * EMO_SQR(0.5 * (a.E[0][p] + a.E[0][p + g.sx]));
-    acc[1] += 0.5 * vol * EMO_SQR(0.5 * (a.E[1][p] + a.E[1][p + g.sy]));
-    acc[2] += 0.5 * vol * EMO_SQR(0.5 * (a.E[2][p] + a.E[2][p + 1]));
-    acc[3] += 0.5 * vol * EMO_SQR(0.5 * (a.B[0][p] + a.B[0][p + g.sx]));
-    acc[4] += 0.5 * vol * EMO_SQR(0.5 * (a.B[1][p] + a.B[1][p + g.sy]));
-    acc[5] += 0.5 * vol * EMO_SQR(0.5 * (a.B[2][p] + a.B[2][p + 1]));
+    acc[0] += 0.5 * vol * EMO_SQR(0.5 * ((double)a.E[0][p] + (double)a.E[0][p + g.sx]));
+    acc[1] += 0.5 * vol * EMO_SQR(0.5 * ((double)a.E[1][p] + (double)a.E[1][p + g.sy]));
+    acc[2] += 0.5 * vol * EMO_SQR(0.5 * ((double)a.E[2][p] + (double)a.E[2][p + 1]));
+    acc[3] += 0.5 * vol * EMO_SQR(0.5 * ((double)a.B[0][p] + (double)a.B[0][p + g.sx]));
+    acc[4] += 0.5 * vol * EMO_SQR(0.5 * ((double)a.B[1][p] + (double)a.B[1][p + g.sy]));
+    acc[5] += 0.5 * vol * EMO_SQR(0.5 * ((double)a.B[2][p] + (double)a.B[2][p + 1]));
   }
   block_reduce_store<6>(acc, a.partials);
 }
@@ -341,50 +351,9 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partials(const double *partia
 
 }  // namespace
 
-void launch_fold_f64(emo_ctx *c) {
-  const Geo &g = c->geo;
-  for (int d = 0; d < 3; d++) {
-    if (c->cfg.fields_bound[2 * d] != EMO_BC_PERIODIC) continue;
-    PlaneArgs a;
-    a.g = g;
-    a.f[0] = d == 0 ? c->J[1] : c->J[0];
-    a.f[1] = d == 2 ? c->J[1] : c->J[2];
-    a.f[2] = c->rho;
-    a.nf = 3;
-    a.d = d;
-    a.src = g.n[d] + 1;
-    a.dst = 1;
-    a.add = 1;
-    int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
-    hipLaunchKernelGGL(k_plane_copy, dim3(grid_for((long long)g.n[u] * g.n[w])), dim3(kBlock), 0,
-                       c->stream, a);
-  }
-}
-
-void launch_evolve_f64(emo_ctx *c, double dt) {
-  // swap the member pointers (src/grid/oGrid.cpp:18-20)
-  for (int q = 0; q < 3; q++) std::swap(c->B[q], c->Bt[q]);
-  YeeArgs a;
-  a.g = c->geo;
-  for (int q = 0; q < 3; q++) {
-    a.E[q] = c->E[q];
-    a.B[q] = c->B[q];
-    a.Bt[q] = c->Bt[q];
-    a.J[q] = c->J[q];
-  }
-  a.dt = dt;
-  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
-  {
-    TimedLaunch t(c, "yee_E_f64");
-    hipLaunchKernelGGL(k_evolve_E, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
-  }
-  TimedLaunch t(c, "yee_B_f64");
-  hipLaunchKernelGGL(k_evolve_B, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
-}
-
-void launch_plane_copy_f64(emo_ctx *c, double *const *fields, int nf, int d, int src, int dst,
-                           int add) {
-  PlaneArgs a;
+template <typename T>
+void fk_plane_copy(emo_ctx *c, T *const *fields, int nf, int d, int src, int dst, int add) {
+  PlaneArgs<T> a;
   a.g = c->geo;
   for (int q = 0; q < nf; q++) a.f[q] = fields[q];
   a.nf = nf;
@@ -393,23 +362,118 @@ void launch_plane_copy_f64(emo_ctx *c, double *const *fields, int nf, int d, int
   a.dst = dst;
   a.add = add;
   int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
-  hipLaunchKernelGGL(k_plane_copy, dim3(grid_for((long long)c->geo.n[u] * c->geo.n[w])),
+  hipLaunchKernelGGL(k_plane_copy<T>, dim3(grid_for((long long)c->geo.n[u] * c->geo.n[w])),
                      dim3(kBlock), 0, c->stream, a);
 }
 
-void launch_plane_fill_f64(emo_ctx *c, const PlaneFill &pf) {
+template <typename T>
+void fk_plane_fill(emo_ctx *c, const PlaneFill &pf) {
   FillArgs a;
   a.g = c->geo;
   a.pf = pf;
   for (int q = 0; q < 3; q++) a.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
   int d = pf.dim;
   int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
-  hipLaunchKernelGGL(k_plane_fill, dim3(grid_for((long long)c->geo.nt[u] * c->geo.nt[w])),
+  hipLaunchKernelGGL(k_plane_fill<T>, dim3(grid_for((long long)c->geo.nt[u] * c->geo.nt[w])),
                      dim3(kBlock), 0, c->stream, a);
 }
 
+template <typename T>
+void fk_const_fill(emo_ctx *c, T *f, double v, long long n) {
+  hipLaunchKernelGGL(k_const_fill<T>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, f, (T)v, n);
+}
+
+template <typename T>
+void fk_fold(emo_ctx *c, FieldSet<T> &fs) {
+  const Geo &g = c->geo;
+  for (int d = 0; d < 3; d++) {
+    if (c->cfg.fields_bound[2 * d] != EMO_BC_PERIODIC) continue;
+    T *f[3] = {d == 0 ? fs.J[1] : fs.J[0], d == 2 ? fs.J[1] : fs.J[2], fs.rho};
+    fk_plane_copy<T>(c, f, fs.rho ? 3 : 2, d, g.n[d] + 1, 1, 1);
+  }
+}
+
+template <typename T>
+void fk_evolve(emo_ctx *c, FieldSet<T> &fs, double dt) {
+  for (int q = 0; q < 3; q++) std::swap(fs.B[q], fs.Bt[q]);
+  YeeArgs<T> a;
+  a.g = c->geo;
+  for (int q = 0; q < 3; q++) {
+    a.E[q] = fs.E[q];
+    a.B[q] = fs.B[q];
+    a.Bt[q] = fs.Bt[q];
+    a.J[q] = fs.J[q];
+  }
+  a.dt = (T)dt;
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  const bool f64 = sizeof(T) == 8;
+  {
+    TimedLaunch t(c, f64 ? "yee_E_f64" : "yee_E_f32");
+    hipLaunchKernelGGL(k_evolve_E<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+  }
+  TimedLaunch t(c, f64 ? "yee_B_f64" : "yee_B_f32");
+  hipLaunchKernelGGL(k_evolve_B<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+}
+
+template <typename T>
+void fk_diag_fields(emo_ctx *c, const FieldSet<T> &fs, double *partials, int nblocks) {
+  DiagFArgs<T> a;
+  a.g = c->geo;
+  for (int q = 0; q < 3; q++) {
+    a.E[q] = fs.E[q];
+    a.B[q] = fs.Bfix[q];  // Q23: the diagnostics read Grid::fieldPtr_
+  }
+  a.partials = partials;
+  hipLaunchKernelGGL(k_diag_fields<T>, dim3(nblocks), dim3(kBlock), 0, c->stream, a);
+}
+
+#define EMO_INSTANTIATE(T)                                                                       \
+  template void fk_plane_copy<T>(emo_ctx *, T *const *, int, int, int, int, int);                \
+  template void fk_plane_fill<T>(emo_ctx *, const PlaneFill &);                                  \
+  template void fk_const_fill<T>(emo_ctx *, T *, double, long long);                             \
+  template void fk_fold<T>(emo_ctx *, FieldSet<T> &);                                            \
+  template void fk_evolve<T>(emo_ctx *, FieldSet<T> &, double);                                  \
+  template void fk_diag_fields<T>(emo_ctx *, const FieldSet<T> &, double *, int);
+EMO_INSTANTIATE(double)
+EMO_INSTANTIATE(float)
+#undef EMO_INSTANTIATE
+
+FieldSet<double> f64_view(emo_ctx *c) {
+  FieldSet<double> v;
+  for (int q = 0; q < 3; q++) {
+    v.E[q] = c->E[q];
+    v.B[q] = c->B[q];
+    v.Bt[q] = c->Bt[q];
+    v.Bfix[q] = c->Bfix[q];
+    v.J[q] = c->J[q];
+  }
+  v.rho = c->rho;
+  return v;
+}
+
+void launch_fold_f64(emo_ctx *c) {
+  FieldSet<double> v = f64_view(c);
+  fk_fold<double>(c, v);
+}
+
+void launch_evolve_f64(emo_ctx *c, double dt) {
+  FieldSet<double> v = f64_view(c);
+  fk_evolve<double>(c, v, dt);
+  for (int q = 0; q < 3; q++) {
+    c->B[q] = v.B[q];
+    c->Bt[q] = v.Bt[q];
+  }
+}
+
+void launch_plane_copy_f64(emo_ctx *c, double *const *fields, int nf, int d, int src, int dst,
+                           int add) {
+  fk_plane_copy<double>(c, fields, nf, d, src, dst, add);
+}
+
+void launch_plane_fill_f64(emo_ctx *c, const PlaneFill &pf) { fk_plane_fill<double>(c, pf); }
+
 void launch_const_fill_f64(emo_ctx *c, double *f, double v, long long n) {
-  hipLaunchKernelGGL(k_const_fill, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, f, v, n);
+  fk_const_fill<double>(c, f, v, n);
 }
 
 void launch_jacobi_f64(emo_ctx *c, const double *src, double *dst, const double *R, double ax,
@@ -464,14 +528,8 @@ void launch_diag_particles_f64(emo_ctx *c, double *partials, int nblocks) {
 }
 
 void launch_diag_fields_f64(emo_ctx *c, double *partials, int nblocks) {
-  DiagFArgs a;
-  a.g = c->geo;
-  for (int q = 0; q < 3; q++) {
-    a.E[q] = c->E[q];
-    a.B[q] = c->Bfix[q];  // Q23: the diagnostics read Grid::fieldPtr_
-  }
-  a.partials = partials;
-  hipLaunchKernelGGL(k_diag_fields, dim3(nblocks), dim3(kBlock), 0, c->stream, a);
+  FieldSet<double> v = f64_view(c);
+  fk_diag_fields<double>(c, v, partials, nblocks);
 }
 
 void launch_reduce_partials(emo_ctx *c, const double *partials, int nblocks, int ncomp,

@@ -9,6 +9,7 @@
 // fixed-order edge/vertex sums.  No atomics on floating-point data.
 #include <hipcub/hipcub.hpp>
 
+#include "deposit_grid.hpp"
 #include "emo_internal.hpp"
 
 namespace emo {
@@ -272,6 +273,7 @@ struct PermuteArgs {
   ParticlesF64 src, dst;
   const uint32_t *perm;
   long long np;
+  int fields;  // 0: the gathered fields are stale (regathered later), not moved
 };
 
 __global__ void __launch_bounds__(kBlock) k_permute(PermuteArgs a) {
@@ -283,8 +285,10 @@ __global__ void __launch_bounds__(kBlock) k_permute(PermuteArgs a) {
     a.dst.x[d][i] = a.src.x[d][s];
     a.dst.v[d][i] = a.src.v[d][s];
   }
+  if (a.fields) {
 #pragma unroll
-  for (int d = 0; d < 6; d++) a.dst.fld[d][i] = a.src.fld[d][s];
+    for (int d = 0; d < 6; d++) a.dst.fld[d][i] = a.src.fld[d][s];
+  }
   a.dst.gam[i] = a.src.gam[s];
   a.dst.sp[i] = a.src.sp[s];
   a.dst.id[i] = a.src.id[s];
@@ -300,11 +304,16 @@ __global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint32_t *key, int
 }
 
 // ---------------------------------------------------------------------------
-// Deposit, stage 1: one thread per cell accumulates Depositor::deposit_particle_J
-// and _Rho (src/particles/deposit.cpp:12-62) over the cell's particles in
-// array order, with the cell's vertex from Grid::getCellVertex
-// (src/grid/oGrid.cpp:233-244).  Test particles are skipped
+// Deposit, stage 1: per cell, the sums of Depositor::deposit_particle_J and
+// _Rho (src/particles/deposit.cpp:12-62) over the cell's particles in array
+// order, with the cell's vertex from Grid::getCellVertex
+// (src/grid/oGrid.cpp:233-244); test particles are skipped
 // (src/particles/particle_handler.cpp:205).
+//
+// One wavefront per cell: the 64 lanes load 64 consecutive particles
+// (coalesced) and compute their 20 contributions in parallel; the rows go
+// to LDS and lanes 0..19 then add column q row by row, so every component is
+// summed in exactly the particle order of the sequential loop.
 // ---------------------------------------------------------------------------
 struct DepCellArgs {
   Geo g;
@@ -316,136 +325,85 @@ struct DepCellArgs {
   int rho;
 };
 
+constexpr int kDepWaves = kBlock / 64;
+constexpr int kDepNV = 20;       // 12 J + 8 rho
+constexpr int kDepStride = 21;   // padded row length in LDS
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ void __launch_bounds__(kBlock) k_deposit_cells(DepCellArgs a) {
-  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double rows[kDepWaves][64 * kDepStride];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
   const Geo &g = a.g;
-  if (c >= g.ncell) return;
-  double J[12];
-  double R[8];
+  const int nv = a.rho ? 20 : 12;
+  double *row = rows[w];
+  for (long long c = (long long)blockIdx.x * kDepWaves + w; c < g.ncell;
+       c += (long long)gridDim.x * kDepWaves) {
+    const int b = a.beg[c], e = a.end[c];
+    double acc = 0.0;
+    if (e > b) {
+      int iz = (int)(c % g.nc[2]);
+      int iy = (int)(((c - iz) / g.nc[2]) % g.nc[1]);
+      int ix = (int)((((c - iz) / g.nc[2]) - iy) / g.nc[1]);
+      double cellverts[3];
+      cellverts[0] = g.x0[0] + g.d[0] * ix;
+      cellverts[1] = g.x0[1] + g.d[1] * iy;
+      cellverts[2] = g.x0[2] + g.d[2] * iz;
+      const double *lcell = g.d;
+      const double cellVolume = lcell[0] * lcell[1] * lcell[2];
+      for (int base = b; base < e; base += 64) {
+        const int p = base + lane;
+        bool valid = false;
+        if (p < e) {
+          int sp = a.sp[p];
+          if (!a.s.test[sp]) {
+            valid = true;
+            double pcharge = a.s.q[sp];
+            double relpos[3] = {a.x[0][p] - cellverts[0], a.x[1][p] - cellverts[1], a.x[2][p] - cellverts[2]};
+            double vel[3] = {a.v[0][p], a.v[1][p], a.v[2][p]};
+            double *r = row + lane * kDepStride;
 #pragma unroll
-  for (int q = 0; q < 12; q++) J[q] = 0.0;
-#pragma unroll
-  for (int q = 0; q < 8; q++) R[q] = 0.0;
-  int b = a.beg[c], e = a.end[c];
-  if (e > b) {
-    int iz = (int)(c % g.nc[2]);
-    int iy = (int)(((c - iz) / g.nc[2]) % g.nc[1]);
-    int ix = (int)((((c - iz) / g.nc[2]) - iy) / g.nc[1]);
-    double cellverts[3];
-    cellverts[0] = g.x0[0] + g.d[0] * ix;
-    cellverts[1] = g.x0[1] + g.d[1] * iy;
-    cellverts[2] = g.x0[2] + g.d[2] * iz;
-    const double *lcell = g.d;
-    double cellVolume = lcell[0] * lcell[1] * lcell[2];
-    for (int p = b; p < e; p++) {
-      int sp = a.sp[p];
-      if (a.s.test[sp]) continue;
-      double pcharge = a.s.q[sp];
-      double pos[3] = {a.x[0][p], a.x[1][p], a.x[2][p]};
-      double vel[3] = {a.v[0][p], a.v[1][p], a.v[2][p]};
-      double relpos[3];
-#pragma unroll
-      for (int q = 0; q < 3; q++) relpos[q] = pos[q] - cellverts[q];
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        int j = (q + 1) % 3, k = (q + 2) % 3;
-        double cellArea = lcell[j] * lcell[k];
-        double curIn1D = kUnitRhoJ * pcharge * vel[q];
-        double cur2AreaRatio = (curIn1D / cellVolume) / cellArea;
-        J[4 * q] += cur2AreaRatio * (lcell[j] - relpos[j]) * (lcell[k] - relpos[k]);
-        J[4 * q + 1] += cur2AreaRatio * relpos[j] * (lcell[k] - relpos[k]);
-        J[4 * q + 2] += cur2AreaRatio * relpos[j] * relpos[k];
-        J[4 * q + 3] += cur2AreaRatio * (lcell[j] - relpos[j]) * relpos[k];
-      }
-      if (a.rho) {
-        double cc = kUnitRhoJ * (pcharge / cellVolume) / cellVolume;
-        R[0] += cc * (lcell[0] - relpos[0]) * (lcell[1] - relpos[1]) * (lcell[2] - relpos[2]);
-        R[1] += cc * relpos[0] * (lcell[1] - relpos[1]) * (lcell[2] - relpos[2]);
-        R[2] += cc * relpos[0] * relpos[1] * (lcell[2] - relpos[2]);
-        R[3] += cc * (lcell[0] - relpos[0]) * relpos[1] * (lcell[2] - relpos[2]);
-        R[4] += cc * (lcell[0] - relpos[0]) * relpos[1] * relpos[2];
-        R[5] += cc * relpos[0] * relpos[1] * relpos[2];
-        R[6] += cc * relpos[0] * (lcell[1] - relpos[1]) * relpos[2];
-        R[7] += cc * (lcell[0] - relpos[0]) * (lcell[1] - relpos[1]) * relpos[2];
+            for (int q = 0; q < 3; q++) {
+              int j = (q + 1) % 3, k = (q + 2) % 3;
+              double cellArea = lcell[j] * lcell[k];
+              double curIn1D = kUnitRhoJ * pcharge * vel[q];
+              double cur2AreaRatio = (curIn1D / cellVolume) / cellArea;
+              r[4 * q] = cur2AreaRatio * (lcell[j] - relpos[j]) * (lcell[k] - relpos[k]);
+              r[4 * q + 1] = cur2AreaRatio * relpos[j] * (lcell[k] - relpos[k]);
+              r[4 * q + 2] = cur2AreaRatio * relpos[j] * relpos[k];
+              r[4 * q + 3] = cur2AreaRatio * (lcell[j] - relpos[j]) * relpos[k];
+            }
+            if (a.rho) {
+              double cc = kUnitRhoJ * (pcharge / cellVolume) / cellVolume;
+              r[12] = cc * (lcell[0] - relpos[0]) * (lcell[1] - relpos[1]) * (lcell[2] - relpos[2]);
+              r[13] = cc * relpos[0] * (lcell[1] - relpos[1]) * (lcell[2] - relpos[2]);
+              r[14] = cc * relpos[0] * relpos[1] * (lcell[2] - relpos[2]);
+              r[15] = cc * (lcell[0] - relpos[0]) * relpos[1] * (lcell[2] - relpos[2]);
+              r[16] = cc * (lcell[0] - relpos[0]) * relpos[1] * relpos[2];
+              r[17] = cc * relpos[0] * relpos[1] * relpos[2];
+              r[18] = cc * relpos[0] * (lcell[1] - relpos[1]) * relpos[2];
+              r[19] = cc * (lcell[0] - relpos[0]) * (lcell[1] - relpos[1]) * relpos[2];
+            }
+          }
+        }
+        const unsigned long long m = __ballot(valid);
+        wave_sync();
+        const int cnt = min(64, e - base);
+        if (lane < nv) {
+          for (int rr = 0; rr < cnt; rr++)
+            if ((m >> rr) & 1ull) acc += row[rr * kDepStride + lane];
+        }
+        wave_sync();
       }
     }
-  }
-  // per-cell vectors, component-major so that stage 2 reads coalesce
-#pragma unroll
-  for (int q = 0; q < 12; q++) a.cj[q * g.ncell + c] = J[q];
-  if (a.rho) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) a.cr[q * g.ncell + c] = R[q];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Deposit, stage 2: every padded grid point sums the cell-vector entries that
-// Grid::addJ / addRho (src/grid/oGrid.cpp:91-139) would add to it, in
-// ascending cell-id order, starting from zero (constJ/constRho).  Includes
-// addJ's duplicate Jx[ix][iy+1][iz] target (entries 1 and 3).
-// ---------------------------------------------------------------------------
-struct DepGridArgs {
-  Geo g;
-  const double *cj, *cr;
-  double *J[3], *rho;
-  int do_rho;
-};
-
-__device__ inline bool cell_ok(const Geo &g, int i, int j, int k) {
-  return i >= 0 && j >= 0 && k >= 0 && i < g.nc[0] && j < g.nc[1] && k < g.nc[2];
-}
-__device__ inline long long cid(const Geo &g, int i, int j, int k) {
-  return ((long long)i * g.nc[1] + j) * g.nc[2] + k;
-}
-
-__global__ void __launch_bounds__(kBlock) k_deposit_grid(DepGridArgs a) {
-  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const Geo &g = a.g;
-  if (p >= g.ntot) return;
-  int k = (int)(p % g.nt[2]);
-  int j = (int)((p / g.nt[2]) % g.nt[1]);
-  int i = (int)(p / ((long long)g.nt[1] * g.nt[2]));
-  const long long N = g.ncell;
-  const double *cj = a.cj;
-  double acc;
-  // Jx(i,j,k) <- J2(i,j-1,k-1), J1(i,j-1,k), J3(i,j-1,k), J0(i,j,k)
-  acc = 0.0;
-  if (cell_ok(g, i, j - 1, k - 1)) acc += cj[2 * N + cid(g, i, j - 1, k - 1)];
-  if (cell_ok(g, i, j - 1, k)) {
-    acc += cj[1 * N + cid(g, i, j - 1, k)];
-    acc += cj[3 * N + cid(g, i, j - 1, k)];
-  }
-  if (cell_ok(g, i, j, k)) acc += cj[0 * N + cid(g, i, j, k)];
-  a.J[0][p] = acc;
-  // Jy(i,j,k) <- J6(i-1,j,k-1), J7(i-1,j,k), J5(i,j,k-1), J4(i,j,k)
-  acc = 0.0;
-  if (cell_ok(g, i - 1, j, k - 1)) acc += cj[6 * N + cid(g, i - 1, j, k - 1)];
-  if (cell_ok(g, i - 1, j, k)) acc += cj[7 * N + cid(g, i - 1, j, k)];
-  if (cell_ok(g, i, j, k - 1)) acc += cj[5 * N + cid(g, i, j, k - 1)];
-  if (cell_ok(g, i, j, k)) acc += cj[4 * N + cid(g, i, j, k)];
-  a.J[1][p] = acc;
-  // Jz(i,j,k) <- J10(i-1,j-1,k), J9(i-1,j,k), J11(i,j-1,k), J8(i,j,k)
-  acc = 0.0;
-  if (cell_ok(g, i - 1, j - 1, k)) acc += cj[10 * N + cid(g, i - 1, j - 1, k)];
-  if (cell_ok(g, i - 1, j, k)) acc += cj[9 * N + cid(g, i - 1, j, k)];
-  if (cell_ok(g, i, j - 1, k)) acc += cj[11 * N + cid(g, i, j - 1, k)];
-  if (cell_ok(g, i, j, k)) acc += cj[8 * N + cid(g, i, j, k)];
-  a.J[2][p] = acc;
-  if (a.do_rho) {
-    // rho(i,j,k) <- R5(i-1,j-1,k-1), R2(i-1,j-1,k), R6(i-1,j,k-1), R1(i-1,j,k),
-    //               R4(i,j-1,k-1), R3(i,j-1,k), R7(i,j,k-1), R0(i,j,k)
-    const double *cr = a.cr;
-    acc = 0.0;
-    if (cell_ok(g, i - 1, j - 1, k - 1)) acc += cr[5 * N + cid(g, i - 1, j - 1, k - 1)];
-    if (cell_ok(g, i - 1, j - 1, k)) acc += cr[2 * N + cid(g, i - 1, j - 1, k)];
-    if (cell_ok(g, i - 1, j, k - 1)) acc += cr[6 * N + cid(g, i - 1, j, k - 1)];
-    if (cell_ok(g, i - 1, j, k)) acc += cr[1 * N + cid(g, i - 1, j, k)];
-    if (cell_ok(g, i, j - 1, k - 1)) acc += cr[4 * N + cid(g, i, j - 1, k - 1)];
-    if (cell_ok(g, i, j - 1, k)) acc += cr[3 * N + cid(g, i, j - 1, k)];
-    if (cell_ok(g, i, j, k - 1)) acc += cr[7 * N + cid(g, i, j, k - 1)];
-    if (cell_ok(g, i, j, k)) acc += cr[0 * N + cid(g, i, j, k)];
-    a.rho[p] = acc;
+    // per-cell vectors, component-major so that stage 2 reads coalesce
+    if (lane < 12) a.cj[lane * g.ncell + c] = acc;
+    else if (lane < nv) a.cr[(lane - 12) * g.ncell + c] = acc;
   }
 }
 
@@ -522,7 +480,7 @@ size_t sort_tmp_bytes_for(int64_t n, int bits) {
 // Stable sort of the particle store by ascending cell id: keys, a stable
 // LSD radix sort of (key, index) pairs over the cell-id bits only, a payload
 // permutation into the alternate buffers, and per-cell [beg,end) ranges.
-int sort_f64(emo_ctx *c) {
+int sort_f64(emo_ctx *c, int with_fields) {
   if (c->np == 0) {
     c->sorted = true;
     return EMO_OK;
@@ -554,11 +512,14 @@ int sort_f64(emo_ctx *c) {
   pa.dst = c->palt;
   pa.perm = c->perm_alt;
   pa.np = c->np;
+  pa.fields = with_fields;
   {
     TimedLaunch t(c, "sort_permute");
     hipLaunchKernelGGL(k_permute, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, pa);
   }
   std::swap(c->p, c->palt);
+  if (!with_fields)
+    for (int d = 0; d < 6; d++) std::swap(c->p.fld[d], c->palt.fld[d]);
   std::swap(c->key, c->key_alt);
   EMO_HIP_CHECK_RET(c, hipMemsetAsync(c->cbeg, 0, c->geo.ncell * sizeof(int32_t), c->stream));
   EMO_HIP_CHECK_RET(c, hipMemsetAsync(c->cend, 0, c->geo.ncell * sizeof(int32_t), c->stream));
@@ -584,9 +545,11 @@ void launch_deposit_f64(emo_ctx *c, int rho) {
   a.rho = rho;
   {
     TimedLaunch t(c, "deposit_cells_f64");
-    hipLaunchKernelGGL(k_deposit_cells, dim3(grid_for(c->geo.ncell)), dim3(kBlock), 0, c->stream, a);
+    long long blocks = (c->geo.ncell + kDepWaves - 1) / kDepWaves;
+    hipLaunchKernelGGL(k_deposit_cells, dim3((unsigned)std::min<long long>(blocks, 1 << 20)), dim3(kBlock), 0,
+                       c->stream, a);
   }
-  DepGridArgs b;
+  DepGridArgs<double> b;
   b.g = c->geo;
   b.cj = c->cj;
   b.cr = c->cr;
@@ -594,7 +557,7 @@ void launch_deposit_f64(emo_ctx *c, int rho) {
   b.rho = c->rho;
   b.do_rho = rho;
   TimedLaunch t(c, "deposit_grid_f64");
-  hipLaunchKernelGGL(k_deposit_grid, dim3(grid_for(c->geo.ntot)), dim3(kBlock), 0, c->stream, b);
+  hipLaunchKernelGGL(k_deposit_grid<double>, dim3(grid_for(c->geo.ntot)), dim3(kBlock), 0, c->stream, b);
 }
 
 void launch_cell_ids(emo_ctx *c, int32_t *out) {

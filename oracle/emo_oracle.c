@@ -377,6 +377,16 @@ int32_t eo_cell_id(const eo_state *s, double x, double y, double z) {
   return (s->nc[1] * s->nc[2]) * ix + s->nc[2] * iy + iz;
 }
 
+/* -1 for a position outside the padded grid (or NaN): the reference would
+   index out of bounds there; the oracle reports it instead. */
+static int32_t cell_checked(const eo_state *s, const double *p) {
+  for (int d = 0; d < 3; d++) {
+    double u = (p[d] - s->c.x0[d]) * s->id[d];
+    if (!(u >= 0.0 && u < (double)(s->c.n[d] + 1))) return -1;
+  }
+  return eo_cell_id(s, p[0], p[1], p[2]);
+}
+
 static inline void cell_decode(const eo_state *s, int32_t cid, int *ix, int *iy, int *iz) {
   /* src/grid/oGrid.cpp:155-157 */
   int z = cid % s->nc[2];
@@ -515,8 +525,10 @@ void eo_particle_bc(eo_state *s) {
 /* std::sort, src/particles/particle_utils.hpp:31-50; only cell membership  */
 /* is part of the contract).                                                 */
 /* ------------------------------------------------------------------------ */
-void eo_sort(eo_state *s) {
+int eo_sort(eo_state *s) {
   int64_t np = s->np;
+  for (int64_t i = 0; i < np; i++)
+    if (cell_checked(s, s->x + 3 * i) < 0) return -4;
   memset(s->cstart, 0, (s->ncell + 1) * sizeof(int64_t));
   for (int64_t i = 0; i < np; i++) {
     int32_t c = eo_cell_id(s, s->x[3 * i], s->x[3 * i + 1], s->x[3 * i + 2]);
@@ -543,6 +555,7 @@ void eo_sort(eo_state *s) {
   t = s->gam; s->gam = s->tg; s->tg = t;
   int32_t *ti = s->sp; s->sp = s->tsp; s->tsp = ti;
   int64_t *tl = s->pid; s->pid = s->tid; s->tid = tl;
+  return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1118,13 +1131,14 @@ void eo_diag(const eo_state *s, double *out) {
 }
 
 /* one step, src/main.cpp:231-272 (sort placed after the particle BCs) */
-void eo_step(eo_state *s, double t, double dt) {
+int eo_step(eo_state *s, double t, double dt) {
   eo_push(s, dt);
   eo_particle_bc(s);
-  eo_sort(s);
+  if (eo_sort(s) != 0) return -4;
   eo_deposit(s, 1);
   eo_fold(s);
   eo_evolve(s, dt);
   eo_refresh(s, t, dt);
   eo_gather(s);
+  return 0;
 }

@@ -118,7 +118,8 @@ double eo_rng_gaussian(eo_rng *r, double mu, double sigma);
 /* ---- the per-step phases ------------------------------------------------ */
 void eo_push(eo_state *s, double dt);
 void eo_particle_bc(eo_state *s);
-void eo_sort(eo_state *s);
+/* -4 (nothing sorted) when a particle lies outside the padded grid or is NaN */
+int eo_sort(eo_state *s);
 void eo_deposit(eo_state *s, int deposit_rho);
 void eo_fold(eo_state *s);
 void eo_evolve(eo_state *s, double dt);
@@ -143,7 +144,7 @@ void eo_const_init(eo_state *s);
 void eo_diag(const eo_state *s, double *out8);
 
 /* one full step in the reference order (src/main.cpp:231-275) */
-void eo_step(eo_state *s, double t, double dt);
+int eo_step(eo_state *s, double t, double dt); /* -4: see eo_sort */
 
 /* the pulse scalar used by LightBC (src/utils/evaluateGaussianPulse.cpp:7-30) */
 double eo_gaussian_pulses(int nw, const double *amp, const double *omg, const double *pha,

@@ -16,10 +16,12 @@
 #define MAIN_CPP
 #include "globals.hpp"
 
+#include <fcntl.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 #include <vector>
 
@@ -74,6 +76,28 @@ struct Harness {
   char boxname[128];
 };
 
+// The reference reports progress with printf; the harness sends stdout to
+// /dev/null while it runs reference code so that test output stays readable.
+struct QuietStdout {
+  int saved = -1;
+  QuietStdout() {
+    fflush(stdout);
+    saved = dup(1);
+    int devnull = open("/dev/null", O_WRONLY);
+    if (devnull >= 0) {
+      dup2(devnull, 1);
+      close(devnull);
+    }
+  }
+  ~QuietStdout() {
+    fflush(stdout);
+    if (saved >= 0) {
+      dup2(saved, 1);
+      close(saved);
+    }
+  }
+};
+
 const char *pbc_name(int k) { return k == EO_BC_PERIODIC ? "periodic" : "reflecting"; }
 const char *fbc_name(int k) { return k == EO_BC_PERIODIC ? "periodic" : "external"; }
 }  // namespace
@@ -87,6 +111,7 @@ extern "C" {
 void *rh_create(const eo_config *cfg, const eo_load_config *lc, int poisson, int nstep_sort,
                 int debug_level) {
   if (lc->seed != -1) return NULL;
+  QuietStdout quiet;
   Harness *h = new Harness();
   memset(h, 0, sizeof(Harness));
   size_MPI = 1;
@@ -282,6 +307,7 @@ int rh_get_field_ptr(void *hp, int fid, double *out) {
 
 /* src/main.cpp:159-179 (initial deposit, fold, InitializeFields, gather) */
 void rh_initialize(void *hp) {
+  QuietStdout quiet;
   Harness *h = (Harness *)hp;
   time_phys = 0.0;
   dt_phys = 0.0;
@@ -300,6 +326,7 @@ void rh_initialize(void *hp) {
 /* Poisson only, from the current rho/J: zero potentials and fields (Q8) and
  * run Poisson_Solver::InitializeFields */
 int rh_poisson_only(void *hp) {
+  QuietStdout quiet;
   Harness *h = (Harness *)hp;
   if (!h->poisson) return -1;
   time_phys = 0.0;

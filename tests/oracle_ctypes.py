@@ -375,7 +375,8 @@ class Oracle:
         self.lib.eo_particle_bc(self.h)
 
     def sort(self):
-        self.lib.eo_sort(self.h)
+        if self.lib.eo_sort(self.h) != 0:
+            raise RuntimeError("oracle: a particle left the padded grid (or is NaN)")
 
     def deposit(self, rho=True):
         self.lib.eo_deposit(self.h, 1 if rho else 0)
@@ -393,7 +394,8 @@ class Oracle:
         self.lib.eo_gather(self.h)
 
     def step(self, t, dt):
-        self.lib.eo_step(self.h, t, dt)
+        if self.lib.eo_step(self.h, t, dt) != 0:
+            raise RuntimeError("oracle: a particle left the padded grid (or is NaN)")
 
     def const_init(self):
         self.lib.eo_const_init(self.h)

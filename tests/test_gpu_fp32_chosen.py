@@ -1,0 +1,117 @@
+"""FP32 chosen form on the GPU against the FP64 CPU oracle.
+
+The chosen form runs the reference algorithm (same order, same defects) in
+FP32 with a cell-index + offset particle representation, so it is compared by
+relative L2 error, with the tolerance written in each test, and by
+determinism (two runs are bitwise identical).  All calls go through the C ABI.
+"""
+import numpy as np
+import pytest
+
+from apc_524_project_amd._lib import FP32_CHOSEN, Context
+from emo_case import emo_config
+from oracle_ctypes import (BX, BY, BZ, EX, EXTERNAL, EY, EZ, JX, JY, JZ, PERIODIC, REFLECTING, Case,
+                           Oracle, Wave, cell_ids_from_x)
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def setup(case):
+    o = Oracle(case)
+    assert o.load() == 0
+    ctx = Context(emo_config(case, precision=FP32_CHOSEN, q=o.q, m=o.m))
+    p = o.particles()
+    ctx.load_particles(p.x, p.v, p.gamma, p.species, p.id)
+    ctx.init_constant()
+    o.initialize()
+    return ctx, o
+
+
+def run(ctx, o, steps, dt):
+    ctx.step(steps, 0.0, dt)
+    t = 0.0
+    for _ in range(steps):
+        o.step(t, dt)
+        t += dt
+
+
+def by_id(ctx):
+    p = ctx.particles(fields=False)
+    order = np.argsort(p["id"])
+    return {k: v[order] for k, v in p.items() if v is not None}
+
+
+THERMAL = dict(n=(8, 8, 8), L=(8.0, 8.0, 8.0), nparticles=2 * 16 * 512, temp=(5000.0, 5000.0),
+               dens_phys=2.824e9, E0=(0.001, -0.002, 0.003), B0=(0.005, 0.002, -0.004))
+
+
+@pytest.mark.parametrize("relativity", [0, 1])
+def test_fp32_tracks_fp64_oracle(relativity):
+    case = Case(**THERMAL, relativity=relativity)
+    ctx, o = setup(case)
+    dt = case.dt_cfl()
+    run(ctx, o, 20, dt)
+    g = by_id(ctx)
+    p = o.particles().by_id()
+    assert np.array_equal(g["id"], p.id)
+    # phase space: FP32 offsets/momenta, 20 steps
+    assert rel(g["x"], p.x) < 1e-6
+    assert rel(g["v"], p.v) < 1e-4
+    # cell membership: all but a few particles that sit within FP32
+    # resolution of a cell face
+    same = np.mean(cell_ids_from_x(case, g["x"]) == cell_ids_from_x(case, p.x))
+    assert same > 0.999
+    for f in (EX, EY, EZ, BX, BY, BZ, JX, JY, JZ):
+        assert rel(ctx.field(f), o.field(f)) < 1e-3, f
+
+
+def test_fp32_lightbc_reflecting_tracks_oracle():
+    w = Wave(side=-1, pol=2, amp=0.3, omega=0.39, phase=30.0, inv_width=0.05, delay=5.0)
+    case = Case(n=(12, 6, 6), L=(12.0, 6.0, 6.0), nparticles=4608, temp=(5000.0, 5000.0),
+                dens_phys=4.355e9, relativity=1, waves=[w],
+                fbc=(EXTERNAL, EXTERNAL, PERIODIC, PERIODIC, PERIODIC, PERIODIC),
+                pbc=(REFLECTING, REFLECTING, PERIODIC, PERIODIC, PERIODIC, PERIODIC))
+    ctx, o = setup(case)
+    dt = case.dt_cfl()
+    run(ctx, o, 25, dt)
+    g = by_id(ctx)
+    p = o.particles().by_id()
+    assert rel(g["x"], p.x) < 1e-6
+    for f in (EY, BZ):
+        assert rel(ctx.field(f), o.field(f)) < 1e-5, f
+    assert g["x"][:, 0].min() >= 0.0 and g["x"][:, 0].max() <= 12.0
+
+
+def test_fp32_is_deterministic():
+    case = Case(**THERMAL, relativity=1)
+    outs = []
+    for _ in range(2):
+        ctx, o = setup(case)
+        ctx.step(10, 0.0, case.dt_cfl())
+        outs.append((ctx.field(JX), ctx.field(EY), ctx.particles(fields=False)["x"]))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_fp32_diag_close_to_oracle():
+    case = Case(**THERMAL, relativity=1)
+    ctx, o = setup(case)
+    dt = case.dt_cfl()
+    run(ctx, o, 5, dt)
+    a, b = ctx.diag(), o.diag()
+    np.testing.assert_allclose(a[:4], b[:4], rtol=1e-5)
+    np.testing.assert_allclose(a[4:7], b[4:7], rtol=2e-3)
+
+
+def test_fp32_rejects_fp64_only_phases():
+    case = Case(**THERMAL)
+    ctx, _ = setup(case)
+    from apc_524_project_amd._lib import EmoError
+    with pytest.raises(EmoError) as e:
+        ctx.push(0.1)
+    assert e.value.code == -6
