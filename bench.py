@@ -128,7 +128,7 @@ def roofline_for(res):
             "kernel_ms_per_step": {k: round(v[0] / v[1], 5) for k, v in sorted(kt.items())}}
 
 
-def cpu_baseline(sample_cells: int = 32, sample_steps: int = 10):
+def cpu_baseline(sample_cells: int = 64, sample_steps: int = 10):
     """The reference's serial loop (oracle/_ref) on one core, c2 physics on a
     sample_cells^3 sub-box at the same density (32 ppc/species)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -183,7 +183,7 @@ def main():
     ap.add_argument("--mode", default="auto", choices=["auto", "fp64", "fp32", "both"])
     ap.add_argument("--scale", type=float, default=1.0, help="particle-count scale (1 = full c2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cells", type=int, default=32)
+    ap.add_argument("--cpu-cells", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
