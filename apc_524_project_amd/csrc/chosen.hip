@@ -4,8 +4,9 @@
 // species in the top 5 bits and the reference's ghost-inclusive cell id
 // ((ny+2)(nz+2) ix + (nz+2) iy + iz, src/grid/oGrid.cpp:208-222) in the low
 // 26 bits, the in-cell offset f in [0,1)^3 (FP32) and u (FP32: v for Boris,
-// gamma*v for the relativistic pusher).  28 bytes per particle; optional
-// int64 ids ride along for comparisons against the FP64 oracle.
+// gamma*v for the relativistic pusher).  28 bytes per particle, plus an
+// optional int32 id that rides along for comparisons against the FP64 oracle.
+// Cell c holds the particles [off[c], off[c+1]) of the arrays.
 //
 // The algorithm is the reference's (same step order, same defects); only the
 // representation and precision change.  With x = x0 + dx (ix - 1 + f) the
@@ -14,18 +15,29 @@
 // f - 1 (src/particles/interpolate.cpp:25-33, src/particles/deposit.cpp:94-97).
 //
 // Per step:
-//   pgd_f32        gather (E, (B + B_tm1)/2) + push + particle BCs, one thread
-//                  per particle, new state written beside the old one
-//   sort           stable radix sort of the new cell ids, payload permutation
-//   deposit_f32    one wavefront per cell: each lane sums its particles' 12
-//                  edge currents with compensated (two-float) accumulation,
-//                  then a fixed butterfly combines the lanes' (hi, lo) pairs:
-//                  deterministic, no atomics
-//   jsum_f32       per grid point, the fixed-order sum of the cell vectors
+//   pgd_f32      gather (E, (B + B_tm1)/2) + push + particle BCs, one thread
+//                per particle; writes the new state beside the old one and
+//                the particle's move code: which of the <= 6 x 6 x 6 one-cell
+//                moves (periodic wraps included) took it to its new cell
+//   far_f32      (only when some particle moved further than one cell) an
+//                ordered list of those particles
+//   rank_f32     per source cell and move code, the particles' ranks in array
+//                order and the group sizes (8 source cells per wavefront)
+//   base_f32     per destination cell, the offsets of its (source, code)
+//                groups in ascending source id; its particle count
+//   scan_f32     exclusive scan of the counts: the new cell ranges
+//   scatter_f32  every particle to off[dst] + base + rank -- the stable
+//                counting sort by cell of the previous order, deterministic
+//                and without radix passes
+//   deposit_f32  16 lanes per cell: each lane sums its particles' 12 edge
+//                currents with compensated (two-float) accumulation, a fixed
+//                butterfly combines the lanes: deterministic, no atomics
+//   jsum_f32     per grid point, the fixed-order sum of the cell vectors
 //   fold, Yee E/B, ghost refresh   shared FP32 instances of the grid kernels
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -37,22 +49,35 @@ namespace emo {
 
 constexpr int kSpeciesShift = 26;
 constexpr int32_t kCellMask = (1 << kSpeciesShift) - 1;
+constexpr uint8_t kFarCode = 255;  // moved beyond the one-cell neighbourhood
+constexpr int kFarCap = 1 << 16;   // ordered far-mover list capacity per step
 
 struct ChosenState {
   int64_t cap = 0;
   int32_t *cs = nullptr, *cs_alt = nullptr;
   float *f[3] = {}, *f_alt[3] = {};
   float *u[3] = {}, *u_alt[3] = {};
-  int64_t *id = nullptr, *id_alt = nullptr;
+  int32_t *id = nullptr, *id_alt = nullptr;
   float *g0 = nullptr;  // stored gamma for the first relativistic push (Q3)
   bool g0_active = false;
+  uint8_t *code = nullptr;       // move code per particle
+  uint16_t *rank = nullptr;      // index within its (source cell, code) group
+  uint16_t *nsk = nullptr;       // [code * ncell + source] group sizes, zero between steps
+  uint16_t *gbase = nullptr;     // [code * ncell + source] group offsets in the destination
+  int32_t *count = nullptr;      // ncell + 1 particles per destination cell
+  int32_t *off = nullptr, *off_new = nullptr;  // ncell + 1 cell ranges
+  int32_t *misc = nullptr;       // [0]: far-mover count
+  int32_t *far_tiles = nullptr;  // far movers per tile
+  int4 *far = nullptr;           // (index, source cell, destination cell, 0)
+  int32_t *far_pos = nullptr;    // far mover's offset in its destination
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  // initial sort of loaded particles (arbitrary order)
   uint32_t *key = nullptr, *key_alt = nullptr, *perm = nullptr, *perm_alt = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   int sort_bits = 0;
-  int32_t *cbeg = nullptr, *cend = nullptr;
   float *cj = nullptr;  // 12 per cell, component-major
-  float *cr = nullptr;  // 8 per cell
   FieldSet<float> fs;
   float *Btfix[3] = {};
 };
@@ -60,6 +85,7 @@ struct ChosenState {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kFarTile = 4096;
 
 inline unsigned grid_for(long long n, int block = kBlock) {
   long long g = (n + block - 1) / block;
@@ -91,6 +117,48 @@ __device__ inline void decode_cell(const Geo &g, int cell, int *ix, int *iy, int
   int t = cell / g.nc[2];
   *iy = t % g.nc[1];
   *ix = t / g.nc[1];
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline unsigned long long lanes_below(int lane) {
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// One-dimensional move between the source index s and destination index t
+// (padded indices): 0, 1, 2 for t - s = -1, 0, +1; with periodic particle
+// BCs 3, 4, 5 for the wraps t - s = n - 1, 1 - n, -n (a particle leaving
+// through the lower / upper face, or from the upper ghost plane where a
+// particle sitting exactly on x = L lives, src/boundaries/b_particles/
+// bc_p_periodic.cpp:44-87).  -1: not a one-cell move.
+__device__ inline int move_code(int sig, int n, int per) {
+  if (sig >= -1 && sig <= 1) return sig + 1;
+  if (per) {
+    // a wrap whose shift equals a one-cell move (n <= 2) takes that code
+    if (n >= 3 && sig == n - 1) return 3;
+    if (n >= 3 && sig == 1 - n) return 4;
+    if (n >= 2 && sig == -n) return 5;
+  }
+  return -1;
+}
+// whether move code k can occur in a dimension of n cells
+__device__ inline bool move_live(int k, int n, int per) {
+  if (k < 3) return true;
+  return per && (k == 5 ? n >= 2 : n >= 3);
+}
+__device__ inline int move_sigma(int k, int n) {
+  switch (k) {
+    case 0: return -1;
+    case 1: return 0;
+    case 2: return 1;
+    case 3: return n - 1;
+    case 4: return 1 - n;
+    default: return -n;
+  }
 }
 
 // The particle BCs of the six sides in the reference's order
@@ -130,7 +198,7 @@ __device__ inline void bc_dim(int &ix, float &f, float &u, int n, int left, int 
 }
 
 // ---------------------------------------------------------------------------
-// Fused gather + push + particle BCs
+// Fused gather + push + particle BCs + move code + arrival counts
 // ---------------------------------------------------------------------------
 struct PgdArgs {
   Geo g;
@@ -139,20 +207,26 @@ struct PgdArgs {
   const float *g0;
   int32_t *cs_out;
   float *f_out[3], *u_out[3];
-  uint32_t *key, *perm;
+  uint8_t *code;
+  int32_t *nfar;   // particles that moved beyond one cell
   const float *E[3], *B[3], *Bt[3];
   SpeciesF32 s;
   float dtid[3];  // dt / dx per dim
   int pbc[6];
+  int per[3];     // periodic particle BCs per dim
   unsigned long long *flag;
   long long np;
 };
 
 template <bool REL>
 __global__ void __launch_bounds__(kBlock) k_pgd_f32(PgdArgs a) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = i < a.np;
   const Geo &g = a.g;
+  int cell = -1;
+  int mcode = 0;
+  if (live) {
   int w = a.cs[i];
   int sp = (int)((uint32_t)w >> kSpeciesShift);
   int ix, iy, iz;
@@ -219,6 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_pgd_f32(PgdArgs a) {
   }
   // x += dt v, in cell units, split into cell index and offset
   float pf[3] = {fx + a.dtid[0] * vx, fy + a.dtid[1] * vy, fz + a.dtid[2] * vz};
+  int src[3] = {ix, iy, iz};
   int ic[3] = {ix, iy, iz};
   float uu[3] = {ux, uy, uz};
 #pragma unroll
@@ -239,54 +314,89 @@ __global__ void __launch_bounds__(kBlock) k_pgd_f32(PgdArgs a) {
     ic[1] = min(max(ic[1], 1), g.n[1]);
     ic[2] = min(max(ic[2], 1), g.n[2]);
   }
-  int cell = (ic[0] * g.nc[1] + ic[1]) * g.nc[2] + ic[2];
+  cell = (ic[0] * g.nc[1] + ic[1]) * g.nc[2] + ic[2];
+  int k0 = move_code(ic[0] - src[0], g.n[0], a.per[0]);
+  int k1 = move_code(ic[1] - src[1], g.n[1], a.per[1]);
+  int k2 = move_code(ic[2] - src[2], g.n[2], a.per[2]);
+  mcode = (k0 < 0 || k1 < 0 || k2 < 0) ? (int)kFarCode : (k0 * 6 + k1) * 6 + k2;
   a.cs_out[i] = (sp << kSpeciesShift) | cell;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     a.f_out[d][i] = pf[d];
     a.u_out[d][i] = uu[d];
   }
-  a.key[i] = (uint32_t)cell;
-  a.perm[i] = (uint32_t)i;
-}
-
-struct PermF32Args {
-  const int32_t *cs;
-  const float *f[3], *u[3];
-  const int64_t *id;
-  int32_t *cs_out;
-  float *f_out[3], *u_out[3];
-  int64_t *id_out;
-  const uint32_t *perm;
-  long long np;
-};
-
-__global__ void __launch_bounds__(kBlock) k_permute_f32(PermF32Args a) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
-  uint32_t s = a.perm[i];
-  a.cs_out[i] = a.cs[s];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    a.f_out[d][i] = a.f[d][s];
-    a.u_out[d][i] = a.u[d][s];
+  a.code[i] = (uint8_t)mcode;
   }
-  if (a.id) a.id_out[i] = a.id[s];
-}
-
-__global__ void __launch_bounds__(kBlock) k_cell_bounds_f32(const uint32_t *key, int32_t *beg, int32_t *end,
-                                                             long long np) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  uint32_t k = key[i];
-  if (i == 0 || key[i - 1] != k) beg[k] = (int32_t)i;
-  if (i == np - 1 || key[i + 1] != k) end[k] = (int32_t)(i + 1);
+  const int nf = __popcll(__ballot(live && mcode == kFarCode));
+  if (lane == 0 && nf) atomicAdd(a.nfar, nf);
 }
 
 // ---------------------------------------------------------------------------
-// Deposit: one wavefront per cell.  Knuth's TwoSum keeps the rounding error
-// of every addition; the (hi, lo) pairs are combined across lanes by a fixed
-// xor butterfly, so the result does not depend on scheduling.
+// Far movers (rare: |v| dt > dx in some dimension).  An ordered list, in
+// particle order (= ascending source cell), built in two passes that return
+// at once when there are none.
+// ---------------------------------------------------------------------------
+__device__ inline int block_sum_256(int v, int *sh) {
+  // sh: 4 ints of LDS
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  int t = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_far_count(const uint8_t *code, long long np, const int32_t *nfar,
+                                                      int32_t *tiles) {
+  __shared__ int sh[4];
+  if (*nfar == 0) return;
+  const long long b = (long long)blockIdx.x * kFarTile;
+  int n = 0;
+  for (int r = threadIdx.x; r < kFarTile; r += kBlock) {
+    long long i = b + r;
+    n += (i < np && code[i] == kFarCode) ? 1 : 0;
+  }
+  n = block_sum_256(n, sh);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = n;
+}
+
+__global__ void __launch_bounds__(kBlock) k_far_compact(const uint8_t *code, const int32_t *cs_src,
+                                                        const int32_t *cs_dst, long long np, const int32_t *nfar,
+                                                        const int32_t *tiles, int4 *far, int far_cap,
+                                                        unsigned long long *flag) {
+  __shared__ int sh[4];
+  __shared__ int wsum[4];
+  if (*nfar == 0) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int base = 0;
+  for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) base += tiles[t];
+  base = block_sum_256(base, sh);
+  const long long b = (long long)blockIdx.x * kFarTile;
+  for (int r0 = 0; r0 < kFarTile; r0 += kBlock) {
+    long long i = b + r0 + threadIdx.x;
+    bool m = i < np && code[i] == kFarCode;
+    unsigned long long bal = __ballot(m);
+    if (lane == 0) wsum[wv] = __popcll(bal);
+    __syncthreads();
+    int wbase = 0;
+    for (int q = 0; q < wv; q++) wbase += wsum[q];
+    int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (m) {
+      int pos = base + wbase + __popcll(bal & lanes_below(lane));
+      if (pos < far_cap)
+        far[pos] = make_int4((int)i, cs_src[i] & kCellMask, cs_dst[i] & kCellMask, 0);
+      else
+        atomicOr(flag, kFlagFarMovers);
+    }
+    base += total;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Migration + deposit: one wavefront per destination cell.
 // ---------------------------------------------------------------------------
 __device__ inline void two_sum(float a, float b, float &s, float &e) {
   s = a + b;
@@ -294,68 +404,311 @@ __device__ inline void two_sum(float a, float b, float &s, float &e) {
   e = (a - (s - bb)) + (b - bb);
 }
 
+// the 12 edge currents of one particle (Depositor::deposit_particle_J,
+// src/particles/deposit.cpp:12-40, weights 2 - f and f - 1), accumulated
+// with TwoSum into (hi, lo)
+__device__ inline void deposit_j(float hi[12], float lo[12], const float f[3], const float v[3], float jc) {
+  float A[3] = {2.f - f[0], 2.f - f[1], 2.f - f[2]};
+  float Bw[3] = {f[0] - 1.f, f[1] - 1.f, f[2] - 1.f};
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    int j = (q + 1) % 3, k = (q + 2) % 3;
+    float cq = jc * v[q];
+    float w4[4] = {A[j] * A[k], Bw[j] * A[k], Bw[j] * Bw[k], A[j] * Bw[k]};
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      float s, err;
+      two_sum(hi[4 * q + r], cq * w4[r], s, err);
+      hi[4 * q + r] = s;
+      lo[4 * q + r] += err;
+    }
+  }
+}
+
+// One-dimensional source candidates of destination index t: the move codes
+// in the order 3, 2, 1, 0, 4, 5, whose sources t - sigma ascend; a code is a
+// candidate if it can occur and its source index lies in [1, n+1].  At most
+// four are ever valid, so a cell has at most 64 (source cell, code) pairs.
+struct Cand1 {
+  int src[6];
+  bool ok[6];
+  int n;
+};
+__device__ inline void cand1(int t, int n, int per, Cand1 &o) {
+  o.n = 0;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int k = q == 0 ? 3 : q == 1 ? 2 : q == 2 ? 1 : q == 3 ? 0 : q == 4 ? 4 : 5;
+    const int s = t - move_sigma(k, n);
+    o.ok[q] = move_live(k, n, per) && s >= 1 && s <= n + 1;
+    o.src[q] = s;
+    o.n += o.ok[q] ? 1 : 0;
+  }
+}
+// the a-th valid candidate (source index, code)
+__device__ inline void cand1_pick(const Cand1 &o, int a, int &src, int &k) {
+  int cnt = 0;
+  src = 0;
+  k = 0;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    if (o.ok[q]) {
+      if (cnt == a) {
+        src = o.src[q];
+        k = q == 0 ? 3 : q == 1 ? 2 : q == 2 ? 1 : q == 3 ? 0 : q == 4 ? 4 : 5;
+      }
+      cnt++;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stable counting sort by destination cell, in four streaming passes.
+// Within a destination cell the order is: source cells in ascending id, and
+// within one source cell the particles' previous order -- the stable sort of
+// the previous order by the new cell.  A particle's position is
+//   off_new[dst] + base[code][src] + rank
+// with rank its index among the particles of its (source cell, move code)
+// group and base the number of particles that reach dst from (source, code)
+// groups of lower source id.
+// ---------------------------------------------------------------------------
+constexpr int kCodes = 216;
+constexpr int kRankCells = 8;   // source cells per wavefront in the rank pass
+constexpr int kRankUnroll = 8;  // 64-particle chunks loaded before use
+constexpr int kMaxGroup = 65535;
+
+struct RankArgs {
+  const uint8_t *code;
+  const int32_t *off;  // ranges of the (sorted) source cells
+  long long ncell;
+  uint16_t *rank;
+  uint16_t *nsk;  // [code * ncell + source]: particles per group
+  unsigned long long *flag;
+};
+
+__global__ void __launch_bounds__(kBlock) k_rank_f32(RankArgs a) {
+  __shared__ int cnt[kBlock / 64][kRankCells * kCodes];
+  __shared__ int bnd[kBlock / 64][kRankCells + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long s0 = ((long long)blockIdx.x * (kBlock / 64) + wv) * kRankCells;
+  if (s0 >= a.ncell) return;
+  const int nc = (int)min((long long)kRankCells, a.ncell - s0);
+  int *ct = cnt[wv];
+  for (int e = lane; e < nc * kCodes; e += 64) ct[e] = 0;
+  if (lane <= nc) bnd[wv][lane] = a.off[s0 + lane];
+  wave_sync();
+  const int pb = bnd[wv][0], pe = bnd[wv][nc];
+  for (int p0 = pb; p0 < pe; p0 += 64 * kRankUnroll) {
+    uint8_t cc[kRankUnroll];
+#pragma unroll
+    for (int u = 0; u < kRankUnroll; u++) {
+      const int p = p0 + u * 64 + lane;
+      cc[u] = p < pe ? a.code[p] : kFarCode;
+    }
+#pragma unroll
+    for (int u = 0; u < kRankUnroll; u++) {
+      const int p = p0 + u * 64 + lane;
+      if (p0 + u * 64 >= pe) break;  // uniform
+      const bool valid = p < pe && cc[u] != kFarCode;
+      int j = 0;
+      for (int t = 1; t < nc; t++) j += bnd[wv][t] <= p ? 1 : 0;
+      const int key = j * kCodes + cc[u];
+      unsigned long long rem = __ballot(valid);
+      int myrank = 0;
+      while (rem) {
+        const int leader = __ffsll((long long)rem) - 1;
+        const int kl = __shfl(key, leader, 64);
+        const bool mine = valid && key == kl;
+        const unsigned long long bal = __ballot(mine);
+        const int base = ct[kl];
+        if (mine) myrank = base + __popcll(bal & lanes_below(lane));
+        wave_sync();
+        if (lane == 0) ct[kl] = base + __popcll(bal);
+        wave_sync();
+        rem &= ~bal;
+      }
+      if (valid) {
+        if (myrank > kMaxGroup) atomicOr(a.flag, kFlagMigrate);
+        a.rank[p] = (uint16_t)myrank;
+      }
+    }
+  }
+  wave_sync();
+  for (int e = lane; e < nc * kCodes; e += 64) {
+    const int v = ct[e];
+    if (v) a.nsk[(long long)(e % kCodes) * a.ncell + s0 + e / kCodes] = (uint16_t)min(v, kMaxGroup);
+  }
+}
+
+struct BaseArgs {
+  Geo g;
+  uint16_t *nsk;
+  uint16_t *base;  // [code * ncell + source]
+  int32_t *count;  // particles per destination cell
+  int per[3];
+  const int32_t *nfar;
+  const int4 *far;
+  int far_cap;
+  int32_t *far_pos;
+  unsigned long long *flag;
+};
+
+// One thread per destination cell: its (source, code) groups in ascending
+// source id, far movers merged in by source id.
+__global__ void __launch_bounds__(kBlock) k_base_f32(BaseArgs a) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const Geo &g = a.g;
+  if (c >= g.ncell) return;
+  int cx, cy, cz;
+  decode_cell(g, (int)c, &cx, &cy, &cz);
+  Cand1 X, Y, Z;
+  cand1(cx, g.n[0], a.per[0], X);
+  cand1(cy, g.n[1], a.per[1], Y);
+  cand1(cz, g.n[2], a.per[2], Z);
+  const int nf = min(*a.nfar, a.far_cap);
+  int fp = 0;
+  int run = 0;
+#pragma unroll
+  for (int ox = 0; ox < 6; ox++) {
+    if (!X.ok[ox]) continue;
+#pragma unroll
+    for (int oy = 0; oy < 6; oy++) {
+      if (!Y.ok[oy]) continue;
+#pragma unroll
+      for (int oz = 0; oz < 6; oz++) {
+        if (!Z.ok[oz]) continue;
+        const int kx = ox == 0 ? 3 : ox == 1 ? 2 : ox == 2 ? 1 : ox == 3 ? 0 : ox == 4 ? 4 : 5;
+        const int ky = oy == 0 ? 3 : oy == 1 ? 2 : oy == 2 ? 1 : oy == 3 ? 0 : oy == 4 ? 4 : 5;
+        const int kz = oz == 0 ? 3 : oz == 1 ? 2 : oz == 2 ? 1 : oz == 3 ? 0 : oz == 4 ? 4 : 5;
+        const int s = (X.src[ox] * g.nc[1] + Y.src[oy]) * g.nc[2] + Z.src[oz];
+        const long long idx = (long long)((kx * 6 + ky) * 6 + kz) * g.ncell + s;
+        while (fp < nf) {
+          const int4 e = a.far[fp];
+          if (e.y >= s) break;
+          if (e.z == (int)c) a.far_pos[fp] = run++;
+          fp++;
+        }
+        const int n = a.nsk[idx];
+        if (n) {
+          a.nsk[idx] = 0;
+          a.base[idx] = (uint16_t)min(run, kMaxGroup);
+          run += n;
+        }
+      }
+    }
+  }
+  for (; fp < nf; fp++)
+    if (a.far[fp].z == (int)c) a.far_pos[fp] = run++;
+  if (run > kMaxGroup) atomicOr(a.flag, kFlagMigrate);
+  a.count[c] = run;
+}
+
+struct ScatterArgs {
+  Geo g;
+  const int32_t *cs_in;
+  const float *f_in[3], *u_in[3];
+  const int32_t *id_in;
+  const uint8_t *code;
+  const uint16_t *rank;
+  const uint16_t *base;
+  const int32_t *off_new;
+  const int32_t *nfar;
+  const int4 *far;
+  int far_cap;
+  const int32_t *far_pos;
+  int32_t *cs_out;
+  float *f_out[3], *u_out[3];
+  int32_t *id_out;
+  unsigned long long *flag;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_scatter_f32(ScatterArgs a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const Geo &g = a.g;
+  const int w = a.cs_in[i];
+  const int dst = w & kCellMask;
+  const int k = a.code[i];
+  int within;
+  if (k != kFarCode) {
+    int dx, dy, dz;
+    decode_cell(g, dst, &dx, &dy, &dz);
+    const int kx = k / 36, ky = (k / 6) % 6, kz = k % 6;
+    const int s = ((dx - move_sigma(kx, g.n[0])) * g.nc[1] + (dy - move_sigma(ky, g.n[1]))) * g.nc[2] +
+                  (dz - move_sigma(kz, g.n[2]));
+    if (s < 0 || s >= g.ncell) {
+      atomicOr(a.flag, kFlagMigrate);
+      return;
+    }
+    within = (int)a.base[(long long)k * g.ncell + s] + (int)a.rank[i];
+  } else {
+    // far mover: its entry in the ordered list (ascending particle index)
+    int lo = 0, hi = min(*a.nfar, a.far_cap);
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (a.far[mid].x < i) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= min(*a.nfar, a.far_cap) || a.far[lo].x != i) {
+      atomicOr(a.flag, kFlagFarMovers);
+      return;
+    }
+    within = a.far_pos[lo];
+  }
+  const long long pos = (long long)a.off_new[dst] + within;
+  if (pos >= a.off_new[dst + 1]) {
+    atomicOr(a.flag, kFlagMigrate);
+    return;
+  }
+  a.cs_out[pos] = w;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    a.f_out[d][pos] = a.f_in[d][i];
+    a.u_out[d][pos] = a.u_in[d][i];
+  }
+  if (a.id_in) a.id_out[pos] = a.id_in[i];
+}
+
+// ---------------------------------------------------------------------------
+// Deposit: 16 lanes per cell, 4 cells per wavefront.  Each lane sums its
+// particles' 12 edge currents with TwoSum; a fixed xor butterfly within the
+// 16-lane group combines the (hi, lo) pairs: deterministic, no atomics.
+// ---------------------------------------------------------------------------
 struct DepF32Args {
   Geo g;
   const int32_t *cs;
   const float *f[3], *u[3];
-  const float *g0;
-  const int32_t *beg, *end;
+  const int32_t *off;
   SpeciesF32 s;
-  float *cj, *cr;
-  int rel;
-  int rho;
+  float *cj;
 };
 
-template <bool RHO>
+template <bool REL>
 __global__ void __launch_bounds__(kBlock) k_deposit_f32(DepF32Args a) {
-  const int lane = threadIdx.x & 63;
-  const long long c = (long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int sub = threadIdx.x & 15;
+  const long long c = (long long)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4);
   const Geo &g = a.g;
-  if (c >= g.ncell) return;
-  constexpr int NV = RHO ? 20 : 12;
-  float hi[NV], lo[NV];
+  float hi[12], lo[12];
 #pragma unroll
-  for (int q = 0; q < NV; q++) hi[q] = lo[q] = 0.f;
-  const int b = a.beg[c], e = a.end[c];
-  for (int p = b + lane; p < e; p += 64) {
-    int sp = (int)((uint32_t)a.cs[p] >> kSpeciesShift);
-    if (a.s.test[sp]) continue;
-    float f[3] = {a.f[0][p], a.f[1][p], a.f[2][p]};
-    float ux = a.u[0][p], uy = a.u[1][p], uz = a.u[2][p];
-    float ig = a.rel ? 1.f / sqrtf(1.f + ux * ux + uy * uy + uz * uz) : 1.f;
-    float v[3] = {ux * ig, uy * ig, uz * ig};
-    float A[3] = {2.f - f[0], 2.f - f[1], 2.f - f[2]};
-    float Bw[3] = {f[0] - 1.f, f[1] - 1.f, f[2] - 1.f};
-    float jc = a.s.jc[sp];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      int j = (q + 1) % 3, k = (q + 2) % 3;
-      float cq = jc * v[q];
-      float w4[4] = {A[j] * A[k], Bw[j] * A[k], Bw[j] * Bw[k], A[j] * Bw[k]};
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        float s, err;
-        two_sum(hi[4 * q + r], cq * w4[r], s, err);
-        hi[4 * q + r] = s;
-        lo[4 * q + r] += err;
-      }
-    }
-    if (RHO) {
-      float w8[8] = {A[0] * A[1] * A[2], Bw[0] * A[1] * A[2], Bw[0] * Bw[1] * A[2], A[0] * Bw[1] * A[2],
-                     A[0] * Bw[1] * Bw[2], Bw[0] * Bw[1] * Bw[2], Bw[0] * A[1] * Bw[2], A[0] * A[1] * Bw[2]};
-#pragma unroll
-      for (int r = 0; r < 8; r++) {
-        float s, err;
-        two_sum(hi[12 + r], jc * w8[r], s, err);
-        hi[12 + r] = s;
-        lo[12 + r] += err;
-      }
+  for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
+  if (c < g.ncell) {
+    const int b = a.off[c], e = a.off[c + 1];
+    for (int p = b + sub; p < e; p += 16) {
+      const int w = a.cs[p];
+      const int sp = (int)((uint32_t)w >> kSpeciesShift);
+      if (a.s.test[sp]) continue;
+      float f[3] = {a.f[0][p], a.f[1][p], a.f[2][p]};
+      float uq[3] = {a.u[0][p], a.u[1][p], a.u[2][p]};
+      float ig = REL ? 1.f / sqrtf(1.f + uq[0] * uq[0] + uq[1] * uq[1] + uq[2] * uq[2]) : 1.f;
+      float v[3] = {uq[0] * ig, uq[1] * ig, uq[2] * ig};
+      deposit_j(hi, lo, f, v, a.s.jc[sp]);
     }
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int off = 8; off > 0; off >>= 1) {
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
+    for (int q = 0; q < 12; q++) {
       float oh = __shfl_xor(hi[q], off, 64);
       float ol = __shfl_xor(lo[q], off, 64);
       float s, err;
@@ -364,14 +717,70 @@ __global__ void __launch_bounds__(kBlock) k_deposit_f32(DepF32Args a) {
       lo[q] = lo[q] + ol + err;
     }
   }
-  if (lane == 0) {
+  if (sub == 0 && c < g.ncell) {
 #pragma unroll
     for (int q = 0; q < 12; q++) a.cj[q * g.ncell + c] = hi[q] + lo[q];
-    if (RHO) {
-#pragma unroll
-      for (int q = 0; q < 8; q++) a.cr[q * g.ncell + c] = hi[12 + q] + lo[12 + q];
-    }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Initial sort of freshly loaded particles (arbitrary order): radix sort of
+// the cell ids, payload permutation, arrival counts.
+// ---------------------------------------------------------------------------
+__global__ void k_keys_from_cs(const int32_t *cs, uint32_t *key, uint32_t *perm, long long np) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  key[i] = (uint32_t)(cs[i] & kCellMask);
+  perm[i] = (uint32_t)i;
+}
+
+struct PermF32Args {
+  const int32_t *cs;
+  const float *f[3], *u[3];
+  const int32_t *id;
+  const float *g0;
+  int32_t *cs_out;
+  float *f_out[3], *u_out[3];
+  int32_t *id_out;
+  float *g0_out;
+  const uint32_t *perm;
+  int32_t *count;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_permute_f32(PermF32Args a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = i < a.np;
+  int cell = -1;
+  if (live) {
+    uint32_t s = a.perm[i];
+    int w = a.cs[s];
+    a.cs_out[i] = w;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      a.f_out[d][i] = a.f[d][s];
+      a.u_out[d][i] = a.u[d][s];
+    }
+    if (a.id) a.id_out[i] = a.id[s];
+    if (a.g0) a.g0_out[i] = a.g0[s];
+    cell = w & kCellMask;
+  }
+  // particles per cell: one atomic per run of equal cells (the output is sorted)
+  const int prev = __shfl_up(cell, 1, 64);
+  const bool head = live && (lane == 0 || prev != cell);
+  const unsigned long long heads = __ballot(head);
+  const int nlive = __popcll(__ballot(live));
+  if (head) {
+    unsigned long long above = heads & ~(lanes_below(lane) | (1ull << lane));
+    int next = above ? __ffsll((long long)above) - 1 : 64;
+    atomicAdd(&a.count[cell], min(next, nlive) - lane);
+  }
+}
+
+__global__ void k_fill_i32(int32_t *p, int32_t v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // FP64 reconstruction of positions and velocities (global-position output)
@@ -446,11 +855,6 @@ __global__ void __launch_bounds__(kBlock) k_diag_particles_f32(DiagF32Args a) {
     for (int q = 0; q < 5; q++) a.partials[5 * blockIdx.x + q] = sh[q][0];
 }
 
-__global__ void k_gather_f32(const float *in, float *out, const uint32_t *perm, long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = in[perm[i]];
-}
-
 int set_err(emo_ctx *c, int code, const std::string &m) {
   c->err = m;
   return code;
@@ -478,77 +882,77 @@ int check_flag(emo_ctx *c) {
   if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, hipGetErrorString(e));
   if (c->hflag[0] & kFlagOutOfGrid)
     return set_err(c, EMO_ERR_INVARIANT, "particle outside the padded grid");
+  if (c->hflag[0] & kFlagFarMovers)
+    return set_err(c, EMO_ERR_NUMERIC,
+                   "too many particles moved more than one cell in a step (|v| dt > dx): reduce dt");
+  if (c->hflag[0] & kFlagMigrate) return set_err(c, EMO_ERR_INVARIANT, "cell migration lost particles");
   return EMO_OK;
 }
 
-int sort_f32(emo_ctx *c) {
+void fill_i32(emo_ctx *c, int32_t *p, int32_t v, long long n) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, p, v, n);
+}
+
+// exclusive scan of the arrival counts into the new cell ranges
+int scan_counts(emo_ctx *c, int32_t *out) {
   ChosenState *s = c->ch;
-  if (c->np == 0) return EMO_OK;
-  size_t bytes = s->sort_tmp_bytes;
-  hipError_t e;
-  {
-    TimedLaunch t(c, "sort_f32_radix");
-    e = hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->key, s->key_alt, s->perm, s->perm_alt,
-                                           (int)c->np, 0, s->sort_bits, c->stream);
-  }
-  if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, std::string("radix sort: ") + hipGetErrorString(e));
-  PermF32Args pa;
-  pa.cs = s->cs_alt;
-  pa.cs_out = s->cs;
-  for (int d = 0; d < 3; d++) {
-    pa.f[d] = s->f_alt[d];
-    pa.u[d] = s->u_alt[d];
-    pa.f_out[d] = s->f[d];
-    pa.u_out[d] = s->u[d];
-  }
-  pa.id = s->id;
-  pa.id_out = s->id_alt;
-  pa.perm = s->perm_alt;
-  pa.np = c->np;
-  {
-    TimedLaunch t(c, "sort_f32_scatter");
-    hipLaunchKernelGGL(k_permute_f32, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, pa);
-  }
-  if (s->id) std::swap(s->id, s->id_alt);
-  EMO_HIP_CHECK_RET(c, hipMemsetAsync(s->cbeg, 0, c->geo.ncell * sizeof(int32_t), c->stream));
-  EMO_HIP_CHECK_RET(c, hipMemsetAsync(s->cend, 0, c->geo.ncell * sizeof(int32_t), c->stream));
-  hipLaunchKernelGGL(k_cell_bounds_f32, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, s->key_alt,
-                     s->cbeg, s->cend, (long long)c->np);
+  size_t bytes = s->scan_tmp_bytes;
+  TimedLaunch t(c, "scan_f32");
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(s->scan_tmp, bytes, s->count, out, (int)(c->geo.ncell + 1),
+                                                  c->stream);
+  if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, std::string("scan: ") + hipGetErrorString(e));
   return EMO_OK;
 }
 
-// initial stable sort of a freshly loaded store (cs/f/u in the main arrays)
-__global__ void k_keys_from_cs(const int32_t *cs, uint32_t *key, uint32_t *perm, long long np) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  key[i] = (uint32_t)(cs[i] & kCellMask);
-  perm[i] = (uint32_t)i;
-}
-
-int resort_loaded(emo_ctx *c) {
+// Sorts freshly loaded particles (main arrays, any order) by cell, stably.
+int sort_loaded(emo_ctx *c) {
   ChosenState *s = c->ch;
-  if (c->np == 0) return EMO_OK;
-  hipLaunchKernelGGL(k_keys_from_cs, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, s->cs, s->key,
-                     s->perm, (long long)c->np);
-  // move the loaded state to the alt arrays so sort_f32 permutes alt -> main
-  std::swap(s->cs, s->cs_alt);
-  for (int d = 0; d < 3; d++) {
-    std::swap(s->f[d], s->f_alt[d]);
-    std::swap(s->u[d], s->u_alt[d]);
+  const long long np = c->np;
+  fill_i32(c, s->count, 0, c->geo.ncell + 1);
+  if (np > 0) {
+    hipLaunchKernelGGL(k_keys_from_cs, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, s->cs, s->key, s->perm,
+                       np);
+    size_t bytes = s->sort_tmp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->key, s->key_alt, s->perm,
+                                                      s->perm_alt, (int)np, 0, s->sort_bits, c->stream);
+    if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, std::string("radix sort: ") + hipGetErrorString(e));
+    float *g0_new = nullptr;
+    if (s->g0_active) {
+      int rc = dalloc(c, &g0_new, np);
+      if (rc) return rc;
+    }
+    PermF32Args pa;
+    pa.cs = s->cs;
+    pa.cs_out = s->cs_alt;
+    for (int d = 0; d < 3; d++) {
+      pa.f[d] = s->f[d];
+      pa.u[d] = s->u[d];
+      pa.f_out[d] = s->f_alt[d];
+      pa.u_out[d] = s->u_alt[d];
+    }
+    pa.id = s->id;
+    pa.id_out = s->id_alt;
+    pa.g0 = s->g0_active ? s->g0 : nullptr;
+    pa.g0_out = g0_new;
+    pa.perm = s->perm_alt;
+    pa.count = s->count;
+    pa.np = np;
+    hipLaunchKernelGGL(k_permute_f32, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, pa);
+    std::swap(s->cs, s->cs_alt);
+    for (int d = 0; d < 3; d++) {
+      std::swap(s->f[d], s->f_alt[d]);
+      std::swap(s->u[d], s->u_alt[d]);
+    }
+    if (s->id) std::swap(s->id, s->id_alt);
+    if (g0_new) {
+      EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+      dfree(s->g0);
+      s->g0 = g0_new;
+    }
   }
-  int rc = sort_f32(c);
+  int rc = scan_counts(c, s->off);
   if (rc) return rc;
-  if (s->g0_active) {
-    // the stored gamma follows its particles through the same permutation
-    float *tmp = nullptr;
-    rc = dalloc(c, &tmp, c->np);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_gather_f32, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream,
-                       (const float *)s->g0, tmp, (const uint32_t *)s->perm_alt, (long long)c->np);
-    EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
-    dfree(s->g0);
-    s->g0 = tmp;
-  }
+  fill_i32(c, s->count, 0, c->geo.ncell + 1);
   return EMO_OK;
 }
 
@@ -567,10 +971,11 @@ void launch_pgd(emo_ctx *c, double dt) {
     a.B[d] = s->fs.B[d];
     a.Bt[d] = s->fs.Bt[d];
     a.dtid[d] = (float)(dt * c->geo.id[d]);
+    a.per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
   }
   a.g0 = s->g0_active ? s->g0 : nullptr;
-  a.key = s->key;
-  a.perm = s->perm;
+  a.code = s->code;
+  a.nfar = s->misc;
   a.s = species_f32(c, dt);
   for (int k = 0; k < 6; k++) a.pbc[k] = c->cfg.parts_bound[k];
   a.flag = c->dflag;
@@ -582,40 +987,111 @@ void launch_pgd(emo_ctx *c, double dt) {
     hipLaunchKernelGGL(k_pgd_f32<false>, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, a);
 }
 
-void launch_deposit_f32(emo_ctx *c, int rho) {
+// far-mover list, ranks, group offsets, new cell ranges, scatter, deposit,
+// grid sum
+int launch_sort_deposit(emo_ctx *c) {
   ChosenState *s = c->ch;
-  DepF32Args a;
-  a.g = c->geo;
-  a.cs = s->cs;
-  for (int d = 0; d < 3; d++) {
-    a.f[d] = s->f[d];
-    a.u[d] = s->u[d];
-  }
-  a.g0 = nullptr;
-  a.beg = s->cbeg;
-  a.end = s->cend;
-  a.s = species_f32(c, 0.0);
-  a.cj = s->cj;
-  a.cr = s->cr;
-  a.rel = c->cfg.relativity;
-  a.rho = rho;
-  unsigned blocks = (unsigned)((c->geo.ncell + (kBlock / 64) - 1) / (kBlock / 64));
+  const Geo &g = c->geo;
+  const long long np = c->np;
+  int per[3];
+  for (int d = 0; d < 3; d++)
+    per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
+  const unsigned ntiles = (unsigned)std::max<long long>((np + kFarTile - 1) / kFarTile, 1);
   {
+    TimedLaunch t(c, "far_f32");
+    hipLaunchKernelGGL(k_far_count, dim3(ntiles), dim3(kBlock), 0, c->stream, (const uint8_t *)s->code, np,
+                       (const int32_t *)s->misc, s->far_tiles);
+    hipLaunchKernelGGL(k_far_compact, dim3(ntiles), dim3(kBlock), 0, c->stream, (const uint8_t *)s->code,
+                       (const int32_t *)s->cs, (const int32_t *)s->cs_alt, np, (const int32_t *)s->misc,
+                       (const int32_t *)s->far_tiles, s->far, kFarCap, c->dflag);
+  }
+  {
+    RankArgs r;
+    r.code = s->code;
+    r.off = s->off;
+    r.ncell = g.ncell;
+    r.rank = s->rank;
+    r.nsk = s->nsk;
+    r.flag = c->dflag;
+    const long long waves = (g.ncell + kRankCells - 1) / kRankCells;
+    TimedLaunch t(c, "rank_f32");
+    hipLaunchKernelGGL(k_rank_f32, dim3((unsigned)((waves + 3) / 4)), dim3(kBlock), 0, c->stream, r);
+  }
+  {
+    BaseArgs b;
+    b.g = g;
+    b.nsk = s->nsk;
+    b.base = s->gbase;
+    b.count = s->count;
+    for (int d = 0; d < 3; d++) b.per[d] = per[d];
+    b.nfar = s->misc;
+    b.far = s->far;
+    b.far_cap = kFarCap;
+    b.far_pos = s->far_pos;
+    b.flag = c->dflag;
+    TimedLaunch t(c, "base_f32");
+    hipLaunchKernelGGL(k_base_f32, dim3(grid_for(g.ncell)), dim3(kBlock), 0, c->stream, b);
+  }
+  int rc = scan_counts(c, s->off_new);
+  if (rc) return rc;
+  {
+    ScatterArgs a;
+    a.g = g;
+    a.cs_in = s->cs_alt;
+    a.cs_out = s->cs;
+    for (int d = 0; d < 3; d++) {
+      a.f_in[d] = s->f_alt[d];
+      a.u_in[d] = s->u_alt[d];
+      a.f_out[d] = s->f[d];
+      a.u_out[d] = s->u[d];
+    }
+    a.id_in = s->id;
+    a.id_out = s->id_alt;
+    a.code = s->code;
+    a.rank = s->rank;
+    a.base = s->gbase;
+    a.off_new = s->off_new;
+    a.nfar = s->misc;
+    a.far = s->far;
+    a.far_cap = kFarCap;
+    a.far_pos = s->far_pos;
+    a.flag = c->dflag;
+    a.np = np;
+    TimedLaunch t(c, "scatter_f32");
+    hipLaunchKernelGGL(k_scatter_f32, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, a);
+  }
+  std::swap(s->off, s->off_new);
+  if (s->id) std::swap(s->id, s->id_alt);
+  // the far-mover count starts from zero again
+  fill_i32(c, s->misc, 0, 1);
+  {
+    DepF32Args d;
+    d.g = g;
+    d.cs = s->cs;
+    for (int q = 0; q < 3; q++) {
+      d.f[q] = s->f[q];
+      d.u[q] = s->u[q];
+    }
+    d.off = s->off;
+    d.s = species_f32(c, 0.0);
+    d.cj = s->cj;
+    const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
     TimedLaunch t(c, "deposit_f32");
-    if (rho)
-      hipLaunchKernelGGL(k_deposit_f32<true>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    if (c->cfg.relativity)
+      hipLaunchKernelGGL(k_deposit_f32<true>, dim3(blocks), dim3(kBlock), 0, c->stream, d);
     else
-      hipLaunchKernelGGL(k_deposit_f32<false>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
+      hipLaunchKernelGGL(k_deposit_f32<false>, dim3(blocks), dim3(kBlock), 0, c->stream, d);
   }
   DepGridArgs<float> b;
-  b.g = c->geo;
+  b.g = g;
   b.cj = s->cj;
-  b.cr = s->cr;
+  b.cr = nullptr;
   for (int d = 0; d < 3; d++) b.J[d] = s->fs.J[d];
   b.rho = s->fs.rho;
-  b.do_rho = rho;
+  b.do_rho = 0;
   TimedLaunch t(c, "jsum_f32");
-  hipLaunchKernelGGL(k_deposit_grid<float>, dim3(grid_for(c->geo.ntot)), dim3(kBlock), 0, c->stream, b);
+  hipLaunchKernelGGL(k_deposit_grid<float>, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, b);
+  return EMO_OK;
 }
 
 }  // namespace
@@ -627,6 +1103,7 @@ int fp32_create(emo_ctx *c) {
   const Geo &g = c->geo;
   if (g.ncell > kCellMask) return set_err(c, EMO_ERR_CONFIG, "grid too large for the FP32 cell word");
   int64_t cap = std::max<int64_t>(c->cfg.max_particles, 1);
+  if (cap > INT_MAX) return set_err(c, EMO_ERR_CONFIG, "the FP32 chosen form holds at most 2^31-1 particles");
   s->cap = cap;
   c->cap = cap;
   int rc = 0;
@@ -638,14 +1115,22 @@ int fp32_create(emo_ctx *c) {
     if (!rc) rc = dalloc(c, &s->u[d], cap);
     if (!rc) rc = dalloc(c, &s->u_alt[d], cap);
   }
+  if (!rc) rc = dalloc(c, &s->code, cap);
+  if (!rc) rc = dalloc(c, &s->rank, cap);
+  if (!rc) rc = dalloc(c, &s->nsk, (size_t)kCodes * g.ncell);
+  if (!rc) rc = dalloc(c, &s->gbase, (size_t)kCodes * g.ncell);
+  if (!rc) rc = dalloc(c, &s->far_pos, kFarCap);
+  if (!rc) rc = dalloc(c, &s->count, g.ncell + 1);
+  if (!rc) rc = dalloc(c, &s->off, g.ncell + 1);
+  if (!rc) rc = dalloc(c, &s->off_new, g.ncell + 1);
+  if (!rc) rc = dalloc(c, &s->misc, 4);
+  if (!rc) rc = dalloc(c, &s->far_tiles, (cap + kFarTile - 1) / kFarTile + 1);
+  if (!rc) rc = dalloc(c, &s->far, kFarCap);
   if (!rc) rc = dalloc(c, &s->key, cap);
   if (!rc) rc = dalloc(c, &s->key_alt, cap);
   if (!rc) rc = dalloc(c, &s->perm, cap);
   if (!rc) rc = dalloc(c, &s->perm_alt, cap);
-  if (!rc) rc = dalloc(c, &s->cbeg, g.ncell);
-  if (!rc) rc = dalloc(c, &s->cend, g.ncell);
   if (!rc) rc = dalloc(c, &s->cj, 12 * g.ncell);
-  if (!rc) rc = dalloc(c, &s->cr, 8 * g.ncell);
   size_t nt = (size_t)g.ntot;
   for (int q = 0; q < 3 && !rc; q++) {
     rc = dalloc(c, &s->fs.E[q], nt);
@@ -662,6 +1147,10 @@ int fp32_create(emo_ctx *c) {
   float *all[13] = {s->fs.E[0], s->fs.E[1], s->fs.E[2], s->fs.Bfix[0], s->fs.Bfix[1], s->fs.Bfix[2],
                     s->Btfix[0], s->Btfix[1], s->Btfix[2], s->fs.J[0], s->fs.J[1], s->fs.J[2], s->fs.rho};
   for (int q = 0; q < 13; q++) EMO_HIP_CHECK_RET(c, hipMemset(all[q], 0, nt * sizeof(float)));
+  EMO_HIP_CHECK_RET(c, hipMemset(s->count, 0, (g.ncell + 1) * sizeof(int32_t)));
+  EMO_HIP_CHECK_RET(c, hipMemset(s->nsk, 0, (size_t)kCodes * g.ncell * sizeof(uint16_t)));
+  EMO_HIP_CHECK_RET(c, hipMemset(s->off, 0, (g.ncell + 1) * sizeof(int32_t)));
+  EMO_HIP_CHECK_RET(c, hipMemset(s->misc, 0, 4 * sizeof(int32_t)));
   int bits = 1;
   while ((1ll << bits) < g.ncell) bits++;
   s->sort_bits = bits;
@@ -670,6 +1159,11 @@ int fp32_create(emo_ctx *c) {
                                      (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)cap, 0, bits);
   if (hipMalloc(&s->sort_tmp, std::max<size_t>(s->sort_tmp_bytes, 16)) != hipSuccess)
     return set_err(c, EMO_ERR_ALLOC, "sort scratch");
+  s->scan_tmp_bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, s->scan_tmp_bytes, (const int32_t *)nullptr, (int32_t *)nullptr,
+                                   (int)(g.ncell + 1));
+  if (hipMalloc(&s->scan_tmp, std::max<size_t>(s->scan_tmp_bytes, 16)) != hipSuccess)
+    return set_err(c, EMO_ERR_ALLOC, "scan scratch");
   return EMO_OK;
 }
 
@@ -692,15 +1186,24 @@ void fp32_destroy(emo_ctx *c) {
   dfree(s->id);
   dfree(s->id_alt);
   dfree(s->g0);
+  dfree(s->code);
+  dfree(s->rank);
+  dfree(s->nsk);
+  dfree(s->gbase);
+  dfree(s->far_pos);
+  dfree(s->count);
+  dfree(s->off);
+  dfree(s->off_new);
+  dfree(s->misc);
+  dfree(s->far_tiles);
+  dfree(s->far);
   dfree(s->key);
   dfree(s->key_alt);
   dfree(s->perm);
   dfree(s->perm_alt);
   dfree(s->sort_tmp);
-  dfree(s->cbeg);
-  dfree(s->cend);
+  dfree(s->scan_tmp);
   dfree(s->cj);
-  dfree(s->cr);
   delete s;
   c->ch = nullptr;
 }
@@ -711,7 +1214,7 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
               const int32_t *species, const int64_t *id) {
   ChosenState *s = c->ch;
   const Geo &g = c->geo;
-  std::vector<int32_t> cs(n);
+  std::vector<int32_t> cs(n), ids;
   std::vector<float> f[3], u[3], g0;
   for (int d = 0; d < 3; d++) {
     f[d].resize(n);
@@ -719,6 +1222,7 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
   }
   const bool rel = c->cfg.relativity != 0;
   if (rel) g0.resize(n);
+  if (id) ids.resize(n);
   for (int64_t i = 0; i < n; i++) {
     int ic[3];
     for (int d = 0; d < 3; d++) {
@@ -737,6 +1241,11 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
     double gm = (rel && gamma) ? gamma[i] : 1.0;
     for (int d = 0; d < 3; d++) u[d][i] = (float)(rel ? gm * v3[3 * i + d] : v3[3 * i + d]);
     if (rel) g0[i] = (float)gm;
+    if (id) {
+      if (id[i] < INT_MIN || id[i] > INT_MAX)
+        return set_err(c, EMO_ERR_UNSUPPORTED, "particle ids must fit in int32 in the FP32 chosen form");
+      ids[i] = (int32_t)id[i];
+    }
   }
   EMO_HIP_CHECK_RET(c, hipMemcpy(s->cs, cs.data(), n * 4, hipMemcpyHostToDevice));
   for (int d = 0; d < 3; d++) {
@@ -750,7 +1259,7 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
     int rc = dalloc(c, &s->id, n);
     if (!rc) rc = dalloc(c, &s->id_alt, n);
     if (rc) return rc;
-    EMO_HIP_CHECK_RET(c, hipMemcpy(s->id, id, n * 8, hipMemcpyHostToDevice));
+    EMO_HIP_CHECK_RET(c, hipMemcpy(s->id, ids.data(), n * 4, hipMemcpyHostToDevice));
   }
   dfree(s->g0);
   s->g0 = nullptr;
@@ -762,10 +1271,10 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
     s->g0_active = true;
   }
   c->np = n;
-  int rc = resort_loaded(c);
+  int rc = sort_loaded(c);
   if (rc) return rc;
   EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
-  return EMO_OK;
+  return check_flag(c);
 }
 
 int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species, int64_t *id) {
@@ -813,7 +1322,9 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
     if (species && !rc && hipMemcpy(species, dsp, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
     if (id && !rc) {
       if (s->id) {
-        if (hipMemcpy(id, s->id, n * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
+        std::vector<int32_t> h(n);
+        if (hipMemcpy(h.data(), s->id, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
+        for (int64_t i = 0; i < n && !rc; i++) id[i] = h[i];
       } else {
         for (int64_t i = 0; i < n; i++) id[i] = -1;
       }
@@ -891,23 +1402,23 @@ int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
   double t = t0;
   for (int k = 0; k < nsteps; k++) {
     launch_pgd(c, dt);
-    int rc = sort_f32(c);
-    if (rc) return rc;
     if (s->g0_active) {
-      // the stored gamma was used by this first push only (Q3); sort_f32
-      // permuted the state, so from now on gamma = sqrt(1 + u^2)
+      // the stored gamma was used by this first push only (Q3): from now on
+      // gamma = sqrt(1 + u^2)
+      EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
       s->g0_active = false;
       dfree(s->g0);
       s->g0 = nullptr;
     }
-    launch_deposit_f32(c, 0);
+    int rc = launch_sort_deposit(c);
+    if (rc) return rc;
     fk_fold<float>(c, s->fs);
     fk_evolve<float>(c, s->fs, dt);
     refresh_fields<float>(c, s->fs, t, dt);
     t += dt;
   }
   EMO_HIP_CHECK_RET(c, hipGetLastError());
-  return EMO_OK;
+  return check_flag(c);
 }
 
 int fp32_diag(emo_ctx *c, double *out8) {

@@ -138,6 +138,11 @@ namespace emo {
 // error flag bits written by kernels into dflag[0]
 constexpr unsigned long long kFlagOutOfGrid = 1ull;
 constexpr unsigned long long kFlagNonFinite = 2ull;
+// FP32 migration sort: more particles left the one-cell neighbourhood of
+// their cell in one step than its side list holds
+constexpr unsigned long long kFlagFarMovers = 4ull;
+// FP32 migration sort: a cell's gathered particles disagree with its count
+constexpr unsigned long long kFlagMigrate = 8ull;
 
 // ---- plain FP64 particle kernels (plain_kernels.hip) ----
 void launch_gather_f64(emo_ctx *c);

@@ -48,9 +48,11 @@ def algorithmic_bytes(kernel: str, np_: int, ncell: int, relativity: int) -> flo
         "deposit_grid_f64": 160 * ncell + 32 * ng,
         "yee_E_f64": 96 * ncell,
         "yee_B_f64": 72 * ncell,
-        # FP32 chosen form
-        "pgd_f32": 64 * np_ + 36 * ncell,
-        "sort_f32_scatter": 60 * np_,
+        # FP32 chosen form (int32 ids ride along: 4 B read + 4 B written in scatter)
+        "pgd_f32": 57 * np_ + 36 * ncell,
+        "rank_f32": 3 * np_ + 8 * ncell,
+        "base_f32": 4 * 27 * ncell,
+        "scatter_f32": 67 * np_ + 8 * ncell,
         "deposit_f32": 28 * np_ + 56 * ncell,
         "jsum_f32": 48 * ncell + 12 * ng,
         "yee_E_f32": 48 * ncell,
@@ -62,7 +64,7 @@ def algorithmic_bytes(kernel: str, np_: int, ncell: int, relativity: int) -> flo
 KERNELS = {
     "fp64": ["gather_f64", "push_f64", "particle_bc_f64", "sort_keys", "sort_radix", "sort_permute",
              "deposit_cells_f64", "deposit_grid_f64", "yee_E_f64", "yee_B_f64"],
-    "fp32": ["pgd_f32", "sort_f32_radix", "sort_f32_scatter", "deposit_f32", "jsum_f32",
+    "fp32": ["pgd_f32", "far_f32", "rank_f32", "base_f32", "scan_f32", "scatter_f32", "deposit_f32", "jsum_f32",
              "yee_E_f32", "yee_B_f32"],
 }
 
@@ -105,7 +107,7 @@ def run_gpu(mode: str, steps: int, warmup: int, scale: float):
 
 
 # library calls (several rocprim kernels each): reported, never the roofline kernel
-LIBRARY_CALLS = {"sort_radix", "sort_f32_radix"}
+LIBRARY_CALLS = {"sort_radix", "scan_f32"}
 
 
 def roofline_for(res):

@@ -115,3 +115,67 @@ def test_fp32_rejects_fp64_only_phases():
     with pytest.raises(EmoError) as e:
         ctx.push(0.1)
     assert e.value.code == -6
+
+
+def array_order(ctx):
+    p = ctx.particles(fields=False)
+    return p["id"].copy(), ctx.cell_ids()
+
+
+def check_stable_sort_step(ctx, dt, steps):
+    """After every step the arrays must be the stable counting sort, by the
+    new cell, of the previous step's order (ascending cell id)."""
+    prev_ids, _ = array_order(ctx)
+    t = 0.0
+    for _ in range(steps):
+        ctx.step(1, t, dt)
+        t += dt
+        ids, cells = array_order(ctx)
+        assert np.all(np.diff(cells) >= 0), "cells not ascending"
+        cell_of = np.empty(ids.max() + 1, dtype=np.int64)
+        cell_of[ids] = cells
+        expect = prev_ids[np.argsort(cell_of[prev_ids], kind="stable")]
+        assert np.array_equal(ids, expect)
+        prev_ids = ids
+
+
+def test_fp32_migration_is_a_stable_counting_sort():
+    case = Case(**THERMAL, relativity=1)
+    ctx, _ = setup(case)
+    check_stable_sort_step(ctx, case.dt_cfl(), 4)
+
+
+@pytest.mark.parametrize("n", [(6, 2, 1), (3, 5, 4)])
+def test_fp32_small_periodic_dimensions(n):
+    """Periodic dimensions of 1-3 cells, where the wrap moves coincide with
+    the one-cell moves."""
+    case = Case(n=n, L=tuple(float(x) for x in n), nparticles=48 * n[0] * n[1] * n[2],
+                temp=(5000.0, 5000.0), dens_phys=2.824e9, relativity=1, B0=(0.0, 0.01, 0.0))
+    ctx, o = setup(case)
+    dt = case.dt_cfl()
+    check_stable_sort_step(ctx, dt, 3)
+    ctx2, o = setup(case)
+    run(ctx2, o, 12, dt)
+    g = by_id(ctx2)
+    p = o.particles().by_id()
+    assert rel(g["x"], p.x) < 1e-6
+
+
+def test_fp32_far_movers_match_oracle():
+    """|v| dt > dx for many particles: the ordered far-mover list path."""
+    case = Case(n=(8, 8, 8), L=(8.0, 8.0, 8.0), nparticles=2 * 16 * 512, temp=(50180.0, 50180.0),
+                dens_phys=2.824e9, relativity=0)
+    ctx, o = setup(case)
+    dt = 1.5
+    # the oracle moves by absolute positions; how many particles jump > 1 cell?
+    p0 = o.particles()
+    far = np.any(np.abs(p0.v * dt) >= 1.0, axis=1).mean()
+    assert far > 0.02
+    check_stable_sort_step(ctx, dt, 2)
+    ctx2, o = setup(case)
+    run(ctx2, o, 3, dt)
+    g = by_id(ctx2)
+    p = o.particles().by_id()
+    assert rel(g["x"], p.x) < 1e-6
+    for f in (JX, JY, JZ):
+        assert rel(ctx2.field(f), o.field(f)) < 1e-4, f
