@@ -172,7 +172,8 @@ def cpu_baseline(sample_cells: int = 64, sample_steps: int = 10):
         pass
     return {"value": case.nparticles * sample_steps / el, "unit": "particle-steps/s", "cores": 1,
             "kind": kind,
-            "sample": f"c2 physics on a {n}^3 sub-box (32 ppc/species, {case.nparticles} particles), "
+            "sample": f"c2 physics on a {n}^3 box{' (the full c2 box)' if n == 64 else ''} "
+                      f"(32 ppc/species, {case.nparticles} particles), "
                       f"{sample_steps} steps of the reference serial loop (nstep_sort=100), "
                       f"1 thread pinned; host CPU: {cpu}; {el:.2f}s"}
 
