@@ -1,0 +1,39 @@
+// geo.hpp -- the padded-grid geometry and the reference's unit constants,
+// without HIP headers, so that the per-particle arithmetic that includes it
+// also compiles as plain host C++.
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#define EMO_HD __host__ __device__
+#else
+#define EMO_HD
+#endif
+
+namespace emo {
+
+// reference constants (src/globals.hpp:12,16,20)
+constexpr double kUnitAcc = 0.58668774;
+constexpr double kUnitRhoJ = 4.80320427e-13;
+constexpr double kUnitVth = 0.002232;
+constexpr double kPi = 3.14159265358979323846;
+
+// Geometry of the padded grid, passed by value to every field kernel.
+struct Geo {
+  int n[3];       // physical cells
+  int nt[3];      // n+3 padded points (src/grid/grid.cpp:25-27)
+  int nc[3];      // n+2 cell-id extents nx_,ny_,nz_ (src/grid/grid.cpp:22-24)
+  double x0[3];
+  double d[3];    // Lxyz/nCell
+  double id[3];   // 1.0/d
+  long long sx, sy;  // strides of i and j in the padded arrays (k stride 1)
+  long long ntot;
+  long long ncell;
+};
+
+EMO_HD inline long long gidx(const Geo &g, int i, int j, int k) {
+  return ((long long)i * g.nt[1] + j) * g.nt[2] + k;
+}
+
+}  // namespace emo

@@ -11,6 +11,7 @@
 
 #include "deposit_grid.hpp"
 #include "emo_internal.hpp"
+#include "pic_math.hpp"
 
 namespace emo {
 namespace {
@@ -20,23 +21,6 @@ constexpr int kBlock = 256;
 inline unsigned grid_for(long long n, int block = kBlock) {
   long long g = (n + block - 1) / block;
   return (unsigned)(g > 0 ? g : 1);
-}
-
-// Grid::getCellID (src/grid/oGrid.cpp:208-222); returns -1 outside the
-// padded grid (where the reference asserts).
-__device__ inline int cell_of(const Geo &g, double x, double y, double z, int *ix, int *iy,
-                              int *iz) {
-  int a = (int)((x - g.x0[0]) * g.id[0]);
-  a += 1;
-  int b = (int)((y - g.x0[1]) * g.id[1]);
-  b += 1;
-  int c = (int)((z - g.x0[2]) * g.id[2]);
-  c += 1;
-  *ix = a;
-  *iy = b;
-  *iz = c;
-  if (a < 0 || a >= g.nc[0] || b < 0 || b >= g.nc[1] || c < 0 || c >= g.nc[2]) return -1;
-  return (g.nc[1] * g.nc[2]) * a + g.nc[2] * b + c;
 }
 
 // ---------------------------------------------------------------------------
@@ -62,49 +46,11 @@ __global__ void __launch_bounds__(kBlock) k_gather(GatherArgs a) {
     atomicOr(a.flag, kFlagOutOfGrid);
     return;
   }
-  double cv[21];
-  cv[0] = g.x0[0] + g.d[0] * ix;
-  cv[1] = g.x0[1] + g.d[1] * iy;
-  cv[2] = g.x0[2] + g.d[2] * iz;
-  const double *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
-  cv[3] = Ex[gidx(g, ix, iy, iz)];
-  cv[4] = Ex[gidx(g, ix, iy + 1, iz)];
-  cv[5] = Ex[gidx(g, ix, iy + 1, iz + 1)];
-  cv[6] = Ex[gidx(g, ix, iy, iz + 1)];
-  cv[7] = Ey[gidx(g, ix, iy, iz)];
-  cv[8] = Ey[gidx(g, ix, iy, iz + 1)];
-  cv[9] = Ey[gidx(g, ix + 1, iy, iz + 1)];
-  cv[10] = Ey[gidx(g, ix + 1, iy, iz)];
-  cv[11] = Ez[gidx(g, ix, iy, iz)];
-  cv[12] = Ez[gidx(g, ix + 1, iy, iz)];
-  cv[13] = Ez[gidx(g, ix + 1, iy + 1, iz)];
-  cv[14] = Ez[gidx(g, ix, iy + 1, iz)];
-  cv[15] = (a.Bt[0][gidx(g, ix, iy, iz)] + a.B[0][gidx(g, ix, iy, iz)]) / 2;
-  cv[16] = (a.Bt[0][gidx(g, ix + 1, iy, iz)] + a.B[0][gidx(g, ix + 1, iy, iz)]) / 2;
-  cv[17] = (a.Bt[1][gidx(g, ix, iy, iz)] + a.B[1][gidx(g, ix, iy, iz)]) / 2;
-  cv[18] = (a.Bt[1][gidx(g, ix, iy + 1, iz)] + a.B[1][gidx(g, ix, iy + 1, iz)]) / 2;
-  cv[19] = (a.Bt[2][gidx(g, ix, iy, iz)] + a.B[2][gidx(g, ix, iy, iz)]) / 2;
-  cv[20] = (a.Bt[2][gidx(g, ix, iy, iz + 1)] + a.B[2][gidx(g, ix, iy, iz + 1)]) / 2;
-  const double *lcell = g.d;
-  double ds[3];
+  double cv[21], fld[6];
+  cell_vector(g, a.E, a.B, a.Bt, ix, iy, iz, cv);
+  interp_fields(g, cv, pos, fld);
 #pragma unroll
-  for (int q = 0; q < 3; q++) ds[q] = pos[q] - cv[q];
-#pragma unroll
-  for (int q = 0; q < 3; q++) {
-    int j = (q + 1) % 3, k = (q + 2) % 3;
-    double eField = 0;
-    eField += (lcell[j] - ds[j]) * (lcell[k] - ds[k]) * cv[3 + 4 * q];
-    eField += ds[j] * (lcell[k] - ds[k]) * cv[3 + 4 * q + 1];
-    eField += ds[j] * ds[k] * cv[3 + 4 * q + 2];
-    eField += (lcell[j] - ds[j]) * ds[k] * cv[3 + 4 * q + 3];
-    eField /= lcell[j] * lcell[k];
-    double bField = 0;
-    bField += (lcell[q] - ds[q]) * cv[3 + 12 + 2 * q];
-    bField += ds[q] * cv[3 + 12 + 2 * q + 1];
-    bField /= lcell[q];
-    a.fld[q][i] = eField;
-    a.fld[3 + q][i] = bField;
-  }
+  for (int q = 0; q < 6; q++) a.fld[q][i] = fld[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -124,86 +70,36 @@ __global__ void __launch_bounds__(kBlock) k_push_boris(PushArgs a) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
   int sp = a.sp[i];
-  double dt = a.dt;
-  double x = a.x[0][i], y = a.x[1][i], z = a.x[2][i];
-  double vx = a.v[0][i], vy = a.v[1][i], vz = a.v[2][i];
-  double q = a.s.q[sp], m = a.s.m[sp];
-  double q_p = (0.5 * dt * q) / m;
-  q_p *= kUnitAcc;
-  double ex = a.fld[0][i], ey = a.fld[1][i], ez = a.fld[2][i];
-  double bx = a.fld[3][i], by = a.fld[4][i], bz = a.fld[5][i];
-  double tx = q_p * bx, ty = q_p * by, tz = q_p * bz;
-  double bsqr = bx * bx + by * by + bz * bz;
-  double tsqr = 2.0 / (1.0 + q_p * q_p * bsqr);
-  double sx = tsqr * tx, sy = tsqr * ty, sz = tsqr * tz;
-  double v_mx = vx + q_p * ex;
-  double v_my = vy + q_p * ey;
-  double v_mz = vz + q_p * ez;
-  double v_ix = v_mx + v_my * tz - v_mz * ty;
-  double v_iy = v_my + v_mz * tx - v_mx * tz;
-  double v_iz = v_mz + v_mx * ty - v_my * tx;
-  double v_px = v_mx + v_iy * sz - v_iz * sy;
-  double v_py = v_my + v_iz * sx - v_ix * sz;
-  double v_pz = v_mz + v_ix * sy - v_iy * sx;
-  double vx_new = v_px + q_p * ex;
-  double vy_new = v_py + q_p * ey;
-  double vz_new = v_pz + q_p * ez;
-  a.x[0][i] = x + dt * vx_new;
-  a.x[1][i] = y + dt * vy_new;
-  a.x[2][i] = z + dt * vz_new;
-  a.v[0][i] = vx_new;
-  a.v[1][i] = vy_new;
-  a.v[2][i] = vz_new;
+  double xp[3] = {a.x[0][i], a.x[1][i], a.x[2][i]};
+  double vp[3] = {a.v[0][i], a.v[1][i], a.v[2][i]};
+  double f[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) f[q] = a.fld[q][i];
+  boris_step(xp, vp, f, a.s.q[sp], a.s.m[sp], a.dt);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    a.x[d][i] = xp[d];
+    a.v[d][i] = vp[d];
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_push_vay(PushArgs a) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
   int sp = a.sp[i];
-  double dt = a.dt;
-  double x = a.x[0][i], y = a.x[1][i], z = a.x[2][i];
-  double vx = a.v[0][i], vy = a.v[1][i], vz = a.v[2][i];
-  double gamma = a.gam[i];
-  double q = a.s.q[sp], m = a.s.m[sp];
-  double q_p = (0.5 * dt * q) / m;
-  q_p *= kUnitAcc;
-  double ex = a.fld[0][i], ey = a.fld[1][i], ez = a.fld[2][i];
-  double bx = a.fld[3][i], by = a.fld[4][i], bz = a.fld[5][i];
-  double vXbx = vy * bz - vz * by;
-  double vXby = vz * bx - vx * bz;
-  double vXbz = vx * by - vy * bx;
-  double uhx = gamma * vx + q_p * (ex + vXbx);
-  double uhy = gamma * vy + q_p * (ey + vXby);
-  double uhz = gamma * vz + q_p * (ez + vXbz);
-  double upx = uhx + q_p * ex;
-  double upy = uhy + q_p * ey;
-  double upz = uhz + q_p * ez;
-  double gammap_sq = 1.0 + upx * upx + upy * upy + upz * upz;
-  double taux = q_p * bx, tauy = q_p * by, tauz = q_p * bz;
-  double tau_sq = taux * taux + tauy * tauy + tauz * tauz;
-  double sigma = gammap_sq - tau_sq;
-  double ustar = upx * taux + upy * tauy + upz * tauz;
-  double gamma_int = sqrt(sigma * sigma + 4 * (tau_sq + ustar * ustar));
-  gamma_int = sqrt(0.5 * (sigma + gamma_int));
-  double tx = taux / gamma_int, ty = tauy / gamma_int, tz = tauz / gamma_int;
-  double s = 1.0 / (1.0 + tx * tx + ty * ty + tz * tz);
-  double udott = ustar / gamma_int;
-  double upXt_x = upy * tz - upz * ty;
-  double upXt_y = upz * tx - upx * tz;
-  double upXt_z = upx * ty - upy * tx;
-  double uint_x = s * (upx + udott * tx + upXt_x);
-  double uint_y = s * (upy + udott * ty + upXt_y);
-  double uint_z = s * (upz + udott * tz + upXt_z);
-  double vx_new = uint_x / gamma_int;
-  double vy_new = uint_y / gamma_int;
-  double vz_new = uint_z / gamma_int;
-  a.x[0][i] = x + dt * vx_new;
-  a.x[1][i] = y + dt * vy_new;
-  a.x[2][i] = z + dt * vz_new;
-  a.v[0][i] = vx_new;
-  a.v[1][i] = vy_new;
-  a.v[2][i] = vz_new;
-  a.gam[i] = gamma_int;
+  double xp[3] = {a.x[0][i], a.x[1][i], a.x[2][i]};
+  double vp[3] = {a.v[0][i], a.v[1][i], a.v[2][i]};
+  double gam = a.gam[i];
+  double f[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) f[q] = a.fld[q][i];
+  vay_step(xp, vp, gam, f, a.s.q[sp], a.s.m[sp], a.dt);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    a.x[d][i] = xp[d];
+    a.v[d][i] = vp[d];
+  }
+  a.gam[i] = gam;
 }
 
 // ---------------------------------------------------------------------------
@@ -223,24 +119,14 @@ struct BcArgs {
 __global__ void __launch_bounds__(kBlock) k_particle_bc(BcArgs a) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
+  double xp[3] = {a.x[0][i], a.x[1][i], a.x[2][i]};
+  double vp[3] = {a.v[0][i], a.v[1][i], a.v[2][i]};
+  int flips[3];
+  particle_bcs(xp, vp, a.xmin, a.xmax, a.pbc, flips);
 #pragma unroll
-  for (int side = 0; side < 6; side++) {
-    int d = side >> 1, right = side & 1;
-    double xMin = a.xmin[d], xMax = a.xmax[d];
-    double xp = a.x[d][i];
-    if (a.pbc[side] == EMO_BC_PERIODIC) {
-      if (!right && xp < xMin) a.x[d][i] = xp + (xMax - xMin);
-      if (right && xp > xMax) a.x[d][i] = xp - (xMax - xMin);
-    } else {
-      if (right && xp > xMax) {
-        a.x[d][i] = 2.0 * xMax - xp;
-        a.v[d][i] = -a.v[d][i];
-      }
-      if (!right && xp < xMin) {
-        a.x[d][i] = 2.0 * xMin - xp;
-        a.v[d][i] = -a.v[d][i];
-      }
-    }
+  for (int d = 0; d < 3; d++) {
+    a.x[d][i] = xp[d];
+    a.v[d][i] = vp[d];
   }
 }
 
@@ -354,8 +240,6 @@ __global__ void __launch_bounds__(kBlock) k_deposit_cells(DepCellArgs a) {
       cellverts[0] = g.x0[0] + g.d[0] * ix;
       cellverts[1] = g.x0[1] + g.d[1] * iy;
       cellverts[2] = g.x0[2] + g.d[2] * iz;
-      const double *lcell = g.d;
-      const double cellVolume = lcell[0] * lcell[1] * lcell[2];
       for (int base = b; base < e; base += 64) {
         const int p = base + lane;
         bool valid = false;
@@ -363,32 +247,14 @@ __global__ void __launch_bounds__(kBlock) k_deposit_cells(DepCellArgs a) {
           int sp = a.sp[p];
           if (!a.s.test[sp]) {
             valid = true;
-            double pcharge = a.s.q[sp];
-            double relpos[3] = {a.x[0][p] - cellverts[0], a.x[1][p] - cellverts[1], a.x[2][p] - cellverts[2]};
+            double pos[3] = {a.x[0][p], a.x[1][p], a.x[2][p]};
             double vel[3] = {a.v[0][p], a.v[1][p], a.v[2][p]};
+            double r20[20];
+            deposit_row(g, cellverts, pos, vel, a.s.q[sp], a.rho != 0, r20);
             double *r = row + lane * kDepStride;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-              int j = (q + 1) % 3, k = (q + 2) % 3;
-              double cellArea = lcell[j] * lcell[k];
-              double curIn1D = kUnitRhoJ * pcharge * vel[q];
-              double cur2AreaRatio = (curIn1D / cellVolume) / cellArea;
-              r[4 * q] = cur2AreaRatio * (lcell[j] - relpos[j]) * (lcell[k] - relpos[k]);
-              r[4 * q + 1] = cur2AreaRatio * relpos[j] * (lcell[k] - relpos[k]);
-              r[4 * q + 2] = cur2AreaRatio * relpos[j] * relpos[k];
-              r[4 * q + 3] = cur2AreaRatio * (lcell[j] - relpos[j]) * relpos[k];
-            }
-            if (a.rho) {
-              double cc = kUnitRhoJ * (pcharge / cellVolume) / cellVolume;
-              r[12] = cc * (lcell[0] - relpos[0]) * (lcell[1] - relpos[1]) * (lcell[2] - relpos[2]);
-              r[13] = cc * relpos[0] * (lcell[1] - relpos[1]) * (lcell[2] - relpos[2]);
-              r[14] = cc * relpos[0] * relpos[1] * (lcell[2] - relpos[2]);
-              r[15] = cc * (lcell[0] - relpos[0]) * relpos[1] * (lcell[2] - relpos[2]);
-              r[16] = cc * (lcell[0] - relpos[0]) * relpos[1] * relpos[2];
-              r[17] = cc * relpos[0] * relpos[1] * relpos[2];
-              r[18] = cc * relpos[0] * (lcell[1] - relpos[1]) * relpos[2];
-              r[19] = cc * (lcell[0] - relpos[0]) * (lcell[1] - relpos[1]) * relpos[2];
-            }
+            for (int q = 0; q < 20; q++)
+              if (q < nv) r[q] = r20[q];
           }
         }
         const unsigned long long m = __ballot(valid);
