@@ -30,7 +30,7 @@ EXPORTS = (
     "emo_init_constant", "emo_init_poisson", "emo_sort", "emo_push", "emo_particle_bc",
     "emo_deposit", "emo_fold", "emo_evolve_fields", "emo_refresh", "emo_interpolate",
     "emo_step", "emo_diag", "emo_synchronize", "emo_timing_enable", "emo_kernel_time",
-    "emo_timing_reset",
+    "emo_timing_reset", "emo_adjoint_field_energy", "emo_loader_dx_deps",
 )
 
 
@@ -136,6 +136,8 @@ def lib():
     L.emo_diag.argtypes = [vp, vp]
     L.emo_timing_enable.argtypes = [vp, i32]
     L.emo_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(dbl), C.POINTER(i64)]
+    L.emo_adjoint_field_energy.argtypes = [vp, i32, dbl, dbl, i32, vp, vp, vp, vp, vp, vp]
+    L.emo_loader_dx_deps.argtypes = [C.POINTER(EmoLoader), C.POINTER(EmoConfig), vp]
     _lib = L
     return L
 
@@ -275,3 +277,30 @@ class Context:
         n = C.c_int64()
         self._chk(self.L.emo_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    # adjoint (FP64 plain form)
+    def adjoint_field_energy(self, nsteps, t0, dt, checkpoint_every=0, particles=True):
+        """Runs nsteps forward and the adjoint back; returns the final field
+        energy and its gradient (see emo_adjoint_field_energy)."""
+        nw = int(self.cfg.nwaves)
+        n = self.num_particles
+        e = np.zeros(1)
+        da = np.zeros(max(nw, 1))
+        dp = np.zeros(max(nw, 1))
+        xb = np.zeros((n, 3)) if particles else None
+        vb = np.zeros((n, 3)) if particles else None
+        gb = np.zeros(n) if particles else None
+        self._chk(self.L.emo_adjoint_field_energy(
+            self.h, nsteps, t0, dt, checkpoint_every, _p(e), _p(da), _p(dp),
+            _p(xb) if particles else None, _p(vb) if particles else None, _p(gb) if particles else None))
+        return dict(energy=float(e[0]), d_amp=da[:nw], d_phase=da[:0] if nw == 0 else dp[:nw],
+                    xbar0=xb, vbar0=vb, gbar0=gb)
+
+
+def loader_dx_deps(ld: EmoLoader, cfg: EmoConfig) -> np.ndarray:
+    """d x0 / d pert_eps of every loaded particle (id order)."""
+    out = np.zeros(int(ld.nparticles_tot))
+    rc = lib().emo_loader_dx_deps(C.byref(ld), C.byref(cfg), _p(out))
+    if rc != OK:
+        raise EmoError(rc, "emo_loader_dx_deps failed")
+    return out

@@ -40,6 +40,7 @@ SOURCES = [
     ("loader.cpp", ["-ffp-contract=off"]),
     ("boundary.cpp", ["-ffp-contract=off"]),
     ("chosen.hip", ["-ffp-contract=fast"]),
+    ("adjoint.hip", ["-ffp-contract=off"]),
 ]
 
 

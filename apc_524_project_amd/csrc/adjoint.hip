@@ -1,0 +1,1050 @@
+// adjoint.hip -- discrete adjoint of the FP64 plain-form step for gfx950.
+//
+// The objective is the final field energy of OutputBoxQuantities
+// (src/IO/output.cpp:90-121): F = E_en + B_en from E and the fixed B
+// allocations (Grid::fieldPtr_).  Its gradient with respect to the injected
+// pulses' amplitudes and phases (LightBC, src/grid/lightBC.cpp:165-198) and
+// to the initial particle state is computed by running the exact transpose
+// of every operation of the step backward in time:
+//
+//   forward step s:  push -> particle BCs -> stable sort -> deposit J -> fold
+//                    -> Yee (swap, E, B) -> ghost refresh / LightBC -> gather
+//   adjoint step s:  gather^T -> refresh^T -> Yee^T -> fold^T -> deposit^T
+//                    -> sort^T -> BCs^T -> push^T
+//
+// Forward states are checkpointed every K steps; each segment is re-run from
+// its checkpoint with a tape (pre-push particle state, the sort permutation
+// and cell ranges, the fields at the start of the step), then walked
+// backward.  Every accumulation is in a fixed order (per-cell sums in array
+// order, per-point sums over neighbouring cells in a fixed order, fixed
+// reduction trees): the gradient is deterministic.
+//
+// Cell changes of particles are piecewise constant in the inputs and the
+// reference's linear weights jump at cell faces (its least vertex is the
+// cell's upper corner), so the adjoint is the derivative of the step away
+// from those faces -- the derivative that central differences reproduce when
+// no particle changes cell under the perturbation.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "adjoint_math.hpp"
+#include "emo_internal.hpp"
+
+namespace emo {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+inline unsigned grid_for(long long n, int block = kBlock) {
+  long long g = (n + block - 1) / block;
+  return (unsigned)(g > 0 ? g : 1);
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline bool is_phys(const Geo &g, int i, int j, int k) {
+  return i >= 1 && i <= g.n[0] && j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2];
+}
+
+__device__ inline void point_ijk(const Geo &g, long long p, int *i, int *j, int *k) {
+  *k = (int)(p % g.nt[2]);
+  long long r = p / g.nt[2];
+  *j = (int)(r % g.nt[1]);
+  *i = (int)(r / g.nt[1]);
+}
+
+// ---------------------------------------------------------------------------
+// Objective seed: dF/dE and dF/dB for F = sum over physical cells of
+// 0.5 V (0.5 (f[p] + f[p+s]))^2 per component (s along its own axis), the
+// expression of OutputBoxQuantities with the SQR macro expanded.
+// ---------------------------------------------------------------------------
+struct SeedArgs {
+  Geo g;
+  const double *E[3], *B[3];
+  double *Eb[3], *Bb[3];
+};
+
+__global__ void __launch_bounds__(kBlock) k_seed_energy(SeedArgs a) {
+  const Geo &g = a.g;
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.ntot) return;
+  int i, j, k;
+  point_ijk(g, p, &i, &j, &k);
+  const double vol = g.d[0] * g.d[1] * g.d[2];
+  const long long st[3] = {g.sx, g.sy, 1};
+  for (int q = 0; q < 3; q++) {
+    int di = q == 0, dj = q == 1, dk = q == 2;
+    double ge = 0.0, gb = 0.0;
+    if (is_phys(g, i, j, k)) {
+      ge += 0.25 * vol * (a.E[q][p] + a.E[q][p + st[q]]);
+      gb += 0.25 * vol * (a.B[q][p] + a.B[q][p + st[q]]);
+    }
+    if (is_phys(g, i - di, j - dj, k - dk)) {
+      ge += 0.25 * vol * (a.E[q][p - st[q]] + a.E[q][p]);
+      gb += 0.25 * vol * (a.B[q][p - st[q]] + a.B[q][p]);
+    }
+    a.Eb[q][p] += ge;
+    a.Bb[q][p] += gb;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost refresh^T.  Periodic copy dst <- src over the physical range of the
+// other two dims: src_b += dst_b, dst_b = 0.  LightBC plane fill over the
+// full plane: the plane's adjoint, weighted by the transverse profile, is the
+// adjoint of the pulse value; the plane's adjoint is then zero.
+// ---------------------------------------------------------------------------
+struct CopyAdjArgs {
+  Geo g;
+  double *f[6];
+  int nf;
+  int d, src, dst;
+};
+
+__global__ void __launch_bounds__(kBlock) k_plane_copy_adj(CopyAdjArgs a) {
+  const Geo &g = a.g;
+  int d = a.d;
+  int u = d == 0 ? 1 : 0;
+  int w = d == 2 ? 1 : 2;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.n[u] * g.n[w]) return;
+  int is[3], id[3];
+  is[d] = a.src;
+  id[d] = a.dst;
+  is[u] = id[u] = 1 + (int)(t / g.n[w]);
+  is[w] = id[w] = 1 + (int)(t % g.n[w]);
+  long long ps = gidx(g, is[0], is[1], is[2]);
+  long long pd = gidx(g, id[0], id[1], id[2]);
+  for (int f = 0; f < a.nf; f++) {
+    a.f[f][ps] += a.f[f][pd];
+    a.f[f][pd] = 0.0;
+  }
+}
+
+struct FillAdjArgs {
+  Geo g;
+  double *f;
+  int dim, idx;
+  double waist;
+  double off[3];
+  double Lc[3];
+  double *out;  // the weighted plane sum
+};
+
+// one block: fixed-order sum of the plane's adjoint x profile, then zero it
+__global__ void __launch_bounds__(kBlock) k_plane_fill_adj(FillAdjArgs a) {
+  __shared__ double sh[kBlock];
+  const Geo &g = a.g;
+  int d = a.dim;
+  int u = d == 0 ? 1 : 0;
+  int w = d == 2 ? 1 : 2;
+  long long n = (long long)g.nt[u] * g.nt[w];
+  double s = 0.0;
+  for (long long t = threadIdx.x; t < n; t += kBlock) {
+    int pu = (int)(t / g.nt[w]);
+    int pw = (int)(t % g.nt[w]);
+    int ii[3];
+    ii[d] = a.idx;
+    ii[u] = pu;
+    ii[w] = pw;
+    long long p = gidx(g, ii[0], ii[1], ii[2]);
+    double prof = 1.0;
+    if (a.waist > 0) {
+      double ya = g.x0[u] + g.d[u] * (pu - 1) + a.off[u] - a.Lc[u];
+      double yb = g.x0[w] + g.d[w] * (pw - 1) + a.off[w] - a.Lc[w];
+      prof = exp(-(ya * ya + yb * yb) / (a.waist * a.waist));
+    }
+    s += a.f[p] * prof;
+    a.f[p] = 0.0;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = kBlock / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *a.out = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// Yee^T (Grid::evolveFields, src/grid/oGrid.cpp:15-60).  With Bn the array
+// the B update writes and Bo the previous B (B_tm1 after the swap):
+//   E' = E - dt (curl_back(Bo) + 4 pi J)     (physical points)
+//   Bn = Bo + dt curl_fwd(E')                (physical points)
+// Reverse: (1) E'_b += dt curl_fwd^T(Bn_b); (2) Bo_b += Bn_b, Bn_b = 0 on the
+// physical points; (3) Bo_b -= dt curl_back^T(E'_b), J_b = -dt 4 pi E'_b.
+// Gather form: every point reads its neighbours' adjoints, no atomics.
+// ---------------------------------------------------------------------------
+struct YeeAdjArgs {
+  Geo g;
+  double *Eb[3];
+  double *Bn[3];
+  double *Bo[3];
+  double *Jb[3];
+  double dt;
+};
+
+__device__ inline double phys_val(const Geo &g, const double *f, int i, int j, int k) {
+  return is_phys(g, i, j, k) ? f[gidx(g, i, j, k)] : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_yee_b_adj(YeeAdjArgs a) {
+  const Geo &g = a.g;
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.ntot) return;
+  int i, j, k;
+  point_ijk(g, p, &i, &j, &k);
+  const double dt = a.dt, dx = g.d[0], dy = g.d[1], dz = g.d[2];
+  const double *bx = a.Bn[0], *by = a.Bn[1], *bz = a.Bn[2];
+  // Bx = .. + dt((Ey[c+1]-Ey)/dz - (Ez[c+sy]-Ez)/dy)
+  // By = .. + dt((Ez[c+sx]-Ez)/dx - (Ex[c+1]-Ex)/dz)
+  // Bz = .. + dt((Ex[c+sy]-Ex)/dy - (Ey[c+sx]-Ey)/dx)
+  double ex = (phys_val(g, by, i, j, k) - phys_val(g, by, i, j, k - 1)) / dz +
+              (phys_val(g, bz, i, j - 1, k) - phys_val(g, bz, i, j, k)) / dy;
+  double ey = (phys_val(g, bx, i, j, k - 1) - phys_val(g, bx, i, j, k)) / dz +
+              (phys_val(g, bz, i, j, k) - phys_val(g, bz, i - 1, j, k)) / dx;
+  double ez = (phys_val(g, bx, i, j, k) - phys_val(g, bx, i, j - 1, k)) / dy +
+              (phys_val(g, by, i - 1, j, k) - phys_val(g, by, i, j, k)) / dx;
+  a.Eb[0][p] += dt * ex;
+  a.Eb[1][p] += dt * ey;
+  a.Eb[2][p] += dt * ez;
+}
+
+__global__ void __launch_bounds__(kBlock) k_yee_move_adj(YeeAdjArgs a) {
+  const Geo &g = a.g;
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.ntot) return;
+  int i, j, k;
+  point_ijk(g, p, &i, &j, &k);
+  if (!is_phys(g, i, j, k)) return;
+  for (int q = 0; q < 3; q++) {
+    a.Bo[q][p] += a.Bn[q][p];
+    a.Bn[q][p] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_yee_e_adj(YeeAdjArgs a) {
+  const Geo &g = a.g;
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.ntot) return;
+  int i, j, k;
+  point_ijk(g, p, &i, &j, &k);
+  const double dt = a.dt, dx = g.d[0], dy = g.d[1], dz = g.d[2];
+  const double *ex = a.Eb[0], *ey = a.Eb[1], *ez = a.Eb[2];
+  // Ex -= dt((By-By[c-1])/dz - (Bz-Bz[c-sy])/dy + 4pi Jx)
+  // Ey -= dt((Bz-Bz[c-sx])/dx - (Bx-Bx[c-1])/dz + 4pi Jy)
+  // Ez -= dt((Bx-Bx[c-sy])/dy - (By-By[c-sx])/dx + 4pi Jz)
+  double bx = (phys_val(g, ey, i, j, k) - phys_val(g, ey, i, j, k + 1)) / dz -
+              (phys_val(g, ez, i, j, k) - phys_val(g, ez, i, j + 1, k)) / dy;
+  double by = -(phys_val(g, ex, i, j, k) - phys_val(g, ex, i, j, k + 1)) / dz +
+              (phys_val(g, ez, i, j, k) - phys_val(g, ez, i + 1, j, k)) / dx;
+  double bz = (phys_val(g, ex, i, j, k) - phys_val(g, ex, i, j + 1, k)) / dy -
+              (phys_val(g, ey, i, j, k) - phys_val(g, ey, i + 1, j, k)) / dx;
+  a.Bo[0][p] += dt * bx;
+  a.Bo[1][p] += dt * by;
+  a.Bo[2][p] += dt * bz;
+  const double fpi = 4 * kPi;
+  const bool ph = is_phys(g, i, j, k);
+  for (int q = 0; q < 3; q++) a.Jb[q][p] = ph ? -dt * fpi * a.Eb[q][p] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Per-particle adjoints.
+// ---------------------------------------------------------------------------
+struct PartView {
+  const double *x[3], *v[3], *gam, *fld[6];
+  const int32_t *sp;
+};
+
+struct PartAdj {
+  double *x[3], *v[3], *gam, *fld[6];
+};
+
+// deposit^T: the particle reads the current adjoint at its 12 targets
+struct DepAdjArgs {
+  Geo g;
+  PartView s;  // post-BC state in the sorted order
+  PartAdj b;
+  const double *Jb[3];
+  SpeciesParams sp;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_deposit_adj(DepAdjArgs a) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const Geo &g = a.g;
+  const int sp = a.s.sp[i];
+  if (a.sp.test[sp]) return;
+  double pos[3] = {a.s.x[0][i], a.s.x[1][i], a.s.x[2][i]};
+  double vel[3] = {a.s.v[0][i], a.s.v[1][i], a.s.v[2][i]};
+  int ix, iy, iz;
+  if (cell_of(g, pos[0], pos[1], pos[2], &ix, &iy, &iz) < 0) return;
+  double cellverts[3] = {g.x0[0] + g.d[0] * ix, g.x0[1] + g.d[1] * iy, g.x0[2] + g.d[2] * iz};
+  long long idx[12];
+  deposit_target_index(g, ix, iy, iz, idx);
+  double rb[12];
+  for (int e = 0; e < 12; e++) rb[e] = a.Jb[e / 4][idx[e]];
+  double posb[3] = {0, 0, 0}, velb[3] = {0, 0, 0};
+  deposit_row_vjp(g, cellverts, pos, vel, a.sp.q[sp], rb, posb, velb);
+  for (int d = 0; d < 3; d++) {
+    a.b.x[d][i] += posb[d];
+    a.b.v[d][i] += velb[d];
+  }
+}
+
+// sort^T: out[perm[j]] = in[j]
+struct SortAdjArgs {
+  const double *in[7];
+  double *out[7];
+  const uint32_t *perm;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_sort_adj(SortAdjArgs a) {
+  long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.np) return;
+  uint32_t s = a.perm[j];
+  for (int q = 0; q < 7; q++) a.out[q][s] = a.in[q][j];
+}
+
+// BCs^T + push^T from the taped pre-push state; writes the pre-push adjoints
+// and the adjoint of the gathered fields
+struct PushAdjArgs {
+  PartView s;
+  PartAdj b;
+  SpeciesParams sp;
+  double dt;
+  double xmin[3], xmax[3];
+  int pbc[6];
+  int rel;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_push_adj(PushAdjArgs a) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const int sp = a.s.sp[i];
+  const double q = a.sp.q[sp], m = a.sp.m[sp];
+  double x[3] = {a.s.x[0][i], a.s.x[1][i], a.s.x[2][i]};
+  double v[3] = {a.s.v[0][i], a.s.v[1][i], a.s.v[2][i]};
+  double gam = a.s.gam[i];
+  double f[6];
+  for (int k = 0; k < 6; k++) f[k] = a.s.fld[k][i];
+  // forward recomputation up to the BCs (for the reflections)
+  double x1[3] = {x[0], x[1], x[2]}, v1[3] = {v[0], v[1], v[2]}, g1 = gam;
+  if (a.rel) vay_step(x1, v1, g1, f, q, m, a.dt);
+  else boris_step(x1, v1, f, q, m, a.dt);
+  int flips[3];
+  particle_bcs(x1, v1, a.xmin, a.xmax, a.pbc, flips);
+  double xb1[3], vb1[3];
+  for (int d = 0; d < 3; d++) {
+    xb1[d] = a.b.x[d][i];
+    vb1[d] = a.b.v[d][i];
+  }
+  double gb1 = a.b.gam[i];
+  particle_bcs_vjp(flips, xb1, vb1);
+  double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, gb = 0.0, fb[6] = {0, 0, 0, 0, 0, 0};
+  if (a.rel) {
+    vay_step_vjp(v, gam, f, q, m, a.dt, xb1, vb1, gb1, xb, vb, gb, fb);
+  } else {
+    boris_step_vjp(v, f, q, m, a.dt, xb1, vb1, xb, vb, fb);
+    gb = gb1;  // Boris leaves the stored gamma untouched
+  }
+  for (int d = 0; d < 3; d++) {
+    a.b.x[d][i] = xb[d];
+    a.b.v[d][i] = vb[d];
+  }
+  a.b.gam[i] = gb;
+  for (int k = 0; k < 6; k++) a.b.fld[k][i] = fb[k];
+}
+
+// gather^T, stage 1: one wavefront per cell; each particle's position
+// adjoint, and the cell's 18 field-entry adjoints summed in array order
+// (rows in LDS, lanes 0..17 add columns row by row)
+struct GatherAdjArgs {
+  Geo g;
+  PartView s;  // positions at gather time, sorted order
+  PartAdj b;   // x += ; reads fld
+  const double *E[3], *B[3], *Bt[3];
+  const int32_t *beg, *end;
+  double *cg;  // 18 per cell, component-major
+};
+
+constexpr int kGStride = 19;
+
+__global__ void __launch_bounds__(kBlock) k_gather_adj_cells(GatherAdjArgs a) {
+  __shared__ double rows[kWaves][64 * kGStride];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const Geo &g = a.g;
+  const long long c = (long long)blockIdx.x * kWaves + w;
+  if (c >= g.ncell) return;
+  double *row = rows[w];
+  const int b = a.beg[c], e = a.end[c];
+  double acc = 0.0;
+  for (int base = b; base < e; base += 64) {
+    const int p = base + lane;
+    if (p < e) {
+      double pos[3] = {a.s.x[0][p], a.s.x[1][p], a.s.x[2][p]};
+      int ix, iy, iz;
+      cell_of(g, pos[0], pos[1], pos[2], &ix, &iy, &iz);
+      double cv[21], cvb[21], fb[6], posb[3] = {0, 0, 0};
+      cell_vector(g, a.E, a.B, a.Bt, ix, iy, iz, cv);
+      for (int q = 0; q < 21; q++) cvb[q] = 0.0;
+      for (int q = 0; q < 6; q++) fb[q] = a.b.fld[q][p];
+      interp_fields_vjp(g, cv, pos, fb, posb, cvb);
+      for (int d = 0; d < 3; d++) a.b.x[d][p] += posb[d];
+      double *r = row + lane * kGStride;
+#pragma unroll
+      for (int q = 0; q < 18; q++) r[q] = cvb[3 + q];
+    }
+    wave_sync();
+    const int cnt = min(64, e - base);
+    if (lane < 18)
+      for (int rr = 0; rr < cnt; rr++) acc += row[rr * kGStride + lane];
+    wave_sync();
+  }
+  if (lane < 18) a.cg[lane * g.ncell + c] = acc;
+}
+
+// gather^T, stage 2: every point adds the entries of the cells whose
+// cell_vector reads it, in a fixed order
+struct GatherGridArgs {
+  Geo g;
+  const double *cg;
+  double *Eb[3], *Bb[3], *Btb[3];
+};
+
+__device__ inline double cg_at(const Geo &g, const double *cg, int q, int i, int j, int k) {
+  if (i < 0 || j < 0 || k < 0 || i >= g.nc[0] || j >= g.nc[1] || k >= g.nc[2]) return 0.0;
+  return cg[(long long)q * g.ncell + ((long long)i * g.nc[1] + j) * g.nc[2] + k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather_adj_grid(GatherGridArgs a) {
+  const Geo &g = a.g;
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.ntot) return;
+  int i, j, k;
+  point_ijk(g, p, &i, &j, &k);
+  const double *cg = a.cg;
+  // Ex: cells (i,j,k)#0, (i,j-1,k)#1, (i,j-1,k-1)#2, (i,j,k-1)#3
+  a.Eb[0][p] += cg_at(g, cg, 0, i, j, k) + cg_at(g, cg, 1, i, j - 1, k) + cg_at(g, cg, 2, i, j - 1, k - 1) +
+                cg_at(g, cg, 3, i, j, k - 1);
+  // Ey: (i,j,k)#4, (i,j,k-1)#5, (i-1,j,k-1)#6, (i-1,j,k)#7
+  a.Eb[1][p] += cg_at(g, cg, 4, i, j, k) + cg_at(g, cg, 5, i, j, k - 1) + cg_at(g, cg, 6, i - 1, j, k - 1) +
+                cg_at(g, cg, 7, i - 1, j, k);
+  // Ez: (i,j,k)#8, (i-1,j,k)#9, (i-1,j-1,k)#10, (i,j-1,k)#11
+  a.Eb[2][p] += cg_at(g, cg, 8, i, j, k) + cg_at(g, cg, 9, i - 1, j, k) + cg_at(g, cg, 10, i - 1, j - 1, k) +
+                cg_at(g, cg, 11, i, j - 1, k);
+  // B faces are (B_tm1 + B)/2: half of the entry to each array
+  double bx = cg_at(g, cg, 12, i, j, k) + cg_at(g, cg, 13, i - 1, j, k);
+  double by = cg_at(g, cg, 14, i, j, k) + cg_at(g, cg, 15, i, j - 1, k);
+  double bz = cg_at(g, cg, 16, i, j, k) + cg_at(g, cg, 17, i, j, k - 1);
+  a.Bb[0][p] += bx / 2;
+  a.Btb[0][p] += bx / 2;
+  a.Bb[1][p] += by / 2;
+  a.Btb[1][p] += by / 2;
+  a.Bb[2][p] += bz / 2;
+  a.Btb[2][p] += bz / 2;
+}
+
+__global__ void k_zero(double *p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+template <typename T>
+int dalloc(emo_ctx *c, T **p, size_t n) {
+  *p = nullptr;
+  if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    c->err = "hipMalloc failed in the adjoint";
+    return EMO_ERR_ALLOC;
+  }
+  return EMO_OK;
+}
+
+void dfree(void *p) {
+  if (p) hipFree(p);
+}
+
+// One taped state: the particles as the push of a step reads them (the
+// state right after the previous gather), the fields at that time, the cell
+// ranges of the particle order, and the sort that follows.
+struct Snap {
+  double *x[3] = {}, *v[3] = {}, *gam = nullptr, *fld[6] = {};
+  int32_t *sp = nullptr;
+  int64_t *id = nullptr;     // checkpoints only
+  double *E[3] = {}, *A[2][3] = {};  // A[0] = Bfix, A[1] = Btfix
+  int32_t *beg = nullptr, *end = nullptr;  // cell ranges of this order
+  uint32_t *perm = nullptr;  // the following sort: new position j <- old index perm[j]
+  bool b_is_a0 = true;       // member B is the Bfix allocation
+  double t = 0.0;
+};
+
+int snap_alloc(emo_ctx *c, Snap &s, bool ids, bool perm) {
+  const int64_t n = std::max<int64_t>(c->np, 1);
+  const size_t nt = (size_t)c->geo.ntot;
+  int rc = 0;
+  for (int d = 0; d < 3 && !rc; d++) {
+    rc = dalloc(c, &s.x[d], n);
+    if (!rc) rc = dalloc(c, &s.v[d], n);
+    if (!rc) rc = dalloc(c, &s.E[d], nt);
+    if (!rc) rc = dalloc(c, &s.A[0][d], nt);
+    if (!rc) rc = dalloc(c, &s.A[1][d], nt);
+  }
+  if (!rc) rc = dalloc(c, &s.gam, n);
+  for (int k = 0; k < 6 && !rc; k++) rc = dalloc(c, &s.fld[k], n);
+  if (!rc) rc = dalloc(c, &s.sp, n);
+  if (!rc && ids) rc = dalloc(c, &s.id, n);
+  if (!rc) rc = dalloc(c, &s.beg, c->geo.ncell);
+  if (!rc) rc = dalloc(c, &s.end, c->geo.ncell);
+  if (!rc && perm) rc = dalloc(c, &s.perm, n);
+  return rc;
+}
+
+void snap_free(Snap &s) {
+  for (int d = 0; d < 3; d++) {
+    dfree(s.x[d]);
+    dfree(s.v[d]);
+    dfree(s.E[d]);
+    dfree(s.A[0][d]);
+    dfree(s.A[1][d]);
+  }
+  dfree(s.gam);
+  for (int k = 0; k < 6; k++) dfree(s.fld[k]);
+  dfree(s.sp);
+  dfree(s.id);
+  dfree(s.beg);
+  dfree(s.end);
+  dfree(s.perm);
+  s = Snap();
+}
+
+#define ADJ_CPY(dst, src, bytes) \
+  EMO_HIP_CHECK_RET(c, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToDevice, c->stream))
+
+// ctx -> snapshot (particles, fields, ranges)
+int snap_take(emo_ctx *c, Snap &s, double t) {
+  const size_t n8 = (size_t)c->np * 8, n4 = (size_t)c->np * 4, nt8 = (size_t)c->geo.ntot * 8;
+  for (int d = 0; d < 3; d++) {
+    ADJ_CPY(s.x[d], c->p.x[d], n8);
+    ADJ_CPY(s.v[d], c->p.v[d], n8);
+    ADJ_CPY(s.E[d], c->E[d], nt8);
+    ADJ_CPY(s.A[0][d], c->Bfix[d], nt8);
+    ADJ_CPY(s.A[1][d], c->Btfix[d], nt8);
+  }
+  ADJ_CPY(s.gam, c->p.gam, n8);
+  for (int k = 0; k < 6; k++) ADJ_CPY(s.fld[k], c->p.fld[k], n8);
+  ADJ_CPY(s.sp, c->p.sp, n4);
+  if (s.id) ADJ_CPY(s.id, c->p.id, (size_t)c->np * 8);
+  ADJ_CPY(s.beg, c->cbeg, (size_t)c->geo.ncell * 4);
+  ADJ_CPY(s.end, c->cend, (size_t)c->geo.ncell * 4);
+  s.b_is_a0 = c->B[0] == c->Bfix[0];
+  s.t = t;
+  return EMO_OK;
+}
+
+// checkpoint -> ctx
+int snap_restore(emo_ctx *c, const Snap &s) {
+  const size_t n8 = (size_t)c->np * 8, n4 = (size_t)c->np * 4, nt8 = (size_t)c->geo.ntot * 8;
+  for (int d = 0; d < 3; d++) {
+    ADJ_CPY(c->p.x[d], s.x[d], n8);
+    ADJ_CPY(c->p.v[d], s.v[d], n8);
+    ADJ_CPY(c->E[d], s.E[d], nt8);
+    ADJ_CPY(c->Bfix[d], s.A[0][d], nt8);
+    ADJ_CPY(c->Btfix[d], s.A[1][d], nt8);
+  }
+  ADJ_CPY(c->p.gam, s.gam, n8);
+  for (int k = 0; k < 6; k++) ADJ_CPY(c->p.fld[k], s.fld[k], n8);
+  ADJ_CPY(c->p.sp, s.sp, n4);
+  if (s.id) ADJ_CPY(c->p.id, s.id, (size_t)c->np * 8);
+  ADJ_CPY(c->cbeg, s.beg, (size_t)c->geo.ncell * 4);
+  ADJ_CPY(c->cend, s.end, (size_t)c->geo.ncell * 4);
+  for (int q = 0; q < 3; q++) {
+    c->B[q] = s.b_is_a0 ? c->Bfix[q] : c->Btfix[q];
+    c->Bt[q] = s.b_is_a0 ? c->Btfix[q] : c->Bfix[q];
+  }
+  c->sorted = true;
+  return EMO_OK;
+}
+
+// one forward step of emo_step; the permutation of its sort goes to perm
+int forward_step(emo_ctx *c, double t, double dt, uint32_t *perm) {
+  launch_push_f64(c, dt);
+  launch_particle_bc_f64(c);
+  int rc = sort_f64(c, 0);
+  if (rc) return rc;
+  if (perm) ADJ_CPY(perm, c->perm_alt, (size_t)c->np * 4);
+  launch_deposit_f64(c, 1);
+  launch_fold_f64(c);
+  launch_evolve_f64(c, dt);
+  FieldSet<double> v = f64_view(c);
+  refresh_fields<double>(c, v, t, dt);
+  launch_gather_f64(c);
+  return EMO_OK;
+}
+
+// the LightBC planes of one side, in the forward fill order
+struct FillPlane {
+  int side, i, pass;  // pass 0: E, 1: B
+  int comp;           // field component
+  int dim, idx;
+  double off[3];
+};
+
+int edge_plane_index(const Geo &g, int fid, int dim, bool upper) {
+  if (!upper) return 1;
+  int n_ = g.nc[dim];
+  int comp = fid % 3;
+  bool is_e = fid < 3;
+  if (is_e) return comp == dim ? n_ - 1 - 1 : n_ - 1;
+  return comp == dim ? n_ - 1 : n_ - 1 - 1;
+}
+
+std::vector<FillPlane> fill_planes(const emo_ctx *c) {
+  static const int EpolT[3][3] = {{1, 2, 0}, {2, 0, 1}, {0, 1, 2}};
+  static const int BpolT[3][3] = {{2, 1, 0}, {0, 2, 1}, {1, 0, 2}};
+  std::vector<FillPlane> out;
+  for (int side = 0; side < 6; side++) {
+    if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) continue;
+    int dim = side / 2;
+    bool upper = side & 1;
+    for (int i = 0; i < 3; i++) {
+      for (int pass = 0; pass < 2; pass++) {
+        FillPlane f;
+        f.side = side;
+        f.i = i;
+        f.pass = pass;
+        int fid = pass == 0 ? EpolT[dim][i] : 3 + BpolT[dim][i];
+        f.comp = fid % 3;
+        f.dim = dim;
+        f.idx = edge_plane_index(c->geo, fid, dim, upper);
+        for (int q = 0; q < 3; q++) {
+          if (fid < 3) f.off[q] = q == f.comp ? 0.5 * c->geo.d[q] : 0.0;
+          else f.off[q] = q == f.comp ? 0.0 : 0.5 * c->geo.d[q];
+        }
+        out.push_back(f);
+      }
+    }
+  }
+  return out;
+}
+
+// d(pulse value)/d(amplitude), d/d(phase in radians) of one LightBC plane at
+// time t, for every wave (GaussianPulses, src/utils/evaluateGaussianPulse.cpp:7-30)
+void pulse_derivs(const emo_ctx *c, const FillPlane &f, double t, double dt, std::vector<double> &damp,
+                  std::vector<double> &dpha) {
+  static const int EpolT[3][3] = {{1, 2, 0}, {2, 0, 1}, {0, 1, 2}};
+  const emo_config &cf = c->cfg;
+  const int dim = f.side / 2;
+  const int sign = (f.side & 1) ? 1 : -1;
+  const int sidev = sign * (dim + 1);
+  const int eid = EpolT[dim][f.i];
+  damp.assign(cf.nwaves, 0.0);
+  dpha.assign(cf.nwaves, 0.0);
+  if (eid == dim) return;  // the normal component carries no wave
+  const double bPhase = (c->geo.d[dim] + sign * dt) / 2;
+  const double shift = f.pass == 0 ? 0.0 : bPhase;
+  const double scale = f.pass == 0 ? 1.0 : (double)(sign * (int)std::pow(-1, f.i + 1));
+  for (int w = 0; w < cf.nwaves; w++) {
+    if (cf.inSide[w] != sidev || cf.inPolE[w] - 1 != eid) continue;
+    const double th = cf.omegas[w] * (t + shift) + cf.phases[w] * kPi / 180.0;
+    const double G = std::exp(-std::pow((t - cf.delays[w]) * cf.invWidths[w], 2));
+    damp[w] = scale * std::cos(th) * G;
+    dpha[w] = -scale * cf.peakamps[w] * std::sin(th) * G;
+  }
+}
+
+struct AdjFields {
+  double *Eb[3] = {}, *Ab[2][3] = {}, *Jb[3] = {};
+  double *cg = nullptr;
+  double *sums = nullptr;  // plane sums of one step
+};
+
+int launch_zero(emo_ctx *c, double *p, long long n) {
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, p, n);
+  return EMO_OK;
+}
+
+// gather^T of a gather that ran on state s (positions, fields, ranges, B roles)
+void gather_adj(emo_ctx *c, const Snap &s, PartAdj &pb, AdjFields &af) {
+  const Geo &g = c->geo;
+  GatherAdjArgs a;
+  a.g = g;
+  for (int d = 0; d < 3; d++) {
+    a.s.x[d] = s.x[d];
+    a.s.v[d] = s.v[d];
+    a.b.x[d] = pb.x[d];
+    a.b.v[d] = pb.v[d];
+    a.E[d] = s.E[d];
+    a.B[d] = s.b_is_a0 ? s.A[0][d] : s.A[1][d];
+    a.Bt[d] = s.b_is_a0 ? s.A[1][d] : s.A[0][d];
+  }
+  a.s.gam = s.gam;
+  a.b.gam = pb.gam;
+  for (int k = 0; k < 6; k++) {
+    a.s.fld[k] = s.fld[k];
+    a.b.fld[k] = pb.fld[k];
+  }
+  a.s.sp = s.sp;
+  a.beg = s.beg;
+  a.end = s.end;
+  a.cg = af.cg;
+  {
+    TimedLaunch t(c, "adj_gather_cells");
+    hipLaunchKernelGGL(k_gather_adj_cells, dim3((unsigned)((g.ncell + kWaves - 1) / kWaves)), dim3(kBlock), 0,
+                       c->stream, a);
+  }
+  GatherGridArgs b;
+  b.g = g;
+  b.cg = af.cg;
+  const int mb = s.b_is_a0 ? 0 : 1;
+  for (int d = 0; d < 3; d++) {
+    b.Eb[d] = af.Eb[d];
+    b.Bb[d] = af.Ab[mb][d];
+    b.Btb[d] = af.Ab[1 - mb][d];
+  }
+  TimedLaunch t(c, "adj_gather_grid");
+  hipLaunchKernelGGL(k_gather_adj_grid, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, b);
+}
+
+}  // namespace
+
+// dx0/d pert_eps of the displacement loader, per particle in load order
+int loader_dx_deps(const emo_config *cfg, const emo_loader *ld, std::vector<double> &out);
+
+}  // namespace emo
+
+using namespace emo;
+
+extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, int32_t K,
+                                        double *energy, double *d_amp, double *d_phase, double *xbar0,
+                                        double *vbar0, double *gbar0) {
+  if (!c || nsteps < 0) return EMO_ERR_CONFIG;
+  if (c->cfg.precision != EMO_FP64_PLAIN) {
+    c->err = "the adjoint runs on the FP64 plain form";
+    return EMO_ERR_UNSUPPORTED;
+  }
+  if (K <= 0) K = std::max(nsteps, 1);
+  const Geo &g = c->geo;
+  const int64_t np = c->np;
+  const int nw = c->cfg.nwaves;
+  int rc = 0;
+  // the reference loop sorts before its first push (src/main.cpp:225-226)
+  rc = sort_f64(c, 1);
+  if (rc) return rc;
+  // ---- forward with checkpoints -------------------------------------------
+  const int nseg = (nsteps + K - 1) / K;
+  std::vector<Snap> ck(std::max(nseg, 1));
+  double t = t0;
+  std::vector<double> tstep(nsteps + 1);
+  for (int s = 0; s < nsteps; s++) {
+    if (s % K == 0) {
+      rc = snap_alloc(c, ck[s / K], true, false);
+      if (!rc) rc = snap_take(c, ck[s / K], t);
+      if (rc) break;
+    }
+    tstep[s] = t;
+    rc = forward_step(c, t, dt, nullptr);
+    if (rc) break;
+    t += dt;
+  }
+  tstep[nsteps] = t;
+  if (nsteps == 0) {
+    rc = snap_alloc(c, ck[0], true, false);
+    if (!rc) rc = snap_take(c, ck[0], t);
+  }
+  // objective
+  double F = 0.0;
+  if (!rc) {
+    double out8[8];
+    rc = emo_diag(c, out8);
+    F = out8[4] + out8[5];
+  }
+  // final state (for the last segment's tape and the objective seed)
+  Snap fin;
+  if (!rc) rc = snap_alloc(c, fin, false, false);
+  if (!rc) rc = snap_take(c, fin, t);
+  // ---- adjoint buffers ----------------------------------------------------
+  AdjFields af;
+  const size_t nt = (size_t)g.ntot;
+  for (int d = 0; d < 3 && !rc; d++) {
+    rc = dalloc(c, &af.Eb[d], nt);
+    if (!rc) rc = dalloc(c, &af.Ab[0][d], nt);
+    if (!rc) rc = dalloc(c, &af.Ab[1][d], nt);
+    if (!rc) rc = dalloc(c, &af.Jb[d], nt);
+  }
+  if (!rc) rc = dalloc(c, &af.cg, 18 * (size_t)g.ncell);
+  std::vector<FillPlane> planes = fill_planes(c);
+  const int nplanes = (int)planes.size();
+  double *dsums = nullptr;  // nsteps x nplanes
+  if (!rc) rc = dalloc(c, &dsums, (size_t)std::max(1, nsteps * nplanes));
+  PartAdj pb, pb2;
+  for (int d = 0; d < 3 && !rc; d++) {
+    rc = dalloc(c, &pb.x[d], np);
+    if (!rc) rc = dalloc(c, &pb.v[d], np);
+    if (!rc) rc = dalloc(c, &pb2.x[d], np);
+    if (!rc) rc = dalloc(c, &pb2.v[d], np);
+  }
+  if (!rc) rc = dalloc(c, &pb.gam, np);
+  if (!rc) rc = dalloc(c, &pb2.gam, np);
+  for (int k = 0; k < 6 && !rc; k++) {
+    rc = dalloc(c, &pb.fld[k], np);
+    pb2.fld[k] = nullptr;
+  }
+  std::vector<Snap> tape(K + 1);
+  for (int k = 0; k <= K && !rc; k++) rc = snap_alloc(c, tape[k], false, true);
+  if (!rc) {
+    for (int d = 0; d < 3; d++) {
+      launch_zero(c, af.Eb[d], nt);
+      launch_zero(c, af.Ab[0][d], nt);
+      launch_zero(c, af.Ab[1][d], nt);
+      launch_zero(c, af.Jb[d], nt);
+      launch_zero(c, pb.x[d], np);
+      launch_zero(c, pb.v[d], np);
+    }
+    launch_zero(c, pb.gam, np);
+    for (int k = 0; k < 6; k++) launch_zero(c, pb.fld[k], np);
+    // seed: dF/dE, dF/dBfix at the final state
+    SeedArgs sa;
+    sa.g = g;
+    for (int d = 0; d < 3; d++) {
+      sa.E[d] = c->E[d];
+      sa.B[d] = c->Bfix[d];
+      sa.Eb[d] = af.Eb[d];
+      sa.Bb[d] = af.Ab[0][d];
+    }
+    hipLaunchKernelGGL(k_seed_energy, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, sa);
+  }
+  // ---- backward over segments ---------------------------------------------
+  for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
+    const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
+    rc = snap_restore(c, ck[seg]);
+    if (rc) break;
+    double tt = ck[seg].t;
+    for (int s = s0; s < s1 && !rc; s++) {
+      rc = snap_take(c, tape[s - s0], tt);
+      if (!rc) rc = forward_step(c, tt, dt, tape[s - s0].perm);
+      tt += dt;
+    }
+    if (!rc) rc = snap_take(c, tape[s1 - s0], tt);
+    for (int s = s1 - 1; s >= s0 && !rc; s--) {
+      const Snap &pre = tape[s - s0];      // state the push of step s reads
+      const Snap &post = tape[s + 1 - s0];  // state after step s (its gather)
+      // gather^T of step s's gather (needs the fields' adjoint of push s+1)
+      if (s < nsteps - 1) gather_adj(c, post, pb, af);
+      // refresh^T: sides in reverse order
+      const int mnew = pre.b_is_a0 ? 1 : 0;  // the array the B update writes
+      int pi = nplanes - 1;
+      for (int side = 5; side >= 0; side--) {
+        const int d = side / 2, right = side & 1;
+        if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
+          CopyAdjArgs ca;
+          ca.g = g;
+          for (int q = 0; q < 3; q++) {
+            ca.f[q] = af.Eb[q];
+            ca.f[3 + q] = af.Ab[0][q];
+          }
+          ca.nf = 6;
+          ca.d = d;
+          ca.src = right ? 1 : g.n[d];
+          ca.dst = right ? g.n[d] + 1 : 0;
+          int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
+          hipLaunchKernelGGL(k_plane_copy_adj, dim3(grid_for((long long)g.n[u] * g.n[w])), dim3(kBlock), 0,
+                             c->stream, ca);
+        } else {
+          for (; pi >= 0 && planes[pi].side == side; pi--) {
+            const FillPlane &f = planes[pi];
+            FillAdjArgs fa;
+            fa.g = g;
+            fa.f = f.pass == 0 ? af.Eb[f.comp] : af.Ab[mnew][f.comp];
+            fa.dim = f.dim;
+            fa.idx = f.idx;
+            fa.waist = c->cfg.waist;
+            for (int q = 0; q < 3; q++) {
+              fa.off[q] = f.off[q];
+              fa.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
+            }
+            fa.out = dsums + (size_t)s * nplanes + pi;
+            hipLaunchKernelGGL(k_plane_fill_adj, dim3(1), dim3(kBlock), 0, c->stream, fa);
+          }
+        }
+      }
+      // Yee^T
+      YeeAdjArgs ya;
+      ya.g = g;
+      ya.dt = dt;
+      for (int q = 0; q < 3; q++) {
+        ya.Eb[q] = af.Eb[q];
+        ya.Bn[q] = af.Ab[mnew][q];
+        ya.Bo[q] = af.Ab[1 - mnew][q];
+        ya.Jb[q] = af.Jb[q];
+      }
+      {
+        TimedLaunch tl(c, "adj_yee");
+        hipLaunchKernelGGL(k_yee_b_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, ya);
+        hipLaunchKernelGGL(k_yee_move_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, ya);
+        hipLaunchKernelGGL(k_yee_e_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, ya);
+      }
+      // fold^T: the ghost plane receives the first plane's adjoint
+      for (int d = 2; d >= 0; d--) {
+        if (c->cfg.fields_bound[2 * d] != EMO_BC_PERIODIC) continue;
+        double *f[2] = {d == 0 ? af.Jb[1] : af.Jb[0], d == 2 ? af.Jb[1] : af.Jb[2]};
+        fk_plane_copy<double>(c, f, 2, d, 1, g.n[d] + 1, 1);
+      }
+      // deposit^T on the post-BC (sorted) state
+      DepAdjArgs da;
+      da.g = g;
+      for (int d = 0; d < 3; d++) {
+        da.s.x[d] = post.x[d];
+        da.s.v[d] = post.v[d];
+        da.b.x[d] = pb.x[d];
+        da.b.v[d] = pb.v[d];
+        da.Jb[d] = af.Jb[d];
+      }
+      da.s.sp = post.sp;
+      da.sp = c->species;
+      da.np = np;
+      {
+        TimedLaunch tl(c, "adj_deposit");
+        hipLaunchKernelGGL(k_deposit_adj, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, da);
+      }
+      // sort^T
+      SortAdjArgs so;
+      for (int d = 0; d < 3; d++) {
+        so.in[d] = pb.x[d];
+        so.in[3 + d] = pb.v[d];
+        so.out[d] = pb2.x[d];
+        so.out[3 + d] = pb2.v[d];
+      }
+      so.in[6] = pb.gam;
+      so.out[6] = pb2.gam;
+      so.perm = pre.perm;
+      so.np = np;
+      hipLaunchKernelGGL(k_sort_adj, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, so);
+      for (int d = 0; d < 3; d++) {
+        std::swap(pb.x[d], pb2.x[d]);
+        std::swap(pb.v[d], pb2.v[d]);
+      }
+      std::swap(pb.gam, pb2.gam);
+      // BCs^T + push^T
+      PushAdjArgs pa;
+      for (int d = 0; d < 3; d++) {
+        pa.s.x[d] = pre.x[d];
+        pa.s.v[d] = pre.v[d];
+        pa.b.x[d] = pb.x[d];
+        pa.b.v[d] = pb.v[d];
+        pa.xmin[d] = c->cfg.xyz0[d];
+        pa.xmax[d] = pa.xmin[d] + c->cfg.Lxyz[d];
+      }
+      pa.s.gam = pre.gam;
+      pa.b.gam = pb.gam;
+      for (int k = 0; k < 6; k++) {
+        pa.s.fld[k] = pre.fld[k];
+        pa.b.fld[k] = pb.fld[k];
+      }
+      pa.s.sp = pre.sp;
+      for (int k = 0; k < 6; k++) pa.pbc[k] = c->cfg.parts_bound[k];
+      pa.sp = c->species;
+      pa.dt = dt;
+      pa.rel = c->cfg.relativity;
+      pa.np = np;
+      {
+        TimedLaunch tl(c, "adj_push");
+        hipLaunchKernelGGL(k_push_adj, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, pa);
+      }
+    }
+  }
+  // gather^T of the gather that produced the first push's fields
+  if (!rc && nsteps > 0) gather_adj(c, ck[0], pb, af);
+  // ---- outputs -------------------------------------------------------------
+  if (!rc) {
+    if (energy) *energy = F;
+    std::vector<double> sums((size_t)std::max(1, nsteps * nplanes));
+    EMO_HIP_CHECK_RET(c, hipMemcpyAsync(sums.data(), dsums, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+    std::vector<double> da(nw, 0.0), dp(nw, 0.0), ta, tp;
+    for (int s = 0; s < nsteps; s++)
+      for (int k = 0; k < nplanes; k++) {
+        pulse_derivs(c, planes[k], tstep[s], dt, ta, tp);
+        for (int w = 0; w < nw; w++) {
+          da[w] += sums[(size_t)s * nplanes + k] * ta[w];
+          dp[w] += sums[(size_t)s * nplanes + k] * tp[w];
+        }
+      }
+    for (int w = 0; w < nw; w++) {
+      if (d_amp) d_amp[w] = da[w];
+      if (d_phase) d_phase[w] = dp[w];
+    }
+    // particle adjoints of the initial state, by id
+    if ((xbar0 || vbar0 || gbar0) && np > 0) {
+      std::vector<int64_t> ids(np);
+      std::vector<double> h(np);
+      EMO_HIP_CHECK_RET(c, hipMemcpy(ids.data(), ck[0].id, np * 8, hipMemcpyDeviceToHost));
+      auto put = [&](double *dst, const double *src, int stride, int comp) -> int {
+        EMO_HIP_CHECK_RET(c, hipMemcpy(h.data(), src, np * 8, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < np; i++) {
+          int64_t k = ids[i];
+          if (k >= 0 && k < np) dst[stride * k + comp] = h[i];
+        }
+        return EMO_OK;
+      };
+      for (int d = 0; d < 3 && !rc; d++) {
+        if (xbar0) rc = put(xbar0, pb.x[d], 3, d);
+        if (vbar0 && !rc) rc = put(vbar0, pb.v[d], 3, d);
+      }
+      if (gbar0 && !rc) rc = put(gbar0, pb.gam, 1, 0);
+    }
+  }
+  // ---- restore the final state and free -------------------------------------
+  if (nsteps > 0) {
+    int rc2 = snap_restore(c, fin);
+    if (!rc) rc = rc2;
+  }
+  for (auto &s : ck) snap_free(s);
+  for (auto &s : tape) snap_free(s);
+  snap_free(fin);
+  for (int d = 0; d < 3; d++) {
+    dfree(af.Eb[d]);
+    dfree(af.Ab[0][d]);
+    dfree(af.Ab[1][d]);
+    dfree(af.Jb[d]);
+    dfree(pb.x[d]);
+    dfree(pb.v[d]);
+    dfree(pb2.x[d]);
+    dfree(pb2.v[d]);
+  }
+  dfree(pb.gam);
+  dfree(pb2.gam);
+  for (int k = 0; k < 6; k++) dfree(pb.fld[k]);
+  dfree(af.cg);
+  dfree(dsums);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      c->err = hipGetErrorString(e);
+      rc = EMO_ERR_HIP;
+    }
+  }
+  return rc;
+}
+
+extern "C" int emo_loader_dx_deps(const emo_loader *ld, const emo_config *cfg, double *dx_deps) {
+  if (!ld || !cfg || !dx_deps) return EMO_ERR_CONFIG;
+  std::vector<double> out;
+  int rc = emo::loader_dx_deps(cfg, ld, out);
+  if (rc) return rc;
+  std::memcpy(dx_deps, out.data(), out.size() * sizeof(double));
+  return EMO_OK;
+}

@@ -14,7 +14,8 @@ namespace emo {
 int process_info(const emo_loader *ld, emo_config *cfg, double *dens_out, double *temp_out);
 int load_maxwellian_host(const emo_config *cfg, const emo_loader *ld, std::vector<double> &x,
                          std::vector<double> &v, std::vector<double> &gam,
-                         std::vector<int32_t> &sp, std::vector<int64_t> &id);
+                         std::vector<int32_t> &sp, std::vector<int64_t> &id,
+                         std::vector<double> *dx_deps = nullptr);
 
 TimedLaunch::TimedLaunch(emo_ctx *ctx, const char *nm) : c(ctx), name(nm) {
   if (!c->timing) return;

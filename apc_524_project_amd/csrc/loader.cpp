@@ -126,13 +126,14 @@ int process_info(const emo_loader *ld, emo_config *cfg, double *dens_out, double
 // Generates the initial particle state on the host in the reference's order.
 int load_maxwellian_host(const emo_config *cfg, const emo_loader *ld, std::vector<double> &x,
                          std::vector<double> &v, std::vector<double> &gam,
-                         std::vector<int32_t> &sp, std::vector<int64_t> &id) {
+                         std::vector<int32_t> &sp, std::vector<int64_t> &id, std::vector<double> *dx_deps) {
   emo_config c = *cfg;
   double dens[EMO_NSPEC], temp[EMO_NSPEC];
   int rc = process_info(ld, &c, dens, temp);
   if (rc) return rc;
   int64_t npart = ld->nparticles_tot;
   x.resize(3 * npart);
+  if (dx_deps) dx_deps->assign(npart, 0.0);
   v.resize(3 * npart);
   gam.resize(npart);
   sp.resize(npart);
@@ -159,6 +160,7 @@ int load_maxwellian_host(const emo_config *cfg, const emo_loader *ld, std::vecto
     if (ld->pert_eps[ispec] != 0.0) {
       double k = ld->pert_k[ispec];
       double xr = xp[0] - cfg->xyz0[0];
+      if (dx_deps) (*dx_deps)[ip] = -std::sin(k * xr) / k;
       xr = xr - (ld->pert_eps[ispec] / k) * std::sin(k * xr);
       if (xr < 0) xr += cfg->Lxyz[0];
       if (xr >= cfg->Lxyz[0]) xr -= cfg->Lxyz[0];
@@ -220,4 +222,15 @@ int load_maxwellian_host(const emo_config *cfg, const emo_loader *ld, std::vecto
   return EMO_OK;
 }
 
+}  // namespace emo
+
+namespace emo {
+// d x0 / d pert_eps of every particle (load order) for the displacement
+// loader above: -sin(k x_u) / k along x, from the same random stream
+int loader_dx_deps(const emo_config *cfg, const emo_loader *ld, std::vector<double> &out) {
+  std::vector<double> x, v, gam;
+  std::vector<int32_t> sp;
+  std::vector<int64_t> id;
+  return load_maxwellian_host(cfg, ld, x, v, gam, sp, id, &out);
+}
 }  // namespace emo

@@ -155,6 +155,33 @@ int emo_step(emo_ctx *ctx, int32_t nsteps, double t0, double dt);
  * out8 = {pMx, pMy, pMz, pEn, B_en, E_en, E_tot, total charge} */
 int emo_diag(emo_ctx *ctx, double *out8);
 
+/* ---- adjoint (FP64 plain form) -------------------------------------------
+ * No counterpart in the reference (it has no derivatives); for callers that
+ * differentiate the step, e.g. the torch.autograd.Function in
+ * apc_524_project_amd/adjoint.py.
+ *
+ * Sorts the particles by cell (as SortParticles before the reference loop's
+ * first push, src/main.cpp:225-226), runs nsteps steps of emo_step from time
+ * t0, then the exact discrete adjoint of those steps backward in time,
+ * re-running each K-step segment from a checkpoint.  The objective is the
+ * final field energy of OutputBoxQuantities, F = E_en + B_en
+ * (emo_diag out8[5] + out8[4]).  The stored interpolated fields must be
+ * current (emo_interpolate after the field initialisation).  Outputs (any
+ * may be NULL):
+ *   energy        F
+ *   d_amp[nw]     dF/d peakamps[w]
+ *   d_phase[nw]   dF/d phases[w], per radian
+ *   xbar0[n][3], vbar0[n][3], gbar0[n]   dF/d the initial positions,
+ *                 velocities and stored gamma, in particle-id order
+ * The context is left at the final state. */
+int emo_adjoint_field_energy(emo_ctx *ctx, int32_t nsteps, double t0, double dt, int32_t checkpoint_every,
+                             double *energy, double *d_amp, double *d_phase, double *xbar0, double *vbar0,
+                             double *gbar0);
+/* d x0 / d pert_eps[species] of emo_load_maxwellian's displacement loader for
+ * every particle (load = id order; 0 for unperturbed species), n doubles.
+ * No counterpart in the reference (the loader extension is this library's). */
+int emo_loader_dx_deps(const emo_loader *ld, const emo_config *cfg, double *dx_deps);
+
 /* ---- stream / timing ---------------------------------------------------- */
 int emo_synchronize(emo_ctx *ctx);
 /* Record HIP events around every launch of the step's dominant kernels on
