@@ -1,0 +1,91 @@
+"""torch.autograd.Function over libemopic's discrete adjoint.
+
+    energy = FieldEnergy.apply(amps, phases, eps, spec)
+    energy.backward()   # amps.grad, phases.grad, eps.grad
+
+`energy` is the final field energy E_en + B_en of the reference's history
+diagnostics (src/IO/output.cpp:90-121) after `spec.nsteps` steps of the FP64
+plain form, as a function of the injected pulses' peak amplitudes and phases
+(LightBC, src/grid/lightBC.cpp:165-198; phases in degrees as in the input
+file) and of the per-species density-perturbation amplitudes of the
+displacement loader.  The forward pass runs emo_adjoint_field_energy, which
+does the forward steps and the adjoint sweep (checkpoint and recompute every
+`spec.checkpoint_every` steps) in one call; backward scales the stored
+gradient.  The computation runs on the GPU through the C ABI; there is no
+CPU path.
+"""
+from __future__ import annotations
+
+import copy
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import FP64_PLAIN, Context, EmoConfig, EmoLoader, loader_dx_deps
+
+
+@dataclass
+class RunSpec:
+    cfg: EmoConfig          # geometry, BCs, waves (amplitudes/phases are overridden)
+    loader: EmoLoader       # species and loader (pert_eps is overridden)
+    nsteps: int
+    dt: float
+    t0: float = 0.0
+    checkpoint_every: int = 20
+
+
+def _setup(spec: RunSpec, amps, phases, eps):
+    cfg = copy.deepcopy(spec.cfg)
+    ld = copy.deepcopy(spec.loader)
+    if cfg.precision != FP64_PLAIN:
+        raise ValueError("the adjoint runs on the FP64 plain form")
+    for w in range(int(cfg.nwaves)):
+        cfg.peakamps[w] = float(amps[w])
+        cfg.phases[w] = float(phases[w])
+    for s in range(int(ld.nspecies)):
+        ld.pert_eps[s] = float(eps[s])
+    ctx = Context(cfg)
+    ctx.load_maxwellian(ld)
+    ctx.init_constant()
+    ctx.interpolate()
+    return ctx, cfg, ld
+
+
+def field_energy_and_grad(spec: RunSpec, amps, phases, eps):
+    """(F, dF/damps, dF/dphases [per degree], dF/deps) as numpy arrays."""
+    ctx, cfg, ld = _setup(spec, amps, phases, eps)
+    ids0 = np.arange(int(ld.nparticles_tot))
+    out = ctx.adjoint_field_energy(spec.nsteps, spec.t0, spec.dt, spec.checkpoint_every)
+    nsp = int(ld.nspecies)
+    d_eps = np.zeros(nsp)
+    if any(float(eps[s]) != 0.0 or ld.pert_k[s] != 0.0 for s in range(nsp)):
+        dxde = loader_dx_deps(ld, cfg)
+        p = ctx.particles(fields=False)
+        species_by_id = np.empty(len(ids0), np.int32)
+        species_by_id[p["id"]] = p["species"]
+        contrib = out["xbar0"][:, 0] * dxde
+        for s in range(nsp):
+            d_eps[s] = float(contrib[species_by_id == s].sum())
+    ctx.close()
+    return (out["energy"], np.asarray(out["d_amp"]), np.asarray(out["d_phase"]) * math.pi / 180.0, d_eps)
+
+
+class FieldEnergy(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, amps, phases, eps, spec):
+        F, da, dp, de = field_energy_and_grad(spec, amps.detach().cpu().numpy(), phases.detach().cpu().numpy(),
+                                              eps.detach().cpu().numpy())
+        ctx.save_for_backward(torch.as_tensor(da, dtype=amps.dtype), torch.as_tensor(dp, dtype=phases.dtype),
+                              torch.as_tensor(de, dtype=eps.dtype))
+        return torch.tensor(F, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, g):
+        da, dp, de = ctx.saved_tensors
+        return g * da, g * dp, g * de, None
+
+
+def field_energy(amps: torch.Tensor, phases: torch.Tensor, eps: torch.Tensor, spec: RunSpec) -> torch.Tensor:
+    return FieldEnergy.apply(amps, phases, eps, spec)

@@ -484,7 +484,8 @@ void dfree(void *p) {
 struct Snap {
   double *x[3] = {}, *v[3] = {}, *gam = nullptr, *fld[6] = {};
   int32_t *sp = nullptr;
-  int64_t *id = nullptr;     // checkpoints only
+  int64_t *id = nullptr;     // checkpoints and the final state only
+  double *J[3] = {}, *rho = nullptr;  // the final state only
   double *E[3] = {}, *A[2][3] = {};  // A[0] = Bfix, A[1] = Btfix
   int32_t *beg = nullptr, *end = nullptr;  // cell ranges of this order
   uint32_t *perm = nullptr;  // the following sort: new position j <- old index perm[j]
@@ -492,7 +493,7 @@ struct Snap {
   double t = 0.0;
 };
 
-int snap_alloc(emo_ctx *c, Snap &s, bool ids, bool perm) {
+int snap_alloc(emo_ctx *c, Snap &s, bool ids, bool perm, bool currents = false) {
   const int64_t n = std::max<int64_t>(c->np, 1);
   const size_t nt = (size_t)c->geo.ntot;
   int rc = 0;
@@ -510,6 +511,8 @@ int snap_alloc(emo_ctx *c, Snap &s, bool ids, bool perm) {
   if (!rc) rc = dalloc(c, &s.beg, c->geo.ncell);
   if (!rc) rc = dalloc(c, &s.end, c->geo.ncell);
   if (!rc && perm) rc = dalloc(c, &s.perm, n);
+  for (int d = 0; d < 3 && !rc && currents; d++) rc = dalloc(c, &s.J[d], nt);
+  if (!rc && currents) rc = dalloc(c, &s.rho, nt);
   return rc;
 }
 
@@ -520,7 +523,9 @@ void snap_free(Snap &s) {
     dfree(s.E[d]);
     dfree(s.A[0][d]);
     dfree(s.A[1][d]);
+    dfree(s.J[d]);
   }
+  dfree(s.rho);
   dfree(s.gam);
   for (int k = 0; k < 6; k++) dfree(s.fld[k]);
   dfree(s.sp);
@@ -548,6 +553,8 @@ int snap_take(emo_ctx *c, Snap &s, double t) {
   for (int k = 0; k < 6; k++) ADJ_CPY(s.fld[k], c->p.fld[k], n8);
   ADJ_CPY(s.sp, c->p.sp, n4);
   if (s.id) ADJ_CPY(s.id, c->p.id, (size_t)c->np * 8);
+  for (int d = 0; d < 3 && s.rho; d++) ADJ_CPY(s.J[d], c->J[d], nt8);
+  if (s.rho) ADJ_CPY(s.rho, c->rho, nt8);
   ADJ_CPY(s.beg, c->cbeg, (size_t)c->geo.ncell * 4);
   ADJ_CPY(s.end, c->cend, (size_t)c->geo.ncell * 4);
   s.b_is_a0 = c->B[0] == c->Bfix[0];
@@ -569,6 +576,8 @@ int snap_restore(emo_ctx *c, const Snap &s) {
   for (int k = 0; k < 6; k++) ADJ_CPY(c->p.fld[k], s.fld[k], n8);
   ADJ_CPY(c->p.sp, s.sp, n4);
   if (s.id) ADJ_CPY(c->p.id, s.id, (size_t)c->np * 8);
+  for (int d = 0; d < 3 && s.rho; d++) ADJ_CPY(c->J[d], s.J[d], nt8);
+  if (s.rho) ADJ_CPY(c->rho, s.rho, nt8);
   ADJ_CPY(c->cbeg, s.beg, (size_t)c->geo.ncell * 4);
   ADJ_CPY(c->cend, s.end, (size_t)c->geo.ncell * 4);
   for (int q = 0; q < 3; q++) {
@@ -773,8 +782,8 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     F = out8[4] + out8[5];
   }
   // final state (for the last segment's tape and the objective seed)
-  Snap fin;
-  if (!rc) rc = snap_alloc(c, fin, false, false);
+  Snap fin;  // with ids: the recomputation permutes the store, ids included
+  if (!rc) rc = snap_alloc(c, fin, true, false, true);
   if (!rc) rc = snap_take(c, fin, t);
   // ---- adjoint buffers ----------------------------------------------------
   AdjFields af;

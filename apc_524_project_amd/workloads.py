@@ -140,6 +140,10 @@ def make_workload(name: str, precision: int = FP64_PLAIN, scale: float = 1.0) ->
         ld.temp[1] = ld.temp[0]
         ld.dens_phys = 0.1 * omega * omega / WPE2_PER_DENS
         ld.seed = -3 if name == "c3" else -4
+        if name == "c4":
+            # the density-perturbation amplitude the adjoint differentiates
+            ld.pert_eps[0] = 0.01
+            ld.pert_k[0] = 2 * math.pi / n[0]
         ld.zero_avg = 0
         ld.poisson_init = 0
         steps, poisson, sweeps = (2000 if name == "c3" else 400), False, 0

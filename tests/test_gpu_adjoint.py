@@ -138,3 +138,41 @@ def test_initial_velocity_gradient_matches_central_differences(relativity):
     fd = (run(p.v + h * direction) - run(p.v - h * direction)) / (2 * h)
     assert g != 0
     assert rel_err(g, fd) < 1e-4, (g, fd)
+
+
+def test_autograd_function_returns_the_adjoint_gradient(base):
+    import torch
+
+    from apc_524_project_amd.adjoint import RunSpec, field_energy
+
+    case, dt, adj = base
+    spec = RunSpec(cfg=emo_config(case), loader=emo_loader(case), nsteps=NSTEPS, dt=dt, checkpoint_every=4)
+    amps = torch.tensor([0.3], dtype=torch.float64, requires_grad=True)
+    phases = torch.tensor([30.0], dtype=torch.float64, requires_grad=True)
+    eps = torch.tensor([0.05, 0.0], dtype=torch.float64, requires_grad=True)
+    F = field_energy(amps, phases, eps, spec)
+    assert F.item() == adj["energy"]
+    (2.0 * F).backward()
+    assert amps.grad[0].item() == 2.0 * adj["d_amp"][0]
+    assert phases.grad[0].item() == pytest.approx(2.0 * adj["d_phase"][0] * math.pi / 180.0, rel=1e-14)
+    dxde = loader_dx_deps(emo_loader(case), emo_config(case))
+    g_eps = 2.0 * float(np.dot(adj["xbar0"][:, 0], dxde))
+    assert eps.grad[0].item() == pytest.approx(g_eps, rel=1e-12)
+    assert eps.grad[1].item() == 0.0
+
+
+def test_adjoint_leaves_the_final_state(base):
+    """After the gradient call the context holds the state emo_step reaches
+    (particles with their own ids, fields), bit for bit."""
+    case, dt, _ = base
+    a = setup(case)
+    a.adjoint_field_energy(NSTEPS, 0.0, dt, 3)
+    b = setup(case)
+    b.sort()
+    b.step(NSTEPS, 0.0, dt)
+    pa, pb = a.particles(), b.particles()
+    oa, ob = np.argsort(pa["id"]), np.argsort(pb["id"])
+    for k in ("x", "v", "gamma", "species", "field"):
+        assert np.array_equal(pa[k][oa], pb[k][ob]), k
+    for f in range(13):
+        assert np.array_equal(a.field(f), b.field(f)), f
