@@ -374,7 +374,7 @@ struct GatherAdjArgs {
   PartView s;  // positions at gather time, sorted order
   PartAdj b;   // x += ; reads fld
   const double *E[3], *B[3], *Bt[3];
-  const int32_t *beg, *end;
+  const int32_t *off;  // cell c holds off[c]..off[c+1]
   double *cg;  // 18 per cell, component-major
 };
 
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_gather_adj_cells(GatherAdjArgs a) {
   const long long c = (long long)blockIdx.x * kWaves + w;
   if (c >= g.ncell) return;
   double *row = rows[w];
-  const int b = a.beg[c], e = a.end[c];
+  const int b = a.off[c], e = a.off[c + 1];
   double acc = 0.0;
   for (int base = b; base < e; base += 64) {
     const int p = base + lane;
@@ -487,7 +487,7 @@ struct Snap {
   int64_t *id = nullptr;     // checkpoints and the final state only
   double *J[3] = {}, *rho = nullptr;  // the final state only
   double *E[3] = {}, *A[2][3] = {};  // A[0] = Bfix, A[1] = Btfix
-  int32_t *beg = nullptr, *end = nullptr;  // cell ranges of this order
+  int32_t *off = nullptr;    // cell ranges of this order (ncell + 1)
   uint32_t *perm = nullptr;  // the following sort: new position j <- old index perm[j]
   bool b_is_a0 = true;       // member B is the Bfix allocation
   double t = 0.0;
@@ -505,11 +505,10 @@ int snap_alloc(emo_ctx *c, Snap &s, bool ids, bool perm, bool currents = false) 
     if (!rc) rc = dalloc(c, &s.A[1][d], nt);
   }
   if (!rc) rc = dalloc(c, &s.gam, n);
-  for (int k = 0; k < 6 && !rc; k++) rc = dalloc(c, &s.fld[k], n);
+  for (int k = 0; k < 6 && !rc && perm; k++) rc = dalloc(c, &s.fld[k], n);
   if (!rc) rc = dalloc(c, &s.sp, n);
   if (!rc && ids) rc = dalloc(c, &s.id, n);
-  if (!rc) rc = dalloc(c, &s.beg, c->geo.ncell);
-  if (!rc) rc = dalloc(c, &s.end, c->geo.ncell);
+  if (!rc) rc = dalloc(c, &s.off, c->geo.ncell + 1);
   if (!rc && perm) rc = dalloc(c, &s.perm, n);
   for (int d = 0; d < 3 && !rc && currents; d++) rc = dalloc(c, &s.J[d], nt);
   if (!rc && currents) rc = dalloc(c, &s.rho, nt);
@@ -530,8 +529,7 @@ void snap_free(Snap &s) {
   for (int k = 0; k < 6; k++) dfree(s.fld[k]);
   dfree(s.sp);
   dfree(s.id);
-  dfree(s.beg);
-  dfree(s.end);
+  dfree(s.off);
   dfree(s.perm);
   s = Snap();
 }
@@ -550,13 +548,12 @@ int snap_take(emo_ctx *c, Snap &s, double t) {
     ADJ_CPY(s.A[1][d], c->Btfix[d], nt8);
   }
   ADJ_CPY(s.gam, c->p.gam, n8);
-  for (int k = 0; k < 6; k++) ADJ_CPY(s.fld[k], c->p.fld[k], n8);
+  for (int k = 0; k < 6 && s.fld[k]; k++) ADJ_CPY(s.fld[k], c->p.fld[k], n8);
   ADJ_CPY(s.sp, c->p.sp, n4);
   if (s.id) ADJ_CPY(s.id, c->p.id, (size_t)c->np * 8);
   for (int d = 0; d < 3 && s.rho; d++) ADJ_CPY(s.J[d], c->J[d], nt8);
   if (s.rho) ADJ_CPY(s.rho, c->rho, nt8);
-  ADJ_CPY(s.beg, c->cbeg, (size_t)c->geo.ncell * 4);
-  ADJ_CPY(s.end, c->cend, (size_t)c->geo.ncell * 4);
+  ADJ_CPY(s.off, c->srt.off, (size_t)(c->geo.ncell + 1) * 4);
   s.b_is_a0 = c->B[0] == c->Bfix[0];
   s.t = t;
   return EMO_OK;
@@ -573,34 +570,48 @@ int snap_restore(emo_ctx *c, const Snap &s) {
     ADJ_CPY(c->Btfix[d], s.A[1][d], nt8);
   }
   ADJ_CPY(c->p.gam, s.gam, n8);
-  for (int k = 0; k < 6; k++) ADJ_CPY(c->p.fld[k], s.fld[k], n8);
   ADJ_CPY(c->p.sp, s.sp, n4);
   if (s.id) ADJ_CPY(c->p.id, s.id, (size_t)c->np * 8);
   for (int d = 0; d < 3 && s.rho; d++) ADJ_CPY(c->J[d], s.J[d], nt8);
   if (s.rho) ADJ_CPY(c->rho, s.rho, nt8);
-  ADJ_CPY(c->cbeg, s.beg, (size_t)c->geo.ncell * 4);
-  ADJ_CPY(c->cend, s.end, (size_t)c->geo.ncell * 4);
+  ADJ_CPY(c->srt.off, s.off, (size_t)(c->geo.ncell + 1) * 4);
   for (int q = 0; q < 3; q++) {
     c->B[q] = s.b_is_a0 ? c->Bfix[q] : c->Btfix[q];
     c->Bt[q] = s.b_is_a0 ? c->Btfix[q] : c->Bfix[q];
   }
   c->sorted = true;
+  c->fld_state = kFldDeferred;  // the particles' fields: the gather of this grid
   return EMO_OK;
 }
 
-// one forward step of emo_step; the permutation of its sort goes to perm
-int forward_step(emo_ctx *c, double t, double dt, uint32_t *perm) {
-  launch_push_f64(c, dt);
-  launch_particle_bc_f64(c);
-  int rc = sort_f64(c, 0);
-  if (rc) return rc;
-  if (perm) ADJ_CPY(perm, c->perm_alt, (size_t)c->np * 4);
+// One forward step of emo_step.  With a tape entry, the state the push reads
+// (with the fields it gathers) goes to the entry before the sort, and the
+// sort's permutation after it.
+int forward_step(emo_ctx *c, double t, double dt, Snap *tape) {
+  int rc;
+  if (c->np > 0) {
+    fused_push_f64(c, dt, tape ? 1 : 0);
+    if (tape) {
+      rc = snap_take(c, *tape, t);
+      if (rc) return rc;
+    }
+    rc = fused_sort_f64(c);
+    if (rc) return rc;
+    if (tape) ADJ_CPY(tape->perm, c->srt.perm, (size_t)c->np * 4);
+  } else {
+    if (tape) {
+      rc = snap_take(c, *tape, t);
+      if (rc) return rc;
+    }
+    rc = sort_f64(c, 0);
+    if (rc) return rc;
+  }
   launch_deposit_f64(c, 1);
   launch_fold_f64(c);
   launch_evolve_f64(c, dt);
   FieldSet<double> v = f64_view(c);
   refresh_fields<double>(c, v, t, dt);
-  launch_gather_f64(c);
+  c->fld_state = kFldDeferred;
   return EMO_OK;
 }
 
@@ -707,8 +718,7 @@ void gather_adj(emo_ctx *c, const Snap &s, PartAdj &pb, AdjFields &af) {
     a.b.fld[k] = pb.fld[k];
   }
   a.s.sp = s.sp;
-  a.beg = s.beg;
-  a.end = s.end;
+  a.off = s.off;
   a.cg = af.cg;
   {
     TimedLaunch t(c, "adj_gather_cells");
@@ -750,8 +760,13 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   const int64_t np = c->np;
   const int nw = c->cfg.nwaves;
   int rc = 0;
+  // the first push's fields enter the derivative as the gather of the grid
+  if (c->fld_state == kFldStale) {
+    c->err = "the adjoint starts from particle fields gathered from the grid (emo_interpolate)";
+    return EMO_ERR_UNSUPPORTED;
+  }
   // the reference loop sorts before its first push (src/main.cpp:225-226)
-  rc = sort_f64(c, 1);
+  rc = sort_f64(c, c->fld_state == kFldFresh ? 1 : 0);
   if (rc) return rc;
   // ---- forward with checkpoints -------------------------------------------
   const int nseg = (nsteps + K - 1) / K;
@@ -843,8 +858,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     if (rc) break;
     double tt = ck[seg].t;
     for (int s = s0; s < s1 && !rc; s++) {
-      rc = snap_take(c, tape[s - s0], tt);
-      if (!rc) rc = forward_step(c, tt, dt, tape[s - s0].perm);
+      rc = forward_step(c, tt, dt, &tape[s - s0]);
       tt += dt;
     }
     if (!rc) rc = snap_take(c, tape[s1 - s0], tt);

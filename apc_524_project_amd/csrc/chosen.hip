@@ -42,6 +42,7 @@
 #include <cstring>
 #include <vector>
 
+#include "cellsort.hpp"
 #include "deposit_grid.hpp"
 #include "emo_internal.hpp"
 
@@ -49,8 +50,6 @@ namespace emo {
 
 constexpr int kSpeciesShift = 26;
 constexpr int32_t kCellMask = (1 << kSpeciesShift) - 1;
-constexpr uint8_t kFarCode = 255;  // moved beyond the one-cell neighbourhood
-constexpr int kFarCap = 1 << 16;   // ordered far-mover list capacity per step
 
 struct ChosenState {
   int64_t cap = 0;
@@ -60,18 +59,7 @@ struct ChosenState {
   int32_t *id = nullptr, *id_alt = nullptr;
   float *g0 = nullptr;  // stored gamma for the first relativistic push (Q3)
   bool g0_active = false;
-  uint8_t *code = nullptr;       // move code per particle
-  uint16_t *rank = nullptr;      // index within its (source cell, code) group
-  uint16_t *nsk = nullptr;       // [code * ncell + source] group sizes, zero between steps
-  uint16_t *gbase = nullptr;     // [code * ncell + source] group offsets in the destination
-  int32_t *count = nullptr;      // ncell + 1 particles per destination cell
-  int32_t *off = nullptr, *off_new = nullptr;  // ncell + 1 cell ranges
-  int32_t *misc = nullptr;       // [0]: far-mover count
-  int32_t *far_tiles = nullptr;  // far movers per tile
-  int4 *far = nullptr;           // (index, source cell, destination cell, 0)
-  int32_t *far_pos = nullptr;    // far mover's offset in its destination
-  void *scan_tmp = nullptr;
-  size_t scan_tmp_bytes = 0;
+  CellSortBufs srt;  // the counting sort's buffers (cellsort.hpp)
   // initial sort of loaded particles (arbitrary order)
   uint32_t *key = nullptr, *key_alt = nullptr, *perm = nullptr, *perm_alt = nullptr;
   void *sort_tmp = nullptr;
@@ -84,13 +72,6 @@ struct ChosenState {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kFarTile = 4096;
-
-inline unsigned grid_for(long long n, int block = kBlock) {
-  long long g = (n + block - 1) / block;
-  return (unsigned)(g > 0 ? g : 1);
-}
 
 struct SpeciesF32 {
   float qp[EMO_NSPEC];  // ((0.5 dt q)/m) UNIT_ACC
@@ -110,55 +91,6 @@ SpeciesF32 species_f32(const emo_ctx *c, double dt) {
     s.test[k] = c->species.test[k];
   }
   return s;
-}
-
-__device__ inline void decode_cell(const Geo &g, int cell, int *ix, int *iy, int *iz) {
-  *iz = cell % g.nc[2];
-  int t = cell / g.nc[2];
-  *iy = t % g.nc[1];
-  *ix = t / g.nc[1];
-}
-
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ inline unsigned long long lanes_below(int lane) {
-  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
-// One-dimensional move between the source index s and destination index t
-// (padded indices): 0, 1, 2 for t - s = -1, 0, +1; with periodic particle
-// BCs 3, 4, 5 for the wraps t - s = n - 1, 1 - n, -n (a particle leaving
-// through the lower / upper face, or from the upper ghost plane where a
-// particle sitting exactly on x = L lives, src/boundaries/b_particles/
-// bc_p_periodic.cpp:44-87).  -1: not a one-cell move.
-__device__ inline int move_code(int sig, int n, int per) {
-  if (sig >= -1 && sig <= 1) return sig + 1;
-  if (per) {
-    // a wrap whose shift equals a one-cell move (n <= 2) takes that code
-    if (n >= 3 && sig == n - 1) return 3;
-    if (n >= 3 && sig == 1 - n) return 4;
-    if (n >= 2 && sig == -n) return 5;
-  }
-  return -1;
-}
-// whether move code k can occur in a dimension of n cells
-__device__ inline bool move_live(int k, int n, int per) {
-  if (k < 3) return true;
-  return per && (k == 5 ? n >= 2 : n >= 3);
-}
-__device__ inline int move_sigma(int k, int n) {
-  switch (k) {
-    case 0: return -1;
-    case 1: return 0;
-    case 2: return 1;
-    case 3: return n - 1;
-    case 4: return 1 - n;
-    default: return -n;
-  }
 }
 
 // The particle BCs of the six sides in the reference's order
@@ -336,65 +268,6 @@ __global__ void __launch_bounds__(kBlock) k_pgd_f32(PgdArgs a) {
 // particle order (= ascending source cell), built in two passes that return
 // at once when there are none.
 // ---------------------------------------------------------------------------
-__device__ inline int block_sum_256(int v, int *sh) {
-  // sh: 4 ints of LDS
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if (lane == 0) sh[wv] = v;
-  __syncthreads();
-  int t = sh[0] + sh[1] + sh[2] + sh[3];
-  __syncthreads();
-  return t;
-}
-
-__global__ void __launch_bounds__(kBlock) k_far_count(const uint8_t *code, long long np, const int32_t *nfar,
-                                                      int32_t *tiles) {
-  __shared__ int sh[4];
-  if (*nfar == 0) return;
-  const long long b = (long long)blockIdx.x * kFarTile;
-  int n = 0;
-  for (int r = threadIdx.x; r < kFarTile; r += kBlock) {
-    long long i = b + r;
-    n += (i < np && code[i] == kFarCode) ? 1 : 0;
-  }
-  n = block_sum_256(n, sh);
-  if (threadIdx.x == 0) tiles[blockIdx.x] = n;
-}
-
-__global__ void __launch_bounds__(kBlock) k_far_compact(const uint8_t *code, const int32_t *cs_src,
-                                                        const int32_t *cs_dst, long long np, const int32_t *nfar,
-                                                        const int32_t *tiles, int4 *far, int far_cap,
-                                                        unsigned long long *flag) {
-  __shared__ int sh[4];
-  __shared__ int wsum[4];
-  if (*nfar == 0) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int base = 0;
-  for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) base += tiles[t];
-  base = block_sum_256(base, sh);
-  const long long b = (long long)blockIdx.x * kFarTile;
-  for (int r0 = 0; r0 < kFarTile; r0 += kBlock) {
-    long long i = b + r0 + threadIdx.x;
-    bool m = i < np && code[i] == kFarCode;
-    unsigned long long bal = __ballot(m);
-    if (lane == 0) wsum[wv] = __popcll(bal);
-    __syncthreads();
-    int wbase = 0;
-    for (int q = 0; q < wv; q++) wbase += wsum[q];
-    int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    if (m) {
-      int pos = base + wbase + __popcll(bal & lanes_below(lane));
-      if (pos < far_cap)
-        far[pos] = make_int4((int)i, cs_src[i] & kCellMask, cs_dst[i] & kCellMask, 0);
-      else
-        atomicOr(flag, kFlagFarMovers);
-    }
-    base += total;
-    __syncthreads();
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Migration + deposit: one wavefront per destination cell.
 // ---------------------------------------------------------------------------
@@ -425,43 +298,6 @@ __device__ inline void deposit_j(float hi[12], float lo[12], const float f[3], c
   }
 }
 
-// One-dimensional source candidates of destination index t: the move codes
-// in the order 3, 2, 1, 0, 4, 5, whose sources t - sigma ascend; a code is a
-// candidate if it can occur and its source index lies in [1, n+1].  At most
-// four are ever valid, so a cell has at most 64 (source cell, code) pairs.
-struct Cand1 {
-  int src[6];
-  bool ok[6];
-  int n;
-};
-__device__ inline void cand1(int t, int n, int per, Cand1 &o) {
-  o.n = 0;
-#pragma unroll
-  for (int q = 0; q < 6; q++) {
-    const int k = q == 0 ? 3 : q == 1 ? 2 : q == 2 ? 1 : q == 3 ? 0 : q == 4 ? 4 : 5;
-    const int s = t - move_sigma(k, n);
-    o.ok[q] = move_live(k, n, per) && s >= 1 && s <= n + 1;
-    o.src[q] = s;
-    o.n += o.ok[q] ? 1 : 0;
-  }
-}
-// the a-th valid candidate (source index, code)
-__device__ inline void cand1_pick(const Cand1 &o, int a, int &src, int &k) {
-  int cnt = 0;
-  src = 0;
-  k = 0;
-#pragma unroll
-  for (int q = 0; q < 6; q++) {
-    if (o.ok[q]) {
-      if (cnt == a) {
-        src = o.src[q];
-        k = q == 0 ? 3 : q == 1 ? 2 : q == 2 ? 1 : q == 3 ? 0 : q == 4 ? 4 : 5;
-      }
-      cnt++;
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Stable counting sort by destination cell, in four streaming passes.
 // Within a destination cell the order is: source cells in ascending id, and
@@ -472,150 +308,12 @@ __device__ inline void cand1_pick(const Cand1 &o, int a, int &src, int &k) {
 // group and base the number of particles that reach dst from (source, code)
 // groups of lower source id.
 // ---------------------------------------------------------------------------
-constexpr int kCodes = 216;
-constexpr int kRankCells = 8;   // source cells per wavefront in the rank pass
-constexpr int kRankUnroll = 8;  // 64-particle chunks loaded before use
-constexpr int kMaxGroup = 65535;
-
-struct RankArgs {
-  const uint8_t *code;
-  const int32_t *off;  // ranges of the (sorted) source cells
-  long long ncell;
-  uint16_t *rank;
-  uint16_t *nsk;  // [code * ncell + source]: particles per group
-  unsigned long long *flag;
-};
-
-__global__ void __launch_bounds__(kBlock) k_rank_f32(RankArgs a) {
-  __shared__ int cnt[kBlock / 64][kRankCells * kCodes];
-  __shared__ int bnd[kBlock / 64][kRankCells + 1];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long long s0 = ((long long)blockIdx.x * (kBlock / 64) + wv) * kRankCells;
-  if (s0 >= a.ncell) return;
-  const int nc = (int)min((long long)kRankCells, a.ncell - s0);
-  int *ct = cnt[wv];
-  for (int e = lane; e < nc * kCodes; e += 64) ct[e] = 0;
-  if (lane <= nc) bnd[wv][lane] = a.off[s0 + lane];
-  wave_sync();
-  const int pb = bnd[wv][0], pe = bnd[wv][nc];
-  for (int p0 = pb; p0 < pe; p0 += 64 * kRankUnroll) {
-    uint8_t cc[kRankUnroll];
-#pragma unroll
-    for (int u = 0; u < kRankUnroll; u++) {
-      const int p = p0 + u * 64 + lane;
-      cc[u] = p < pe ? a.code[p] : kFarCode;
-    }
-#pragma unroll
-    for (int u = 0; u < kRankUnroll; u++) {
-      const int p = p0 + u * 64 + lane;
-      if (p0 + u * 64 >= pe) break;  // uniform
-      const bool valid = p < pe && cc[u] != kFarCode;
-      int j = 0;
-      for (int t = 1; t < nc; t++) j += bnd[wv][t] <= p ? 1 : 0;
-      const int key = j * kCodes + cc[u];
-      unsigned long long rem = __ballot(valid);
-      int myrank = 0;
-      while (rem) {
-        const int leader = __ffsll((long long)rem) - 1;
-        const int kl = __shfl(key, leader, 64);
-        const bool mine = valid && key == kl;
-        const unsigned long long bal = __ballot(mine);
-        const int base = ct[kl];
-        if (mine) myrank = base + __popcll(bal & lanes_below(lane));
-        wave_sync();
-        if (lane == 0) ct[kl] = base + __popcll(bal);
-        wave_sync();
-        rem &= ~bal;
-      }
-      if (valid) {
-        if (myrank > kMaxGroup) atomicOr(a.flag, kFlagMigrate);
-        a.rank[p] = (uint16_t)myrank;
-      }
-    }
-  }
-  wave_sync();
-  for (int e = lane; e < nc * kCodes; e += 64) {
-    const int v = ct[e];
-    if (v) a.nsk[(long long)(e % kCodes) * a.ncell + s0 + e / kCodes] = (uint16_t)min(v, kMaxGroup);
-  }
-}
-
-struct BaseArgs {
-  Geo g;
-  uint16_t *nsk;
-  uint16_t *base;  // [code * ncell + source]
-  int32_t *count;  // particles per destination cell
-  int per[3];
-  const int32_t *nfar;
-  const int4 *far;
-  int far_cap;
-  int32_t *far_pos;
-  unsigned long long *flag;
-};
-
-// One thread per destination cell: its (source, code) groups in ascending
-// source id, far movers merged in by source id.
-__global__ void __launch_bounds__(kBlock) k_base_f32(BaseArgs a) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const Geo &g = a.g;
-  if (c >= g.ncell) return;
-  int cx, cy, cz;
-  decode_cell(g, (int)c, &cx, &cy, &cz);
-  Cand1 X, Y, Z;
-  cand1(cx, g.n[0], a.per[0], X);
-  cand1(cy, g.n[1], a.per[1], Y);
-  cand1(cz, g.n[2], a.per[2], Z);
-  const int nf = min(*a.nfar, a.far_cap);
-  int fp = 0;
-  int run = 0;
-#pragma unroll
-  for (int ox = 0; ox < 6; ox++) {
-    if (!X.ok[ox]) continue;
-#pragma unroll
-    for (int oy = 0; oy < 6; oy++) {
-      if (!Y.ok[oy]) continue;
-#pragma unroll
-      for (int oz = 0; oz < 6; oz++) {
-        if (!Z.ok[oz]) continue;
-        const int kx = ox == 0 ? 3 : ox == 1 ? 2 : ox == 2 ? 1 : ox == 3 ? 0 : ox == 4 ? 4 : 5;
-        const int ky = oy == 0 ? 3 : oy == 1 ? 2 : oy == 2 ? 1 : oy == 3 ? 0 : oy == 4 ? 4 : 5;
-        const int kz = oz == 0 ? 3 : oz == 1 ? 2 : oz == 2 ? 1 : oz == 3 ? 0 : oz == 4 ? 4 : 5;
-        const int s = (X.src[ox] * g.nc[1] + Y.src[oy]) * g.nc[2] + Z.src[oz];
-        const long long idx = (long long)((kx * 6 + ky) * 6 + kz) * g.ncell + s;
-        while (fp < nf) {
-          const int4 e = a.far[fp];
-          if (e.y >= s) break;
-          if (e.z == (int)c) a.far_pos[fp] = run++;
-          fp++;
-        }
-        const int n = a.nsk[idx];
-        if (n) {
-          a.nsk[idx] = 0;
-          a.base[idx] = (uint16_t)min(run, kMaxGroup);
-          run += n;
-        }
-      }
-    }
-  }
-  for (; fp < nf; fp++)
-    if (a.far[fp].z == (int)c) a.far_pos[fp] = run++;
-  if (run > kMaxGroup) atomicOr(a.flag, kFlagMigrate);
-  a.count[c] = run;
-}
-
 struct ScatterArgs {
   Geo g;
+  CellSortBufs b;
   const int32_t *cs_in;
   const float *f_in[3], *u_in[3];
   const int32_t *id_in;
-  const uint8_t *code;
-  const uint16_t *rank;
-  const uint16_t *base;
-  const int32_t *off_new;
-  const int32_t *nfar;
-  const int4 *far;
-  int far_cap;
-  const int32_t *far_pos;
   int32_t *cs_out;
   float *f_out[3], *u_out[3];
   int32_t *id_out;
@@ -626,41 +324,9 @@ struct ScatterArgs {
 __global__ void __launch_bounds__(kBlock) k_scatter_f32(ScatterArgs a) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
-  const Geo &g = a.g;
   const int w = a.cs_in[i];
-  const int dst = w & kCellMask;
-  const int k = a.code[i];
-  int within;
-  if (k != kFarCode) {
-    int dx, dy, dz;
-    decode_cell(g, dst, &dx, &dy, &dz);
-    const int kx = k / 36, ky = (k / 6) % 6, kz = k % 6;
-    const int s = ((dx - move_sigma(kx, g.n[0])) * g.nc[1] + (dy - move_sigma(ky, g.n[1]))) * g.nc[2] +
-                  (dz - move_sigma(kz, g.n[2]));
-    if (s < 0 || s >= g.ncell) {
-      atomicOr(a.flag, kFlagMigrate);
-      return;
-    }
-    within = (int)a.base[(long long)k * g.ncell + s] + (int)a.rank[i];
-  } else {
-    // far mover: its entry in the ordered list (ascending particle index)
-    int lo = 0, hi = min(*a.nfar, a.far_cap);
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (a.far[mid].x < i) lo = mid + 1;
-      else hi = mid;
-    }
-    if (lo >= min(*a.nfar, a.far_cap) || a.far[lo].x != i) {
-      atomicOr(a.flag, kFlagFarMovers);
-      return;
-    }
-    within = a.far_pos[lo];
-  }
-  const long long pos = (long long)a.off_new[dst] + within;
-  if (pos >= a.off_new[dst + 1]) {
-    atomicOr(a.flag, kFlagMigrate);
-    return;
-  }
+  const long long pos = cellsort_pos(a.g, a.b, i, a.b.code[i], w & kCellMask, a.flag);
+  if (pos < 0) return;
   a.cs_out[pos] = w;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -669,6 +335,15 @@ __global__ void __launch_bounds__(kBlock) k_scatter_f32(ScatterArgs a) {
   }
   if (a.id_in) a.id_out[pos] = a.id_in[i];
 }
+
+// source and destination cells of an FP32 particle (far-mover list)
+struct CellsF32 {
+  const int32_t *src, *dst;
+  __device__ void operator()(long long i, int &s, int &d) const {
+    s = src[i] & kCellMask;
+    d = dst[i] & kCellMask;
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Deposit: 16 lanes per cell, 4 cells per wavefront.  Each lane sums its
@@ -778,12 +453,6 @@ __global__ void __launch_bounds__(kBlock) k_permute_f32(PermF32Args a) {
   }
 }
 
-__global__ void k_fill_i32(int32_t *p, int32_t v, long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-// FP64 reconstruction of positions and velocities (global-position output)
 struct OutArgs {
   Geo g;
   const int32_t *cs;
@@ -889,26 +558,11 @@ int check_flag(emo_ctx *c) {
   return EMO_OK;
 }
 
-void fill_i32(emo_ctx *c, int32_t *p, int32_t v, long long n) {
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, p, v, n);
-}
-
-// exclusive scan of the arrival counts into the new cell ranges
-int scan_counts(emo_ctx *c, int32_t *out) {
-  ChosenState *s = c->ch;
-  size_t bytes = s->scan_tmp_bytes;
-  TimedLaunch t(c, "scan_f32");
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(s->scan_tmp, bytes, s->count, out, (int)(c->geo.ncell + 1),
-                                                  c->stream);
-  if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, std::string("scan: ") + hipGetErrorString(e));
-  return EMO_OK;
-}
-
 // Sorts freshly loaded particles (main arrays, any order) by cell, stably.
 int sort_loaded(emo_ctx *c) {
   ChosenState *s = c->ch;
   const long long np = c->np;
-  fill_i32(c, s->count, 0, c->geo.ncell + 1);
+  cs_fill_i32(c, s->srt.count, 0, c->geo.ncell + 1);
   if (np > 0) {
     hipLaunchKernelGGL(k_keys_from_cs, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, s->cs, s->key, s->perm,
                        np);
@@ -935,7 +589,7 @@ int sort_loaded(emo_ctx *c) {
     pa.g0 = s->g0_active ? s->g0 : nullptr;
     pa.g0_out = g0_new;
     pa.perm = s->perm_alt;
-    pa.count = s->count;
+    pa.count = s->srt.count;
     pa.np = np;
     hipLaunchKernelGGL(k_permute_f32, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, pa);
     std::swap(s->cs, s->cs_alt);
@@ -950,9 +604,9 @@ int sort_loaded(emo_ctx *c) {
       s->g0 = g0_new;
     }
   }
-  int rc = scan_counts(c, s->off);
+  int rc = cellsort_scan(c, s->srt, s->srt.off, "scan_f32");
   if (rc) return rc;
-  fill_i32(c, s->count, 0, c->geo.ncell + 1);
+  cs_fill_i32(c, s->srt.count, 0, c->geo.ncell + 1);
   return EMO_OK;
 }
 
@@ -974,8 +628,8 @@ void launch_pgd(emo_ctx *c, double dt) {
     a.per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
   }
   a.g0 = s->g0_active ? s->g0 : nullptr;
-  a.code = s->code;
-  a.nfar = s->misc;
+  a.code = s->srt.code;
+  a.nfar = s->srt.misc;
   a.s = species_f32(c, dt);
   for (int k = 0; k < 6; k++) a.pbc[k] = c->cfg.parts_bound[k];
   a.flag = c->dflag;
@@ -993,50 +647,14 @@ int launch_sort_deposit(emo_ctx *c) {
   ChosenState *s = c->ch;
   const Geo &g = c->geo;
   const long long np = c->np;
-  int per[3];
-  for (int d = 0; d < 3; d++)
-    per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
-  const unsigned ntiles = (unsigned)std::max<long long>((np + kFarTile - 1) / kFarTile, 1);
-  {
-    TimedLaunch t(c, "far_f32");
-    hipLaunchKernelGGL(k_far_count, dim3(ntiles), dim3(kBlock), 0, c->stream, (const uint8_t *)s->code, np,
-                       (const int32_t *)s->misc, s->far_tiles);
-    hipLaunchKernelGGL(k_far_compact, dim3(ntiles), dim3(kBlock), 0, c->stream, (const uint8_t *)s->code,
-                       (const int32_t *)s->cs, (const int32_t *)s->cs_alt, np, (const int32_t *)s->misc,
-                       (const int32_t *)s->far_tiles, s->far, kFarCap, c->dflag);
-  }
-  {
-    RankArgs r;
-    r.code = s->code;
-    r.off = s->off;
-    r.ncell = g.ncell;
-    r.rank = s->rank;
-    r.nsk = s->nsk;
-    r.flag = c->dflag;
-    const long long waves = (g.ncell + kRankCells - 1) / kRankCells;
-    TimedLaunch t(c, "rank_f32");
-    hipLaunchKernelGGL(k_rank_f32, dim3((unsigned)((waves + 3) / 4)), dim3(kBlock), 0, c->stream, r);
-  }
-  {
-    BaseArgs b;
-    b.g = g;
-    b.nsk = s->nsk;
-    b.base = s->gbase;
-    b.count = s->count;
-    for (int d = 0; d < 3; d++) b.per[d] = per[d];
-    b.nfar = s->misc;
-    b.far = s->far;
-    b.far_cap = kFarCap;
-    b.far_pos = s->far_pos;
-    b.flag = c->dflag;
-    TimedLaunch t(c, "base_f32");
-    hipLaunchKernelGGL(k_base_f32, dim3(grid_for(g.ncell)), dim3(kBlock), 0, c->stream, b);
-  }
-  int rc = scan_counts(c, s->off_new);
+  static const char *const names[4] = {"far_f32", "rank_f32", "base_f32", "scan_f32"};
+  CellsF32 cells{s->cs, s->cs_alt};
+  int rc = cellsort_offsets(c, s->srt, np, cells, names);
   if (rc) return rc;
   {
     ScatterArgs a;
     a.g = g;
+    a.b = s->srt;
     a.cs_in = s->cs_alt;
     a.cs_out = s->cs;
     for (int d = 0; d < 3; d++) {
@@ -1047,23 +665,15 @@ int launch_sort_deposit(emo_ctx *c) {
     }
     a.id_in = s->id;
     a.id_out = s->id_alt;
-    a.code = s->code;
-    a.rank = s->rank;
-    a.base = s->gbase;
-    a.off_new = s->off_new;
-    a.nfar = s->misc;
-    a.far = s->far;
-    a.far_cap = kFarCap;
-    a.far_pos = s->far_pos;
     a.flag = c->dflag;
     a.np = np;
     TimedLaunch t(c, "scatter_f32");
     hipLaunchKernelGGL(k_scatter_f32, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, a);
   }
-  std::swap(s->off, s->off_new);
+  std::swap(s->srt.off, s->srt.off_new);
   if (s->id) std::swap(s->id, s->id_alt);
   // the far-mover count starts from zero again
-  fill_i32(c, s->misc, 0, 1);
+  cs_fill_i32(c, s->srt.misc, 0, 1);
   {
     DepF32Args d;
     d.g = g;
@@ -1072,7 +682,7 @@ int launch_sort_deposit(emo_ctx *c) {
       d.f[q] = s->f[q];
       d.u[q] = s->u[q];
     }
-    d.off = s->off;
+    d.off = s->srt.off;
     d.s = species_f32(c, 0.0);
     d.cj = s->cj;
     const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
@@ -1115,17 +725,7 @@ int fp32_create(emo_ctx *c) {
     if (!rc) rc = dalloc(c, &s->u[d], cap);
     if (!rc) rc = dalloc(c, &s->u_alt[d], cap);
   }
-  if (!rc) rc = dalloc(c, &s->code, cap);
-  if (!rc) rc = dalloc(c, &s->rank, cap);
-  if (!rc) rc = dalloc(c, &s->nsk, (size_t)kCodes * g.ncell);
-  if (!rc) rc = dalloc(c, &s->gbase, (size_t)kCodes * g.ncell);
-  if (!rc) rc = dalloc(c, &s->far_pos, kFarCap);
-  if (!rc) rc = dalloc(c, &s->count, g.ncell + 1);
-  if (!rc) rc = dalloc(c, &s->off, g.ncell + 1);
-  if (!rc) rc = dalloc(c, &s->off_new, g.ncell + 1);
-  if (!rc) rc = dalloc(c, &s->misc, 4);
-  if (!rc) rc = dalloc(c, &s->far_tiles, (cap + kFarTile - 1) / kFarTile + 1);
-  if (!rc) rc = dalloc(c, &s->far, kFarCap);
+  if (!rc) rc = cellsort_alloc(c, s->srt, cap, g.ncell);
   if (!rc) rc = dalloc(c, &s->key, cap);
   if (!rc) rc = dalloc(c, &s->key_alt, cap);
   if (!rc) rc = dalloc(c, &s->perm, cap);
@@ -1147,10 +747,6 @@ int fp32_create(emo_ctx *c) {
   float *all[13] = {s->fs.E[0], s->fs.E[1], s->fs.E[2], s->fs.Bfix[0], s->fs.Bfix[1], s->fs.Bfix[2],
                     s->Btfix[0], s->Btfix[1], s->Btfix[2], s->fs.J[0], s->fs.J[1], s->fs.J[2], s->fs.rho};
   for (int q = 0; q < 13; q++) EMO_HIP_CHECK_RET(c, hipMemset(all[q], 0, nt * sizeof(float)));
-  EMO_HIP_CHECK_RET(c, hipMemset(s->count, 0, (g.ncell + 1) * sizeof(int32_t)));
-  EMO_HIP_CHECK_RET(c, hipMemset(s->nsk, 0, (size_t)kCodes * g.ncell * sizeof(uint16_t)));
-  EMO_HIP_CHECK_RET(c, hipMemset(s->off, 0, (g.ncell + 1) * sizeof(int32_t)));
-  EMO_HIP_CHECK_RET(c, hipMemset(s->misc, 0, 4 * sizeof(int32_t)));
   int bits = 1;
   while ((1ll << bits) < g.ncell) bits++;
   s->sort_bits = bits;
@@ -1159,11 +755,6 @@ int fp32_create(emo_ctx *c) {
                                      (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)cap, 0, bits);
   if (hipMalloc(&s->sort_tmp, std::max<size_t>(s->sort_tmp_bytes, 16)) != hipSuccess)
     return set_err(c, EMO_ERR_ALLOC, "sort scratch");
-  s->scan_tmp_bytes = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, s->scan_tmp_bytes, (const int32_t *)nullptr, (int32_t *)nullptr,
-                                   (int)(g.ncell + 1));
-  if (hipMalloc(&s->scan_tmp, std::max<size_t>(s->scan_tmp_bytes, 16)) != hipSuccess)
-    return set_err(c, EMO_ERR_ALLOC, "scan scratch");
   return EMO_OK;
 }
 
@@ -1186,23 +777,12 @@ void fp32_destroy(emo_ctx *c) {
   dfree(s->id);
   dfree(s->id_alt);
   dfree(s->g0);
-  dfree(s->code);
-  dfree(s->rank);
-  dfree(s->nsk);
-  dfree(s->gbase);
-  dfree(s->far_pos);
-  dfree(s->count);
-  dfree(s->off);
-  dfree(s->off_new);
-  dfree(s->misc);
-  dfree(s->far_tiles);
-  dfree(s->far);
+  cellsort_free(s->srt);
   dfree(s->key);
   dfree(s->key_alt);
   dfree(s->perm);
   dfree(s->perm_alt);
   dfree(s->sort_tmp);
-  dfree(s->scan_tmp);
   dfree(s->cj);
   delete s;
   c->ch = nullptr;

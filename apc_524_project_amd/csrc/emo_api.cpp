@@ -210,8 +210,7 @@ int emo_create(const emo_config *cfg, emo_ctx **out) {
     if (!rc) rc = dalloc(c, &c->key_alt, cap);
     if (!rc) rc = dalloc(c, &c->perm, cap);
     if (!rc) rc = dalloc(c, &c->perm_alt, cap);
-    if (!rc) rc = dalloc(c, &c->cbeg, g.ncell);
-    if (!rc) rc = dalloc(c, &c->cend, g.ncell);
+    if (!rc) rc = plain_sort_create(c);
     if (!rc) rc = dalloc(c, &c->cj, 12 * g.ncell);
     if (!rc) rc = dalloc(c, &c->cr, 8 * g.ncell);
     if (!rc) {
@@ -259,8 +258,7 @@ void emo_destroy(emo_ctx *c) {
   dfree(c->perm);
   dfree(c->perm_alt);
   dfree(c->sort_tmp);
-  dfree(c->cbeg);
-  dfree(c->cend);
+  plain_sort_destroy(c);
   dfree(c->cj);
   dfree(c->cr);
   dfree(c->red);
@@ -306,6 +304,7 @@ int emo_load_particles(emo_ctx *c, int64_t n, const double *x3, const double *v3
   for (int d = 0; d < 6; d++) EMO_HIP_CHECK_RET(c, hipMemset(c->p.fld[d], 0, std::max<int64_t>(n, 1) * sizeof(double)));
   c->np = n;
   c->sorted = false;
+  c->fld_state = kFldStale;  // zero until the first gather
   return EMO_OK;
 }
 
@@ -346,6 +345,7 @@ int emo_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t
     return EMO_OK;
   };
   int rc = 0;
+  if (field6) fld_materialize(c);
   if (x3)
     for (int d = 0; d < 3 && !rc; d++) rc = down(x3, c->p.x[d], 3, d);
   if (v3)
@@ -409,6 +409,7 @@ int emo_set_field(emo_ctx *c, int fid, const double *in) {
   }
   double *p = field_ptr(c, fid);
   if (!p) return set_err(c, EMO_ERR_CONFIG, "unknown field id");
+  if (fid < EMO_JX || fid >= EMO_BX_TM1) fld_invalidate(c);  // E, B, B_tm1 feed the gather
   EMO_HIP_CHECK_RET(c, hipMemcpyAsync(p, in, c->geo.ntot * sizeof(double), hipMemcpyHostToDevice, c->stream));
   EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
   return EMO_OK;
@@ -437,6 +438,7 @@ int emo_sort(emo_ctx *c) {
   if (!c) return EMO_ERR_CONFIG;
   int rc = require_plain(c, "emo_sort");
   if (rc) return rc;
+  fld_materialize(c);
   rc = sort_f64(c, 1);
   if (rc) return rc;
   return last_launch(c);
@@ -446,8 +448,10 @@ int emo_push(emo_ctx *c, double dt) {
   if (!c) return EMO_ERR_CONFIG;
   int rc = require_plain(c, "emo_push");
   if (rc) return rc;
+  fld_materialize(c);
   launch_push_f64(c, dt);
   c->sorted = false;
+  c->fld_state = kFldStale;
   return last_launch(c);
 }
 
@@ -455,6 +459,7 @@ int emo_particle_bc(emo_ctx *c) {
   if (!c) return EMO_ERR_CONFIG;
   int rc = require_plain(c, "emo_particle_bc");
   if (rc) return rc;
+  fld_invalidate(c);
   launch_particle_bc_f64(c);
   c->sorted = false;
   return last_launch(c);
@@ -465,6 +470,7 @@ int emo_deposit(emo_ctx *c, int32_t rho) {
   int rc = require_plain(c, "emo_deposit");
   if (rc) return rc;
   if (!c->sorted) {
+    fld_materialize(c);
     rc = sort_f64(c, 1);
     if (rc) return rc;
   }
@@ -484,6 +490,7 @@ int emo_evolve_fields(emo_ctx *c, double dt) {
   if (!c) return EMO_ERR_CONFIG;
   int rc = require_plain(c, "emo_evolve_fields");
   if (rc) return rc;
+  fld_invalidate(c);
   launch_evolve_f64(c, dt);
   return last_launch(c);
 }
@@ -492,6 +499,7 @@ int emo_refresh(emo_ctx *c, double t, double dt) {
   if (!c) return EMO_ERR_CONFIG;
   int rc = require_plain(c, "emo_refresh");
   if (rc) return rc;
+  fld_invalidate(c);
   FieldSet<double> v = f64_view(c);
   refresh_fields<double>(c, v, t, dt);
   return last_launch(c);
@@ -514,6 +522,7 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
   const Geo &g = c->geo;
   size_t nt = (size_t)g.ntot;
   // deposit + fold (src/main.cpp:160-166)
+  fld_invalidate(c);
   int rc = sort_f64(c, 1);
   if (rc) return rc;
   launch_deposit_f64(c, 1);
@@ -622,10 +631,22 @@ int emo_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
   }
   double t = t0;
   for (int s = 0; s < nsteps; s++) {
-    launch_push_f64(c, dt);
-    launch_particle_bc_f64(c);
-    int rc = sort_f64(c, 0);  // the gather at the end of the step refills the fields
-    if (rc) return rc;
+    int rc;
+    if (c->sorted && c->fld_state != kFldStale && c->np > 0) {
+      // fused: the push gathers the fields from the grid, which the stored
+      // per-particle fields equal (fresh) or would equal (deferred); the
+      // store is in cell order, so the counting sort keeps the radix
+      // sort's stable order
+      fused_push_f64(c, dt, 0);
+      rc = fused_sort_f64(c);
+      if (rc) return rc;
+    } else {
+      fld_materialize(c);
+      launch_push_f64(c, dt);
+      launch_particle_bc_f64(c);
+      rc = sort_f64(c, 0);
+      if (rc) return rc;
+    }
     launch_deposit_f64(c, 1);
     launch_fold_f64(c);
     launch_evolve_f64(c, dt);
@@ -633,10 +654,12 @@ int emo_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
       FieldSet<double> v = f64_view(c);
       refresh_fields<double>(c, v, t, dt);
     }
-    launch_gather_f64(c);
+    c->fld_state = kFldDeferred;  // the gather that ends the step
     t += dt;
   }
-  return last_launch(c);
+  int rc = last_launch(c);
+  if (rc) return rc;
+  return check_flags(c);
 }
 
 int emo_diag(emo_ctx *c, double *out8) {

@@ -49,6 +49,27 @@ struct FieldSet {
 
 struct ChosenState;  // FP32 chosen form (chosen.hip)
 
+// Device buffers of the stable counting sort by destination cell
+// (cellsort.hpp): per particle a move code, a rank and the permutation
+// (new position -> old index); per cell the ranges off[ncell+1] and, for
+// each of the 216 move codes, the (source cell, code) group sizes and
+// offsets.
+struct CellSortBufs {
+  uint8_t *code = nullptr;
+  uint16_t *rank = nullptr;
+  uint16_t *nsk = nullptr;    // [code * ncell + source], zero between steps
+  uint16_t *gbase = nullptr;  // [code * ncell + source]
+  int32_t *count = nullptr;   // ncell + 1 particles per destination cell
+  int32_t *off = nullptr, *off_new = nullptr;  // ncell + 1 cell ranges
+  int32_t *misc = nullptr;       // [0]: far-mover count
+  int32_t *far_tiles = nullptr;  // far movers per tile
+  int4 *far = nullptr;           // (index, source cell, destination cell, 0)
+  int32_t *far_pos = nullptr;    // far mover's offset in its destination
+  uint32_t *perm = nullptr;      // new position -> old index
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+};
+
 // FP64 particle store (structure of arrays).
 struct ParticlesF64 {
   double *x[3] = {nullptr, nullptr, nullptr};
@@ -88,10 +109,18 @@ struct emo_ctx {
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   int sort_bits = 0;
-  int32_t *cbeg = nullptr, *cend = nullptr;  // per cell [beg,end) after the sort
-  double *cj = nullptr;                      // 12 per cell
-  double *cr = nullptr;                      // 8 per cell
-  bool sorted = false;
+  emo::CellSortBufs srt;  // counting sort; srt.off[c]..srt.off[c+1] is cell c
+  double *cj = nullptr;   // 12 per cell
+  double *cr = nullptr;   // 8 per cell
+  bool sorted = false;  // store in ascending cell order and srt.off valid
+  // The per-particle fields p.fld against the grid: kFldFresh = they are the
+  // gather of the grid at the particles' positions; kFldStale = the grid or
+  // the positions changed since the last gather (p.fld is what the next push
+  // uses); kFldDeferred = the gather of the grid is due but not stored (the
+  // fused step gathers inside its push and leaves the stored copy to be made
+  // when something reads it).
+  int fld_state = 1;
+  int64_t fallback_sorts = 0;  // fused steps that overflowed the counting sort
 
   // reductions / flags
   double *red = nullptr;   // scratch for FP64 reductions
@@ -129,6 +158,15 @@ void launch_particle_bc_f64(emo_ctx *c);
 // with_fields = 0 leaves the per-particle gathered fields behind (valid
 // only when a gather follows before the next push)
 int sort_f64(emo_ctx *c, int with_fields);
+int plain_sort_create(emo_ctx *c);
+void plain_sort_destroy(emo_ctx *c);
+void fused_push_f64(emo_ctx *c, double dt, int write_fld);
+int fused_sort_f64(emo_ctx *c);
+constexpr int kFldFresh = 0, kFldStale = 1, kFldDeferred = 2;
+// stores the deferred gather (kFldDeferred -> kFldFresh)
+void fld_materialize(emo_ctx *c);
+// before a change of the grid or of the positions: materialize, then stale
+void fld_invalidate(emo_ctx *c);
 void launch_deposit_f64(emo_ctx *c, int rho);
 void launch_cell_ids(emo_ctx *c, int32_t *out);
 size_t sort_tmp_bytes_for(int64_t n, int bits);

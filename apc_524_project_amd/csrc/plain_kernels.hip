@@ -9,6 +9,7 @@
 // fixed-order edge/vertex sums.  No atomics on floating-point data.
 #include <hipcub/hipcub.hpp>
 
+#include "cellsort.hpp"
 #include "deposit_grid.hpp"
 #include "emo_internal.hpp"
 #include "pic_math.hpp"
@@ -16,12 +17,6 @@
 namespace emo {
 namespace {
 
-constexpr int kBlock = 256;
-
-inline unsigned grid_for(long long n, int block = kBlock) {
-  long long g = (n + block - 1) / block;
-  return (unsigned)(g > 0 ? g : 1);
-}
 
 // ---------------------------------------------------------------------------
 // Gather: Grid::getFieldInterpolatorVec (src/grid/oGrid.cpp:153-197) and
@@ -180,13 +175,21 @@ __global__ void __launch_bounds__(kBlock) k_permute(PermuteArgs a) {
   a.dst.id[i] = a.src.id[s];
 }
 
-__global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint32_t *key, int32_t *beg, int32_t *end,
-                                                         long long np) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  uint32_t k = key[i];
-  if (i == 0 || key[i - 1] != k) beg[k] = (int32_t)i;
-  if (i == np - 1 || key[i + 1] != k) end[k] = (int32_t)(i + 1);
+// particles per cell of a sorted key array: one atomic per run of equal keys
+__global__ void __launch_bounds__(kBlock) k_count_sorted(const uint32_t *key, int32_t *count, long long np) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = i < np;
+  const int cell = live ? (int)key[i] : -1;
+  const int prev = __shfl_up(cell, 1, 64);
+  const bool head = live && (lane == 0 || prev != cell);
+  const unsigned long long heads = __ballot(head);
+  const int nlive = __popcll(__ballot(live));
+  if (head) {
+    unsigned long long above = heads & ~(lanes_below(lane) | (1ull << lane));
+    int next = above ? __ffsll((long long)above) - 1 : 64;
+    atomicAdd(&count[cell], min(next, nlive) - lane);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -205,7 +208,7 @@ struct DepCellArgs {
   Geo g;
   const double *x[3], *v[3];
   const int32_t *sp;
-  const int32_t *beg, *end;
+  const int32_t *off;  // cell c holds [off[c], off[c+1])
   SpeciesParams s;
   double *cj, *cr;
   int rho;
@@ -215,11 +218,6 @@ constexpr int kDepWaves = kBlock / 64;
 constexpr int kDepNV = 20;       // 12 J + 8 rho
 constexpr int kDepStride = 21;   // padded row length in LDS
 
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __global__ void __launch_bounds__(kBlock) k_deposit_cells(DepCellArgs a) {
   __shared__ double rows[kDepWaves][64 * kDepStride];
@@ -230,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_deposit_cells(DepCellArgs a) {
   double *row = rows[w];
   for (long long c = (long long)blockIdx.x * kDepWaves + w; c < g.ncell;
        c += (long long)gridDim.x * kDepWaves) {
-    const int b = a.beg[c], e = a.end[c];
+    const int b = a.off[c], e = a.off[c + 1];
     double acc = 0.0;
     if (e > b) {
       int iz = (int)(c % g.nc[2]);
@@ -281,9 +279,139 @@ __global__ void __launch_bounds__(kBlock) k_cell_ids(Geo g, const double *x0, co
   out[i] = cell_of(g, x0[i], x1[i], x2[i], &ix, &iy, &iz);
 }
 
+// ---------------------------------------------------------------------------
+// Fused step of the plain form (emo_step on a cell-sorted store):
+// gather + push + particle BCs + move code in one pass, then the stable
+// counting sort of cellsort.hpp and a scatter.  The per-particle arithmetic
+// is pic_math.hpp's, so the result equals the separate phases bit for bit;
+// the gathered fields are not stored (the next push gathers them again from
+// the same grid).
+// ---------------------------------------------------------------------------
+struct Pgd64Args {
+  Geo g;
+  const double *x[3], *v[3], *gam;
+  const int32_t *sp;
+  double *fld[6];  // written only with write_fld (the adjoint's tape)
+  double *xo[3], *vo[3], *go;
+  const double *E[3], *B[3], *Bt[3];
+  SpeciesParams s;
+  double dt;
+  double xmin[3], xmax[3];
+  int pbc[6];
+  int per[3];
+  uint8_t *code;
+  int32_t *nfar;
+  int write_fld;
+  unsigned long long *flag;
+  long long np;
+};
+
+template <bool REL>
+__global__ void __launch_bounds__(kBlock) k_pgd64(Pgd64Args a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = i < a.np;
+  const Geo &g = a.g;
+  int mcode = 0;
+  if (live) {
+    double pos[3] = {a.x[0][i], a.x[1][i], a.x[2][i]};
+    double vel[3] = {a.v[0][i], a.v[1][i], a.v[2][i]};
+    double gam = a.gam[i];
+    int sx, sy, sz;
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    if (cell_of(g, pos[0], pos[1], pos[2], &sx, &sy, &sz) < 0) {
+      atomicOr(a.flag, kFlagOutOfGrid);
+      sx = sy = sz = 1;
+    } else {
+      double cv[21];
+      cell_vector(g, a.E, a.B, a.Bt, sx, sy, sz, cv);
+      interp_fields(g, cv, pos, f);
+    }
+    if (a.write_fld)
+      for (int q = 0; q < 6; q++) a.fld[q][i] = f[q];
+    const int sp = a.sp[i];
+    if (REL) vay_step(pos, vel, gam, f, a.s.q[sp], a.s.m[sp], a.dt);
+    else boris_step(pos, vel, f, a.s.q[sp], a.s.m[sp], a.dt);
+    int flips[3];
+    particle_bcs(pos, vel, a.xmin, a.xmax, a.pbc, flips);
+    int dx, dy, dz;
+    if (cell_of(g, pos[0], pos[1], pos[2], &dx, &dy, &dz) < 0) {
+      atomicOr(a.flag, kFlagOutOfGrid);
+      mcode = kFarCode;
+    } else {
+      int k0 = move_code(dx - sx, g.n[0], a.per[0]);
+      int k1 = move_code(dy - sy, g.n[1], a.per[1]);
+      int k2 = move_code(dz - sz, g.n[2], a.per[2]);
+      mcode = (k0 < 0 || k1 < 0 || k2 < 0) ? (int)kFarCode : (k0 * 6 + k1) * 6 + k2;
+    }
+    for (int d = 0; d < 3; d++) {
+      a.xo[d][i] = pos[d];
+      a.vo[d][i] = vel[d];
+    }
+    a.go[i] = gam;
+    a.code[i] = (uint8_t)mcode;
+  }
+  const int nf = __popcll(__ballot(live && mcode == kFarCode));
+  if (lane == 0 && nf) atomicAdd(a.nfar, nf);
+}
+
+// a particle's cell from its position, clamped into the grid (an
+// out-of-grid particle has already raised the flag)
+__device__ inline int cell_clamped(const Geo &g, double x, double y, double z) {
+  int ix, iy, iz;
+  int c = cell_of(g, x, y, z, &ix, &iy, &iz);
+  if (c >= 0) return c;
+  ix = min(max(ix, 1), g.n[0]);
+  iy = min(max(iy, 1), g.n[1]);
+  iz = min(max(iz, 1), g.n[2]);
+  return (g.nc[1] * g.nc[2]) * ix + g.nc[2] * iy + iz;
+}
+
+// source (old position) and destination (new position) cells
+struct CellsF64 {
+  Geo g;
+  const double *xs[3], *xd[3];
+  __device__ void operator()(long long i, int &s, int &d) const {
+    s = cell_clamped(g, xs[0][i], xs[1][i], xs[2][i]);
+    d = cell_clamped(g, xd[0][i], xd[1][i], xd[2][i]);
+  }
+};
+
+struct Scatter64Args {
+  Geo g;
+  CellSortBufs b;
+  const double *xi[3], *vi[3], *gi;  // new state, previous order
+  const int32_t *spi;
+  const int64_t *idi;
+  double *xo[3], *vo[3], *go;        // new state, sorted
+  int32_t *spo;
+  int64_t *ido;
+  uint32_t *perm;                    // sorted position -> previous index
+  unsigned long long *flag;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_scatter64(Scatter64Args a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const double x0 = a.xi[0][i], x1 = a.xi[1][i], x2 = a.xi[2][i];
+  const int dst = cell_clamped(a.g, x0, x1, x2);
+  const long long pos = cellsort_pos(a.g, a.b, i, a.b.code[i], dst, a.flag);
+  if (pos < 0) return;
+  a.xo[0][pos] = x0;
+  a.xo[1][pos] = x1;
+  a.xo[2][pos] = x2;
+  for (int d = 0; d < 3; d++) a.vo[d][pos] = a.vi[d][i];
+  a.go[pos] = a.gi[i];
+  a.spo[pos] = a.spi[i];
+  a.ido[pos] = a.idi[i];
+  a.perm[pos] = (uint32_t)i;
+}
+
 }  // namespace
 
 void launch_gather_f64(emo_ctx *c) {
+  c->fld_state = kFldFresh;
   if (c->np == 0) return;
   GatherArgs a;
   a.g = c->geo;
@@ -348,6 +476,7 @@ size_t sort_tmp_bytes_for(int64_t n, int bits) {
 // permutation into the alternate buffers, and per-cell [beg,end) ranges.
 int sort_f64(emo_ctx *c, int with_fields) {
   if (c->np == 0) {
+    cs_fill_i32(c, c->srt.off, 0, c->geo.ncell + 1);
     c->sorted = true;
     return EMO_OK;
   }
@@ -387,10 +516,11 @@ int sort_f64(emo_ctx *c, int with_fields) {
   if (!with_fields)
     for (int d = 0; d < 6; d++) std::swap(c->p.fld[d], c->palt.fld[d]);
   std::swap(c->key, c->key_alt);
-  EMO_HIP_CHECK_RET(c, hipMemsetAsync(c->cbeg, 0, c->geo.ncell * sizeof(int32_t), c->stream));
-  EMO_HIP_CHECK_RET(c, hipMemsetAsync(c->cend, 0, c->geo.ncell * sizeof(int32_t), c->stream));
-  hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, c->key,
-                     c->cbeg, c->cend, (long long)c->np);
+  cs_fill_i32(c, c->srt.count, 0, c->geo.ncell + 1);
+  hipLaunchKernelGGL(k_count_sorted, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, (const uint32_t *)c->key,
+                     c->srt.count, (long long)c->np);
+  int rc = cellsort_scan(c, c->srt, c->srt.off, "sort_ranges");
+  if (rc) return rc;
   c->sorted = true;
   return EMO_OK;
 }
@@ -403,8 +533,7 @@ void launch_deposit_f64(emo_ctx *c, int rho) {
     a.v[d] = c->p.v[d];
   }
   a.sp = c->p.sp;
-  a.beg = c->cbeg;
-  a.end = c->cend;
+  a.off = c->srt.off;
   a.s = c->species;
   a.cj = c->cj;
   a.cr = c->cr;
@@ -431,6 +560,148 @@ void launch_cell_ids(emo_ctx *c, int32_t *out) {
   hipLaunchKernelGGL(k_cell_ids, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, c->geo,
                      (const double *)c->p.x[0], (const double *)c->p.x[1],
                      (const double *)c->p.x[2], out, (long long)c->np);
+}
+
+
+void fld_materialize(emo_ctx *c) {
+  if (c->fld_state != kFldDeferred) return;
+  launch_gather_f64(c);
+  c->fld_state = kFldFresh;
+}
+
+void fld_invalidate(emo_ctx *c) {
+  fld_materialize(c);
+  c->fld_state = kFldStale;
+}
+
+int plain_sort_create(emo_ctx *c) { return cellsort_alloc(c, c->srt, c->cap, c->geo.ncell); }
+
+void plain_sort_destroy(emo_ctx *c) { cellsort_free(c->srt); }
+
+// The fused particle phase of the plain form on a cell-sorted store, part 1:
+// push (gathering the fields from the grid) and BCs into palt, with the move
+// code of every particle.  With write_fld the gathered fields are stored in
+// p.fld (the pre-push order).
+void fused_push_f64(emo_ctx *c, double dt, int write_fld) {
+  const Geo &g = c->geo;
+  const long long np = c->np;
+  Pgd64Args a;
+  a.g = g;
+  for (int d = 0; d < 3; d++) {
+    a.x[d] = c->p.x[d];
+    a.v[d] = c->p.v[d];
+    a.xo[d] = c->palt.x[d];
+    a.vo[d] = c->palt.v[d];
+    a.E[d] = c->E[d];
+    a.B[d] = c->B[d];
+    a.Bt[d] = c->Bt[d];
+    a.xmin[d] = c->cfg.xyz0[d];
+    a.xmax[d] = a.xmin[d] + c->cfg.Lxyz[d];
+    a.per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
+  }
+  for (int q = 0; q < 6; q++) a.fld[q] = c->p.fld[q];
+  a.gam = c->p.gam;
+  a.go = c->palt.gam;
+  a.sp = c->p.sp;
+  a.s = c->species;
+  a.dt = dt;
+  for (int s = 0; s < 6; s++) a.pbc[s] = c->cfg.parts_bound[s];
+  a.code = c->srt.code;
+  a.nfar = c->srt.misc;
+  a.write_fld = write_fld;
+  a.flag = c->dflag;
+  a.np = np;
+  {
+    TimedLaunch t(c, "pgd_f64");
+    if (c->cfg.relativity)
+      hipLaunchKernelGGL(k_pgd64<true>, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL(k_pgd64<false>, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, a);
+  }
+}
+
+// part 2: the stable counting sort by the new cell and the scatter into the
+// main store; the permutation (new position -> previous index) is left in
+// c->srt.perm and the new cell ranges in c->srt.off.
+int fused_sort_f64(emo_ctx *c) {
+  const Geo &g = c->geo;
+  const long long np = c->np;
+  static const char *const names[4] = {"far_f64", "rank_f64", "base_f64", "scan_f64"};
+  CellsF64 cells;
+  cells.g = g;
+  for (int d = 0; d < 3; d++) {
+    cells.xs[d] = c->p.x[d];
+    cells.xd[d] = c->palt.x[d];
+  }
+  int rc = cellsort_offsets(c, c->srt, np, cells, names);
+  if (rc) return rc;
+  // The counting sort holds at most kFarCap particles that move further than
+  // one cell and 65535 per (cell, move) group.  Its flags are read back
+  // before the scatter: on overflow this step sorts with the radix sort
+  // instead (the same stable order), from the pushed state in palt.
+  if (hipMemcpyAsync(c->hflag, c->dflag, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    c->err = "flag readback failed";
+    return EMO_ERR_HIP;
+  }
+  const unsigned long long fl = c->hflag[0];
+  if (fl & kFlagOutOfGrid) {
+    c->err = "particle outside the padded grid (Grid::getCellID asserts, src/grid/oGrid.cpp:212-220)";
+    return EMO_ERR_INVARIANT;
+  }
+  if (fl & (kFlagFarMovers | kFlagMigrate)) {
+    c->hflag[0] = fl & ~(kFlagFarMovers | kFlagMigrate);
+    if (hipMemcpyAsync(c->dflag, c->hflag, sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      c->err = "flag reset failed";
+      return EMO_ERR_HIP;
+    }
+    cs_fill_i32(c, c->srt.misc, 0, 1);
+    for (int d = 0; d < 3; d++) {
+      std::swap(c->p.x[d], c->palt.x[d]);
+      std::swap(c->p.v[d], c->palt.v[d]);
+    }
+    std::swap(c->p.gam, c->palt.gam);
+    rc = sort_f64(c, 0);
+    if (rc) return rc;
+    if (hipMemcpyAsync(c->srt.perm, c->perm_alt, (size_t)np * 4, hipMemcpyDeviceToDevice, c->stream) !=
+        hipSuccess) {
+      c->err = "permutation copy failed";
+      return EMO_ERR_HIP;
+    }
+    c->fallback_sorts++;
+    return EMO_OK;
+  }
+  Scatter64Args sa;
+  sa.g = g;
+  sa.b = c->srt;
+  for (int d = 0; d < 3; d++) {
+    sa.xi[d] = c->palt.x[d];
+    sa.vi[d] = c->palt.v[d];
+    sa.xo[d] = c->p.x[d];
+    sa.vo[d] = c->p.v[d];
+  }
+  sa.gi = c->palt.gam;
+  sa.go = c->p.gam;
+  sa.spi = c->p.sp;
+  sa.idi = c->p.id;
+  sa.spo = c->palt.sp;
+  sa.ido = c->palt.id;
+  sa.perm = c->srt.perm;
+  sa.flag = c->dflag;
+  sa.np = np;
+  {
+    TimedLaunch t(c, "scatter_f64");
+    hipLaunchKernelGGL(k_scatter64, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, sa);
+  }
+  std::swap(c->p.sp, c->palt.sp);
+  std::swap(c->p.id, c->palt.id);
+  std::swap(c->srt.off, c->srt.off_new);
+  cs_fill_i32(c, c->srt.misc, 0, 1);
+  c->sorted = true;
+  return EMO_OK;
 }
 
 }  // namespace emo

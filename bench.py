@@ -44,6 +44,10 @@ def algorithmic_bytes(kernel: str, np_: int, ncell: int, relativity: int) -> flo
         "particle_bc_f64": 24 * np_,
         "sort_keys": 32 * np_,
         "sort_permute": 236 * np_,
+        "pgd_f64": (117 if relativity else 101) * np_ + 72 * ncell,
+        "rank_f64": 3 * np_ + 8 * ncell,
+        "base_f64": 4 * 27 * ncell,
+        "scatter_f64": 145 * np_ + 8 * ncell,
         "deposit_cells_f64": 52 * np_ + 168 * ncell,
         "deposit_grid_f64": 160 * ncell + 32 * ng,
         "yee_E_f64": 96 * ncell,
@@ -62,8 +66,9 @@ def algorithmic_bytes(kernel: str, np_: int, ncell: int, relativity: int) -> flo
 
 
 KERNELS = {
-    "fp64": ["gather_f64", "push_f64", "particle_bc_f64", "sort_keys", "sort_radix", "sort_permute",
-             "deposit_cells_f64", "deposit_grid_f64", "yee_E_f64", "yee_B_f64"],
+    "fp64": ["pgd_f64", "far_f64", "rank_f64", "base_f64", "scan_f64", "scatter_f64", "gather_f64", "push_f64",
+             "particle_bc_f64", "sort_keys", "sort_radix", "sort_permute", "deposit_cells_f64", "deposit_grid_f64",
+             "yee_E_f64", "yee_B_f64"],
     "fp32": ["pgd_f32", "far_f32", "rank_f32", "base_f32", "scan_f32", "scatter_f32", "deposit_f32", "jsum_f32",
              "yee_E_f32", "yee_B_f32"],
 }
@@ -107,7 +112,7 @@ def run_gpu(mode: str, steps: int, warmup: int, scale: float):
 
 
 # library calls (several rocprim kernels each): reported, never the roofline kernel
-LIBRARY_CALLS = {"sort_radix", "scan_f32"}
+LIBRARY_CALLS = {"sort_radix", "scan_f32", "scan_f64"}
 
 
 def roofline_for(res):
@@ -251,6 +256,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true", help="only the c4 adjoint leg (for profiling)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run an elementwise copy of 256 MiB (a counter reference of known bytes)")
     ap.add_argument("--adjoint-steps", type=int, default=0, help="c4 steps for the adjoint leg (0: 400)")
     ap.add_argument("--checkpoint-every", type=int, default=20)
     ap.add_argument("--cpu-cells", type=int, default=64)
@@ -263,6 +270,14 @@ def main():
 
     import apc_524_project_amd as emo
 
+    if args.calibrate_copy:
+        import torch
+
+        a = torch.ones(64 << 20, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     if args.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint_c4(args.adjoint_steps or None, args.checkpoint_every)}), flush=True)
         return

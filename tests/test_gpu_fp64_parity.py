@@ -229,3 +229,112 @@ def test_out_of_grid_particle_is_an_error():
     with pytest.raises(EmoError) as e:
         ctx.init_constant()
     assert e.value.code == -5
+
+
+# ---------------------------------------------------------------------------
+# emo_step on a cell-sorted store runs the fused phase (push with its gather,
+# BCs, move codes, stable counting sort, scatter).  The counting sort keeps
+# at most 65536 particles that move further than one cell and 65535 per
+# (cell, move) group; beyond that the step sorts with the radix sort.  Both
+# must give the oracle's bits.
+# ---------------------------------------------------------------------------
+def fused_run(case, steps, dt, ctx_o=None):
+    ctx, o = ctx_o if ctx_o is not None else pair(case)
+    ctx.init_constant()
+    o.initialize()
+    ctx.timing(True)
+    ctx.step(steps, 0.0, dt)
+    t = 0.0
+    for _ in range(steps):
+        o.step(t, dt)
+        t += dt
+    return ctx, o
+
+
+def launches(ctx, name):
+    return ctx.kernel_time(name)[1]
+
+
+FAST = dict(n=(8, 8, 8), L=(8.0, 8.0, 8.0), nparticles=2 * 16 * 512, temp=(50180.0, 50180.0),
+            dens_phys=2.824e9, relativity=0)
+
+
+def test_fused_step_with_far_movers_bit_exact():
+    case = Case(**FAST)
+    dt = 1.5  # a few percent of the particles cross more than one cell a step
+    ctx, o = fused_run(case, 4, dt)
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
+    # the first step sorts the unsorted load with the radix sort, the rest fuse
+    assert launches(ctx, "sort_radix") == 1
+    assert launches(ctx, "scatter_f64") == 3
+
+
+def test_fused_step_far_mover_overflow_falls_back_bit_exact():
+    case = Case(n=(16, 16, 16), L=(16.0, 16.0, 16.0), nparticles=2 * 24 * 4096, temp=(50180.0, 50180.0),
+                dens_phys=2.824e9, relativity=0)
+    dt = 3.0  # about 86000 particles cross more than one cell a step
+    ctx, o = fused_run(case, 3, dt)
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
+    assert launches(ctx, "scatter_f64") == 0
+    assert launches(ctx, "sort_radix") == 3
+
+
+def test_fused_step_crowded_cell_falls_back_bit_exact():
+    """70016 particles in one cell: a (cell, move) group beyond 65535."""
+    case = Case(n=(4, 4, 4), L=(4.0, 4.0, 4.0), nparticles=2 * 64 * 547, temp=(5000.0, 5000.0),
+                dens_phys=2.824e9, relativity=0)
+    rng = np.random.default_rng(3)
+    n = case.nparticles
+    x = 2.0 + rng.uniform(0.3, 0.7, size=(n, 3))
+    v = np.zeros((n, 3))
+    sp = (np.arange(n) % 2).astype(np.int32)
+    ids = np.arange(n, dtype=np.int64)
+    o = Oracle(case)
+    assert o.load() == 0
+    q, m = o.q, o.m
+    o.set_particles(x, v, np.ones(n), sp, ids)
+    ctx = Context(emo_config(case, q=q, m=m))
+    ctx.load_particles(x, v, np.ones(n), sp, ids)
+    dt = case.dt_cfl()
+    ctx, o = fused_run(case, 3, dt, (ctx, o))
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
+    assert launches(ctx, "sort_radix") == 3
+
+
+def test_fused_step_after_phase_calls_and_set_field():
+    """Steps mixed with per-phase calls and a field overwrite: the stored
+    per-particle fields are used where they differ from the grid (a set_field
+    after the last gather), and the fused phase only where they agree."""
+    case = Case(**{**THERMAL.__dict__, "relativity": 1})
+    ctx, o = pair(case)
+    dt = case.dt_cfl()
+    ctx.init_constant()
+    o.initialize()
+    ctx.step(3, 0.0, dt)
+    t = 0.0
+    for _ in range(3):
+        o.step(t, dt)
+        t += dt
+    assert_particles_equal(ctx, o)  # materializes the deferred gather
+    ex = o.field(EX) * 1.5
+    ctx.set_field(EX, ex)
+    o.set_field(EX, ex)
+    ctx.step(2, t, dt)  # first of these pushes with the stored (pre-overwrite) fields
+    for _ in range(2):
+        o.step(t, dt)
+        t += dt
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
+    ctx.push(dt)
+    o.push(dt)
+    ctx.particle_bc()
+    o.particle_bc()
+    ctx.sort()
+    o.sort()
+    ctx.deposit()
+    o.deposit()
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
