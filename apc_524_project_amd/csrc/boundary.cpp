@@ -58,6 +58,8 @@ void light_bc(emo_ctx *c, FieldSet<T> &fs, int side, double t, double dt) {
   static const int BpolT[3][3] = {{2, 1, 0}, {0, 2, 1}, {1, 0, 2}};
   double dx = c->geo.d[dim];
   double bPhase = (dx + sign * dt) / 2;
+  PlaneFill fills[6];
+  int nf = 0;
   for (int i = 0; i < 3; i++) {
     int eid = EpolT[dim][i], bid = BpolT[dim][i];
     double ewave = 0.0, bwave = 0.0;
@@ -82,27 +84,36 @@ void light_bc(emo_ctx *c, FieldSet<T> &fs, int side, double t, double dt) {
         if (fid < 3) pf.off[q] = q == comp ? 0.5 * c->geo.d[q] : 0.0;
         else pf.off[q] = q == comp ? 0.0 : 0.5 * c->geo.d[q];
       }
-      fk_plane_fill<T>(c, pf);
+      fills[nf++] = pf;
     }
   }
+  fk_plane_fill_multi<T>(c, fills, nf);  // six different arrays
 }
 
 }  // namespace
 
 template <typename T>
 void refresh_fields(emo_ctx *c, FieldSet<T> &fs, double t, double dt) {
+  // Q23: B through the fixed fieldPtr_ allocations
+  T *f[6] = {fs.E[0], fs.E[1], fs.E[2], fs.Bfix[0], fs.Bfix[1], fs.Bfix[2]};
+  // runs of consecutive periodic sides go in one launch; a LightBC side
+  // between them keeps its place in the side order (its fills cover the
+  // whole plane, ghosts included, and the later copies read them)
+  int ns = 0, ds[6], srcs[6], dsts[6];
   for (int side = 0; side < 6; side++) {
     int d = side / 2, right = side & 1;
     if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
-      int src = right ? 1 : c->geo.n[d];
-      int dst = right ? c->geo.n[d] + 1 : 0;
-      // Q23: B through the fixed fieldPtr_ allocations
-      T *f[6] = {fs.E[0], fs.E[1], fs.E[2], fs.Bfix[0], fs.Bfix[1], fs.Bfix[2]};
-      fk_plane_copy<T>(c, f, 6, d, src, dst, 0);
+      ds[ns] = d;
+      srcs[ns] = right ? 1 : c->geo.n[d];
+      dsts[ns] = right ? c->geo.n[d] + 1 : 0;
+      ns++;
     } else {
+      fk_plane_copy_sides<T>(c, f, 6, ns, ds, srcs, dsts);
+      ns = 0;
       light_bc<T>(c, fs, side, t, dt);
     }
   }
+  fk_plane_copy_sides<T>(c, f, 6, ns, ds, srcs, dsts);
 }
 
 template <typename T>

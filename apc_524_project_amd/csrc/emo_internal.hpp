@@ -195,6 +195,13 @@ template <typename T>
 void fk_plane_copy(emo_ctx *c, T *const *fields, int nf, int d, int src, int dst, int add);
 template <typename T>
 void fk_plane_fill(emo_ctx *c, const PlaneFill &pf);
+// up to six fills of different planes in one launch
+template <typename T>
+void fk_plane_fill_multi(emo_ctx *c, const PlaneFill *pf, int n);
+// up to six periodic sides (plain copies over the physical range) in one launch
+template <typename T>
+void fk_plane_copy_sides(emo_ctx *c, T *const *fields, int nf, int nsides, const int *d, const int *src,
+                         const int *dst);
 template <typename T>
 void fk_const_fill(emo_ctx *c, T *f, double v, long long n);
 template <typename T>

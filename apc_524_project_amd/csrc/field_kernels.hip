@@ -66,6 +66,131 @@ __global__ void __launch_bounds__(kBlock) k_plane_copy(PlaneArgs<T> a) {
   }
 }
 
+// All periodic sides of one refresh in one launch: each side's ghost plane
+// takes the opposite physical plane over the physical range of the other
+// two dims.  No side reads a point another side writes (the sources are
+// physical points, the destinations ghost points), so their order is free.
+template <typename T>
+struct SidesArgs {
+  Geo g;
+  T *f[6];
+  int nf, nsides;
+  int d[6], src[6], dst[6];
+  long long start[7];  // thread ranges of the sides
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_plane_copy_sides(SidesArgs<T> a) {
+  const Geo &g = a.g;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.start[a.nsides]) return;
+  int s = 0;
+  while (t >= a.start[s + 1]) s++;
+  t -= a.start[s];
+  const int d = a.d[s];
+  const int u = d == 0 ? 1 : 0;
+  const int w = d == 2 ? 1 : 2;
+  int is[3], id[3];
+  is[d] = a.src[s];
+  id[d] = a.dst[s];
+  is[u] = id[u] = 1 + (int)(t / g.n[w]);
+  is[w] = id[w] = 1 + (int)(t % g.n[w]);
+  const long long ps = gidx(g, is[0], is[1], is[2]);
+  const long long pd = gidx(g, id[0], id[1], id[2]);
+  for (int f = 0; f < a.nf; f++) a.f[f][pd] = a.f[f][ps];
+}
+
+// The fold of every periodic dim in one launch.  A thread owns one point of
+// the union of the planes x=1, y=1, z=1 (physical range) and adds the ghost
+// values of the dims that fold onto it in the order d = 0, 1, 2, as the
+// per-dim passes do; the ghost sources are never destinations.
+template <typename T>
+struct FoldArgs {
+  Geo g;
+  T *J[3];
+  T *rho;
+  int per[3];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fold_all(FoldArgs<T> a) {
+  const Geo &g = a.g;
+  const long long n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (t < n1 * n2) {  // x = 1
+    i = 1;
+    j = 1 + (int)(t / n2);
+    k = 1 + (int)(t % n2);
+  } else if ((t -= n1 * n2) < (n0 - 1) * n2) {  // y = 1, x >= 2
+    i = 2 + (int)(t / n2);
+    j = 1;
+    k = 1 + (int)(t % n2);
+  } else if ((t -= (n0 - 1) * n2) < (n0 - 1) * (n1 - 1)) {  // z = 1, x, y >= 2
+    i = 2 + (int)(t / (n1 - 1));
+    j = 2 + (int)(t % (n1 - 1));
+    k = 1;
+  } else {
+    return;
+  }
+  const long long p = gidx(g, i, j, k);
+  const int at[3] = {i, j, k};
+  // arrays folded along d: J of the two transverse components, and rho
+  T *arr[4] = {a.J[0], a.J[1], a.J[2], a.rho};
+  for (int q = 0; q < 4; q++) {
+    if (!arr[q]) continue;
+    T val = arr[q][p];
+    bool touched = false;
+    for (int d = 0; d < 3; d++) {
+      if (!a.per[d] || at[d] != 1 || q == d) continue;
+      int src[3] = {i, j, k};
+      src[d] = g.n[d] + 1;
+      val += arr[q][gidx(g, src[0], src[1], src[2])];
+      touched = true;
+    }
+    if (touched) arr[q][p] = val;
+  }
+}
+
+// Several plane fills in one launch (the six of one LightBC side write six
+// different arrays).
+struct FillMultiArgs {
+  Geo g;
+  PlaneFill pf[6];
+  int npf;
+  long long start[7];
+  double Lc[3];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_plane_fill_multi(FillMultiArgs a) {
+  const Geo &g = a.g;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.start[a.npf]) return;
+  int s = 0;
+  while (t >= a.start[s + 1]) s++;
+  t -= a.start[s];
+  const PlaneFill &pf = a.pf[s];
+  const int d = pf.dim;
+  const int u = d == 0 ? 1 : 0;
+  const int w = d == 2 ? 1 : 2;
+  const int pu = (int)(t / g.nt[w]);
+  const int pw = (int)(t % g.nt[w]);
+  int ii[3];
+  ii[d] = pf.idx;
+  ii[u] = pu;
+  ii[w] = pw;
+  double v;
+  if (pf.waist > 0) {
+    double ya = g.x0[u] + g.d[u] * (pu - 1) + pf.off[u] - a.Lc[u];
+    double yb = g.x0[w] + g.d[w] * (pw - 1) + pf.off[w] - a.Lc[w];
+    v = pf.base + pf.wave * exp(-(ya * ya + yb * yb) / (pf.waist * pf.waist));
+  } else {
+    v = pf.base + pf.wave;
+  }
+  ((T *)pf.f)[gidx(g, ii[0], ii[1], ii[2])] = (T)v;
+}
+
 // ---------------------------------------------------------------------------
 // Yee update, Grid::evolveFields (src/grid/oGrid.cpp:23-57): E on the
 // physical points from B_tm1 and J; then B from B_tm1 and the new E.  Two
@@ -379,6 +504,43 @@ void fk_plane_fill(emo_ctx *c, const PlaneFill &pf) {
 }
 
 template <typename T>
+void fk_plane_fill_multi(emo_ctx *c, const PlaneFill *pf, int n) {
+  if (n <= 0) return;
+  FillMultiArgs a;
+  a.g = c->geo;
+  a.npf = n;
+  a.start[0] = 0;
+  for (int s = 0; s < n; s++) {
+    a.pf[s] = pf[s];
+    const int d = pf[s].dim;
+    const int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
+    a.start[s + 1] = a.start[s] + (long long)c->geo.nt[u] * c->geo.nt[w];
+  }
+  for (int q = 0; q < 3; q++) a.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
+  hipLaunchKernelGGL(k_plane_fill_multi<T>, dim3(grid_for(a.start[n])), dim3(kBlock), 0, c->stream, a);
+}
+
+template <typename T>
+void fk_plane_copy_sides(emo_ctx *c, T *const *fields, int nf, int nsides, const int *d, const int *src,
+                         const int *dst) {
+  if (nsides <= 0) return;
+  SidesArgs<T> a;
+  a.g = c->geo;
+  for (int q = 0; q < nf; q++) a.f[q] = fields[q];
+  a.nf = nf;
+  a.nsides = nsides;
+  a.start[0] = 0;
+  for (int s = 0; s < nsides; s++) {
+    a.d[s] = d[s];
+    a.src[s] = src[s];
+    a.dst[s] = dst[s];
+    const int u = d[s] == 0 ? 1 : 0, w = d[s] == 2 ? 1 : 2;
+    a.start[s + 1] = a.start[s] + (long long)c->geo.n[u] * c->geo.n[w];
+  }
+  hipLaunchKernelGGL(k_plane_copy_sides<T>, dim3(grid_for(a.start[nsides])), dim3(kBlock), 0, c->stream, a);
+}
+
+template <typename T>
 void fk_const_fill(emo_ctx *c, T *f, double v, long long n) {
   hipLaunchKernelGGL(k_const_fill<T>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, f, (T)v, n);
 }
@@ -386,11 +548,19 @@ void fk_const_fill(emo_ctx *c, T *f, double v, long long n) {
 template <typename T>
 void fk_fold(emo_ctx *c, FieldSet<T> &fs) {
   const Geo &g = c->geo;
+  FoldArgs<T> a;
+  a.g = g;
+  int any = 0;
   for (int d = 0; d < 3; d++) {
-    if (c->cfg.fields_bound[2 * d] != EMO_BC_PERIODIC) continue;
-    T *f[3] = {d == 0 ? fs.J[1] : fs.J[0], d == 2 ? fs.J[1] : fs.J[2], fs.rho};
-    fk_plane_copy<T>(c, f, fs.rho ? 3 : 2, d, g.n[d] + 1, 1, 1);
+    a.J[d] = fs.J[d];
+    a.per[d] = c->cfg.fields_bound[2 * d] == EMO_BC_PERIODIC;
+    any |= a.per[d];
   }
+  a.rho = fs.rho;
+  if (!any) return;
+  const long long n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  const long long pts = n1 * n2 + (n0 - 1) * n2 + (n0 - 1) * (n1 - 1);
+  hipLaunchKernelGGL(k_fold_all<T>, dim3(grid_for(pts)), dim3(kBlock), 0, c->stream, a);
 }
 
 template <typename T>
@@ -430,6 +600,9 @@ void fk_diag_fields(emo_ctx *c, const FieldSet<T> &fs, double *partials, int nbl
 #define EMO_INSTANTIATE(T)                                                                       \
   template void fk_plane_copy<T>(emo_ctx *, T *const *, int, int, int, int, int);                \
   template void fk_plane_fill<T>(emo_ctx *, const PlaneFill &);                                  \
+  template void fk_plane_fill_multi<T>(emo_ctx *, const PlaneFill *, int);                       \
+  template void fk_plane_copy_sides<T>(emo_ctx *, T *const *, int, int, const int *, const int *, \
+                                       const int *);                                             \
   template void fk_const_fill<T>(emo_ctx *, T *, double, long long);                             \
   template void fk_fold<T>(emo_ctx *, FieldSet<T> &);                                            \
   template void fk_evolve<T>(emo_ctx *, FieldSet<T> &, double);                                  \

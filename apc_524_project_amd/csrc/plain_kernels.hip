@@ -199,10 +199,13 @@ __global__ void __launch_bounds__(kBlock) k_count_sorted(const uint32_t *key, in
 // (src/grid/oGrid.cpp:233-244); test particles are skipped
 // (src/particles/particle_handler.cpp:205).
 //
-// One wavefront per cell: the 64 lanes load 64 consecutive particles
-// (coalesced) and compute their 20 contributions in parallel; the rows go
-// to LDS and lanes 0..19 then add column q row by row, so every component is
-// summed in exactly the particle order of the sequential loop.
+// Sixteen lanes per cell, four cells per wavefront: in batches of 16
+// consecutive particles (coalesced) each lane computes one particle's 20
+// contributions; the rows go to LDS, and lane l then adds column l (and
+// column 16 + l) of the batch row by row, so every component is summed in
+// exactly the particle order of the sequential loop.  Rows past the cell's
+// end and rows of test particles are zero: adding +0.0 to a sum that starts
+// at +0.0 leaves every bit unchanged (such a sum is never -0.0).
 // ---------------------------------------------------------------------------
 struct DepCellArgs {
   Geo g;
@@ -214,60 +217,65 @@ struct DepCellArgs {
   int rho;
 };
 
-constexpr int kDepWaves = kBlock / 64;
-constexpr int kDepNV = 20;       // 12 J + 8 rho
-constexpr int kDepStride = 21;   // padded row length in LDS
-
+constexpr int kDepLanes = 16;                       // lanes per cell
+constexpr int kDepCells = kBlock / kDepLanes;       // cells per block
+constexpr int kDepStride = 21;                      // padded row length in LDS
 
 __global__ void __launch_bounds__(kBlock) k_deposit_cells(DepCellArgs a) {
-  __shared__ double rows[kDepWaves][64 * kDepStride];
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
+  __shared__ double rows[kDepCells][kDepLanes * kDepStride];
+  const int l = threadIdx.x & (kDepLanes - 1);
+  const int slot = threadIdx.x / kDepLanes;
   const Geo &g = a.g;
   const int nv = a.rho ? 20 : 12;
-  double *row = rows[w];
-  for (long long c = (long long)blockIdx.x * kDepWaves + w; c < g.ncell;
-       c += (long long)gridDim.x * kDepWaves) {
-    const int b = a.off[c], e = a.off[c + 1];
-    double acc = 0.0;
-    if (e > b) {
-      int iz = (int)(c % g.nc[2]);
-      int iy = (int)(((c - iz) / g.nc[2]) % g.nc[1]);
-      int ix = (int)((((c - iz) / g.nc[2]) - iy) / g.nc[1]);
-      double cellverts[3];
-      cellverts[0] = g.x0[0] + g.d[0] * ix;
-      cellverts[1] = g.x0[1] + g.d[1] * iy;
-      cellverts[2] = g.x0[2] + g.d[2] * iz;
-      for (int base = b; base < e; base += 64) {
-        const int p = base + lane;
-        bool valid = false;
-        if (p < e) {
-          int sp = a.sp[p];
-          if (!a.s.test[sp]) {
-            valid = true;
-            double pos[3] = {a.x[0][p], a.x[1][p], a.x[2][p]};
-            double vel[3] = {a.v[0][p], a.v[1][p], a.v[2][p]};
-            double r20[20];
-            deposit_row(g, cellverts, pos, vel, a.s.q[sp], a.rho != 0, r20);
-            double *r = row + lane * kDepStride;
+  double *row = rows[slot];
+  const long long c = (long long)blockIdx.x * kDepCells + slot;
+  const bool cell_ok = c < g.ncell;
+  int b = 0, e = 0;
+  double cellverts[3] = {0.0, 0.0, 0.0};
+  if (cell_ok) {
+    b = a.off[c];
+    e = a.off[c + 1];
+    int iz = (int)(c % g.nc[2]);
+    int iy = (int)(((c - iz) / g.nc[2]) % g.nc[1]);
+    int ix = (int)((((c - iz) / g.nc[2]) - iy) / g.nc[1]);
+    cellverts[0] = g.x0[0] + g.d[0] * ix;
+    cellverts[1] = g.x0[1] + g.d[1] * iy;
+    cellverts[2] = g.x0[2] + g.d[2] * iz;
+  }
+  // the wavefront runs as many batches as its fullest cell needs
+  int nb = (e - b + kDepLanes - 1) / kDepLanes;
+  for (int o = kDepLanes; o < 64; o <<= 1) nb = max(nb, __shfl_xor(nb, o));
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int t = 0; t < nb; t++) {
+    const int p = b + t * kDepLanes + l;
+    double r20[20];
 #pragma unroll
-            for (int q = 0; q < 20; q++)
-              if (q < nv) r[q] = r20[q];
-          }
-        }
-        const unsigned long long m = __ballot(valid);
-        wave_sync();
-        const int cnt = min(64, e - base);
-        if (lane < nv) {
-          for (int rr = 0; rr < cnt; rr++)
-            if ((m >> rr) & 1ull) acc += row[rr * kDepStride + lane];
-        }
-        wave_sync();
+    for (int q = 0; q < 20; q++) r20[q] = 0.0;
+    if (p < e) {
+      const int sp = a.sp[p];
+      if (!a.s.test[sp]) {
+        double pos[3] = {a.x[0][p], a.x[1][p], a.x[2][p]};
+        double vel[3] = {a.v[0][p], a.v[1][p], a.v[2][p]};
+        deposit_row(g, cellverts, pos, vel, a.s.q[sp], a.rho != 0, r20);
       }
     }
-    // per-cell vectors, component-major so that stage 2 reads coalesce
-    if (lane < 12) a.cj[lane * g.ncell + c] = acc;
-    else if (lane < nv) a.cr[(lane - 12) * g.ncell + c] = acc;
+    double *r = row + l * kDepStride;
+#pragma unroll
+    for (int q = 0; q < 20; q++) r[q] = r20[q];
+    wave_sync();
+#pragma unroll
+    for (int rr = 0; rr < kDepLanes; rr++) acc0 += row[rr * kDepStride + l];
+    if (l < 4) {
+#pragma unroll
+      for (int rr = 0; rr < kDepLanes; rr++) acc1 += row[rr * kDepStride + 16 + l];
+    }
+    wave_sync();
+  }
+  // per-cell vectors, component-major so that stage 2 reads coalesce
+  if (cell_ok) {
+    if (l < 12) a.cj[l * g.ncell + c] = acc0;
+    else if (l < nv) a.cr[(l - 12) * g.ncell + c] = acc0;
+    if (l < 4 && nv == 20) a.cr[(l + 4) * g.ncell + c] = acc1;
   }
 }
 
@@ -540,9 +548,8 @@ void launch_deposit_f64(emo_ctx *c, int rho) {
   a.rho = rho;
   {
     TimedLaunch t(c, "deposit_cells_f64");
-    long long blocks = (c->geo.ncell + kDepWaves - 1) / kDepWaves;
-    hipLaunchKernelGGL(k_deposit_cells, dim3((unsigned)std::min<long long>(blocks, 1 << 20)), dim3(kBlock), 0,
-                       c->stream, a);
+    const long long blocks = (c->geo.ncell + kDepCells - 1) / kDepCells;
+    hipLaunchKernelGGL(k_deposit_cells, dim3((unsigned)blocks), dim3(kBlock), 0, c->stream, a);
   }
   DepGridArgs<double> b;
   b.g = c->geo;

@@ -198,6 +198,96 @@ def run_adjoint_c4(steps: int | None, k: int):
     return res
 
 
+def run_accuracy_c2(checkpoints=(10, 100, 1000), scale: float = 1.0):
+    """Workload c2 in both forms from the same loaded particles: relative L2
+    error of E, B, J (physical points) of the FP32 chosen form against the FP64
+    plain form (bit-exact with the oracle) after each checkpoint step count,
+    of the x-directed Langmuir mode (k = 2 pi / 64 component of the y-z mean
+    of E_x) that the density perturbation drives, and of the particles'
+    positions and velocities at the last checkpoint.
+
+    The same errors are given for an FP64 plain run started from the FP32
+    form's initial particles (positions and velocities as the FP32 store
+    holds them): it isolates how fast the loop itself amplifies a perturbation
+    of FP32 representation size, independent of FP32 arithmetic."""
+    import numpy as np
+
+    import apc_524_project_amd as emo
+    from apc_524_project_amd import FP32_CHOSEN, FP64_PLAIN
+    from apc_524_project_amd import _lib as L_
+
+    ctxs = {}
+    for name, prec in (("fp64", FP64_PLAIN), ("fp32", FP32_CHOSEN)):
+        w = emo.make_workload("c2", precision=prec, scale=scale)
+        c = emo.Context(w.cfg)
+        c.load_maxwellian(w.loader)
+        c.init_constant()
+        ctxs[name] = c
+    p32 = ctxs["fp32"].particles(fields=False)
+    w64 = emo.make_workload("c2", precision=FP64_PLAIN, scale=scale)
+    c = emo.Context(w64.cfg)
+    c.load_particles(p32["x"], p32["v"], p32["gamma"], p32["species"], p32["id"])
+    c.init_constant()
+    ctxs["fp64_from_fp32_state"] = c
+    del p32
+    n = w.cfg.nCell
+    inner = (slice(1, n[0] + 1), slice(1, n[1] + 1), slice(1, n[2] + 1))
+
+    def group(c, fids):
+        return np.concatenate([c.field(f)[inner].ravel() for f in fids])
+
+    def rel(a, b):
+        nb = float(np.linalg.norm(b))
+        return float(np.linalg.norm(a - b) / nb) if nb > 0 else None
+
+    def langmuir(c):
+        ex = c.field(L_.EX)[inner].mean(axis=(1, 2))
+        return np.fft.fft(ex)[1]
+
+    groups = {"E": (L_.EX, L_.EY, L_.EZ), "B": (L_.BX, L_.BY, L_.BZ), "J": (L_.JX, L_.JY, L_.JZ)}
+    rows = {"fp32": [], "fp64_from_fp32_state": []}
+    done = 0
+    for cp in checkpoints:
+        for c in ctxs.values():
+            c.step(cp - done, done * w.dt, w.dt)
+            c.synchronize()
+        done = cp
+        ref = {g: group(ctxs["fp64"], fids) for g, fids in groups.items()}
+        d64 = ctxs["fp64"].diag()
+        m64 = langmuir(ctxs["fp64"])
+        for name in rows:
+            row = {"steps": cp}
+            for g, fids in groups.items():
+                row[g] = rel(group(ctxs[name], fids), ref[g])
+            d = ctxs[name].diag()
+            row["field_energy"] = rel(np.array(d[4:6]), np.array(d64[4:6]))
+            row["langmuir_mode"] = float(abs(langmuir(ctxs[name]) - m64) / abs(m64))
+            rows[name].append(row)
+            print(f"[bench] accuracy c2 {name} vs fp64 after {cp} steps: " +
+                  ", ".join(f"{k} {v:.3g}" for k, v in row.items() if k != "steps" and v is not None),
+                  file=sys.stderr, flush=True)
+    parts = {}
+    for name, c in ctxs.items():
+        p = c.particles(fields=False)
+        o = np.argsort(p["id"])
+        parts[name] = (p["x"][o], p["v"][o])
+        c.close()
+    L = np.array(w.cfg.Lxyz)
+    res = {
+        "workload": f"c2 ({w.nparticles} particles, 64^3): FP32 chosen form vs FP64 plain form from the same "
+                    f"loaded particles; and an FP64 plain run from the FP32 store's initial particles",
+    }
+    for name in rows:
+        dx = parts[name][0] - parts["fp64"][0]
+        dx -= L * np.round(dx / L)  # periodic images
+        res[name] = {
+            "rel_l2": rows[name],
+            "positions_rel_l2": float(np.linalg.norm(dx) / np.linalg.norm(parts["fp64"][0])),
+            "velocities_rel_l2": rel(parts[name][1], parts["fp64"][1]),
+        }
+    return res
+
+
 def cpu_baseline(sample_cells: int = 64, sample_steps: int = 10):
     """The reference's serial loop (oracle/_ref) on one core, c2 physics on a
     sample_cells^3 sub-box at the same density (32 ppc/species)."""
@@ -260,6 +350,8 @@ def main():
                     help="first run an elementwise copy of 256 MiB (a counter reference of known bytes)")
     ap.add_argument("--adjoint-steps", type=int, default=0, help="c4 steps for the adjoint leg (0: 400)")
     ap.add_argument("--checkpoint-every", type=int, default=20)
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="c2 steps of the FP32-vs-FP64 field error leg (0: skip)")
     ap.add_argument("--cpu-cells", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=10)
     args = ap.parse_args()
@@ -324,9 +416,13 @@ def main():
     }
     for m, rr in results.items():
         out[f"{m}_particle_steps_per_s"] = round(rr["rate"], 1)
+        out[f"{m}_kernel_ms_per_step"] = {k: round(v[0] / v[1], 5) for k, v in sorted(rr["ktimes"].items())}
         out[f"{m}_yee_cell_updates_per_s"] = round(rr["workload"].ncells * args.steps / rr["elapsed"], 1)
     if "fp32" in results and "fp64" in results:
         out["fp32_over_fp64"] = round(results["fp32"]["rate"] / results["fp64"]["rate"], 3)
+    if args.accuracy_steps > 0 and "fp32" in results:
+        cps = sorted({c for c in (10, 100, args.accuracy_steps) if c <= args.accuracy_steps})
+        out["fp32_accuracy"] = run_accuracy_c2(tuple(cps), args.scale)
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint_c4(args.adjoint_steps or None, args.checkpoint_every)
     if not args.no_cpu_baseline:
