@@ -101,33 +101,6 @@ __global__ void __launch_bounds__(kBlock) k_seed_energy(SeedArgs a) {
 // full plane: the plane's adjoint, weighted by the transverse profile, is the
 // adjoint of the pulse value; the plane's adjoint is then zero.
 // ---------------------------------------------------------------------------
-struct CopyAdjArgs {
-  Geo g;
-  double *f[6];
-  int nf;
-  int d, src, dst;
-};
-
-__global__ void __launch_bounds__(kBlock) k_plane_copy_adj(CopyAdjArgs a) {
-  const Geo &g = a.g;
-  int d = a.d;
-  int u = d == 0 ? 1 : 0;
-  int w = d == 2 ? 1 : 2;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)g.n[u] * g.n[w]) return;
-  int is[3], id[3];
-  is[d] = a.src;
-  id[d] = a.dst;
-  is[u] = id[u] = 1 + (int)(t / g.n[w]);
-  is[w] = id[w] = 1 + (int)(t % g.n[w]);
-  long long ps = gidx(g, is[0], is[1], is[2]);
-  long long pd = gidx(g, id[0], id[1], id[2]);
-  for (int f = 0; f < a.nf; f++) {
-    a.f[f][ps] += a.f[f][pd];
-    a.f[f][pd] = 0.0;
-  }
-}
-
 struct FillAdjArgs {
   Geo g;
   double *f;
@@ -137,41 +110,6 @@ struct FillAdjArgs {
   double Lc[3];
   double *out;  // the weighted plane sum
 };
-
-// one block: fixed-order sum of the plane's adjoint x profile, then zero it
-__global__ void __launch_bounds__(kBlock) k_plane_fill_adj(FillAdjArgs a) {
-  __shared__ double sh[kBlock];
-  const Geo &g = a.g;
-  int d = a.dim;
-  int u = d == 0 ? 1 : 0;
-  int w = d == 2 ? 1 : 2;
-  long long n = (long long)g.nt[u] * g.nt[w];
-  double s = 0.0;
-  for (long long t = threadIdx.x; t < n; t += kBlock) {
-    int pu = (int)(t / g.nt[w]);
-    int pw = (int)(t % g.nt[w]);
-    int ii[3];
-    ii[d] = a.idx;
-    ii[u] = pu;
-    ii[w] = pw;
-    long long p = gidx(g, ii[0], ii[1], ii[2]);
-    double prof = 1.0;
-    if (a.waist > 0) {
-      double ya = g.x0[u] + g.d[u] * (pu - 1) + a.off[u] - a.Lc[u];
-      double yb = g.x0[w] + g.d[w] * (pw - 1) + a.off[w] - a.Lc[w];
-      prof = exp(-(ya * ya + yb * yb) / (a.waist * a.waist));
-    }
-    s += a.f[p] * prof;
-    a.f[p] = 0.0;
-  }
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = kBlock / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *a.out = sh[0];
-}
 
 // ---------------------------------------------------------------------------
 // Yee^T (Grid::evolveFields, src/grid/oGrid.cpp:15-60).  With Bn the array
@@ -267,105 +205,6 @@ struct PartAdj {
   double *x[3], *v[3], *gam, *fld[6];
 };
 
-// deposit^T: the particle reads the current adjoint at its 12 targets
-struct DepAdjArgs {
-  Geo g;
-  PartView s;  // post-BC state in the sorted order
-  PartAdj b;
-  const double *Jb[3];
-  SpeciesParams sp;
-  long long np;
-};
-
-__global__ void __launch_bounds__(kBlock) k_deposit_adj(DepAdjArgs a) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
-  const Geo &g = a.g;
-  const int sp = a.s.sp[i];
-  if (a.sp.test[sp]) return;
-  double pos[3] = {a.s.x[0][i], a.s.x[1][i], a.s.x[2][i]};
-  double vel[3] = {a.s.v[0][i], a.s.v[1][i], a.s.v[2][i]};
-  int ix, iy, iz;
-  if (cell_of(g, pos[0], pos[1], pos[2], &ix, &iy, &iz) < 0) return;
-  double cellverts[3] = {g.x0[0] + g.d[0] * ix, g.x0[1] + g.d[1] * iy, g.x0[2] + g.d[2] * iz};
-  long long idx[12];
-  deposit_target_index(g, ix, iy, iz, idx);
-  double rb[12];
-  for (int e = 0; e < 12; e++) rb[e] = a.Jb[e / 4][idx[e]];
-  double posb[3] = {0, 0, 0}, velb[3] = {0, 0, 0};
-  deposit_row_vjp(g, cellverts, pos, vel, a.sp.q[sp], rb, posb, velb);
-  for (int d = 0; d < 3; d++) {
-    a.b.x[d][i] += posb[d];
-    a.b.v[d][i] += velb[d];
-  }
-}
-
-// sort^T: out[perm[j]] = in[j]
-struct SortAdjArgs {
-  const double *in[7];
-  double *out[7];
-  const uint32_t *perm;
-  long long np;
-};
-
-__global__ void __launch_bounds__(kBlock) k_sort_adj(SortAdjArgs a) {
-  long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.np) return;
-  uint32_t s = a.perm[j];
-  for (int q = 0; q < 7; q++) a.out[q][s] = a.in[q][j];
-}
-
-// BCs^T + push^T from the taped pre-push state; writes the pre-push adjoints
-// and the adjoint of the gathered fields
-struct PushAdjArgs {
-  PartView s;
-  PartAdj b;
-  SpeciesParams sp;
-  double dt;
-  double xmin[3], xmax[3];
-  int pbc[6];
-  int rel;
-  long long np;
-};
-
-__global__ void __launch_bounds__(kBlock) k_push_adj(PushAdjArgs a) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
-  const int sp = a.s.sp[i];
-  const double q = a.sp.q[sp], m = a.sp.m[sp];
-  double x[3] = {a.s.x[0][i], a.s.x[1][i], a.s.x[2][i]};
-  double v[3] = {a.s.v[0][i], a.s.v[1][i], a.s.v[2][i]};
-  double gam = a.s.gam[i];
-  double f[6];
-  for (int k = 0; k < 6; k++) f[k] = a.s.fld[k][i];
-  // forward recomputation up to the BCs (for the reflections)
-  double x1[3] = {x[0], x[1], x[2]}, v1[3] = {v[0], v[1], v[2]}, g1 = gam;
-  if (a.rel) vay_step(x1, v1, g1, f, q, m, a.dt);
-  else boris_step(x1, v1, f, q, m, a.dt);
-  int flips[3];
-  particle_bcs(x1, v1, a.xmin, a.xmax, a.pbc, flips);
-  double xb1[3], vb1[3];
-  for (int d = 0; d < 3; d++) {
-    xb1[d] = a.b.x[d][i];
-    vb1[d] = a.b.v[d][i];
-  }
-  double gb1 = a.b.gam[i];
-  particle_bcs_vjp(flips, xb1, vb1);
-  double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, gb = 0.0, fb[6] = {0, 0, 0, 0, 0, 0};
-  if (a.rel) {
-    vay_step_vjp(v, gam, f, q, m, a.dt, xb1, vb1, gb1, xb, vb, gb, fb);
-  } else {
-    boris_step_vjp(v, f, q, m, a.dt, xb1, vb1, xb, vb, fb);
-    gb = gb1;  // Boris leaves the stored gamma untouched
-  }
-  for (int d = 0; d < 3; d++) {
-    a.b.x[d][i] = xb[d];
-    a.b.v[d][i] = vb[d];
-  }
-  a.b.gam[i] = gb;
-  for (int k = 0; k < 6; k++) a.b.fld[k][i] = fb[k];
-}
-
 // gather^T, stage 1: one wavefront per cell; each particle's position
 // adjoint, and the cell's 18 field-entry adjoints summed in array order
 // (rows in LDS, lanes 0..17 add columns row by row)
@@ -378,40 +217,60 @@ struct GatherAdjArgs {
   double *cg;  // 18 per cell, component-major
 };
 
+constexpr int kGLanes = 16;                 // lanes per cell
+constexpr int kGCells = kBlock / kGLanes;   // cells per block
 constexpr int kGStride = 19;
 
+// sixteen lanes per cell, four cells per wavefront: batches of 16
+// consecutive particles; lane l then adds column l (and 16 + l) of the batch
+// row by row (rows past the cell's end are zero)
 __global__ void __launch_bounds__(kBlock) k_gather_adj_cells(GatherAdjArgs a) {
-  __shared__ double rows[kWaves][64 * kGStride];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ double rows[kGCells][kGLanes * kGStride];
+  const int l = threadIdx.x & (kGLanes - 1);
+  const int slot = threadIdx.x / kGLanes;
   const Geo &g = a.g;
-  const long long c = (long long)blockIdx.x * kWaves + w;
-  if (c >= g.ncell) return;
-  double *row = rows[w];
-  const int b = a.off[c], e = a.off[c + 1];
-  double acc = 0.0;
-  for (int base = b; base < e; base += 64) {
-    const int p = base + lane;
+  const long long c = (long long)blockIdx.x * kGCells + slot;
+  const bool cell_ok = c < g.ncell;
+  double *row = rows[slot];
+  int b = 0, e = 0;
+  if (cell_ok) {
+    b = a.off[c];
+    e = a.off[c + 1];
+  }
+  int nb = (e - b + kGLanes - 1) / kGLanes;
+  for (int o = kGLanes; o < 64; o <<= 1) nb = max(nb, __shfl_xor(nb, o));
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int t = 0; t < nb; t++) {
+    const int p = b + t * kGLanes + l;
+    double cvb[21];
+#pragma unroll
+    for (int q = 0; q < 21; q++) cvb[q] = 0.0;
     if (p < e) {
       double pos[3] = {a.s.x[0][p], a.s.x[1][p], a.s.x[2][p]};
       int ix, iy, iz;
       cell_of(g, pos[0], pos[1], pos[2], &ix, &iy, &iz);
-      double cv[21], cvb[21], fb[6], posb[3] = {0, 0, 0};
+      double cv[21], fb[6], posb[3] = {0, 0, 0};
       cell_vector(g, a.E, a.B, a.Bt, ix, iy, iz, cv);
-      for (int q = 0; q < 21; q++) cvb[q] = 0.0;
       for (int q = 0; q < 6; q++) fb[q] = a.b.fld[q][p];
       interp_fields_vjp(g, cv, pos, fb, posb, cvb);
       for (int d = 0; d < 3; d++) a.b.x[d][p] += posb[d];
-      double *r = row + lane * kGStride;
+    }
+    double *r = row + l * kGStride;
 #pragma unroll
-      for (int q = 0; q < 18; q++) r[q] = cvb[3 + q];
+    for (int q = 0; q < 18; q++) r[q] = cvb[3 + q];
+    wave_sync();
+#pragma unroll
+    for (int rr = 0; rr < kGLanes; rr++) acc0 += row[rr * kGStride + l];
+    if (l < 2) {
+#pragma unroll
+      for (int rr = 0; rr < kGLanes; rr++) acc1 += row[rr * kGStride + 16 + l];
     }
     wave_sync();
-    const int cnt = min(64, e - base);
-    if (lane < 18)
-      for (int rr = 0; rr < cnt; rr++) acc += row[rr * kGStride + lane];
-    wave_sync();
   }
-  if (lane < 18) a.cg[lane * g.ncell + c] = acc;
+  if (cell_ok) {
+    a.cg[l * g.ncell + c] = acc0;
+    if (l < 2) a.cg[(16 + l) * g.ncell + c] = acc1;
+  }
 }
 
 // gather^T, stage 2: every point adds the entries of the cells whose
@@ -458,6 +317,192 @@ __global__ void __launch_bounds__(kBlock) k_gather_adj_grid(GatherGridArgs a) {
 __global__ void k_zero(double *p, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched device copies (the tape): up to kMaxCopies arrays in one launch,
+// 16 bytes per thread.  Every array starts 16-byte aligned and holds a whole
+// number of 4-byte words.
+// ---------------------------------------------------------------------------
+constexpr int kMaxCopies = 32;
+struct CopyList {
+  const uint32_t *src[kMaxCopies];
+  uint32_t *dst[kMaxCopies];
+  long long words[kMaxCopies];
+  long long start[kMaxCopies + 1];  // 16-byte chunks before array i
+  int n;
+};
+
+__global__ void __launch_bounds__(kBlock) k_copy_many(CopyList a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.start[a.n]) return;
+  int s = 0;
+  while (t >= a.start[s + 1]) s++;
+  long long w = (t - a.start[s]) * 4;
+  const long long nw = a.words[s];
+  if (w + 4 <= nw) {
+    *(uint4 *)(a.dst[s] + w) = *(const uint4 *)(a.src[s] + w);
+  } else {
+    for (; w < nw; w++) a.dst[s][w] = a.src[s][w];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// refresh^T of one run of periodic sides (one forward launch), gather form:
+// a thread per physical point adds the ghost adjoints of the sides whose
+// source plane holds it, in reverse side order, and zeroes those ghosts
+// (each ghost point has exactly one source point).
+// ---------------------------------------------------------------------------
+struct SidesAdjArgs {
+  Geo g;
+  double *f[6];
+  int nsides;                      // in reverse forward order
+  int d[6], src[6], dst[6];
+};
+
+__global__ void __launch_bounds__(kBlock) k_copy_sides_adj(SidesAdjArgs a) {
+  const Geo &g = a.g;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
+  const int at[3] = {1 + (int)(t / ((long long)g.n[1] * g.n[2])), 1 + (int)((t / g.n[2]) % g.n[1]),
+                     1 + (int)(t % g.n[2])};
+  bool any = false;
+  for (int s = 0; s < a.nsides; s++) any |= at[a.d[s]] == a.src[s];
+  if (!any) return;
+  const long long p = gidx(g, at[0], at[1], at[2]);
+  for (int f = 0; f < 6; f++) {
+    double val = a.f[f][p];
+    for (int s = 0; s < a.nsides; s++) {
+      if (at[a.d[s]] != a.src[s]) continue;
+      int gh[3] = {at[0], at[1], at[2]};
+      gh[a.d[s]] = a.dst[s];
+      const long long q = gidx(g, gh[0], gh[1], gh[2]);
+      val += a.f[f][q];
+      a.f[f][q] = 0.0;
+    }
+    a.f[f][p] = val;
+  }
+}
+
+// fill^T of the (up to six) planes of one LightBC side: one block per plane
+struct FillAdjMulti {
+  FillAdjArgs p[6];
+};
+
+__global__ void __launch_bounds__(kBlock) k_plane_fill_adj_multi(FillAdjMulti m) {
+  __shared__ double sh[kBlock];
+  const FillAdjArgs &a = m.p[blockIdx.x];
+  const Geo &g = a.g;
+  int d = a.dim;
+  int u = d == 0 ? 1 : 0;
+  int w = d == 2 ? 1 : 2;
+  long long n = (long long)g.nt[u] * g.nt[w];
+  double s = 0.0;
+  for (long long t = threadIdx.x; t < n; t += kBlock) {
+    int pu = (int)(t / g.nt[w]);
+    int pw = (int)(t % g.nt[w]);
+    int ii[3];
+    ii[d] = a.idx;
+    ii[u] = pu;
+    ii[w] = pw;
+    long long p = gidx(g, ii[0], ii[1], ii[2]);
+    double prof = 1.0;
+    if (a.waist > 0) {
+      double ya = g.x0[u] + g.d[u] * (pu - 1) + a.off[u] - a.Lc[u];
+      double yb = g.x0[w] + g.d[w] * (pw - 1) + a.off[w] - a.Lc[w];
+      prof = exp(-(ya * ya + yb * yb) / (a.waist * a.waist));
+    }
+    s += a.f[p] * prof;
+    a.f[p] = 0.0;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = kBlock / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *a.out = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// deposit^T + sort^T + BCs^T + push^T in one pass over the sorted order j:
+// the current adjoint is read at the particle's deposit targets, the
+// particle's adjoint moves to its pre-sort index perm[j], and the push is
+// transposed there from the taped pre-push state.
+// ---------------------------------------------------------------------------
+struct PartStepAdjArgs {
+  Geo g;
+  PartView post;  // post-BC state, sorted order (x, v, sp)
+  PartView pre;   // pre-push state, previous order (x, v, gam, fld, sp)
+  const double *bx[3], *bv[3], *bg;  // adjoint, sorted order (in)
+  double *ox[3], *ov[3], *og;        // adjoint, previous order (out)
+  double *of[6];                     // field adjoint, previous order (out)
+  const double *Jb[3];
+  const uint32_t *perm;
+  SpeciesParams sp;
+  double dt;
+  double xmin[3], xmax[3];
+  int pbc[6];
+  int rel;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_particle_step_adj(PartStepAdjArgs a) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.np) return;
+  const Geo &g = a.g;
+  double xb1[3], vb1[3];
+  for (int d = 0; d < 3; d++) {
+    xb1[d] = a.bx[d][j];
+    vb1[d] = a.bv[d][j];
+  }
+  double gb1 = a.bg[j];
+  const int sp = a.post.sp[j];
+  const double q = a.sp.q[sp], m = a.sp.m[sp];
+  if (!a.sp.test[sp]) {
+    double pos[3] = {a.post.x[0][j], a.post.x[1][j], a.post.x[2][j]};
+    double vel[3] = {a.post.v[0][j], a.post.v[1][j], a.post.v[2][j]};
+    int ix, iy, iz;
+    if (cell_of(g, pos[0], pos[1], pos[2], &ix, &iy, &iz) >= 0) {
+      double cellverts[3] = {g.x0[0] + g.d[0] * ix, g.x0[1] + g.d[1] * iy, g.x0[2] + g.d[2] * iz};
+      long long idx[12];
+      deposit_target_index(g, ix, iy, iz, idx);
+      double rb[12];
+      for (int e = 0; e < 12; e++) rb[e] = a.Jb[e / 4][idx[e]];
+      double posb[3] = {0, 0, 0}, velb[3] = {0, 0, 0};
+      deposit_row_vjp(g, cellverts, pos, vel, q, rb, posb, velb);
+      for (int d = 0; d < 3; d++) {
+        xb1[d] += posb[d];
+        vb1[d] += velb[d];
+      }
+    }
+  }
+  const long long i = a.perm[j];
+  double x[3] = {a.pre.x[0][i], a.pre.x[1][i], a.pre.x[2][i]};
+  double v[3] = {a.pre.v[0][i], a.pre.v[1][i], a.pre.v[2][i]};
+  double gam = a.pre.gam[i];
+  double f[6];
+  for (int k = 0; k < 6; k++) f[k] = a.pre.fld[k][i];
+  // forward recomputation up to the BCs (for the reflections)
+  double x1[3] = {x[0], x[1], x[2]}, v1[3] = {v[0], v[1], v[2]}, g1 = gam;
+  if (a.rel) vay_step(x1, v1, g1, f, q, m, a.dt);
+  else boris_step(x1, v1, f, q, m, a.dt);
+  int flips[3];
+  particle_bcs(x1, v1, a.xmin, a.xmax, a.pbc, flips);
+  particle_bcs_vjp(flips, xb1, vb1);
+  double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, gb = 0.0, fb[6] = {0, 0, 0, 0, 0, 0};
+  if (a.rel) {
+    vay_step_vjp(v, gam, f, q, m, a.dt, xb1, vb1, gb1, xb, vb, gb, fb);
+  } else {
+    boris_step_vjp(v, f, q, m, a.dt, xb1, vb1, xb, vb, fb);
+    gb = gb1;  // Boris leaves the stored gamma untouched
+  }
+  for (int d = 0; d < 3; d++) {
+    a.ox[d][i] = xb[d];
+    a.ov[d][i] = vb[d];
+  }
+  a.og[i] = gb;
+  for (int k = 0; k < 6; k++) a.of[k][i] = fb[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -537,44 +582,72 @@ void snap_free(Snap &s) {
 #define ADJ_CPY(dst, src, bytes) \
   EMO_HIP_CHECK_RET(c, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToDevice, c->stream))
 
+// Collects device copies and issues them as k_copy_many launches.
+struct CopyBatch {
+  emo_ctx *c;
+  CopyList l;
+  explicit CopyBatch(emo_ctx *ctx) : c(ctx) {
+    l.n = 0;
+    l.start[0] = 0;
+  }
+  void add(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return;
+    if (l.n == kMaxCopies) flush();
+    l.src[l.n] = (const uint32_t *)src;
+    l.dst[l.n] = (uint32_t *)dst;
+    l.words[l.n] = (long long)(bytes / 4);
+    l.start[l.n + 1] = l.start[l.n] + (l.words[l.n] + 3) / 4;
+    l.n++;
+  }
+  void flush() {
+    if (!l.n) return;
+    hipLaunchKernelGGL(k_copy_many, dim3(grid_for(l.start[l.n])), dim3(kBlock), 0, c->stream, l);
+    l.n = 0;
+  }
+};
+
 // ctx -> snapshot (particles, fields, ranges)
 int snap_take(emo_ctx *c, Snap &s, double t) {
   const size_t n8 = (size_t)c->np * 8, n4 = (size_t)c->np * 4, nt8 = (size_t)c->geo.ntot * 8;
+  CopyBatch cb(c);
   for (int d = 0; d < 3; d++) {
-    ADJ_CPY(s.x[d], c->p.x[d], n8);
-    ADJ_CPY(s.v[d], c->p.v[d], n8);
-    ADJ_CPY(s.E[d], c->E[d], nt8);
-    ADJ_CPY(s.A[0][d], c->Bfix[d], nt8);
-    ADJ_CPY(s.A[1][d], c->Btfix[d], nt8);
+    cb.add(s.x[d], c->p.x[d], n8);
+    cb.add(s.v[d], c->p.v[d], n8);
+    cb.add(s.E[d], c->E[d], nt8);
+    cb.add(s.A[0][d], c->Bfix[d], nt8);
+    cb.add(s.A[1][d], c->Btfix[d], nt8);
   }
-  ADJ_CPY(s.gam, c->p.gam, n8);
-  for (int k = 0; k < 6 && s.fld[k]; k++) ADJ_CPY(s.fld[k], c->p.fld[k], n8);
-  ADJ_CPY(s.sp, c->p.sp, n4);
-  if (s.id) ADJ_CPY(s.id, c->p.id, (size_t)c->np * 8);
-  for (int d = 0; d < 3 && s.rho; d++) ADJ_CPY(s.J[d], c->J[d], nt8);
-  if (s.rho) ADJ_CPY(s.rho, c->rho, nt8);
-  ADJ_CPY(s.off, c->srt.off, (size_t)(c->geo.ncell + 1) * 4);
+  cb.add(s.gam, c->p.gam, n8);
+  cb.add(s.sp, c->p.sp, n4);
+  if (s.id) cb.add(s.id, c->p.id, (size_t)c->np * 8);
+  for (int d = 0; d < 3 && s.rho; d++) cb.add(s.J[d], c->J[d], nt8);
+  if (s.rho) cb.add(s.rho, c->rho, nt8);
+  cb.add(s.off, c->srt.off, (size_t)(c->geo.ncell + 1) * 4);
+  cb.flush();
   s.b_is_a0 = c->B[0] == c->Bfix[0];
   s.t = t;
   return EMO_OK;
 }
 
-// checkpoint -> ctx
+// checkpoint -> ctx (the particles' gathered fields are not restored: the
+// fused step gathers them from the restored grid)
 int snap_restore(emo_ctx *c, const Snap &s) {
   const size_t n8 = (size_t)c->np * 8, n4 = (size_t)c->np * 4, nt8 = (size_t)c->geo.ntot * 8;
+  CopyBatch cb(c);
   for (int d = 0; d < 3; d++) {
-    ADJ_CPY(c->p.x[d], s.x[d], n8);
-    ADJ_CPY(c->p.v[d], s.v[d], n8);
-    ADJ_CPY(c->E[d], s.E[d], nt8);
-    ADJ_CPY(c->Bfix[d], s.A[0][d], nt8);
-    ADJ_CPY(c->Btfix[d], s.A[1][d], nt8);
+    cb.add(c->p.x[d], s.x[d], n8);
+    cb.add(c->p.v[d], s.v[d], n8);
+    cb.add(c->E[d], s.E[d], nt8);
+    cb.add(c->Bfix[d], s.A[0][d], nt8);
+    cb.add(c->Btfix[d], s.A[1][d], nt8);
   }
-  ADJ_CPY(c->p.gam, s.gam, n8);
-  ADJ_CPY(c->p.sp, s.sp, n4);
-  if (s.id) ADJ_CPY(c->p.id, s.id, (size_t)c->np * 8);
-  for (int d = 0; d < 3 && s.rho; d++) ADJ_CPY(c->J[d], s.J[d], nt8);
-  if (s.rho) ADJ_CPY(c->rho, s.rho, nt8);
-  ADJ_CPY(c->srt.off, s.off, (size_t)(c->geo.ncell + 1) * 4);
+  cb.add(c->p.gam, s.gam, n8);
+  cb.add(c->p.sp, s.sp, n4);
+  if (s.id) cb.add(c->p.id, s.id, (size_t)c->np * 8);
+  for (int d = 0; d < 3 && s.rho; d++) cb.add(c->J[d], s.J[d], nt8);
+  if (s.rho) cb.add(c->rho, s.rho, nt8);
+  cb.add(c->srt.off, s.off, (size_t)(c->geo.ncell + 1) * 4);
+  cb.flush();
   for (int q = 0; q < 3; q++) {
     c->B[q] = s.b_is_a0 ? c->Bfix[q] : c->Btfix[q];
     c->Bt[q] = s.b_is_a0 ? c->Btfix[q] : c->Bfix[q];
@@ -590,14 +663,13 @@ int snap_restore(emo_ctx *c, const Snap &s) {
 int forward_step(emo_ctx *c, double t, double dt, Snap *tape) {
   int rc;
   if (c->np > 0) {
-    fused_push_f64(c, dt, tape ? 1 : 0);
+    fused_push_f64(c, dt, tape ? tape->fld : nullptr);
     if (tape) {
       rc = snap_take(c, *tape, t);
       if (rc) return rc;
     }
-    rc = fused_sort_f64(c);
+    rc = fused_sort_f64(c, tape ? tape->perm : nullptr);
     if (rc) return rc;
-    if (tape) ADJ_CPY(tape->perm, c->srt.perm, (size_t)c->np * 4);
   } else {
     if (tape) {
       rc = snap_take(c, *tape, t);
@@ -722,7 +794,7 @@ void gather_adj(emo_ctx *c, const Snap &s, PartAdj &pb, AdjFields &af) {
   a.cg = af.cg;
   {
     TimedLaunch t(c, "adj_gather_cells");
-    hipLaunchKernelGGL(k_gather_adj_cells, dim3((unsigned)((g.ncell + kWaves - 1) / kWaves)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_gather_adj_cells, dim3((unsigned)((g.ncell + kGCells - 1) / kGCells)), dim3(kBlock), 0,
                        c->stream, a);
   }
   GatherGridArgs b;
@@ -867,29 +939,34 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
       const Snap &post = tape[s + 1 - s0];  // state after step s (its gather)
       // gather^T of step s's gather (needs the fields' adjoint of push s+1)
       if (s < nsteps - 1) gather_adj(c, post, pb, af);
-      // refresh^T: sides in reverse order
+      // refresh^T, the forward's launches in reverse: a LightBC side's six
+      // fills as one launch, a run of periodic sides as one gather pass
       const int mnew = pre.b_is_a0 ? 1 : 0;  // the array the B update writes
       int pi = nplanes - 1;
-      for (int side = 5; side >= 0; side--) {
-        const int d = side / 2, right = side & 1;
+      for (int side = 5; side >= 0;) {
         if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
-          CopyAdjArgs ca;
-          ca.g = g;
+          SidesAdjArgs sa;
+          sa.g = g;
           for (int q = 0; q < 3; q++) {
-            ca.f[q] = af.Eb[q];
-            ca.f[3 + q] = af.Ab[0][q];
+            sa.f[q] = af.Eb[q];
+            sa.f[3 + q] = af.Ab[0][q];  // Q23: the refresh writes B through Bfix
           }
-          ca.nf = 6;
-          ca.d = d;
-          ca.src = right ? 1 : g.n[d];
-          ca.dst = right ? g.n[d] + 1 : 0;
-          int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
-          hipLaunchKernelGGL(k_plane_copy_adj, dim3(grid_for((long long)g.n[u] * g.n[w])), dim3(kBlock), 0,
-                             c->stream, ca);
+          sa.nsides = 0;
+          for (; side >= 0 && c->cfg.fields_bound[side] == EMO_BC_PERIODIC; side--) {
+            const int d = side / 2, right = side & 1;
+            sa.d[sa.nsides] = d;
+            sa.src[sa.nsides] = right ? 1 : g.n[d];
+            sa.dst[sa.nsides] = right ? g.n[d] + 1 : 0;
+            sa.nsides++;
+          }
+          hipLaunchKernelGGL(k_copy_sides_adj, dim3(grid_for((long long)g.n[0] * g.n[1] * g.n[2])), dim3(kBlock),
+                             0, c->stream, sa);
         } else {
+          FillAdjMulti fm;
+          int nb = 0;
           for (; pi >= 0 && planes[pi].side == side; pi--) {
             const FillPlane &f = planes[pi];
-            FillAdjArgs fa;
+            FillAdjArgs &fa = fm.p[nb++];
             fa.g = g;
             fa.f = f.pass == 0 ? af.Eb[f.comp] : af.Ab[mnew][f.comp];
             fa.dim = f.dim;
@@ -900,8 +977,10 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
               fa.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
             }
             fa.out = dsums + (size_t)s * nplanes + pi;
-            hipLaunchKernelGGL(k_plane_fill_adj, dim3(1), dim3(kBlock), 0, c->stream, fa);
           }
+          if (nb)
+            hipLaunchKernelGGL(k_plane_fill_adj_multi, dim3(nb), dim3(kBlock), 0, c->stream, fm);
+          side--;
         }
       }
       // Yee^T
@@ -926,67 +1005,46 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
         double *f[2] = {d == 0 ? af.Jb[1] : af.Jb[0], d == 2 ? af.Jb[1] : af.Jb[2]};
         fk_plane_copy<double>(c, f, 2, d, 1, g.n[d] + 1, 1);
       }
-      // deposit^T on the post-BC (sorted) state
-      DepAdjArgs da;
-      da.g = g;
+      // deposit^T + sort^T + BCs^T + push^T
+      PartStepAdjArgs pa;
+      pa.g = g;
       for (int d = 0; d < 3; d++) {
-        da.s.x[d] = post.x[d];
-        da.s.v[d] = post.v[d];
-        da.b.x[d] = pb.x[d];
-        da.b.v[d] = pb.v[d];
-        da.Jb[d] = af.Jb[d];
+        pa.post.x[d] = post.x[d];
+        pa.post.v[d] = post.v[d];
+        pa.pre.x[d] = pre.x[d];
+        pa.pre.v[d] = pre.v[d];
+        pa.bx[d] = pb.x[d];
+        pa.bv[d] = pb.v[d];
+        pa.ox[d] = pb2.x[d];
+        pa.ov[d] = pb2.v[d];
+        pa.Jb[d] = af.Jb[d];
+        pa.xmin[d] = c->cfg.xyz0[d];
+        pa.xmax[d] = pa.xmin[d] + c->cfg.Lxyz[d];
       }
-      da.s.sp = post.sp;
-      da.sp = c->species;
-      da.np = np;
+      pa.post.sp = post.sp;
+      pa.pre.sp = pre.sp;
+      pa.pre.gam = pre.gam;
+      for (int k = 0; k < 6; k++) {
+        pa.pre.fld[k] = pre.fld[k];
+        pa.of[k] = pb.fld[k];
+      }
+      pa.bg = pb.gam;
+      pa.og = pb2.gam;
+      pa.perm = pre.perm;
+      pa.sp = c->species;
+      pa.dt = dt;
+      for (int k = 0; k < 6; k++) pa.pbc[k] = c->cfg.parts_bound[k];
+      pa.rel = c->cfg.relativity;
+      pa.np = np;
       {
-        TimedLaunch tl(c, "adj_deposit");
-        hipLaunchKernelGGL(k_deposit_adj, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, da);
+        TimedLaunch tl(c, "adj_particles");
+        hipLaunchKernelGGL(k_particle_step_adj, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, pa);
       }
-      // sort^T
-      SortAdjArgs so;
-      for (int d = 0; d < 3; d++) {
-        so.in[d] = pb.x[d];
-        so.in[3 + d] = pb.v[d];
-        so.out[d] = pb2.x[d];
-        so.out[3 + d] = pb2.v[d];
-      }
-      so.in[6] = pb.gam;
-      so.out[6] = pb2.gam;
-      so.perm = pre.perm;
-      so.np = np;
-      hipLaunchKernelGGL(k_sort_adj, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, so);
       for (int d = 0; d < 3; d++) {
         std::swap(pb.x[d], pb2.x[d]);
         std::swap(pb.v[d], pb2.v[d]);
       }
       std::swap(pb.gam, pb2.gam);
-      // BCs^T + push^T
-      PushAdjArgs pa;
-      for (int d = 0; d < 3; d++) {
-        pa.s.x[d] = pre.x[d];
-        pa.s.v[d] = pre.v[d];
-        pa.b.x[d] = pb.x[d];
-        pa.b.v[d] = pb.v[d];
-        pa.xmin[d] = c->cfg.xyz0[d];
-        pa.xmax[d] = pa.xmin[d] + c->cfg.Lxyz[d];
-      }
-      pa.s.gam = pre.gam;
-      pa.b.gam = pb.gam;
-      for (int k = 0; k < 6; k++) {
-        pa.s.fld[k] = pre.fld[k];
-        pa.b.fld[k] = pb.fld[k];
-      }
-      pa.s.sp = pre.sp;
-      for (int k = 0; k < 6; k++) pa.pbc[k] = c->cfg.parts_bound[k];
-      pa.sp = c->species;
-      pa.dt = dt;
-      pa.rel = c->cfg.relativity;
-      pa.np = np;
-      {
-        TimedLaunch tl(c, "adj_push");
-        hipLaunchKernelGGL(k_push_adj, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, pa);
-      }
     }
   }
   // gather^T of the gather that produced the first push's fields

@@ -268,7 +268,8 @@ struct BaseArgs {
 };
 
 // One thread per destination cell: its (source, code) groups in ascending
-// source id, far movers merged in by source id.
+// source id, far movers merged in by source id.  The group sizes of all
+// candidates are loaded first (independent loads), then walked in order.
 __global__ void __launch_bounds__(kBlock) k_base(BaseArgs a) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const Geo &g = a.g;
@@ -279,38 +280,55 @@ __global__ void __launch_bounds__(kBlock) k_base(BaseArgs a) {
   cand1(cx, g.n[0], a.per[0], X);
   cand1(cy, g.n[1], a.per[1], Y);
   cand1(cz, g.n[2], a.per[2], Z);
+  // the valid candidates of each dimension, in ascending source index
+  int xs[4], xk[4], ys[4], yk[4], zs[4], zk[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    cand1_pick(X, q, xs[q], xk[q]);
+    cand1_pick(Y, q, ys[q], yk[q]);
+    cand1_pick(Z, q, zs[q], zk[q]);
+  }
+  const uint16_t *__restrict__ nsk_in = a.nsk;
+  int nn[64];
+#pragma unroll
+  for (int ia = 0; ia < 4; ia++)
+#pragma unroll
+    for (int ib = 0; ib < 4; ib++)
+#pragma unroll
+      for (int ic = 0; ic < 4; ic++) {
+        const int slot = (ia * 4 + ib) * 4 + ic;
+        nn[slot] = 0;
+        if (ia < X.n && ib < Y.n && ic < Z.n) {
+          const int s = (xs[ia] * g.nc[1] + ys[ib]) * g.nc[2] + zs[ic];
+          nn[slot] = nsk_in[(long long)((xk[ia] * 6 + yk[ib]) * 6 + zk[ic]) * g.ncell + s];
+        }
+      }
   const int nf = min(*a.nfar, a.far_cap);
   int fp = 0;
   int run = 0;
 #pragma unroll
-  for (int ox = 0; ox < 6; ox++) {
-    if (!X.ok[ox]) continue;
+  for (int ia = 0; ia < 4; ia++)
 #pragma unroll
-    for (int oy = 0; oy < 6; oy++) {
-      if (!Y.ok[oy]) continue;
+    for (int ib = 0; ib < 4; ib++)
 #pragma unroll
-      for (int oz = 0; oz < 6; oz++) {
-        if (!Z.ok[oz]) continue;
-        const int kx = ox == 0 ? 3 : ox == 1 ? 2 : ox == 2 ? 1 : ox == 3 ? 0 : ox == 4 ? 4 : 5;
-        const int ky = oy == 0 ? 3 : oy == 1 ? 2 : oy == 2 ? 1 : oy == 3 ? 0 : oy == 4 ? 4 : 5;
-        const int kz = oz == 0 ? 3 : oz == 1 ? 2 : oz == 2 ? 1 : oz == 3 ? 0 : oz == 4 ? 4 : 5;
-        const int s = (X.src[ox] * g.nc[1] + Y.src[oy]) * g.nc[2] + Z.src[oz];
-        const long long idx = (long long)((kx * 6 + ky) * 6 + kz) * g.ncell + s;
+      for (int ic = 0; ic < 4; ic++) {
+        const int slot = (ia * 4 + ib) * 4 + ic;
+        if (!(ia < X.n && ib < Y.n && ic < Z.n)) continue;
+        const int s = (xs[ia] * g.nc[1] + ys[ib]) * g.nc[2] + zs[ic];
+        const long long idx = (long long)((xk[ia] * 6 + yk[ib]) * 6 + zk[ic]) * g.ncell + s;
         while (fp < nf) {
           const int4 e = a.far[fp];
           if (e.y >= s) break;
           if (e.z == (int)c) a.far_pos[fp] = run++;
           fp++;
         }
-        const int n = a.nsk[idx];
+        const int n = nn[slot];
         if (n) {
           a.nsk[idx] = 0;
           a.base[idx] = (uint16_t)min(run, kMaxGroup);
           run += n;
         }
       }
-    }
-  }
   for (; fp < nf; fp++)
     if (a.far[fp].z == (int)c) a.far_pos[fp] = run++;
   if (run > kMaxGroup) atomicOr(a.flag, kFlagMigrate);

@@ -637,8 +637,8 @@ int emo_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
       // per-particle fields equal (fresh) or would equal (deferred); the
       // store is in cell order, so the counting sort keeps the radix
       // sort's stable order
-      fused_push_f64(c, dt, 0);
-      rc = fused_sort_f64(c);
+      fused_push_f64(c, dt, nullptr);
+      rc = fused_sort_f64(c, nullptr);
       if (rc) return rc;
     } else {
       fld_materialize(c);

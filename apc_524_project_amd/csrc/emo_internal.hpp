@@ -160,8 +160,8 @@ void launch_particle_bc_f64(emo_ctx *c);
 int sort_f64(emo_ctx *c, int with_fields);
 int plain_sort_create(emo_ctx *c);
 void plain_sort_destroy(emo_ctx *c);
-void fused_push_f64(emo_ctx *c, double dt, int write_fld);
-int fused_sort_f64(emo_ctx *c);
+void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out);
+int fused_sort_f64(emo_ctx *c, uint32_t *perm_out);
 constexpr int kFldFresh = 0, kFldStale = 1, kFldDeferred = 2;
 // stores the deferred gather (kFldDeferred -> kFldFresh)
 void fld_materialize(emo_ctx *c);

@@ -587,9 +587,9 @@ void plain_sort_destroy(emo_ctx *c) { cellsort_free(c->srt); }
 
 // The fused particle phase of the plain form on a cell-sorted store, part 1:
 // push (gathering the fields from the grid) and BCs into palt, with the move
-// code of every particle.  With write_fld the gathered fields are stored in
-// p.fld (the pre-push order).
-void fused_push_f64(emo_ctx *c, double dt, int write_fld) {
+// code of every particle.  With fld_out the gathered fields are stored there
+// (six arrays, the pre-push order).
+void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out) {
   const Geo &g = c->geo;
   const long long np = c->np;
   Pgd64Args a;
@@ -606,7 +606,7 @@ void fused_push_f64(emo_ctx *c, double dt, int write_fld) {
     a.xmax[d] = a.xmin[d] + c->cfg.Lxyz[d];
     a.per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
   }
-  for (int q = 0; q < 6; q++) a.fld[q] = c->p.fld[q];
+  for (int q = 0; q < 6; q++) a.fld[q] = fld_out ? fld_out[q] : nullptr;
   a.gam = c->p.gam;
   a.go = c->palt.gam;
   a.sp = c->p.sp;
@@ -615,7 +615,7 @@ void fused_push_f64(emo_ctx *c, double dt, int write_fld) {
   for (int s = 0; s < 6; s++) a.pbc[s] = c->cfg.parts_bound[s];
   a.code = c->srt.code;
   a.nfar = c->srt.misc;
-  a.write_fld = write_fld;
+  a.write_fld = fld_out != nullptr;
   a.flag = c->dflag;
   a.np = np;
   {
@@ -628,9 +628,10 @@ void fused_push_f64(emo_ctx *c, double dt, int write_fld) {
 }
 
 // part 2: the stable counting sort by the new cell and the scatter into the
-// main store; the permutation (new position -> previous index) is left in
-// c->srt.perm and the new cell ranges in c->srt.off.
-int fused_sort_f64(emo_ctx *c) {
+// main store; the permutation (new position -> previous index) goes to
+// perm_out (c->srt.perm if null) and the new cell ranges to c->srt.off.
+int fused_sort_f64(emo_ctx *c, uint32_t *perm_out) {
+  if (!perm_out) perm_out = c->srt.perm;
   const Geo &g = c->geo;
   const long long np = c->np;
   static const char *const names[4] = {"far_f64", "rank_f64", "base_f64", "scan_f64"};
@@ -673,7 +674,7 @@ int fused_sort_f64(emo_ctx *c) {
     std::swap(c->p.gam, c->palt.gam);
     rc = sort_f64(c, 0);
     if (rc) return rc;
-    if (hipMemcpyAsync(c->srt.perm, c->perm_alt, (size_t)np * 4, hipMemcpyDeviceToDevice, c->stream) !=
+    if (hipMemcpyAsync(perm_out, c->perm_alt, (size_t)np * 4, hipMemcpyDeviceToDevice, c->stream) !=
         hipSuccess) {
       c->err = "permutation copy failed";
       return EMO_ERR_HIP;
@@ -696,7 +697,7 @@ int fused_sort_f64(emo_ctx *c) {
   sa.idi = c->p.id;
   sa.spo = c->palt.sp;
   sa.ido = c->palt.id;
-  sa.perm = c->srt.perm;
+  sa.perm = perm_out;
   sa.flag = c->dflag;
   sa.np = np;
   {
