@@ -508,20 +508,18 @@ __global__ void __launch_bounds__(kBlock) k_particle_step_adj(PartStepAdjArgs a)
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-template <typename T>
-int dalloc(emo_ctx *c, T **p, size_t n) {
-  *p = nullptr;
-  if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
-    *p = nullptr;
-    c->err = "hipMalloc failed in the adjoint";
-    return EMO_ERR_ALLOC;
+// Every buffer of a gradient call comes from one device block that the
+// context keeps between calls; a pass without a base sizes it.
+struct Pool {
+  char *base = nullptr;
+  size_t off = 0;
+  template <typename T>
+  void take(T **p, size_t n) {
+    const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+    *p = base ? (T *)(base + off) : nullptr;
+    off += bytes;
   }
-  return EMO_OK;
-}
-
-void dfree(void *p) {
-  if (p) hipFree(p);
-}
+};
 
 // One taped state: the particles as the push of a step reads them (the
 // state right after the previous gather), the fields at that time, the cell
@@ -538,49 +536,25 @@ struct Snap {
   double t = 0.0;
 };
 
-int snap_alloc(emo_ctx *c, Snap &s, bool ids, bool perm, bool currents = false) {
-  const int64_t n = std::max<int64_t>(c->np, 1);
+void snap_alloc(Pool &P, const emo_ctx *c, Snap &s, bool ids, bool perm, bool currents = false) {
+  const size_t n = (size_t)std::max<int64_t>(c->np, 1);
   const size_t nt = (size_t)c->geo.ntot;
-  int rc = 0;
-  for (int d = 0; d < 3 && !rc; d++) {
-    rc = dalloc(c, &s.x[d], n);
-    if (!rc) rc = dalloc(c, &s.v[d], n);
-    if (!rc) rc = dalloc(c, &s.E[d], nt);
-    if (!rc) rc = dalloc(c, &s.A[0][d], nt);
-    if (!rc) rc = dalloc(c, &s.A[1][d], nt);
-  }
-  if (!rc) rc = dalloc(c, &s.gam, n);
-  for (int k = 0; k < 6 && !rc && perm; k++) rc = dalloc(c, &s.fld[k], n);
-  if (!rc) rc = dalloc(c, &s.sp, n);
-  if (!rc && ids) rc = dalloc(c, &s.id, n);
-  if (!rc) rc = dalloc(c, &s.off, c->geo.ncell + 1);
-  if (!rc && perm) rc = dalloc(c, &s.perm, n);
-  for (int d = 0; d < 3 && !rc && currents; d++) rc = dalloc(c, &s.J[d], nt);
-  if (!rc && currents) rc = dalloc(c, &s.rho, nt);
-  return rc;
-}
-
-void snap_free(Snap &s) {
   for (int d = 0; d < 3; d++) {
-    dfree(s.x[d]);
-    dfree(s.v[d]);
-    dfree(s.E[d]);
-    dfree(s.A[0][d]);
-    dfree(s.A[1][d]);
-    dfree(s.J[d]);
+    P.take(&s.x[d], n);
+    P.take(&s.v[d], n);
+    P.take(&s.E[d], nt);
+    P.take(&s.A[0][d], nt);
+    P.take(&s.A[1][d], nt);
   }
-  dfree(s.rho);
-  dfree(s.gam);
-  for (int k = 0; k < 6; k++) dfree(s.fld[k]);
-  dfree(s.sp);
-  dfree(s.id);
-  dfree(s.off);
-  dfree(s.perm);
-  s = Snap();
+  P.take(&s.gam, n);
+  for (int k = 0; k < 6 && perm; k++) P.take(&s.fld[k], n);
+  P.take(&s.sp, n);
+  if (ids) P.take(&s.id, n);
+  P.take(&s.off, (size_t)c->geo.ncell + 1);
+  if (perm) P.take(&s.perm, n);
+  for (int d = 0; d < 3 && currents; d++) P.take(&s.J[d], nt);
+  if (currents) P.take(&s.rho, nt);
 }
-
-#define ADJ_CPY(dst, src, bytes) \
-  EMO_HIP_CHECK_RET(c, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToDevice, c->stream))
 
 // Collects device copies and issues them as k_copy_many launches.
 struct CopyBatch {
@@ -840,15 +814,63 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   // the reference loop sorts before its first push (src/main.cpp:225-226)
   rc = sort_f64(c, c->fld_state == kFldFresh ? 1 : 0);
   if (rc) return rc;
-  // ---- forward with checkpoints -------------------------------------------
+  // ---- buffers: checkpoints, final state, tape, adjoints -------------------
   const int nseg = (nsteps + K - 1) / K;
-  std::vector<Snap> ck(std::max(nseg, 1));
+  const size_t nt = (size_t)g.ntot;
+  std::vector<FillPlane> planes = fill_planes(c);
+  const int nplanes = (int)planes.size();
+  std::vector<Snap> ck(std::max(nseg, 1)), tape(K + 1);
+  Snap fin;  // with ids: the recomputation permutes the store, ids included
+  AdjFields af;
+  double *dsums = nullptr;  // nsteps x nplanes
+  PartAdj pb, pb2;
+  auto layout = [&](Pool &P) {
+    for (auto &sn : ck) snap_alloc(P, c, sn, true, false);
+    snap_alloc(P, c, fin, true, false, true);
+    for (auto &sn : tape) snap_alloc(P, c, sn, false, true);
+    for (int d = 0; d < 3; d++) {
+      P.take(&af.Eb[d], nt);
+      P.take(&af.Ab[0][d], nt);
+      P.take(&af.Ab[1][d], nt);
+      P.take(&af.Jb[d], nt);
+      P.take(&pb.x[d], (size_t)np);
+      P.take(&pb.v[d], (size_t)np);
+      P.take(&pb2.x[d], (size_t)np);
+      P.take(&pb2.v[d], (size_t)np);
+    }
+    P.take(&af.cg, 18 * (size_t)g.ncell);
+    P.take(&dsums, (size_t)std::max(1, nsteps * nplanes));
+    P.take(&pb.gam, (size_t)np);
+    P.take(&pb2.gam, (size_t)np);
+    for (int k = 0; k < 6; k++) {
+      P.take(&pb.fld[k], (size_t)np);
+      pb2.fld[k] = nullptr;
+    }
+  };
+  {
+    Pool dry;
+    layout(dry);
+    if (dry.off > c->adj_pool_bytes) {
+      if (c->adj_pool) hipFree(c->adj_pool);
+      c->adj_pool = nullptr;
+      c->adj_pool_bytes = 0;
+      if (hipMalloc(&c->adj_pool, dry.off) != hipSuccess) {
+        c->adj_pool = nullptr;
+        c->err = "hipMalloc failed for the adjoint's checkpoints and tape";
+        return EMO_ERR_ALLOC;
+      }
+      c->adj_pool_bytes = dry.off;
+    }
+    Pool real;
+    real.base = (char *)c->adj_pool;
+    layout(real);
+  }
+  // ---- forward with checkpoints -------------------------------------------
   double t = t0;
   std::vector<double> tstep(nsteps + 1);
   for (int s = 0; s < nsteps; s++) {
     if (s % K == 0) {
-      rc = snap_alloc(c, ck[s / K], true, false);
-      if (!rc) rc = snap_take(c, ck[s / K], t);
+      rc = snap_take(c, ck[s / K], t);
       if (rc) break;
     }
     tstep[s] = t;
@@ -857,10 +879,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     t += dt;
   }
   tstep[nsteps] = t;
-  if (nsteps == 0) {
-    rc = snap_alloc(c, ck[0], true, false);
-    if (!rc) rc = snap_take(c, ck[0], t);
-  }
+  if (nsteps == 0) rc = snap_take(c, ck[0], t);
   // objective
   double F = 0.0;
   if (!rc) {
@@ -869,38 +888,11 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     F = out8[4] + out8[5];
   }
   // final state (for the last segment's tape and the objective seed)
-  Snap fin;  // with ids: the recomputation permutes the store, ids included
-  if (!rc) rc = snap_alloc(c, fin, true, false, true);
-  if (!rc) rc = snap_take(c, fin, t);
-  // ---- adjoint buffers ----------------------------------------------------
-  AdjFields af;
-  const size_t nt = (size_t)g.ntot;
-  for (int d = 0; d < 3 && !rc; d++) {
-    rc = dalloc(c, &af.Eb[d], nt);
-    if (!rc) rc = dalloc(c, &af.Ab[0][d], nt);
-    if (!rc) rc = dalloc(c, &af.Ab[1][d], nt);
-    if (!rc) rc = dalloc(c, &af.Jb[d], nt);
+  bool fin_taken = false;
+  if (!rc) {
+    rc = snap_take(c, fin, t);
+    fin_taken = rc == 0;
   }
-  if (!rc) rc = dalloc(c, &af.cg, 18 * (size_t)g.ncell);
-  std::vector<FillPlane> planes = fill_planes(c);
-  const int nplanes = (int)planes.size();
-  double *dsums = nullptr;  // nsteps x nplanes
-  if (!rc) rc = dalloc(c, &dsums, (size_t)std::max(1, nsteps * nplanes));
-  PartAdj pb, pb2;
-  for (int d = 0; d < 3 && !rc; d++) {
-    rc = dalloc(c, &pb.x[d], np);
-    if (!rc) rc = dalloc(c, &pb.v[d], np);
-    if (!rc) rc = dalloc(c, &pb2.x[d], np);
-    if (!rc) rc = dalloc(c, &pb2.v[d], np);
-  }
-  if (!rc) rc = dalloc(c, &pb.gam, np);
-  if (!rc) rc = dalloc(c, &pb2.gam, np);
-  for (int k = 0; k < 6 && !rc; k++) {
-    rc = dalloc(c, &pb.fld[k], np);
-    pb2.fld[k] = nullptr;
-  }
-  std::vector<Snap> tape(K + 1);
-  for (int k = 0; k <= K && !rc; k++) rc = snap_alloc(c, tape[k], false, true);
   if (!rc) {
     for (int d = 0; d < 3; d++) {
       launch_zero(c, af.Eb[d], nt);
@@ -1088,29 +1080,11 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
       if (gbar0 && !rc) rc = put(gbar0, pb.gam, 1, 0);
     }
   }
-  // ---- restore the final state and free -------------------------------------
-  if (nsteps > 0) {
+  // ---- restore the final state (the buffers stay with the context) --------
+  if (nsteps > 0 && fin_taken) {
     int rc2 = snap_restore(c, fin);
     if (!rc) rc = rc2;
   }
-  for (auto &s : ck) snap_free(s);
-  for (auto &s : tape) snap_free(s);
-  snap_free(fin);
-  for (int d = 0; d < 3; d++) {
-    dfree(af.Eb[d]);
-    dfree(af.Ab[0][d]);
-    dfree(af.Ab[1][d]);
-    dfree(af.Jb[d]);
-    dfree(pb.x[d]);
-    dfree(pb.v[d]);
-    dfree(pb2.x[d]);
-    dfree(pb2.v[d]);
-  }
-  dfree(pb.gam);
-  dfree(pb2.gam);
-  for (int k = 0; k < 6; k++) dfree(pb.fld[k]);
-  dfree(af.cg);
-  dfree(dsums);
   if (!rc) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {

@@ -259,6 +259,7 @@ void emo_destroy(emo_ctx *c) {
   dfree(c->perm_alt);
   dfree(c->sort_tmp);
   plain_sort_destroy(c);
+  dfree(c->adj_pool);
   dfree(c->cj);
   dfree(c->cr);
   dfree(c->red);

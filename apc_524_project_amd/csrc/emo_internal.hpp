@@ -121,6 +121,10 @@ struct emo_ctx {
   // when something reads it).
   int fld_state = 1;
   int64_t fallback_sorts = 0;  // fused steps that overflowed the counting sort
+  // checkpoints, tape and adjoint buffers of emo_adjoint_field_energy, kept
+  // for later calls
+  void *adj_pool = nullptr;
+  size_t adj_pool_bytes = 0;
 
   // reductions / flags
   double *red = nullptr;   // scratch for FP64 reductions
