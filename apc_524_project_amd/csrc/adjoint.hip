@@ -205,74 +205,6 @@ struct PartAdj {
   double *x[3], *v[3], *gam, *fld[6];
 };
 
-// gather^T, stage 1: one wavefront per cell; each particle's position
-// adjoint, and the cell's 18 field-entry adjoints summed in array order
-// (rows in LDS, lanes 0..17 add columns row by row)
-struct GatherAdjArgs {
-  Geo g;
-  PartView s;  // positions at gather time, sorted order
-  PartAdj b;   // x += ; reads fld
-  const double *E[3], *B[3], *Bt[3];
-  const int32_t *off;  // cell c holds off[c]..off[c+1]
-  double *cg;  // 18 per cell, component-major
-};
-
-constexpr int kGLanes = 16;                 // lanes per cell
-constexpr int kGCells = kBlock / kGLanes;   // cells per block
-constexpr int kGStride = 19;
-
-// sixteen lanes per cell, four cells per wavefront: batches of 16
-// consecutive particles; lane l then adds column l (and 16 + l) of the batch
-// row by row (rows past the cell's end are zero)
-__global__ void __launch_bounds__(kBlock) k_gather_adj_cells(GatherAdjArgs a) {
-  __shared__ double rows[kGCells][kGLanes * kGStride];
-  const int l = threadIdx.x & (kGLanes - 1);
-  const int slot = threadIdx.x / kGLanes;
-  const Geo &g = a.g;
-  const long long c = (long long)blockIdx.x * kGCells + slot;
-  const bool cell_ok = c < g.ncell;
-  double *row = rows[slot];
-  int b = 0, e = 0;
-  if (cell_ok) {
-    b = a.off[c];
-    e = a.off[c + 1];
-  }
-  int nb = (e - b + kGLanes - 1) / kGLanes;
-  for (int o = kGLanes; o < 64; o <<= 1) nb = max(nb, __shfl_xor(nb, o));
-  double acc0 = 0.0, acc1 = 0.0;
-  for (int t = 0; t < nb; t++) {
-    const int p = b + t * kGLanes + l;
-    double cvb[21];
-#pragma unroll
-    for (int q = 0; q < 21; q++) cvb[q] = 0.0;
-    if (p < e) {
-      double pos[3] = {a.s.x[0][p], a.s.x[1][p], a.s.x[2][p]};
-      int ix, iy, iz;
-      cell_of(g, pos[0], pos[1], pos[2], &ix, &iy, &iz);
-      double cv[21], fb[6], posb[3] = {0, 0, 0};
-      cell_vector(g, a.E, a.B, a.Bt, ix, iy, iz, cv);
-      for (int q = 0; q < 6; q++) fb[q] = a.b.fld[q][p];
-      interp_fields_vjp(g, cv, pos, fb, posb, cvb);
-      for (int d = 0; d < 3; d++) a.b.x[d][p] += posb[d];
-    }
-    double *r = row + l * kGStride;
-#pragma unroll
-    for (int q = 0; q < 18; q++) r[q] = cvb[3 + q];
-    wave_sync();
-#pragma unroll
-    for (int rr = 0; rr < kGLanes; rr++) acc0 += row[rr * kGStride + l];
-    if (l < 2) {
-#pragma unroll
-      for (int rr = 0; rr < kGLanes; rr++) acc1 += row[rr * kGStride + 16 + l];
-    }
-    wave_sync();
-  }
-  if (cell_ok) {
-    a.cg[l * g.ncell + c] = acc0;
-    if (l < 2) a.cg[(16 + l) * g.ncell + c] = acc1;
-  }
-}
-
 // gather^T, stage 2: every point adds the entries of the cells whose
 // cell_vector reads it, in a fixed order
 struct GatherGridArgs {
@@ -425,84 +357,172 @@ __global__ void __launch_bounds__(kBlock) k_plane_fill_adj_multi(FillAdjMulti m)
 }
 
 // ---------------------------------------------------------------------------
-// deposit^T + sort^T + BCs^T + push^T in one pass over the sorted order j:
-// the current adjoint is read at the particle's deposit targets, the
-// particle's adjoint moves to its pre-sort index perm[j], and the push is
-// transposed there from the taped pre-push state.
+// The particle part of one adjoint step, per cell of the pre-push order
+// (sixteen lanes per cell, four cells per wavefront, batches of 16
+// consecutive particles):
+//   deposit^T  the current adjoint at the particle's deposit targets, read at
+//              its post-sort slot j = inv[i];
+//   BCs^T + push^T  from the taped pre-push state i;
+//   gather^T   of the gather that produced the push's fields (the previous
+//              step's last phase, on this same state): the position adjoint
+//              gains the interpolation term, and lane l adds column l (and
+//              16 + l) of the batch's 18 field-entry adjoints row by row, so
+//              each cell's sums follow the particle order.
 // ---------------------------------------------------------------------------
-struct PartStepAdjArgs {
+constexpr int kGLanes = 16;                 // lanes per cell
+constexpr int kGCells = kBlock / kGLanes;   // cells per block
+constexpr int kGStride = 19;
+
+struct CellStepAdjArgs {
   Geo g;
-  PartView post;  // post-BC state, sorted order (x, v, sp)
-  PartView pre;   // pre-push state, previous order (x, v, gam, fld, sp)
-  const double *bx[3], *bv[3], *bg;  // adjoint, sorted order (in)
-  double *ox[3], *ov[3], *og;        // adjoint, previous order (out)
-  double *of[6];                     // field adjoint, previous order (out)
+  PartView post;        // post-BC state, sorted order (x, v, sp)
+  PartView pre;         // pre-push state, cell-sorted (x, v, gam, fld, sp)
+  const uint8_t *flips; // the push's reflections, pre-push order
+  const int32_t *off;   // cell c of the pre-push order holds off[c]..off[c+1]
+  const uint32_t *inv;  // pre-push index -> post-sort slot
+  const double *E[3], *B[3], *Bt[3];  // the grid the pre-push fields came from
+  const double *bx[3], *bv[3], *bg;   // adjoint, sorted order (in)
+  double *ox[3], *ov[3], *og;         // adjoint, pre-push order (out)
   const double *Jb[3];
-  const uint32_t *perm;
+  double *cg;  // 18 per cell, component-major
   SpeciesParams sp;
   double dt;
-  double xmin[3], xmax[3];
-  int pbc[6];
   int rel;
-  long long np;
 };
 
-__global__ void __launch_bounds__(kBlock) k_particle_step_adj(PartStepAdjArgs a) {
-  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.np) return;
+__global__ void __launch_bounds__(kBlock, 4) k_cell_step_adj(CellStepAdjArgs a) {
+  __shared__ double rows[kGCells][kGLanes * kGStride];
+  const int l = threadIdx.x & (kGLanes - 1);
+  const int slot = threadIdx.x / kGLanes;
   const Geo &g = a.g;
-  double xb1[3], vb1[3];
-  for (int d = 0; d < 3; d++) {
-    xb1[d] = a.bx[d][j];
-    vb1[d] = a.bv[d][j];
+  const long long c = (long long)blockIdx.x * kGCells + slot;
+  const bool cell_ok = c < g.ncell;
+  double *row = rows[slot];
+  int b = 0, e = 0;
+  if (cell_ok) {
+    b = a.off[c];
+    e = a.off[c + 1];
   }
-  double gb1 = a.bg[j];
-  const int sp = a.post.sp[j];
-  const double q = a.sp.q[sp], m = a.sp.m[sp];
-  if (!a.sp.test[sp]) {
-    double pos[3] = {a.post.x[0][j], a.post.x[1][j], a.post.x[2][j]};
-    double vel[3] = {a.post.v[0][j], a.post.v[1][j], a.post.v[2][j]};
-    int ix, iy, iz;
-    if (cell_of(g, pos[0], pos[1], pos[2], &ix, &iy, &iz) >= 0) {
-      double cellverts[3] = {g.x0[0] + g.d[0] * ix, g.x0[1] + g.d[1] * iy, g.x0[2] + g.d[2] * iz};
-      long long idx[12];
-      deposit_target_index(g, ix, iy, iz, idx);
-      double rb[12];
-      for (int e = 0; e < 12; e++) rb[e] = a.Jb[e / 4][idx[e]];
-      double posb[3] = {0, 0, 0}, velb[3] = {0, 0, 0};
-      deposit_row_vjp(g, cellverts, pos, vel, q, rb, posb, velb);
+  int nb = (e - b + kGLanes - 1) / kGLanes;
+  for (int o = kGLanes; o < 64; o <<= 1) nb = max(nb, __shfl_xor(nb, o));
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int t = 0; t < nb; t++) {
+    const int i = b + t * kGLanes + l;
+    double cvb[21];
+#pragma unroll
+    for (int q = 0; q < 21; q++) cvb[q] = 0.0;
+    if (i < e) {
+      const long long j = a.inv[i];
+      double xb1[3], vb1[3];
       for (int d = 0; d < 3; d++) {
-        xb1[d] += posb[d];
-        vb1[d] += velb[d];
+        xb1[d] = a.bx[d][j];
+        vb1[d] = a.bv[d][j];
       }
+      const double gb1 = a.bg[j];
+      const int sp = a.pre.sp[i];
+      const double q = a.sp.q[sp], m = a.sp.m[sp];
+      if (!a.sp.test[sp]) {
+        double pos[3] = {a.post.x[0][j], a.post.x[1][j], a.post.x[2][j]};
+        double vel[3] = {a.post.v[0][j], a.post.v[1][j], a.post.v[2][j]};
+        int ix, iy, iz;
+        if (cell_of(g, pos[0], pos[1], pos[2], &ix, &iy, &iz) >= 0) {
+          double cellverts[3] = {g.x0[0] + g.d[0] * ix, g.x0[1] + g.d[1] * iy, g.x0[2] + g.d[2] * iz};
+          long long idx[12];
+          deposit_target_index(g, ix, iy, iz, idx);
+          double rb[12];
+          for (int k = 0; k < 12; k++) rb[k] = a.Jb[k / 4][idx[k]];
+          double posb[3] = {0, 0, 0}, velb[3] = {0, 0, 0};
+          deposit_row_vjp(g, cellverts, pos, vel, q, rb, posb, velb);
+          for (int d = 0; d < 3; d++) {
+            xb1[d] += posb[d];
+            vb1[d] += velb[d];
+          }
+        }
+      }
+      double x[3] = {a.pre.x[0][i], a.pre.x[1][i], a.pre.x[2][i]};
+      double v[3] = {a.pre.v[0][i], a.pre.v[1][i], a.pre.v[2][i]};
+      const double gam = a.pre.gam[i];
+      double f[6];
+      for (int k = 0; k < 6; k++) f[k] = a.pre.fld[k][i];
+      const int fl = a.flips[i];
+      const int flips[3] = {fl & 1, (fl >> 1) & 1, (fl >> 2) & 1};
+      particle_bcs_vjp(flips, xb1, vb1);
+      double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, gb = 0.0, fb[6] = {0, 0, 0, 0, 0, 0};
+      if (a.rel) {
+        vay_step_vjp(v, gam, f, q, m, a.dt, xb1, vb1, gb1, xb, vb, gb, fb);
+      } else {
+        boris_step_vjp(v, f, q, m, a.dt, xb1, vb1, xb, vb, fb);
+        gb = gb1;  // Boris leaves the stored gamma untouched
+      }
+      // gather^T at the pre-push position
+      int ix, iy, iz;
+      cell_of(g, x[0], x[1], x[2], &ix, &iy, &iz);
+      double cv[21], posb[3] = {0, 0, 0};
+      cell_vector(g, a.E, a.B, a.Bt, ix, iy, iz, cv);
+      interp_fields_vjp(g, cv, x, fb, posb, cvb);
+      for (int d = 0; d < 3; d++) {
+        a.ox[d][i] = xb[d] + posb[d];
+        a.ov[d][i] = vb[d];
+      }
+      a.og[i] = gb;
     }
+    double *r = row + l * kGStride;
+#pragma unroll
+    for (int q = 0; q < 18; q++) r[q] = cvb[3 + q];
+    wave_sync();
+#pragma unroll
+    for (int rr = 0; rr < kGLanes; rr++) acc0 += row[rr * kGStride + l];
+    if (l < 2) {
+#pragma unroll
+      for (int rr = 0; rr < kGLanes; rr++) acc1 += row[rr * kGStride + 16 + l];
+    }
+    wave_sync();
   }
-  const long long i = a.perm[j];
-  double x[3] = {a.pre.x[0][i], a.pre.x[1][i], a.pre.x[2][i]};
-  double v[3] = {a.pre.v[0][i], a.pre.v[1][i], a.pre.v[2][i]};
-  double gam = a.pre.gam[i];
-  double f[6];
-  for (int k = 0; k < 6; k++) f[k] = a.pre.fld[k][i];
-  // forward recomputation up to the BCs (for the reflections)
-  double x1[3] = {x[0], x[1], x[2]}, v1[3] = {v[0], v[1], v[2]}, g1 = gam;
-  if (a.rel) vay_step(x1, v1, g1, f, q, m, a.dt);
-  else boris_step(x1, v1, f, q, m, a.dt);
-  int flips[3];
-  particle_bcs(x1, v1, a.xmin, a.xmax, a.pbc, flips);
-  particle_bcs_vjp(flips, xb1, vb1);
-  double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, gb = 0.0, fb[6] = {0, 0, 0, 0, 0, 0};
-  if (a.rel) {
-    vay_step_vjp(v, gam, f, q, m, a.dt, xb1, vb1, gb1, xb, vb, gb, fb);
+  if (cell_ok) {
+    a.cg[l * g.ncell + c] = acc0;
+    if (l < 2) a.cg[(16 + l) * g.ncell + c] = acc1;
+  }
+}
+
+// fold^T of every periodic dim in one launch: the ghost plane n_d + 1 gains
+// the adjoint of plane 1 (physical range); each ghost point has one source
+template <typename T>
+struct FoldAdjArgs {
+  Geo g;
+  T *J[3];
+  int per[3];
+};
+
+__global__ void __launch_bounds__(kBlock) k_fold_adj_all(FoldAdjArgs<double> a) {
+  const Geo &g = a.g;
+  const long long n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (t < n1 * n2) {
+    i = 1;
+    j = 1 + (int)(t / n2);
+    k = 1 + (int)(t % n2);
+  } else if ((t -= n1 * n2) < (n0 - 1) * n2) {
+    i = 2 + (int)(t / n2);
+    j = 1;
+    k = 1 + (int)(t % n2);
+  } else if ((t -= (n0 - 1) * n2) < (n0 - 1) * (n1 - 1)) {
+    i = 2 + (int)(t / (n1 - 1));
+    j = 2 + (int)(t % (n1 - 1));
+    k = 1;
   } else {
-    boris_step_vjp(v, f, q, m, a.dt, xb1, vb1, xb, vb, fb);
-    gb = gb1;  // Boris leaves the stored gamma untouched
+    return;
   }
+  const long long p = gidx(g, i, j, k);
+  const int at[3] = {i, j, k};
   for (int d = 0; d < 3; d++) {
-    a.ox[d][i] = xb[d];
-    a.ov[d][i] = vb[d];
+    if (!a.per[d] || at[d] != 1) continue;
+    int gh[3] = {i, j, k};
+    gh[d] = g.n[d] + 1;
+    const long long q = gidx(g, gh[0], gh[1], gh[2]);
+    for (int c = 0; c < 3; c++)
+      if (c != d) a.J[c][q] += a.J[c][p];
   }
-  a.og[i] = gb;
-  for (int k = 0; k < 6; k++) a.of[k][i] = fb[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -531,7 +551,8 @@ struct Snap {
   double *J[3] = {}, *rho = nullptr;  // the final state only
   double *E[3] = {}, *A[2][3] = {};  // A[0] = Bfix, A[1] = Btfix
   int32_t *off = nullptr;    // cell ranges of this order (ncell + 1)
-  uint32_t *perm = nullptr;  // the following sort: new position j <- old index perm[j]
+  uint32_t *inv = nullptr;   // the following sort: old index i -> new position inv[i]
+  uint8_t *flips = nullptr;  // the push's reflections (bit d: dim d)
   bool b_is_a0 = true;       // member B is the Bfix allocation
   double t = 0.0;
 };
@@ -551,7 +572,8 @@ void snap_alloc(Pool &P, const emo_ctx *c, Snap &s, bool ids, bool perm, bool cu
   P.take(&s.sp, n);
   if (ids) P.take(&s.id, n);
   P.take(&s.off, (size_t)c->geo.ncell + 1);
-  if (perm) P.take(&s.perm, n);
+  if (perm) P.take(&s.inv, n);
+  if (perm) P.take(&s.flips, n);
   for (int d = 0; d < 3 && currents; d++) P.take(&s.J[d], nt);
   if (currents) P.take(&s.rho, nt);
 }
@@ -633,16 +655,16 @@ int snap_restore(emo_ctx *c, const Snap &s) {
 
 // One forward step of emo_step.  With a tape entry, the state the push reads
 // (with the fields it gathers) goes to the entry before the sort, and the
-// sort's permutation after it.
+// sort's inverse permutation after it.
 int forward_step(emo_ctx *c, double t, double dt, Snap *tape) {
   int rc;
   if (c->np > 0) {
-    fused_push_f64(c, dt, tape ? tape->fld : nullptr);
+    fused_push_f64(c, dt, tape ? tape->fld : nullptr, tape ? tape->flips : nullptr);
     if (tape) {
       rc = snap_take(c, *tape, t);
       if (rc) return rc;
     }
-    rc = fused_sort_f64(c, tape ? tape->perm : nullptr);
+    rc = fused_sort_f64(c, nullptr, tape ? tape->inv : nullptr);
     if (rc) return rc;
   } else {
     if (tape) {
@@ -735,44 +757,17 @@ void pulse_derivs(const emo_ctx *c, const FillPlane &f, double t, double dt, std
 struct AdjFields {
   double *Eb[3] = {}, *Ab[2][3] = {}, *Jb[3] = {};
   double *cg = nullptr;
-  double *sums = nullptr;  // plane sums of one step
 };
 
-int launch_zero(emo_ctx *c, double *p, long long n) {
-  hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, p, n);
-  return EMO_OK;
+void launch_zero(hipStream_t st, double *p, long long n) {
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(kBlock), 0, st, p, n);
 }
 
-// gather^T of a gather that ran on state s (positions, fields, ranges, B roles)
-void gather_adj(emo_ctx *c, const Snap &s, PartAdj &pb, AdjFields &af) {
-  const Geo &g = c->geo;
-  GatherAdjArgs a;
-  a.g = g;
-  for (int d = 0; d < 3; d++) {
-    a.s.x[d] = s.x[d];
-    a.s.v[d] = s.v[d];
-    a.b.x[d] = pb.x[d];
-    a.b.v[d] = pb.v[d];
-    a.E[d] = s.E[d];
-    a.B[d] = s.b_is_a0 ? s.A[0][d] : s.A[1][d];
-    a.Bt[d] = s.b_is_a0 ? s.A[1][d] : s.A[0][d];
-  }
-  a.s.gam = s.gam;
-  a.b.gam = pb.gam;
-  for (int k = 0; k < 6; k++) {
-    a.s.fld[k] = s.fld[k];
-    a.b.fld[k] = pb.fld[k];
-  }
-  a.s.sp = s.sp;
-  a.off = s.off;
-  a.cg = af.cg;
-  {
-    TimedLaunch t(c, "adj_gather_cells");
-    hipLaunchKernelGGL(k_gather_adj_cells, dim3((unsigned)((g.ncell + kGCells - 1) / kGCells)), dim3(kBlock), 0,
-                       c->stream, a);
-  }
+// gather^T, stage 2, for a gather that ran on state s: the per-cell sums in
+// af.cg go to the points (B roles of that state)
+void gather_adj_grid(emo_ctx *c, hipStream_t st, const Snap &s, AdjFields &af) {
   GatherGridArgs b;
-  b.g = g;
+  b.g = c->geo;
   b.cg = af.cg;
   const int mb = s.b_is_a0 ? 0 : 1;
   for (int d = 0; d < 3; d++) {
@@ -780,8 +775,168 @@ void gather_adj(emo_ctx *c, const Snap &s, PartAdj &pb, AdjFields &af) {
     b.Bb[d] = af.Ab[mb][d];
     b.Btb[d] = af.Ab[1 - mb][d];
   }
-  TimedLaunch t(c, "adj_gather_grid");
-  hipLaunchKernelGGL(k_gather_adj_grid, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, b);
+  TimedLaunch t(c, "adj_gather_grid", st);
+  hipLaunchKernelGGL(k_gather_adj_grid, dim3(grid_for(c->geo.ntot)), dim3(kBlock), 0, st, b);
+}
+
+// Everything one gradient call needs besides the context.
+struct AdjRun {
+  emo_ctx *c;
+  hipStream_t as;  // the backward sweep's stream; the context's stream recomputes
+  double dt;
+  int nsteps, K, nseg;
+  std::vector<Snap> ck, tape[2];
+  Snap fin;
+  AdjFields af;
+  double *dsums = nullptr;  // nsteps x nplanes
+  PartAdj pb, pb2;
+  std::vector<FillPlane> planes;
+};
+
+// Segment seg re-run from its checkpoint on the context's stream, taping
+// into tape set `set`.
+int recompute_segment(AdjRun &R, int seg, int set) {
+  emo_ctx *c = R.c;
+  const int s0 = seg * R.K, s1 = std::min(R.nsteps, s0 + R.K);
+  int rc = snap_restore(c, R.ck[seg]);
+  double tt = R.ck[seg].t;
+  for (int s = s0; s < s1 && !rc; s++) {
+    rc = forward_step(c, tt, R.dt, &R.tape[set][s - s0]);
+    tt += R.dt;
+  }
+  if (!rc) rc = snap_take(c, R.tape[set][s1 - s0], tt);
+  return rc;
+}
+
+// The adjoint of segment seg's steps, newest first, on the adjoint stream.
+void backward_segment(AdjRun &R, int seg, int set) {
+  emo_ctx *c = R.c;
+  const Geo &g = c->geo;
+  hipStream_t as = R.as;
+  AdjFields &af = R.af;
+  const int s0 = seg * R.K, s1 = std::min(R.nsteps, s0 + R.K);
+  const int nplanes = (int)R.planes.size();
+  for (int s = s1 - 1; s >= s0; s--) {
+    const Snap &pre = R.tape[set][s - s0];      // the state the push of step s reads
+    const Snap &post = R.tape[set][s + 1 - s0];  // the state after step s's sort
+    // refresh^T, the forward's launches in reverse: a LightBC side's six
+    // fills as one launch, a run of periodic sides as one gather pass
+    const int mnew = pre.b_is_a0 ? 1 : 0;  // the array the B update writes
+    int pi = nplanes - 1;
+    for (int side = 5; side >= 0;) {
+      if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
+        SidesAdjArgs sa;
+        sa.g = g;
+        for (int q = 0; q < 3; q++) {
+          sa.f[q] = af.Eb[q];
+          sa.f[3 + q] = af.Ab[0][q];  // Q23: the refresh writes B through Bfix
+        }
+        sa.nsides = 0;
+        for (; side >= 0 && c->cfg.fields_bound[side] == EMO_BC_PERIODIC; side--) {
+          const int d = side / 2, right = side & 1;
+          sa.d[sa.nsides] = d;
+          sa.src[sa.nsides] = right ? 1 : g.n[d];
+          sa.dst[sa.nsides] = right ? g.n[d] + 1 : 0;
+          sa.nsides++;
+        }
+        hipLaunchKernelGGL(k_copy_sides_adj, dim3(grid_for((long long)g.n[0] * g.n[1] * g.n[2])), dim3(kBlock), 0,
+                           as, sa);
+      } else {
+        FillAdjMulti fm;
+        int nb = 0;
+        for (; pi >= 0 && R.planes[pi].side == side; pi--) {
+          const FillPlane &f = R.planes[pi];
+          FillAdjArgs &fa = fm.p[nb++];
+          fa.g = g;
+          fa.f = f.pass == 0 ? af.Eb[f.comp] : af.Ab[mnew][f.comp];
+          fa.dim = f.dim;
+          fa.idx = f.idx;
+          fa.waist = c->cfg.waist;
+          for (int q = 0; q < 3; q++) {
+            fa.off[q] = f.off[q];
+            fa.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
+          }
+          fa.out = R.dsums + (size_t)s * nplanes + pi;
+        }
+        if (nb) hipLaunchKernelGGL(k_plane_fill_adj_multi, dim3(nb), dim3(kBlock), 0, as, fm);
+        side--;
+      }
+    }
+    // Yee^T
+    YeeAdjArgs ya;
+    ya.g = g;
+    ya.dt = R.dt;
+    for (int q = 0; q < 3; q++) {
+      ya.Eb[q] = af.Eb[q];
+      ya.Bn[q] = af.Ab[mnew][q];
+      ya.Bo[q] = af.Ab[1 - mnew][q];
+      ya.Jb[q] = af.Jb[q];
+    }
+    {
+      TimedLaunch tl(c, "adj_yee", as);
+      hipLaunchKernelGGL(k_yee_b_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
+      hipLaunchKernelGGL(k_yee_move_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
+      hipLaunchKernelGGL(k_yee_e_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
+    }
+    // fold^T: each ghost plane receives its first plane's adjoint
+    {
+      FoldAdjArgs<double> fa;
+      fa.g = g;
+      int any = 0;
+      for (int d = 0; d < 3; d++) {
+        fa.J[d] = af.Jb[d];
+        fa.per[d] = c->cfg.fields_bound[2 * d] == EMO_BC_PERIODIC;
+        any |= fa.per[d];
+      }
+      if (any) {
+        const long long n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+        const long long pts = n1 * n2 + (n0 - 1) * n2 + (n0 - 1) * (n1 - 1);
+        hipLaunchKernelGGL(k_fold_adj_all, dim3(grid_for(pts)), dim3(kBlock), 0, as, fa);
+      }
+    }
+    // deposit^T + sort^T + BCs^T + push^T, then gather^T of the gather that
+    // fed this push
+    CellStepAdjArgs ca;
+    ca.g = g;
+    for (int d = 0; d < 3; d++) {
+      ca.post.x[d] = post.x[d];
+      ca.post.v[d] = post.v[d];
+      ca.pre.x[d] = pre.x[d];
+      ca.pre.v[d] = pre.v[d];
+      ca.E[d] = pre.E[d];
+      ca.B[d] = pre.b_is_a0 ? pre.A[0][d] : pre.A[1][d];
+      ca.Bt[d] = pre.b_is_a0 ? pre.A[1][d] : pre.A[0][d];
+      ca.bx[d] = R.pb.x[d];
+      ca.bv[d] = R.pb.v[d];
+      ca.ox[d] = R.pb2.x[d];
+      ca.ov[d] = R.pb2.v[d];
+      ca.Jb[d] = af.Jb[d];
+    }
+    ca.post.sp = post.sp;
+    ca.pre.sp = pre.sp;
+    ca.pre.gam = pre.gam;
+    for (int k = 0; k < 6; k++) ca.pre.fld[k] = pre.fld[k];
+    ca.off = pre.off;
+    ca.inv = pre.inv;
+    ca.flips = pre.flips;
+    ca.bg = R.pb.gam;
+    ca.og = R.pb2.gam;
+    ca.cg = af.cg;
+    ca.sp = c->species;
+    ca.dt = R.dt;
+    ca.rel = c->cfg.relativity;
+    {
+      TimedLaunch tl(c, "adj_cells", as);
+      hipLaunchKernelGGL(k_cell_step_adj, dim3((unsigned)((g.ncell + kGCells - 1) / kGCells)), dim3(kBlock), 0,
+                         as, ca);
+    }
+    gather_adj_grid(c, as, pre, af);
+    for (int d = 0; d < 3; d++) {
+      std::swap(R.pb.x[d], R.pb2.x[d]);
+      std::swap(R.pb.v[d], R.pb2.v[d]);
+    }
+    std::swap(R.pb.gam, R.pb2.gam);
+  }
 }
 
 }  // namespace
@@ -811,41 +966,48 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     c->err = "the adjoint starts from particle fields gathered from the grid (emo_interpolate)";
     return EMO_ERR_UNSUPPORTED;
   }
+  if (!c->adj_stream && hipStreamCreate(&c->adj_stream) != hipSuccess) {
+    c->adj_stream = nullptr;
+    c->err = "hipStreamCreate failed for the adjoint";
+    return EMO_ERR_HIP;
+  }
   // the reference loop sorts before its first push (src/main.cpp:225-226)
   rc = sort_f64(c, c->fld_state == kFldFresh ? 1 : 0);
   if (rc) return rc;
-  // ---- buffers: checkpoints, final state, tape, adjoints -------------------
-  const int nseg = (nsteps + K - 1) / K;
+  // ---- buffers: checkpoints, final state, two tape sets, adjoints -----------
+  AdjRun R;
+  R.c = c;
+  R.as = c->adj_stream;
+  R.dt = dt;
+  R.nsteps = nsteps;
+  R.K = K;
+  R.nseg = (nsteps + K - 1) / K;
+  R.planes = fill_planes(c);
+  R.ck.resize(std::max(R.nseg, 1));
+  R.tape[0].resize(K + 1);
+  R.tape[1].resize(K + 1);
   const size_t nt = (size_t)g.ntot;
-  std::vector<FillPlane> planes = fill_planes(c);
-  const int nplanes = (int)planes.size();
-  std::vector<Snap> ck(std::max(nseg, 1)), tape(K + 1);
-  Snap fin;  // with ids: the recomputation permutes the store, ids included
-  AdjFields af;
-  double *dsums = nullptr;  // nsteps x nplanes
-  PartAdj pb, pb2;
+  const int nplanes = (int)R.planes.size();
+  const int nsets = R.nseg > 1 ? 2 : 1;
   auto layout = [&](Pool &P) {
-    for (auto &sn : ck) snap_alloc(P, c, sn, true, false);
-    snap_alloc(P, c, fin, true, false, true);
-    for (auto &sn : tape) snap_alloc(P, c, sn, false, true);
+    for (auto &sn : R.ck) snap_alloc(P, c, sn, true, false);
+    snap_alloc(P, c, R.fin, true, false, true);
+    for (int st = 0; st < nsets; st++)
+      for (auto &sn : R.tape[st]) snap_alloc(P, c, sn, false, true);
     for (int d = 0; d < 3; d++) {
-      P.take(&af.Eb[d], nt);
-      P.take(&af.Ab[0][d], nt);
-      P.take(&af.Ab[1][d], nt);
-      P.take(&af.Jb[d], nt);
-      P.take(&pb.x[d], (size_t)np);
-      P.take(&pb.v[d], (size_t)np);
-      P.take(&pb2.x[d], (size_t)np);
-      P.take(&pb2.v[d], (size_t)np);
+      P.take(&R.af.Eb[d], nt);
+      P.take(&R.af.Ab[0][d], nt);
+      P.take(&R.af.Ab[1][d], nt);
+      P.take(&R.af.Jb[d], nt);
+      P.take(&R.pb.x[d], (size_t)np);
+      P.take(&R.pb.v[d], (size_t)np);
+      P.take(&R.pb2.x[d], (size_t)np);
+      P.take(&R.pb2.v[d], (size_t)np);
     }
-    P.take(&af.cg, 18 * (size_t)g.ncell);
-    P.take(&dsums, (size_t)std::max(1, nsteps * nplanes));
-    P.take(&pb.gam, (size_t)np);
-    P.take(&pb2.gam, (size_t)np);
-    for (int k = 0; k < 6; k++) {
-      P.take(&pb.fld[k], (size_t)np);
-      pb2.fld[k] = nullptr;
-    }
+    P.take(&R.af.cg, 18 * (size_t)g.ncell);
+    P.take(&R.dsums, (size_t)std::max(1, nsteps * nplanes));
+    P.take(&R.pb.gam, (size_t)np);
+    P.take(&R.pb2.gam, (size_t)np);
   };
   {
     Pool dry;
@@ -870,7 +1032,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   std::vector<double> tstep(nsteps + 1);
   for (int s = 0; s < nsteps; s++) {
     if (s % K == 0) {
-      rc = snap_take(c, ck[s / K], t);
+      rc = snap_take(c, R.ck[s / K], t);
       if (rc) break;
     }
     tstep[s] = t;
@@ -879,7 +1041,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     t += dt;
   }
   tstep[nsteps] = t;
-  if (nsteps == 0) rc = snap_take(c, ck[0], t);
+  if (!rc && nsteps == 0) rc = snap_take(c, R.ck[0], t);
   // objective
   double F = 0.0;
   if (!rc) {
@@ -890,167 +1052,83 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   // final state (for the last segment's tape and the objective seed)
   bool fin_taken = false;
   if (!rc) {
-    rc = snap_take(c, fin, t);
+    rc = snap_take(c, R.fin, t);
     fin_taken = rc == 0;
+  }
+  std::vector<hipEvent_t> ev;  // every event of this call, destroyed at the end
+  auto record = [&](hipStream_t st) -> hipEvent_t {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    ev.push_back(e);
+    if (hipEventRecord(e, st) != hipSuccess) return nullptr;
+    return e;
+  };
+  auto wait = [&](hipStream_t st, hipEvent_t e) -> int {
+    if (!e || hipStreamWaitEvent(st, e, 0) != hipSuccess) {
+      c->err = "stream ordering of the adjoint failed";
+      return EMO_ERR_HIP;
+    }
+    return EMO_OK;
+  };
+  if (!rc) {
+    // the seed reads the final fields: the adjoint stream waits for the forward
+    rc = wait(R.as, record(c->stream));
   }
   if (!rc) {
     for (int d = 0; d < 3; d++) {
-      launch_zero(c, af.Eb[d], nt);
-      launch_zero(c, af.Ab[0][d], nt);
-      launch_zero(c, af.Ab[1][d], nt);
-      launch_zero(c, af.Jb[d], nt);
-      launch_zero(c, pb.x[d], np);
-      launch_zero(c, pb.v[d], np);
+      launch_zero(R.as, R.af.Eb[d], nt);
+      launch_zero(R.as, R.af.Ab[0][d], nt);
+      launch_zero(R.as, R.af.Ab[1][d], nt);
+      launch_zero(R.as, R.af.Jb[d], nt);
+      launch_zero(R.as, R.pb.x[d], np);
+      launch_zero(R.as, R.pb.v[d], np);
     }
-    launch_zero(c, pb.gam, np);
-    for (int k = 0; k < 6; k++) launch_zero(c, pb.fld[k], np);
-    // seed: dF/dE, dF/dBfix at the final state
+    launch_zero(R.as, R.pb.gam, np);
+    // from the final-state snapshot: the context's grid is overwritten by
+    // the recomputation running beside this stream
     SeedArgs sa;
     sa.g = g;
     for (int d = 0; d < 3; d++) {
-      sa.E[d] = c->E[d];
-      sa.B[d] = c->Bfix[d];
-      sa.Eb[d] = af.Eb[d];
-      sa.Bb[d] = af.Ab[0][d];
+      sa.E[d] = R.fin.E[d];
+      sa.B[d] = R.fin.A[0][d];
+      sa.Eb[d] = R.af.Eb[d];
+      sa.Bb[d] = R.af.Ab[0][d];
     }
-    hipLaunchKernelGGL(k_seed_energy, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, sa);
+    hipLaunchKernelGGL(k_seed_energy, dim3(grid_for(g.ntot)), dim3(kBlock), 0, R.as, sa);
   }
-  // ---- backward over segments ---------------------------------------------
-  for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
-    const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
-    rc = snap_restore(c, ck[seg]);
-    if (rc) break;
-    double tt = ck[seg].t;
-    for (int s = s0; s < s1 && !rc; s++) {
-      rc = forward_step(c, tt, dt, &tape[s - s0]);
-      tt += dt;
-    }
-    if (!rc) rc = snap_take(c, tape[s1 - s0], tt);
-    for (int s = s1 - 1; s >= s0 && !rc; s--) {
-      const Snap &pre = tape[s - s0];      // state the push of step s reads
-      const Snap &post = tape[s + 1 - s0];  // state after step s (its gather)
-      // gather^T of step s's gather (needs the fields' adjoint of push s+1)
-      if (s < nsteps - 1) gather_adj(c, post, pb, af);
-      // refresh^T, the forward's launches in reverse: a LightBC side's six
-      // fills as one launch, a run of periodic sides as one gather pass
-      const int mnew = pre.b_is_a0 ? 1 : 0;  // the array the B update writes
-      int pi = nplanes - 1;
-      for (int side = 5; side >= 0;) {
-        if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
-          SidesAdjArgs sa;
-          sa.g = g;
-          for (int q = 0; q < 3; q++) {
-            sa.f[q] = af.Eb[q];
-            sa.f[3 + q] = af.Ab[0][q];  // Q23: the refresh writes B through Bfix
-          }
-          sa.nsides = 0;
-          for (; side >= 0 && c->cfg.fields_bound[side] == EMO_BC_PERIODIC; side--) {
-            const int d = side / 2, right = side & 1;
-            sa.d[sa.nsides] = d;
-            sa.src[sa.nsides] = right ? 1 : g.n[d];
-            sa.dst[sa.nsides] = right ? g.n[d] + 1 : 0;
-            sa.nsides++;
-          }
-          hipLaunchKernelGGL(k_copy_sides_adj, dim3(grid_for((long long)g.n[0] * g.n[1] * g.n[2])), dim3(kBlock),
-                             0, c->stream, sa);
-        } else {
-          FillAdjMulti fm;
-          int nb = 0;
-          for (; pi >= 0 && planes[pi].side == side; pi--) {
-            const FillPlane &f = planes[pi];
-            FillAdjArgs &fa = fm.p[nb++];
-            fa.g = g;
-            fa.f = f.pass == 0 ? af.Eb[f.comp] : af.Ab[mnew][f.comp];
-            fa.dim = f.dim;
-            fa.idx = f.idx;
-            fa.waist = c->cfg.waist;
-            for (int q = 0; q < 3; q++) {
-              fa.off[q] = f.off[q];
-              fa.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
-            }
-            fa.out = dsums + (size_t)s * nplanes + pi;
-          }
-          if (nb)
-            hipLaunchKernelGGL(k_plane_fill_adj_multi, dim3(nb), dim3(kBlock), 0, c->stream, fm);
-          side--;
-        }
+  // ---- backward: segment seg's adjoint (adjoint stream) runs while segment
+  // seg - 1 is recomputed (context stream) into the other tape set ----------
+  if (!rc && nsteps > 0) {
+    std::vector<hipEvent_t> done_r(R.nseg, nullptr), done_a(R.nseg, nullptr);
+    rc = recompute_segment(R, R.nseg - 1, (R.nseg - 1) % nsets);
+    if (!rc) done_r[R.nseg - 1] = record(c->stream);
+    for (int seg = R.nseg - 1; seg >= 0 && !rc; seg--) {
+      rc = wait(R.as, done_r[seg]);
+      if (rc) break;
+      backward_segment(R, seg, seg % nsets);
+      done_a[seg] = record(R.as);
+      if (seg > 0) {
+        // the set seg - 1 tapes into was last read by segment seg + 1's adjoint
+        if (seg + 1 < R.nseg) rc = wait(c->stream, done_a[seg + 1]);
+        if (!rc) rc = recompute_segment(R, seg - 1, (seg - 1) % nsets);
+        if (!rc) done_r[seg - 1] = record(c->stream);
       }
-      // Yee^T
-      YeeAdjArgs ya;
-      ya.g = g;
-      ya.dt = dt;
-      for (int q = 0; q < 3; q++) {
-        ya.Eb[q] = af.Eb[q];
-        ya.Bn[q] = af.Ab[mnew][q];
-        ya.Bo[q] = af.Ab[1 - mnew][q];
-        ya.Jb[q] = af.Jb[q];
-      }
-      {
-        TimedLaunch tl(c, "adj_yee");
-        hipLaunchKernelGGL(k_yee_b_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, ya);
-        hipLaunchKernelGGL(k_yee_move_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, ya);
-        hipLaunchKernelGGL(k_yee_e_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, ya);
-      }
-      // fold^T: the ghost plane receives the first plane's adjoint
-      for (int d = 2; d >= 0; d--) {
-        if (c->cfg.fields_bound[2 * d] != EMO_BC_PERIODIC) continue;
-        double *f[2] = {d == 0 ? af.Jb[1] : af.Jb[0], d == 2 ? af.Jb[1] : af.Jb[2]};
-        fk_plane_copy<double>(c, f, 2, d, 1, g.n[d] + 1, 1);
-      }
-      // deposit^T + sort^T + BCs^T + push^T
-      PartStepAdjArgs pa;
-      pa.g = g;
-      for (int d = 0; d < 3; d++) {
-        pa.post.x[d] = post.x[d];
-        pa.post.v[d] = post.v[d];
-        pa.pre.x[d] = pre.x[d];
-        pa.pre.v[d] = pre.v[d];
-        pa.bx[d] = pb.x[d];
-        pa.bv[d] = pb.v[d];
-        pa.ox[d] = pb2.x[d];
-        pa.ov[d] = pb2.v[d];
-        pa.Jb[d] = af.Jb[d];
-        pa.xmin[d] = c->cfg.xyz0[d];
-        pa.xmax[d] = pa.xmin[d] + c->cfg.Lxyz[d];
-      }
-      pa.post.sp = post.sp;
-      pa.pre.sp = pre.sp;
-      pa.pre.gam = pre.gam;
-      for (int k = 0; k < 6; k++) {
-        pa.pre.fld[k] = pre.fld[k];
-        pa.of[k] = pb.fld[k];
-      }
-      pa.bg = pb.gam;
-      pa.og = pb2.gam;
-      pa.perm = pre.perm;
-      pa.sp = c->species;
-      pa.dt = dt;
-      for (int k = 0; k < 6; k++) pa.pbc[k] = c->cfg.parts_bound[k];
-      pa.rel = c->cfg.relativity;
-      pa.np = np;
-      {
-        TimedLaunch tl(c, "adj_particles");
-        hipLaunchKernelGGL(k_particle_step_adj, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, pa);
-      }
-      for (int d = 0; d < 3; d++) {
-        std::swap(pb.x[d], pb2.x[d]);
-        std::swap(pb.v[d], pb2.v[d]);
-      }
-      std::swap(pb.gam, pb2.gam);
     }
   }
-  // gather^T of the gather that produced the first push's fields
-  if (!rc && nsteps > 0) gather_adj(c, ck[0], pb, af);
+  if (!rc && hipStreamSynchronize(R.as) != hipSuccess) {
+    c->err = "adjoint stream failed";
+    rc = EMO_ERR_HIP;
+  }
   // ---- outputs -------------------------------------------------------------
   if (!rc) {
     if (energy) *energy = F;
     std::vector<double> sums((size_t)std::max(1, nsteps * nplanes));
-    EMO_HIP_CHECK_RET(c, hipMemcpyAsync(sums.data(), dsums, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    EMO_HIP_CHECK_RET(c, hipMemcpyAsync(sums.data(), R.dsums, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
     EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
     std::vector<double> da(nw, 0.0), dp(nw, 0.0), ta, tp;
     for (int s = 0; s < nsteps; s++)
       for (int k = 0; k < nplanes; k++) {
-        pulse_derivs(c, planes[k], tstep[s], dt, ta, tp);
+        pulse_derivs(c, R.planes[k], tstep[s], dt, ta, tp);
         for (int w = 0; w < nw; w++) {
           da[w] += sums[(size_t)s * nplanes + k] * ta[w];
           dp[w] += sums[(size_t)s * nplanes + k] * tp[w];
@@ -1064,7 +1142,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     if ((xbar0 || vbar0 || gbar0) && np > 0) {
       std::vector<int64_t> ids(np);
       std::vector<double> h(np);
-      EMO_HIP_CHECK_RET(c, hipMemcpy(ids.data(), ck[0].id, np * 8, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, hipMemcpy(ids.data(), R.ck[0].id, np * 8, hipMemcpyDeviceToHost));
       auto put = [&](double *dst, const double *src, int stride, int comp) -> int {
         EMO_HIP_CHECK_RET(c, hipMemcpy(h.data(), src, np * 8, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < np; i++) {
@@ -1074,15 +1152,16 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
         return EMO_OK;
       };
       for (int d = 0; d < 3 && !rc; d++) {
-        if (xbar0) rc = put(xbar0, pb.x[d], 3, d);
-        if (vbar0 && !rc) rc = put(vbar0, pb.v[d], 3, d);
+        if (xbar0) rc = put(xbar0, R.pb.x[d], 3, d);
+        if (vbar0 && !rc) rc = put(vbar0, R.pb.v[d], 3, d);
       }
-      if (gbar0 && !rc) rc = put(gbar0, pb.gam, 1, 0);
+      if (gbar0 && !rc) rc = put(gbar0, R.pb.gam, 1, 0);
     }
   }
   // ---- restore the final state (the buffers stay with the context) --------
+  hipStreamSynchronize(R.as);
   if (nsteps > 0 && fin_taken) {
-    int rc2 = snap_restore(c, fin);
+    int rc2 = snap_restore(c, R.fin);
     if (!rc) rc = rc2;
   }
   if (!rc) {
@@ -1092,6 +1171,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
       rc = EMO_ERR_HIP;
     }
   }
+  for (hipEvent_t e : ev) hipEventDestroy(e);
   return rc;
 }
 

@@ -2,8 +2,10 @@
 // the per-particle FP64 arithmetic in pic_math.hpp: the field interpolation,
 // the Boris and relativistic (Vay) pushes, the particle BCs and the current
 // deposit.  Each *_vjp takes the forward inputs, recomputes what it needs and
-// adds the input adjoints to the caller's accumulators.  Compiles as device
-// code and as plain host C++.
+// adds the input adjoints to the caller's accumulators.  Divisions by a
+// recomputed quantity are multiplications by its reciprocal here (the
+// forward's bit-exact operation order matters only in pic_math.hpp).
+// Compiles as device code and as plain host C++.
 #pragma once
 
 #include <cmath>
@@ -32,12 +34,12 @@ EMO_HD inline void cross3_vjp(const double a[3], const double b[3], const double
 EMO_HD inline void interp_fields_vjp(const Geo &g, const double cv[21], const double pos[3], const double fb[6],
                                      double posb[3], double cvb[21]) {
   const double *L = g.d;
+  const double iL[3] = {1.0 / L[0], 1.0 / L[1], 1.0 / L[2]};
   double ds[3], dsb[3] = {0, 0, 0};
   for (int q = 0; q < 3; q++) ds[q] = pos[q] - cv[q];
   for (int q = 0; q < 3; q++) {
     int j = (q + 1) % 3, k = (q + 2) % 3;
-    const double A = L[j] * L[k];
-    const double eb = fb[q] / A;
+    const double eb = fb[q] * (iL[j] * iL[k]);
     const double c0 = cv[3 + 4 * q], c1 = cv[3 + 4 * q + 1], c2 = cv[3 + 4 * q + 2], c3 = cv[3 + 4 * q + 3];
     cvb[3 + 4 * q] += eb * (L[j] - ds[j]) * (L[k] - ds[k]);
     cvb[3 + 4 * q + 1] += eb * ds[j] * (L[k] - ds[k]);
@@ -45,7 +47,7 @@ EMO_HD inline void interp_fields_vjp(const Geo &g, const double cv[21], const do
     cvb[3 + 4 * q + 3] += eb * (L[j] - ds[j]) * ds[k];
     dsb[j] += eb * (-(L[k] - ds[k]) * c0 + (L[k] - ds[k]) * c1 + ds[k] * c2 - ds[k] * c3);
     dsb[k] += eb * (-(L[j] - ds[j]) * c0 - ds[j] * c1 + ds[j] * c2 + (L[j] - ds[j]) * c3);
-    const double bb = fb[3 + q] / L[q];
+    const double bb = fb[3 + q] * iL[q];
     const double d0 = cv[15 + 2 * q], d1 = cv[15 + 2 * q + 1];
     cvb[15 + 2 * q] += bb * (L[q] - ds[q]);
     cvb[15 + 2 * q + 1] += bb * ds[q];
@@ -121,16 +123,17 @@ EMO_HD inline void vay_step_vjp(const double v[3], double gamma, const double f[
   double ustar = up[0] * tau[0] + up[1] * tau[1] + up[2] * tau[2];
   double r = sqrt(sigma * sigma + 4 * (tau2 + ustar * ustar));
   double gi = sqrt(0.5 * (sigma + r));
-  double tt[3] = {tau[0] / gi, tau[1] / gi, tau[2] / gi};
+  const double ig = 1.0 / gi, ir = 1.0 / r;
+  double tt[3] = {tau[0] * ig, tau[1] * ig, tau[2] * ig};
   double s = 1.0 / (1.0 + tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
-  double ud = ustar / gi;
+  double ud = ustar * ig;
   double upxt[3];
   cross3(up, tt, upxt);
   double w[3], ui[3], vn[3];
   for (int d = 0; d < 3; d++) {
     w[d] = up[d] + ud * tt[d] + upxt[d];
     ui[d] = s * w[d];
-    vn[d] = ui[d] / gi;
+    vn[d] = ui[d] * ig;
   }
   // reverse
   double vnb[3];
@@ -142,10 +145,10 @@ EMO_HD inline void vay_step_vjp(const double v[3], double gamma, const double f[
   double uib[3];
   double vnb_vn = 0.0;
   for (int d = 0; d < 3; d++) {
-    uib[d] = vnb[d] / gi;
+    uib[d] = vnb[d] * ig;
     vnb_vn += vnb[d] * vn[d];
   }
-  gib += -vnb_vn / gi;
+  gib += -vnb_vn * ig;
   double sb = 0.0, wb[3];
   for (int d = 0; d < 3; d++) {
     sb += uib[d] * w[d];
@@ -158,21 +161,21 @@ EMO_HD inline void vay_step_vjp(const double v[3], double gamma, const double f[
     ttb[d] += ud * wb[d];
     udb += wb[d] * tt[d];
   }
-  double ustarb = udb / gi;
-  gib += -udb * ustar / (gi * gi);
+  double ustarb = udb * ig;
+  gib += -udb * ustar * (ig * ig);
   for (int d = 0; d < 3; d++) ttb[d] += sb * (-s * s) * 2.0 * tt[d];
   double taub[3];
   double ttb_tau = 0.0;
   for (int d = 0; d < 3; d++) {
-    taub[d] = ttb[d] / gi;
+    taub[d] = ttb[d] * ig;
     ttb_tau += ttb[d] * tau[d];
   }
-  gib += -ttb_tau / (gi * gi);
-  double sigmab = gib * 0.25 / gi;
-  const double rb = gib * 0.25 / gi;
-  sigmab += rb * sigma / r;
-  double tau2b = rb * 2.0 / r;
-  ustarb += rb * 4.0 * ustar / r;
+  gib += -ttb_tau * (ig * ig);
+  double sigmab = gib * 0.25 * ig;
+  const double rb = gib * 0.25 * ig;
+  sigmab += rb * sigma * ir;
+  double tau2b = rb * 2.0 * ir;
+  ustarb += rb * 4.0 * ustar * ir;
   for (int d = 0; d < 3; d++) {
     upb[d] += ustarb * tau[d];
     taub[d] += ustarb * up[d];
@@ -221,14 +224,14 @@ EMO_HD inline void deposit_row_vjp(const Geo &g, const double cellverts[3], cons
                                    const double vel[3], double pcharge, const double rb[12], double posb[3],
                                    double velb[3]) {
   const double *L = g.d;
-  const double cellVolume = L[0] * L[1] * L[2];
+  const double iL[3] = {1.0 / L[0], 1.0 / L[1], 1.0 / L[2]};
+  const double qv = (kUnitRhoJ * pcharge) * (iL[0] * iL[1] * iL[2]);  // charge / cell volume
   double rp[3] = {pos[0] - cellverts[0], pos[1] - cellverts[1], pos[2] - cellverts[2]};
   double rpb[3] = {0, 0, 0};
   for (int q = 0; q < 3; q++) {
     int j = (q + 1) % 3, k = (q + 2) % 3;
-    const double cellArea = L[j] * L[k];
-    const double dcdv = ((kUnitRhoJ * pcharge) / cellVolume) / cellArea;
-    const double c = ((kUnitRhoJ * pcharge * vel[q]) / cellVolume) / cellArea;
+    const double dcdv = qv * (iL[j] * iL[k]);
+    const double c = dcdv * vel[q];
     const double w0 = (L[j] - rp[j]) * (L[k] - rp[k]), w1 = rp[j] * (L[k] - rp[k]), w2 = rp[j] * rp[k],
                  w3 = (L[j] - rp[j]) * rp[k];
     const double b0 = rb[4 * q], b1 = rb[4 * q + 1], b2 = rb[4 * q + 2], b3 = rb[4 * q + 3];

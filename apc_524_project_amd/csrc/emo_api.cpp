@@ -17,18 +17,19 @@ int load_maxwellian_host(const emo_config *cfg, const emo_loader *ld, std::vecto
                          std::vector<int32_t> &sp, std::vector<int64_t> &id,
                          std::vector<double> *dx_deps = nullptr);
 
-TimedLaunch::TimedLaunch(emo_ctx *ctx, const char *nm) : c(ctx), name(nm) {
+TimedLaunch::TimedLaunch(emo_ctx *ctx, const char *nm, hipStream_t stream)
+    : c(ctx), name(nm), s(stream ? stream : ctx->stream) {
   if (!c->timing) return;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
     a = b = nullptr;
     return;
   }
-  hipEventRecord(a, c->stream);
+  hipEventRecord(a, s);
 }
 
 TimedLaunch::~TimedLaunch() {
   if (!c->timing || !a) return;
-  hipEventRecord(b, c->stream);
+  hipEventRecord(b, s);
   c->pending[name].push_back({a, b});
 }
 
@@ -260,6 +261,7 @@ void emo_destroy(emo_ctx *c) {
   dfree(c->sort_tmp);
   plain_sort_destroy(c);
   dfree(c->adj_pool);
+  if (c->adj_stream) hipStreamDestroy(c->adj_stream);
   dfree(c->cj);
   dfree(c->cr);
   dfree(c->red);
@@ -638,8 +640,8 @@ int emo_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
       // per-particle fields equal (fresh) or would equal (deferred); the
       // store is in cell order, so the counting sort keeps the radix
       // sort's stable order
-      fused_push_f64(c, dt, nullptr);
-      rc = fused_sort_f64(c, nullptr);
+      fused_push_f64(c, dt, nullptr, nullptr);
+      rc = fused_sort_f64(c, nullptr, nullptr);
       if (rc) return rc;
     } else {
       fld_materialize(c);

@@ -125,6 +125,7 @@ struct emo_ctx {
   // for later calls
   void *adj_pool = nullptr;
   size_t adj_pool_bytes = 0;
+  hipStream_t adj_stream = nullptr;  // the adjoint's backward sweep
 
   // reductions / flags
   double *red = nullptr;   // scratch for FP64 reductions
@@ -164,8 +165,8 @@ void launch_particle_bc_f64(emo_ctx *c);
 int sort_f64(emo_ctx *c, int with_fields);
 int plain_sort_create(emo_ctx *c);
 void plain_sort_destroy(emo_ctx *c);
-void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out);
-int fused_sort_f64(emo_ctx *c, uint32_t *perm_out);
+void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out, uint8_t *flips_out);
+int fused_sort_f64(emo_ctx *c, uint32_t *perm_out, uint32_t *inv_out);
 constexpr int kFldFresh = 0, kFldStale = 1, kFldDeferred = 2;
 // stores the deferred gather (kFldDeferred -> kFldFresh)
 void fld_materialize(emo_ctx *c);
@@ -244,8 +245,9 @@ int fp32_diag(emo_ctx *c, double *out8);
 struct TimedLaunch {
   emo_ctx *c;
   const char *name;
+  hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
-  TimedLaunch(emo_ctx *ctx, const char *nm);
+  TimedLaunch(emo_ctx *ctx, const char *nm, hipStream_t stream = nullptr);
   ~TimedLaunch();
 };
 

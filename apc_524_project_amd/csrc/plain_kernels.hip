@@ -300,6 +300,7 @@ struct Pgd64Args {
   const double *x[3], *v[3], *gam;
   const int32_t *sp;
   double *fld[6];  // written only with write_fld (the adjoint's tape)
+  uint8_t *flips;  // with write_fld: bit d = odd number of reflections in dim d
   double *xo[3], *vo[3], *go;
   const double *E[3], *B[3], *Bt[3];
   SpeciesParams s;
@@ -342,6 +343,7 @@ __global__ void __launch_bounds__(kBlock) k_pgd64(Pgd64Args a) {
     else boris_step(pos, vel, f, a.s.q[sp], a.s.m[sp], a.dt);
     int flips[3];
     particle_bcs(pos, vel, a.xmin, a.xmax, a.pbc, flips);
+    if (a.write_fld) a.flips[i] = (uint8_t)((flips[0] & 1) | (flips[1] & 1) << 1 | (flips[2] & 1) << 2);
     int dx, dy, dz;
     if (cell_of(g, pos[0], pos[1], pos[2], &dx, &dy, &dz) < 0) {
       atomicOr(a.flag, kFlagOutOfGrid);
@@ -395,9 +397,15 @@ struct Scatter64Args {
   int32_t *spo;
   int64_t *ido;
   uint32_t *perm;                    // sorted position -> previous index
+  uint32_t *inv;                     // previous index -> sorted position (optional)
   unsigned long long *flag;
   long long np;
 };
+
+__global__ void __launch_bounds__(kBlock) k_invert_perm(const uint32_t *perm, uint32_t *inv, long long np) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < np) inv[perm[j]] = (uint32_t)j;
+}
 
 __global__ void __launch_bounds__(kBlock) k_scatter64(Scatter64Args a) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -414,6 +422,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter64(Scatter64Args a) {
   a.spo[pos] = a.spi[i];
   a.ido[pos] = a.idi[i];
   a.perm[pos] = (uint32_t)i;
+  if (a.inv) a.inv[i] = (uint32_t)pos;
 }
 
 }  // namespace
@@ -588,8 +597,9 @@ void plain_sort_destroy(emo_ctx *c) { cellsort_free(c->srt); }
 // The fused particle phase of the plain form on a cell-sorted store, part 1:
 // push (gathering the fields from the grid) and BCs into palt, with the move
 // code of every particle.  With fld_out the gathered fields are stored there
-// (six arrays, the pre-push order).
-void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out) {
+// (six arrays, the pre-push order), and the reflections of the particle BCs
+// in flips_out.
+void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out, uint8_t *flips_out) {
   const Geo &g = c->geo;
   const long long np = c->np;
   Pgd64Args a;
@@ -615,7 +625,8 @@ void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out) {
   for (int s = 0; s < 6; s++) a.pbc[s] = c->cfg.parts_bound[s];
   a.code = c->srt.code;
   a.nfar = c->srt.misc;
-  a.write_fld = fld_out != nullptr;
+  a.write_fld = fld_out != nullptr && flips_out != nullptr;
+  a.flips = flips_out;
   a.flag = c->dflag;
   a.np = np;
   {
@@ -629,8 +640,9 @@ void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out) {
 
 // part 2: the stable counting sort by the new cell and the scatter into the
 // main store; the permutation (new position -> previous index) goes to
-// perm_out (c->srt.perm if null) and the new cell ranges to c->srt.off.
-int fused_sort_f64(emo_ctx *c, uint32_t *perm_out) {
+// perm_out (c->srt.perm if null), its inverse to inv_out (if given), and the
+// new cell ranges to c->srt.off.
+int fused_sort_f64(emo_ctx *c, uint32_t *perm_out, uint32_t *inv_out) {
   if (!perm_out) perm_out = c->srt.perm;
   const Geo &g = c->geo;
   const long long np = c->np;
@@ -679,6 +691,9 @@ int fused_sort_f64(emo_ctx *c, uint32_t *perm_out) {
       c->err = "permutation copy failed";
       return EMO_ERR_HIP;
     }
+    if (inv_out)
+      hipLaunchKernelGGL(k_invert_perm, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, (const uint32_t *)perm_out,
+                         inv_out, (long long)np);
     c->fallback_sorts++;
     return EMO_OK;
   }
@@ -698,6 +713,7 @@ int fused_sort_f64(emo_ctx *c, uint32_t *perm_out) {
   sa.spo = c->palt.sp;
   sa.ido = c->palt.id;
   sa.perm = perm_out;
+  sa.inv = inv_out;
   sa.flag = c->dflag;
   sa.np = np;
   {
