@@ -39,7 +39,7 @@ SOURCES = [
     ("emo_api.cpp", ["-ffp-contract=off"]),
     ("loader.cpp", ["-ffp-contract=off"]),
     ("boundary.cpp", ["-ffp-contract=off"]),
-    ("chosen.hip", ["-ffp-contract=fast"]),
+    ("chosen.hip", ["-ffp-contract=off"]),
     ("adjoint.hip", ["-ffp-contract=off"]),
 ]
 

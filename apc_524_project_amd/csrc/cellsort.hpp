@@ -191,6 +191,28 @@ constexpr int kRankCells = 8;   // source cells per wavefront in the rank pass
 constexpr int kRankUnroll = 8;  // 64-particle chunks loaded before use
 constexpr int kMaxGroup = 65535;
 
+// Rank of each participating lane's key among the lanes of the wavefront
+// with the same key (in lane order), offset by the running count ct[key]
+// (LDS, one table per wavefront), which advances by the number of lanes
+// holding the key.  One ballot per distinct key.
+__device__ inline int wave_rank(int key, bool part, int *ct, int lane) {
+  unsigned long long rem = __ballot(part);
+  int myrank = 0;
+  while (rem) {
+    const int leader = __ffsll((long long)rem) - 1;
+    const int kl = __builtin_amdgcn_readlane(key, leader);
+    const bool mine = part && key == kl;
+    const unsigned long long bal = __ballot(mine);
+    const int base = ct[kl];
+    if (mine) myrank = base + __popcll(bal & lanes_below(lane));
+    wave_sync();
+    if (lane == 0) ct[kl] = base + __popcll(bal);
+    wave_sync();
+    rem &= ~bal;
+  }
+  return myrank;
+}
+
 struct RankArgs {
   const uint8_t *code;
   const int32_t *off;  // ranges of the (sorted) source cells
@@ -226,21 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_rank(RankArgs a) {
       const bool valid = p < pe && cc[u] != kFarCode;
       int j = 0;
       for (int t = 1; t < nc; t++) j += bnd[wv][t] <= p ? 1 : 0;
-      const int key = j * kCodes + cc[u];
-      unsigned long long rem = __ballot(valid);
-      int myrank = 0;
-      while (rem) {
-        const int leader = __ffsll((long long)rem) - 1;
-        const int kl = __shfl(key, leader, 64);
-        const bool mine = valid && key == kl;
-        const unsigned long long bal = __ballot(mine);
-        const int base = ct[kl];
-        if (mine) myrank = base + __popcll(bal & lanes_below(lane));
-        wave_sync();
-        if (lane == 0) ct[kl] = base + __popcll(bal);
-        wave_sync();
-        rem &= ~bal;
-      }
+      const int myrank = wave_rank(j * kCodes + cc[u], valid, ct, lane);
       if (valid) {
         if (myrank > kMaxGroup) atomicOr(a.flag, kFlagMigrate);
         a.rank[p] = (uint16_t)myrank;
@@ -406,22 +414,50 @@ inline int cellsort_scan(emo_ctx *c, CellSortBufs &b, int32_t *out, const char *
   return EMO_OK;
 }
 
+// the ordered far-mover list of the codes written by the push
+template <class CellPair>
+void cellsort_far(emo_ctx *c, CellSortBufs &b, long long np, CellPair cells, const char *name) {
+  const unsigned ntiles = (unsigned)std::max<long long>((np + kFarTile - 1) / kFarTile, 1);
+  TimedLaunch t(c, name);
+  hipLaunchKernelGGL(k_far_count, dim3(ntiles), dim3(kBlock), 0, c->stream, (const uint8_t *)b.code, np,
+                     (const int32_t *)b.misc, b.far_tiles);
+  hipLaunchKernelGGL(k_far_compact<CellPair>, dim3(ntiles), dim3(kBlock), 0, c->stream, (const uint8_t *)b.code,
+                     cells, np, (const int32_t *)b.misc, (const int32_t *)b.far_tiles, b.far, kFarCap, c->dflag);
+}
+
+inline void cellsort_periodic(const emo_ctx *c, int per[3]) {
+  for (int d = 0; d < 3; d++)
+    per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
+}
+
+// group offsets and the new ranges (b.off_new) from the group sizes b.nsk
+// and the far-mover list
+inline int cellsort_base_scan(emo_ctx *c, CellSortBufs &b, const char *base_name, const char *scan_name) {
+  const Geo &g = c->geo;
+  BaseArgs a;
+  a.g = g;
+  a.nsk = b.nsk;
+  a.base = b.gbase;
+  a.count = b.count;
+  cellsort_periodic(c, a.per);
+  a.nfar = b.misc;
+  a.far = b.far;
+  a.far_cap = kFarCap;
+  a.far_pos = b.far_pos;
+  a.flag = c->dflag;
+  {
+    TimedLaunch t(c, base_name);
+    hipLaunchKernelGGL(k_base, dim3(grid_for(g.ncell)), dim3(kBlock), 0, c->stream, a);
+  }
+  return cellsort_scan(c, b, b.off_new, scan_name);
+}
+
 // far lists, ranks, group offsets and the new ranges (b.off_new) for the
 // codes written by the push; b.off holds the current (source) ranges
 template <class CellPair>
 int cellsort_offsets(emo_ctx *c, CellSortBufs &b, long long np, CellPair cells, const char *const names[4]) {
   const Geo &g = c->geo;
-  int per[3];
-  for (int d = 0; d < 3; d++)
-    per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
-  const unsigned ntiles = (unsigned)std::max<long long>((np + kFarTile - 1) / kFarTile, 1);
-  {
-    TimedLaunch t(c, names[0]);
-    hipLaunchKernelGGL(k_far_count, dim3(ntiles), dim3(kBlock), 0, c->stream, (const uint8_t *)b.code, np,
-                       (const int32_t *)b.misc, b.far_tiles);
-    hipLaunchKernelGGL(k_far_compact<CellPair>, dim3(ntiles), dim3(kBlock), 0, c->stream, (const uint8_t *)b.code,
-                       cells, np, (const int32_t *)b.misc, (const int32_t *)b.far_tiles, b.far, kFarCap, c->dflag);
-  }
+  cellsort_far(c, b, np, cells, names[0]);
   {
     RankArgs r;
     r.code = b.code;
@@ -434,22 +470,7 @@ int cellsort_offsets(emo_ctx *c, CellSortBufs &b, long long np, CellPair cells, 
     TimedLaunch t(c, names[1]);
     hipLaunchKernelGGL(k_rank, dim3((unsigned)((waves + 3) / 4)), dim3(kBlock), 0, c->stream, r);
   }
-  {
-    BaseArgs a;
-    a.g = g;
-    a.nsk = b.nsk;
-    a.base = b.gbase;
-    a.count = b.count;
-    for (int d = 0; d < 3; d++) a.per[d] = per[d];
-    a.nfar = b.misc;
-    a.far = b.far;
-    a.far_cap = kFarCap;
-    a.far_pos = b.far_pos;
-    a.flag = c->dflag;
-    TimedLaunch t(c, names[2]);
-    hipLaunchKernelGGL(k_base, dim3(grid_for(g.ncell)), dim3(kBlock), 0, c->stream, a);
-  }
-  return cellsort_scan(c, b, b.off_new, names[3]);
+  return cellsort_base_scan(c, b, names[2], names[3]);
 }
 
 // the position of particle i with move code k and destination cell dst

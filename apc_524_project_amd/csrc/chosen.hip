@@ -1,12 +1,14 @@
 // chosen.hip -- the FP32 chosen form of the PIC step for gfx950.
 //
-// Particles are a cell-sorted structure of arrays: one int32 word with the
-// species in the top 5 bits and the reference's ghost-inclusive cell id
-// ((ny+2)(nz+2) ix + (nz+2) iy + iz, src/grid/oGrid.cpp:208-222) in the low
-// 26 bits, the in-cell offset f in [0,1)^3 (FP32) and u (FP32: v for Boris,
-// gamma*v for the relativistic pusher).  28 bytes per particle, plus an
-// optional int32 id that rides along for comparisons against the FP64 oracle.
-// Cell c holds the particles [off[c], off[c+1]) of the arrays.
+// Particles are kept sorted by cell as two arrays of 16-byte records:
+//   A = {word, fx, fy, fz}: word holds the species in the top 6 bits and the
+//       reference's ghost-inclusive cell id ((ny+2)(nz+2) ix + (nz+2) iy + iz,
+//       src/grid/oGrid.cpp:208-222) in the low 26; f in [0,1)^3 is the
+//       in-cell offset (FP32);
+//   B = {ux, uy, uz, id}: u (FP32) is v for Boris and gamma*v for the
+//       relativistic pusher; id is the particle's int32 id (-1 if none).
+// 32 bytes per particle, each record one 16-byte vector access.  Cell c holds
+// the particles [off[c], off[c+1]).
 //
 // The algorithm is the reference's (same step order, same defects); only the
 // representation and precision change.  With x = x0 + dx (ix - 1 + f) the
@@ -14,26 +16,41 @@
 // relpos / l = f - 1, so its linear weights (l - r)/l, r/l become 2 - f and
 // f - 1 (src/particles/interpolate.cpp:25-33, src/particles/deposit.cpp:94-97).
 //
-// Per step:
-//   pgd_f32      gather (E, (B + B_tm1)/2) + push + particle BCs, one thread
-//                per particle; writes the new state beside the old one and
-//                the particle's move code: which of the <= 6 x 6 x 6 one-cell
-//                moves (periodic wraps included) took it to its new cell
-//   far_f32      (only when some particle moved further than one cell) an
-//                ordered list of those particles
-//   rank_f32     per source cell and move code, the particles' ranks in array
-//                order and the group sizes (8 source cells per wavefront)
-//   base_f32     per destination cell, the offsets of its (source, code)
-//                groups in ascending source id; its particle count
-//   scan_f32     exclusive scan of the counts: the new cell ranges
-//   scatter_f32  every particle to off[dst] + base + rank -- the stable
-//                counting sort by cell of the previous order, deterministic
-//                and without radix passes
-//   deposit_f32  16 lanes per cell: each lane sums its particles' 12 edge
-//                currents with compensated (two-float) accumulation, a fixed
-//                butterfly combines the lanes: deterministic, no atomics
-//   jsum_f32     per grid point, the fixed-order sum of the cell vectors
+// Per step, with the state double-buffered (old -> new):
+//   cellvec_f32       per cell, the 18 values the gather reads (E on the 4
+//                     edges of each component, (B + B_tm1)/2 on the 2 faces):
+//                     five 16-byte loads per particle instead of 24 scattered
+//                     ones
+//   push_rank_f32     source order, one wavefront per 4 source cells: gather
+//                     + push + particle BCs of every particle, which yield its
+//                     move code (which of the <= 6 x 6 x 6 one-cell moves,
+//                     periodic wraps included, took it to its new cell) and
+//                     its rank among the particles of its (source cell, code)
+//                     group; the group sizes.  Nothing else is written.
+//   far_f32           (only when some particle moved further than one cell)
+//                     an ordered list of those particles
+//   base_f32, scan    per destination cell, the offsets of its (source,
+//                     code) groups in ascending source id, its count, the new
+//                     cell ranges
+//   inv_f32           source order: new position off[dst] + base + rank of
+//                     every particle -> the inverse permutation
+//   move_deposit_f32  destination order, 16 lanes per cell: each new slot
+//                     takes its particle from the old state, repeats the
+//                     push of push_rank_f32 (the same function on the same
+//                     inputs, so the same result bit for bit; a disagreement
+//                     on the cell would be reported), writes the new state
+//                     and sums the 12 edge currents with compensated
+//                     (two-float) accumulation; a fixed butterfly combines
+//                     the lanes: deterministic, no atomics
+//   jsum_f32          per grid point, the fixed-order sum of the cell vectors
 //   fold, Yee E/B, ghost refresh   shared FP32 instances of the grid kernels
+// The permutation is the stable counting sort of the previous order by the
+// new cell.  Repeating the push costs arithmetic but saves writing and
+// re-reading an intermediate state: particle traffic is 32 B read in the
+// first pass and 4 + 32 + 32 B in the last.  Both particle passes run
+// software-pipelined (the next particle's loads are in flight during a
+// push), with the per-species constants in LDS, and are specialised for
+// all-periodic particle BCs.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -53,25 +70,25 @@ constexpr int32_t kCellMask = (1 << kSpeciesShift) - 1;
 
 struct ChosenState {
   int64_t cap = 0;
-  int32_t *cs = nullptr, *cs_alt = nullptr;
-  float *f[3] = {}, *f_alt[3] = {};
-  float *u[3] = {}, *u_alt[3] = {};
-  int32_t *id = nullptr, *id_alt = nullptr;
+  int4 *pa = nullptr, *pa_alt = nullptr;  // {word, f[3]}
+  int4 *pb = nullptr, *pb_alt = nullptr;  // {u[3], id}
+  bool have_id = false;
   float *g0 = nullptr;  // stored gamma for the first relativistic push (Q3)
   bool g0_active = false;
-  CellSortBufs srt;  // the counting sort's buffers (cellsort.hpp)
+  CellSortBufs srt;  // the counting sort's buffers (cellsort.hpp); srt.perm is the inverse permutation
+  int32_t *fdst = nullptr;  // destination cell of a far mover
   // initial sort of loaded particles (arbitrary order)
   uint32_t *key = nullptr, *key_alt = nullptr, *perm = nullptr, *perm_alt = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   int sort_bits = 0;
   float *cj = nullptr;  // 12 per cell, component-major
+  float4 *cv = nullptr;  // gather vectors, kCellVec per cell
   FieldSet<float> fs;
   float *Btfix[3] = {};
 };
 
 namespace {
-
 
 struct SpeciesF32 {
   float qp[EMO_NSPEC];  // ((0.5 dt q)/m) UNIT_ACC
@@ -93,183 +110,408 @@ SpeciesF32 species_f32(const emo_ctx *c, double dt) {
   return s;
 }
 
-// The particle BCs of the six sides in the reference's order
-// (src/particles/particle_handler.cpp:287-295) on one dimension in cell
-// units: p = ix-1 is the physical cell, P = p + f the position / dx.
-// Periodic (bc_p_periodic.cpp:44-87): P < 0 -> P + n; P > n -> P - n.
-// Reflecting (bc_p_reflecting.cpp:43-53): mirror at 0 or n and flip u.
-__device__ inline void bc_dim(int &ix, float &f, float &u, int n, int left, int right) {
-  int p = ix - 1;
-  if (p < 0) {
-    if (left == EMO_BC_PERIODIC) {
-      p += n;
-    } else {
-      if (f == 0.f) {
-        p = -p;
-      } else {
-        p = -p - 1;
-        f = 1.f - f;
-      }
-      u = -u;
-    }
-  }
-  if (p > n || (p == n && f > 0.f)) {
-    if (right == EMO_BC_PERIODIC) {
-      p -= n;
-    } else {
-      if (f == 0.f) {
-        p = 2 * n - p;
-      } else {
-        p = 2 * n - p - 1;
-        f = 1.f - f;
-      }
-      u = -u;
-    }
-  }
-  ix = p + 1;
+__device__ inline float as_f(int v) { return __int_as_float(v); }
+
+// Work-groups go to the 8 XCDs round-robin (block b to XCD b % 8).  This
+// renumbers them so that each XCD takes a contiguous eighth of the tiles,
+// and its L2 holds only the gather vectors and cell ranges of its own part
+// of the grid.  The grid is a multiple of 8 blocks; blocks past the last
+// tile find no work.
+constexpr unsigned kXcds = 8;
+__device__ inline unsigned xcd_tile() {
+  const unsigned per = gridDim.x / kXcds;
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
 }
+inline unsigned xcd_grid(unsigned blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
+__device__ inline int as_i(float v) { return __float_as_int(v); }
 
 // ---------------------------------------------------------------------------
-// Fused gather + push + particle BCs + move code + arrival counts
+// One particle's gather + push + particle BCs.  Both particle passes call
+// this on the same inputs; the file is compiled without floating-point
+// contraction (every fused multiply-add is an explicit fmaf), so the two calls
+// give the same result bit for bit.  Branch-free: a wavefront's lanes take
+// different sides of the boundary tests.
 // ---------------------------------------------------------------------------
-struct PgdArgs {
+struct PushK {
   Geo g;
-  const int32_t *cs;
-  const float *f[3], *u[3];
-  const float *g0;
-  int32_t *cs_out;
-  float *f_out[3], *u_out[3];
-  uint8_t *code;
-  int32_t *nfar;   // particles that moved beyond one cell
-  const float *E[3], *B[3], *Bt[3];
+  const float4 *cv;  // per cell, the 18 gather values (k_cellvec_f32)
   SpeciesF32 s;
   float dtid[3];  // dt / dx per dim
-  int pbc[6];
+  int lper[3], rper[3];  // periodic particle BC on the lower / upper side
   int per[3];     // periodic particle BCs per dim
-  unsigned long long *flag;
-  long long np;
+  FastDiv div1, div2;  // division by nc[1], nc[2]
 };
 
-template <bool REL>
-__global__ void __launch_bounds__(kBlock) k_pgd_f32(PgdArgs a) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const bool live = i < a.np;
+struct Pushed {
+  int cell;     // new cell id
+  int code;     // move code (kFarCode: further than one cell)
+  float f[3];   // new offset
+  float u[3];   // new u (after the BCs)
+  float v[3];   // new velocity (deposit)
+};
+
+// The gather vector of a cell (Grid::getFieldInterpolatorVec,
+// src/grid/oGrid.cpp:153-197, without the vertex): the 4 edge values of each
+// E component and the 2 face values of each (B + B_tm1)/2 component, as five
+// 16-byte vectors:
+//   {Ex(i,j,k), Ex(i,j+1,k), Ex(i,j+1,k+1), Ex(i,j,k+1)}
+//   {Ey(i,j,k), Ey(i,j,k+1), Ey(i+1,j,k+1), Ey(i+1,j,k)}
+//   {Ez(i,j,k), Ez(i+1,j,k), Ez(i+1,j+1,k), Ez(i,j+1,k)}
+//   {Bx(i), Bx(i+1), By(j), By(j+1)}   (face averages)
+//   {Bz(k), Bz(k+1), 0, 0}
+constexpr int kCellVec = 5;
+
+struct CellVecArgs {
+  Geo g;
+  const float *E[3], *B[3], *Bt[3];
+  float4 *cv;
+};
+
+__global__ void __launch_bounds__(kBlock) k_cellvec_f32(CellVecArgs a) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const Geo &g = a.g;
-  int cell = -1;
-  int mcode = 0;
-  if (live) {
-  int w = a.cs[i];
-  int sp = (int)((uint32_t)w >> kSpeciesShift);
+  if (c >= g.ncell) return;
   int ix, iy, iz;
-  decode_cell(g, w & kCellMask, &ix, &iy, &iz);
-  float fx = a.f[0][i], fy = a.f[1][i], fz = a.f[2][i];
-  float ux = a.u[0][i], uy = a.u[1][i], uz = a.u[2][i];
-  // gather (Grid::getFieldInterpolatorVec + Interpolator::interpolate_fields)
+  decode_cell(g, (int)c, &ix, &iy, &iz);
+  const long long o = gidx(g, ix, iy, iz), X = g.sx, Y = g.sy;
+  const float *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+  float4 *v = a.cv + c * kCellVec;
+  v[0] = make_float4(Ex[o], Ex[o + Y], Ex[o + Y + 1], Ex[o + 1]);
+  v[1] = make_float4(Ey[o], Ey[o + 1], Ey[o + X + 1], Ey[o + X]);
+  v[2] = make_float4(Ez[o], Ez[o + X], Ez[o + X + Y], Ez[o + Y]);
+  v[3] = make_float4(0.5f * (a.Bt[0][o] + a.B[0][o]), 0.5f * (a.Bt[0][o + X] + a.B[0][o + X]),
+                     0.5f * (a.Bt[1][o] + a.B[1][o]), 0.5f * (a.Bt[1][o + Y] + a.B[1][o + Y]));
+  v[4] = make_float4(0.5f * (a.Bt[2][o] + a.B[2][o]), 0.5f * (a.Bt[2][o + 1] + a.B[2][o + 1]), 0.f, 0.f);
+}
+
+// padded indices of a cell id (division by multiply-high)
+__device__ inline void decode_fast(const PushK &a, int cell, int &ix, int &iy, int &iz) {
+  const uint32_t t = fastdiv((uint32_t)cell, a.div2);
+  iz = cell - (int)(t * (uint32_t)a.g.nc[2]);
+  const uint32_t x = fastdiv(t, a.div1);
+  iy = (int)(t - x * (uint32_t)a.g.nc[1]);
+  ix = (int)x;
+}
+
+// One dimension of the move in cell units: split the new position pf of a
+// particle of padded cell ix into cell and offset, then the particle BCs of
+// the two sides in the reference's order (src/particles/particle_handler.cpp:
+// 287-295).  p = ix-1 is the physical cell, P = p + f the position / dx.
+// Periodic (bc_p_periodic.cpp:44-87): P < 0 -> P + n; P > n -> P - n.
+// Reflecting (bc_p_reflecting.cpp:43-53): mirror at 0 or n and flip u.
+__device__ inline void move_dim(float pf, int ix, int n, bool lper, bool rper, int &ix_out, float &f_out,
+                                float &u) {
+  const float fl = floorf(pf);
+  float fr = pf - fl;
+  int p = ix - 1 + (int)fl;
+  const bool round_up = fr >= 1.f;  // rounding of a tiny negative offset
+  fr = round_up ? 0.f : fr;
+  p = round_up ? p + 1 : p;
+  {
+    const bool L = p < 0;
+    const bool fz = fr == 0.f;
+    const int pl = lper ? p + n : (fz ? -p : -p - 1);
+    const float fl2 = (lper || fz) ? fr : 1.f - fr;
+    p = L ? pl : p;
+    fr = L ? fl2 : fr;
+    u = (L && !lper) ? -u : u;
+  }
+  {
+    const bool R = p > n || (p == n && fr > 0.f);
+    const bool fz = fr == 0.f;
+    const int pr = rper ? p - n : (fz ? 2 * n - p : 2 * n - p - 1);
+    const float fr2 = (rper || fz) ? fr : 1.f - fr;
+    p = R ? pr : p;
+    fr = R ? fr2 : fr;
+    u = (R && !rper) ? -u : u;
+  }
+  ix_out = p + 1;
+  f_out = fr;
+}
+
+// move_dim when both sides of the dimension are periodic
+__device__ inline void move_dim_periodic(float pf, int ix, int n, int &ix_out, float &f_out) {
+  const float fl = floorf(pf);
+  float fr = pf - fl;
+  int p = ix - 1 + (int)fl;
+  const bool round_up = fr >= 1.f;
+  fr = round_up ? 0.f : fr;
+  p = round_up ? p + 1 : p;
+  p = p < 0 ? p + n : p;
+  p = (p > n || (p == n && fr > 0.f)) ? p - n : p;
+  ix_out = p + 1;
+  f_out = fr;
+}
+
+// The cell's gather vector (five 16-byte loads).
+struct CellVec {
+  float4 Ex, Ey, Ez, B, Bz;
+};
+__device__ inline CellVec load_cellvec(const PushK &a, int w) {
+  const float4 *cv = a.cv + (long long)(w & kCellMask) * kCellVec;
+  return CellVec{cv[0], cv[1], cv[2], cv[3], cv[4]};
+}
+
+// q_p: ((0.5 dt q)/m) UNIT_ACC of the particle's species; g0: the stored
+// gamma for the first relativistic push (Q3), if use_g0
+template <bool REL, bool PER>
+__device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, int w, float fx, float fy,
+                                              float fz, float ux, float uy, float uz, float q_p, bool use_g0,
+                                              float g0, Pushed &o, unsigned long long *flag) {
+  const Geo &g = a.g;
+  int ix, iy, iz;
+  decode_fast(a, w & kCellMask, ix, iy, iz);
+  // gather (Interpolator::interpolate_fields, src/particles/interpolate.cpp:11-47)
+  // from the cell's gather vector: E bilinear over the 4 edges, B linear
+  // over 2 faces
   const float ax = 2.f - fx, bx = fx - 1.f;
   const float ay = 2.f - fy, by = fy - 1.f;
   const float az = 2.f - fz, bz = fz - 1.f;
-  const long long c000 = gidx(g, ix, iy, iz);
-  const long long sx = g.sx, sy = g.sy;
-  const float *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
-  float ex = ay * az * Ex[c000] + by * az * Ex[c000 + sy] + by * bz * Ex[c000 + sy + 1] +
-             ay * bz * Ex[c000 + 1];
-  float ey = az * ax * Ey[c000] + bz * ax * Ey[c000 + 1] + bz * bx * Ey[c000 + sx + 1] +
-             az * bx * Ey[c000 + sx];
-  float ez = ax * ay * Ez[c000] + bx * ay * Ez[c000 + sx] + bx * by * Ez[c000 + sx + sy] +
-             ax * by * Ez[c000 + sy];
-  float bxf = ax * 0.5f * (a.Bt[0][c000] + a.B[0][c000]) + bx * 0.5f * (a.Bt[0][c000 + sx] + a.B[0][c000 + sx]);
-  float byf = ay * 0.5f * (a.Bt[1][c000] + a.B[1][c000]) + by * 0.5f * (a.Bt[1][c000 + sy] + a.B[1][c000 + sy]);
-  float bzf = az * 0.5f * (a.Bt[2][c000] + a.B[2][c000]) + bz * 0.5f * (a.Bt[2][c000 + 1] + a.B[2][c000 + 1]);
-  const float q_p = a.s.qp[sp];
-  float vx, vy, vz;
+  const float4 vEx = V.Ex, vEy = V.Ey, vEz = V.Ez, vB = V.B, vBz = V.Bz;
+  const float ex = fmaf(az, fmaf(ay, vEx.x, by * vEx.y), bz * fmaf(by, vEx.z, ay * vEx.w));
+  const float ey = fmaf(ax, fmaf(az, vEy.x, bz * vEy.y), bx * fmaf(bz, vEy.z, az * vEy.w));
+  const float ez = fmaf(ay, fmaf(ax, vEz.x, bx * vEz.y), by * fmaf(bx, vEz.z, ax * vEz.w));
+  const float bxf = fmaf(ax, vB.x, bx * vB.y);
+  const float byf = fmaf(ay, vB.z, by * vB.w);
+  const float bzf = fmaf(az, vBz.x, bz * vBz.y);
+  float ig_new = 1.f;
   if (REL) {
-    // Relativistic_Boris::Step (src/pusher/relativisticBoris.cpp:28-140) on u = gamma v
-    float gam = a.g0 ? a.g0[i] : sqrtf(1.f + ux * ux + uy * uy + uz * uz);
-    float ig = 1.f / gam;
-    float vx0 = ux * ig, vy0 = uy * ig, vz0 = uz * ig;
-    float upx = ux + q_p * (ex + (vy0 * bzf - vz0 * byf)) + q_p * ex;
-    float upy = uy + q_p * (ey + (vz0 * bxf - vx0 * bzf)) + q_p * ey;
-    float upz = uz + q_p * (ez + (vx0 * byf - vy0 * bxf)) + q_p * ez;
-    float taux = q_p * bxf, tauy = q_p * byf, tauz = q_p * bzf;
-    float tau_sq = taux * taux + tauy * tauy + tauz * tauz;
-    float sigma = 1.f + upx * upx + upy * upy + upz * upz - tau_sq;
-    float ustar = upx * taux + upy * tauy + upz * tauz;
-    float gi = sqrtf(0.5f * (sigma + sqrtf(sigma * sigma + 4.f * (tau_sq + ustar * ustar))));
-    float igi = 1.f / gi;
-    float tx = taux * igi, ty = tauy * igi, tz = tauz * igi;
-    float s = 1.f / (1.f + tx * tx + ty * ty + tz * tz);
-    float udott = ustar * igi;
-    ux = s * (upx + udott * tx + (upy * tz - upz * ty));
-    uy = s * (upy + udott * ty + (upz * tx - upx * tz));
-    uz = s * (upz + udott * tz + (upx * ty - upy * tx));
-    vx = ux * igi;
-    vy = uy * igi;
-    vz = uz * igi;
+    // Relativistic_Boris::Step (src/pusher/relativisticBoris.cpp:28-140) on
+    // u = gamma v.  Reciprocals and square roots are the hardware's
+    // single-instruction forms (about 1 ulp).
+    const float ig = use_g0 ? __builtin_amdgcn_rcpf(g0)
+                            : __builtin_amdgcn_rsqf(fmaf(ux, ux, fmaf(uy, uy, fmaf(uz, uz, 1.f))));
+    const float vx0 = ux * ig, vy0 = uy * ig, vz0 = uz * ig;
+    const float upx = fmaf(q_p, ex, fmaf(q_p, ex + fmaf(vy0, bzf, -(vz0 * byf)), ux));
+    const float upy = fmaf(q_p, ey, fmaf(q_p, ey + fmaf(vz0, bxf, -(vx0 * bzf)), uy));
+    const float upz = fmaf(q_p, ez, fmaf(q_p, ez + fmaf(vx0, byf, -(vy0 * bxf)), uz));
+    const float taux = q_p * bxf, tauy = q_p * byf, tauz = q_p * bzf;
+    const float tau_sq = fmaf(taux, taux, fmaf(tauy, tauy, tauz * tauz));
+    const float sigma = fmaf(upx, upx, fmaf(upy, upy, fmaf(upz, upz, 1.f))) - tau_sq;
+    const float ustar = fmaf(upx, taux, fmaf(upy, tauy, upz * tauz));
+    const float root = __builtin_amdgcn_sqrtf(fmaf(sigma, sigma, 4.f * fmaf(ustar, ustar, tau_sq)));
+    const float igi = __builtin_amdgcn_rsqf(0.5f * (sigma + root));
+    const float tx = taux * igi, ty = tauy * igi, tz = tauz * igi;
+    const float s = __builtin_amdgcn_rcpf(fmaf(tx, tx, fmaf(ty, ty, fmaf(tz, tz, 1.f))));
+    const float udott = ustar * igi;
+    ux = s * fmaf(udott, tx, upx + fmaf(upy, tz, -(upz * ty)));
+    uy = s * fmaf(udott, ty, upy + fmaf(upz, tx, -(upx * tz)));
+    uz = s * fmaf(udott, tz, upz + fmaf(upx, ty, -(upy * tx)));
+    ig_new = igi;
   } else {
     // Boris::Step (src/pusher/boris.cpp:12-75)
-    float tx = q_p * bxf, ty = q_p * byf, tz = q_p * bzf;
-    float bsqr = bxf * bxf + byf * byf + bzf * bzf;
-    float tsqr = 2.f / (1.f + q_p * q_p * bsqr);
-    float sx_ = tsqr * tx, sy_ = tsqr * ty, sz_ = tsqr * tz;
-    float v_mx = ux + q_p * ex, v_my = uy + q_p * ey, v_mz = uz + q_p * ez;
-    float v_ix = v_mx + v_my * tz - v_mz * ty;
-    float v_iy = v_my + v_mz * tx - v_mx * tz;
-    float v_iz = v_mz + v_mx * ty - v_my * tx;
-    ux = v_mx + v_iy * sz_ - v_iz * sy_ + q_p * ex;
-    uy = v_my + v_iz * sx_ - v_ix * sz_ + q_p * ey;
-    uz = v_mz + v_ix * sy_ - v_iy * sx_ + q_p * ez;
-    vx = ux;
-    vy = uy;
-    vz = uz;
+    const float tx = q_p * bxf, ty = q_p * byf, tz = q_p * bzf;
+    const float bsqr = fmaf(bxf, bxf, fmaf(byf, byf, bzf * bzf));
+    const float tsqr = 2.f * __builtin_amdgcn_rcpf(fmaf(q_p * q_p, bsqr, 1.f));
+    const float sx_ = tsqr * tx, sy_ = tsqr * ty, sz_ = tsqr * tz;
+    const float v_mx = fmaf(q_p, ex, ux), v_my = fmaf(q_p, ey, uy), v_mz = fmaf(q_p, ez, uz);
+    const float v_ix = v_mx + fmaf(v_my, tz, -(v_mz * ty));
+    const float v_iy = v_my + fmaf(v_mz, tx, -(v_mx * tz));
+    const float v_iz = v_mz + fmaf(v_mx, ty, -(v_my * tx));
+    ux = fmaf(q_p, ex, v_mx + fmaf(v_iy, sz_, -(v_iz * sy_)));
+    uy = fmaf(q_p, ey, v_my + fmaf(v_iz, sx_, -(v_ix * sz_)));
+    uz = fmaf(q_p, ez, v_mz + fmaf(v_ix, sy_, -(v_iy * sx_)));
   }
-  // x += dt v, in cell units, split into cell index and offset
-  float pf[3] = {fx + a.dtid[0] * vx, fy + a.dtid[1] * vy, fz + a.dtid[2] * vz};
-  int src[3] = {ix, iy, iz};
-  int ic[3] = {ix, iy, iz};
-  float uu[3] = {ux, uy, uz};
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    float fl = floorf(pf[d]);
-    ic[d] += (int)fl;
-    float fr = pf[d] - fl;
-    if (fr >= 1.f) {  // rounding of a tiny negative offset
-      fr = 0.f;
-      ic[d] += 1;
-    }
-    pf[d] = fr;
-    bc_dim(ic[d], pf[d], uu[d], g.n[d], a.pbc[2 * d], a.pbc[2 * d + 1]);
+  // x += dt v, in cell units, split into cell index and offset; BCs
+  int jx, jy, jz;
+  if (PER) {  // every particle BC periodic
+    move_dim_periodic(fmaf(a.dtid[0], ux * ig_new, fx), ix, g.n[0], jx, o.f[0]);
+    move_dim_periodic(fmaf(a.dtid[1], uy * ig_new, fy), iy, g.n[1], jy, o.f[1]);
+    move_dim_periodic(fmaf(a.dtid[2], uz * ig_new, fz), iz, g.n[2], jz, o.f[2]);
+  } else {
+    move_dim(fmaf(a.dtid[0], ux * ig_new, fx), ix, g.n[0], a.lper[0], a.rper[0], jx, o.f[0], ux);
+    move_dim(fmaf(a.dtid[1], uy * ig_new, fy), iy, g.n[1], a.lper[1], a.rper[1], jy, o.f[1], uy);
+    move_dim(fmaf(a.dtid[2], uz * ig_new, fz), iz, g.n[2], a.lper[2], a.rper[2], jz, o.f[2], uz);
   }
-  if (ic[0] < 0 || ic[0] >= g.nc[0] || ic[1] < 0 || ic[1] >= g.nc[1] || ic[2] < 0 || ic[2] >= g.nc[2]) {
-    atomicOr(a.flag, kFlagOutOfGrid);
-    ic[0] = min(max(ic[0], 1), g.n[0]);
-    ic[1] = min(max(ic[1], 1), g.n[1]);
-    ic[2] = min(max(ic[2], 1), g.n[2]);
+  if (jx < 0 || jx >= g.nc[0] || jy < 0 || jy >= g.nc[1] || jz < 0 || jz >= g.nc[2]) {
+    if (flag) atomicOr(flag, kFlagOutOfGrid);
+    jx = min(max(jx, 1), g.n[0]);
+    jy = min(max(jy, 1), g.n[1]);
+    jz = min(max(jz, 1), g.n[2]);
   }
-  cell = (ic[0] * g.nc[1] + ic[1]) * g.nc[2] + ic[2];
-  int k0 = move_code(ic[0] - src[0], g.n[0], a.per[0]);
-  int k1 = move_code(ic[1] - src[1], g.n[1], a.per[1]);
-  int k2 = move_code(ic[2] - src[2], g.n[2], a.per[2]);
-  mcode = (k0 < 0 || k1 < 0 || k2 < 0) ? (int)kFarCode : (k0 * 6 + k1) * 6 + k2;
-  a.cs_out[i] = (sp << kSpeciesShift) | cell;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    a.f_out[d][i] = pf[d];
-    a.u_out[d][i] = uu[d];
-  }
-  a.code[i] = (uint8_t)mcode;
-  }
-  const int nf = __popcll(__ballot(live && mcode == kFarCode));
-  if (lane == 0 && nf) atomicAdd(a.nfar, nf);
+  o.cell = (jx * g.nc[1] + jy) * g.nc[2] + jz;
+  const int k0 = move_code(jx - ix, g.n[0], PER ? 1 : a.per[0]);
+  const int k1 = move_code(jy - iy, g.n[1], PER ? 1 : a.per[1]);
+  const int k2 = move_code(jz - iz, g.n[2], PER ? 1 : a.per[2]);
+  o.code = (k0 < 0 || k1 < 0 || k2 < 0) ? (int)kFarCode : (k0 * 6 + k1) * 6 + k2;
+  o.u[0] = ux;
+  o.u[1] = uy;
+  o.u[2] = uz;
+  o.v[0] = ux * ig_new;
+  o.v[1] = uy * ig_new;
+  o.v[2] = uz * ig_new;
 }
 
 // ---------------------------------------------------------------------------
-// Far movers (rare: |v| dt > dx in some dimension).  An ordered list, in
-// particle order (= ascending source cell), built in two passes that return
-// at once when there are none.
+// Pass 1: push, move codes, ranks and group sizes.  One wavefront owns 8
+// consecutive source cells, so the ranks within each (source cell, code)
+// group are complete when it finishes.
 // ---------------------------------------------------------------------------
+constexpr int kCellsPerWave = 8;  // source cells per wavefront of the inverse-permutation pass
+constexpr int kPushCells = 4;     // source cells per wavefront of the push pass
+
+struct PushRankArgs {
+  PushK k;
+  const int4 *pa, *pb;
+  const float *g0;
+  const int32_t *off;
+  long long ncell;
+  uint8_t *code;
+  uint16_t *rank;
+  uint16_t *nsk;   // [code * ncell + source]
+  int32_t *fdst;   // destination cells of far movers
+  int32_t *nfar;
+  unsigned long long *flag;
+};
+
+template <bool REL, bool PER>
+__global__ void __launch_bounds__(kBlock) k_push_rank_f32(PushRankArgs a) {
+  __shared__ int cnt[kBlock / 64][kPushCells * kCodes];
+  __shared__ int bnd[kBlock / 64][kPushCells + 1];
+  __shared__ float qp_s[EMO_NSPEC];
+  // species constants in LDS: a per-lane load from the argument block would
+  // make every iteration wait for all loads in flight
+  if (threadIdx.x < EMO_NSPEC) qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long s0 = ((long long)xcd_tile() * (kBlock / 64) + wv) * kPushCells;
+  if (s0 >= a.ncell) return;
+  const int nc = (int)min((long long)kPushCells, a.ncell - s0);
+  int *ct = cnt[wv];
+  for (int e = lane; e < nc * kCodes; e += 64) ct[e] = 0;
+  if (lane <= nc) bnd[wv][lane] = a.off[s0 + lane];
+  wave_sync();
+  const int pb = bnd[wv][0], pe = bnd[wv][nc];
+  int nfar = 0;
+  const bool use_g0 = a.g0 != nullptr;
+  // software pipeline: the next chunk's states are in flight during a push.
+  // Lanes past the end load (and push) the last particle again, so that the
+  // loads are unconditional; only their stores are skipped.
+  int4 A = make_int4(0, 0, 0, 0), Bv = A;
+  float G = 0.f;
+  if (pb < pe) {
+    const int q = min(pb + lane, pe - 1);
+    A = a.pa[q];
+    Bv = a.pb[q];
+    if (use_g0) G = a.g0[q];
+  }
+  for (int p0 = pb; p0 < pe; p0 += 64) {
+    const int p = p0 + lane;
+    const bool valid = p < pe;
+    const CellVec V = load_cellvec(a.k, A.x);
+    const int pn = min(p + 64, pe - 1);
+    const int4 An = a.pa[pn], Bn = a.pb[pn];
+    float Gn = 0.f;
+    if (use_g0) Gn = a.g0[pn];
+    const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
+    Pushed o;
+    push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
+                       qp_s[sp], use_g0, G, o, a.flag);
+    const int mc = valid ? o.code : (int)kFarCode;
+    if (valid) {
+      if (mc == kFarCode) a.fdst[p] = o.cell;
+    }
+    A = An;
+    Bv = Bn;
+    G = Gn;
+    int j = 0;
+    for (int t = 1; t < nc; t++) j += bnd[wv][t] <= p ? 1 : 0;
+    const bool ranked = valid && mc != kFarCode;
+    const int myrank = wave_rank(j * kCodes + mc, ranked, ct, lane);
+    if (valid) {
+      a.code[p] = (uint8_t)mc;
+      if (ranked) {
+        if (myrank > kMaxGroup) atomicOr(a.flag, kFlagMigrate);
+        a.rank[p] = (uint16_t)myrank;
+      }
+    }
+    nfar += __popcll(__ballot(valid && mc == kFarCode));
+  }
+  wave_sync();
+  for (int e = lane; e < nc * kCodes; e += 64) {
+    const int v = ct[e];
+    if (v) a.nsk[(long long)(e % kCodes) * a.ncell + s0 + e / kCodes] = (uint16_t)min(v, kMaxGroup);
+  }
+  if (lane == 0 && nfar) atomicAdd(a.nfar, nfar);
+}
+
+// source and destination cells of a far mover (far-mover list)
+struct CellsF32 {
+  const int4 *pa;
+  const int32_t *fdst;
+  __device__ void operator()(long long i, int &s, int &d) const {
+    s = pa[i].x & kCellMask;
+    d = fdst[i];
+  }
+};
+
 // ---------------------------------------------------------------------------
-// Migration + deposit: one wavefront per destination cell.
+// Pass 2: the inverse permutation.  Same wavefront-to-cells mapping as pass 1;
+// particle p of source cell s with move code k goes to
+// off_new[s + shift(k)] + base[k][s] + rank[p].
+// ---------------------------------------------------------------------------
+struct InvArgs {
+  Geo g;
+  CellSortBufs b;
+  const int32_t *fdst;
+  int per[3];
+  uint32_t *inv;
+  unsigned long long *flag;
+};
+
+constexpr int kInvUnroll = 4;  // particles per lane in flight
+
+__global__ void __launch_bounds__(kBlock) k_inv_f32(InvArgs a) {
+  __shared__ int bnd[kBlock / 64][kCellsPerWave + 1];
+  __shared__ int shift[kCodes];  // cell-id shift of each move code
+  const Geo &g = a.g;
+  for (int k = threadIdx.x; k < kCodes; k += kBlock) {
+    const int kx = k / 36, ky = (k / 6) % 6, kz = k % 6;
+    shift[k] = (move_sigma(kx, g.n[0]) * g.nc[1] + move_sigma(ky, g.n[1])) * g.nc[2] + move_sigma(kz, g.n[2]);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long s0 = ((long long)xcd_tile() * (kBlock / 64) + wv) * kCellsPerWave;
+  if (s0 >= g.ncell) return;
+  const int nc = (int)min((long long)kCellsPerWave, g.ncell - s0);
+  if (lane <= nc) bnd[wv][lane] = a.b.off[s0 + lane];
+  wave_sync();
+  const int pb = bnd[wv][0], pe = bnd[wv][nc];
+  for (int p0 = pb + lane; p0 < pe; p0 += 64 * kInvUnroll) {
+    int k[kInvUnroll], r[kInvUnroll];
+#pragma unroll
+    for (int u = 0; u < kInvUnroll; u++) {
+      const int p = p0 + 64 * u;
+      k[u] = p < pe ? (int)a.b.code[p] : -1;
+      r[u] = p < pe ? (int)a.b.rank[p] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kInvUnroll; u++) {
+      const int p = p0 + 64 * u;
+      if (k[u] < 0) continue;
+      int j = 0;
+      for (int t = 1; t < nc; t++) j += bnd[wv][t] <= p ? 1 : 0;
+      const int src = (int)s0 + j;
+      long long pos;
+      if (k[u] != kFarCode) {
+        const int dst = src + shift[k[u]];
+        pos = -1;
+        if (dst >= 0 && dst < g.ncell) {
+          pos = (long long)a.b.off_new[dst] + a.b.gbase[(long long)k[u] * g.ncell + src] + r[u];
+          if (pos >= a.b.off_new[dst + 1]) pos = -1;
+        }
+        if (pos < 0) atomicOr(a.flag, kFlagMigrate);
+      } else {
+        pos = cellsort_pos(g, a.b, p, k[u], a.fdst[p], a.flag);
+      }
+      if (pos >= 0) a.inv[pos] = (uint32_t)p;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: move + deposit in destination order, 16 lanes per cell, 4 cells
+// per wavefront.
 // ---------------------------------------------------------------------------
 __device__ inline void two_sum(float a, float b, float &s, float &e) {
   s = a + b;
@@ -281,104 +523,99 @@ __device__ inline void two_sum(float a, float b, float &s, float &e) {
 // src/particles/deposit.cpp:12-40, weights 2 - f and f - 1), accumulated
 // with TwoSum into (hi, lo)
 __device__ inline void deposit_j(float hi[12], float lo[12], const float f[3], const float v[3], float jc) {
-  float A[3] = {2.f - f[0], 2.f - f[1], 2.f - f[2]};
-  float Bw[3] = {f[0] - 1.f, f[1] - 1.f, f[2] - 1.f};
+  const float A[3] = {2.f - f[0], 2.f - f[1], 2.f - f[2]};
+  const float Bw[3] = {f[0] - 1.f, f[1] - 1.f, f[2] - 1.f};
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    int j = (q + 1) % 3, k = (q + 2) % 3;
-    float cq = jc * v[q];
-    float w4[4] = {A[j] * A[k], Bw[j] * A[k], Bw[j] * Bw[k], A[j] * Bw[k]};
+    const int j = (q + 1) % 3, k = (q + 2) % 3;
+    const float cq = jc * v[q];
+    const float ca = cq * A[j], cb = cq * Bw[j];
+    const float w4[4] = {ca * A[k], cb * A[k], cb * Bw[k], ca * Bw[k]};
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       float s, err;
-      two_sum(hi[4 * q + r], cq * w4[r], s, err);
+      two_sum(hi[4 * q + r], w4[r], s, err);
       hi[4 * q + r] = s;
       lo[4 * q + r] += err;
     }
   }
 }
 
-// ---------------------------------------------------------------------------
-// Stable counting sort by destination cell, in four streaming passes.
-// Within a destination cell the order is: source cells in ascending id, and
-// within one source cell the particles' previous order -- the stable sort of
-// the previous order by the new cell.  A particle's position is
-//   off_new[dst] + base[code][src] + rank
-// with rank its index among the particles of its (source cell, move code)
-// group and base the number of particles that reach dst from (source, code)
-// groups of lower source id.
-// ---------------------------------------------------------------------------
-struct ScatterArgs {
-  Geo g;
-  CellSortBufs b;
-  const int32_t *cs_in;
-  const float *f_in[3], *u_in[3];
-  const int32_t *id_in;
-  int32_t *cs_out;
-  float *f_out[3], *u_out[3];
-  int32_t *id_out;
-  unsigned long long *flag;
+struct MoveDepArgs {
+  PushK k;
+  const int4 *pa, *pb;  // old state, source order
+  const float *g0;
+  int4 *pa_out, *pb_out;
+  const uint32_t *inv;
+  const int32_t *off;  // new cell ranges
+  long long ncell;
   long long np;
-};
-
-__global__ void __launch_bounds__(kBlock) k_scatter_f32(ScatterArgs a) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
-  const int w = a.cs_in[i];
-  const long long pos = cellsort_pos(a.g, a.b, i, a.b.code[i], w & kCellMask, a.flag);
-  if (pos < 0) return;
-  a.cs_out[pos] = w;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    a.f_out[d][pos] = a.f_in[d][i];
-    a.u_out[d][pos] = a.u_in[d][i];
-  }
-  if (a.id_in) a.id_out[pos] = a.id_in[i];
-}
-
-// source and destination cells of an FP32 particle (far-mover list)
-struct CellsF32 {
-  const int32_t *src, *dst;
-  __device__ void operator()(long long i, int &s, int &d) const {
-    s = src[i] & kCellMask;
-    d = dst[i] & kCellMask;
-  }
-};
-
-// ---------------------------------------------------------------------------
-// Deposit: 16 lanes per cell, 4 cells per wavefront.  Each lane sums its
-// particles' 12 edge currents with TwoSum; a fixed xor butterfly within the
-// 16-lane group combines the (hi, lo) pairs: deterministic, no atomics.
-// ---------------------------------------------------------------------------
-struct DepF32Args {
-  Geo g;
-  const int32_t *cs;
-  const float *f[3], *u[3];
-  const int32_t *off;
-  SpeciesF32 s;
   float *cj;
+  unsigned long long *flag;
 };
 
-template <bool REL>
-__global__ void __launch_bounds__(kBlock) k_deposit_f32(DepF32Args a) {
+template <bool REL, bool PER>
+__global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
+  __shared__ float qp_s[EMO_NSPEC], jc_s[EMO_NSPEC];
+  // species constants in LDS (a per-lane load from the argument block would
+  // make every iteration wait for all loads in flight); test particles
+  // deposit nothing (src/particles/particle_handler.cpp:205)
+  if (threadIdx.x < EMO_NSPEC) {
+    qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
+    jc_s[threadIdx.x] = a.k.s.test[threadIdx.x] ? 0.f : a.k.s.jc[threadIdx.x];
+  }
+  __syncthreads();
   const int sub = threadIdx.x & 15;
-  const long long c = (long long)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4);
-  const Geo &g = a.g;
+  const long long c = (long long)xcd_tile() * (kBlock / 16) + (threadIdx.x >> 4);
   float hi[12], lo[12];
 #pragma unroll
   for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
-  if (c < g.ncell) {
+  if (c < a.ncell) {
     const int b = a.off[c], e = a.off[c + 1];
-    for (int p = b + sub; p < e; p += 16) {
-      const int w = a.cs[p];
-      const int sp = (int)((uint32_t)w >> kSpeciesShift);
-      if (a.s.test[sp]) continue;
-      float f[3] = {a.f[0][p], a.f[1][p], a.f[2][p]};
-      float uq[3] = {a.u[0][p], a.u[1][p], a.u[2][p]};
-      float ig = REL ? 1.f / sqrtf(1.f + uq[0] * uq[0] + uq[1] * uq[1] + uq[2] * uq[2]) : 1.f;
-      float v[3] = {uq[0] * ig, uq[1] * ig, uq[2] * ig};
-      deposit_j(hi, lo, f, v, a.s.jc[sp]);
+    // software pipeline: the source index two slots ahead and the state one
+    // slot ahead are in flight while a particle is pushed.  A source index
+    // outside the store (a slot that the permutation did not fill) is an
+    // error; it is replaced by 0 where it is used, so that no access leaves
+    // the store.  Loads past the cell's end read its last slot (b < e
+    // there), so that the loads are unconditional.
+    const uint32_t np = (uint32_t)a.np;
+    const bool use_g0 = a.g0 != nullptr;
+    const int last = e - 1;
+    int j = b + sub;
+    bool bad = false;
+    uint32_t src = 0u, src_next = 0u;
+    if (j < e) {
+      src = a.inv[j];
+      src_next = a.inv[min(j + 16, last)];
     }
+    bad = src >= np;
+    src = bad ? 0u : src;
+    int4 A = a.pa[src], Bv = a.pb[src];
+    float G = 0.f;
+    if (use_g0) G = a.g0[src];
+    for (; j < e; j += 16) {
+      const CellVec V = load_cellvec(a.k, A.x);
+      const uint32_t src_next2 = a.inv[min(j + 32, last)];
+      bad = bad || src_next >= np;
+      const uint32_t sn = src_next < np ? src_next : 0u;
+      const int4 An = a.pa[sn], Bn = a.pb[sn];
+      float Gn = 0.f;
+      if (use_g0) Gn = a.g0[sn];
+      const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
+      Pushed o;
+      push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
+                         qp_s[sp], use_g0, G, o, nullptr);
+      bad = bad || o.cell != (int)c;
+      a.pa_out[j] = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)c), as_i(o.f[0]), as_i(o.f[1]),
+                              as_i(o.f[2]));
+      a.pb_out[j] = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
+      deposit_j(hi, lo, o.f, o.v, jc_s[sp]);
+      src_next = src_next2;
+      A = An;
+      Bv = Bn;
+      G = Gn;
+    }
+    if (bad) atomicOr(a.flag, kFlagMigrate);
   }
 #pragma unroll
   for (int off = 8; off > 0; off >>= 1) {
@@ -392,31 +629,27 @@ __global__ void __launch_bounds__(kBlock) k_deposit_f32(DepF32Args a) {
       lo[q] = lo[q] + ol + err;
     }
   }
-  if (sub == 0 && c < g.ncell) {
+  if (sub == 0 && c < a.ncell) {
 #pragma unroll
-    for (int q = 0; q < 12; q++) a.cj[q * g.ncell + c] = hi[q] + lo[q];
+    for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = hi[q] + lo[q];
   }
 }
 
 // ---------------------------------------------------------------------------
 // Initial sort of freshly loaded particles (arbitrary order): radix sort of
-// the cell ids, payload permutation, arrival counts.
+// the cell ids, payload permutation, particles per cell.
 // ---------------------------------------------------------------------------
-__global__ void k_keys_from_cs(const int32_t *cs, uint32_t *key, uint32_t *perm, long long np) {
+__global__ void k_keys_from_pa(const int4 *pa, uint32_t *key, uint32_t *perm, long long np) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  key[i] = (uint32_t)(cs[i] & kCellMask);
+  key[i] = (uint32_t)(pa[i].x & kCellMask);
   perm[i] = (uint32_t)i;
 }
 
 struct PermF32Args {
-  const int32_t *cs;
-  const float *f[3], *u[3];
-  const int32_t *id;
+  const int4 *pa, *pb;
   const float *g0;
-  int32_t *cs_out;
-  float *f_out[3], *u_out[3];
-  int32_t *id_out;
+  int4 *pa_out, *pb_out;
   float *g0_out;
   const uint32_t *perm;
   int32_t *count;
@@ -430,16 +663,11 @@ __global__ void __launch_bounds__(kBlock) k_permute_f32(PermF32Args a) {
   int cell = -1;
   if (live) {
     uint32_t s = a.perm[i];
-    int w = a.cs[s];
-    a.cs_out[i] = w;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      a.f_out[d][i] = a.f[d][s];
-      a.u_out[d][i] = a.u[d][s];
-    }
-    if (a.id) a.id_out[i] = a.id[s];
+    const int4 A = a.pa[s];
+    a.pa_out[i] = A;
+    a.pb_out[i] = a.pb[s];
     if (a.g0) a.g0_out[i] = a.g0[s];
-    cell = w & kCellMask;
+    cell = A.x & kCellMask;
   }
   // particles per cell: one atomic per run of equal cells (the output is sorted)
   const int prev = __shfl_up(cell, 1, 64);
@@ -455,13 +683,12 @@ __global__ void __launch_bounds__(kBlock) k_permute_f32(PermF32Args a) {
 
 struct OutArgs {
   Geo g;
-  const int32_t *cs;
-  const float *f[3], *u[3];
+  const int4 *pa, *pb;
   const float *g0;
   int rel;
   double *x[3], *v[3], *gam;
   int32_t *sp;
-  int32_t *cell;
+  int32_t *id;
   long long np;
 };
 
@@ -469,26 +696,30 @@ __global__ void __launch_bounds__(kBlock) k_output_f32(OutArgs a) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.np) return;
   const Geo &g = a.g;
-  int w = a.cs[i];
-  int cell = w & kCellMask;
+  const int4 A = a.pa[i], Bv = a.pb[i];
+  int cell = A.x & kCellMask;
   int ic[3];
   decode_cell(g, cell, &ic[0], &ic[1], &ic[2]);
-  double ux = a.u[0][i], uy = a.u[1][i], uz = a.u[2][i];
+  const double uu[3] = {as_f(Bv.x), as_f(Bv.y), as_f(Bv.z)};
+  const double ff[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
   double gam = 1.0;
-  if (a.rel) gam = a.g0 ? (double)a.g0[i] : sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-  double uu[3] = {ux, uy, uz};
+  if (a.rel) gam = a.g0 ? (double)a.g0[i] : sqrt(1.0 + uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2]);
   for (int d = 0; d < 3; d++) {
-    if (a.x[d]) a.x[d][i] = g.x0[d] + g.d[d] * ((double)(ic[d] - 1) + (double)a.f[d][i]);
+    if (a.x[d]) a.x[d][i] = g.x0[d] + g.d[d] * ((double)(ic[d] - 1) + ff[d]);
     if (a.v[d]) a.v[d][i] = a.rel ? uu[d] / gam : uu[d];
   }
   if (a.gam) a.gam[i] = gam;
-  if (a.sp) a.sp[i] = (int)((uint32_t)w >> kSpeciesShift);
-  if (a.cell) a.cell[i] = cell;
+  if (a.sp) a.sp[i] = (int)((uint32_t)A.x >> kSpeciesShift);
+  if (a.id) a.id[i] = Bv.w;
+}
+
+__global__ void k_cells_f32(const int4 *pa, int32_t *out, long long np) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) out[i] = pa[i].x & kCellMask;
 }
 
 struct DiagF32Args {
-  const int32_t *cs;
-  const float *u[3];
+  const int4 *pa, *pb;
   const float *g0;
   int rel;
   SpeciesParams s;
@@ -501,9 +732,10 @@ __global__ void __launch_bounds__(kBlock) k_diag_particles_f32(DiagF32Args a) {
   double acc[5] = {0, 0, 0, 0, 0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.np;
        i += (long long)gridDim.x * blockDim.x) {
-    int sp = (int)((uint32_t)a.cs[i] >> kSpeciesShift);
+    int sp = (int)((uint32_t)a.pa[i].x >> kSpeciesShift);
+    const int4 Bv = a.pb[i];
     double m = a.s.m[sp];
-    double ux = a.u[0][i], uy = a.u[1][i], uz = a.u[2][i];
+    double ux = as_f(Bv.x), uy = as_f(Bv.y), uz = as_f(Bv.z);
     double ig = 1.0;
     if (a.rel) ig = 1.0 / (a.g0 ? (double)a.g0[i] : sqrt(1.0 + ux * ux + uy * uy + uz * uz));
     double vx = ux * ig, vy = uy * ig, vz = uz * ig;
@@ -564,8 +796,7 @@ int sort_loaded(emo_ctx *c) {
   const long long np = c->np;
   cs_fill_i32(c, s->srt.count, 0, c->geo.ncell + 1);
   if (np > 0) {
-    hipLaunchKernelGGL(k_keys_from_cs, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, s->cs, s->key, s->perm,
-                       np);
+    hipLaunchKernelGGL(k_keys_from_pa, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, s->pa, s->key, s->perm, np);
     size_t bytes = s->sort_tmp_bytes;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->key, s->key_alt, s->perm,
                                                       s->perm_alt, (int)np, 0, s->sort_bits, c->stream);
@@ -576,28 +807,18 @@ int sort_loaded(emo_ctx *c) {
       if (rc) return rc;
     }
     PermF32Args pa;
-    pa.cs = s->cs;
-    pa.cs_out = s->cs_alt;
-    for (int d = 0; d < 3; d++) {
-      pa.f[d] = s->f[d];
-      pa.u[d] = s->u[d];
-      pa.f_out[d] = s->f_alt[d];
-      pa.u_out[d] = s->u_alt[d];
-    }
-    pa.id = s->id;
-    pa.id_out = s->id_alt;
+    pa.pa = s->pa;
+    pa.pb = s->pb;
+    pa.pa_out = s->pa_alt;
+    pa.pb_out = s->pb_alt;
     pa.g0 = s->g0_active ? s->g0 : nullptr;
     pa.g0_out = g0_new;
     pa.perm = s->perm_alt;
     pa.count = s->srt.count;
     pa.np = np;
     hipLaunchKernelGGL(k_permute_f32, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, pa);
-    std::swap(s->cs, s->cs_alt);
-    for (int d = 0; d < 3; d++) {
-      std::swap(s->f[d], s->f_alt[d]);
-      std::swap(s->u[d], s->u_alt[d]);
-    }
-    if (s->id) std::swap(s->id, s->id_alt);
+    std::swap(s->pa, s->pa_alt);
+    std::swap(s->pb, s->pb_alt);
     if (g0_new) {
       EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
       dfree(s->g0);
@@ -610,88 +831,120 @@ int sort_loaded(emo_ctx *c) {
   return EMO_OK;
 }
 
-void launch_pgd(emo_ctx *c, double dt) {
+PushK push_consts(emo_ctx *c, double dt) {
   ChosenState *s = c->ch;
-  PgdArgs a;
-  a.g = c->geo;
-  a.cs = s->cs;
-  a.cs_out = s->cs_alt;
+  const Geo &g = c->geo;
+  PushK k;
+  k.g = g;
+  k.cv = s->cv;
   for (int d = 0; d < 3; d++) {
-    a.f[d] = s->f[d];
-    a.u[d] = s->u[d];
-    a.f_out[d] = s->f_alt[d];
-    a.u_out[d] = s->u_alt[d];
-    a.E[d] = s->fs.E[d];
-    a.B[d] = s->fs.B[d];
-    a.Bt[d] = s->fs.Bt[d];
-    a.dtid[d] = (float)(dt * c->geo.id[d]);
-    a.per[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC || c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
+    k.dtid[d] = (float)(dt * g.id[d]);
+    k.lper[d] = c->cfg.parts_bound[2 * d] == EMO_BC_PERIODIC;
+    k.rper[d] = c->cfg.parts_bound[2 * d + 1] == EMO_BC_PERIODIC;
   }
-  a.g0 = s->g0_active ? s->g0 : nullptr;
-  a.code = s->srt.code;
-  a.nfar = s->srt.misc;
-  a.s = species_f32(c, dt);
-  for (int k = 0; k < 6; k++) a.pbc[k] = c->cfg.parts_bound[k];
-  a.flag = c->dflag;
-  a.np = c->np;
-  TimedLaunch t(c, "pgd_f32");
-  if (c->cfg.relativity)
-    hipLaunchKernelGGL(k_pgd_f32<true>, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, a);
-  else
-    hipLaunchKernelGGL(k_pgd_f32<false>, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, a);
+  cellsort_periodic(c, k.per);
+  k.s = species_f32(c, dt);
+  k.div1 = make_fastdiv((uint32_t)g.nc[1]);
+  k.div2 = make_fastdiv((uint32_t)g.nc[2]);
+  return k;
 }
 
-// far-mover list, ranks, group offsets, new cell ranges, scatter, deposit,
-// grid sum
-int launch_sort_deposit(emo_ctx *c) {
+unsigned cell_wave_blocks(long long ncell, int cells_per_wave) {
+  const long long waves = (ncell + cells_per_wave - 1) / cells_per_wave;
+  return (unsigned)std::max<long long>((waves + kBlock / 64 - 1) / (kBlock / 64), 1);
+}
+
+// One step's particle work: push + ranks, far list, group offsets, new
+// ranges, inverse permutation, move + deposit, grid sum of the currents.
+int particle_step(emo_ctx *c, double dt) {
   ChosenState *s = c->ch;
   const Geo &g = c->geo;
   const long long np = c->np;
-  static const char *const names[4] = {"far_f32", "rank_f32", "base_f32", "scan_f32"};
-  CellsF32 cells{s->cs, s->cs_alt};
-  int rc = cellsort_offsets(c, s->srt, np, cells, names);
+  {
+    CellVecArgs a;
+    a.g = g;
+    for (int d = 0; d < 3; d++) {
+      a.E[d] = s->fs.E[d];
+      a.B[d] = s->fs.B[d];
+      a.Bt[d] = s->fs.Bt[d];
+    }
+    a.cv = s->cv;
+    TimedLaunch t(c, "cellvec_f32");
+    hipLaunchKernelGGL(k_cellvec_f32, dim3(grid_for(g.ncell)), dim3(kBlock), 0, c->stream, a);
+  }
+  const PushK k = push_consts(c, dt);
+  const float *g0 = s->g0_active ? s->g0 : nullptr;
+  bool all_per = true;
+  for (int q = 0; q < 6; q++) all_per = all_per && c->cfg.parts_bound[q] == EMO_BC_PERIODIC;
+  {
+    PushRankArgs a;
+    a.k = k;
+    a.pa = s->pa;
+    a.pb = s->pb;
+    a.g0 = g0;
+    a.off = s->srt.off;
+    a.ncell = g.ncell;
+    a.code = s->srt.code;
+    a.rank = s->srt.rank;
+    a.nsk = s->srt.nsk;
+    a.fdst = s->fdst;
+    a.nfar = s->srt.misc;
+    a.flag = c->dflag;
+    TimedLaunch t(c, "push_rank_f32");
+    const dim3 grid(xcd_grid(cell_wave_blocks(g.ncell, kPushCells)));
+    if (c->cfg.relativity && all_per)
+      hipLaunchKernelGGL((k_push_rank_f32<true, true>), grid, dim3(kBlock), 0, c->stream, a);
+    else if (c->cfg.relativity)
+      hipLaunchKernelGGL((k_push_rank_f32<true, false>), grid, dim3(kBlock), 0, c->stream, a);
+    else if (all_per)
+      hipLaunchKernelGGL((k_push_rank_f32<false, true>), grid, dim3(kBlock), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_push_rank_f32<false, false>), grid, dim3(kBlock), 0, c->stream, a);
+  }
+  cellsort_far(c, s->srt, np, CellsF32{s->pa, s->fdst}, "far_f32");
+  int rc = cellsort_base_scan(c, s->srt, "base_f32", "scan_f32");
   if (rc) return rc;
   {
-    ScatterArgs a;
+    InvArgs a;
     a.g = g;
     a.b = s->srt;
-    a.cs_in = s->cs_alt;
-    a.cs_out = s->cs;
-    for (int d = 0; d < 3; d++) {
-      a.f_in[d] = s->f_alt[d];
-      a.u_in[d] = s->u_alt[d];
-      a.f_out[d] = s->f[d];
-      a.u_out[d] = s->u[d];
-    }
-    a.id_in = s->id;
-    a.id_out = s->id_alt;
+    a.fdst = s->fdst;
+    cellsort_periodic(c, a.per);
+    a.inv = s->srt.perm;
     a.flag = c->dflag;
-    a.np = np;
-    TimedLaunch t(c, "scatter_f32");
-    hipLaunchKernelGGL(k_scatter_f32, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, a);
+    TimedLaunch t(c, "inv_f32");
+    hipLaunchKernelGGL(k_inv_f32, dim3(xcd_grid(cell_wave_blocks(g.ncell, kCellsPerWave))), dim3(kBlock), 0, c->stream, a);
   }
+  {
+    MoveDepArgs a;
+    a.k = k;
+    a.pa = s->pa;
+    a.pb = s->pb;
+    a.g0 = g0;
+    a.pa_out = s->pa_alt;
+    a.pb_out = s->pb_alt;
+    a.inv = s->srt.perm;
+    a.off = s->srt.off_new;
+    a.ncell = g.ncell;
+    a.np = np;
+    a.cj = s->cj;
+    a.flag = c->dflag;
+    const unsigned blocks = xcd_grid((unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16)));
+    TimedLaunch t(c, "move_deposit_f32");
+    if (c->cfg.relativity && all_per)
+      hipLaunchKernelGGL((k_move_deposit_f32<true, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    else if (c->cfg.relativity)
+      hipLaunchKernelGGL((k_move_deposit_f32<true, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    else if (all_per)
+      hipLaunchKernelGGL((k_move_deposit_f32<false, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_move_deposit_f32<false, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+  }
+  std::swap(s->pa, s->pa_alt);
+  std::swap(s->pb, s->pb_alt);
   std::swap(s->srt.off, s->srt.off_new);
-  if (s->id) std::swap(s->id, s->id_alt);
   // the far-mover count starts from zero again
   cs_fill_i32(c, s->srt.misc, 0, 1);
-  {
-    DepF32Args d;
-    d.g = g;
-    d.cs = s->cs;
-    for (int q = 0; q < 3; q++) {
-      d.f[q] = s->f[q];
-      d.u[q] = s->u[q];
-    }
-    d.off = s->srt.off;
-    d.s = species_f32(c, 0.0);
-    d.cj = s->cj;
-    const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
-    TimedLaunch t(c, "deposit_f32");
-    if (c->cfg.relativity)
-      hipLaunchKernelGGL(k_deposit_f32<true>, dim3(blocks), dim3(kBlock), 0, c->stream, d);
-    else
-      hipLaunchKernelGGL(k_deposit_f32<false>, dim3(blocks), dim3(kBlock), 0, c->stream, d);
-  }
   DepGridArgs<float> b;
   b.g = g;
   b.cj = s->cj;
@@ -717,20 +970,18 @@ int fp32_create(emo_ctx *c) {
   s->cap = cap;
   c->cap = cap;
   int rc = 0;
-  if (!rc) rc = dalloc(c, &s->cs, cap);
-  if (!rc) rc = dalloc(c, &s->cs_alt, cap);
-  for (int d = 0; d < 3 && !rc; d++) {
-    rc = dalloc(c, &s->f[d], cap);
-    if (!rc) rc = dalloc(c, &s->f_alt[d], cap);
-    if (!rc) rc = dalloc(c, &s->u[d], cap);
-    if (!rc) rc = dalloc(c, &s->u_alt[d], cap);
-  }
+  if (!rc) rc = dalloc(c, &s->pa, cap);
+  if (!rc) rc = dalloc(c, &s->pa_alt, cap);
+  if (!rc) rc = dalloc(c, &s->pb, cap);
+  if (!rc) rc = dalloc(c, &s->pb_alt, cap);
+  if (!rc) rc = dalloc(c, &s->fdst, cap);
   if (!rc) rc = cellsort_alloc(c, s->srt, cap, g.ncell);
   if (!rc) rc = dalloc(c, &s->key, cap);
   if (!rc) rc = dalloc(c, &s->key_alt, cap);
   if (!rc) rc = dalloc(c, &s->perm, cap);
   if (!rc) rc = dalloc(c, &s->perm_alt, cap);
   if (!rc) rc = dalloc(c, &s->cj, 12 * g.ncell);
+  if (!rc) rc = dalloc(c, &s->cv, (size_t)kCellVec * g.ncell);
   size_t nt = (size_t)g.ntot;
   for (int q = 0; q < 3 && !rc; q++) {
     rc = dalloc(c, &s->fs.E[q], nt);
@@ -761,21 +1012,18 @@ int fp32_create(emo_ctx *c) {
 void fp32_destroy(emo_ctx *c) {
   ChosenState *s = c->ch;
   if (!s) return;
-  dfree(s->cs);
-  dfree(s->cs_alt);
+  dfree(s->pa);
+  dfree(s->pa_alt);
+  dfree(s->pb);
+  dfree(s->pb_alt);
+  dfree(s->fdst);
   for (int d = 0; d < 3; d++) {
-    dfree(s->f[d]);
-    dfree(s->f_alt[d]);
-    dfree(s->u[d]);
-    dfree(s->u_alt[d]);
     dfree(s->fs.E[d]);
     dfree(s->fs.Bfix[d]);
     dfree(s->Btfix[d]);
     dfree(s->fs.J[d]);
   }
   dfree(s->fs.rho);
-  dfree(s->id);
-  dfree(s->id_alt);
   dfree(s->g0);
   cellsort_free(s->srt);
   dfree(s->key);
@@ -784,6 +1032,7 @@ void fp32_destroy(emo_ctx *c) {
   dfree(s->perm_alt);
   dfree(s->sort_tmp);
   dfree(s->cj);
+  dfree(s->cv);
   delete s;
   c->ch = nullptr;
 }
@@ -794,17 +1043,13 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
               const int32_t *species, const int64_t *id) {
   ChosenState *s = c->ch;
   const Geo &g = c->geo;
-  std::vector<int32_t> cs(n), ids;
-  std::vector<float> f[3], u[3], g0;
-  for (int d = 0; d < 3; d++) {
-    f[d].resize(n);
-    u[d].resize(n);
-  }
+  std::vector<int4> ha(n), hb(n);
+  std::vector<float> g0;
   const bool rel = c->cfg.relativity != 0;
   if (rel) g0.resize(n);
-  if (id) ids.resize(n);
   for (int64_t i = 0; i < n; i++) {
     int ic[3];
+    float ff[3];
     for (int d = 0; d < 3; d++) {
       double r = (x3[3 * i + d] - g.x0[d]) * g.id[d];
       int ix = (int)r;
@@ -812,39 +1057,37 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
       if (ic[d] < 0 || ic[d] >= g.nc[d]) return set_err(c, EMO_ERR_INVARIANT, "particle outside the padded grid");
       // true in-cell fraction of the reference's cell (floor via the int cast)
       double fr = (x3[3 * i + d] - g.x0[d]) * g.id[d] - (double)ix;
-      float ff = (float)fr;
-      if (ff >= 1.f) ff = std::nextafter(1.f, 0.f);
-      if (ff < 0.f) ff = 0.f;
-      f[d][i] = ff;
+      float f = (float)fr;
+      if (f >= 1.f) f = std::nextafter(1.f, 0.f);
+      if (f < 0.f) f = 0.f;
+      ff[d] = f;
     }
-    cs[i] = (species[i] << kSpeciesShift) | ((ic[0] * g.nc[1] + ic[1]) * g.nc[2] + ic[2]);
+    int32_t w = (species[i] << kSpeciesShift) | ((ic[0] * g.nc[1] + ic[1]) * g.nc[2] + ic[2]);
     double gm = (rel && gamma) ? gamma[i] : 1.0;
-    for (int d = 0; d < 3; d++) u[d][i] = (float)(rel ? gm * v3[3 * i + d] : v3[3 * i + d]);
+    float u[3];
+    for (int d = 0; d < 3; d++) u[d] = (float)(rel ? gm * v3[3 * i + d] : v3[3 * i + d]);
     if (rel) g0[i] = (float)gm;
+    int32_t pid = -1;
     if (id) {
       if (id[i] < INT_MIN || id[i] > INT_MAX)
         return set_err(c, EMO_ERR_UNSUPPORTED, "particle ids must fit in int32 in the FP32 chosen form");
-      ids[i] = (int32_t)id[i];
+      pid = (int32_t)id[i];
     }
+    int fb[3], ub[3];
+    std::memcpy(fb, ff, 12);
+    std::memcpy(ub, u, 12);
+    ha[i] = make_int4(w, fb[0], fb[1], fb[2]);
+    hb[i] = make_int4(ub[0], ub[1], ub[2], pid);
   }
-  EMO_HIP_CHECK_RET(c, hipMemcpy(s->cs, cs.data(), n * 4, hipMemcpyHostToDevice));
-  for (int d = 0; d < 3; d++) {
-    EMO_HIP_CHECK_RET(c, hipMemcpy(s->f[d], f[d].data(), n * 4, hipMemcpyHostToDevice));
-    EMO_HIP_CHECK_RET(c, hipMemcpy(s->u[d], u[d].data(), n * 4, hipMemcpyHostToDevice));
+  if (n > 0) {
+    EMO_HIP_CHECK_RET(c, hipMemcpy(s->pa, ha.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+    EMO_HIP_CHECK_RET(c, hipMemcpy(s->pb, hb.data(), n * sizeof(int4), hipMemcpyHostToDevice));
   }
-  dfree(s->id);
-  dfree(s->id_alt);
-  s->id = s->id_alt = nullptr;
-  if (id) {
-    int rc = dalloc(c, &s->id, n);
-    if (!rc) rc = dalloc(c, &s->id_alt, n);
-    if (rc) return rc;
-    EMO_HIP_CHECK_RET(c, hipMemcpy(s->id, ids.data(), n * 4, hipMemcpyHostToDevice));
-  }
+  s->have_id = id != nullptr;
   dfree(s->g0);
   s->g0 = nullptr;
   s->g0_active = false;
-  if (rel) {
+  if (rel && n > 0) {
     int rc = dalloc(c, &s->g0, n);
     if (rc) return rc;
     EMO_HIP_CHECK_RET(c, hipMemcpy(s->g0, g0.data(), n * 4, hipMemcpyHostToDevice));
@@ -862,7 +1105,7 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
   int64_t n = c->np;
   if (n == 0) return EMO_OK;
   double *dx[3] = {}, *dv[3] = {}, *dg = nullptr;
-  int32_t *dsp = nullptr;
+  int32_t *dsp = nullptr, *did = nullptr;
   int rc = 0;
   for (int d = 0; d < 3 && !rc; d++) {
     rc = dalloc(c, &dx[d], n);
@@ -870,13 +1113,13 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
   }
   if (!rc) rc = dalloc(c, &dg, n);
   if (!rc) rc = dalloc(c, &dsp, n);
+  if (!rc) rc = dalloc(c, &did, n);
   if (!rc) {
     OutArgs a;
     a.g = c->geo;
-    a.cs = s->cs;
+    a.pa = s->pa;
+    a.pb = s->pb;
     for (int d = 0; d < 3; d++) {
-      a.f[d] = s->f[d];
-      a.u[d] = s->u[d];
       a.x[d] = dx[d];
       a.v[d] = dv[d];
     }
@@ -884,7 +1127,7 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
     a.rel = c->cfg.relativity;
     a.gam = dg;
     a.sp = dsp;
-    a.cell = nullptr;
+    a.id = did;
     a.np = n;
     hipLaunchKernelGGL(k_output_f32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);
     std::vector<double> buf(n);
@@ -901,13 +1144,9 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
     if (gamma && !rc) rc = down(gamma, dg, 1, 0);
     if (species && !rc && hipMemcpy(species, dsp, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
     if (id && !rc) {
-      if (s->id) {
-        std::vector<int32_t> h(n);
-        if (hipMemcpy(h.data(), s->id, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
-        for (int64_t i = 0; i < n && !rc; i++) id[i] = h[i];
-      } else {
-        for (int64_t i = 0; i < n; i++) id[i] = -1;
-      }
+      std::vector<int32_t> h(n);
+      if (hipMemcpy(h.data(), did, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
+      for (int64_t i = 0; i < n && !rc; i++) id[i] = s->have_id ? h[i] : -1;
     }
   }
   for (int d = 0; d < 3; d++) {
@@ -916,6 +1155,7 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
   }
   dfree(dg);
   dfree(dsp);
+  dfree(did);
   return rc;
 }
 
@@ -923,9 +1163,14 @@ int fp32_get_cells(emo_ctx *c, int32_t *cells) {
   ChosenState *s = c->ch;
   int64_t n = c->np;
   if (n == 0) return EMO_OK;
-  std::vector<int32_t> cs(n);
-  EMO_HIP_CHECK_RET(c, hipMemcpy(cs.data(), s->cs, n * 4, hipMemcpyDeviceToHost));
-  for (int64_t i = 0; i < n; i++) cells[i] = cs[i] & kCellMask;
+  int32_t *d = nullptr;
+  int rc = dalloc(c, &d, n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cells_f32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, s->pa, d, (long long)n);
+  hipError_t e = hipMemcpyAsync(cells, d, n * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dfree(d);
+  if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, hipGetErrorString(e));
   return EMO_OK;
 }
 
@@ -981,7 +1226,8 @@ int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
   ChosenState *s = c->ch;
   double t = t0;
   for (int k = 0; k < nsteps; k++) {
-    launch_pgd(c, dt);
+    int rc = particle_step(c, dt);
+    if (rc) return rc;
     if (s->g0_active) {
       // the stored gamma was used by this first push only (Q3): from now on
       // gamma = sqrt(1 + u^2)
@@ -990,8 +1236,6 @@ int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
       dfree(s->g0);
       s->g0 = nullptr;
     }
-    int rc = launch_sort_deposit(c);
-    if (rc) return rc;
     fk_fold<float>(c, s->fs);
     fk_evolve<float>(c, s->fs, dt);
     refresh_fields<float>(c, s->fs, t, dt);
@@ -1006,8 +1250,8 @@ int fp32_diag(emo_ctx *c, double *out8) {
   int nb = 256;
   double *pp = c->red, *pf = c->red + 5 * nb, *res = c->red + 11 * nb;
   DiagF32Args a;
-  a.cs = s->cs;
-  for (int d = 0; d < 3; d++) a.u[d] = s->u[d];
+  a.pa = s->pa;
+  a.pb = s->pb;
   a.g0 = s->g0_active ? s->g0 : nullptr;
   a.rel = c->cfg.relativity;
   a.s = c->species;

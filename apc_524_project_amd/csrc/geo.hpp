@@ -32,6 +32,29 @@ struct Geo {
   long long ncell;
 };
 
+// Division by a fixed divisor d >= 1 through a multiply-high (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994):
+// with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+// floor(n / d) = (mulhi(m, n) + n) >> l for every n < 2^31.
+struct FastDiv {
+  uint32_t m;
+  int l;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  int l = 0;
+  while ((1ull << l) < d) l++;
+  FastDiv f;
+  f.m = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+  f.l = l;
+  return f;
+}
+
+EMO_HD inline uint32_t fastdiv(uint32_t n, FastDiv f) {
+  const uint32_t t = (uint32_t)(((uint64_t)f.m * n) >> 32);
+  return (t + n) >> f.l;
+}
+
 EMO_HD inline long long gidx(const Geo &g, int i, int j, int k) {
   return ((long long)i * g.nt[1] + j) * g.nt[2] + k;
 }
