@@ -25,7 +25,7 @@ EX, EY, EZ, BX, BY, BZ, JX, JY, JZ, RHO, BX_TM1, BY_TM1, BZ_TM1 = range(13)
 # every function declared in include/emo_api.h
 EXPORTS = (
     "emo_create", "emo_destroy", "emo_last_error", "emo_version",
-    "emo_load_particles", "emo_process_info", "emo_load_maxwellian", "emo_num_particles",
+    "emo_load_particles", "emo_process_info", "emo_load_maxwellian", "emo_load_uniform", "emo_num_particles",
     "emo_get_particles", "emo_get_cell_ids", "emo_field_points", "emo_get_field", "emo_set_field",
     "emo_init_constant", "emo_init_poisson", "emo_sort", "emo_push", "emo_particle_bc",
     "emo_deposit", "emo_fold", "emo_evolve_fields", "emo_refresh", "emo_interpolate",
@@ -63,6 +63,7 @@ class EmoConfig(C.Structure):
         ("bound_Ay", C.c_double * 6),
         ("bound_Az", C.c_double * 6),
         ("max_particles", C.c_int64),
+        ("neutralize", C.c_int32),
     ]
 
 
@@ -115,6 +116,7 @@ def lib():
     L.emo_load_particles.argtypes = [vp, i64, vp, vp, vp, vp, vp]
     L.emo_process_info.argtypes = [C.POINTER(EmoLoader), C.POINTER(EmoConfig)]
     L.emo_load_maxwellian.argtypes = [vp, C.POINTER(EmoLoader)]
+    L.emo_load_uniform.argtypes = [vp, C.POINTER(EmoLoader)]
     L.emo_num_particles.argtypes = [vp]
     L.emo_num_particles.restype = i64
     L.emo_get_particles.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -187,6 +189,11 @@ class Context:
 
     def load_maxwellian(self, ld: EmoLoader):
         self._chk(self.L.emo_load_maxwellian(self.h, C.byref(ld)))
+
+    def load_uniform(self, ld: EmoLoader):
+        """Device-side loader (emo_load_uniform): per-cell uniform positions,
+        counter-based random velocities."""
+        self._chk(self.L.emo_load_uniform(self.h, C.byref(ld)))
 
     @property
     def num_particles(self):

@@ -62,6 +62,7 @@
 #include "cellsort.hpp"
 #include "deposit_grid.hpp"
 #include "emo_internal.hpp"
+#include "uniform_load.hpp"
 
 namespace emo {
 
@@ -633,6 +634,117 @@ __global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
 #pragma unroll
     for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = hi[q] + lo[q];
   }
+}
+
+// ---------------------------------------------------------------------------
+// rho + J of the current (sorted) state, for the Poisson initialisation
+// (src/main.cpp:160-166): the 12 edge currents and the 8 vertex charges of
+// Depositor::deposit_particle_J / _Rho (src/particles/deposit.cpp:12-62)
+// with the weights 2 - f and f - 1, 16 lanes per cell, compensated sums and a
+// fixed butterfly as in the step's deposit.
+// ---------------------------------------------------------------------------
+struct RhoJArgs {
+  const int4 *pa, *pb;
+  const float *g0;
+  const int32_t *off;
+  SpeciesF32 s;
+  int rel;
+  long long ncell;
+  float *cj, *cr;
+};
+
+__global__ void __launch_bounds__(kBlock) k_deposit_rhoj_f32(RhoJArgs a) {
+  const int sub = threadIdx.x & 15;
+  const long long c = (long long)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4);
+  float hi[20], lo[20];
+#pragma unroll
+  for (int q = 0; q < 20; q++) hi[q] = lo[q] = 0.f;
+  if (c < a.ncell) {
+    const int b = a.off[c], e = a.off[c + 1];
+    for (int j = b + sub; j < e; j += 16) {
+      const int4 A = a.pa[j], Bv = a.pb[j];
+      const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
+      if (a.s.test[sp]) continue;
+      const float f[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
+      const float ux = as_f(Bv.x), uy = as_f(Bv.y), uz = as_f(Bv.z);
+      float ig = 1.f;
+      if (a.rel) ig = a.g0 ? __builtin_amdgcn_rcpf(a.g0[j]) : __builtin_amdgcn_rsqf(1.f + ux * ux + uy * uy + uz * uz);
+      const float v[3] = {ux * ig, uy * ig, uz * ig};
+      const float jc = a.s.jc[sp];
+      deposit_j(hi, lo, f, v, jc);
+      const float Aw[3] = {2.f - f[0], 2.f - f[1], 2.f - f[2]};
+      const float Bw[3] = {f[0] - 1.f, f[1] - 1.f, f[2] - 1.f};
+      // vertex order of Depositor::deposit_particle_Rho
+      const float w8[8] = {Aw[0] * Aw[1] * Aw[2], Bw[0] * Aw[1] * Aw[2], Bw[0] * Bw[1] * Aw[2],
+                           Aw[0] * Bw[1] * Aw[2], Aw[0] * Bw[1] * Bw[2], Bw[0] * Bw[1] * Bw[2],
+                           Bw[0] * Aw[1] * Bw[2], Aw[0] * Aw[1] * Bw[2]};
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        float s2, err;
+        two_sum(hi[12 + r], jc * w8[r], s2, err);
+        hi[12 + r] = s2;
+        lo[12 + r] += err;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 20; q++) {
+      float oh = __shfl_xor(hi[q], off, 64);
+      float ol = __shfl_xor(lo[q], off, 64);
+      float s2, err;
+      two_sum(hi[q], oh, s2, err);
+      hi[q] = s2;
+      lo[q] = lo[q] + ol + err;
+    }
+  }
+  if (sub == 0 && c < a.ncell) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = hi[q] + lo[q];
+#pragma unroll
+    for (int q = 0; q < 8; q++) a.cr[q * a.ncell + c] = hi[12 + q] + lo[12 + q];
+  }
+}
+
+// emo_load_uniform's particles in the chosen representation (generation
+// order; sorted afterwards like any load): the cell and offset from the FP64
+// position with Grid::getCellID's arithmetic, as fp32_load does on the host
+struct GenF32Args {
+  UniformGen u;
+  int rel;
+  int4 *pa, *pb;
+  float *g0;
+  long long np;
+  unsigned long long *flag;
+};
+
+__global__ void __launch_bounds__(kBlock) k_uniform_f32(GenF32Args a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const Geo &g = a.u.g;
+  double x[3], v[3], gam;
+  int sp;
+  ugen_particle(a.u, i, x, v, &gam, &sp);
+  int ic[3];
+  int fb[3];
+  for (int d = 0; d < 3; d++) {
+    const double r = (x[d] - g.x0[d]) * g.id[d];
+    const int ix = (int)r;
+    ic[d] = ix + 1;
+    if (ic[d] < 0 || ic[d] >= g.nc[d]) {
+      atomicOr(a.flag, kFlagOutOfGrid);
+      ic[d] = 1;
+    }
+    float f = (float)(r - (double)ix);
+    f = f >= 1.f ? 0x1.fffffep-1f : (f < 0.f ? 0.f : f);
+    fb[d] = as_i(f);
+  }
+  const double gm = a.rel ? gam : 1.0;
+  a.pa[i] = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)((ic[0] * g.nc[1] + ic[1]) * g.nc[2] + ic[2])),
+                      fb[0], fb[1], fb[2]);
+  a.pb[i] = make_int4(as_i((float)(gm * v[0])), as_i((float)(gm * v[1])), as_i((float)(gm * v[2])), (int)i);
+  if (a.rel) a.g0[i] = (float)gm;
 }
 
 // ---------------------------------------------------------------------------
@@ -1218,8 +1330,157 @@ int fp32_init_constant(emo_ctx *c) {
   return check_flag(c);
 }
 
-int fp32_init_poisson(emo_ctx *c, int32_t, double, int32_t *, double *) {
-  return set_err(c, EMO_ERR_UNSUPPORTED, "Poisson initialisation is implemented in the FP64 plain form");
+int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n) {
+  ChosenState *s = c->ch;
+  const bool rel = c->cfg.relativity != 0;
+  dfree(s->g0);
+  s->g0 = nullptr;
+  s->g0_active = false;
+  if (rel && n > 0) {
+    int rc = dalloc(c, &s->g0, n);
+    if (rc) return rc;
+    s->g0_active = true;
+  }
+  if (n > 0) {
+    GenF32Args a;
+    a.u = u;
+    a.rel = rel;
+    a.pa = s->pa;
+    a.pb = s->pb;
+    a.g0 = s->g0;
+    a.np = n;
+    a.flag = c->dflag;
+    hipLaunchKernelGGL(k_uniform_f32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);
+  }
+  s->have_id = true;
+  c->np = n;
+  int rc = sort_loaded(c);
+  if (rc) return rc;
+  EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+  return check_flag(c);
+}
+
+// src/main.cpp:160-174 on the Poisson path in the chosen precision: deposit
+// (rho and J) + fold, E = B = B_tm1 = 0 (Q8), the four Jacobi solves of
+// Poisson_Solver::run_poisson_solver_ on FP32 arrays with the FP64 residual
+// norms deciding convergence (the reference's rule), phiToE, AToB, the ghost
+// refresh.  sweeps/resid as emo_init_poisson.
+int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_out, double *resid_out) {
+  if (iter_max < 0) return set_err(c, EMO_ERR_CONFIG, "iter_max must be >= 0");
+  ChosenState *s = c->ch;
+  const Geo &g = c->geo;
+  const size_t nt = (size_t)g.ntot;
+  float *cr = nullptr;
+  int rc = dalloc(c, &cr, 8 * g.ncell);
+  if (rc) return rc;
+  {
+    RhoJArgs a;
+    a.pa = s->pa;
+    a.pb = s->pb;
+    a.g0 = s->g0_active ? s->g0 : nullptr;
+    a.off = s->srt.off;
+    a.s = species_f32(c, 0.0);
+    a.rel = c->cfg.relativity;
+    a.ncell = g.ncell;
+    a.cj = s->cj;
+    a.cr = cr;
+    const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
+    hipLaunchKernelGGL(k_deposit_rhoj_f32, dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    DepGridArgs<float> b;
+    b.g = g;
+    b.cj = s->cj;
+    b.cr = cr;
+    for (int d = 0; d < 3; d++) b.J[d] = s->fs.J[d];
+    b.rho = s->fs.rho;
+    b.do_rho = 1;
+    hipLaunchKernelGGL(k_deposit_grid<float>, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, b);
+  }
+  fk_fold<float>(c, s->fs);
+  for (int q = 0; q < 3; q++) {
+    fk_const_fill<float>(c, s->fs.E[q], 0.0, (long long)nt);
+    fk_const_fill<float>(c, s->fs.B[q], 0.0, (long long)nt);
+    fk_const_fill<float>(c, s->fs.Bt[q], 0.0, (long long)nt);
+  }
+  const int nb = residual_blocks(c);
+  double *partials = nullptr;
+  float *u[8] = {};
+  rc = dalloc(c, &partials, 2 * nb + 2);
+  for (int q = 0; q < 8 && !rc; q++) rc = dalloc(c, &u[q], nt);
+  if (!rc && c->cfg.neutralize) fk_subtract_mean<float>(c, s->fs.rho, partials, nb, partials + 2 * nb);
+  for (int q = 0; q < 8 && !rc; q++) fk_const_fill<float>(c, u[q], 0.0, (long long)nt);
+  // run_poisson_solver_ coefficients (src/poisson/poisson.cpp:181-185)
+  const double dx_ = g.d[0], dy_ = g.d[1], dz_ = g.d[2];
+  const double celldist2 = dx_ * dx_ * dy_ * dy_ + dy_ * dy_ * dz_ * dz_ + dx_ * dx_ * dz_ * dz_;
+  const double ax = dy_ * dy_ * dz_ * dz_ / (2.0 * celldist2);
+  const double ay = dx_ * dx_ * dz_ * dz_ / (2.0 * celldist2);
+  const double az = dx_ * dx_ * dy_ * dy_ / (2.0 * celldist2);
+  const double af = dx_ * dx_ * dy_ * dy_ * dz_ * dz_ / (2.0 * celldist2);
+  const double sourceMult = -4 * kPi;
+  const emo_config &cf = c->cfg;
+  double pmax = cf.bound_phi[0], pmin = cf.bound_phi[0];
+  for (int i = 1; i < 6; i++) {
+    pmax = std::max(pmax, cf.bound_phi[i]);
+    pmin = std::min(pmin, cf.bound_phi[i]);
+  }
+  double amax = cf.bound_Ax[0], amin = cf.bound_Ax[0];
+  for (int i = 0; i < 6; i++) {
+    for (double v : {cf.bound_Ax[i], cf.bound_Ay[i], cf.bound_Az[i]}) {
+      amax = std::max(amax, v);
+      amin = std::min(amin, v);
+    }
+  }
+  const double tphi = tol >= 0 ? tol : std::max((pmax - pmin) * 1e-9, 1e-12);
+  const double tA = tol >= 0 ? tol : std::max((amax - amin) * 1e-9, 1e-12);
+  const double *bounds[4] = {cf.bound_phi, cf.bound_Ax, cf.bound_Ay, cf.bound_Az};
+  const float *srcs[4] = {s->fs.rho, s->fs.J[0], s->fs.J[1], s->fs.J[2]};
+  double resid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int32_t sweeps[4] = {0, 0, 0, 0};
+  std::vector<double> hpart(2 * nb);
+  for (int sv = 0; sv < 4 && !rc; sv++) {
+    float *u1 = u[2 * sv], *u2 = u[2 * sv + 1];
+    const double tl = sv == 0 ? tphi : tA;
+    potential_bc<float>(c, u1, bounds[sv]);
+    int iternum = -1;
+    bool converged = false;
+    do {
+      iternum++;
+      const float *src = (iternum % 2 == 0) ? u1 : u2;
+      float *dst = (iternum % 2 == 0) ? u2 : u1;
+      fk_jacobi<float>(c, src, dst, srcs[sv], ax, ay, az, af, sourceMult);
+      potential_bc<float>(c, dst, bounds[sv]);
+      fk_residual<float>(c, u1, u2, partials, nb);
+      hipError_t e = hipMemcpyAsync(hpart.data(), partials, 2 * nb * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) {
+        rc = set_err(c, EMO_ERR_HIP, std::string("poisson residual: ") + hipGetErrorString(e));
+        break;
+      }
+      double maxDiff = 0.0, l2 = 0.0;
+      for (int b = 0; b < nb; b++) {
+        maxDiff = std::max(maxDiff, hpart[2 * b]);
+        l2 += hpart[2 * b + 1];
+      }
+      resid[2 * sv] = maxDiff;
+      resid[2 * sv + 1] = std::sqrt(l2);
+      if (maxDiff < tl) converged = true;
+    } while (!converged && iternum < iter_max);
+    sweeps[sv] = iternum + 1;
+    if (sv == 0) fk_phi_to_e<float>(c, u1, s->fs.E);  // phi1_ holds the odd-numbered iterate (Q9)
+  }
+  if (!rc) {
+    fk_a_to_b<float>(c, u[2], u[4], u[6], s->fs.B);
+    refresh_fields<float>(c, s->fs, 0.0, 0.0);
+  }
+  hipStreamSynchronize(c->stream);
+  for (int q = 0; q < 8; q++) dfree(u[q]);
+  dfree(partials);
+  dfree(cr);
+  if (rc) return rc;
+  if (sweeps_out) memcpy(sweeps_out, sweeps, sizeof(sweeps));
+  if (resid_out) memcpy(resid_out, resid, sizeof(resid));
+  EMO_HIP_CHECK_RET(c, hipGetLastError());
+  return check_flag(c);
 }
 
 int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "emo_internal.hpp"
+#include "uniform_load.hpp"
 
 namespace emo {
 int process_info(const emo_loader *ld, emo_config *cfg, double *dens_out, double *temp_out);
@@ -331,6 +332,46 @@ int emo_load_maxwellian(emo_ctx *c, const emo_loader *ld) {
   return emo_load_particles(c, (int64_t)gam.size(), x.data(), v.data(), gam.data(), sp.data(), id.data());
 }
 
+int emo_load_uniform(emo_ctx *c, const emo_loader *ld) {
+  if (!c || !ld) return EMO_ERR_CONFIG;
+  double dens[EMO_NSPEC], temp[EMO_NSPEC];
+  int rc = process_info(ld, &c->cfg, dens, temp);
+  if (rc) return set_err(c, rc, "invalid loader configuration");
+  for (int s = 0; s < EMO_NSPEC; s++) {
+    c->species.q[s] = c->cfg.charge[s];
+    c->species.m[s] = c->cfg.mass[s] != 0 ? c->cfg.mass[s] : 1.0;
+    c->species.test[s] = c->cfg.isTestParticle[s];
+  }
+  const Geo &g = c->geo;
+  const int64_t nphys = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  const int64_t n = ld->nparticles_tot;
+  if (n < 0 || n > c->cap) return set_err(c, EMO_ERR_CONFIG, "particle count exceeds max_particles");
+  UniformGen u;
+  u.g = g;
+  u.key = splitmix64((uint64_t)ld->seed);
+  u.nspec = ld->nspecies;
+  int64_t tot = 0;
+  for (int s = 0; s < ld->nspecies; s++) {
+    const int64_t cnt = std::llround(dens[s] * (double)n);
+    if (cnt % nphys != 0)
+      return set_err(c, EMO_ERR_CONFIG, "the uniform loader needs a whole number of particles per cell and species");
+    tot += cnt / nphys;
+    if (tot > INT_MAX) return set_err(c, EMO_ERR_CONFIG, "too many particles per cell");
+    u.ppc_end[s] = (int)tot;
+    u.vth[s] = kUnitVth * std::sqrt(temp[s] / c->cfg.mass[s]);
+    for (int d = 0; d < 3; d++) u.drift[s][d] = ld->drift[s][d];
+  }
+  if (tot * nphys != n) return set_err(c, EMO_ERR_CONFIG, "species counts do not add up to nparticles_tot");
+  u.ppc = (int)tot;
+  if (c->cfg.precision != EMO_FP64_PLAIN) return fp32_load_uniform(c, u, n);
+  launch_uniform_f64(c, u, n);
+  EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+  c->np = n;
+  c->sorted = false;
+  c->fld_state = kFldStale;
+  return last_launch(c);
+}
+
 int64_t emo_num_particles(const emo_ctx *c) { return c ? c->np : -1; }
 
 int emo_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species, int64_t *id,
@@ -547,6 +588,8 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
     for (int q = 0; q < 8; q++) dfree(u[q]);
     return rc;
   }
+  // extension: a uniform neutralising background (emo_config.neutralize)
+  if (c->cfg.neutralize) fk_subtract_mean<double>(c, c->rho, partials, nb, partials + 2 * nb);
   // run_poisson_solver_ coefficients (src/poisson/poisson.cpp:181-185)
   double dx_ = g.d[0], dy_ = g.d[1], dz_ = g.d[2];
   double celldist2 = std::pow(dx_, 2.0) * std::pow(dy_, 2.0) + std::pow(dy_, 2.0) * std::pow(dz_, 2.0) +

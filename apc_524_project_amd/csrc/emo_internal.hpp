@@ -216,6 +216,22 @@ template <typename T>
 void fk_evolve(emo_ctx *c, FieldSet<T> &fs, double dt);
 template <typename T>
 void fk_diag_fields(emo_ctx *c, const FieldSet<T> &fs, double *partials, int nblocks);
+// Poisson initialisation kernels (src/poisson/poisson.cpp:223-260,
+// src/poisson/convertFields.cpp): one Jacobi sweep over the physical points,
+// FP64 residual partials of two iterates, phiToE into E[3], AToB into B[3]
+template <typename T>
+void fk_jacobi(emo_ctx *c, const T *src, T *dst, const T *R, double ax, double ay, double az, double af,
+               double sourceMult);
+template <typename T>
+void fk_residual(emo_ctx *c, const T *u1, const T *u2, double *partials, int nblocks);
+template <typename T>
+void fk_phi_to_e(emo_ctx *c, const T *phi, T *const *E);
+template <typename T>
+void fk_a_to_b(emo_ctx *c, const T *Ax, const T *Ay, const T *Az, T *const *B);
+// f -= its FP64 mean over the physical points (partials: nblocks doubles,
+// sum_out: one double, both device memory)
+template <typename T>
+void fk_subtract_mean(emo_ctx *c, T *f, double *partials, int nblocks, double *sum_out);
 
 // ---- boundary host logic (boundary.cpp) ----
 // Grid::executeBC(-1): periodic ghost copies (through Bfix for B) and LightBC
@@ -238,6 +254,9 @@ int fp32_get_field(emo_ctx *c, int fid, double *out);
 int fp32_set_field(emo_ctx *c, int fid, const double *in);
 int fp32_init_constant(emo_ctx *c);
 int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps, double *resid);
+struct UniformGen;  // uniform_load.hpp
+int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n);
+void launch_uniform_f64(emo_ctx *c, const UniformGen &u, int64_t n);
 int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt);
 int fp32_diag(emo_ctx *c, double *out8);
 

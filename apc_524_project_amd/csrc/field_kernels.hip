@@ -291,31 +291,33 @@ __global__ void __launch_bounds__(kBlock) k_const_fill(T *f, T v, long long n) {
 // Jacobi sweep, Poisson_Solver::run_poisson_solver_ (src/poisson/poisson.cpp:
 // 223-240) over the physical points.
 // ---------------------------------------------------------------------------
+template <typename T>
 struct JacobiArgs {
   Geo g;
-  const double *src;
-  double *dst;
-  const double *R;
-  double ax, ay, az, af, sourceMult;
+  const T *src;
+  T *dst;
+  const T *R;
+  T ax, ay, az, af, sourceMult;
 };
 
-__global__ void __launch_bounds__(kBlock) k_jacobi(JacobiArgs a) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_jacobi(JacobiArgs<T> a) {
   const Geo &g = a.g;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
   int i, j, k;
   phys_ijk(g, t, &i, &j, &k);
   long long c = gidx(g, i, j, k);
-  const double *u = a.src;
+  const T *u = a.src;
   a.dst[c] = a.ax * (u[c - g.sx] + u[c + g.sx]) + a.ay * (u[c - g.sy] + u[c + g.sy]) +
              a.az * (u[c - 1] + u[c + 1]) - a.af * a.R[c] * a.sourceMult;
 }
 
-// max|u2-u1| (:250-260) and sum (u2-u1)^2 per block: partials[2*b], [2*b+1].
-// The max is exact in any order; the L2 partials are reduced in a fixed
-// order by launch_reduce_partials.
-__global__ void __launch_bounds__(kBlock) k_residual(Geo g, const double *u1, const double *u2,
-                                                     double *partials) {
+// max|u2-u1| (:250-260) and sum (u2-u1)^2 per block in FP64 (for either
+// precision of u): partials[2*b], [2*b+1].  The max is exact in any order;
+// the L2 partials are reduced in a fixed order by launch_reduce_partials.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_residual(Geo g, const T *u1, const T *u2, double *partials) {
   __shared__ double smax[kBlock];
   __shared__ double ssum[kBlock];
   long long nphys = (long long)g.n[0] * g.n[1] * g.n[2];
@@ -325,7 +327,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(Geo g, const double *u1, co
     int i, j, k;
     phys_ijk(g, t, &i, &j, &k);
     long long c = gidx(g, i, j, k);
-    double dd = u2[c] - u1[c];
+    double dd = (double)u2[c] - (double)u1[c];
     double ad = fabs(dd);
     mx = ad > mx ? ad : mx;
     sm += dd * dd;
@@ -347,33 +349,67 @@ __global__ void __launch_bounds__(kBlock) k_residual(Geo g, const double *u1, co
 }
 
 // phiToE, src/poisson/convertFields.cpp:20-67
-__global__ void __launch_bounds__(kBlock) k_phi_to_e(Geo g, const double *phi, double *Ex, double *Ey,
-                                                     double *Ez) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_phi_to_e(Geo g, const T *phi, T *Ex, T *Ey, T *Ez) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
   int i, j, k;
   phys_ijk(g, t, &i, &j, &k);
   long long p = gidx(g, i, j, k);
-  Ex[p] = -(phi[p + g.sx] - phi[p]) / g.d[0];
-  Ey[p] = -(phi[p + g.sy] - phi[p]) / g.d[1];
-  Ez[p] = -(phi[p + 1] - phi[p]) / g.d[2];
+  Ex[p] = -(phi[p + g.sx] - phi[p]) / (T)g.d[0];
+  Ey[p] = -(phi[p + g.sy] - phi[p]) / (T)g.d[1];
+  Ez[p] = -(phi[p + 1] - phi[p]) / (T)g.d[2];
 }
 
 // AToB, src/poisson/convertFields.cpp:83-153 (not a curl: Q10)
-__global__ void __launch_bounds__(kBlock) k_a_to_b(Geo g, const double *Ax, const double *Ay,
-                                                   const double *Az, double *Bx, double *By,
-                                                   double *Bz) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_a_to_b(Geo g, const T *Ax, const T *Ay, const T *Az, T *Bx, T *By,
+                                                   T *Bz) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
   int i, j, k;
   phys_ijk(g, t, &i, &j, &k);
   long long p = gidx(g, i, j, k);
+  const T dx = (T)g.d[0], dy = (T)g.d[1], dz = (T)g.d[2];
   // Bx: Ak=Az (hk=dz), Aj=Ay (hj=dy); nextValk=Ak[j+1], nextValj=Ak[k+1]
-  Bx[p] = (Az[p + g.sy] - Az[p]) / g.d[1] + (Az[p + 1] - Ay[p]) / g.d[2];
+  Bx[p] = (Az[p + g.sy] - Az[p]) / dy + (Az[p + 1] - Ay[p]) / dz;
   // By: Ak=Ax (hk=dx), Aj=Az (hj=dz); nextValk=Ak[k+1], nextValj=Ak[i+1]
-  By[p] = (Ax[p + 1] - Ax[p]) / g.d[2] + (Ax[p + g.sx] - Az[p]) / g.d[0];
+  By[p] = (Ax[p + 1] - Ax[p]) / dz + (Ax[p + g.sx] - Az[p]) / dx;
   // Bz: Ak=Ay (hk=dy), Aj=Ax (hj=dx); nextValk=Ak[i+1], nextValj=Ak[j+1]
-  Bz[p] = (Ay[p + g.sx] - Ay[p]) / g.d[0] + (Ay[p + g.sy] - Ax[p]) / g.d[1];
+  Bz[p] = (Ay[p + g.sx] - Ay[p]) / dx + (Ay[p + g.sy] - Ax[p]) / dy;
+}
+
+// FP64 sum of a field over the physical points: per-block partials
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sum_phys(Geo g, const T *f, double *partials) {
+  __shared__ double sh[kBlock];
+  long long nphys = (long long)g.n[0] * g.n[1] * g.n[2];
+  double sm = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nphys;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    phys_ijk(g, t, &i, &j, &k);
+    sm += (double)f[gidx(g, i, j, k)];
+  }
+  sh[threadIdx.x] = sm;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// f -= mean over the physical points, the mean read from the device
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sub_mean(Geo g, T *f, const double *sum) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long nphys = (long long)g.n[0] * g.n[1] * g.n[2];
+  if (t >= nphys) return;
+  int i, j, k;
+  phys_ijk(g, t, &i, &j, &k);
+  long long p = gidx(g, i, j, k);
+  f[p] = (T)((double)f[p] - *sum / (double)nphys);
 }
 
 // ---------------------------------------------------------------------------
@@ -586,6 +622,52 @@ void fk_evolve(emo_ctx *c, FieldSet<T> &fs, double dt) {
 }
 
 template <typename T>
+void fk_jacobi(emo_ctx *c, const T *src, T *dst, const T *R, double ax, double ay, double az, double af,
+               double sourceMult) {
+  JacobiArgs<T> a;
+  a.g = c->geo;
+  a.src = src;
+  a.dst = dst;
+  a.R = R;
+  a.ax = (T)ax;
+  a.ay = (T)ay;
+  a.az = (T)az;
+  a.af = (T)af;
+  a.sourceMult = (T)sourceMult;
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  TimedLaunch t(c, sizeof(T) == 8 ? "jacobi_f64" : "jacobi_f32");
+  hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+}
+
+template <typename T>
+void fk_residual(emo_ctx *c, const T *u1, const T *u2, double *partials, int nblocks) {
+  hipLaunchKernelGGL(k_residual<T>, dim3(nblocks), dim3(kBlock), 0, c->stream, c->geo, u1, u2, partials);
+}
+
+template <typename T>
+void fk_phi_to_e(emo_ctx *c, const T *phi, T *const *E) {
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  hipLaunchKernelGGL(k_phi_to_e<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, c->geo, phi, E[0], E[1],
+                     E[2]);
+}
+
+template <typename T>
+void fk_a_to_b(emo_ctx *c, const T *Ax, const T *Ay, const T *Az, T *const *B) {
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  hipLaunchKernelGGL(k_a_to_b<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, c->geo, Ax, Ay, Az, B[0],
+                     B[1], B[2]);
+}
+
+template <typename T>
+void fk_subtract_mean(emo_ctx *c, T *f, double *partials, int nblocks, double *sum_out) {
+  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
+  hipLaunchKernelGGL(k_sum_phys<T>, dim3(nblocks), dim3(kBlock), 0, c->stream, c->geo, f, partials);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, partials, nblocks, 1, sum_out);
+  hipLaunchKernelGGL(k_sub_mean<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, c->geo, f,
+                     (const double *)sum_out);
+}
+
+template <typename T>
 void fk_diag_fields(emo_ctx *c, const FieldSet<T> &fs, double *partials, int nblocks) {
   DiagFArgs<T> a;
   a.g = c->geo;
@@ -606,7 +688,12 @@ void fk_diag_fields(emo_ctx *c, const FieldSet<T> &fs, double *partials, int nbl
   template void fk_const_fill<T>(emo_ctx *, T *, double, long long);                             \
   template void fk_fold<T>(emo_ctx *, FieldSet<T> &);                                            \
   template void fk_evolve<T>(emo_ctx *, FieldSet<T> &, double);                                  \
-  template void fk_diag_fields<T>(emo_ctx *, const FieldSet<T> &, double *, int);
+  template void fk_diag_fields<T>(emo_ctx *, const FieldSet<T> &, double *, int);                \
+  template void fk_jacobi<T>(emo_ctx *, const T *, T *, const T *, double, double, double, double, double); \
+  template void fk_residual<T>(emo_ctx *, const T *, const T *, double *, int);                  \
+  template void fk_phi_to_e<T>(emo_ctx *, const T *, T *const *);                                \
+  template void fk_a_to_b<T>(emo_ctx *, const T *, const T *, const T *, T *const *);            \
+  template void fk_subtract_mean<T>(emo_ctx *, T *, double *, int, double *);
 EMO_INSTANTIATE(double)
 EMO_INSTANTIATE(float)
 #undef EMO_INSTANTIATE
@@ -651,19 +738,7 @@ void launch_const_fill_f64(emo_ctx *c, double *f, double v, long long n) {
 
 void launch_jacobi_f64(emo_ctx *c, const double *src, double *dst, const double *R, double ax,
                        double ay, double az, double af, double sourceMult) {
-  JacobiArgs a;
-  a.g = c->geo;
-  a.src = src;
-  a.dst = dst;
-  a.R = R;
-  a.ax = ax;
-  a.ay = ay;
-  a.az = az;
-  a.af = af;
-  a.sourceMult = sourceMult;
-  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
-  TimedLaunch t(c, "jacobi_f64");
-  hipLaunchKernelGGL(k_jacobi, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+  fk_jacobi<double>(c, src, dst, R, ax, ay, az, af, sourceMult);
 }
 
 int residual_blocks(emo_ctx *c) {
@@ -674,20 +749,13 @@ int residual_blocks(emo_ctx *c) {
 
 void launch_residual_f64(emo_ctx *c, const double *u1, const double *u2, double *partials,
                          int nblocks) {
-  hipLaunchKernelGGL(k_residual, dim3(nblocks), dim3(kBlock), 0, c->stream, c->geo, u1, u2,
-                     partials);
+  fk_residual<double>(c, u1, u2, partials, nblocks);
 }
 
-void launch_phi_to_e_f64(emo_ctx *c, const double *phi) {
-  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
-  hipLaunchKernelGGL(k_phi_to_e, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, c->geo, phi,
-                     c->E[0], c->E[1], c->E[2]);
-}
+void launch_phi_to_e_f64(emo_ctx *c, const double *phi) { fk_phi_to_e<double>(c, phi, c->E); }
 
 void launch_a_to_b_f64(emo_ctx *c, const double *Ax, const double *Ay, const double *Az) {
-  long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
-  hipLaunchKernelGGL(k_a_to_b, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, c->geo, Ax, Ay,
-                     Az, c->B[0], c->B[1], c->B[2]);
+  fk_a_to_b<double>(c, Ax, Ay, Az, c->B);
 }
 
 void launch_diag_particles_f64(emo_ctx *c, double *partials, int nblocks) {

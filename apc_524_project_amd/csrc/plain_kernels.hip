@@ -13,6 +13,7 @@
 #include "deposit_grid.hpp"
 #include "emo_internal.hpp"
 #include "pic_math.hpp"
+#include "uniform_load.hpp"
 
 namespace emo {
 namespace {
@@ -569,6 +570,38 @@ void launch_deposit_f64(emo_ctx *c, int rho) {
   b.do_rho = rho;
   TimedLaunch t(c, "deposit_grid_f64");
   hipLaunchKernelGGL(k_deposit_grid<double>, dim3(grid_for(c->geo.ntot)), dim3(kBlock), 0, c->stream, b);
+}
+
+// emo_load_uniform's particles in the FP64 store (generation order)
+struct GenF64Args {
+  UniformGen u;
+  ParticlesF64 p;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_uniform_f64(GenF64Args a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  double x[3], v[3], gam;
+  int sp;
+  ugen_particle(a.u, i, x, v, &gam, &sp);
+  for (int d = 0; d < 3; d++) {
+    a.p.x[d][i] = x[d];
+    a.p.v[d][i] = v[d];
+  }
+  a.p.gam[i] = gam;
+  a.p.sp[i] = sp;
+  a.p.id[i] = i;
+  for (int q = 0; q < 6; q++) a.p.fld[q][i] = 0.0;
+}
+
+void launch_uniform_f64(emo_ctx *c, const UniformGen &u, int64_t n) {
+  if (n <= 0) return;
+  GenF64Args a;
+  a.u = u;
+  a.p = c->p;
+  a.np = n;
+  hipLaunchKernelGGL(k_uniform_f64, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);
 }
 
 void launch_cell_ids(emo_ctx *c, int32_t *out) {

@@ -32,6 +32,7 @@ class Workload:
     steps: int
     poisson: bool
     poisson_sweeps: int  # fixed sweep count (0: the reference's convergence rule)
+    device_loader: bool = False  # emo_load_uniform instead of the reference's ran2 stream
 
     @property
     def nparticles(self) -> int:
@@ -77,8 +78,10 @@ def _process_info(ld: EmoLoader, c: EmoConfig) -> None:
 
 
 def make_workload(name: str, precision: int = FP64_PLAIN, scale: float = 1.0) -> Workload:
-    """c1 .. c4 of the benchmark set.  scale < 1 shrinks the particle count
-    (same grid) for quick parity runs; benchmarks use scale = 1."""
+    """c1 .. c5 of the benchmark set, and c5s, c5's physics on a 128 x 32 x 32
+    box (1/64 of c5's volume, the FP64 comparison of c5).  scale < 1 shrinks
+    the particle count (same grid) for quick parity runs (c1 .. c4);
+    benchmarks use scale = 1."""
     dt = 0.5 / math.sqrt(3.0)
     if name == "c1":
         # 16^3 periodic thermal plasma, 8 ppc/species, Boris, Poisson init (50 sweeps)
@@ -147,6 +150,28 @@ def make_workload(name: str, precision: int = FP64_PLAIN, scale: float = 1.0) ->
         ld.zero_avg = 0
         ld.poisson_init = 0
         steps, poisson, sweeps = (2000 if name == "c3" else 400), False, 0
+    elif name in ("c5", "c5s"):
+        # two-stream instability: two electron beams at +-0.2c along x
+        # (sigma 0.01c), 64 ppc per beam, on a fixed uniform ion background
+        # (the Poisson source minus its mean); periodic; relativistic Boris;
+        # w_pe = 0.1.  Particles from the device loader (uniform in each cell).
+        n = (512, 128, 128) if name == "c5" else (128, 32, 32)
+        c = _base(n, precision, 1)
+        c.neutralize = 1
+        ld = EmoLoader()
+        ppc = 64
+        ld.nparticles_tot = 2 * ppc * n[0] * n[1] * n[2]
+        _species(ld, [(1.0, -1.0, 0.5, 0.01), (1.0, -1.0, 0.5, 0.01)])
+        ld.drift[0][0] = 0.2
+        ld.drift[1][0] = -0.2
+        ld.dens_phys = 0.01 / WPE2_PER_DENS
+        ld.seed = -5
+        ld.zero_avg = 0
+        ld.poisson_init = 1
+        steps, poisson, sweeps = 1000, True, 0
+        c.max_particles = ld.nparticles_tot
+        _process_info(ld, c)
+        return Workload(name, c, ld, dt, steps, poisson, sweeps, device_loader=True)
     else:
         raise ValueError(f"unknown workload {name}")
     c.max_particles = ld.nparticles_tot

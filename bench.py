@@ -197,18 +197,50 @@ def run_adjoint_c4(steps: int | None, k: int):
     return res
 
 
-def run_accuracy_c2(checkpoints=(10, 100, 1000), scale: float = 1.0):
-    """Workload c2 in both forms from the same loaded particles: relative L2
-    error of E, B, J (physical points) of the FP32 chosen form against the FP64
-    plain form (bit-exact with the oracle) after each checkpoint step count,
-    of the x-directed Langmuir mode (k = 2 pi / 64 component of the y-z mean
-    of E_x) that the density perturbation drives, and of the particles'
-    positions and velocities at the last checkpoint.
+ACCURACY_CHECKPOINTS = {
+    "c1": (10, 100),
+    "c2": (10, 100, 1000),
+    "c3": (10, 100, 1000, 2000),
+    "c4": (10, 100, 400),
+    "c5s": (10, 100, 1000),
+}
+C5_POISSON_ITER_MAX = 499  # at most 500 Jacobi sweeps per solve (reference convergence rule)
 
-    The same errors are given for an FP64 plain run started from the FP32
-    form's initial particles (positions and velocities as the FP32 store
-    holds them): it isolates how fast the loop itself amplifies a perturbation
-    of FP32 representation size, independent of FP32 arithmetic."""
+
+def init_fields(ctx, w):
+    """The workload's field initialisation: Poisson (c1: 50 sweeps; c5: the
+    reference's convergence rule, at most 500 sweeps) or constant."""
+    if not w.poisson:
+        ctx.init_constant()
+        return None
+    if w.poisson_sweeps:
+        sw, res = ctx.init_poisson(iter_max=w.poisson_sweeps - 1, tol=0.0)
+    else:
+        sw, res = ctx.init_poisson(iter_max=C5_POISSON_ITER_MAX, tol=-1.0)
+    return {"sweeps": [int(x) for x in sw], "resid_max": [float(x) for x in res[::2]],
+            "resid_l2": [float(x) for x in res[1::2]]}
+
+
+def load(ctx, w):
+    if w.device_loader:
+        ctx.load_uniform(w.loader)
+    else:
+        ctx.load_maxwellian(w.loader)
+
+
+def run_accuracy(name, checkpoints, control=False, scale: float = 1.0):
+    """Workload `name` in both forms from the same loaded particles: relative
+    L2 error of E, B, J (physical points) of the FP32 chosen form against the
+    FP64 plain form (bit-exact with the oracle) after each checkpoint step
+    count, of the field energies, of the k = 2 pi / L_x component of the y-z
+    mean of E_x, and of the particles' positions and velocities at the last
+    checkpoint.
+
+    With control, the same errors are given for an FP64 plain run started
+    from the FP32 form's initial particles (positions and velocities as the
+    FP32 store holds them): it isolates how fast the loop itself amplifies a
+    perturbation of FP32 representation size, independent of FP32
+    arithmetic."""
     import numpy as np
 
     import apc_524_project_amd as emo
@@ -216,19 +248,21 @@ def run_accuracy_c2(checkpoints=(10, 100, 1000), scale: float = 1.0):
     from apc_524_project_amd import _lib as L_
 
     ctxs = {}
-    for name, prec in (("fp64", FP64_PLAIN), ("fp32", FP32_CHOSEN)):
-        w = emo.make_workload("c2", precision=prec, scale=scale)
+    poisson = {}
+    for form, prec in (("fp64", FP64_PLAIN), ("fp32", FP32_CHOSEN)):
+        w = emo.make_workload(name, precision=prec, scale=scale)
         c = emo.Context(w.cfg)
-        c.load_maxwellian(w.loader)
-        c.init_constant()
-        ctxs[name] = c
-    p32 = ctxs["fp32"].particles(fields=False)
-    w64 = emo.make_workload("c2", precision=FP64_PLAIN, scale=scale)
-    c = emo.Context(w64.cfg)
-    c.load_particles(p32["x"], p32["v"], p32["gamma"], p32["species"], p32["id"])
-    c.init_constant()
-    ctxs["fp64_from_fp32_state"] = c
-    del p32
+        load(c, w)
+        poisson[form] = init_fields(c, w)
+        ctxs[form] = c
+    if control:
+        p32 = ctxs["fp32"].particles(fields=False)
+        w64 = emo.make_workload(name, precision=FP64_PLAIN, scale=scale)
+        c = emo.Context(w64.cfg)
+        c.load_particles(p32["x"], p32["v"], p32["gamma"], p32["species"], p32["id"])
+        init_fields(c, w64)
+        ctxs["fp64_from_fp32_state"] = c
+        del p32
     n = w.cfg.nCell
     inner = (slice(1, n[0] + 1), slice(1, n[1] + 1), slice(1, n[2] + 1))
 
@@ -239,12 +273,12 @@ def run_accuracy_c2(checkpoints=(10, 100, 1000), scale: float = 1.0):
         nb = float(np.linalg.norm(b))
         return float(np.linalg.norm(a - b) / nb) if nb > 0 else None
 
-    def langmuir(c):
+    def mode(c):
         ex = c.field(L_.EX)[inner].mean(axis=(1, 2))
         return np.fft.fft(ex)[1]
 
     groups = {"E": (L_.EX, L_.EY, L_.EZ), "B": (L_.BX, L_.BY, L_.BZ), "J": (L_.JX, L_.JY, L_.JZ)}
-    rows = {"fp32": [], "fp64_from_fp32_state": []}
+    rows = {k: [] for k in ctxs if k != "fp64"}
     done = 0
     for cp in checkpoints:
         for c in ctxs.values():
@@ -253,38 +287,79 @@ def run_accuracy_c2(checkpoints=(10, 100, 1000), scale: float = 1.0):
         done = cp
         ref = {g: group(ctxs["fp64"], fids) for g, fids in groups.items()}
         d64 = ctxs["fp64"].diag()
-        m64 = langmuir(ctxs["fp64"])
-        for name in rows:
+        m64 = mode(ctxs["fp64"])
+        for form in rows:
             row = {"steps": cp}
             for g, fids in groups.items():
-                row[g] = rel(group(ctxs[name], fids), ref[g])
-            d = ctxs[name].diag()
+                row[g] = rel(group(ctxs[form], fids), ref[g])
+            d = ctxs[form].diag()
             row["field_energy"] = rel(np.array(d[4:6]), np.array(d64[4:6]))
-            row["langmuir_mode"] = float(abs(langmuir(ctxs[name]) - m64) / abs(m64))
-            rows[name].append(row)
-            print(f"[bench] accuracy c2 {name} vs fp64 after {cp} steps: " +
+            row["ex_mode_k1"] = float(abs(mode(ctxs[form]) - m64) / abs(m64)) if abs(m64) > 0 else None
+            rows[form].append(row)
+            print(f"[bench] accuracy {name} {form} vs fp64 after {cp} steps: " +
                   ", ".join(f"{k} {v:.3g}" for k, v in row.items() if k != "steps" and v is not None),
                   file=sys.stderr, flush=True)
     parts = {}
-    for name, c in ctxs.items():
+    for form, c in ctxs.items():
         p = c.particles(fields=False)
         o = np.argsort(p["id"])
-        parts[name] = (p["x"][o], p["v"][o])
+        parts[form] = (p["x"][o], p["v"][o])
         c.close()
     L = np.array(w.cfg.Lxyz)
-    res = {
-        "workload": f"c2 ({w.nparticles} particles, 64^3): FP32 chosen form vs FP64 plain form from the same "
-                    f"loaded particles; and an FP64 plain run from the FP32 store's initial particles",
-    }
-    for name in rows:
-        dx = parts[name][0] - parts["fp64"][0]
+    res = {"workload": f"{name} ({w.nparticles} particles, {n[0]}x{n[1]}x{n[2]} cells): FP32 chosen form vs FP64 "
+                       f"plain form from the same loaded particles",
+           "dt": w.dt}
+    if poisson["fp64"]:
+        res["poisson_init"] = poisson
+    for form in rows:
+        dx = parts[form][0] - parts["fp64"][0]
         dx -= L * np.round(dx / L)  # periodic images
-        res[name] = {
-            "rel_l2": rows[name],
+        res[form] = {
+            "rel_l2": rows[form],
             "positions_rel_l2": float(np.linalg.norm(dx) / np.linalg.norm(parts["fp64"][0])),
-            "velocities_rel_l2": rel(parts[name][1], parts["fp64"][1]),
+            "velocities_rel_l2": rel(parts[form][1], parts["fp64"][1]),
         }
     return res
+
+
+def run_c5(steps: int, warmup: int):
+    """Workload c5 at full size in the FP32 chosen form: 1.07e9 particles from
+    the device loader, FP32 Poisson initialisation with FP64 residual norms,
+    then `steps` timed steps; FP64 diagnostics before and after."""
+    import apc_524_project_amd as emo
+    from apc_524_project_amd import FP32_CHOSEN
+
+    w = emo.make_workload("c5", precision=FP32_CHOSEN)
+    t0 = time.perf_counter()
+    ctx = emo.Context(w.cfg)
+    ctx.load_uniform(w.loader)
+    ctx.synchronize()
+    t_load = time.perf_counter() - t0
+    a = time.perf_counter()
+    pois = init_fields(ctx, w)
+    ctx.synchronize()
+    t_pois = time.perf_counter() - a
+    d0 = ctx.diag()
+    t = 0.0
+    ctx.step(warmup, t, w.dt)
+    t += warmup * w.dt
+    ctx.synchronize()
+    a = time.perf_counter()
+    ctx.step(steps, t, w.dt)
+    ctx.synchronize()
+    el = time.perf_counter() - a
+    d1 = ctx.diag()
+    ctx.close()
+    rate = w.nparticles * steps / el
+    print(f"[bench] c5 fp32: {w.nparticles} particles, {steps} steps in {el:.3f}s -> {rate:.4g} particle-steps/s; "
+          f"load {t_load:.1f}s, poisson {t_pois:.2f}s ({pois['sweeps']} sweeps)", file=sys.stderr, flush=True)
+    keys = ("pMx", "pMy", "pMz", "kinetic", "B_energy", "E_energy", "total_energy", "charge")
+    return {"workload": "c5: 512x128x128 periodic, two electron beams +-0.2c (sigma 0.01c), 64 ppc per beam "
+                        f"({w.nparticles} particles), fixed neutralising ion background, relativistic Boris, "
+                        "device loader (seed -5)",
+            "particle_steps_per_s": round(rate, 1), "ms_per_step": round(1e3 * el / steps, 4), "steps": steps,
+            "warmup": warmup, "load_s": round(t_load, 2), "poisson_s": round(t_pois, 3), "poisson_init": pois,
+            "diag_initial": dict(zip(keys, map(float, d0))), "diag_final": dict(zip(keys, map(float, d1)))}
 
 
 def cpu_baseline(sample_cells: int = 64, sample_steps: int = 10):
@@ -349,8 +424,9 @@ def main():
                     help="first run an elementwise copy of 256 MiB (a counter reference of known bytes)")
     ap.add_argument("--adjoint-steps", type=int, default=0, help="c4 steps for the adjoint leg (0: 400)")
     ap.add_argument("--checkpoint-every", type=int, default=20)
-    ap.add_argument("--accuracy-steps", type=int, default=1000,
-                    help="c2 steps of the FP32-vs-FP64 field error leg (0: skip)")
+    ap.add_argument("--accuracy", default="all",
+                    help="configs of the FP32-vs-FP64 error leg: all, none, or a comma list of c1,c2,c3,c4,c5s")
+    ap.add_argument("--c5-steps", type=int, default=20, help="timed c5 steps (0: skip the c5 leg)")
     ap.add_argument("--cpu-cells", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=10)
     args = ap.parse_args()
@@ -419,9 +495,12 @@ def main():
         out[f"{m}_yee_cell_updates_per_s"] = round(rr["workload"].ncells * args.steps / rr["elapsed"], 1)
     if "fp32" in results and "fp64" in results:
         out["fp32_over_fp64"] = round(results["fp32"]["rate"] / results["fp64"]["rate"], 3)
-    if args.accuracy_steps > 0 and "fp32" in results:
-        cps = sorted({c for c in (10, 100, args.accuracy_steps) if c <= args.accuracy_steps})
-        out["fp32_accuracy"] = run_accuracy_c2(tuple(cps), args.scale)
+    if args.accuracy != "none" and "fp32" in results:
+        names = list(ACCURACY_CHECKPOINTS) if args.accuracy == "all" else args.accuracy.split(",")
+        out["fp32_accuracy"] = {nm: run_accuracy(nm, ACCURACY_CHECKPOINTS[nm], control=(nm == "c2"), scale=args.scale)
+                                for nm in names}
+    if args.c5_steps > 0 and "fp32" in results:
+        out["c5"] = run_c5(args.c5_steps, 2)
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint_c4(args.adjoint_steps or None, args.checkpoint_every)
     if not args.no_cpu_baseline:

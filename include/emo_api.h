@@ -76,6 +76,11 @@ typedef struct emo_config {
   double waist;                /* 0: plane-uniform pulse (reference); >0: transverse Gaussian waist */
   double bound_phi[6], bound_Ax[6], bound_Ay[6], bound_Az[6];
   int64_t max_particles;       /* capacity of the particle store */
+  int32_t neutralize;          /* extension (0 = reference): the Poisson initialisation subtracts the
+                                  mean charge density, i.e. adds a fixed uniform neutralising
+                                  background, so that a net-charge plasma between periodic
+                                  boundaries (where the reference asserts, particle_handler.cpp:
+                                  435-438) has a solution */
 } emo_config;
 
 typedef struct emo_ctx emo_ctx;
@@ -111,6 +116,18 @@ typedef struct emo_loader {
 } emo_loader;
 int emo_process_info(const emo_loader *ld, emo_config *cfg_inout);
 int emo_load_maxwellian(emo_ctx *ctx, const emo_loader *ld);
+/* No counterpart in the reference (its loader is the serial ran2 stream of
+ * emo_load_maxwellian); for loads too large for a host stream (10^9
+ * particles).  Generates the particles on the device, per physical cell and
+ * species nparticles_tot * dens_frac / ncells of them (which must be whole),
+ * uniformly distributed in the cell, velocities Maxwellian (UNIT_VTH
+ * sqrt(T/m) per component, T and m of Input::ProcessInfo) plus drift, stored
+ * gamma = sqrt(1 - v^2) as the reference loader stores it.  The random
+ * numbers come from a counter-based generator (SplitMix64 of seed, particle
+ * and draw index; Box-Muller pairs), so both precisions load the same
+ * particles.  Ids are the generation order (cell by cell).  Writes q, m
+ * into the context as emo_load_maxwellian does. */
+int emo_load_uniform(emo_ctx *ctx, const emo_loader *ld);
 int64_t emo_num_particles(const emo_ctx *ctx);
 /* copy out in the current array order; any pointer may be NULL */
 int emo_get_particles(emo_ctx *ctx, double *x3, double *v3, double *gamma, int32_t *species,

@@ -2,6 +2,7 @@
 declares (no compute calls: these run without a GPU)."""
 import ctypes
 import os
+import subprocess
 import re
 
 import pytest
@@ -34,7 +35,18 @@ def test_version_string_without_gpu():
     assert b"gfx950" in _lib.lib().emo_version()
 
 
-def test_struct_layouts_match_header():
-    # offsets of the last members pin the ctypes mirrors to the C layout
-    assert _lib.EmoConfig.max_particles.offset == ctypes.sizeof(_lib.EmoConfig) - 8
-    assert ctypes.sizeof(_lib.EmoLoader) % 8 == 0
+def test_struct_layouts_match_header(tmp_path):
+    # the C compiler's sizes and offsets of the header's structs against the
+    # ctypes mirrors
+    exe = str(tmp_path / "layout")
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "layout_check.c")
+    subprocess.run(["gcc", "-std=c99", src, "-o", exe], check=True)
+    out = subprocess.run([exe], check=True, stdout=subprocess.PIPE, text=True).stdout.split()
+    vals = {out[i]: int(out[i + 1]) for i in range(0, len(out), 2)}
+    assert vals["emo_config"] == ctypes.sizeof(_lib.EmoConfig)
+    assert vals["max_particles"] == _lib.EmoConfig.max_particles.offset
+    assert vals["neutralize"] == _lib.EmoConfig.neutralize.offset
+    assert vals["waist"] == _lib.EmoConfig.waist.offset
+    assert vals["emo_loader"] == ctypes.sizeof(_lib.EmoLoader)
+    assert vals["drift"] == _lib.EmoLoader.drift.offset
+    assert vals["seed"] == _lib.EmoLoader.seed.offset
