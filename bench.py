@@ -107,11 +107,70 @@ def run_gpu(mode: str, steps: int, warmup: int, scale: float):
     print(f"[bench] {mode}: {steps} steps in {el:.3f}s -> {rate:.4g} particle-steps/s", file=sys.stderr,
           flush=True)
     ctx.close()
-    return dict(rate=rate, elapsed=el, ms_per_step=1e3 * el / steps, ktimes=ktimes, workload=w)
+    return dict(rate=rate, elapsed=el, ms_per_step=1e3 * el / steps, ktimes=ktimes, workload=w, mode=mode,
+                scale=scale)
+
+
+def run_fp64_per_phase(steps: int, warmup: int, scale: float):
+    """The FP64 plain form driven phase by phase through the C ABI in the
+    reference loop's order (src/main.cpp:225-275: SortParticles, Push,
+    particle BCs + clearGhosts, depositRhoJ, executeBC(-2), evolveFields,
+    executeBC(-1), InterpolateEB): the reference's pass structure, one
+    library call per phase."""
+    import apc_524_project_amd as emo
+    from apc_524_project_amd import FP64_PLAIN
+
+    w = emo.make_workload("c2", precision=FP64_PLAIN, scale=scale)
+    ctx = emo.Context(w.cfg)
+    ctx.load_maxwellian(w.loader)
+    ctx.init_constant()
+    ctx.interpolate()
+    t = 0.0
+
+    def one(t):
+        ctx.sort()
+        ctx.push(w.dt)
+        ctx.particle_bc()
+        ctx.deposit(True)
+        ctx.fold()
+        ctx.evolve(w.dt)
+        ctx.refresh(t, w.dt)
+        ctx.interpolate()
+
+    for _ in range(warmup):
+        one(t)
+        t += w.dt
+    ctx.synchronize()
+    a = time.perf_counter()
+    for _ in range(steps):
+        one(t)
+        t += w.dt
+    ctx.synchronize()
+    el = time.perf_counter() - a
+    ctx.close()
+    rate = w.nparticles * steps / el
+    print(f"[bench] fp64 per-phase: {steps} steps in {el:.3f}s -> {rate:.4g} particle-steps/s", file=sys.stderr,
+          flush=True)
+    return rate, 1e3 * el / steps
 
 
 # library calls (several rocprim kernels each): reported, never the roofline kernel
 LIBRARY_CALLS = {"sort_radix", "scan_f32", "scan_f64"}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_fp32_step_hbm_traffic.json")
+
+
+def measured_traffic(kernel: str, mode: str, scale: float):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
+    and WRITE_SIZE runs of the full-size c2 FP32 step (each counter scaled by
+    what it reads for a copy kernel of known size); None where not measured."""
+    if mode != "fp32" or scale != 1.0 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        d = json.load(f)
+    e = d.get("per_launch", {}).get(kernel)
+    return int(e["hbm_bytes"]) if e else None
 
 
 def roofline_for(res):
@@ -128,17 +187,21 @@ def roofline_for(res):
         return {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None, "avg_ms": ms / n}
     ach = b / avg_s / 1e9
+    traffic = measured_traffic(dom, res["mode"], res["scale"])
     return {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_ms": round(ms / n, 5),
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "avg_ms": round(ms / n, 5),
             "algorithmic_bytes": int(b),
+            "traffic_over_algorithmic": round(traffic / b, 3) if traffic else None,
             "kernel_ms_per_step": {k: round(v[0] / v[1], 5) for k, v in sorted(kt.items())}}
 
 
 def run_adjoint_c4(steps: int | None, k: int):
     """Workload c4 (64x32x32, x: LightBC pulse + reflecting walls, 8 ppc/species,
     FP64): wall time of the forward steps alone vs. the gradient call
-    (forward + checkpointed adjoint), and a central-difference check of
-    dF/d(pulse amplitude)."""
+    (the same forward with checkpoints, then the adjoint: the backward sweep
+    with every segment re-run from its checkpoint), and a central-difference
+    check of dF/d(pulse amplitude).  adjoint_s = gradient_s - forward_s is the
+    time the derivative adds to a forward run."""
     import apc_524_project_amd as emo
     from apc_524_project_amd import FP64_PLAIN
 
@@ -186,14 +249,17 @@ def run_adjoint_c4(steps: int | None, k: int):
                     f"({w.nparticles} particles), relativistic, FP64, {n} steps, checkpoint every {k}",
         "forward_s": round(t_fwd, 4),
         "gradient_s": round(t_adj, 4),
-        "adjoint_over_forward": round(t_adj / t_fwd, 3),
+        "adjoint_s": round(t_adj - t_fwd, 4),
+        "adjoint_over_forward": round((t_adj - t_fwd) / t_fwd, 3),
+        "gradient_over_forward": round(t_adj / t_fwd, 3),
         "energy_matches_forward": bool(out["energy"] == f0),
         "dF_damp": float(out["d_amp"][0]),
         "dF_damp_central_diff": float(fd),
         "fd_rel_err": float(abs(out["d_amp"][0] - fd) / abs(fd)) if fd else None,
     }
-    print(f"[bench] adjoint c4: forward {t_fwd:.3f}s, gradient {t_adj:.3f}s, "
-          f"ratio {t_adj / t_fwd:.2f}, fd rel err {res['fd_rel_err']}", file=sys.stderr, flush=True)
+    print(f"[bench] adjoint c4: forward {t_fwd:.3f}s, gradient {t_adj:.3f}s, adjoint/forward "
+          f"{(t_adj - t_fwd) / t_fwd:.2f}, gradient/forward {t_adj / t_fwd:.2f}, fd rel err {res['fd_rel_err']}",
+          file=sys.stderr, flush=True)
     return res
 
 
@@ -495,6 +561,10 @@ def main():
         out[f"{m}_yee_cell_updates_per_s"] = round(rr["workload"].ncells * args.steps / rr["elapsed"], 1)
     if "fp32" in results and "fp64" in results:
         out["fp32_over_fp64"] = round(results["fp32"]["rate"] / results["fp64"]["rate"], 3)
+        rate_pp, ms_pp = run_fp64_per_phase(max(args.steps // 5, 5), 2, args.scale)
+        out["fp64_per_phase_particle_steps_per_s"] = round(rate_pp, 1)
+        out["fp64_per_phase_ms_per_step"] = round(ms_pp, 4)
+        out["fp32_over_fp64_per_phase"] = round(results["fp32"]["rate"] / rate_pp, 3)
     if args.accuracy != "none" and "fp32" in results:
         names = list(ACCURACY_CHECKPOINTS) if args.accuracy == "all" else args.accuracy.split(",")
         out["fp32_accuracy"] = {nm: run_accuracy(nm, ACCURACY_CHECKPOINTS[nm], control=(nm == "c2"), scale=args.scale)
