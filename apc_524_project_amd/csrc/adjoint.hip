@@ -1030,17 +1030,21 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   // ---- forward with checkpoints -------------------------------------------
   double t = t0;
   std::vector<double> tstep(nsteps + 1);
+  // the last segment is taped on the way (it needs no recomputation)
+  const int last0 = nsteps > 0 ? (R.nseg - 1) * K : 0;
+  const int last_set = nsteps > 0 ? (R.nseg - 1) % nsets : 0;
   for (int s = 0; s < nsteps; s++) {
     if (s % K == 0) {
       rc = snap_take(c, R.ck[s / K], t);
       if (rc) break;
     }
     tstep[s] = t;
-    rc = forward_step(c, t, dt, nullptr);
+    rc = forward_step(c, t, dt, s >= last0 ? &R.tape[last_set][s - last0] : nullptr);
     if (rc) break;
     t += dt;
   }
   tstep[nsteps] = t;
+  if (!rc && nsteps > 0) rc = snap_take(c, R.tape[last_set][nsteps - last0], t);
   if (!rc && nsteps == 0) rc = snap_take(c, R.ck[0], t);
   // objective
   double F = 0.0;
@@ -1100,8 +1104,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   // seg - 1 is recomputed (context stream) into the other tape set ----------
   if (!rc && nsteps > 0) {
     std::vector<hipEvent_t> done_r(R.nseg, nullptr), done_a(R.nseg, nullptr);
-    rc = recompute_segment(R, R.nseg - 1, (R.nseg - 1) % nsets);
-    if (!rc) done_r[R.nseg - 1] = record(c->stream);
+    done_r[R.nseg - 1] = record(c->stream);  // taped by the forward pass
     for (int seg = R.nseg - 1; seg >= 0 && !rc; seg--) {
       rc = wait(R.as, done_r[seg]);
       if (rc) break;

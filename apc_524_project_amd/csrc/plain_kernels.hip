@@ -300,8 +300,8 @@ struct Pgd64Args {
   Geo g;
   const double *x[3], *v[3], *gam;
   const int32_t *sp;
-  double *fld[6];  // written only with write_fld (the adjoint's tape)
-  uint8_t *flips;  // with write_fld: bit d = odd number of reflections in dim d
+  double *fld[6];  // written only with write_fld
+  uint8_t *flips;  // when not null (the adjoint's tape): bit d = odd number of reflections in dim d
   double *xo[3], *vo[3], *go;
   const double *E[3], *B[3], *Bt[3];
   SpeciesParams s;
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(kBlock) k_pgd64(Pgd64Args a) {
     else boris_step(pos, vel, f, a.s.q[sp], a.s.m[sp], a.dt);
     int flips[3];
     particle_bcs(pos, vel, a.xmin, a.xmax, a.pbc, flips);
-    if (a.write_fld) a.flips[i] = (uint8_t)((flips[0] & 1) | (flips[1] & 1) << 1 | (flips[2] & 1) << 2);
+    if (a.flips) a.flips[i] = (uint8_t)((flips[0] & 1) | (flips[1] & 1) << 1 | (flips[2] & 1) << 2);
     int dx, dy, dz;
     if (cell_of(g, pos[0], pos[1], pos[2], &dx, &dy, &dz) < 0) {
       atomicOr(a.flag, kFlagOutOfGrid);
@@ -630,8 +630,8 @@ void plain_sort_destroy(emo_ctx *c) { cellsort_free(c->srt); }
 // The fused particle phase of the plain form on a cell-sorted store, part 1:
 // push (gathering the fields from the grid) and BCs into palt, with the move
 // code of every particle.  With fld_out the gathered fields are stored there
-// (six arrays, the pre-push order), and the reflections of the particle BCs
-// in flips_out.
+// (six arrays, the pre-push order); with flips_out the reflections of the
+// particle BCs.
 void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out, uint8_t *flips_out) {
   const Geo &g = c->geo;
   const long long np = c->np;
@@ -658,7 +658,7 @@ void fused_push_f64(emo_ctx *c, double dt, double *const *fld_out, uint8_t *flip
   for (int s = 0; s < 6; s++) a.pbc[s] = c->cfg.parts_bound[s];
   a.code = c->srt.code;
   a.nfar = c->srt.misc;
-  a.write_fld = fld_out != nullptr && flips_out != nullptr;
+  a.write_fld = fld_out != nullptr;
   a.flips = flips_out;
   a.flag = c->dflag;
   a.np = np;
