@@ -4,8 +4,9 @@
     energy.backward()   # amps.grad, phases.grad, eps.grad
 
 `energy` is the final field energy E_en + B_en of the reference's history
-diagnostics (src/IO/output.cpp:90-121) after `spec.nsteps` steps of the FP64
-plain form, as a function of the injected pulses' peak amplitudes and phases
+diagnostics (src/IO/output.cpp:90-121) after `spec.nsteps` steps of the form
+`spec.cfg.precision` names (FP64 plain, or the FP32 chosen form with its
+FP32 particle adjoint and FP64 field adjoints), as a function of the injected pulses' peak amplitudes and phases
 (LightBC, src/grid/lightBC.cpp:165-198; phases in degrees as in the input
 file) and of the per-species density-perturbation amplitudes of the
 displacement loader.  The forward pass runs emo_adjoint_field_energy, which
@@ -39,8 +40,6 @@ class RunSpec:
 def _setup(spec: RunSpec, amps, phases, eps):
     cfg = copy.deepcopy(spec.cfg)
     ld = copy.deepcopy(spec.loader)
-    if cfg.precision != FP64_PLAIN:
-        raise ValueError("the adjoint runs on the FP64 plain form")
     for w in range(int(cfg.nwaves)):
         cfg.peakamps[w] = float(amps[w])
         cfg.phases[w] = float(phases[w])
@@ -49,7 +48,8 @@ def _setup(spec: RunSpec, amps, phases, eps):
     ctx = Context(cfg)
     ctx.load_maxwellian(ld)
     ctx.init_constant()
-    ctx.interpolate()
+    if cfg.precision == FP64_PLAIN:
+        ctx.interpolate()  # the FP32 form gathers from the grid inside its step
     return ctx, cfg, ld
 
 

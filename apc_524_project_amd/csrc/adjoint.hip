@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "adjoint_math.hpp"
+#include "chosen_math.hpp"
 #include "emo_internal.hpp"
 
 namespace emo {
@@ -65,13 +66,15 @@ __device__ inline void point_ijk(const Geo &g, long long p, int *i, int *j, int 
 // 0.5 V (0.5 (f[p] + f[p+s]))^2 per component (s along its own axis), the
 // expression of OutputBoxQuantities with the SQR macro expanded.
 // ---------------------------------------------------------------------------
+template <typename T>
 struct SeedArgs {
   Geo g;
-  const double *E[3], *B[3];
+  const T *E[3], *B[3];
   double *Eb[3], *Bb[3];
 };
 
-__global__ void __launch_bounds__(kBlock) k_seed_energy(SeedArgs a) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_seed_energy(SeedArgs<T> a) {
   const Geo &g = a.g;
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.ntot) return;
@@ -83,12 +86,12 @@ __global__ void __launch_bounds__(kBlock) k_seed_energy(SeedArgs a) {
     int di = q == 0, dj = q == 1, dk = q == 2;
     double ge = 0.0, gb = 0.0;
     if (is_phys(g, i, j, k)) {
-      ge += 0.25 * vol * (a.E[q][p] + a.E[q][p + st[q]]);
-      gb += 0.25 * vol * (a.B[q][p] + a.B[q][p + st[q]]);
+      ge += 0.25 * vol * ((double)a.E[q][p] + (double)a.E[q][p + st[q]]);
+      gb += 0.25 * vol * ((double)a.B[q][p] + (double)a.B[q][p + st[q]]);
     }
     if (is_phys(g, i - di, j - dj, k - dk)) {
-      ge += 0.25 * vol * (a.E[q][p - st[q]] + a.E[q][p]);
-      gb += 0.25 * vol * (a.B[q][p - st[q]] + a.B[q][p]);
+      ge += 0.25 * vol * ((double)a.E[q][p - st[q]] + (double)a.E[q][p]);
+      gb += 0.25 * vol * ((double)a.B[q][p - st[q]] + (double)a.B[q][p]);
     }
     a.Eb[q][p] += ge;
     a.Bb[q][p] += gb;
@@ -765,11 +768,11 @@ void launch_zero(hipStream_t st, double *p, long long n) {
 
 // gather^T, stage 2, for a gather that ran on state s: the per-cell sums in
 // af.cg go to the points (B roles of that state)
-void gather_adj_grid(emo_ctx *c, hipStream_t st, const Snap &s, AdjFields &af) {
+void gather_adj_grid_roles(emo_ctx *c, hipStream_t st, bool b_is_a0, AdjFields &af) {
   GatherGridArgs b;
   b.g = c->geo;
   b.cg = af.cg;
-  const int mb = s.b_is_a0 ? 0 : 1;
+  const int mb = b_is_a0 ? 0 : 1;
   for (int d = 0; d < 3; d++) {
     b.Eb[d] = af.Eb[d];
     b.Bb[d] = af.Ab[mb][d];
@@ -777,6 +780,95 @@ void gather_adj_grid(emo_ctx *c, hipStream_t st, const Snap &s, AdjFields &af) {
   }
   TimedLaunch t(c, "adj_gather_grid", st);
   hipLaunchKernelGGL(k_gather_adj_grid, dim3(grid_for(c->geo.ntot)), dim3(kBlock), 0, st, b);
+}
+
+void gather_adj_grid(emo_ctx *c, hipStream_t st, const Snap &s, AdjFields &af) {
+  gather_adj_grid_roles(c, st, s.b_is_a0, af);
+}
+
+// refresh^T, Yee^T and fold^T of one step on stream `as` (the field part of
+// the adjoint step, common to both precisions' adjoints: these operations
+// are linear and read no forward state); dsums_row receives the LightBC
+// plane sums of the step, pre_b_is_a0 the member-B role at the step's start.
+void fields_step_adj(emo_ctx *c, hipStream_t as, AdjFields &af, const std::vector<FillPlane> &planes,
+                     double *dsums_row, double dt, bool pre_b_is_a0) {
+  const Geo &g = c->geo;
+  const int nplanes = (int)planes.size();
+  // refresh^T, the forward's launches in reverse: a LightBC side's six
+  // fills as one launch, a run of periodic sides as one gather pass
+  const int mnew = pre_b_is_a0 ? 1 : 0;  // the array the B update writes
+  int pi = nplanes - 1;
+  for (int side = 5; side >= 0;) {
+    if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
+      SidesAdjArgs sa;
+      sa.g = g;
+      for (int q = 0; q < 3; q++) {
+        sa.f[q] = af.Eb[q];
+        sa.f[3 + q] = af.Ab[0][q];  // Q23: the refresh writes B through Bfix
+      }
+      sa.nsides = 0;
+      for (; side >= 0 && c->cfg.fields_bound[side] == EMO_BC_PERIODIC; side--) {
+        const int d = side / 2, right = side & 1;
+        sa.d[sa.nsides] = d;
+        sa.src[sa.nsides] = right ? 1 : g.n[d];
+        sa.dst[sa.nsides] = right ? g.n[d] + 1 : 0;
+        sa.nsides++;
+      }
+      hipLaunchKernelGGL(k_copy_sides_adj, dim3(grid_for((long long)g.n[0] * g.n[1] * g.n[2])), dim3(kBlock), 0,
+                         as, sa);
+    } else {
+      FillAdjMulti fm;
+      int nb = 0;
+      for (; pi >= 0 && planes[pi].side == side; pi--) {
+        const FillPlane &f = planes[pi];
+        FillAdjArgs &fa = fm.p[nb++];
+        fa.g = g;
+        fa.f = f.pass == 0 ? af.Eb[f.comp] : af.Ab[mnew][f.comp];
+        fa.dim = f.dim;
+        fa.idx = f.idx;
+        fa.waist = c->cfg.waist;
+        for (int q = 0; q < 3; q++) {
+          fa.off[q] = f.off[q];
+          fa.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
+        }
+        fa.out = dsums_row + pi;
+      }
+      if (nb) hipLaunchKernelGGL(k_plane_fill_adj_multi, dim3(nb), dim3(kBlock), 0, as, fm);
+      side--;
+    }
+  }
+  // Yee^T
+  YeeAdjArgs ya;
+  ya.g = g;
+  ya.dt = dt;
+  for (int q = 0; q < 3; q++) {
+    ya.Eb[q] = af.Eb[q];
+    ya.Bn[q] = af.Ab[mnew][q];
+    ya.Bo[q] = af.Ab[1 - mnew][q];
+    ya.Jb[q] = af.Jb[q];
+  }
+  {
+    TimedLaunch tl(c, "adj_yee", as);
+    hipLaunchKernelGGL(k_yee_b_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
+    hipLaunchKernelGGL(k_yee_move_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
+    hipLaunchKernelGGL(k_yee_e_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
+  }
+  // fold^T: each ghost plane receives its first plane's adjoint
+  {
+    FoldAdjArgs<double> fa;
+    fa.g = g;
+    int any = 0;
+    for (int d = 0; d < 3; d++) {
+      fa.J[d] = af.Jb[d];
+      fa.per[d] = c->cfg.fields_bound[2 * d] == EMO_BC_PERIODIC;
+      any |= fa.per[d];
+    }
+    if (any) {
+      const long long n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+      const long long pts = n1 * n2 + (n0 - 1) * n2 + (n0 - 1) * (n1 - 1);
+      hipLaunchKernelGGL(k_fold_adj_all, dim3(grid_for(pts)), dim3(kBlock), 0, as, fa);
+    }
+  }
 }
 
 // Everything one gradient call needs besides the context.
@@ -819,81 +911,7 @@ void backward_segment(AdjRun &R, int seg, int set) {
   for (int s = s1 - 1; s >= s0; s--) {
     const Snap &pre = R.tape[set][s - s0];      // the state the push of step s reads
     const Snap &post = R.tape[set][s + 1 - s0];  // the state after step s's sort
-    // refresh^T, the forward's launches in reverse: a LightBC side's six
-    // fills as one launch, a run of periodic sides as one gather pass
-    const int mnew = pre.b_is_a0 ? 1 : 0;  // the array the B update writes
-    int pi = nplanes - 1;
-    for (int side = 5; side >= 0;) {
-      if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
-        SidesAdjArgs sa;
-        sa.g = g;
-        for (int q = 0; q < 3; q++) {
-          sa.f[q] = af.Eb[q];
-          sa.f[3 + q] = af.Ab[0][q];  // Q23: the refresh writes B through Bfix
-        }
-        sa.nsides = 0;
-        for (; side >= 0 && c->cfg.fields_bound[side] == EMO_BC_PERIODIC; side--) {
-          const int d = side / 2, right = side & 1;
-          sa.d[sa.nsides] = d;
-          sa.src[sa.nsides] = right ? 1 : g.n[d];
-          sa.dst[sa.nsides] = right ? g.n[d] + 1 : 0;
-          sa.nsides++;
-        }
-        hipLaunchKernelGGL(k_copy_sides_adj, dim3(grid_for((long long)g.n[0] * g.n[1] * g.n[2])), dim3(kBlock), 0,
-                           as, sa);
-      } else {
-        FillAdjMulti fm;
-        int nb = 0;
-        for (; pi >= 0 && R.planes[pi].side == side; pi--) {
-          const FillPlane &f = R.planes[pi];
-          FillAdjArgs &fa = fm.p[nb++];
-          fa.g = g;
-          fa.f = f.pass == 0 ? af.Eb[f.comp] : af.Ab[mnew][f.comp];
-          fa.dim = f.dim;
-          fa.idx = f.idx;
-          fa.waist = c->cfg.waist;
-          for (int q = 0; q < 3; q++) {
-            fa.off[q] = f.off[q];
-            fa.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
-          }
-          fa.out = R.dsums + (size_t)s * nplanes + pi;
-        }
-        if (nb) hipLaunchKernelGGL(k_plane_fill_adj_multi, dim3(nb), dim3(kBlock), 0, as, fm);
-        side--;
-      }
-    }
-    // Yee^T
-    YeeAdjArgs ya;
-    ya.g = g;
-    ya.dt = R.dt;
-    for (int q = 0; q < 3; q++) {
-      ya.Eb[q] = af.Eb[q];
-      ya.Bn[q] = af.Ab[mnew][q];
-      ya.Bo[q] = af.Ab[1 - mnew][q];
-      ya.Jb[q] = af.Jb[q];
-    }
-    {
-      TimedLaunch tl(c, "adj_yee", as);
-      hipLaunchKernelGGL(k_yee_b_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
-      hipLaunchKernelGGL(k_yee_move_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
-      hipLaunchKernelGGL(k_yee_e_adj, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, ya);
-    }
-    // fold^T: each ghost plane receives its first plane's adjoint
-    {
-      FoldAdjArgs<double> fa;
-      fa.g = g;
-      int any = 0;
-      for (int d = 0; d < 3; d++) {
-        fa.J[d] = af.Jb[d];
-        fa.per[d] = c->cfg.fields_bound[2 * d] == EMO_BC_PERIODIC;
-        any |= fa.per[d];
-      }
-      if (any) {
-        const long long n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-        const long long pts = n1 * n2 + (n0 - 1) * n2 + (n0 - 1) * (n1 - 1);
-        hipLaunchKernelGGL(k_fold_adj_all, dim3(grid_for(pts)), dim3(kBlock), 0, as, fa);
-      }
-    }
+    fields_step_adj(c, as, af, R.planes, R.dsums + (size_t)s * nplanes, R.dt, pre.b_is_a0);
     // deposit^T + sort^T + BCs^T + push^T, then gather^T of the gather that
     // fed this push
     CellStepAdjArgs ca;
@@ -939,7 +957,549 @@ void backward_segment(AdjRun &R, int seg, int set) {
   }
 }
 
+
+// ===========================================================================
+// The adjoint of the FP32 chosen form (chosen.hip).  The field part of an
+// adjoint step is the FP64 one above (fields_step_adj): those operations
+// are linear and read no forward state, so the field adjoints are carried
+// in FP64.  The particle part is the transpose of the chosen form's push,
+// BCs, sort and deposit in FP32 arithmetic (push_deposit_vjp,
+// chosen_math.hpp), per cell of the pre-push order, with the particle's
+// forward recomputed from the taped pre-push state.
+// ===========================================================================
+
+// Particle adjoints of the FP32 form: the offsets' and u's adjoints, SoA.
+struct PartAdj32 {
+  float *f[3], *u[3];
+};
+
+struct CellStepAdj32Args {
+  PushK k;                      // geometry, species q_p, dt / dx, BCs
+  const int4 *pa, *pb;          // pre-push state of step s, cell-sorted
+  const float *g0;              // the stored gamma of the first push, or null
+  const int32_t *off;           // cell c of this order holds off[c]..off[c+1]
+  const uint32_t *fwd;          // pre-push index -> post-sort slot
+  const float *E[3], *B[3], *Bt[3];  // the fields the push gathered from
+  const double *Jb[3];          // current adjoint (after fold^T)
+  PartAdj32 in, out;            // adjoints: post-sort order in, pre-push order out
+  float *g0b;                   // adjoint of the stored gamma (first step), or null
+  double *cg;                   // 18 per cell, component-major
+  long long ncell;
+};
+
+template <bool REL, bool PER>
+__global__ void __launch_bounds__(kBlock) k_cell_step_adj32(CellStepAdj32Args a) {
+  __shared__ float qp_s[EMO_NSPEC], jc_s[EMO_NSPEC];
+  if (threadIdx.x < EMO_NSPEC) {
+    qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
+    jc_s[threadIdx.x] = a.k.s.test[threadIdx.x] ? 0.f : a.k.s.jc[threadIdx.x];
+  }
+  __syncthreads();
+  const int l = threadIdx.x & 15;
+  const long long c = (long long)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4);
+  double acc[18];
+#pragma unroll
+  for (int q = 0; q < 18; q++) acc[q] = 0.0;
+  if (c < a.ncell) {
+    const int b = a.off[c], e = a.off[c + 1];
+    if (b < e) {
+      // the cell's gather vector (the cell is the same for all its particles)
+      const Geo &g = a.k.g;
+      int ix, iy, iz;
+      decode_fast(a.k, (int)c, ix, iy, iz);
+      const long long o = gidx(g, ix, iy, iz), X = g.sx, Y = g.sy;
+      CellVec V;
+      V.Ex = make_float4(a.E[0][o], a.E[0][o + Y], a.E[0][o + Y + 1], a.E[0][o + 1]);
+      V.Ey = make_float4(a.E[1][o], a.E[1][o + 1], a.E[1][o + X + 1], a.E[1][o + X]);
+      V.Ez = make_float4(a.E[2][o], a.E[2][o + X], a.E[2][o + X + Y], a.E[2][o + Y]);
+      V.B = make_float4(0.5f * (a.Bt[0][o] + a.B[0][o]), 0.5f * (a.Bt[0][o + X] + a.B[0][o + X]),
+                        0.5f * (a.Bt[1][o] + a.B[1][o]), 0.5f * (a.Bt[1][o + Y] + a.B[1][o + Y]));
+      V.Bz = make_float4(0.5f * (a.Bt[2][o] + a.B[2][o]), 0.5f * (a.Bt[2][o + 1] + a.B[2][o + 1]), 0.f, 0.f);
+      for (int i = b + l; i < e; i += 16) {
+        const int4 A = a.pa[i], Bv = a.pb[i];
+        const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
+        const uint32_t j = a.fwd[i];
+        const float fin[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
+        const float uin[3] = {as_f(Bv.x), as_f(Bv.y), as_f(Bv.z)};
+        const float fbo[3] = {a.in.f[0][j], a.in.f[1][j], a.in.f[2][j]};
+        const float ubo[3] = {a.in.u[0][j], a.in.u[1][j], a.in.u[2][j]};
+        float fb[3], ub[3], g0b, cvb[18];
+        push_deposit_vjp<REL, PER>(a.k, V, A.x, fin, uin, qp_s[sp], a.g0 != nullptr, a.g0 ? a.g0[i] : 0.f,
+                                   jc_s[sp], a.Jb, fbo, ubo, fb, ub, g0b, cvb);
+        for (int d = 0; d < 3; d++) {
+          a.out.f[d][i] = fb[d];
+          a.out.u[d][i] = ub[d];
+        }
+        if (a.g0b) a.g0b[i] = g0b;
+#pragma unroll
+        for (int q = 0; q < 18; q++) acc[q] += (double)cvb[q];
+      }
+    }
+  }
+  // fixed butterfly over the cell's 16 lanes: deterministic
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc[q] += __shfl_xor(acc[q], off, 64);
+  }
+  if (l == 0 && c < a.ncell) {
+#pragma unroll
+    for (int q = 0; q < 18; q++) a.cg[q * a.ncell + c] = acc[q];
+  }
+}
+
+// One FP32 snapshot: the particle records, the stored gamma (first push),
+// the fields, the cell ranges; on the tape also the sort's forward map.
+struct Snap32 {
+  int4 *pa = nullptr, *pb = nullptr;
+  float *g0 = nullptr;
+  bool has_g0 = false;
+  float *E[3] = {}, *A[2][3] = {};  // A[0] = Bfix, A[1] = Btfix
+  float *J[3] = {};                 // the final state only
+  int32_t *off = nullptr;
+  uint32_t *fwd = nullptr;          // tape: pre-push index -> new slot
+  bool b_is_a0 = true;
+  double t = 0.0;
+};
+
+void snap32_alloc(Pool &P, const emo_ctx *c, Snap32 &s, bool particles, bool g0, bool tape, bool currents) {
+  const size_t n = (size_t)std::max<int64_t>(c->np, 1);
+  const size_t nt = (size_t)c->geo.ntot;
+  if (particles) {
+    P.take(&s.pa, n);
+    P.take(&s.pb, n);
+  }
+  if (g0) P.take(&s.g0, n);
+  for (int d = 0; d < 3; d++) {
+    P.take(&s.E[d], nt);
+    P.take(&s.A[0][d], nt);
+    P.take(&s.A[1][d], nt);
+    if (currents) P.take(&s.J[d], nt);
+  }
+  P.take(&s.off, (size_t)c->geo.ncell + 1);
+  if (tape) P.take(&s.fwd, n);
+}
+
+// the fields and ranges (and, with particles, the records) of the context
+void snap32_take(emo_ctx *c, Snap32 &s, double t, bool particles) {
+  Fp32View v = fp32_view(c);
+  const size_t np = (size_t)c->np, nt4 = (size_t)c->geo.ntot * 4;
+  CopyBatch cb(c);
+  if (particles && s.pa) {
+    cb.add(s.pa, *v.pa, np * 16);
+    cb.add(s.pb, *v.pb, np * 16);
+  }
+  s.has_g0 = s.g0 && *v.g0_active;
+  if (s.has_g0) cb.add(s.g0, *v.g0, np * 4);
+  for (int d = 0; d < 3; d++) {
+    cb.add(s.E[d], v.fs->E[d], nt4);
+    cb.add(s.A[0][d], v.fs->Bfix[d], nt4);
+    cb.add(s.A[1][d], v.Btfix[d], nt4);
+    if (s.J[d]) cb.add(s.J[d], v.fs->J[d], nt4);
+  }
+  cb.add(s.off, *v.off, (size_t)(c->geo.ncell + 1) * 4);
+  cb.flush();
+  s.b_is_a0 = v.fs->B[0] == v.fs->Bfix[0];
+  s.t = t;
+}
+
+// snapshot -> context: the records into pa/pb (which the caller may point
+// at tape buffers), fields, ranges, B roles; the stored gamma by reference
+void snap32_restore(emo_ctx *c, const Snap32 &s, int4 *pa, int4 *pb) {
+  Fp32View v = fp32_view(c);
+  const size_t np = (size_t)c->np, nt4 = (size_t)c->geo.ntot * 4;
+  CopyBatch cb(c);
+  cb.add(pa, s.pa, np * 16);
+  cb.add(pb, s.pb, np * 16);
+  for (int d = 0; d < 3; d++) {
+    cb.add(v.fs->E[d], s.E[d], nt4);
+    cb.add(v.fs->Bfix[d], s.A[0][d], nt4);
+    cb.add(v.Btfix[d], s.A[1][d], nt4);
+    if (s.J[d]) cb.add(v.fs->J[d], s.J[d], nt4);
+  }
+  cb.add(*v.off, s.off, (size_t)(c->geo.ncell + 1) * 4);
+  cb.flush();
+  *v.pa = pa;
+  *v.pb = pb;
+  for (int q = 0; q < 3; q++) {
+    v.fs->B[q] = s.b_is_a0 ? v.fs->Bfix[q] : v.Btfix[q];
+    v.fs->Bt[q] = s.b_is_a0 ? v.Btfix[q] : v.fs->Bfix[q];
+  }
+  *v.g0_active = s.has_g0;
+  *v.g0 = s.has_g0 ? s.g0 : nullptr;
+  *v.g0_owned = false;
+}
+
+struct AdjRun32 {
+  emo_ctx *c;
+  hipStream_t as;
+  double dt;
+  int nsteps, K, nseg;
+  std::vector<Snap32> ck, tape[2];
+  Snap32 fin;
+  AdjFields af;
+  double *dsums = nullptr;
+  PartAdj32 pa, pa2;
+  float *g0b = nullptr;
+  std::vector<FillPlane> planes;
+  int4 *own[4];  // the context's own record buffers (pa, pb, pa_alt, pb_alt)
+};
+
+void restore_own_buffers(AdjRun32 &R) {
+  Fp32View v = fp32_view(R.c);
+  *v.pa = R.own[0];
+  *v.pb = R.own[1];
+  *v.pa_alt = R.own[2];
+  *v.pb_alt = R.own[3];
+}
+
+// Segment seg re-run from its checkpoint on the context's stream.  Each
+// step writes its new records straight into the next tape entry's buffers
+// (the state's alternate buffers point there), so the tape holds every
+// step's pre-push records without copies.
+int recompute32(AdjRun32 &R, int seg, int set) {
+  emo_ctx *c = R.c;
+  Fp32View v = fp32_view(c);
+  const int s0 = seg * R.K, s1 = std::min(R.nsteps, s0 + R.K);
+  std::vector<Snap32> &tp = R.tape[set];
+  snap32_restore(c, R.ck[seg], tp[0].pa, tp[0].pb);
+  double t = R.ck[seg].t;
+  int rc = 0;
+  for (int st = s0; st < s1 && !rc; st++) {
+    snap32_take(c, tp[st - s0], t, false);
+    *v.pa_alt = tp[st - s0 + 1].pa;
+    *v.pb_alt = tp[st - s0 + 1].pb;
+    rc = fp32_step_one(c, t, R.dt, tp[st - s0].fwd);
+    t += R.dt;
+  }
+  restore_own_buffers(R);
+  *v.g0 = nullptr;
+  *v.g0_active = false;
+  *v.g0_owned = true;
+  return rc;
+}
+
+void backward32(AdjRun32 &R, int seg, int set) {
+  emo_ctx *c = R.c;
+  const Geo &g = c->geo;
+  hipStream_t as = R.as;
+  const int s0 = seg * R.K, s1 = std::min(R.nsteps, s0 + R.K);
+  const int nplanes = (int)R.planes.size();
+  bool all_per = true;
+  for (int q = 0; q < 6; q++) all_per = all_per && c->cfg.parts_bound[q] == EMO_BC_PERIODIC;
+  const PushK k = fp32_push_consts(c, R.dt);
+  for (int st = s1 - 1; st >= s0; st--) {
+    const Snap32 &pre = R.tape[set][st - s0];
+    fields_step_adj(c, as, R.af, R.planes, R.dsums + (size_t)st * nplanes, R.dt, pre.b_is_a0);
+    CellStepAdj32Args ca;
+    ca.k = k;
+    ca.pa = pre.pa;
+    ca.pb = pre.pb;
+    ca.g0 = (st == 0 && R.ck[0].has_g0) ? R.ck[0].g0 : nullptr;
+    ca.off = pre.off;
+    ca.fwd = pre.fwd;
+    for (int d = 0; d < 3; d++) {
+      ca.E[d] = pre.E[d];
+      ca.B[d] = pre.b_is_a0 ? pre.A[0][d] : pre.A[1][d];
+      ca.Bt[d] = pre.b_is_a0 ? pre.A[1][d] : pre.A[0][d];
+      ca.Jb[d] = R.af.Jb[d];
+    }
+    ca.in = R.pa;
+    ca.out = R.pa2;
+    ca.g0b = ca.g0 ? R.g0b : nullptr;
+    ca.cg = R.af.cg;
+    ca.ncell = g.ncell;
+    const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
+    {
+      TimedLaunch tl(c, "adj_cells_f32", as);
+      if (c->cfg.relativity && all_per)
+        hipLaunchKernelGGL((k_cell_step_adj32<true, true>), dim3(blocks), dim3(kBlock), 0, as, ca);
+      else if (c->cfg.relativity)
+        hipLaunchKernelGGL((k_cell_step_adj32<true, false>), dim3(blocks), dim3(kBlock), 0, as, ca);
+      else if (all_per)
+        hipLaunchKernelGGL((k_cell_step_adj32<false, true>), dim3(blocks), dim3(kBlock), 0, as, ca);
+      else
+        hipLaunchKernelGGL((k_cell_step_adj32<false, false>), dim3(blocks), dim3(kBlock), 0, as, ca);
+    }
+    gather_adj_grid_roles(c, as, pre.b_is_a0, R.af);
+    std::swap(R.pa, R.pa2);
+  }
+}
+
+__global__ void k_zero_f(float *p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
 }  // namespace
+
+// The gradient call of the FP32 chosen form: forward with checkpoints every
+// K steps (the last segment taped on the way), then segment by segment
+// backwards, each segment's recomputation (context stream) overlapping the
+// adjoint of the segment after it (adjoint stream).
+int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, int32_t K, double *energy,
+                              double *d_amp, double *d_phase, double *xbar0, double *vbar0, double *gbar0) {
+  if (K <= 0) K = std::max(nsteps, 1);
+  const Geo &g = c->geo;
+  const int64_t np = c->np;
+  const int nw = c->cfg.nwaves;
+  const bool rel = c->cfg.relativity != 0;
+  int rc = 0;
+  if (!c->adj_stream && hipStreamCreate(&c->adj_stream) != hipSuccess) {
+    c->adj_stream = nullptr;
+    c->err = "hipStreamCreate failed for the adjoint";
+    return EMO_ERR_HIP;
+  }
+  Fp32View v = fp32_view(c);
+  AdjRun32 R;
+  R.c = c;
+  R.as = c->adj_stream;
+  R.dt = dt;
+  R.nsteps = nsteps;
+  R.K = K;
+  R.nseg = (nsteps + K - 1) / K;
+  R.planes = fill_planes(c);
+  R.ck.resize(std::max(R.nseg, 1));
+  const int nsets = R.nseg > 1 ? 2 : 1;
+  for (int st = 0; st < nsets; st++) R.tape[st].resize(K + 1);
+  const size_t nt = (size_t)g.ntot;
+  const size_t npa = (size_t)std::max<int64_t>(np, 1);
+  const int nplanes = (int)R.planes.size();
+  const bool has_g0 = *v.g0_active && rel;
+  auto layout = [&](Pool &P) {
+    for (size_t k = 0; k < R.ck.size(); k++) snap32_alloc(P, c, R.ck[k], true, k == 0 && has_g0, false, false);
+    snap32_alloc(P, c, R.fin, false, false, false, true);
+    for (int st = 0; st < nsets; st++)
+      for (auto &sn : R.tape[st]) snap32_alloc(P, c, sn, true, false, true, false);
+    for (int d = 0; d < 3; d++) {
+      P.take(&R.af.Eb[d], nt);
+      P.take(&R.af.Ab[0][d], nt);
+      P.take(&R.af.Ab[1][d], nt);
+      P.take(&R.af.Jb[d], nt);
+      P.take(&R.pa.f[d], npa);
+      P.take(&R.pa.u[d], npa);
+      P.take(&R.pa2.f[d], npa);
+      P.take(&R.pa2.u[d], npa);
+    }
+    P.take(&R.af.cg, 18 * (size_t)g.ncell);
+    P.take(&R.dsums, (size_t)std::max(1, nsteps * nplanes));
+    P.take(&R.g0b, npa);
+  };
+  {
+    Pool dry;
+    layout(dry);
+    if (dry.off > c->adj_pool_bytes) {
+      if (c->adj_pool) hipFree(c->adj_pool);
+      c->adj_pool = nullptr;
+      c->adj_pool_bytes = 0;
+      if (hipMalloc(&c->adj_pool, dry.off) != hipSuccess) {
+        c->adj_pool = nullptr;
+        c->err = "hipMalloc failed for the adjoint's checkpoints and tape";
+        return EMO_ERR_ALLOC;
+      }
+      c->adj_pool_bytes = dry.off;
+    }
+    Pool real;
+    real.base = (char *)c->adj_pool;
+    layout(real);
+  }
+  // ---- forward with checkpoints -------------------------------------------
+  double t = t0;
+  std::vector<double> tstep(nsteps + 1);
+  for (int s = 0; s <= nsteps; s++) tstep[s] = t0 + s * dt;
+  const int last_set = R.nseg > 0 ? (R.nseg - 1) % nsets : 0;
+  for (int seg = 0; seg < R.nseg && !rc; seg++) {
+    snap32_take(c, R.ck[seg], t, true);
+    if (seg == 0 && has_g0) {
+      // the first push reads the checkpoint's copy of the stored gamma
+      if (*v.g0_owned) {
+        EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+        hipFree(*v.g0);
+      }
+      *v.g0 = R.ck[0].g0;
+      *v.g0_owned = false;
+    }
+    const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
+    if (seg == R.nseg - 1) {
+      // the own buffers of the final state's records
+      R.own[0] = *v.pa;
+      R.own[1] = *v.pb;
+      R.own[2] = *v.pa_alt;
+      R.own[3] = *v.pb_alt;
+      rc = recompute32(R, seg, last_set);
+      if (!rc && np > 0) {
+        CopyBatch cb(c);
+        cb.add(R.own[0], R.tape[last_set][s1 - s0].pa, (size_t)np * 16);
+        cb.add(R.own[1], R.tape[last_set][s1 - s0].pb, (size_t)np * 16);
+        cb.flush();
+      }
+    } else {
+      for (int st = s0; st < s1 && !rc; st++) rc = fp32_step_one(c, tstep[st], dt, nullptr);
+    }
+    t = tstep[s1];
+  }
+  if (R.nseg == 0) {
+    snap32_take(c, R.ck[0], t, true);  // the ids of the outputs
+    R.own[0] = *v.pa;
+    R.own[1] = *v.pb;
+    R.own[2] = *v.pa_alt;
+    R.own[3] = *v.pb_alt;
+  }
+  if (!rc) rc = fp32_check(c);
+  double F = 0.0;
+  if (!rc) {
+    double out8[8];
+    rc = emo_diag(c, out8);
+    F = out8[4] + out8[5];
+  }
+  bool fin_taken = false;
+  if (!rc) {
+    snap32_take(c, R.fin, t, false);
+    fin_taken = true;
+  }
+  std::vector<hipEvent_t> ev;
+  auto record = [&](hipStream_t st) -> hipEvent_t {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    ev.push_back(e);
+    if (hipEventRecord(e, st) != hipSuccess) return nullptr;
+    return e;
+  };
+  auto wait = [&](hipStream_t st, hipEvent_t e) -> int {
+    if (!e || hipStreamWaitEvent(st, e, 0) != hipSuccess) {
+      c->err = "stream ordering of the adjoint failed";
+      return EMO_ERR_HIP;
+    }
+    return EMO_OK;
+  };
+  if (!rc) rc = wait(R.as, record(c->stream));
+  if (!rc) {
+    for (int d = 0; d < 3; d++) {
+      launch_zero(R.as, R.af.Eb[d], nt);
+      launch_zero(R.as, R.af.Ab[0][d], nt);
+      launch_zero(R.as, R.af.Ab[1][d], nt);
+      launch_zero(R.as, R.af.Jb[d], nt);
+      hipLaunchKernelGGL(k_zero_f, dim3(grid_for(npa)), dim3(kBlock), 0, R.as, R.pa.f[d], (long long)npa);
+      hipLaunchKernelGGL(k_zero_f, dim3(grid_for(npa)), dim3(kBlock), 0, R.as, R.pa.u[d], (long long)npa);
+    }
+    hipLaunchKernelGGL(k_zero_f, dim3(grid_for(npa)), dim3(kBlock), 0, R.as, R.g0b, (long long)npa);
+    SeedArgs<float> sa;
+    sa.g = g;
+    for (int d = 0; d < 3; d++) {
+      sa.E[d] = R.fin.E[d];
+      sa.B[d] = R.fin.A[0][d];
+      sa.Eb[d] = R.af.Eb[d];
+      sa.Bb[d] = R.af.Ab[0][d];
+    }
+    hipLaunchKernelGGL(k_seed_energy<float>, dim3(grid_for(g.ntot)), dim3(kBlock), 0, R.as, sa);
+  }
+  if (!rc && nsteps > 0) {
+    std::vector<hipEvent_t> done_r(R.nseg, nullptr), done_a(R.nseg, nullptr);
+    done_r[R.nseg - 1] = record(c->stream);
+    for (int seg = R.nseg - 1; seg >= 0 && !rc; seg--) {
+      rc = wait(R.as, done_r[seg]);
+      if (rc) break;
+      backward32(R, seg, seg % nsets);
+      done_a[seg] = record(R.as);
+      if (seg > 0) {
+        if (seg + 1 < R.nseg) rc = wait(c->stream, done_a[seg + 1]);
+        if (!rc) rc = recompute32(R, seg - 1, (seg - 1) % nsets);
+        if (!rc) done_r[seg - 1] = record(c->stream);
+      }
+    }
+  }
+  if (!rc && hipStreamSynchronize(R.as) != hipSuccess) {
+    c->err = "adjoint stream failed";
+    rc = EMO_ERR_HIP;
+  }
+  if (!rc) rc = fp32_check(c);
+  // ---- outputs -------------------------------------------------------------
+  if (!rc) {
+    if (energy) *energy = F;
+    std::vector<double> sums((size_t)std::max(1, nsteps * nplanes));
+    EMO_HIP_CHECK_RET(c, hipMemcpyAsync(sums.data(), R.dsums, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+    std::vector<double> da(nw, 0.0), dp(nw, 0.0), ta, tp;
+    for (int s = 0; s < nsteps; s++)
+      for (int k = 0; k < nplanes; k++) {
+        pulse_derivs(c, R.planes[k], tstep[s], dt, ta, tp);
+        for (int w = 0; w < nw; w++) {
+          da[w] += sums[(size_t)s * nplanes + k] * ta[w];
+          dp[w] += sums[(size_t)s * nplanes + k] * tp[w];
+        }
+      }
+    for (int w = 0; w < nw; w++) {
+      if (d_amp) d_amp[w] = da[w];
+      if (d_phase) d_phase[w] = dp[w];
+    }
+    // particle adjoints of the initial state, by id: the offsets f = (x -
+    // x0) / dx - ix and, relativistic, u = gamma0 v with the stored gamma0
+    if ((xbar0 || vbar0 || gbar0) && np > 0) {
+      std::vector<int4> rb(np);
+      std::vector<float> g0h;
+      std::vector<float> fb[3], ub[3], g0bh;
+      EMO_HIP_CHECK_RET(c, hipMemcpy(rb.data(), R.ck[0].pb, np * 16, hipMemcpyDeviceToHost));
+      for (int d = 0; d < 3; d++) {
+        fb[d].resize(np);
+        ub[d].resize(np);
+        EMO_HIP_CHECK_RET(c, hipMemcpy(fb[d].data(), R.pa.f[d], np * 4, hipMemcpyDeviceToHost));
+        EMO_HIP_CHECK_RET(c, hipMemcpy(ub[d].data(), R.pa.u[d], np * 4, hipMemcpyDeviceToHost));
+      }
+      if (has_g0) {
+        g0h.resize(np);
+        g0bh.resize(np);
+        EMO_HIP_CHECK_RET(c, hipMemcpy(g0h.data(), R.ck[0].g0, np * 4, hipMemcpyDeviceToHost));
+        EMO_HIP_CHECK_RET(c, hipMemcpy(g0bh.data(), R.g0b, np * 4, hipMemcpyDeviceToHost));
+      }
+      for (int64_t i = 0; i < np; i++) {
+        const int64_t k = rb[i].w;
+        if (k < 0 || k >= np) continue;
+        float u[3];
+        std::memcpy(u, &rb[i], 12);
+        const double gm = has_g0 ? (double)g0h[i] : 1.0;
+        double gb = has_g0 ? (double)g0bh[i] : 0.0;
+        for (int d = 0; d < 3; d++) {
+          if (xbar0) xbar0[3 * k + d] = (double)fb[d][i] * g.id[d];
+          if (vbar0) vbar0[3 * k + d] = (double)ub[d][i] * (rel ? gm : 1.0);
+          gb += (double)ub[d][i] * ((double)u[d] / gm);
+        }
+        if (gbar0) gbar0[k] = rel ? gb : 0.0;
+      }
+    }
+  }
+  // ---- the final state back (the records stayed in the own buffers) --------
+  hipStreamSynchronize(R.as);
+  if (R.nseg > 0) restore_own_buffers(R);
+  if (nsteps > 0 && fin_taken) {
+    Fp32View w = fp32_view(c);
+    const size_t nt4 = (size_t)g.ntot * 4;
+    CopyBatch cb(c);
+    for (int d = 0; d < 3; d++) {
+      cb.add(w.fs->E[d], R.fin.E[d], nt4);
+      cb.add(w.fs->Bfix[d], R.fin.A[0][d], nt4);
+      cb.add(w.Btfix[d], R.fin.A[1][d], nt4);
+      cb.add(w.fs->J[d], R.fin.J[d], nt4);
+    }
+    cb.add(*w.off, R.fin.off, (size_t)(g.ncell + 1) * 4);
+    cb.flush();
+    for (int q = 0; q < 3; q++) {
+      w.fs->B[q] = R.fin.b_is_a0 ? w.fs->Bfix[q] : w.Btfix[q];
+      w.fs->Bt[q] = R.fin.b_is_a0 ? w.Btfix[q] : w.fs->Bfix[q];
+    }
+    *w.g0 = nullptr;
+    *w.g0_active = false;
+    *w.g0_owned = true;
+  }
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      c->err = hipGetErrorString(e);
+      rc = EMO_ERR_HIP;
+    }
+  }
+  for (hipEvent_t e : ev) hipEventDestroy(e);
+  return rc;
+}
 
 // dx0/d pert_eps of the displacement loader, per particle in load order
 int loader_dx_deps(const emo_config *cfg, const emo_loader *ld, std::vector<double> &out);
@@ -952,10 +1512,8 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
                                         double *energy, double *d_amp, double *d_phase, double *xbar0,
                                         double *vbar0, double *gbar0) {
   if (!c || nsteps < 0) return EMO_ERR_CONFIG;
-  if (c->cfg.precision != EMO_FP64_PLAIN) {
-    c->err = "the adjoint runs on the FP64 plain form";
-    return EMO_ERR_UNSUPPORTED;
-  }
+  if (c->cfg.precision != EMO_FP64_PLAIN)
+    return fp32_adjoint_field_energy(c, nsteps, t0, dt, K, energy, d_amp, d_phase, xbar0, vbar0, gbar0);
   if (K <= 0) K = std::max(nsteps, 1);
   const Geo &g = c->geo;
   const int64_t np = c->np;
@@ -1090,7 +1648,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     launch_zero(R.as, R.pb.gam, np);
     // from the final-state snapshot: the context's grid is overwritten by
     // the recomputation running beside this stream
-    SeedArgs sa;
+    SeedArgs<double> sa;
     sa.g = g;
     for (int d = 0; d < 3; d++) {
       sa.E[d] = R.fin.E[d];
@@ -1098,7 +1656,7 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
       sa.Eb[d] = R.af.Eb[d];
       sa.Bb[d] = R.af.Ab[0][d];
     }
-    hipLaunchKernelGGL(k_seed_energy, dim3(grid_for(g.ntot)), dim3(kBlock), 0, R.as, sa);
+    hipLaunchKernelGGL(k_seed_energy<double>, dim3(grid_for(g.ntot)), dim3(kBlock), 0, R.as, sa);
   }
   // ---- backward: segment seg's adjoint (adjoint stream) runs while segment
   // seg - 1 is recomputed (context stream) into the other tape set ----------

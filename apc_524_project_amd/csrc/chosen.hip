@@ -63,11 +63,9 @@
 #include "deposit_grid.hpp"
 #include "emo_internal.hpp"
 #include "uniform_load.hpp"
+#include "chosen_math.hpp"
 
 namespace emo {
-
-constexpr int kSpeciesShift = 26;
-constexpr int32_t kCellMask = (1 << kSpeciesShift) - 1;
 
 struct ChosenState {
   int64_t cap = 0;
@@ -76,6 +74,7 @@ struct ChosenState {
   bool have_id = false;
   float *g0 = nullptr;  // stored gamma for the first relativistic push (Q3)
   bool g0_active = false;
+  bool g0_owned = true;  // g0 is this state's allocation (not a snapshot's)
   CellSortBufs srt;  // the counting sort's buffers (cellsort.hpp); srt.perm is the inverse permutation
   int32_t *fdst = nullptr;  // destination cell of a far mover
   // initial sort of loaded particles (arbitrary order)
@@ -91,28 +90,6 @@ struct ChosenState {
 
 namespace {
 
-struct SpeciesF32 {
-  float qp[EMO_NSPEC];  // ((0.5 dt q)/m) UNIT_ACC
-  float jc[EMO_NSPEC];  // UNIT_RHOJ q / V: current per unit velocity per particle
-  int test[EMO_NSPEC];
-};
-
-SpeciesF32 species_f32(const emo_ctx *c, double dt) {
-  SpeciesF32 s;
-  double V = c->geo.d[0] * c->geo.d[1] * c->geo.d[2];
-  for (int k = 0; k < EMO_NSPEC; k++) {
-    double q = c->species.q[k], m = c->species.m[k];
-    double qp = (0.5 * dt * q) / m;
-    qp *= kUnitAcc;
-    s.qp[k] = (float)qp;
-    s.jc[k] = (float)(kUnitRhoJ * q / V);
-    s.test[k] = c->species.test[k];
-  }
-  return s;
-}
-
-__device__ inline float as_f(int v) { return __int_as_float(v); }
-
 // Work-groups go to the 8 XCDs round-robin (block b to XCD b % 8).  This
 // renumbers them so that each XCD takes a contiguous eighth of the tiles,
 // and its L2 holds only the gather vectors and cell ranges of its own part
@@ -124,32 +101,6 @@ __device__ inline unsigned xcd_tile() {
   return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
 }
 inline unsigned xcd_grid(unsigned blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
-__device__ inline int as_i(float v) { return __float_as_int(v); }
-
-// ---------------------------------------------------------------------------
-// One particle's gather + push + particle BCs.  Both particle passes call
-// this on the same inputs; the file is compiled without floating-point
-// contraction (every fused multiply-add is an explicit fmaf), so the two calls
-// give the same result bit for bit.  Branch-free: a wavefront's lanes take
-// different sides of the boundary tests.
-// ---------------------------------------------------------------------------
-struct PushK {
-  Geo g;
-  const float4 *cv;  // per cell, the 18 gather values (k_cellvec_f32)
-  SpeciesF32 s;
-  float dtid[3];  // dt / dx per dim
-  int lper[3], rper[3];  // periodic particle BC on the lower / upper side
-  int per[3];     // periodic particle BCs per dim
-  FastDiv div1, div2;  // division by nc[1], nc[2]
-};
-
-struct Pushed {
-  int cell;     // new cell id
-  int code;     // move code (kFarCode: further than one cell)
-  float f[3];   // new offset
-  float u[3];   // new u (after the BCs)
-  float v[3];   // new velocity (deposit)
-};
 
 // The gather vector of a cell (Grid::getFieldInterpolatorVec,
 // src/grid/oGrid.cpp:153-197, without the vertex): the 4 edge values of each
@@ -160,8 +111,6 @@ struct Pushed {
 //   {Ez(i,j,k), Ez(i+1,j,k), Ez(i+1,j+1,k), Ez(i,j+1,k)}
 //   {Bx(i), Bx(i+1), By(j), By(j+1)}   (face averages)
 //   {Bz(k), Bz(k+1), 0, 0}
-constexpr int kCellVec = 5;
-
 struct CellVecArgs {
   Geo g;
   const float *E[3], *B[3], *Bt[3];
@@ -183,164 +132,6 @@ __global__ void __launch_bounds__(kBlock) k_cellvec_f32(CellVecArgs a) {
   v[3] = make_float4(0.5f * (a.Bt[0][o] + a.B[0][o]), 0.5f * (a.Bt[0][o + X] + a.B[0][o + X]),
                      0.5f * (a.Bt[1][o] + a.B[1][o]), 0.5f * (a.Bt[1][o + Y] + a.B[1][o + Y]));
   v[4] = make_float4(0.5f * (a.Bt[2][o] + a.B[2][o]), 0.5f * (a.Bt[2][o + 1] + a.B[2][o + 1]), 0.f, 0.f);
-}
-
-// padded indices of a cell id (division by multiply-high)
-__device__ inline void decode_fast(const PushK &a, int cell, int &ix, int &iy, int &iz) {
-  const uint32_t t = fastdiv((uint32_t)cell, a.div2);
-  iz = cell - (int)(t * (uint32_t)a.g.nc[2]);
-  const uint32_t x = fastdiv(t, a.div1);
-  iy = (int)(t - x * (uint32_t)a.g.nc[1]);
-  ix = (int)x;
-}
-
-// One dimension of the move in cell units: split the new position pf of a
-// particle of padded cell ix into cell and offset, then the particle BCs of
-// the two sides in the reference's order (src/particles/particle_handler.cpp:
-// 287-295).  p = ix-1 is the physical cell, P = p + f the position / dx.
-// Periodic (bc_p_periodic.cpp:44-87): P < 0 -> P + n; P > n -> P - n.
-// Reflecting (bc_p_reflecting.cpp:43-53): mirror at 0 or n and flip u.
-__device__ inline void move_dim(float pf, int ix, int n, bool lper, bool rper, int &ix_out, float &f_out,
-                                float &u) {
-  const float fl = floorf(pf);
-  float fr = pf - fl;
-  int p = ix - 1 + (int)fl;
-  const bool round_up = fr >= 1.f;  // rounding of a tiny negative offset
-  fr = round_up ? 0.f : fr;
-  p = round_up ? p + 1 : p;
-  {
-    const bool L = p < 0;
-    const bool fz = fr == 0.f;
-    const int pl = lper ? p + n : (fz ? -p : -p - 1);
-    const float fl2 = (lper || fz) ? fr : 1.f - fr;
-    p = L ? pl : p;
-    fr = L ? fl2 : fr;
-    u = (L && !lper) ? -u : u;
-  }
-  {
-    const bool R = p > n || (p == n && fr > 0.f);
-    const bool fz = fr == 0.f;
-    const int pr = rper ? p - n : (fz ? 2 * n - p : 2 * n - p - 1);
-    const float fr2 = (rper || fz) ? fr : 1.f - fr;
-    p = R ? pr : p;
-    fr = R ? fr2 : fr;
-    u = (R && !rper) ? -u : u;
-  }
-  ix_out = p + 1;
-  f_out = fr;
-}
-
-// move_dim when both sides of the dimension are periodic
-__device__ inline void move_dim_periodic(float pf, int ix, int n, int &ix_out, float &f_out) {
-  const float fl = floorf(pf);
-  float fr = pf - fl;
-  int p = ix - 1 + (int)fl;
-  const bool round_up = fr >= 1.f;
-  fr = round_up ? 0.f : fr;
-  p = round_up ? p + 1 : p;
-  p = p < 0 ? p + n : p;
-  p = (p > n || (p == n && fr > 0.f)) ? p - n : p;
-  ix_out = p + 1;
-  f_out = fr;
-}
-
-// The cell's gather vector (five 16-byte loads).
-struct CellVec {
-  float4 Ex, Ey, Ez, B, Bz;
-};
-__device__ inline CellVec load_cellvec(const PushK &a, int w) {
-  const float4 *cv = a.cv + (long long)(w & kCellMask) * kCellVec;
-  return CellVec{cv[0], cv[1], cv[2], cv[3], cv[4]};
-}
-
-// q_p: ((0.5 dt q)/m) UNIT_ACC of the particle's species; g0: the stored
-// gamma for the first relativistic push (Q3), if use_g0
-template <bool REL, bool PER>
-__device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, int w, float fx, float fy,
-                                              float fz, float ux, float uy, float uz, float q_p, bool use_g0,
-                                              float g0, Pushed &o, unsigned long long *flag) {
-  const Geo &g = a.g;
-  int ix, iy, iz;
-  decode_fast(a, w & kCellMask, ix, iy, iz);
-  // gather (Interpolator::interpolate_fields, src/particles/interpolate.cpp:11-47)
-  // from the cell's gather vector: E bilinear over the 4 edges, B linear
-  // over 2 faces
-  const float ax = 2.f - fx, bx = fx - 1.f;
-  const float ay = 2.f - fy, by = fy - 1.f;
-  const float az = 2.f - fz, bz = fz - 1.f;
-  const float4 vEx = V.Ex, vEy = V.Ey, vEz = V.Ez, vB = V.B, vBz = V.Bz;
-  const float ex = fmaf(az, fmaf(ay, vEx.x, by * vEx.y), bz * fmaf(by, vEx.z, ay * vEx.w));
-  const float ey = fmaf(ax, fmaf(az, vEy.x, bz * vEy.y), bx * fmaf(bz, vEy.z, az * vEy.w));
-  const float ez = fmaf(ay, fmaf(ax, vEz.x, bx * vEz.y), by * fmaf(bx, vEz.z, ax * vEz.w));
-  const float bxf = fmaf(ax, vB.x, bx * vB.y);
-  const float byf = fmaf(ay, vB.z, by * vB.w);
-  const float bzf = fmaf(az, vBz.x, bz * vBz.y);
-  float ig_new = 1.f;
-  if (REL) {
-    // Relativistic_Boris::Step (src/pusher/relativisticBoris.cpp:28-140) on
-    // u = gamma v.  Reciprocals and square roots are the hardware's
-    // single-instruction forms (about 1 ulp).
-    const float ig = use_g0 ? __builtin_amdgcn_rcpf(g0)
-                            : __builtin_amdgcn_rsqf(fmaf(ux, ux, fmaf(uy, uy, fmaf(uz, uz, 1.f))));
-    const float vx0 = ux * ig, vy0 = uy * ig, vz0 = uz * ig;
-    const float upx = fmaf(q_p, ex, fmaf(q_p, ex + fmaf(vy0, bzf, -(vz0 * byf)), ux));
-    const float upy = fmaf(q_p, ey, fmaf(q_p, ey + fmaf(vz0, bxf, -(vx0 * bzf)), uy));
-    const float upz = fmaf(q_p, ez, fmaf(q_p, ez + fmaf(vx0, byf, -(vy0 * bxf)), uz));
-    const float taux = q_p * bxf, tauy = q_p * byf, tauz = q_p * bzf;
-    const float tau_sq = fmaf(taux, taux, fmaf(tauy, tauy, tauz * tauz));
-    const float sigma = fmaf(upx, upx, fmaf(upy, upy, fmaf(upz, upz, 1.f))) - tau_sq;
-    const float ustar = fmaf(upx, taux, fmaf(upy, tauy, upz * tauz));
-    const float root = __builtin_amdgcn_sqrtf(fmaf(sigma, sigma, 4.f * fmaf(ustar, ustar, tau_sq)));
-    const float igi = __builtin_amdgcn_rsqf(0.5f * (sigma + root));
-    const float tx = taux * igi, ty = tauy * igi, tz = tauz * igi;
-    const float s = __builtin_amdgcn_rcpf(fmaf(tx, tx, fmaf(ty, ty, fmaf(tz, tz, 1.f))));
-    const float udott = ustar * igi;
-    ux = s * fmaf(udott, tx, upx + fmaf(upy, tz, -(upz * ty)));
-    uy = s * fmaf(udott, ty, upy + fmaf(upz, tx, -(upx * tz)));
-    uz = s * fmaf(udott, tz, upz + fmaf(upx, ty, -(upy * tx)));
-    ig_new = igi;
-  } else {
-    // Boris::Step (src/pusher/boris.cpp:12-75)
-    const float tx = q_p * bxf, ty = q_p * byf, tz = q_p * bzf;
-    const float bsqr = fmaf(bxf, bxf, fmaf(byf, byf, bzf * bzf));
-    const float tsqr = 2.f * __builtin_amdgcn_rcpf(fmaf(q_p * q_p, bsqr, 1.f));
-    const float sx_ = tsqr * tx, sy_ = tsqr * ty, sz_ = tsqr * tz;
-    const float v_mx = fmaf(q_p, ex, ux), v_my = fmaf(q_p, ey, uy), v_mz = fmaf(q_p, ez, uz);
-    const float v_ix = v_mx + fmaf(v_my, tz, -(v_mz * ty));
-    const float v_iy = v_my + fmaf(v_mz, tx, -(v_mx * tz));
-    const float v_iz = v_mz + fmaf(v_mx, ty, -(v_my * tx));
-    ux = fmaf(q_p, ex, v_mx + fmaf(v_iy, sz_, -(v_iz * sy_)));
-    uy = fmaf(q_p, ey, v_my + fmaf(v_iz, sx_, -(v_ix * sz_)));
-    uz = fmaf(q_p, ez, v_mz + fmaf(v_ix, sy_, -(v_iy * sx_)));
-  }
-  // x += dt v, in cell units, split into cell index and offset; BCs
-  int jx, jy, jz;
-  if (PER) {  // every particle BC periodic
-    move_dim_periodic(fmaf(a.dtid[0], ux * ig_new, fx), ix, g.n[0], jx, o.f[0]);
-    move_dim_periodic(fmaf(a.dtid[1], uy * ig_new, fy), iy, g.n[1], jy, o.f[1]);
-    move_dim_periodic(fmaf(a.dtid[2], uz * ig_new, fz), iz, g.n[2], jz, o.f[2]);
-  } else {
-    move_dim(fmaf(a.dtid[0], ux * ig_new, fx), ix, g.n[0], a.lper[0], a.rper[0], jx, o.f[0], ux);
-    move_dim(fmaf(a.dtid[1], uy * ig_new, fy), iy, g.n[1], a.lper[1], a.rper[1], jy, o.f[1], uy);
-    move_dim(fmaf(a.dtid[2], uz * ig_new, fz), iz, g.n[2], a.lper[2], a.rper[2], jz, o.f[2], uz);
-  }
-  if (jx < 0 || jx >= g.nc[0] || jy < 0 || jy >= g.nc[1] || jz < 0 || jz >= g.nc[2]) {
-    if (flag) atomicOr(flag, kFlagOutOfGrid);
-    jx = min(max(jx, 1), g.n[0]);
-    jy = min(max(jy, 1), g.n[1]);
-    jz = min(max(jz, 1), g.n[2]);
-  }
-  o.cell = (jx * g.nc[1] + jy) * g.nc[2] + jz;
-  const int k0 = move_code(jx - ix, g.n[0], PER ? 1 : a.per[0]);
-  const int k1 = move_code(jy - iy, g.n[1], PER ? 1 : a.per[1]);
-  const int k2 = move_code(jz - iz, g.n[2], PER ? 1 : a.per[2]);
-  o.code = (k0 < 0 || k1 < 0 || k2 < 0) ? (int)kFarCode : (k0 * 6 + k1) * 6 + k2;
-  o.u[0] = ux;
-  o.u[1] = uy;
-  o.u[2] = uz;
-  o.v[0] = ux * ig_new;
-  o.v[1] = uy * ig_new;
-  o.v[2] = uz * ig_new;
 }
 
 // ---------------------------------------------------------------------------
@@ -634,6 +425,12 @@ __global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
 #pragma unroll
     for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = hi[q] + lo[q];
   }
+}
+
+// the sort as pre-push index -> new slot, from the inverse permutation
+__global__ void k_fwd_from_inv(const uint32_t *inv, uint32_t *fwd, long long np) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < np) fwd[inv[j]] = (uint32_t)j;
 }
 
 // ---------------------------------------------------------------------------
@@ -1483,27 +1280,61 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
   return check_flag(c);
 }
 
-int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
+PushK fp32_push_consts(emo_ctx *c, double dt) { return push_consts(c, dt); }
+
+int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out) {
   ChosenState *s = c->ch;
-  double t = t0;
-  for (int k = 0; k < nsteps; k++) {
-    int rc = particle_step(c, dt);
-    if (rc) return rc;
-    if (s->g0_active) {
-      // the stored gamma was used by this first push only (Q3): from now on
-      // gamma = sqrt(1 + u^2)
+  int rc = particle_step(c, dt);
+  if (rc) return rc;
+  if (fwd_out && c->np > 0)
+    hipLaunchKernelGGL(k_fwd_from_inv, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, (const uint32_t *)s->srt.perm,
+                       fwd_out, (long long)c->np);
+  if (s->g0_active) {
+    // the stored gamma was used by this first push only (Q3): from now on
+    // gamma = sqrt(1 + u^2)
+    if (s->g0_owned) {
       EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
-      s->g0_active = false;
       dfree(s->g0);
-      s->g0 = nullptr;
     }
-    fk_fold<float>(c, s->fs);
-    fk_evolve<float>(c, s->fs, dt);
-    refresh_fields<float>(c, s->fs, t, dt);
-    t += dt;
+    s->g0_active = false;
+    s->g0 = nullptr;
+    s->g0_owned = true;
   }
+  fk_fold<float>(c, s->fs);
+  fk_evolve<float>(c, s->fs, dt);
+  refresh_fields<float>(c, s->fs, t, dt);
+  return EMO_OK;
+}
+
+int fp32_check(emo_ctx *c) {
   EMO_HIP_CHECK_RET(c, hipGetLastError());
   return check_flag(c);
+}
+
+Fp32View fp32_view(emo_ctx *c) {
+  ChosenState *s = c->ch;
+  Fp32View v;
+  v.pa = &s->pa;
+  v.pb = &s->pb;
+  v.pa_alt = &s->pa_alt;
+  v.pb_alt = &s->pb_alt;
+  v.g0 = &s->g0;
+  v.g0_active = &s->g0_active;
+  v.g0_owned = &s->g0_owned;
+  v.fs = &s->fs;
+  for (int q = 0; q < 3; q++) v.Btfix[q] = s->Btfix[q];
+  v.off = &s->srt.off;
+  return v;
+}
+
+int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
+  double t = t0;
+  for (int k = 0; k < nsteps; k++) {
+    int rc = fp32_step_one(c, t, dt, nullptr);
+    if (rc) return rc;
+    t += dt;
+  }
+  return fp32_check(c);
 }
 
 int fp32_diag(emo_ctx *c, double *out8) {

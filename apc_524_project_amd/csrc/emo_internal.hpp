@@ -254,6 +254,25 @@ int fp32_get_field(emo_ctx *c, int fid, double *out);
 int fp32_set_field(emo_ctx *c, int fid, const double *in);
 int fp32_init_constant(emo_ctx *c);
 int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps, double *resid);
+// The FP32 form's device state, by reference, for its adjoint (adjoint.hip):
+// the particle records (current and alternate buffers), the stored gamma of
+// the first push, the fields and the cell ranges of the current order.
+struct Fp32View {
+  int4 **pa, **pb, **pa_alt, **pb_alt;
+  float **g0;
+  bool *g0_active, *g0_owned;
+  FieldSet<float> *fs;
+  float *Btfix[3];
+  int32_t **off;
+};
+Fp32View fp32_view(emo_ctx *c);
+// one step of fp32_step without the flag readback; with fwd_out, the step's
+// sort as pre-push index -> new slot
+int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out);
+// the device error flags of the FP32 form (stream synchronisation)
+int fp32_check(emo_ctx *c);
+int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, int32_t K, double *energy,
+                              double *d_amp, double *d_phase, double *xbar0, double *vbar0, double *gbar0);
 struct UniformGen;  // uniform_load.hpp
 int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n);
 void launch_uniform_f64(emo_ctx *c, const UniformGen &u, int64_t n);
