@@ -29,7 +29,7 @@ EXPORTS = (
     "emo_get_particles", "emo_get_cell_ids", "emo_field_points", "emo_get_field", "emo_set_field",
     "emo_init_constant", "emo_init_poisson", "emo_sort", "emo_push", "emo_particle_bc",
     "emo_deposit", "emo_fold", "emo_evolve_fields", "emo_refresh", "emo_interpolate",
-    "emo_step", "emo_diag", "emo_synchronize", "emo_timing_enable", "emo_kernel_time",
+    "emo_step", "emo_diag", "emo_synchronize", "emo_timing_enable", "emo_timing_select", "emo_kernel_time",
     "emo_timing_reset", "emo_adjoint_field_energy", "emo_loader_dx_deps",
 )
 
@@ -137,6 +137,8 @@ def lib():
     L.emo_step.argtypes = [vp, i32, dbl, dbl]
     L.emo_diag.argtypes = [vp, vp]
     L.emo_timing_enable.argtypes = [vp, i32]
+    L.emo_timing_select.argtypes = [vp, C.c_char_p]
+    L.emo_timing_select.restype = i32
     L.emo_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(dbl), C.POINTER(i64)]
     L.emo_adjoint_field_energy.argtypes = [vp, i32, dbl, dbl, i32, vp, vp, vp, vp, vp, vp]
     L.emo_loader_dx_deps.argtypes = [C.POINTER(EmoLoader), C.POINTER(EmoConfig), vp]
@@ -275,6 +277,10 @@ class Context:
     # timing
     def timing(self, on=True):
         self._chk(self.L.emo_timing_enable(self.h, 1 if on else 0))
+
+    def timing_select(self, name=None):
+        """Time only the launches of kernel `name` (None: all of them)."""
+        self._chk(self.L.emo_timing_select(self.h, name.encode() if name else None))
 
     def timing_reset(self):
         self._chk(self.L.emo_timing_reset(self.h))

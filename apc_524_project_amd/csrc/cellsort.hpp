@@ -244,8 +244,9 @@ struct BaseArgs {
 };
 
 // One thread per destination cell: its (source, code) groups in ascending
-// source id, far movers merged in by source id.  The group sizes of all
-// candidates are loaded first (independent loads), then walked in order.
+// source id, far movers merged in by source id.  The group sizes of the
+// candidates with one source x are loaded together (independent loads),
+// then walked in order; 16 live sizes per thread keep the occupancy up.
 __global__ void __launch_bounds__(kBlock) k_base(BaseArgs a) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const Geo &g = a.g;
@@ -265,31 +266,30 @@ __global__ void __launch_bounds__(kBlock) k_base(BaseArgs a) {
     cand1_pick(Z, q, zs[q], zk[q]);
   }
   const uint16_t *__restrict__ nsk_in = a.nsk;
-  int nn[64];
-#pragma unroll
-  for (int ia = 0; ia < 4; ia++)
-#pragma unroll
-    for (int ib = 0; ib < 4; ib++)
-#pragma unroll
-      for (int ic = 0; ic < 4; ic++) {
-        const int slot = (ia * 4 + ib) * 4 + ic;
-        nn[slot] = 0;
-        if (ia < X.n && ib < Y.n && ic < Z.n) {
-          const int s = (xs[ia] * g.nc[1] + ys[ib]) * g.nc[2] + zs[ic];
-          nn[slot] = nsk_in[(long long)((xk[ia] * 6 + yk[ib]) * 6 + zk[ic]) * g.ncell + s];
-        }
-      }
   const int nf = min(*a.nfar, a.far_cap);
   int fp = 0;
   int run = 0;
 #pragma unroll
-  for (int ia = 0; ia < 4; ia++)
+  for (int ia = 0; ia < 4; ia++) {
+    if (ia >= X.n) break;
+    int nn[16];
 #pragma unroll
     for (int ib = 0; ib < 4; ib++)
 #pragma unroll
       for (int ic = 0; ic < 4; ic++) {
-        const int slot = (ia * 4 + ib) * 4 + ic;
-        if (!(ia < X.n && ib < Y.n && ic < Z.n)) continue;
+        const int slot = ib * 4 + ic;
+        nn[slot] = 0;
+        if (ib < Y.n && ic < Z.n) {
+          const int s = (xs[ia] * g.nc[1] + ys[ib]) * g.nc[2] + zs[ic];
+          nn[slot] = nsk_in[(long long)((xk[ia] * 6 + yk[ib]) * 6 + zk[ic]) * g.ncell + s];
+        }
+      }
+#pragma unroll
+    for (int ib = 0; ib < 4; ib++)
+#pragma unroll
+      for (int ic = 0; ic < 4; ic++) {
+        const int slot = ib * 4 + ic;
+        if (!(ib < Y.n && ic < Z.n)) continue;
         const int s = (xs[ia] * g.nc[1] + ys[ib]) * g.nc[2] + zs[ic];
         const long long idx = (long long)((xk[ia] * 6 + yk[ib]) * 6 + zk[ic]) * g.ncell + s;
         while (fp < nf) {
@@ -305,6 +305,7 @@ __global__ void __launch_bounds__(kBlock) k_base(BaseArgs a) {
           run += n;
         }
       }
+  }
   for (; fp < nf; fp++)
     if (a.far[fp].z == (int)c) a.far_pos[fp] = run++;
   if (run > kMaxGroup) atomicOr(a.flag, kFlagMigrate);

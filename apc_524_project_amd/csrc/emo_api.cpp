@@ -20,7 +20,7 @@ int load_maxwellian_host(const emo_config *cfg, const emo_loader *ld, std::vecto
 
 TimedLaunch::TimedLaunch(emo_ctx *ctx, const char *nm, hipStream_t stream)
     : c(ctx), name(nm), s(stream ? stream : ctx->stream) {
-  if (!c->timing) return;
+  if (!c->timing || (!c->timing_only.empty() && c->timing_only != nm)) return;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
     a = b = nullptr;
     return;
@@ -29,7 +29,7 @@ TimedLaunch::TimedLaunch(emo_ctx *ctx, const char *nm, hipStream_t stream)
 }
 
 TimedLaunch::~TimedLaunch() {
-  if (!c->timing || !a) return;
+  if (!a) return;
   hipEventRecord(b, s);
   c->pending[name].push_back({a, b});
 }
@@ -746,6 +746,12 @@ int emo_synchronize(emo_ctx *c) {
 int emo_timing_enable(emo_ctx *c, int32_t on) {
   if (!c) return EMO_ERR_CONFIG;
   c->timing = on != 0;
+  return EMO_OK;
+}
+
+int emo_timing_select(emo_ctx *c, const char *name) {
+  if (!c) return EMO_ERR_CONFIG;
+  c->timing_only = name ? name : "";
   return EMO_OK;
 }
 

@@ -138,6 +138,7 @@ struct emo_ctx {
 
   // timing
   bool timing = false;
+  std::string timing_only;  // non-empty: time only the launches of this name
   struct Rec {
     hipEvent_t a, b;
   };

@@ -205,6 +205,10 @@ int emo_synchronize(emo_ctx *ctx);
  * the context stream; emo_kernel_time returns the summed milliseconds and
  * launch count per kernel name since the last reset. */
 int emo_timing_enable(emo_ctx *ctx, int32_t on);
+/* Restrict the event timing to the launches of one kernel name (NULL or ""
+ * times them all), so that a timed run of many steps carries two events per
+ * step instead of two per launch.  No counterpart in the reference. */
+int emo_timing_select(emo_ctx *ctx, const char *name);
 int emo_kernel_time(emo_ctx *ctx, const char *name, double *ms_total, int64_t *launches);
 int emo_timing_reset(emo_ctx *ctx);
 
