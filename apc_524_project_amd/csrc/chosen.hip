@@ -117,21 +117,30 @@ struct CellVecArgs {
   float4 *cv;
 };
 
+// One thread per cell; the block's vectors are staged in LDS and stored as
+// one contiguous run (a thread's own five vectors lie 80 bytes apart).
 __global__ void __launch_bounds__(kBlock) k_cellvec_f32(CellVecArgs a) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float4 buf[kBlock * kCellVec];
+  const long long c0 = (long long)blockIdx.x * blockDim.x;
+  const long long c = c0 + threadIdx.x;
   const Geo &g = a.g;
-  if (c >= g.ncell) return;
-  int ix, iy, iz;
-  decode_cell(g, (int)c, &ix, &iy, &iz);
-  const long long o = gidx(g, ix, iy, iz), X = g.sx, Y = g.sy;
-  const float *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
-  float4 *v = a.cv + c * kCellVec;
-  v[0] = make_float4(Ex[o], Ex[o + Y], Ex[o + Y + 1], Ex[o + 1]);
-  v[1] = make_float4(Ey[o], Ey[o + 1], Ey[o + X + 1], Ey[o + X]);
-  v[2] = make_float4(Ez[o], Ez[o + X], Ez[o + X + Y], Ez[o + Y]);
-  v[3] = make_float4(0.5f * (a.Bt[0][o] + a.B[0][o]), 0.5f * (a.Bt[0][o + X] + a.B[0][o + X]),
-                     0.5f * (a.Bt[1][o] + a.B[1][o]), 0.5f * (a.Bt[1][o + Y] + a.B[1][o + Y]));
-  v[4] = make_float4(0.5f * (a.Bt[2][o] + a.B[2][o]), 0.5f * (a.Bt[2][o + 1] + a.B[2][o + 1]), 0.f, 0.f);
+  if (c < g.ncell) {
+    int ix, iy, iz;
+    decode_cell(g, (int)c, &ix, &iy, &iz);
+    const long long o = gidx(g, ix, iy, iz), X = g.sx, Y = g.sy;
+    const float *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+    float4 *v = buf + threadIdx.x * kCellVec;
+    v[0] = make_float4(Ex[o], Ex[o + Y], Ex[o + Y + 1], Ex[o + 1]);
+    v[1] = make_float4(Ey[o], Ey[o + 1], Ey[o + X + 1], Ey[o + X]);
+    v[2] = make_float4(Ez[o], Ez[o + X], Ez[o + X + Y], Ez[o + Y]);
+    v[3] = make_float4(0.5f * (a.Bt[0][o] + a.B[0][o]), 0.5f * (a.Bt[0][o + X] + a.B[0][o + X]),
+                       0.5f * (a.Bt[1][o] + a.B[1][o]), 0.5f * (a.Bt[1][o + Y] + a.B[1][o + Y]));
+    v[4] = make_float4(0.5f * (a.Bt[2][o] + a.B[2][o]), 0.5f * (a.Bt[2][o + 1] + a.B[2][o + 1]), 0.f, 0.f);
+  }
+  __syncthreads();
+  const int nv = (int)min((long long)kBlock, g.ncell - c0) * kCellVec;
+  float4 *out = a.cv + c0 * kCellVec;
+  for (int i = threadIdx.x; i < nv; i += kBlock) out[i] = buf[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -385,6 +394,7 @@ struct MoveDepArgs {
   long long ncell;
   long long np;
   float *cj;
+  int32_t *nfar;  // the far-mover count, read by the earlier passes: reset here for the next step
   unsigned long long *flag;
 };
 
@@ -399,6 +409,7 @@ __global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
     jc_s[threadIdx.x] = a.k.s.test[threadIdx.x] ? 0.f : a.k.s.jc[threadIdx.x];
   }
   __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.nfar = 0;
   const int sub = threadIdx.x & (kDepLanes - 1);
   const long long c = (long long)xcd_tile() * (kBlock / kDepLanes) + threadIdx.x / kDepLanes;
   float hi[12], lo[12];
@@ -871,6 +882,7 @@ int particle_step(emo_ctx *c, double dt) {
     a.ncell = g.ncell;
     a.np = np;
     a.cj = s->cj;
+    a.nfar = s->srt.misc;
     a.flag = c->dflag;
     const unsigned blocks = xcd_grid((unsigned)((g.ncell + (kBlock / kDepLanes) - 1) / (kBlock / kDepLanes)));
     TimedLaunch t(c, "move_deposit_f32");
@@ -886,8 +898,6 @@ int particle_step(emo_ctx *c, double dt) {
   std::swap(s->pa, s->pa_alt);
   std::swap(s->pb, s->pb_alt);
   std::swap(s->srt.off, s->srt.off_new);
-  // the far-mover count starts from zero again
-  cs_fill_i32(c, s->srt.misc, 0, 1);
   DepGridArgs<float> b;
   b.g = g;
   b.cj = s->cj;
