@@ -259,7 +259,7 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
     f0, t_fwd = forward()
     c = ready()
     a = time.perf_counter()
-    out = c.adjoint_field_energy(n, 0.0, w.dt, k)
+    out = c.adjoint_field_energy(n, 0.0, w.dt, k, particles=False)
     c.synchronize()
     t_adj = time.perf_counter() - a
     c.close()
@@ -290,6 +290,88 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
           f"{(t_adj - t_fwd) / t_fwd:.2f}, gradient/forward {t_adj / t_fwd:.2f}, fd rel err {res['fd_rel_err']}",
           file=sys.stderr, flush=True)
     return res
+
+
+def run_adjoint_c5s(steps: int, k: int):
+    """c5's physics on the 128x32x32 box (16,777,216 particles: two electron
+    beams at +-0.2c, sigma 0.01c, 64 ppc each, neutralising background,
+    Poisson-initialised), `steps` steps with a checkpoint every k: the FP32
+    gradient call against its own forward run and against the FP64 adjoint
+    from the same loaded particles (the FP64 adjoint is checked against
+    central differences at c4 and in the tests).  Both hold the initial
+    fields fixed.  Reported gradient: dF/d(a uniform shift of beam 1's v_x),
+    the sum of its particles' x-velocity adjoints (stored gamma held fixed)."""
+    import numpy as np
+
+    import apc_524_project_amd as emo
+    from apc_524_project_amd import FP32_CHOSEN, FP64_PLAIN
+
+    out = {"workload": f"c5s: 128x32x32 periodic, two electron beams +-0.2c (sigma 0.01c, 64 ppc each), "
+                       f"16,777,216 particles, neutralising background, Poisson init; {steps} steps, "
+                       f"checkpoint every {k}"}
+    grads = {}
+    for prec, tag in ((FP64_PLAIN, "fp64"), (FP32_CHOSEN, "fp32")):
+        w = emo.make_workload("c5s", precision=prec)
+
+        def ready():
+            c = emo.Context(w.cfg)
+            load(c, w)
+            init_fields(c, w)
+            if prec == FP64_PLAIN:
+                c.interpolate()
+            c.synchronize()
+            return c
+
+        c = ready()
+        if prec == FP64_PLAIN:
+            c.sort()
+        c.step(5, 0.0, w.dt)  # warm-up (first launches, allocations)
+        c.close()
+        c = ready()
+        if prec == FP64_PLAIN:
+            c.sort()
+        c.synchronize()
+        a = time.perf_counter()
+        c.step(steps, 0.0, w.dt)
+        c.synchronize()
+        t_fwd = time.perf_counter() - a
+        d = c.diag()
+        f_fwd = d[4] + d[5]
+        c.close()
+        # timed without the per-particle outputs (by-id copies to the host of
+        # ~1 GB at this size); the gradient itself comes from a second call
+        c = ready()
+        a = time.perf_counter()
+        c.adjoint_field_energy(steps, 0.0, w.dt, k, particles=False)
+        c.synchronize()
+        t_grad = time.perf_counter() - a
+        c.close()
+        c = ready()
+        g = c.adjoint_field_energy(steps, 0.0, w.dt, k)
+        p = c.particles(fields=False)
+        c.close()
+        sp = np.empty(len(p["id"]), np.int32)
+        sp[p["id"]] = p["species"]
+        grads[tag] = g
+        out[tag] = {"forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+                    "adjoint_over_forward": round((t_grad - t_fwd) / t_fwd, 3),
+                    "gradient_over_forward": round(t_grad / t_fwd, 3),
+                    "energy_matches_forward": bool(g["energy"] == f_fwd), "energy": g["energy"],
+                    "dF_dvx_beam1": float(g["vbar0"][sp == 0, 0].sum())}
+        print(f"[bench] adjoint c5s {tag}: forward {t_fwd:.3f}s, gradient {t_grad:.3f}s, adjoint/forward "
+              f"{(t_grad - t_fwd) / t_fwd:.2f}, dF/dvx(beam 1) {out[tag]['dF_dvx_beam1']:.6e}", file=sys.stderr,
+              flush=True)
+    a32, a64 = out["fp32"]["dF_dvx_beam1"], out["fp64"]["dF_dvx_beam1"]
+    out["fp32_vs_fp64"] = {
+        "energy_rel": abs(out["fp32"]["energy"] - out["fp64"]["energy"]) / abs(out["fp64"]["energy"]),
+        "dF_dvx_beam1_rel": abs(a32 - a64) / abs(a64) if a64 else None,
+        "xbar0_rel_l2": float(np.linalg.norm(grads["fp32"]["xbar0"] - grads["fp64"]["xbar0"]) /
+                              np.linalg.norm(grads["fp64"]["xbar0"])),
+        "vbar0_rel_l2": float(np.linalg.norm(grads["fp32"]["vbar0"] - grads["fp64"]["vbar0"]) /
+                              np.linalg.norm(grads["fp64"]["vbar0"])),
+    }
+    print(f"[bench] adjoint c5s fp32 vs fp64: {out['fp32_vs_fp64']}", file=sys.stderr, flush=True)
+    return out
 
 
 ACCURACY_CHECKPOINTS = {
@@ -522,6 +604,9 @@ def main():
     ap.add_argument("--accuracy", default="all",
                     help="configs of the FP32-vs-FP64 error leg: all, none, or a comma list of c1,c2,c3,c4,c5s")
     ap.add_argument("--c5-steps", type=int, default=20, help="timed c5 steps (0: skip the c5 leg)")
+    ap.add_argument("--c5s-adjoint-steps", type=int, default=200,
+                    help="steps of the c5s FP32-vs-FP64 adjoint leg (0: skip it)")
+    ap.add_argument("--c5s-adjoint-only", action="store_true", help="only the c5s adjoint leg")
     ap.add_argument("--cpu-cells", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=10)
     args = ap.parse_args()
@@ -544,6 +629,10 @@ def main():
         a64 = run_adjoint_c4(args.adjoint_steps or None, args.checkpoint_every)
         a32 = run_adjoint_c4(args.adjoint_steps or None, args.checkpoint_every, True, a64)
         print(json.dumps({"adjoint": a64, "adjoint_fp32": a32}), flush=True)
+        return
+    if args.c5s_adjoint_only:
+        print(json.dumps({"adjoint_c5s": run_adjoint_c5s(args.c5s_adjoint_steps or 200, args.checkpoint_every)}),
+              flush=True)
         return
 
     have_fp32 = True
@@ -608,6 +697,8 @@ def main():
         if "fp32" in results:
             out["adjoint_fp32"] = run_adjoint_c4(args.adjoint_steps or None, args.checkpoint_every, True,
                                                  out["adjoint"])
+            if args.c5s_adjoint_steps > 0:
+                out["adjoint_c5s"] = run_adjoint_c5s(args.c5s_adjoint_steps, args.checkpoint_every)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_cells, args.cpu_steps)
