@@ -291,7 +291,7 @@ class Context:
         self._chk(self.L.emo_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
-    # adjoint (FP64 plain form)
+    # adjoint (both forms)
     def adjoint_field_energy(self, nsteps, t0, dt, checkpoint_every=0, particles=True):
         """Runs nsteps forward and the adjoint back; returns the final field
         energy and its gradient (see emo_adjoint_field_energy)."""

@@ -179,3 +179,23 @@ def test_fp32_far_movers_match_oracle():
     assert rel(g["x"], p.x) < 1e-6
     for f in (JX, JY, JZ):
         assert rel(ctx2.field(f), o.field(f)) < 1e-4, f
+
+
+def test_timing_select_times_only_the_named_kernel():
+    """emo_timing_select: events only around the named kernel's launches;
+    None times them all again."""
+    case = Case(**THERMAL, relativity=1)
+    ctx, _ = setup(case)
+    dt = case.dt_cfl()
+    ctx.timing(True)
+    ctx.timing_select("move_deposit_f32")
+    ctx.timing_reset()
+    ctx.step(3, 0.0, dt)
+    ms, n = ctx.kernel_time("move_deposit_f32")
+    assert n == 3 and ms > 0
+    assert ctx.kernel_time("push_rank_f32")[1] == 0
+    ctx.timing_select(None)
+    ctx.timing_reset()
+    ctx.step(2, 3 * dt, dt)
+    assert ctx.kernel_time("push_rank_f32")[1] == 2 and ctx.kernel_time("move_deposit_f32")[1] == 2
+    ctx.timing(False)
