@@ -980,7 +980,7 @@ struct CellStepAdj32Args {
   const int32_t *off;           // cell c of this order holds off[c]..off[c+1]
   const uint32_t *fwd;          // pre-push index -> post-sort slot
   const float *E[3], *B[3], *Bt[3];  // the fields the push gathered from
-  const double *Jb[3];          // current adjoint (after fold^T)
+  const float *Jb[3];           // current adjoint (after fold^T), rounded to FP32
   PartAdj32 in, out;            // adjoints: post-sort order in, pre-push order out
   float *g0b;                   // adjoint of the stored gamma (first step), or null
   double *cg;                   // 18 per cell, component-major
@@ -1015,10 +1015,18 @@ __global__ void __launch_bounds__(kBlock) k_cell_step_adj32(CellStepAdj32Args a)
       V.B = make_float4(0.5f * (a.Bt[0][o] + a.B[0][o]), 0.5f * (a.Bt[0][o + X] + a.B[0][o + X]),
                         0.5f * (a.Bt[1][o] + a.B[1][o]), 0.5f * (a.Bt[1][o + Y] + a.B[1][o + Y]));
       V.Bz = make_float4(0.5f * (a.Bt[2][o] + a.B[2][o]), 0.5f * (a.Bt[2][o + 1] + a.B[2][o + 1]), 0.f, 0.f);
-      for (int i = b + l; i < e; i += 16) {
-        const int4 A = a.pa[i], Bv = a.pb[i];
+      // software pipeline: the next particle's records and slot are in
+      // flight during the current one (loads past the cell's end read its
+      // last particle, so that they are unconditional)
+      const int last = e - 1;
+      int i = b + l;
+      int4 A = a.pa[min(i, last)], Bv = a.pb[min(i, last)];
+      uint32_t j = a.fwd[min(i, last)];
+      for (; i < e; i += 16) {
+        const int inx = min(i + 16, last);
+        const int4 An = a.pa[inx], Bn = a.pb[inx];
+        const uint32_t jn = a.fwd[inx];
         const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
-        const uint32_t j = a.fwd[i];
         const float fin[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
         const float uin[3] = {as_f(Bv.x), as_f(Bv.y), as_f(Bv.z)};
         const float fbo[3] = {a.in.f[0][j], a.in.f[1][j], a.in.f[2][j]};
@@ -1033,6 +1041,9 @@ __global__ void __launch_bounds__(kBlock) k_cell_step_adj32(CellStepAdj32Args a)
         if (a.g0b) a.g0b[i] = g0b;
 #pragma unroll
         for (int q = 0; q < 18; q++) acc[q] += (double)cvb[q];
+        A = An;
+        Bv = Bn;
+        j = jn;
       }
     }
   }
@@ -1130,6 +1141,18 @@ void snap32_restore(emo_ctx *c, const Snap32 &s, int4 *pa, int4 *pb) {
   *v.g0_owned = false;
 }
 
+// FP64 -> FP32 copy of the three current adjoints: the particle VJP reads
+// them in FP32 (it rounded each value it read to FP32 before)
+__global__ void k_round_f32(const double *a, const double *b, const double *c, float *x, float *y, float *z,
+                            long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] = (float)a[i];
+    y[i] = (float)b[i];
+    z[i] = (float)c[i];
+  }
+}
+
 struct AdjRun32 {
   emo_ctx *c;
   hipStream_t as;
@@ -1141,6 +1164,7 @@ struct AdjRun32 {
   double *dsums = nullptr;
   PartAdj32 pa, pa2;
   float *g0b = nullptr;
+  float *Jb32[3] = {};
   std::vector<FillPlane> planes;
   int4 *own[4];  // the context's own record buffers (pa, pb, pa_alt, pb_alt)
 };
@@ -1191,6 +1215,9 @@ void backward32(AdjRun32 &R, int seg, int set) {
   for (int st = s1 - 1; st >= s0; st--) {
     const Snap32 &pre = R.tape[set][st - s0];
     fields_step_adj(c, as, R.af, R.planes, R.dsums + (size_t)st * nplanes, R.dt, pre.b_is_a0);
+    hipLaunchKernelGGL(k_round_f32, dim3(grid_for(g.ntot)), dim3(kBlock), 0, as, (const double *)R.af.Jb[0],
+                       (const double *)R.af.Jb[1], (const double *)R.af.Jb[2], R.Jb32[0], R.Jb32[1], R.Jb32[2],
+                       (long long)g.ntot);
     CellStepAdj32Args ca;
     ca.k = k;
     ca.pa = pre.pa;
@@ -1202,7 +1229,7 @@ void backward32(AdjRun32 &R, int seg, int set) {
       ca.E[d] = pre.E[d];
       ca.B[d] = pre.b_is_a0 ? pre.A[0][d] : pre.A[1][d];
       ca.Bt[d] = pre.b_is_a0 ? pre.A[1][d] : pre.A[0][d];
-      ca.Jb[d] = R.af.Jb[d];
+      ca.Jb[d] = R.Jb32[d];
     }
     ca.in = R.pa;
     ca.out = R.pa2;
@@ -1276,6 +1303,7 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
       P.take(&R.af.Ab[0][d], nt);
       P.take(&R.af.Ab[1][d], nt);
       P.take(&R.af.Jb[d], nt);
+      P.take(&R.Jb32[d], nt);
       P.take(&R.pa.f[d], npa);
       P.take(&R.pa.u[d], npa);
       P.take(&R.pa2.f[d], npa);

@@ -260,7 +260,7 @@ __device__ inline float dot3(const float a[3], const float b[3]) { return a[0] *
 template <bool REL, bool PER>
 __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w, const float fin[3],
                                         const float uin[3], float q_p, bool use_g0, float g0, float jc,
-                                        const double *const Jb[3], const float fbo[3], const float ubo[3],
+                                        const float *const Jb[3], const float fbo[3], const float ubo[3],
                                         float fb[3], float ub[3], float &g0b, float cvb[18]) {
   const Geo &g = a.g;
   int ic[3];
@@ -335,7 +335,8 @@ __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w,
   const bool in_grid = jc3[0] >= 0 && jc3[0] < g.nc[0] && jc3[1] >= 0 && jc3[1] < g.nc[1] && jc3[2] >= 0 &&
                        jc3[2] < g.nc[2];
   if (jc != 0.f && in_grid) {
-    long long idx[12];
+    // 32-bit point indices (a padded grid holds fewer than 2^31 points)
+    int idx[12];
     idx[0] = gidx(g, jc3[0], jc3[1], jc3[2]);
     idx[1] = gidx(g, jc3[0], jc3[1] + 1, jc3[2]);
     idx[2] = gidx(g, jc3[0], jc3[1] + 1, jc3[2] + 1);
@@ -352,8 +353,8 @@ __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w,
     const float Bo[3] = {fo[0] - 1.f, fo[1] - 1.f, fo[2] - 1.f};
     for (int q = 0; q < 3; q++) {
       const int j = (q + 1) % 3, k = (q + 2) % 3;
-      const float r0 = (float)Jb[q][idx[4 * q]], r1 = (float)Jb[q][idx[4 * q + 1]];
-      const float r2 = (float)Jb[q][idx[4 * q + 2]], r3 = (float)Jb[q][idx[4 * q + 3]];
+      const float r0 = Jb[q][idx[4 * q]], r1 = Jb[q][idx[4 * q + 1]];
+      const float r2 = Jb[q][idx[4 * q + 2]], r3 = Jb[q][idx[4 * q + 3]];
       const float vq = uo[q] * igi;
       // w = {A_j A_k, B_j A_k, B_j B_k, A_j B_k}
       vob[q] += jc * (Ao[j] * Ao[k] * r0 + Bo[j] * Ao[k] * r1 + Bo[j] * Bo[k] * r2 + Ao[j] * Bo[k] * r3);
