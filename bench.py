@@ -34,8 +34,8 @@ METRIC = "particle-steps/s (push+gather+deposit) FP32 vs FP64; adjoint/forward t
 
 def algorithmic_bytes(kernel: str, np_: int, ncell: int, relativity: int) -> float | None:
     """Bytes a kernel must move per launch at minimum (each operand read or
-    written once).  Per-particle and per-cell figures are documented in
-    DESIGN.md."""
+    written once): the per-particle and per-cell figures of the table below
+    times the particle and cell counts."""
     ng = ncell  # physical cells ~ padded points for the grid kernels
     table = {
         # FP64 plain form
