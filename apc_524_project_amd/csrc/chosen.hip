@@ -169,6 +169,7 @@ template <bool REL, bool PER>
 __global__ void __launch_bounds__(kBlock) k_push_rank_f32(PushRankArgs a) {
   __shared__ int cnt[kBlock / 64][kPushCells * kCodes];
   __shared__ int bnd[kBlock / 64][kPushCells + 1];
+  __shared__ float4 cvs[kBlock / 64][kPushCells * kCellVec];  // the wavefront's gather vectors
   __shared__ float qp_s[EMO_NSPEC];
   // species constants in LDS: a per-lane load from the argument block would
   // make every iteration wait for all loads in flight
@@ -181,6 +182,9 @@ __global__ void __launch_bounds__(kBlock) k_push_rank_f32(PushRankArgs a) {
   int *ct = cnt[wv];
   for (int e = lane; e < nc * kCodes; e += 64) ct[e] = 0;
   if (lane <= nc) bnd[wv][lane] = a.off[s0 + lane];
+  // every particle of this wavefront lies in its own nc source cells: their
+  // gather vectors are read once, into LDS
+  if (lane < nc * kCellVec) cvs[wv][lane] = a.k.cv[s0 * kCellVec + lane];
   wave_sync();
   const int pb = bnd[wv][0], pe = bnd[wv][nc];
   // the inner cell boundaries, wave-uniform (scalar registers); past the
@@ -204,7 +208,14 @@ __global__ void __launch_bounds__(kBlock) k_push_rank_f32(PushRankArgs a) {
   for (int p0 = pb; p0 < pe; p0 += 64) {
     const int p = p0 + lane;
     const bool valid = p < pe;
-    const CellVec V = load_cellvec(a.k, A.x);
+    // the source cell of the state held (lanes past the end hold the last
+    // particle)
+    const int pq = min(p, pe - 1);
+    int j = 0;
+#pragma unroll
+    for (int t = 0; t < kPushCells - 1; t++) j += sb[t] <= pq ? 1 : 0;
+    const float4 *cvl = cvs[wv] + j * kCellVec;
+    const CellVec V{cvl[0], cvl[1], cvl[2], cvl[3], cvl[4]};
     const int pn = min(p + 64, pe - 1);
     const int4 An = a.pa[pn], Bn = a.pb[pn];
     float Gn = 0.f;
@@ -220,9 +231,6 @@ __global__ void __launch_bounds__(kBlock) k_push_rank_f32(PushRankArgs a) {
     A = An;
     Bv = Bn;
     G = Gn;
-    int j = 0;
-#pragma unroll
-    for (int t = 0; t < kPushCells - 1; t++) j += sb[t] <= p ? 1 : 0;
     const bool ranked = valid && mc != kFarCode;
     const int myrank = wave_rank(j * kCodes + mc, ranked, ct, lane);
     if (valid) {
