@@ -446,8 +446,8 @@ __global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
     int4 A = a.pa[src], Bv = a.pb[src];
     float G = 0.f;
     if (use_g0) G = a.g0[src];
+    CellVec V = load_cellvec(a.k, A.x);
     for (; j < e; j += kDepLanes) {
-      const CellVec V = load_cellvec(a.k, A.x);
       const uint32_t src_next2 = a.inv[min(j + 2 * kDepLanes, last)];
       bad = bad || src_next >= np;
       const uint32_t sn = src_next < np ? src_next : 0u;
@@ -459,6 +459,8 @@ __global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
       push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
                          qp_s[sp], use_g0, G, o, nullptr);
       bad = bad || o.cell != (int)c;
+      // the next particle's gather vector: in flight during the deposit
+      V = load_cellvec(a.k, An.x);
       a.pa_out[j] = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)c), as_i(o.f[0]), as_i(o.f[1]),
                               as_i(o.f[2]));
       a.pb_out[j] = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
