@@ -275,10 +275,37 @@ struct InvArgs {
 };
 
 constexpr int kInvUnroll = 4;  // particles per lane in flight (2 and 8 measured the same)
+constexpr int kStayCode = 43;  // (1 * 6 + 1) * 6 + 1: no move in any dimension
 
+// New slot of particle p (move code k, rank r) of source cell src, or -1
+// (flagged) if the group tables do not hold it.
+__device__ inline long long inv_pos(const InvArgs &a, const int *shift, int src, int p, int k, int r) {
+  const Geo &g = a.g;
+  long long pos;
+  if (k != kFarCode) {
+    const int dst = src + shift[k];
+    pos = -1;
+    if (dst >= 0 && dst < g.ncell) {
+      pos = (long long)a.b.off_new[dst] + a.b.gbase[(long long)k * g.ncell + src] + r;
+      if (pos >= a.b.off_new[dst + 1]) pos = -1;
+    }
+    if (pos < 0) atomicOr(a.flag, kFlagMigrate);
+  } else {
+    pos = cellsort_pos(g, a.b, p, k, a.fdst[p], a.flag);
+  }
+  return pos;
+}
+
+// The particles that stay in their cell (most of them) take the cell's
+// stayer base plus their rank, from a per-wavefront table; the others are
+// collected in LDS and placed 64 at a time, one per lane, so that their
+// dependent table loads are not waited for once per particle batch.
 __global__ void __launch_bounds__(kBlock) k_inv_f32(InvArgs a) {
   __shared__ int bnd[kBlock / 64][kCellsPerWave + 1];
-  __shared__ int shift[kCodes];  // cell-id shift of each move code
+  __shared__ int stay0[kBlock / 64][kCellsPerWave];  // first new slot of each source cell's stayers
+  __shared__ int stay1[kBlock / 64][kCellsPerWave];  // end of that cell's new range
+  __shared__ int mov[kBlock / 64][128];              // movers waiting for their slot
+  __shared__ int shift[kCodes];                      // cell-id shift of each move code
   const Geo &g = a.g;
   for (int k = threadIdx.x; k < kCodes; k += kBlock) {
     const int kx = k / 36, ky = (k / 6) % 6, kz = k % 6;
@@ -290,6 +317,11 @@ __global__ void __launch_bounds__(kBlock) k_inv_f32(InvArgs a) {
   if (s0 >= g.ncell) return;
   const int nc = (int)min((long long)kCellsPerWave, g.ncell - s0);
   if (lane <= nc) bnd[wv][lane] = a.b.off[s0 + lane];
+  if (lane < nc) {
+    // (the base is stale for a cell without stayers; it is then not read)
+    stay0[wv][lane] = a.b.off_new[s0 + lane] + a.b.gbase[(long long)kStayCode * g.ncell + s0 + lane];
+    stay1[wv][lane] = a.b.off_new[s0 + lane + 1];
+  }
   wave_sync();
   const int pb = bnd[wv][0], pe = bnd[wv][nc];
   // the inner cell boundaries in scalar registers (past the last cell they
@@ -297,7 +329,24 @@ __global__ void __launch_bounds__(kBlock) k_inv_f32(InvArgs a) {
   int sb[kCellsPerWave - 1];
 #pragma unroll
   for (int t = 1; t < kCellsPerWave; t++) sb[t - 1] = __builtin_amdgcn_readfirstlane(t < nc ? bnd[wv][t] : INT_MAX);
-  for (int p0 = pb + lane; p0 < pe; p0 += 64 * kInvUnroll) {
+  auto cell_of = [&](int p) {
+    int j = 0;
+#pragma unroll
+    for (int t = 0; t < kCellsPerWave - 1; t++) j += sb[t] <= p ? 1 : 0;
+    return j;
+  };
+  // the first n (<= 64) waiting movers, one per lane
+  auto place_movers = [&](int n) {
+    if (lane < n) {
+      const int p = mov[wv][lane];
+      const long long pos = inv_pos(a, shift, (int)s0 + cell_of(p), p, (int)a.b.code[p], (int)a.b.rank[p]);
+      if (pos >= 0) a.inv[pos] = (uint32_t)p;
+    }
+  };
+  int nmov = 0;  // movers waiting in mov[wv] (wavefront-uniform, below 64 between batches)
+  // a wavefront-uniform loop: every lane takes part in the batches
+  for (int q0 = pb; q0 < pe; q0 += 64 * kInvUnroll) {
+    const int p0 = q0 + lane;
     int k[kInvUnroll], r[kInvUnroll];
 #pragma unroll
     for (int u = 0; u < kInvUnroll; u++) {
@@ -308,26 +357,31 @@ __global__ void __launch_bounds__(kBlock) k_inv_f32(InvArgs a) {
 #pragma unroll
     for (int u = 0; u < kInvUnroll; u++) {
       const int p = p0 + 64 * u;
-      if (k[u] < 0) continue;
-      int j = 0;
-#pragma unroll
-      for (int t = 0; t < kCellsPerWave - 1; t++) j += sb[t] <= p ? 1 : 0;
-      const int src = (int)s0 + j;
-      long long pos;
-      if (k[u] != kFarCode) {
-        const int dst = src + shift[k[u]];
-        pos = -1;
-        if (dst >= 0 && dst < g.ncell) {
-          pos = (long long)a.b.off_new[dst] + a.b.gbase[(long long)k[u] * g.ncell + src] + r[u];
-          if (pos >= a.b.off_new[dst + 1]) pos = -1;
-        }
-        if (pos < 0) atomicOr(a.flag, kFlagMigrate);
-      } else {
-        pos = cellsort_pos(g, a.b, p, k[u], a.fdst[p], a.flag);
+      if (k[u] == kStayCode) {
+        const int j = cell_of(p);
+        const long long pos = (long long)stay0[wv][j] + r[u];
+        if (pos < stay1[wv][j])
+          a.inv[pos] = (uint32_t)p;
+        else
+          atomicOr(a.flag, kFlagMigrate);
       }
-      if (pos >= 0) a.inv[pos] = (uint32_t)p;
+      const bool mover = k[u] >= 0 && k[u] != kStayCode;
+      const unsigned long long bal = __ballot(mover);
+      if (mover) mov[wv][nmov + __popcll(bal & lanes_below(lane))] = p;
+      nmov += __popcll(bal);
+      if (nmov >= 64) {
+        wave_sync();
+        place_movers(64);
+        wave_sync();
+        // the remainder (fewer than 64) moves to the front
+        if (lane < nmov - 64) mov[wv][lane] = mov[wv][64 + lane];
+        nmov -= 64;
+        wave_sync();
+      }
     }
   }
+  wave_sync();
+  place_movers(nmov);
 }
 
 // ---------------------------------------------------------------------------
