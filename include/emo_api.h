@@ -172,19 +172,21 @@ int emo_step(emo_ctx *ctx, int32_t nsteps, double t0, double dt);
  * out8 = {pMx, pMy, pMz, pEn, B_en, E_en, E_tot, total charge} */
 int emo_diag(emo_ctx *ctx, double *out8);
 
-/* ---- adjoint (FP64 plain form) -------------------------------------------
+/* ---- adjoint (both forms) ------------------------------------------------
  * No counterpart in the reference (it has no derivatives); for callers that
  * differentiate the step, e.g. the torch.autograd.Function in
  * apc_524_project_amd/adjoint.py.
  *
- * Sorts the particles by cell (as SortParticles before the reference loop's
- * first push, src/main.cpp:225-226), runs nsteps steps of emo_step from time
- * t0, then the exact discrete adjoint of those steps backward in time,
- * re-running each K-step segment from a checkpoint.  The objective is the
- * final field energy of OutputBoxQuantities, F = E_en + B_en
- * (emo_diag out8[5] + out8[4]).  The stored interpolated fields must be
- * current (emo_interpolate after the field initialisation).  Outputs (any
- * may be NULL):
+ * Runs nsteps steps of emo_step from time t0, then the exact discrete
+ * adjoint of those steps backward in time, re-running each K-step segment
+ * from a checkpoint.  The objective is the final field energy of
+ * OutputBoxQuantities, F = E_en + B_en (emo_diag out8[5] + out8[4]).
+ * FP64 plain form: the particles are sorted by cell first (as SortParticles
+ * before the reference loop's first push, src/main.cpp:225-226), and the
+ * stored interpolated fields must be current (emo_interpolate after the
+ * field initialisation).  FP32 chosen form: the adjoint of the FP32 step
+ * itself (FP32 particle adjoints, FP64 field adjoints).  Outputs (any may be
+ * NULL):
  *   energy        F
  *   d_amp[nw]     dF/d peakamps[w]
  *   d_phase[nw]   dF/d phases[w], per radian
