@@ -188,3 +188,36 @@ def test_fp32_autograd_function(base):
     assert F.item() == a32["energy"]
     F.backward()
     assert amps.grad[0].item() == a32["d_amp"][0]
+
+
+def test_fp32_adjoint_of_zero_steps():
+    """nsteps = 0: the objective is the current field energy and every
+    gradient is zero."""
+    case = case_with()
+    ctx = setup(case, FP32_CHOSEN)
+    d = ctx.diag()
+    out = ctx.adjoint_field_energy(0, 0.0, case.dt_cfl(), 4)
+    assert out["energy"] == d[4] + d[5]
+    assert out["d_amp"][0] == 0 and out["d_phase"][0] == 0
+    for key in ("xbar0", "vbar0", "gbar0"):
+        assert not np.any(out[key]), key
+    ctx.close()
+
+
+def test_fp32_adjoint_calls_in_a_row_continue_the_run():
+    """Two gradient calls of 8 steps on one context (the second reuses the
+    first's device pool and starts from its final state) reach the state
+    and energy of 16 plain steps."""
+    case = case_with()
+    dt = case.dt_cfl()
+    a = setup(case, FP32_CHOSEN)
+    a.adjoint_field_energy(8, 0.0, dt, 3)
+    second = a.adjoint_field_energy(8, 8 * dt, dt, 3)
+    b = setup(case, FP32_CHOSEN)
+    b.step(16, 0.0, dt)
+    d = b.diag()
+    assert second["energy"] == d[4] + d[5]
+    assert np.array_equal(a.particles(fields=False)["x"], b.particles(fields=False)["x"])
+    assert second["d_amp"][0] != 0
+    a.close()
+    b.close()
