@@ -75,6 +75,9 @@ struct ChosenState {
   float *g0 = nullptr;  // stored gamma for the first relativistic push (Q3)
   bool g0_active = false;
   bool g0_owned = true;  // g0 is this state's allocation (not a snapshot's)
+  // the step deposits J only; rho (an output, not read by the step) is
+  // deposited from the current particles when it is read
+  bool rho_stale = false;
   CellSortBufs srt;  // the counting sort's buffers (cellsort.hpp); srt.perm is the inverse permutation
   int32_t *fdst = nullptr;  // destination cell of a far mover
   // initial sort of loaded particles (arbitrary order)
@@ -1202,9 +1205,56 @@ static float *f32_field(ChosenState *s, int fid) {
   }
 }
 
+// rho of the current particles (Depositor::deposit_particle_Rho and the fold,
+// as the reference's depositRhoJ + executeBC(-2) leave it after a step);
+// the J the step deposited is left as it is
+int fp32_deposit_rho(emo_ctx *c) {
+  ChosenState *s = c->ch;
+  const Geo &g = c->geo;
+  float *cj = nullptr, *cr = nullptr, *jt = nullptr;
+  int rc = dalloc(c, &cj, 12 * g.ncell);
+  if (!rc) rc = dalloc(c, &cr, 8 * g.ncell);
+  if (!rc) rc = dalloc(c, &jt, 3 * g.ntot);
+  if (!rc) {
+    RhoJArgs a;
+    a.pa = s->pa;
+    a.pb = s->pb;
+    a.g0 = s->g0_active ? s->g0 : nullptr;
+    a.off = s->srt.off;
+    a.s = species_f32(c, 0.0);
+    a.rel = c->cfg.relativity;
+    a.ncell = g.ncell;
+    a.cj = cj;
+    a.cr = cr;
+    const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
+    hipLaunchKernelGGL(k_deposit_rhoj_f32, dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    DepGridArgs<float> b;
+    b.g = g;
+    b.cj = cj;
+    b.cr = cr;
+    for (int d = 0; d < 3; d++) b.J[d] = jt + d * g.ntot;  // discarded
+    b.rho = s->fs.rho;
+    b.do_rho = 1;
+    hipLaunchKernelGGL(k_deposit_grid<float>, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, b);
+    FieldSet<float> fr = s->fs;
+    for (int d = 0; d < 3; d++) fr.J[d] = nullptr;
+    fk_fold<float>(c, fr);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, EMO_ERR_HIP, "rho deposit failed");
+  }
+  dfree(cj);
+  dfree(cr);
+  dfree(jt);
+  if (!rc) s->rho_stale = false;
+  return rc;
+}
+
 int fp32_get_field(emo_ctx *c, int fid, double *out) {
   float *p = f32_field(c->ch, fid);
   if (!p) return set_err(c, EMO_ERR_CONFIG, "unknown field id");
+  if (fid == EMO_RHO && c->ch->rho_stale) {
+    int rc = fp32_deposit_rho(c);
+    if (rc) return rc;
+  }
   std::vector<float> h(c->geo.ntot);
   EMO_HIP_CHECK_RET(c, hipMemcpyAsync(h.data(), p, c->geo.ntot * 4, hipMemcpyDeviceToHost, c->stream));
   EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
@@ -1215,6 +1265,7 @@ int fp32_get_field(emo_ctx *c, int fid, double *out) {
 int fp32_set_field(emo_ctx *c, int fid, const double *in) {
   float *p = f32_field(c->ch, fid);
   if (!p) return set_err(c, EMO_ERR_CONFIG, "unknown field id");
+  if (fid == EMO_RHO) c->ch->rho_stale = false;
   std::vector<float> h(c->geo.ntot);
   for (long long i = 0; i < c->geo.ntot; i++) h[i] = (float)in[i];
   EMO_HIP_CHECK_RET(c, hipMemcpy(p, h.data(), c->geo.ntot * 4, hipMemcpyHostToDevice));
@@ -1231,6 +1282,7 @@ int fp32_init_constant(emo_ctx *c) {
     fk_const_fill<float>(c, s->fs.J[q], 0.0, nt);
   }
   fk_const_fill<float>(c, s->fs.rho, 0.0, nt);
+  s->rho_stale = false;
   EMO_HIP_CHECK_RET(c, hipGetLastError());
   return check_flag(c);
 }
@@ -1301,6 +1353,7 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
     hipLaunchKernelGGL(k_deposit_grid<float>, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, b);
   }
   fk_fold<float>(c, s->fs);
+  s->rho_stale = false;
   for (int q = 0; q < 3; q++) {
     fk_const_fill<float>(c, s->fs.E[q], 0.0, (long long)nt);
     fk_const_fill<float>(c, s->fs.B[q], 0.0, (long long)nt);
@@ -1408,7 +1461,12 @@ int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out) {
     s->g0 = nullptr;
     s->g0_owned = true;
   }
-  fk_fold<float>(c, s->fs);
+  {
+    FieldSet<float> fj = s->fs;
+    fj.rho = nullptr;  // not deposited by the step (rho_stale)
+    fk_fold<float>(c, fj);
+  }
+  s->rho_stale = true;
   fk_evolve<float>(c, s->fs, dt);
   refresh_fields<float>(c, s->fs, t, dt);
   return EMO_OK;

@@ -10,7 +10,7 @@ import pytest
 
 from apc_524_project_amd._lib import FP32_CHOSEN, Context
 from emo_case import emo_config
-from oracle_ctypes import (BX, BY, BZ, EX, EXTERNAL, EY, EZ, JX, JY, JZ, PERIODIC, REFLECTING, Case,
+from oracle_ctypes import (BX, BY, BZ, EX, EXTERNAL, EY, EZ, JX, JY, JZ, PERIODIC, REFLECTING, RHO, Case,
                            Oracle, Wave, cell_ids_from_x)
 
 pytestmark = pytest.mark.gpu
@@ -68,6 +68,12 @@ def test_fp32_tracks_fp64_oracle(relativity):
     assert same > 0.999
     for f in (EX, EY, EZ, BX, BY, BZ, JX, JY, JZ):
         assert rel(ctx.field(f), o.field(f)) < 1e-3, f
+    # rho: the step deposits J only; reading rho deposits it from the
+    # current particles, as the reference's step leaves it
+    assert rel(ctx.field(RHO), o.field(RHO)) < 1e-3
+    ctx.step(1, 20 * dt, dt)
+    o.step(20 * dt, dt)
+    assert rel(ctx.field(RHO), o.field(RHO)) < 1e-3
 
 
 def test_fp32_lightbc_reflecting_tracks_oracle():
