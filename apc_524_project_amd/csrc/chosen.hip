@@ -78,6 +78,10 @@ struct ChosenState {
   // the step deposits J only; rho (an output, not read by the step) is
   // deposited from the current particles when it is read
   bool rho_stale = false;
+  // non-relativistic forms: the loaded gamma by particle id.  No Boris step
+  // reads or updates it (the reference keeps the loader's value, Q3); it is
+  // only handed back.  Kept in id order, so the sort never moves it.
+  float *gam_id = nullptr;
   CellSortBufs srt;  // the counting sort's buffers (cellsort.hpp); srt.perm is the inverse permutation
   int32_t *fdst = nullptr;  // destination cell of a far mover
   // initial sort of loaded particles (arbitrary order)
@@ -623,6 +627,7 @@ struct GenF32Args {
   int rel;
   int4 *pa, *pb;
   float *g0;
+  float *gam_id;  // non-relativistic: the loaded gamma by id
   long long np;
   unsigned long long *flag;
 };
@@ -653,6 +658,7 @@ __global__ void __launch_bounds__(kBlock) k_uniform_f32(GenF32Args a) {
                       fb[0], fb[1], fb[2]);
   a.pb[i] = make_int4(as_i((float)(gm * v[0])), as_i((float)(gm * v[1])), as_i((float)(gm * v[2])), (int)i);
   if (a.rel) a.g0[i] = (float)gm;
+  if (!a.rel) a.gam_id[i] = (float)gam;  // the id of load index i is i
 }
 
 // ---------------------------------------------------------------------------
@@ -705,6 +711,7 @@ struct OutArgs {
   Geo g;
   const int4 *pa, *pb;
   const float *g0;
+  const float *gam_id;  // non-relativistic: the loaded gamma by id, or null
   int rel;
   double *x[3], *v[3], *gam;
   int32_t *sp;
@@ -728,6 +735,7 @@ __global__ void __launch_bounds__(kBlock) k_output_f32(OutArgs a) {
     if (a.x[d]) a.x[d][i] = g.x0[d] + g.d[d] * ((double)(ic[d] - 1) + ff[d]);
     if (a.v[d]) a.v[d][i] = a.rel ? uu[d] / gam : uu[d];
   }
+  if (!a.rel && a.gam_id && Bv.w >= 0 && Bv.w < a.np) gam = a.gam_id[Bv.w];
   if (a.gam) a.gam[i] = gam;
   if (a.sp) a.sp[i] = (int)((uint32_t)A.x >> kSpeciesShift);
   if (a.id) a.id[i] = Bv.w;
@@ -1045,6 +1053,7 @@ void fp32_destroy(emo_ctx *c) {
   }
   dfree(s->fs.rho);
   dfree(s->g0);
+  dfree(s->gam_id);
   cellsort_free(s->srt);
   dfree(s->key);
   dfree(s->key_alt);
@@ -1107,11 +1116,27 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
   dfree(s->g0);
   s->g0 = nullptr;
   s->g0_active = false;
+  dfree(s->gam_id);
+  s->gam_id = nullptr;
   if (rel && n > 0) {
     int rc = dalloc(c, &s->g0, n);
     if (rc) return rc;
     EMO_HIP_CHECK_RET(c, hipMemcpy(s->g0, g0.data(), n * 4, hipMemcpyHostToDevice));
     s->g0_active = true;
+  }
+  if (!rel && gamma && id && n > 0) {
+    // by id, when the ids are 0 .. n-1
+    std::vector<float> gi(n, 1.f);
+    bool ok = true;
+    for (int64_t i = 0; i < n && ok; i++) {
+      ok = id[i] >= 0 && id[i] < n;
+      if (ok) gi[id[i]] = (float)gamma[i];
+    }
+    if (ok) {
+      int rc = dalloc(c, &s->gam_id, n);
+      if (rc) return rc;
+      EMO_HIP_CHECK_RET(c, hipMemcpy(s->gam_id, gi.data(), n * 4, hipMemcpyHostToDevice));
+    }
   }
   c->np = n;
   int rc = sort_loaded(c);
@@ -1144,6 +1169,7 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
       a.v[d] = dv[d];
     }
     a.g0 = s->g0_active ? s->g0 : nullptr;
+    a.gam_id = s->gam_id;
     a.rel = c->cfg.relativity;
     a.gam = dg;
     a.sp = dsp;
@@ -1293,10 +1319,16 @@ int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n) {
   dfree(s->g0);
   s->g0 = nullptr;
   s->g0_active = false;
+  dfree(s->gam_id);
+  s->gam_id = nullptr;
   if (rel && n > 0) {
     int rc = dalloc(c, &s->g0, n);
     if (rc) return rc;
     s->g0_active = true;
+  }
+  if (!rel && n > 0) {
+    int rc = dalloc(c, &s->gam_id, n);
+    if (rc) return rc;
   }
   if (n > 0) {
     GenF32Args a;
@@ -1305,6 +1337,7 @@ int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n) {
     a.pa = s->pa;
     a.pb = s->pb;
     a.g0 = s->g0;
+    a.gam_id = s->gam_id;
     a.np = n;
     a.flag = c->dflag;
     hipLaunchKernelGGL(k_uniform_f32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);

@@ -76,6 +76,23 @@ def test_fp32_tracks_fp64_oracle(relativity):
     assert rel(ctx.field(RHO), o.field(RHO)) < 1e-3
 
 
+def test_fp32_hands_back_gamma_as_the_oracle_holds_it():
+    """gamma: the stored value of the first relativistic push, then
+    sqrt(1 + u^2); a Boris run keeps the loader's value (Q3), in both the
+    reference and this form."""
+    for relativity in (0, 1):
+        case = Case(**THERMAL, relativity=relativity)
+        ctx, o = setup(case)
+        g0 = by_id(ctx)["gamma"]
+        assert np.allclose(g0, o.particles().by_id().gamma, rtol=1e-7, atol=0)
+        run(ctx, o, 3, case.dt_cfl())
+        g = by_id(ctx)["gamma"]
+        go = o.particles().by_id().gamma
+        assert np.allclose(g, go, rtol=1e-6, atol=0), relativity
+        if not relativity:
+            assert np.array_equal(g, g0)
+
+
 def test_fp32_lightbc_reflecting_tracks_oracle():
     w = Wave(side=-1, pol=2, amp=0.3, omega=0.39, phase=30.0, inv_width=0.05, delay=5.0)
     case = Case(n=(12, 6, 6), L=(12.0, 6.0, 6.0), nparticles=4608, temp=(5000.0, 5000.0),
