@@ -712,8 +712,10 @@ struct OutArgs {
   const int4 *pa, *pb;
   const float *g0;
   const float *gam_id;  // non-relativistic: the loaded gamma by id, or null
+  const float4 *cv;     // with fld: the current grid's gather vectors
   int rel;
   double *x[3], *v[3], *gam;
+  double *fld[6];       // the gathered E, B per particle (Field_part), or null
   int32_t *sp;
   int32_t *id;
   long long np;
@@ -737,6 +739,16 @@ __global__ void __launch_bounds__(kBlock) k_output_f32(OutArgs a) {
   }
   if (!a.rel && a.gam_id && Bv.w >= 0 && Bv.w < a.np) gam = a.gam_id[Bv.w];
   if (a.gam) a.gam[i] = gam;
+  if (a.fld[0]) {
+    const float4 *cv = a.cv + (long long)cell * kCellVec;
+    const CellVec V{cv[0], cv[1], cv[2], cv[3], cv[4]};
+    float e[3], b[3];
+    gather_cell(V, as_f(A.y), as_f(A.z), as_f(A.w), e, b);
+    for (int d = 0; d < 3; d++) {
+      a.fld[d][i] = e[d];
+      a.fld[3 + d][i] = b[d];
+    }
+  }
   if (a.sp) a.sp[i] = (int)((uint32_t)A.x >> kSpeciesShift);
   if (a.id) a.id[i] = Bv.w;
 }
@@ -1145,16 +1157,31 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
   return check_flag(c);
 }
 
-int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species, int64_t *id) {
+int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species, int64_t *id,
+                       double *field6) {
   ChosenState *s = c->ch;
   int64_t n = c->np;
   if (n == 0) return EMO_OK;
-  double *dx[3] = {}, *dv[3] = {}, *dg = nullptr;
+  double *dx[3] = {}, *dv[3] = {}, *dg = nullptr, *df[6] = {};
   int32_t *dsp = nullptr, *did = nullptr;
   int rc = 0;
   for (int d = 0; d < 3 && !rc; d++) {
     rc = dalloc(c, &dx[d], n);
     if (!rc) rc = dalloc(c, &dv[d], n);
+  }
+  for (int d = 0; d < 6 && !rc && field6; d++) rc = dalloc(c, &df[d], n);
+  if (!rc && field6) {
+    // the gather of the current grid (Particle_Handler::InterpolateEB, which
+    // ends the reference's step): the gather vectors of this grid first
+    CellVecArgs ca;
+    ca.g = c->geo;
+    for (int d = 0; d < 3; d++) {
+      ca.E[d] = s->fs.E[d];
+      ca.B[d] = s->fs.B[d];
+      ca.Bt[d] = s->fs.Bt[d];
+    }
+    ca.cv = s->cv;
+    hipLaunchKernelGGL(k_cellvec_f32, dim3(grid_for(c->geo.ncell)), dim3(kBlock), 0, c->stream, ca);
   }
   if (!rc) rc = dalloc(c, &dg, n);
   if (!rc) rc = dalloc(c, &dsp, n);
@@ -1170,6 +1197,8 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
     }
     a.g0 = s->g0_active ? s->g0 : nullptr;
     a.gam_id = s->gam_id;
+    a.cv = s->cv;
+    for (int d = 0; d < 6; d++) a.fld[d] = df[d];
     a.rel = c->cfg.relativity;
     a.gam = dg;
     a.sp = dsp;
@@ -1188,6 +1217,7 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
       if (v3 && !rc) rc = down(v3, dv[d], 3, d);
     }
     if (gamma && !rc) rc = down(gamma, dg, 1, 0);
+    for (int d = 0; d < 6 && field6 && !rc; d++) rc = down(field6, df[d], 6, d);
     if (species && !rc && hipMemcpy(species, dsp, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
     if (id && !rc) {
       std::vector<int32_t> h(n);
@@ -1200,6 +1230,7 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
     dfree(dv[d]);
   }
   dfree(dg);
+  for (int d = 0; d < 6; d++) dfree(df[d]);
   dfree(dsp);
   dfree(did);
   return rc;

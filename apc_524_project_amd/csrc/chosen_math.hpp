@@ -145,6 +145,22 @@ __device__ inline CellVec load_cellvec(const PushK &a, int w) {
   return CellVec{cv[0], cv[1], cv[2], cv[3], cv[4]};
 }
 
+// The gather (Interpolator::interpolate_fields, src/particles/interpolate.cpp:
+// 11-47) from a cell's gather vector at offsets f: E bilinear over the 4
+// edges, B linear over 2 faces, with the weights 2 - f and f - 1 (Q22).
+__device__ __forceinline__ void gather_cell(const CellVec &V, float fx, float fy, float fz, float e[3], float b[3]) {
+  const float ax = 2.f - fx, bx = fx - 1.f;
+  const float ay = 2.f - fy, by = fy - 1.f;
+  const float az = 2.f - fz, bz = fz - 1.f;
+  const float4 vEx = V.Ex, vEy = V.Ey, vEz = V.Ez, vB = V.B, vBz = V.Bz;
+  e[0] = fmaf(az, fmaf(ay, vEx.x, by * vEx.y), bz * fmaf(by, vEx.z, ay * vEx.w));
+  e[1] = fmaf(ax, fmaf(az, vEy.x, bz * vEy.y), bx * fmaf(bz, vEy.z, az * vEy.w));
+  e[2] = fmaf(ay, fmaf(ax, vEz.x, bx * vEz.y), by * fmaf(bx, vEz.z, ax * vEz.w));
+  b[0] = fmaf(ax, vB.x, bx * vB.y);
+  b[1] = fmaf(ay, vB.z, by * vB.w);
+  b[2] = fmaf(az, vBz.x, bz * vBz.y);
+}
+
 // q_p: ((0.5 dt q)/m) UNIT_ACC of the particle's species; g0: the stored
 // gamma for the first relativistic push (Q3), if use_g0
 template <bool REL, bool PER>
@@ -154,19 +170,10 @@ __device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, 
   const Geo &g = a.g;
   int ix, iy, iz;
   decode_fast(a, w & kCellMask, ix, iy, iz);
-  // gather (Interpolator::interpolate_fields, src/particles/interpolate.cpp:11-47)
-  // from the cell's gather vector: E bilinear over the 4 edges, B linear
-  // over 2 faces
-  const float ax = 2.f - fx, bx = fx - 1.f;
-  const float ay = 2.f - fy, by = fy - 1.f;
-  const float az = 2.f - fz, bz = fz - 1.f;
-  const float4 vEx = V.Ex, vEy = V.Ey, vEz = V.Ez, vB = V.B, vBz = V.Bz;
-  const float ex = fmaf(az, fmaf(ay, vEx.x, by * vEx.y), bz * fmaf(by, vEx.z, ay * vEx.w));
-  const float ey = fmaf(ax, fmaf(az, vEy.x, bz * vEy.y), bx * fmaf(bz, vEy.z, az * vEy.w));
-  const float ez = fmaf(ay, fmaf(ax, vEz.x, bx * vEz.y), by * fmaf(bx, vEz.z, ax * vEz.w));
-  const float bxf = fmaf(ax, vB.x, bx * vB.y);
-  const float byf = fmaf(ay, vB.z, by * vB.w);
-  const float bzf = fmaf(az, vBz.x, bz * vBz.y);
+  float eg[3], bg[3];
+  gather_cell(V, fx, fy, fz, eg, bg);
+  const float ex = eg[0], ey = eg[1], ez = eg[2];
+  const float bxf = bg[0], byf = bg[1], bzf = bg[2];
   float ig_new = 1.f;
   if (REL) {
     // Relativistic_Boris::Step (src/pusher/relativisticBoris.cpp:28-140) on

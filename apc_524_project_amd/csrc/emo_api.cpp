@@ -378,7 +378,7 @@ int emo_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t
                       double *field6) {
   if (!c) return EMO_ERR_CONFIG;
   if (c->cfg.precision != EMO_FP64_PLAIN) {
-    return fp32_get_particles(c, x3, v3, gamma, species, id);
+    return fp32_get_particles(c, x3, v3, gamma, species, id, field6);
   }
   int64_t n = c->np;
   std::vector<double> buf(std::max<int64_t>(n, 1));

@@ -248,8 +248,8 @@ int fp32_create(emo_ctx *c);
 void fp32_destroy(emo_ctx *c);
 int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const double *gamma,
               const int32_t *species, const int64_t *id);
-int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species,
-                       int64_t *id);
+int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species, int64_t *id,
+                       double *field6);
 int fp32_get_cells(emo_ctx *c, int32_t *cells);
 int fp32_get_field(emo_ctx *c, int fid, double *out);
 int fp32_set_field(emo_ctx *c, int fid, const double *in);

@@ -68,6 +68,11 @@ def test_fp32_tracks_fp64_oracle(relativity):
     assert same > 0.999
     for f in (EX, EY, EZ, BX, BY, BZ, JX, JY, JZ):
         assert rel(ctx.field(f), o.field(f)) < 1e-3, f
+    # the particles' gathered fields (Field_part, the gather that ends the
+    # reference's step), gathered from the current grid when read
+    pf = ctx.particles(fields=True)
+    fo = pf["field"][np.argsort(pf["id"])]
+    assert rel(fo, p.field) < 1e-3
     # rho: the step deposits J only; reading rho deposits it from the
     # current particles, as the reference's step leaves it
     assert rel(ctx.field(RHO), o.field(RHO)) < 1e-3
