@@ -238,7 +238,7 @@ class Context:
         self._chk(self.L.emo_init_poisson(self.h, iter_max, tol, _p(sw), _p(res)))
         return sw, res
 
-    # phases (FP64 plain form)
+    # phases (FP64 plain form; evolve and refresh in both forms)
     def sort(self):
         self._chk(self.L.emo_sort(self.h))
 

@@ -532,8 +532,10 @@ int emo_fold(emo_ctx *c) {
 
 int emo_evolve_fields(emo_ctx *c, double dt) {
   if (!c) return EMO_ERR_CONFIG;
-  int rc = require_plain(c, "emo_evolve_fields");
-  if (rc) return rc;
+  if (c->cfg.precision != EMO_FP64_PLAIN) {  // field-only phase: both forms
+    fk_evolve<float>(c, *fp32_view(c).fs, dt);
+    return last_launch(c);
+  }
   fld_invalidate(c);
   launch_evolve_f64(c, dt);
   return last_launch(c);
@@ -541,8 +543,10 @@ int emo_evolve_fields(emo_ctx *c, double dt) {
 
 int emo_refresh(emo_ctx *c, double t, double dt) {
   if (!c) return EMO_ERR_CONFIG;
-  int rc = require_plain(c, "emo_refresh");
-  if (rc) return rc;
+  if (c->cfg.precision != EMO_FP64_PLAIN) {  // field-only phase: both forms
+    refresh_fields<float>(c, *fp32_view(c).fs, t, dt);
+    return last_launch(c);
+  }
   fld_invalidate(c);
   FieldSet<double> v = f64_view(c);
   refresh_fields<double>(c, v, t, dt);

@@ -153,7 +153,9 @@ int emo_init_constant(emo_ctx *ctx);
 int emo_init_poisson(emo_ctx *ctx, int32_t iter_max, double tol, int32_t *sweeps_out,
                      double *resid_out);
 
-/* ---- the step's phases (FP64 plain form; EMO_ERR_UNSUPPORTED in FP32) --- */
+/* ---- the step's phases (FP64 plain form; in FP32 the particle phases and
+ * the fold return EMO_ERR_UNSUPPORTED, the field-only phases emo_evolve_fields
+ * and emo_refresh run on the FP32 fields) ------------------------------- */
 int emo_sort(emo_ctx *ctx);                  /* Particle_Handler::SortParticles, particle_handler.cpp:180-183 */
 int emo_push(emo_ctx *ctx, double dt);       /* Particle_Handler::Push, particle_handler.cpp:124-128 */
 int emo_particle_bc(emo_ctx *ctx);           /* executeParticleBoundaryConditions + clearGhosts, :271-295 */

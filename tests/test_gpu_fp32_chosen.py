@@ -136,6 +136,31 @@ def test_fp32_diag_close_to_oracle():
     np.testing.assert_allclose(a[4:7], b[4:7], rtol=2e-3)
 
 
+def test_fp32_field_only_phases_track_fp64():
+    """emo_evolve_fields and emo_refresh on the FP32 fields: a LightBC pulse
+    into vacuum, 40 Yee steps driven phase by phase, against the same calls
+    on the FP64 form (bit-exact with the oracle)."""
+    w = Wave(side=-1, pol=2, amp=0.3, omega=0.39, phase=30.0, inv_width=0.05, delay=5.0)
+    case = Case(n=(16, 4, 4), L=(16.0, 4.0, 4.0), nparticles=0, temp=(0.0, 0.0), dens_phys=0.0,
+                relativity=1, waves=[w],
+                fbc=(EXTERNAL, EXTERNAL, PERIODIC, PERIODIC, PERIODIC, PERIODIC),
+                pbc=(REFLECTING, REFLECTING, PERIODIC, PERIODIC, PERIODIC, PERIODIC))
+    from apc_524_project_amd._lib import FP64_PLAIN
+    ctxs = [Context(emo_config(case, precision=prec, max_particles=1)) for prec in (FP64_PLAIN, FP32_CHOSEN)]
+    dt = case.dt_cfl()
+    for c in ctxs:
+        c.init_constant()
+        t = 0.0
+        for _ in range(40):
+            c.evolve(dt)
+            c.refresh(t, dt)
+            t += dt
+    for f in (EY, BZ):
+        ref = ctxs[0].field(f)
+        assert np.linalg.norm(ref) > 0
+        assert rel(ctxs[1].field(f), ref) < 1e-5, f
+
+
 def test_fp32_rejects_fp64_only_phases():
     case = Case(**THERMAL)
     ctx, _ = setup(case)
