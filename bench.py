@@ -590,7 +590,7 @@ def cpu_baseline(sample_cells: int = 64, sample_steps: int = 10):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=1000, help="timed steps (1000: the c2 configuration's step count)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", default="auto", choices=["auto", "fp64", "fp32", "both"])
     ap.add_argument("--scale", type=float, default=1.0, help="particle-count scale (1 = full c2)")
