@@ -221,3 +221,16 @@ def test_fp32_adjoint_calls_in_a_row_continue_the_run():
     assert second["d_amp"][0] != 0
     a.close()
     b.close()
+
+
+def test_fp32_boris_lightbc_gradients_match_the_fp64_adjoint():
+    """The non-relativistic (Boris) particle adjoint with reflecting walls."""
+    case = case_with(relativity=0)
+    dt = case.dt_cfl()
+    a32 = adjoint(case, dt, 4, FP32_CHOSEN)
+    a64 = adjoint(case, dt, 4, FP64_PLAIN)
+    assert a64["d_amp"][0] != 0
+    assert rel_err(a32["d_amp"][0], a64["d_amp"][0]) < 1e-4
+    assert rel_err(a32["d_phase"][0], a64["d_phase"][0]) < 1e-4
+    for key in ("xbar0", "vbar0"):
+        assert rel_l2(a32[key], a64[key]) < 1e-3, (key, rel_l2(a32[key], a64[key]))
