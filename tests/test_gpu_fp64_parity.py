@@ -338,3 +338,30 @@ def test_fused_step_after_phase_calls_and_set_field():
     o.deposit()
     assert_particles_equal(ctx, o)
     assert_fields_equal(ctx, o)
+
+
+def test_c1_at_its_full_size_bit_exact():
+    """The first benchmark configuration at its full size -- 16^3 periodic,
+    65,536 particles (8 per cell and species, seed -1), Boris, the Poisson
+    initialisation with 50 sweeps, 100 steps of dt = 0.5 CFL -- bit-exact
+    against the oracle (fields, particles); the diagnostics' sums within
+    1e-12 (a tree reduction on the GPU, a serial sum in the oracle)."""
+    case = Case(n=(16, 16, 16), L=(16.0, 16.0, 16.0), nparticles=2 * 8 * 4096, temp=(501.8, 501.8),
+                dens_phys=2.824e9, poisson=True, seed=-1, zero_avg=-1)
+    o = Oracle(case)
+    assert o.load() == 0
+    ctx = Context(emo_config(case, q=o.q, m=o.m))
+    p = o.particles()
+    ctx.load_particles(p.x, p.v, p.gamma, p.species, p.id)
+    sw, _ = ctx.init_poisson(iter_max=49, tol=0.0)
+    osw = o.initialize(iter_max=49, tol=0.0)
+    assert list(sw) == list(osw) == [50, 50, 50, 50]
+    dt = case.dt_cfl()
+    t = 0.0
+    for _ in range(100):
+        o.step(t, dt)
+        t += dt
+    ctx.step(100, 0.0, dt)
+    assert_particles_equal(ctx, o)
+    assert_fields_equal(ctx, o)
+    np.testing.assert_allclose(ctx.diag(), o.diag(), rtol=1e-12, atol=0)

@@ -117,3 +117,16 @@ def test_c5_fp32_full_size_stays_sorted_and_conserves():
     assert np.all(cells[1:] >= cells[:-1])
     del cells
     c.close()
+
+
+def test_c2_fp32_full_size_is_deterministic():
+    """Two runs of 3 steps from the same load give the same bits (fixed
+    reduction orders, no floating-point atomics)."""
+    out = []
+    for _ in range(2):
+        c, w = ready("c2", FP32_CHOSEN)
+        c.step(3, 0.0, w.dt)
+        out.append((c.field(L_.JX), c.field(L_.EY), c.particles(fields=False)["x"]))
+        c.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
