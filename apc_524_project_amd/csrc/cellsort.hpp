@@ -337,8 +337,10 @@ inline int cellsort_alloc(emo_ctx *c, CellSortBufs &b, int64_t cap, long long nc
     return EMO_OK;
   };
   int rc = 0;
-  if (!rc) rc = al((void **)&b.code, (size_t)cap);
-  if (!rc) rc = al((void **)&b.rank, (size_t)cap * 2);
+  // (8 entries of padding: the FP32 inverse pass reads codes and ranks four
+  // at a time from 4-aligned indices)
+  if (!rc) rc = al((void **)&b.code, (size_t)cap + 8);
+  if (!rc) rc = al((void **)&b.rank, ((size_t)cap + 8) * 2);
   if (!rc) rc = al((void **)&b.nsk, (size_t)kCodes * ncell * 2);
   if (!rc) rc = al((void **)&b.gbase, (size_t)kCodes * ncell * 2);
   if (!rc) rc = al((void **)&b.count, (size_t)(ncell + 1) * 4);

@@ -352,18 +352,28 @@ __global__ void __launch_bounds__(kBlock) k_inv_f32(InvArgs a) {
   };
   int nmov = 0;  // movers waiting in mov[wv] (wavefront-uniform, below 64 between batches)
   // a wavefront-uniform loop: every lane takes part in the batches
-  for (int q0 = pb; q0 < pe; q0 += 64 * kInvUnroll) {
-    const int p0 = q0 + lane;
+  // each lane takes 4 consecutive particles: their codes are one 4-byte
+  // load and their ranks one 8-byte load, from 4-aligned indices (the first
+  // chunk starts at pb rounded down; indices outside [pb, pe) are skipped)
+  static_assert(kInvUnroll == 4, "four particles per lane");
+  for (int q0 = pb & ~3; q0 < pe; q0 += 64 * kInvUnroll) {
+    const int p0 = q0 + 4 * lane;
+    uint32_t kw = 0xffffffffu;
+    uint2 rw = make_uint2(0u, 0u);
+    if (p0 < pe) {
+      kw = *(const uint32_t *)(a.b.code + p0);
+      rw = *(const uint2 *)(a.b.rank + p0);
+    }
     int k[kInvUnroll], r[kInvUnroll];
 #pragma unroll
     for (int u = 0; u < kInvUnroll; u++) {
-      const int p = p0 + 64 * u;
-      k[u] = p < pe ? (int)a.b.code[p] : -1;
-      r[u] = p < pe ? (int)a.b.rank[p] : 0;
+      const int p = p0 + u;
+      k[u] = (p >= pb && p < pe) ? (int)((kw >> (8 * u)) & 0xffu) : -1;
+      r[u] = (int)(((u < 2 ? rw.x : rw.y) >> (16 * (u & 1))) & 0xffffu);
     }
 #pragma unroll
     for (int u = 0; u < kInvUnroll; u++) {
-      const int p = p0 + 64 * u;
+      const int p = p0 + u;
       if (k[u] == kStayCode) {
         const int j = cell_of(p);
         const long long pos = (long long)stay0[wv][j] + r[u];
