@@ -494,8 +494,8 @@ __global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
   float hi[12], lo[12];
 #pragma unroll
   for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
-  if (c < a.ncell) {
-    const int b = a.off[c], e = a.off[c + 1];
+  const int b = c < a.ncell ? a.off[c] : 0, e = c < a.ncell ? a.off[c + 1] : 0;
+  if (b < e) {  // (an empty cell reads nothing: with no particles, slot 0 holds no record)
     // software pipeline: the source index two slots ahead and the state one
     // slot ahead are in flight while a particle is pushed.  A source index
     // outside the store (a slot that the permutation did not fill) is an
