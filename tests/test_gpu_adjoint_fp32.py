@@ -234,3 +234,25 @@ def test_fp32_boris_lightbc_gradients_match_the_fp64_adjoint():
     assert rel_err(a32["d_phase"][0], a64["d_phase"][0]) < 1e-4
     for key in ("xbar0", "vbar0"):
         assert rel_l2(a32[key], a64[key]) < 1e-3, (key, rel_l2(a32[key], a64[key]))
+
+
+def test_fp32_adjoint_without_particles_matches_fp64():
+    """A LightBC pulse into vacuum (no particles): the gradient of the field
+    energy with respect to the pulse is the field adjoint's alone, in both
+    forms."""
+    w = Wave(side=-1, pol=2, amp=0.3, omega=0.39, phase=30.0, inv_width=0.08, delay=3.0)
+    case = Case(n=(12, 4, 4), L=(12.0, 4.0, 4.0), nparticles=0, temp=(0.0, 0.0), dens_phys=0.0,
+                relativity=1, waves=[w],
+                fbc=(EXTERNAL, EXTERNAL, PERIODIC, PERIODIC, PERIODIC, PERIODIC),
+                pbc=(REFLECTING, REFLECTING, PERIODIC, PERIODIC, PERIODIC, PERIODIC))
+    out = {}
+    for prec in (FP64_PLAIN, FP32_CHOSEN):
+        ctx = Context(emo_config(case, precision=prec, max_particles=1))
+        ctx.init_constant()
+        if prec == FP64_PLAIN:
+            ctx.interpolate()
+        out[prec] = ctx.adjoint_field_energy(NSTEPS, 0.0, case.dt_cfl(), 4)
+        ctx.close()
+    assert out[FP64_PLAIN]["d_amp"][0] != 0
+    assert rel_err(out[FP32_CHOSEN]["d_amp"][0], out[FP64_PLAIN]["d_amp"][0]) < 1e-5
+    assert rel_err(out[FP32_CHOSEN]["d_phase"][0], out[FP64_PLAIN]["d_phase"][0]) < 1e-5

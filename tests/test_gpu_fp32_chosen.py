@@ -287,3 +287,23 @@ def test_fp32_test_particles_move_but_deposit_nothing():
     assert rel(g["x"], p.x) < 1e-6
     for f in (JX, JY, JZ, EX):
         assert rel(ctx.field(f), o.field(f)) < 1e-3, f
+
+
+def test_fp32_three_species_track_oracle():
+    """Three species (electrons, ions, a heavier positive species) with
+    their own charges and masses: against the oracle."""
+    case = Case(n=(8, 8, 8), L=(8.0, 8.0, 8.0), nparticles=3 * 16 * 512, temp=(5000.0, 5000.0, 5000.0),
+                mass_ratio=(1.0, 1836.0, 4.0), charge_ratio=(-1.0, 1.0, 2.0), dens_frac=(0.5, 0.25, 0.25),
+                is_test=(0, 0, 0),
+                dens_phys=2.824e9, relativity=1, E0=(0.001, -0.002, 0.003), B0=(0.005, 0.002, -0.004))
+    ctx, o = setup(case)
+    dt = case.dt_cfl()
+    run(ctx, o, 10, dt)
+    g = by_id(ctx)
+    p = o.particles().by_id()
+    assert np.array_equal(g["species"], p.species)
+    assert set(np.unique(p.species)) == {0, 1, 2}
+    assert rel(g["x"], p.x) < 1e-6
+    assert rel(g["v"], p.v) < 1e-4
+    for f in (EX, JX, JY, JZ):
+        assert rel(ctx.field(f), o.field(f)) < 1e-3, f
