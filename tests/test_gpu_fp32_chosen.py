@@ -307,3 +307,17 @@ def test_fp32_three_species_track_oracle():
     assert rel(g["v"], p.v) < 1e-4
     for f in (EX, JX, JY, JZ):
         assert rel(ctx.field(f), o.field(f)) < 1e-3, f
+
+
+def test_fp32_crowded_cell_is_reported():
+    """More than 65,535 particles in one cell exceeds the FP32 sort's 16-bit
+    group offsets: the step reports EMO_ERR_INVARIANT (every table access is
+    checked against the cell's range, so the kernels end normally)."""
+    from apc_524_project_amd._lib import EmoError
+    case = Case(n=(2, 2, 2), L=(2.0, 2.0, 2.0), nparticles=8 * 70000, temp=(5000.0, 5000.0),
+                dens_phys=2.824e9, relativity=1)
+    ctx, _ = setup(case)
+    with pytest.raises(EmoError) as e:
+        ctx.step(1, 0.0, case.dt_cfl())
+    assert e.value.code == -5
+    ctx.close()
