@@ -9,10 +9,11 @@ diagnostics (src/IO/output.cpp:90-121) after `spec.nsteps` steps of the form
 FP32 particle adjoint and FP64 field adjoints), as a function of the injected pulses' peak amplitudes and phases
 (LightBC, src/grid/lightBC.cpp:165-198; phases in degrees as in the input
 file) and of the per-species density-perturbation amplitudes of the
-displacement loader.  The forward pass runs emo_adjoint_field_energy, which
-does the forward steps and the adjoint sweep (checkpoint and recompute every
-`spec.checkpoint_every` steps) in one call; backward scales the stored
-gradient.  The computation runs on the GPU through the C ABI; there is no
+displacement loader.  When an input requires a gradient, the forward pass
+runs emo_adjoint_field_energy, which does the forward steps and the adjoint
+sweep (checkpoint and recompute every `spec.checkpoint_every` steps) in one
+call, and backward scales the stored gradient; otherwise it runs the forward
+steps alone.  The computation runs on the GPU through the C ABI; there is no
 CPU path.
 """
 from __future__ import annotations
@@ -72,14 +73,37 @@ def field_energy_and_grad(spec: RunSpec, amps, phases, eps):
     return (out["energy"], np.asarray(out["d_amp"]), np.asarray(out["d_phase"]) * math.pi / 180.0, d_eps)
 
 
+def field_energy_only(spec: RunSpec, amps, phases, eps) -> float:
+    """F alone: the forward steps and the diagnostics, no adjoint sweep."""
+    ctx, cfg, ld = _setup(spec, amps, phases, eps)
+    if cfg.precision == FP64_PLAIN:
+        ctx.sort()  # as the gradient call does before its first push (src/main.cpp:225-226)
+    ctx.step(spec.nsteps, spec.t0, spec.dt)
+    d = ctx.diag()
+    ctx.close()
+    return float(d[5] + d[4])
+
+
 class FieldEnergy(torch.autograd.Function):
+    """F(amps, phases, eps); the output and the saved gradients live on the
+    inputs' device and dtype.  The gradient call (forward steps plus the
+    adjoint sweep) runs only when an input requires a gradient; otherwise
+    the forward steps alone give F."""
+
     @staticmethod
     def forward(ctx, amps, phases, eps, spec):
-        F, da, dp, de = field_energy_and_grad(spec, amps.detach().cpu().numpy(), phases.detach().cpu().numpy(),
-                                              eps.detach().cpu().numpy())
-        ctx.save_for_backward(torch.as_tensor(da, dtype=amps.dtype), torch.as_tensor(dp, dtype=phases.dtype),
-                              torch.as_tensor(de, dtype=eps.dtype))
-        return torch.tensor(F, dtype=torch.float64)
+        a = amps.detach().cpu().numpy()
+        p = phases.detach().cpu().numpy()
+        e = eps.detach().cpu().numpy()
+        if any(ctx.needs_input_grad[:3]):
+            F, da, dp, de = field_energy_and_grad(spec, a, p, e)
+        else:
+            F = field_energy_only(spec, a, p, e)
+            da, dp, de = np.zeros(len(a)), np.zeros(len(p)), np.zeros(len(e))
+        ctx.save_for_backward(torch.as_tensor(da, dtype=amps.dtype, device=amps.device),
+                              torch.as_tensor(dp, dtype=phases.dtype, device=phases.device),
+                              torch.as_tensor(de, dtype=eps.dtype, device=eps.device))
+        return torch.tensor(F, dtype=amps.dtype, device=amps.device)
 
     @staticmethod
     def backward(ctx, g):

@@ -99,6 +99,8 @@ int process_info(const emo_loader *ld, emo_config *cfg, double *dens_out, double
   double cden = 0.0;
   for (int i = 0; i < ld->nspecies; i++) {
     if (ld->dens_frac[i] < 0) return EMO_ERR_CONFIG;
+    // the displacement loader divides by the wavenumber
+    if (ld->pert_eps[i] != 0.0 && ld->pert_k[i] == 0.0) return EMO_ERR_CONFIG;
     cden += ld->dens_frac[i];
   }
   for (int i = 0; i < ld->nspecies; i++) {
@@ -156,11 +158,17 @@ int load_maxwellian_host(const emo_config *cfg, const emo_loader *ld, std::vecto
     vp[0] = rng.getGaussian(0.0, vth);
     vp[1] = rng.getGaussian(0.0, vth);
     vp[2] = rng.getGaussian(0.0, vth);
-    // extensions (zero by default): displacement loader along x and drift
+    // extensions (zero by default): displacement loader along x and drift.
+    // The sensitivity dx/deps = -sin(k x_u)/k exists wherever a wavenumber
+    // is set, also at eps = 0 (the linear response about the uniform load);
+    // the displacement itself is applied only for eps != 0.
+    if (dx_deps && ld->pert_k[ispec] != 0.0) {
+      double k = ld->pert_k[ispec];
+      (*dx_deps)[ip] = -std::sin(k * (xp[0] - cfg->xyz0[0])) / k;
+    }
     if (ld->pert_eps[ispec] != 0.0) {
       double k = ld->pert_k[ispec];
       double xr = xp[0] - cfg->xyz0[0];
-      if (dx_deps) (*dx_deps)[ip] = -std::sin(k * xr) / k;
       xr = xr - (ld->pert_eps[ispec] / k) * std::sin(k * xr);
       if (xr < 0) xr += cfg->Lxyz[0];
       if (xr >= cfg->Lxyz[0]) xr -= cfg->Lxyz[0];

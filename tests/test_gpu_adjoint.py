@@ -176,3 +176,39 @@ def test_adjoint_leaves_the_final_state(base):
         assert np.array_equal(pa[k][oa], pb[k][ob]), k
     for f in range(13):
         assert np.array_equal(a.field(f), b.field(f)), f
+
+
+def test_autograd_on_device_at_zero_perturbation():
+    """Inputs on cuda:0: the output and the gradients come back on that
+    device.  At eps = 0 with a wavenumber set, dF/deps is the linear
+    response about the uniform load (nonzero), the dot product of the
+    position adjoint with the loader's sensitivity, and central differences
+    in eps agree with it.  Without a gradient the forward alone gives F."""
+    import torch
+
+    from apc_524_project_amd.adjoint import RunSpec, field_energy
+
+    case = case_with(eps=0.0)
+    dt = case.dt_cfl()
+    spec = RunSpec(cfg=emo_config(case), loader=emo_loader(case), nsteps=NSTEPS, dt=dt, checkpoint_every=4)
+    dev = torch.device("cuda:0")
+    amps = torch.tensor([0.3], dtype=torch.float64, device=dev, requires_grad=True)
+    phases = torch.tensor([30.0], dtype=torch.float64, device=dev, requires_grad=True)
+    eps = torch.tensor([0.0, 0.0], dtype=torch.float64, device=dev, requires_grad=True)
+    F = field_energy(amps, phases, eps, spec)
+    assert F.device == dev
+    F.backward()
+    assert amps.grad.device == dev and eps.grad.device == dev
+    adj = adjoint(case, dt, 4)
+    assert F.item() == adj["energy"]
+    dxde = loader_dx_deps(emo_loader(case), emo_config(case))
+    assert np.abs(dxde).max() > 1.0
+    g_eps = float(np.dot(adj["xbar0"][:, 0], dxde))
+    assert g_eps != 0.0
+    assert eps.grad[0].item() == pytest.approx(g_eps, rel=1e-12)
+    h = 1e-6
+    fd = (energy(case_with(eps=h), dt) - energy(case_with(eps=-h), dt)) / (2 * h)
+    assert rel_err(g_eps, fd) < 1e-4, (g_eps, fd)
+    with torch.no_grad():
+        F0 = field_energy(amps.detach(), phases.detach(), eps.detach(), spec)
+    assert F0.item() == pytest.approx(adj["energy"], rel=1e-14)
