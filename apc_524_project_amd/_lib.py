@@ -95,6 +95,19 @@ class EmoError(RuntimeError):
 _lib = None
 
 
+def _torch_runtime_first():
+    """torch-ROCm ships its own HIP runtime beside the system one that
+    libemopic links.  The two coexist in one process only when torch's
+    initialises first (the other order leaves torch with no devices), so
+    when torch is installed its runtime is initialised before libemopic's
+    first HIP call.  Without torch this does nothing."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def lib():
     """Load the in-tree libemopic.so (raises if it has not been built)."""
     global _lib
@@ -105,6 +118,7 @@ def lib():
             f"{LIB_PATH} is missing: build it with `python -m apc_524_project_amd.build` "
             "(the HIP extension is required; there is no CPU fallback)"
         )
+    _torch_runtime_first()
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
     L.emo_create.argtypes = [C.POINTER(EmoConfig), C.POINTER(vp)]
