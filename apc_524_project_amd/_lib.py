@@ -30,7 +30,7 @@ EXPORTS = (
     "emo_init_constant", "emo_init_poisson", "emo_sort", "emo_push", "emo_particle_bc",
     "emo_deposit", "emo_fold", "emo_evolve_fields", "emo_refresh", "emo_interpolate",
     "emo_step", "emo_diag", "emo_synchronize", "emo_timing_enable", "emo_timing_select", "emo_kernel_time",
-    "emo_timing_reset", "emo_adjoint_field_energy", "emo_loader_dx_deps",
+    "emo_timing_reset", "emo_adjoint_field_energy", "emo_loader_dx_deps", "emo_init_poisson_noalloc",
 )
 
 
@@ -141,6 +141,7 @@ def lib():
     L.emo_set_field.argtypes = [vp, C.c_int, vp]
     L.emo_init_constant.argtypes = [vp]
     L.emo_init_poisson.argtypes = [vp, i32, dbl, vp, vp]
+    L.emo_init_poisson_noalloc.argtypes = [vp, i32, dbl, vp, vp]
     for fn in ("emo_sort", "emo_particle_bc", "emo_fold", "emo_interpolate", "emo_synchronize",
                "emo_timing_reset"):
         getattr(L, fn).argtypes = [vp]
@@ -250,6 +251,13 @@ class Context:
         sw = np.zeros(4, np.int32)
         res = np.zeros(8)
         self._chk(self.L.emo_init_poisson(self.h, iter_max, tol, _p(sw), _p(res)))
+        return sw, res
+
+    def init_poisson_noalloc(self, iter_max=10000, tol=-1.0):
+        """emo_init_poisson on the library's static device workspace."""
+        sw = np.zeros(4, np.int32)
+        res = np.zeros(8)
+        self._chk(self.L.emo_init_poisson_noalloc(self.h, iter_max, tol, _p(sw), _p(res)))
         return sw, res
 
     # phases (FP64 plain form; evolve and refresh in both forms)

@@ -1396,14 +1396,14 @@ int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n) {
 // Poisson_Solver::run_poisson_solver_ on FP32 arrays with the FP64 residual
 // norms deciding convergence (the reference's rule), phiToE, AToB, the ghost
 // refresh.  sweeps/resid as emo_init_poisson.
-int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_out, double *resid_out) {
+int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_out, double *resid_out,
+                      const PoissonWs &ws) {
   if (iter_max < 0) return set_err(c, EMO_ERR_CONFIG, "iter_max must be >= 0");
   ChosenState *s = c->ch;
   const Geo &g = c->geo;
   const size_t nt = (size_t)g.ntot;
-  float *cr = nullptr;
-  int rc = dalloc(c, &cr, 8 * g.ncell);
-  if (rc) return rc;
+  float *cr = ws.cr;
+  int rc = 0;
   {
     RhoJArgs a;
     a.pa = s->pa;
@@ -1434,12 +1434,11 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
     fk_const_fill<float>(c, s->fs.Bt[q], 0.0, (long long)nt);
   }
   const int nb = residual_blocks(c);
-  double *partials = nullptr;
-  float *u[8] = {};
-  rc = dalloc(c, &partials, 2 * nb + 2);
-  for (int q = 0; q < 8 && !rc; q++) rc = dalloc(c, &u[q], nt);
-  if (!rc && c->cfg.neutralize) fk_subtract_mean<float>(c, s->fs.rho, partials, nb, partials + 2 * nb);
-  for (int q = 0; q < 8 && !rc; q++) fk_const_fill<float>(c, u[q], 0.0, (long long)nt);
+  double *partials = ws.partials;
+  float *u[8];
+  for (int q = 0; q < 8; q++) u[q] = (float *)ws.u[q];
+  if (c->cfg.neutralize) fk_subtract_mean<float>(c, s->fs.rho, partials, nb, partials + 2 * nb);
+  for (int q = 0; q < 8; q++) fk_const_fill<float>(c, u[q], 0.0, (long long)nt);
   // run_poisson_solver_ coefficients (src/poisson/poisson.cpp:181-185)
   const double dx_ = g.d[0], dy_ = g.d[1], dz_ = g.d[2];
   const double celldist2 = dx_ * dx_ * dy_ * dy_ + dy_ * dy_ * dz_ * dz_ + dx_ * dx_ * dz_ * dz_;
@@ -1467,7 +1466,7 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
   const float *srcs[4] = {s->fs.rho, s->fs.J[0], s->fs.J[1], s->fs.J[2]};
   double resid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int32_t sweeps[4] = {0, 0, 0, 0};
-  std::vector<double> hpart(2 * nb);
+  double hpart[2 * kMaxResidualBlocks];
   for (int sv = 0; sv < 4 && !rc; sv++) {
     float *u1 = u[2 * sv], *u2 = u[2 * sv + 1];
     const double tl = sv == 0 ? tphi : tA;
@@ -1481,7 +1480,7 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
       fk_jacobi<float>(c, src, dst, srcs[sv], ax, ay, az, af, sourceMult);
       potential_bc<float>(c, dst, bounds[sv]);
       fk_residual<float>(c, u1, u2, partials, nb);
-      hipError_t e = hipMemcpyAsync(hpart.data(), partials, 2 * nb * sizeof(double), hipMemcpyDeviceToHost,
+      hipError_t e = hipMemcpyAsync(hpart, partials, 2 * nb * sizeof(double), hipMemcpyDeviceToHost,
                                     c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       if (e != hipSuccess) {
@@ -1505,9 +1504,6 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
     refresh_fields<float>(c, s->fs, 0.0, 0.0);
   }
   hipStreamSynchronize(c->stream);
-  for (int q = 0; q < 8; q++) dfree(u[q]);
-  dfree(partials);
-  dfree(cr);
   if (rc) return rc;
   if (sweeps_out) memcpy(sweeps_out, sweeps, sizeof(sweeps));
   if (resid_out) memcpy(resid_out, resid, sizeof(resid));

@@ -5,7 +5,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "emo_internal.hpp"
@@ -138,6 +140,31 @@ int last_launch(emo_ctx *c) {
   return EMO_OK;
 }
 
+// The static device workspace of emo_init_poisson_noalloc: one copy per GPU
+// the library runs on, held until the process ends.  Host code never uses
+// the array's name as an address: each device's copy is looked up once with
+// hipGetSymbolAddress (emo_create, that device current) and kept per device.
+constexpr size_t kNoAllocWsBytes = 64ull << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kNoAllocWsBytes];
+std::mutex g_ws_mu;
+void *g_ws_ptr[kMaxDevices] = {};
+
+int noalloc_ws_for_device(emo_ctx *c) {
+  if (c->device < 0 || c->device >= kMaxDevices) return set_err(c, EMO_ERR_CONFIG, "device id out of range");
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  if (!g_ws_ptr[c->device]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_err(c, EMO_ERR_HIP, std::string("hipGetSymbolAddress(workspace): ") + hipGetErrorString(e));
+    g_ws_ptr[c->device] = ptr;
+  }
+  c->noalloc_ws = g_ws_ptr[c->device];
+  return EMO_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -176,12 +203,16 @@ int emo_create(const emo_config *cfg, emo_ctx **out) {
     c->species.m[s] = cfg->mass[s] != 0 ? cfg->mass[s] : 1.0;
     c->species.test[s] = cfg->isTestParticle[s];
   }
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     fprintf(stderr, "emo_create: hipStreamCreate: %s\n", hipGetErrorString(e));
     delete c;
     return EMO_ERR_HIP;
   }
+  // the static workspace of emo_init_poisson_noalloc on this device; a
+  // failure here only disables that entry point (it reports the error)
+  if (noalloc_ws_for_device(c) != EMO_OK) fprintf(stderr, "emo_create: %s\n", c->err.c_str());
   rc = 0;
   if (!rc) rc = dalloc(c, &c->dflag, 8);
   if (!rc && hipHostMalloc((void **)&c->hflag, 8 * sizeof(unsigned long long)) != hipSuccess) rc = EMO_ERR_ALLOC;
@@ -561,11 +592,42 @@ int emo_interpolate(emo_ctx *c) {
   return last_launch(c);
 }
 
-int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_out, double *resid_out) {
-  if (!c) return EMO_ERR_CONFIG;
+}  // extern "C"
+
+namespace emo {
+
+static size_t ws_round(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t poisson_ws_bytes(const emo_ctx *c) {
+  const size_t el = c->cfg.precision == EMO_FP64_PLAIN ? sizeof(double) : sizeof(float);
+  size_t b = ws_round((2 * kMaxResidualBlocks + 2) * sizeof(double));
+  if (c->cfg.precision != EMO_FP64_PLAIN) b += ws_round((size_t)8 * c->geo.ncell * sizeof(float));
+  return b + 8 * ws_round((size_t)c->geo.ntot * el);
+}
+
+void poisson_ws_carve(const emo_ctx *c, void *base, PoissonWs &ws) {
+  const size_t el = c->cfg.precision == EMO_FP64_PLAIN ? sizeof(double) : sizeof(float);
+  char *p = (char *)base;
+  ws.partials = (double *)p;
+  p += ws_round((2 * kMaxResidualBlocks + 2) * sizeof(double));
+  ws.cr = nullptr;
   if (c->cfg.precision != EMO_FP64_PLAIN) {
-    return fp32_init_poisson(c, iter_max, tol, sweeps_out, resid_out);
+    ws.cr = (float *)p;
+    p += ws_round((size_t)8 * c->geo.ncell * sizeof(float));
   }
+  for (int q = 0; q < 8; q++) {
+    ws.u[q] = p;
+    p += ws_round((size_t)c->geo.ntot * el);
+  }
+}
+
+}  // namespace emo
+
+namespace {
+
+// the FP64 Poisson initialisation on a carved workspace
+int init_poisson_f64(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_out, double *resid_out,
+                     const PoissonWs &ws) {
   if (iter_max < 0) return set_err(c, EMO_ERR_CONFIG, "iter_max must be >= 0");
   const Geo &g = c->geo;
   size_t nt = (size_t)g.ntot;
@@ -581,17 +643,13 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
     launch_const_fill_f64(c, c->B[q], 0.0, nt);
     launch_const_fill_f64(c, c->Bt[q], 0.0, nt);
   }
-  double *u[8] = {};
-  for (int q = 0; q < 8 && !rc; q++) rc = dalloc(c, &u[q], nt);
-  for (int q = 0; q < 8 && !rc; q++)
-    if (hipMemsetAsync(u[q], 0, nt * sizeof(double), c->stream) != hipSuccess) rc = EMO_ERR_HIP;
-  int nb = residual_blocks(c);
-  double *partials = nullptr;
-  if (!rc) rc = dalloc(c, &partials, 2 * nb + 2);
-  if (rc) {
-    for (int q = 0; q < 8; q++) dfree(u[q]);
-    return rc;
+  double *u[8];
+  for (int q = 0; q < 8; q++) {
+    u[q] = (double *)ws.u[q];
+    launch_const_fill_f64(c, u[q], 0.0, nt);
   }
+  int nb = residual_blocks(c);
+  double *partials = ws.partials;
   // extension: a uniform neutralising background (emo_config.neutralize)
   if (c->cfg.neutralize) fk_subtract_mean<double>(c, c->rho, partials, nb, partials + 2 * nb);
   // run_poisson_solver_ coefficients (src/poisson/poisson.cpp:181-185)
@@ -623,7 +681,7 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
   const double *srcs[4] = {c->rho, c->J[0], c->J[1], c->J[2]};
   double resid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int32_t sweeps[4] = {0, 0, 0, 0};
-  std::vector<double> hpart(2 * nb);
+  double hpart[2 * kMaxResidualBlocks];
   for (int s = 0; s < 4 && !rc; s++) {
     double *u1 = u[2 * s], *u2 = u[2 * s + 1];
     double tl = s == 0 ? tphi : tA;
@@ -638,7 +696,7 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
       potential_bc<double>(c, dst, bounds[s]);
       launch_residual_f64(c, u1, u2, partials, nb);
       // per-block max|du| and sum du^2, combined on the host in block order
-      hipError_t e = hipMemcpyAsync(hpart.data(), partials, 2 * nb * sizeof(double),
+      hipError_t e = hipMemcpyAsync(hpart, partials, 2 * nb * sizeof(double),
                                     hipMemcpyDeviceToHost, c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       if (e != hipSuccess) {
@@ -664,14 +722,50 @@ int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
     launch_gather_f64(c);         // src/main.cpp:179
   }
   hipStreamSynchronize(c->stream);
-  for (int q = 0; q < 8; q++) dfree(u[q]);
-  dfree(partials);
   if (rc) return rc;
   if (sweeps_out) memcpy(sweeps_out, sweeps, sizeof(sweeps));
   if (resid_out) memcpy(resid_out, resid, sizeof(resid));
   rc = last_launch(c);
   if (rc) return rc;
   return check_flags(c);
+}
+
+int init_poisson_ws(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_out, double *resid_out,
+                    const PoissonWs &ws) {
+  if (c->cfg.precision != EMO_FP64_PLAIN) return fp32_init_poisson(c, iter_max, tol, sweeps_out, resid_out, ws);
+  return init_poisson_f64(c, iter_max, tol, sweeps_out, resid_out, ws);
+}
+
+}  // namespace
+
+extern "C" {
+
+int emo_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_out, double *resid_out) {
+  if (!c) return EMO_ERR_CONFIG;
+  void *base = nullptr;
+  const size_t bytes = poisson_ws_bytes(c);
+  if (hipMalloc(&base, bytes) != hipSuccess)
+    return set_err(c, EMO_ERR_ALLOC, "hipMalloc failed for the Poisson initialisation's workspace");
+  PoissonWs ws;
+  poisson_ws_carve(c, base, ws);
+  int rc = init_poisson_ws(c, iter_max, tol, sweeps_out, resid_out, ws);
+  hipStreamSynchronize(c->stream);
+  hipFree(base);
+  return rc;
+}
+
+int emo_init_poisson_noalloc(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_out, double *resid_out) {
+  if (!c) return EMO_ERR_CONFIG;
+  const size_t bytes = poisson_ws_bytes(c);
+  if (bytes > kNoAllocWsBytes)
+    return set_err(c, EMO_ERR_CONFIG,
+                   "emo_init_poisson_noalloc: this grid needs " + std::to_string(bytes) +
+                       " bytes of workspace, more than the static workspace's " + std::to_string(kNoAllocWsBytes) +
+                       " (use emo_init_poisson)");
+  if (!c->noalloc_ws) return set_err(c, EMO_ERR_HIP, "emo_init_poisson_noalloc: no static workspace on this device");
+  PoissonWs ws;
+  poisson_ws_carve(c, c->noalloc_ws, ws);
+  return init_poisson_ws(c, iter_max, tol, sweeps_out, resid_out, ws);
 }
 
 int emo_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {

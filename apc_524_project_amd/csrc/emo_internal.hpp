@@ -127,6 +127,11 @@ struct emo_ctx {
   size_t adj_pool_bytes = 0;
   hipStream_t adj_stream = nullptr;  // the adjoint's backward sweep
 
+  // the device this context was created on, and that device's copy of the
+  // static workspace of emo_init_poisson_noalloc (emo_api.cpp)
+  int device = 0;
+  void *noalloc_ws = nullptr;
+
   // reductions / flags
   double *red = nullptr;   // scratch for FP64 reductions
   int64_t red_len = 0;
@@ -254,7 +259,22 @@ int fp32_get_cells(emo_ctx *c, int32_t *cells);
 int fp32_get_field(emo_ctx *c, int fid, double *out);
 int fp32_set_field(emo_ctx *c, int fid, const double *in);
 int fp32_init_constant(emo_ctx *c);
-int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps, double *resid);
+// The Poisson initialisation's workspace: the two Jacobi iterates of each of
+// the four solves (phi, Ax, Ay, Az), the FP64 residual partial sums and, in
+// the FP32 form, the cell charge vectors of the initial deposit.  Both
+// entry points carve it from one block with the same layout (every piece
+// 256-byte aligned): emo_init_poisson from an allocation, the no-allocation
+// variant from the library's static device array.
+struct PoissonWs {
+  void *u[8] = {};
+  double *partials = nullptr;  // 2 * kMaxResidualBlocks + 2 doubles
+  float *cr = nullptr;         // FP32 form: 8 per cell
+};
+constexpr int kMaxResidualBlocks = 1024;
+size_t poisson_ws_bytes(const emo_ctx *c);
+void poisson_ws_carve(const emo_ctx *c, void *base, PoissonWs &ws);
+int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps, double *resid,
+                      const PoissonWs &ws);
 // The FP32 form's device state, by reference, for its adjoint (adjoint.hip):
 // the particle records (current and alternate buffers), the stored gamma of
 // the first push, the fields and the cell ranges of the current order.

@@ -744,7 +744,7 @@ void launch_jacobi_f64(emo_ctx *c, const double *src, double *dst, const double 
 int residual_blocks(emo_ctx *c) {
   long long nphys = (long long)c->geo.n[0] * c->geo.n[1] * c->geo.n[2];
   long long b = (nphys + kBlock - 1) / kBlock;
-  return (int)(b < 1024 ? (b > 0 ? b : 1) : 1024);
+  return (int)(b < kMaxResidualBlocks ? (b > 0 ? b : 1) : kMaxResidualBlocks);
 }
 
 void launch_residual_f64(emo_ctx *c, const double *u1, const double *u2, double *partials,

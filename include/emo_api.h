@@ -152,6 +152,19 @@ int emo_init_constant(emo_ctx *ctx);
  * the last sweep per solve) may be NULL. */
 int emo_init_poisson(emo_ctx *ctx, int32_t iter_max, double tol, int32_t *sweeps_out,
                      double *resid_out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): emo_init_poisson with the same arguments and the same
+ * results, whose workspace -- the Jacobi iterates of the four solves, the
+ * residual partial sums and, in the FP32 form, the cell charge vectors --
+ * lies in a static device array of 64 MiB per GPU instead of an allocation.
+ * It allocates nothing.  A grid whose workspace exceeds 64 MiB (FP64: about
+ * 1.0e6 padded points; FP32: about 2.0e6 minus the cell vectors) returns
+ * EMO_ERR_CONFIG before any work.  The array is shared by every context on
+ * the GPU and keeps the last call's contents: calls that overlap on one GPU
+ * (two host threads, two contexts' streams) race. */
+int emo_init_poisson_noalloc(emo_ctx *ctx, int32_t iter_max, double tol, int32_t *sweeps_out,
+                             double *resid_out);
 
 /* ---- the step's phases (FP64 plain form; in FP32 the particle phases and
  * the fold return EMO_ERR_UNSUPPORTED, the field-only phases emo_evolve_fields
