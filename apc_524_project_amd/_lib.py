@@ -31,6 +31,7 @@ EXPORTS = (
     "emo_deposit", "emo_fold", "emo_evolve_fields", "emo_refresh", "emo_interpolate",
     "emo_step", "emo_diag", "emo_synchronize", "emo_timing_enable", "emo_timing_select", "emo_kernel_time",
     "emo_timing_reset", "emo_adjoint_field_energy", "emo_loader_dx_deps", "emo_init_poisson_noalloc",
+    "emo_layout_info",
 )
 
 
@@ -157,6 +158,7 @@ def lib():
     L.emo_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(dbl), C.POINTER(i64)]
     L.emo_adjoint_field_energy.argtypes = [vp, i32, dbl, dbl, i32, vp, vp, vp, vp, vp, vp]
     L.emo_loader_dx_deps.argtypes = [C.POINTER(EmoLoader), C.POINTER(EmoConfig), vp]
+    L.emo_layout_info.argtypes = [vp, vp]
     _lib = L
     return L
 
@@ -227,6 +229,13 @@ class Context:
         self._chk(self.L.emo_get_particles(self.h, _p(x), _p(v), _p(g), _p(s), _p(i),
                                            _p(f) if fields else None))
         return dict(x=x, v=v, gamma=g, species=s, id=i, field=f)
+
+    def layout_info(self):
+        """FP32 chosen form: slots per record buffer, slots in the cells'
+        ranges, layout rebuilds so far."""
+        out = np.zeros(3, np.int64)
+        self._chk(self.L.emo_layout_info(self.h, _p(out)))
+        return {"slots": int(out[0]), "used": int(out[1]), "rebuilds": int(out[2])}
 
     def cell_ids(self):
         out = np.zeros(self.num_particles, np.int32)

@@ -977,14 +977,15 @@ struct CellStepAdj32Args {
   PushK k;                      // geometry, species q_p, dt / dx, BCs
   const int4 *pa, *pb;          // pre-push state of step s, cell-sorted
   const float *g0;              // the stored gamma of the first push, or null
-  const int32_t *off;           // cell c of this order holds off[c]..off[c+1]
-  const uint32_t *fwd;          // pre-push index -> post-sort slot
+  const int32_t *base, *cnt;    // cell c holds cnt[c] particles from slot base[c]
+  const uint32_t *fwd;          // pre-push slot -> post-step slot
   const float *E[3], *B[3], *Bt[3];  // the fields the push gathered from
   const float *Jb[3];           // current adjoint (after fold^T), rounded to FP32
   PartAdj32 in, out;            // adjoints: post-sort order in, pre-push order out
   float *g0b;                   // adjoint of the stored gamma (first step), or null
   double *cg;                   // 18 per cell, component-major
   long long ncell;
+  uint32_t nslots;
 };
 
 template <bool REL, bool PER>
@@ -1001,7 +1002,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_step_adj32(CellStepAdj32Args a)
 #pragma unroll
   for (int q = 0; q < 18; q++) acc[q] = 0.0;
   if (c < a.ncell) {
-    const int b = a.off[c], e = a.off[c + 1];
+    const int b = a.base[c], e = b + a.cnt[c];
     if (b < e) {
       // the cell's gather vector (the cell is the same for all its particles)
       const Geo &g = a.k.g;
@@ -1021,11 +1022,14 @@ __global__ void __launch_bounds__(kBlock) k_cell_step_adj32(CellStepAdj32Args a)
       const int last = e - 1;
       int i = b + l;
       int4 A = a.pa[min(i, last)], Bv = a.pb[min(i, last)];
+      // (a slot the step did not map -- only after a flagged error -- reads slot 0)
       uint32_t j = a.fwd[min(i, last)];
+      j = j < a.nslots ? j : 0u;
       for (; i < e; i += 16) {
         const int inx = min(i + 16, last);
         const int4 An = a.pa[inx], Bn = a.pb[inx];
-        const uint32_t jn = a.fwd[inx];
+        uint32_t jn = a.fwd[inx];
+        jn = jn < a.nslots ? jn : 0u;
         const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
         const float fin[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
         const float uin[3] = {as_f(Bv.x), as_f(Bv.y), as_f(Bv.z)};
@@ -1059,22 +1063,23 @@ __global__ void __launch_bounds__(kBlock) k_cell_step_adj32(CellStepAdj32Args a)
   }
 }
 
-// One FP32 snapshot: the particle records, the stored gamma (first push),
-// the fields, the cell ranges; on the tape also the sort's forward map.
+// One FP32 snapshot: the particle records (every slot), the stored gamma
+// (first push), the fields, the slot layout; on the tape also the step's
+// map source slot -> new slot.
 struct Snap32 {
   int4 *pa = nullptr, *pb = nullptr;
   float *g0 = nullptr;
   bool has_g0 = false;
   float *E[3] = {}, *A[2][3] = {};  // A[0] = Bfix, A[1] = Btfix
   float *J[3] = {};                 // the final state only
-  int32_t *off = nullptr;
-  uint32_t *fwd = nullptr;          // tape: pre-push index -> new slot
+  int32_t *base = nullptr, *cnt = nullptr;
+  uint32_t *fwd = nullptr;          // tape: source slot -> new slot
   bool b_is_a0 = true;
   double t = 0.0;
 };
 
-void snap32_alloc(Pool &P, const emo_ctx *c, Snap32 &s, bool particles, bool g0, bool tape, bool currents) {
-  const size_t n = (size_t)std::max<int64_t>(c->np, 1);
+void snap32_alloc(Pool &P, emo_ctx *c, Snap32 &s, bool particles, bool g0, bool tape, bool currents) {
+  const size_t n = (size_t)fp32_view(c).nslots;
   const size_t nt = (size_t)c->geo.ntot;
   if (particles) {
     P.take(&s.pa, n);
@@ -1087,14 +1092,15 @@ void snap32_alloc(Pool &P, const emo_ctx *c, Snap32 &s, bool particles, bool g0,
     P.take(&s.A[1][d], nt);
     if (currents) P.take(&s.J[d], nt);
   }
-  P.take(&s.off, (size_t)c->geo.ncell + 1);
+  P.take(&s.base, (size_t)c->geo.ncell + 1);
+  P.take(&s.cnt, (size_t)c->geo.ncell + 1);
   if (tape) P.take(&s.fwd, n);
 }
 
-// the fields and ranges (and, with particles, the records) of the context
+// the fields and layout (and, with particles, the records) of the context
 void snap32_take(emo_ctx *c, Snap32 &s, double t, bool particles) {
   Fp32View v = fp32_view(c);
-  const size_t np = (size_t)c->np, nt4 = (size_t)c->geo.ntot * 4;
+  const size_t np = (size_t)v.nslots, nt4 = (size_t)c->geo.ntot * 4;
   CopyBatch cb(c);
   if (particles && s.pa) {
     cb.add(s.pa, *v.pa, np * 16);
@@ -1108,7 +1114,8 @@ void snap32_take(emo_ctx *c, Snap32 &s, double t, bool particles) {
     cb.add(s.A[1][d], v.Btfix[d], nt4);
     if (s.J[d]) cb.add(s.J[d], v.fs->J[d], nt4);
   }
-  cb.add(s.off, *v.off, (size_t)(c->geo.ncell + 1) * 4);
+  cb.add(s.base, *v.base, (size_t)(c->geo.ncell + 1) * 4);
+  cb.add(s.cnt, *v.cnt, (size_t)(c->geo.ncell + 1) * 4);
   cb.flush();
   s.b_is_a0 = v.fs->B[0] == v.fs->Bfix[0];
   s.t = t;
@@ -1118,7 +1125,7 @@ void snap32_take(emo_ctx *c, Snap32 &s, double t, bool particles) {
 // at tape buffers), fields, ranges, B roles; the stored gamma by reference
 void snap32_restore(emo_ctx *c, const Snap32 &s, int4 *pa, int4 *pb) {
   Fp32View v = fp32_view(c);
-  const size_t np = (size_t)c->np, nt4 = (size_t)c->geo.ntot * 4;
+  const size_t np = (size_t)v.nslots, nt4 = (size_t)c->geo.ntot * 4;
   CopyBatch cb(c);
   cb.add(pa, s.pa, np * 16);
   cb.add(pb, s.pb, np * 16);
@@ -1128,7 +1135,8 @@ void snap32_restore(emo_ctx *c, const Snap32 &s, int4 *pa, int4 *pb) {
     cb.add(v.Btfix[d], s.A[1][d], nt4);
     if (s.J[d]) cb.add(v.fs->J[d], s.J[d], nt4);
   }
-  cb.add(*v.off, s.off, (size_t)(c->geo.ncell + 1) * 4);
+  cb.add(*v.base, s.base, (size_t)(c->geo.ncell + 1) * 4);
+  cb.add(*v.cnt, s.cnt, (size_t)(c->geo.ncell + 1) * 4);
   cb.flush();
   *v.pa = pa;
   *v.pb = pb;
@@ -1166,10 +1174,12 @@ struct AdjRun32 {
   float *g0b = nullptr;
   float *Jb32[3] = {};
   std::vector<FillPlane> planes;
-  int4 *own[4];  // the context's own record buffers (pa, pb, pa_alt, pb_alt)
+  int4 *own[4] = {};  // the context's own record buffers (pa, pb, pa_alt, pb_alt)
+  int4 *stage_a = nullptr, *stage_b = nullptr;  // the recomputed movers' new records
 };
 
 void restore_own_buffers(AdjRun32 &R) {
+  if (!R.own[0]) return;  // never replaced
   Fp32View v = fp32_view(R.c);
   *v.pa = R.own[0];
   *v.pb = R.own[1];
@@ -1193,7 +1203,7 @@ int recompute32(AdjRun32 &R, int seg, int set) {
     snap32_take(c, tp[st - s0], t, false);
     *v.pa_alt = tp[st - s0 + 1].pa;
     *v.pb_alt = tp[st - s0 + 1].pb;
-    rc = fp32_step_one(c, t, R.dt, tp[st - s0].fwd);
+    rc = fp32_step_one(c, t, R.dt, tp[st - s0].fwd, R.stage_a, R.stage_b);
     t += R.dt;
   }
   restore_own_buffers(R);
@@ -1223,7 +1233,8 @@ void backward32(AdjRun32 &R, int seg, int set) {
     ca.pa = pre.pa;
     ca.pb = pre.pb;
     ca.g0 = (st == 0 && R.ck[0].has_g0) ? R.ck[0].g0 : nullptr;
-    ca.off = pre.off;
+    ca.base = pre.base;
+    ca.cnt = pre.cnt;
     ca.fwd = pre.fwd;
     for (int d = 0; d < 3; d++) {
       ca.E[d] = pre.E[d];
@@ -1236,6 +1247,7 @@ void backward32(AdjRun32 &R, int seg, int set) {
     ca.g0b = ca.g0 ? R.g0b : nullptr;
     ca.cg = R.af.cg;
     ca.ncell = g.ncell;
+    ca.nslots = (uint32_t)fp32_view(c).nslots;
     const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
     {
       TimedLaunch tl(c, "adj_cells_f32", as);
@@ -1290,7 +1302,7 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
   const int nsets = R.nseg > 1 ? 2 : 1;
   for (int st = 0; st < nsets; st++) R.tape[st].resize(K + 1);
   const size_t nt = (size_t)g.ntot;
-  const size_t npa = (size_t)std::max<int64_t>(np, 1);
+  const size_t npa = (size_t)v.nslots;  // particle arrays are by slot
   const int nplanes = (int)R.planes.size();
   const bool has_g0 = *v.g0_active && rel;
   auto layout = [&](Pool &P) {
@@ -1312,6 +1324,8 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
     P.take(&R.af.cg, 18 * (size_t)g.ncell);
     P.take(&R.dsums, (size_t)std::max(1, nsteps * nplanes));
     P.take(&R.g0b, npa);
+    P.take(&R.stage_a, npa);
+    P.take(&R.stage_b, npa);
   };
   {
     Pool dry;
@@ -1355,14 +1369,14 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
       R.own[2] = *v.pa_alt;
       R.own[3] = *v.pb_alt;
       rc = recompute32(R, seg, last_set);
-      if (!rc && np > 0) {
+      if (!rc) {
         CopyBatch cb(c);
-        cb.add(R.own[0], R.tape[last_set][s1 - s0].pa, (size_t)np * 16);
-        cb.add(R.own[1], R.tape[last_set][s1 - s0].pb, (size_t)np * 16);
+        cb.add(R.own[0], R.tape[last_set][s1 - s0].pa, npa * 16);
+        cb.add(R.own[1], R.tape[last_set][s1 - s0].pb, npa * 16);
         cb.flush();
       }
     } else {
-      for (int st = s0; st < s1 && !rc; st++) rc = fp32_step_one(c, tstep[st], dt, nullptr);
+      for (int st = s0; st < s1 && !rc; st++) rc = fp32_step_one(c, tstep[st], dt, nullptr, nullptr, nullptr);
     }
     t = tstep[s1];
   }
@@ -1463,36 +1477,43 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
     // particle adjoints of the initial state, by id: the offsets f = (x -
     // x0) / dx - ix and, relativistic, u = gamma0 v with the stored gamma0
     if ((xbar0 || vbar0 || gbar0) && np > 0) {
-      std::vector<int4> rb(np);
+      // the initial state's slots: cell c holds cnt[c] particles from base[c]
+      const size_t nc1 = (size_t)g.ncell + 1;
+      std::vector<int32_t> hbase(nc1), hcnt(nc1);
+      EMO_HIP_CHECK_RET(c, hipMemcpy(hbase.data(), R.ck[0].base, nc1 * 4, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, hipMemcpy(hcnt.data(), R.ck[0].cnt, nc1 * 4, hipMemcpyDeviceToHost));
+      std::vector<int4> rb(npa);
       std::vector<float> g0h;
       std::vector<float> fb[3], ub[3], g0bh;
-      EMO_HIP_CHECK_RET(c, hipMemcpy(rb.data(), R.ck[0].pb, np * 16, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, hipMemcpy(rb.data(), R.ck[0].pb, npa * 16, hipMemcpyDeviceToHost));
       for (int d = 0; d < 3; d++) {
-        fb[d].resize(np);
-        ub[d].resize(np);
-        EMO_HIP_CHECK_RET(c, hipMemcpy(fb[d].data(), R.pa.f[d], np * 4, hipMemcpyDeviceToHost));
-        EMO_HIP_CHECK_RET(c, hipMemcpy(ub[d].data(), R.pa.u[d], np * 4, hipMemcpyDeviceToHost));
+        fb[d].resize(npa);
+        ub[d].resize(npa);
+        EMO_HIP_CHECK_RET(c, hipMemcpy(fb[d].data(), R.pa.f[d], npa * 4, hipMemcpyDeviceToHost));
+        EMO_HIP_CHECK_RET(c, hipMemcpy(ub[d].data(), R.pa.u[d], npa * 4, hipMemcpyDeviceToHost));
       }
       if (has_g0) {
-        g0h.resize(np);
-        g0bh.resize(np);
-        EMO_HIP_CHECK_RET(c, hipMemcpy(g0h.data(), R.ck[0].g0, np * 4, hipMemcpyDeviceToHost));
-        EMO_HIP_CHECK_RET(c, hipMemcpy(g0bh.data(), R.g0b, np * 4, hipMemcpyDeviceToHost));
+        g0h.resize(npa);
+        g0bh.resize(npa);
+        EMO_HIP_CHECK_RET(c, hipMemcpy(g0h.data(), R.ck[0].g0, npa * 4, hipMemcpyDeviceToHost));
+        EMO_HIP_CHECK_RET(c, hipMemcpy(g0bh.data(), R.g0b, npa * 4, hipMemcpyDeviceToHost));
       }
-      for (int64_t i = 0; i < np; i++) {
-        const int64_t k = rb[i].w;
-        if (k < 0 || k >= np) continue;
-        float u[3];
-        std::memcpy(u, &rb[i], 12);
-        const double gm = has_g0 ? (double)g0h[i] : 1.0;
-        double gb = has_g0 ? (double)g0bh[i] : 0.0;
-        for (int d = 0; d < 3; d++) {
-          if (xbar0) xbar0[3 * k + d] = (double)fb[d][i] * g.id[d];
-          if (vbar0) vbar0[3 * k + d] = (double)ub[d][i] * (rel ? gm : 1.0);
-          gb += (double)ub[d][i] * ((double)u[d] / gm);
+      for (long long cell = 0; cell < g.ncell; cell++)
+        for (int j = 0; j < hcnt[cell]; j++) {
+          const int64_t i = (int64_t)hbase[cell] + j;
+          const int64_t k = rb[i].w;
+          if (k < 0 || k >= np) continue;
+          float u[3];
+          std::memcpy(u, &rb[i], 12);
+          const double gm = has_g0 ? (double)g0h[i] : 1.0;
+          double gb = has_g0 ? (double)g0bh[i] : 0.0;
+          for (int d = 0; d < 3; d++) {
+            if (xbar0) xbar0[3 * k + d] = (double)fb[d][i] * g.id[d];
+            if (vbar0) vbar0[3 * k + d] = (double)ub[d][i] * (rel ? gm : 1.0);
+            gb += (double)ub[d][i] * ((double)u[d] / gm);
+          }
+          if (gbar0) gbar0[k] = rel ? gb : 0.0;
         }
-        if (gbar0) gbar0[k] = rel ? gb : 0.0;
-      }
     }
   }
   // ---- the final state back (the records stayed in the own buffers) --------
@@ -1508,7 +1529,8 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
       cb.add(w.Btfix[d], R.fin.A[1][d], nt4);
       cb.add(w.fs->J[d], R.fin.J[d], nt4);
     }
-    cb.add(*w.off, R.fin.off, (size_t)(g.ncell + 1) * 4);
+    cb.add(*w.base, R.fin.base, (size_t)(g.ncell + 1) * 4);
+    cb.add(*w.cnt, R.fin.cnt, (size_t)(g.ncell + 1) * 4);
     cb.flush();
     for (int q = 0; q < 3; q++) {
       w.fs->B[q] = R.fin.b_is_a0 ? w.fs->Bfix[q] : w.Btfix[q];

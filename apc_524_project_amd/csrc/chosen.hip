@@ -1,14 +1,19 @@
 // chosen.hip -- the FP32 chosen form of the PIC step for gfx950.
 //
-// Particles are kept sorted by cell as two arrays of 16-byte records:
+// Particles are two arrays of 16-byte records:
 //   A = {word, fx, fy, fz}: word holds the species in the top 6 bits and the
 //       reference's ghost-inclusive cell id ((ny+2)(nz+2) ix + (nz+2) iy + iz,
 //       src/grid/oGrid.cpp:208-222) in the low 26; f in [0,1)^3 is the
 //       in-cell offset (FP32);
 //   B = {ux, uy, uz, id}: u (FP32) is v for Boris and gamma*v for the
-//       relativistic pusher; id is the particle's int32 id (-1 if none).
-// 32 bytes per particle, each record one 16-byte vector access.  Cell c holds
-// the particles [off[c], off[c+1]).
+//       relativistic pusher; id is the particle's int32 id.
+// 32 bytes per particle, each record one 16-byte vector access.
+//
+// Slot layout: cell c owns the slots [base[c], base[c+1]) -- its count plus
+// headroom, each range starting on a 128-byte boundary -- and holds its
+// cnt[c] particles in [base[c], base[c] + cnt[c]), in ascending cell order
+// overall.  The layout is rebuilt only when a cell's arrivals exceed its
+// headroom.
 //
 // The algorithm is the reference's (same step order, same defects); only the
 // representation and precision change.  With x = x0 + dx (ix - 1 + f) the
@@ -16,41 +21,34 @@
 // relpos / l = f - 1, so its linear weights (l - r)/l, r/l become 2 - f and
 // f - 1 (src/particles/interpolate.cpp:25-33, src/particles/deposit.cpp:94-97).
 //
-// Per step, with the state double-buffered (old -> new):
+// Per step, from the record buffers S (before) to D (after):
 //   cellvec_f32       per cell, the 18 values the gather reads (E on the 4
-//                     edges of each component, (B + B_tm1)/2 on the 2 faces):
-//                     five 16-byte loads per particle instead of 24 scattered
-//                     ones
-//   push_rank_f32     source order, one wavefront per 4 source cells: gather
-//                     + push + particle BCs of every particle, which yield its
-//                     move code (which of the <= 6 x 6 x 6 one-cell moves,
-//                     periodic wraps included, took it to its new cell) and
-//                     its rank among the particles of its (source cell, code)
-//                     group; the group sizes.  Nothing else is written.
-//   far_f32           (only when some particle moved further than one cell)
-//                     an ordered list of those particles
-//   base_f32, scan    per destination cell, the offsets of its (source,
-//                     code) groups in ascending source id, its count, the new
-//                     cell ranges
-//   inv_f32           source order: new position off[dst] + base + rank of
-//                     every particle -> the inverse permutation
-//   move_deposit_f32  destination order, 16 lanes per cell: each new slot
-//                     takes its particle from the old state, repeats the
-//                     push of push_rank_f32 (the same function on the same
-//                     inputs, so the same result bit for bit; a disagreement
-//                     on the cell would be reported), writes the new state
-//                     and sums the 12 edge currents with compensated
-//                     (two-float) accumulation; a fixed butterfly combines
-//                     the lanes: deterministic, no atomics
+//                     edges of each component, (B + B_tm1)/2 on the 2 faces)
+//   push_deposit_f32  16 lanes per cell: gather + push + particle BCs of
+//                     every particle, once.  A particle that stays in its
+//                     cell (~93% at 64 particles per cell) is written to D at
+//                     the cell's next free slot (in order) and its 12 edge
+//                     currents are summed with compensated (two-float)
+//                     accumulation, combined over the lanes by a fixed
+//                     butterfly.  A particle that leaves writes its new
+//                     record over its own slot of S, its velocity beside it,
+//                     and its slot to the movers' list; it counts itself in
+//                     its destination's arrivals.
+//   check_f32         does every cell's stayers + arrivals fit its range?
+//   rebase_f32,       only when one does not: a new layout from the new
+//   push_deposit_redo counts, the stayers written again at their new slots
+//   place_f32         each mover reserves a slot in its destination's
+//                     arrival range (integer atomics: provisional order)
+//   arrive_f32        per cell, the arrivals in ascending order of their old
+//                     slot (a canonical order, independent of the atomics):
+//                     records to D, their currents added to the cell's sum
+//   commit_f32        the rebuilt layout becomes current
 //   jsum_f32          per grid point, the fixed-order sum of the cell vectors
 //   fold, Yee E/B, ghost refresh   shared FP32 instances of the grid kernels
-// The permutation is the stable counting sort of the previous order by the
-// new cell.  Repeating the push costs arithmetic but saves writing and
-// re-reading an intermediate state: particle traffic is 32 B read in the
-// first pass and 4 + 32 + 32 B in the last.  Both particle passes run
-// software-pipelined (the next particle's loads are in flight during a
-// push), with the per-species constants in LDS, and are specialised for
-// all-periodic particle BCs.
+// Within a cell the new order is the stayers in their previous order, then
+// the arrivals in their previous order: deterministic, with no atomics on
+// floating-point data.  Particle traffic is 32 B read and 32 B written per
+// particle, plus about 90 B per mover.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -68,11 +66,12 @@
 namespace emo {
 
 struct ChosenState {
-  int64_t cap = 0;
+  int64_t cap = 0;        // particle capacity
+  int64_t nslots = 0;     // slots of each record buffer
   int4 *pa = nullptr, *pa_alt = nullptr;  // {word, f[3]}
   int4 *pb = nullptr, *pb_alt = nullptr;  // {u[3], id}
   bool have_id = false;
-  float *g0 = nullptr;  // stored gamma for the first relativistic push (Q3)
+  float *g0 = nullptr;  // stored gamma for the first relativistic push (Q3), by slot
   bool g0_active = false;
   bool g0_owned = true;  // g0 is this state's allocation (not a snapshot's)
   // the step deposits J only; rho (an output, not read by the step) is
@@ -80,10 +79,22 @@ struct ChosenState {
   bool rho_stale = false;
   // non-relativistic forms: the loaded gamma by particle id.  No Boris step
   // reads or updates it (the reference keeps the loader's value, Q3); it is
-  // only handed back.  Kept in id order, so the sort never moves it.
+  // only handed back.  Kept in id order, so the layout never moves it.
   float *gam_id = nullptr;
-  CellSortBufs srt;  // the counting sort's buffers (cellsort.hpp); srt.perm is the inverse permutation
-  int32_t *fdst = nullptr;  // destination cell of a far mover
+  // slot layout (ncell + 1 bases, ncell counts) and its rebuild target
+  int32_t *base = nullptr, *base_new = nullptr, *cnt = nullptr;
+  // per step: stayers and arrivals per cell, placement counters, the
+  // movers' list and count, their velocities by source slot, the arrival
+  // ranges' provisional source slots; lay[0] = rebuild pending, lay[1] =
+  // rebuilds so far
+  int32_t *ns = nullptr, *arr = nullptr, *put = nullptr, *list = nullptr, *nmv = nullptr;
+  float4 *mv = nullptr;
+  int32_t *akey = nullptr, *akey2 = nullptr;
+  int32_t *lay = nullptr;
+  // per-cell dense offsets (readback in array order) and scan scratch
+  int32_t *dcount = nullptr, *doff = nullptr;
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
   // initial sort of loaded particles (arbitrary order)
   uint32_t *key = nullptr, *key_alt = nullptr, *perm = nullptr, *perm_alt = nullptr;
   void *sort_tmp = nullptr;
@@ -108,6 +119,22 @@ __device__ inline unsigned xcd_tile() {
   return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
 }
 inline unsigned xcd_grid(unsigned blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
+
+// A cell's slot range for n particles: n, a quarter more and 8, rounded up
+// to 8 slots (so every range starts on a 128-byte boundary of both record
+// arrays); the tight form (n rounded up, plus 8) when the generous layout
+// does not fit the buffers.  The buffers hold 1.25 cap + 16 ncell slots, so
+// the tight layout always fits.
+__host__ __device__ inline long long slot_cap(long long n, bool tight) {
+  return tight ? ((n + 7) & ~7ll) + 8 : (n + (n >> 2) + 8 + 7) & ~7ll;
+}
+inline long long slots_for(long long cap_particles, long long ncell) {
+  return ((cap_particles + (cap_particles >> 2) + 7) & ~7ll) + 16 * ncell + 64;
+}
+
+constexpr int kCellLanes = 16;                   // lanes per cell
+constexpr int kTileCells = kBlock / kCellLanes;  // cells per block
+constexpr int kArrFast = 64;                     // arrivals a cell sorts in LDS
 
 // The gather vector of a cell (Grid::getFieldInterpolatorVec,
 // src/grid/oGrid.cpp:153-197, without the vertex): the 4 edge values of each
@@ -151,263 +178,8 @@ __global__ void __launch_bounds__(kBlock) k_cellvec_f32(CellVecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Pass 1: push, move codes, ranks and group sizes.  One wavefront owns 8
-// consecutive source cells, so the ranks within each (source cell, code)
-// group are complete when it finishes.
+// Compensated deposit helpers
 // ---------------------------------------------------------------------------
-constexpr int kCellsPerWave = 8;  // source cells per wavefront of the inverse-permutation pass
-constexpr int kPushCells = 4;     // source cells per wavefront of the push pass
-
-struct PushRankArgs {
-  PushK k;
-  const int4 *pa, *pb;
-  const float *g0;
-  const int32_t *off;
-  long long ncell;
-  uint8_t *code;
-  uint16_t *rank;
-  uint16_t *nsk;   // [code * ncell + source]
-  int32_t *fdst;   // destination cells of far movers
-  int32_t *nfar;
-  unsigned long long *flag;
-};
-
-template <bool REL, bool PER>
-__global__ void __launch_bounds__(kBlock) k_push_rank_f32(PushRankArgs a) {
-  __shared__ int cnt[kBlock / 64][kPushCells * kCodes];
-  __shared__ int bnd[kBlock / 64][kPushCells + 1];
-  __shared__ float4 cvs[kBlock / 64][kPushCells * kCellVec];  // the wavefront's gather vectors
-  __shared__ float qp_s[EMO_NSPEC];
-  // species constants in LDS: a per-lane load from the argument block would
-  // make every iteration wait for all loads in flight
-  if (threadIdx.x < EMO_NSPEC) qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long long s0 = ((long long)xcd_tile() * (kBlock / 64) + wv) * kPushCells;
-  if (s0 >= a.ncell) return;
-  const int nc = (int)min((long long)kPushCells, a.ncell - s0);
-  int *ct = cnt[wv];
-  for (int e = lane; e < nc * kCodes; e += 64) ct[e] = 0;
-  if (lane <= nc) bnd[wv][lane] = a.off[s0 + lane];
-  // every particle of this wavefront lies in its own nc source cells: their
-  // gather vectors are read once, into LDS
-  if (lane < nc * kCellVec) cvs[wv][lane] = a.k.cv[s0 * kCellVec + lane];
-  wave_sync();
-  const int pb = bnd[wv][0], pe = bnd[wv][nc];
-  // the inner cell boundaries, wave-uniform (scalar registers); past the
-  // last cell they never compare below a particle index
-  int sb[kPushCells - 1];
-#pragma unroll
-  for (int t = 1; t < kPushCells; t++) sb[t - 1] = __builtin_amdgcn_readfirstlane(t < nc ? bnd[wv][t] : INT_MAX);
-  int nfar = 0;
-  const bool use_g0 = a.g0 != nullptr;
-  // software pipeline: the next chunk's states are in flight during a push.
-  // Lanes past the end load (and push) the last particle again, so that the
-  // loads are unconditional; only their stores are skipped.
-  int4 A = make_int4(0, 0, 0, 0), Bv = A;
-  float G = 0.f;
-  if (pb < pe) {
-    const int q = min(pb + lane, pe - 1);
-    A = a.pa[q];
-    Bv = a.pb[q];
-    if (use_g0) G = a.g0[q];
-  }
-  for (int p0 = pb; p0 < pe; p0 += 64) {
-    const int p = p0 + lane;
-    const bool valid = p < pe;
-    // the source cell of the state held (lanes past the end hold the last
-    // particle)
-    const int pq = min(p, pe - 1);
-    int j = 0;
-#pragma unroll
-    for (int t = 0; t < kPushCells - 1; t++) j += sb[t] <= pq ? 1 : 0;
-    const float4 *cvl = cvs[wv] + j * kCellVec;
-    const CellVec V{cvl[0], cvl[1], cvl[2], cvl[3], cvl[4]};
-    const int pn = min(p + 64, pe - 1);
-    const int4 An = a.pa[pn], Bn = a.pb[pn];
-    float Gn = 0.f;
-    if (use_g0) Gn = a.g0[pn];
-    const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
-    Pushed o;
-    push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
-                       qp_s[sp], use_g0, G, o, a.flag);
-    const int mc = valid ? o.code : (int)kFarCode;
-    if (valid) {
-      if (mc == kFarCode) a.fdst[p] = o.cell;
-    }
-    A = An;
-    Bv = Bn;
-    G = Gn;
-    const bool ranked = valid && mc != kFarCode;
-    const int myrank = wave_rank(j * kCodes + mc, ranked, ct, lane);
-    if (valid) {
-      a.code[p] = (uint8_t)mc;
-      if (ranked) {
-        if (myrank > kMaxGroup) atomicOr(a.flag, kFlagMigrate);
-        a.rank[p] = (uint16_t)myrank;
-      }
-    }
-    nfar += __popcll(__ballot(valid && mc == kFarCode));
-  }
-  wave_sync();
-  for (int e = lane; e < nc * kCodes; e += 64) {
-    const int v = ct[e];
-    if (v) a.nsk[(long long)(e % kCodes) * a.ncell + s0 + e / kCodes] = (uint16_t)min(v, kMaxGroup);
-  }
-  if (lane == 0 && nfar) atomicAdd(a.nfar, nfar);
-}
-
-// source and destination cells of a far mover (far-mover list)
-struct CellsF32 {
-  const int4 *pa;
-  const int32_t *fdst;
-  __device__ void operator()(long long i, int &s, int &d) const {
-    s = pa[i].x & kCellMask;
-    d = fdst[i];
-  }
-};
-
-// ---------------------------------------------------------------------------
-// Pass 2: the inverse permutation.  Same wavefront-to-cells mapping as pass 1;
-// particle p of source cell s with move code k goes to
-// off_new[s + shift(k)] + base[k][s] + rank[p].
-// ---------------------------------------------------------------------------
-struct InvArgs {
-  Geo g;
-  CellSortBufs b;
-  const int32_t *fdst;
-  int per[3];
-  uint32_t *inv;
-  unsigned long long *flag;
-};
-
-constexpr int kInvUnroll = 4;  // particles per lane in flight (2 and 8 measured the same)
-constexpr int kStayCode = 43;  // (1 * 6 + 1) * 6 + 1: no move in any dimension
-
-// New slot of particle p (move code k, rank r) of source cell src, or -1
-// (flagged) if the group tables do not hold it.
-__device__ inline long long inv_pos(const InvArgs &a, const int *shift, int src, int p, int k, int r) {
-  const Geo &g = a.g;
-  long long pos;
-  if (k != kFarCode) {
-    const int dst = src + shift[k];
-    pos = -1;
-    if (dst >= 0 && dst < g.ncell) {
-      pos = (long long)a.b.off_new[dst] + a.b.gbase[(long long)k * g.ncell + src] + r;
-      if (pos >= a.b.off_new[dst + 1]) pos = -1;
-    }
-    if (pos < 0) atomicOr(a.flag, kFlagMigrate);
-  } else {
-    pos = cellsort_pos(g, a.b, p, k, a.fdst[p], a.flag);
-  }
-  return pos;
-}
-
-// The particles that stay in their cell (most of them) take the cell's
-// stayer base plus their rank, from a per-wavefront table; the others are
-// collected in LDS and placed 64 at a time, one per lane, so that their
-// dependent table loads are not waited for once per particle batch.
-__global__ void __launch_bounds__(kBlock) k_inv_f32(InvArgs a) {
-  __shared__ int bnd[kBlock / 64][kCellsPerWave + 1];
-  __shared__ int stay0[kBlock / 64][kCellsPerWave];  // first new slot of each source cell's stayers
-  __shared__ int stay1[kBlock / 64][kCellsPerWave];  // end of that cell's new range
-  __shared__ int mov[kBlock / 64][128];              // movers waiting for their slot
-  __shared__ int shift[kCodes];                      // cell-id shift of each move code
-  const Geo &g = a.g;
-  for (int k = threadIdx.x; k < kCodes; k += kBlock) {
-    const int kx = k / 36, ky = (k / 6) % 6, kz = k % 6;
-    shift[k] = (move_sigma(kx, g.n[0]) * g.nc[1] + move_sigma(ky, g.n[1])) * g.nc[2] + move_sigma(kz, g.n[2]);
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long long s0 = ((long long)xcd_tile() * (kBlock / 64) + wv) * kCellsPerWave;
-  if (s0 >= g.ncell) return;
-  const int nc = (int)min((long long)kCellsPerWave, g.ncell - s0);
-  if (lane <= nc) bnd[wv][lane] = a.b.off[s0 + lane];
-  if (lane < nc) {
-    // (the base is stale for a cell without stayers; it is then not read)
-    stay0[wv][lane] = a.b.off_new[s0 + lane] + a.b.gbase[(long long)kStayCode * g.ncell + s0 + lane];
-    stay1[wv][lane] = a.b.off_new[s0 + lane + 1];
-  }
-  wave_sync();
-  const int pb = bnd[wv][0], pe = bnd[wv][nc];
-  // the inner cell boundaries in scalar registers (past the last cell they
-  // never compare below a particle index)
-  int sb[kCellsPerWave - 1];
-#pragma unroll
-  for (int t = 1; t < kCellsPerWave; t++) sb[t - 1] = __builtin_amdgcn_readfirstlane(t < nc ? bnd[wv][t] : INT_MAX);
-  auto cell_of = [&](int p) {
-    int j = 0;
-#pragma unroll
-    for (int t = 0; t < kCellsPerWave - 1; t++) j += sb[t] <= p ? 1 : 0;
-    return j;
-  };
-  // the first n (<= 64) waiting movers, one per lane
-  auto place_movers = [&](int n) {
-    if (lane < n) {
-      const int p = mov[wv][lane];
-      const long long pos = inv_pos(a, shift, (int)s0 + cell_of(p), p, (int)a.b.code[p], (int)a.b.rank[p]);
-      if (pos >= 0) a.inv[pos] = (uint32_t)p;
-    }
-  };
-  int nmov = 0;  // movers waiting in mov[wv] (wavefront-uniform, below 64 between batches)
-  // a wavefront-uniform loop: every lane takes part in the batches
-  // each lane takes 4 consecutive particles: their codes are one 4-byte
-  // load and their ranks one 8-byte load, from 4-aligned indices (the first
-  // chunk starts at pb rounded down; indices outside [pb, pe) are skipped)
-  static_assert(kInvUnroll == 4, "four particles per lane");
-  for (int q0 = pb & ~3; q0 < pe; q0 += 64 * kInvUnroll) {
-    const int p0 = q0 + 4 * lane;
-    uint32_t kw = 0xffffffffu;
-    uint2 rw = make_uint2(0u, 0u);
-    if (p0 < pe) {
-      kw = *(const uint32_t *)(a.b.code + p0);
-      rw = *(const uint2 *)(a.b.rank + p0);
-    }
-    int k[kInvUnroll], r[kInvUnroll];
-#pragma unroll
-    for (int u = 0; u < kInvUnroll; u++) {
-      const int p = p0 + u;
-      k[u] = (p >= pb && p < pe) ? (int)((kw >> (8 * u)) & 0xffu) : -1;
-      r[u] = (int)(((u < 2 ? rw.x : rw.y) >> (16 * (u & 1))) & 0xffffu);
-    }
-#pragma unroll
-    for (int u = 0; u < kInvUnroll; u++) {
-      const int p = p0 + u;
-      if (k[u] == kStayCode) {
-        const int j = cell_of(p);
-        const long long pos = (long long)stay0[wv][j] + r[u];
-        if (pos < stay1[wv][j])
-          a.inv[pos] = (uint32_t)p;
-        else
-          atomicOr(a.flag, kFlagMigrate);
-      }
-      const bool mover = k[u] >= 0 && k[u] != kStayCode;
-      const unsigned long long bal = __ballot(mover);
-      if (mover) mov[wv][nmov + __popcll(bal & lanes_below(lane))] = p;
-      nmov += __popcll(bal);
-      if (nmov >= 64) {
-        wave_sync();
-        place_movers(64);
-        wave_sync();
-        // the remainder (fewer than 64) moves to the front
-        if (lane < nmov - 64) mov[wv][lane] = mov[wv][64 + lane];
-        nmov -= 64;
-        wave_sync();
-      }
-    }
-  }
-  wave_sync();
-  place_movers(nmov);
-}
-
-// ---------------------------------------------------------------------------
-// Pass 3: move + deposit in destination order, 16 lanes per cell, 4 cells per
-// wavefront.  (8 lanes per cell, with a cheaper butterfly, measured 25%
-// slower on the 64^3, 64-particles-per-cell step.)
-// ---------------------------------------------------------------------------
-constexpr int kDepLanes = 16;
-
 __device__ inline void two_sum(float a, float b, float &s, float &e) {
   s = a + b;
   float bb = s - a;
@@ -463,22 +235,37 @@ __device__ inline void deposit_j(float hi[12], float lo[12], const float f[3], c
   }
 }
 
-struct MoveDepArgs {
+// ---------------------------------------------------------------------------
+// The step's particle pass.  16 lanes per cell, 16 cells per block; the
+// lanes take the cell's particles 16 at a time, in order, with the next 16
+// records in flight during a push.  The redo form (REDO) runs only after a
+// rebuild: it writes the stayers again, at the new layout's slots, from the
+// same inputs (the push is a deterministic function of them); it skips the
+// slots whose record a mover has already replaced (their cell is not the
+// slot's) and leaves currents, counts and the movers' data as they are.
+// ---------------------------------------------------------------------------
+struct PushDepArgs {
   PushK k;
-  const int4 *pa, *pb;  // old state, source order
-  const float *g0;
-  int4 *pa_out, *pb_out;
-  const uint32_t *inv;
-  const int32_t *off;  // new cell ranges
-  long long ncell;
-  long long np;
-  float *cj;
-  int32_t *nfar;  // the far-mover count, read by the earlier passes: reset here for the next step
+  const int4 *pa, *pb;        // records before the step, source layout
+  int4 *pa_out, *pb_out;      // records after the step (the stayers here)
+  int4 *ma, *mb;              // movers' new records, at their source slot
+  float4 *mv;                 // movers' velocities (deposit), at their source slot
+  const float *g0;            // the stored gamma of the first push, by source slot, or null
+  const int32_t *base, *cnt;  // source layout
+  const int32_t *base_out;    // layout of the output (base, or the rebuilt one)
+  int32_t *ns;                // stayers per cell
+  int32_t *arr;               // arrivals per cell (counted here, zero on entry)
+  int32_t *list, *nmv;        // movers' source slots, their number (zero on entry)
+  float *cj;                  // 12 per cell: the stayers' currents
+  uint32_t *fwd;              // with a tape: source slot -> new slot, or null
+  int32_t *lay;               // lay[0]: rebuild pending (cleared here; the redo runs only when set)
+  long long ncell, ntiles;
   unsigned long long *flag;
 };
 
-template <bool REL, bool PER>
-__global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
+template <bool REL, bool PER, bool REDO>
+__global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
+  if (REDO && a.lay[0] == 0) return;
   __shared__ float qp_s[EMO_NSPEC], jc_s[EMO_NSPEC];
   // species constants in LDS (a per-lane load from the argument block would
   // make every iteration wait for all loads in flight); test particles
@@ -487,79 +274,273 @@ __global__ void __launch_bounds__(kBlock) k_move_deposit_f32(MoveDepArgs a) {
     qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
     jc_s[threadIdx.x] = a.k.s.test[threadIdx.x] ? 0.f : a.k.s.jc[threadIdx.x];
   }
+  if (!REDO && blockIdx.x == 0 && threadIdx.x == 0) a.lay[0] = 0;
   __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) *a.nfar = 0;
-  const int sub = threadIdx.x & (kDepLanes - 1);
-  const long long c = (long long)xcd_tile() * (kBlock / kDepLanes) + threadIdx.x / kDepLanes;
-  float hi[12], lo[12];
+  const int sub = threadIdx.x & (kCellLanes - 1);
+  const int lane = threadIdx.x & 63;
+  const int gsh = lane & ~(kCellLanes - 1);  // the cell's first lane in the wavefront
+  const bool use_g0 = a.g0 != nullptr;
+  const long long first = REDO ? (long long)blockIdx.x : (long long)xcd_tile();
+  const long long stride = REDO ? (long long)gridDim.x : a.ntiles;
+  for (long long tile = first; tile < a.ntiles; tile += stride) {
+    const long long c = tile * kTileCells + (threadIdx.x >> 4);
+    const bool in = c < a.ncell;
+    const int b = in ? a.base[c] : 0;
+    const int n = in ? a.cnt[c] : 0;
+    const int bo = in ? a.base_out[c] : 0;
+    // the wavefront runs to its largest cell (ballots need every lane)
+    int nmax = n;
+    nmax = max(nmax, __shfl_xor(nmax, 16, 64));
+    nmax = max(nmax, __shfl_xor(nmax, 32, 64));
+    CellVec V{};
+    if (n > 0) V = load_cellvec(a.k, (int)c);
+    float hi[12], lo[12];
 #pragma unroll
-  for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
-  const int b = c < a.ncell ? a.off[c] : 0, e = c < a.ncell ? a.off[c + 1] : 0;
-  if (b < e) {  // (an empty cell reads nothing: with no particles, slot 0 holds no record)
-    // software pipeline: the source index two slots ahead and the state one
-    // slot ahead are in flight while a particle is pushed.  A source index
-    // outside the store (a slot that the permutation did not fill) is an
-    // error; it is replaced by 0 where it is used, so that no access leaves
-    // the store.  Loads past the cell's end read its last slot (b < e
-    // there), so that the loads are unconditional.
-    const uint32_t np = (uint32_t)a.np;
-    const bool use_g0 = a.g0 != nullptr;
-    const int last = e - 1;
-    int j = b + sub;
-    bool bad = false;
-    uint32_t src = 0u, src_next = 0u;
-    if (j < e) {
-      src = a.inv[j];
-      src_next = a.inv[min(j + kDepLanes, last)];
-    }
-    bad = src >= np;
-    src = bad ? 0u : src;
-    int4 A = a.pa[src], Bv = a.pb[src];
-    float G = 0.f;
-    if (use_g0) G = a.g0[src];
-    CellVec V = load_cellvec(a.k, A.x);
-    for (; j < e; j += kDepLanes) {
-      const uint32_t src_next2 = a.inv[min(j + 2 * kDepLanes, last)];
-      bad = bad || src_next >= np;
-      const uint32_t sn = src_next < np ? src_next : 0u;
-      const int4 An = a.pa[sn], Bn = a.pb[sn];
-      float Gn = 0.f;
-      if (use_g0) Gn = a.g0[sn];
+    for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
+    int nstay = 0;  // the same in all lanes of the cell
+    // loads past the cell's end read its last particle (a slot of the cell's
+    // own range even when it is empty), so that they are unconditional
+    const int last = b + max(n - 1, 0);
+    int4 A = a.pa[min(b + sub, last)], Bv = a.pb[min(b + sub, last)];
+    float G = use_g0 ? a.g0[min(b + sub, last)] : 0.f;
+    for (int j0 = 0; j0 < nmax; j0 += kCellLanes) {
+      const int jn = min(b + j0 + kCellLanes + sub, last);
+      const int4 An = a.pa[jn], Bn = a.pb[jn];
+      const float Gn = use_g0 ? a.g0[jn] : 0.f;
+      const int src = b + j0 + sub;
+      const bool valid = j0 + sub < n;
+      const bool live = valid && (!REDO || (A.x & kCellMask) == (int)c);
       const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
       Pushed o;
       push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
-                         qp_s[sp], use_g0, G, o, nullptr);
-      bad = bad || o.cell != (int)c;
-      // the next particle's gather vector: in flight during the deposit
-      V = load_cellvec(a.k, An.x);
-      a.pa_out[j] = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)c), as_i(o.f[0]), as_i(o.f[1]),
-                              as_i(o.f[2]));
-      a.pb_out[j] = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
-      deposit_j(hi, lo, o.f, o.v, jc_s[sp]);
-      src_next = src_next2;
+                              qp_s[sp], use_g0, G, o, (!REDO && valid) ? a.flag : nullptr);
+      const bool stay = live && o.cell == (int)c;
+      const bool move = live && !stay;
+      const unsigned gm = (unsigned)(__ballot(stay) >> gsh) & 0xffffu;
+      if (stay) {
+        const int dst = bo + nstay + __popc(gm & ((1u << sub) - 1u));
+        a.pa_out[dst] = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)c), as_i(o.f[0]), as_i(o.f[1]),
+                                  as_i(o.f[2]));
+        a.pb_out[dst] = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
+        if (a.fwd) a.fwd[src] = (uint32_t)dst;
+      }
+      nstay += __popc(gm);
+      if (!REDO) {
+        if (stay) deposit_j(hi, lo, o.f, o.v, jc_s[sp]);
+        const unsigned long long mbal = __ballot(move);
+        if (mbal) {
+          int mb0 = 0;
+          if (lane == 0) mb0 = atomicAdd(a.nmv, __popcll(mbal));
+          mb0 = __shfl(mb0, 0, 64);
+          if (move) {
+            a.ma[src] = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)o.cell), as_i(o.f[0]),
+                                  as_i(o.f[1]), as_i(o.f[2]));
+            a.mb[src] = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
+            a.mv[src] = make_float4(o.v[0], o.v[1], o.v[2], 0.f);
+            a.list[mb0 + __popcll(mbal & lanes_below(lane))] = src;
+            atomicAdd(&a.arr[o.cell], 1);
+          }
+        }
+      }
       A = An;
       Bv = Bn;
       G = Gn;
     }
-    if (bad) atomicOr(a.flag, kFlagMigrate);
-  }
-  // all 64 lanes take part (the DPP moves read every lane of the group)
-  static_assert(kDepLanes == 16, "the butterfly covers 16 lanes");
-  dep_reduce16(hi, lo);
-  if (sub == 0 && c < a.ncell) {
+    if (!REDO) {
+      // all 64 lanes take part (the DPP moves read every lane of the group)
+      static_assert(kCellLanes == 16, "the butterfly covers 16 lanes");
+      dep_reduce16(hi, lo);
+      if (sub == 0 && in) {
 #pragma unroll
-    for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = hi[q] + lo[q];
+        for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = hi[q] + lo[q];
+        a.ns[c] = nstay;
+      }
+    }
   }
 }
 
-// the sort as pre-push index -> new slot, from the inverse permutation
-__global__ void k_fwd_from_inv(const uint32_t *inv, uint32_t *fwd, long long np) {
-  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < np) fwd[inv[j]] = (uint32_t)j;
+// Does every cell's range hold its stayers and arrivals?  If not, a rebuild
+// is pending for this step.
+__global__ void k_check_f32(const int32_t *ns, const int32_t *arr, const int32_t *base, long long ncell,
+                            int32_t *lay) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncell && ns[c] + arr[c] > base[c + 1] - base[c]) atomicOr(&lay[0], 1);
+}
+
+// The rebuilt layout from the new counts (one block; runs only when a
+// rebuild is pending): generous ranges if they fit the buffers, else tight
+// ones (which always fit).
+constexpr int kRebaseThreads = 1024;
+__global__ void __launch_bounds__(kRebaseThreads) k_rebase_f32(const int32_t *ns, const int32_t *arr, long long ncell,
+                                                               long long nslots, int32_t *base_new, int32_t *lay,
+                                                               unsigned long long *flag) {
+  if (lay[0] == 0) return;
+  __shared__ long long sh[kRebaseThreads];
+  const int t = threadIdx.x;
+  const long long per = (ncell + kRebaseThreads - 1) / kRebaseThreads;
+  const long long c0 = min((long long)t * per, ncell), c1 = min(c0 + per, ncell);
+  for (int pass = 0; pass < 2; pass++) {
+    const bool tight = pass == 1;
+    long long s = 0;
+    for (long long c = c0; c < c1; c++) s += slot_cap(ns[c] + arr[c], tight);
+    sh[t] = s;
+    __syncthreads();
+    for (int o = 1; o < kRebaseThreads; o <<= 1) {
+      const long long v = t >= o ? sh[t - o] : 0;
+      __syncthreads();
+      sh[t] += v;
+      __syncthreads();
+    }
+    const long long total = sh[kRebaseThreads - 1];
+    if (total <= nslots) {
+      long long run = sh[t] - s;
+      for (long long c = c0; c < c1; c++) {
+        base_new[c] = (int32_t)run;
+        run += slot_cap(ns[c] + arr[c], tight);
+      }
+      if (t == kRebaseThreads - 1) {
+        base_new[ncell] = (int32_t)total;
+        lay[1] += 1;
+      }
+      return;
+    }
+    __syncthreads();
+  }
+  // (cannot happen: the buffers hold the tight layout of any counts) no
+  // rebuild, so that every later access stays in the current layout
+  if (t == 0) {
+    atomicOr(flag, kFlagMigrate);
+    lay[0] = 0;
+  }
+}
+
+// Each mover reserves a slot in its destination's arrival range, which
+// follows the destination's stayers.  The order the atomics give is only
+// provisional: arrive_f32 puts each range in a canonical order.
+__global__ void __launch_bounds__(kBlock) k_place_f32(const int32_t *list, const int32_t *nmv, const int4 *ma,
+                                                      const int32_t *base, const int32_t *base_new,
+                                                      const int32_t *lay, const int32_t *ns, int32_t *put,
+                                                      int32_t *akey, unsigned long long *flag) {
+  const int n = *nmv;
+  const int32_t *B = lay[0] ? base_new : base;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int src = list[i];
+    const int d = ma[src].x & kCellMask;
+    const int k = atomicAdd(&put[d], 1);
+    const int pos = B[d] + ns[d] + k;
+    if (pos < B[d + 1])
+      akey[pos] = src;
+    else
+      atomicOr(flag, kFlagMigrate);
+  }
+}
+
+// Per cell (16 lanes): the arrivals in ascending order of their old slot --
+// ranked in LDS for up to kArrFast arrivals, else ranked from global memory
+// into akey2 -- the rank-r arrival taken by lane r % 16: its record written after the stayers,
+// its currents summed as in the particle pass and added to the stayers' sum.
+// Then the cell's new count; the per-step counters are reset for the next
+// step.
+struct ArriveArgs {
+  const int32_t *base, *base_new, *lay;
+  const int32_t *ns;
+  int32_t *arr, *put, *cnt, *nmv;
+  const int32_t *akey;
+  int32_t *akey2;  // the sorted keys of the cells with more than kArrFast arrivals
+  const int4 *ma, *mb;
+  const float4 *mv;
+  int4 *pa_out, *pb_out;
+  uint32_t *fwd;
+  float *cj;
+  float jc[EMO_NSPEC];  // 0 for test particles
+  long long ncell;
+  unsigned long long *flag;
+};
+
+__global__ void __launch_bounds__(kBlock) k_arrive_f32(ArriveArgs a) {
+  __shared__ int keys[kTileCells][kArrFast];
+  __shared__ int sorted[kTileCells][kArrFast];
+  __shared__ float jc_s[EMO_NSPEC];
+  if (threadIdx.x < EMO_NSPEC) jc_s[threadIdx.x] = a.jc[threadIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.nmv = 0;
+  const int grp = threadIdx.x >> 4, sub = threadIdx.x & (kCellLanes - 1);
+  const long long c = (long long)blockIdx.x * kTileCells + grp;
+  const bool in = c < a.ncell;
+  const int32_t *B = a.lay[0] ? a.base_new : a.base;
+  int na = in ? a.arr[c] : 0;
+  const int s0 = in ? B[c] + a.ns[c] : 0;
+  if (in && s0 + na > B[c + 1]) {  // (place_f32 has flagged it)
+    atomicOr(a.flag, kFlagMigrate);
+    na = max(B[c + 1] - s0, 0);
+  }
+  const bool fast = na <= kArrFast;
+  if (fast)
+    for (int i = sub; i < na; i += kCellLanes) keys[grp][i] = a.akey[s0 + i];
+  __syncthreads();
+  if (fast) {
+    for (int i = sub; i < na; i += kCellLanes) {
+      const int k = keys[grp][i];
+      int r = 0;
+      for (int j = 0; j < na; j++) r += keys[grp][j] < k ? 1 : 0;
+      sorted[grp][r] = k;
+    }
+  } else {
+    for (int i = sub; i < na; i += kCellLanes) {
+      const int k = a.akey[s0 + i];
+      int r = 0;
+      for (int j = 0; j < na; j++) r += a.akey[s0 + j] < k ? 1 : 0;
+      a.akey2[s0 + r] = k;
+    }
+  }
+  // the block's global stores complete, then every lane reads past its L1
+  __threadfence();
+  __syncthreads();
+  __threadfence();
+  float hi[12], lo[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
+  auto take = [&](int src, int r) {
+    const int4 A = a.ma[src], Bv = a.mb[src];
+    const float4 v4 = a.mv[src];
+    const int pos = s0 + r;
+    a.pa_out[pos] = A;
+    a.pb_out[pos] = Bv;
+    if (a.fwd) a.fwd[src] = (uint32_t)pos;
+    const float f[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
+    const float v[3] = {v4.x, v4.y, v4.z};
+    deposit_j(hi, lo, f, v, jc_s[(uint32_t)A.x >> kSpeciesShift]);
+  };
+  if (fast) {
+    for (int r = sub; r < na; r += kCellLanes) take(sorted[grp][r], r);
+  } else {
+    for (int r = sub; r < na; r += kCellLanes) take(a.akey2[s0 + r], r);
+  }
+  dep_reduce16(hi, lo);
+  if (sub == 0 && in) {
+    if (na > 0) {
+#pragma unroll
+      for (int q = 0; q < 12; q++) {
+        float s, e;
+        two_sum(a.cj[q * a.ncell + c], hi[q], s, e);
+        a.cj[q * a.ncell + c] = s + (e + lo[q]);
+      }
+    }
+    a.cnt[c] = a.ns[c] + na;
+    a.arr[c] = 0;
+    a.put[c] = 0;
+  }
+}
+
+// The rebuilt layout becomes current (only after a rebuild).
+__global__ void k_commit_f32(int32_t *base, const int32_t *base_new, const int32_t *lay, long long ncell) {
+  if (lay[0] == 0) return;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c <= ncell;
+       c += (long long)gridDim.x * blockDim.x)
+    base[c] = base_new[c];
 }
 
 // ---------------------------------------------------------------------------
-// rho + J of the current (sorted) state, for the Poisson initialisation
+// rho + J of the current state, for the Poisson initialisation
 // (src/main.cpp:160-166): the 12 edge currents and the 8 vertex charges of
 // Depositor::deposit_particle_J / _Rho (src/particles/deposit.cpp:12-62)
 // with the weights 2 - f and f - 1, 16 lanes per cell, compensated sums and a
@@ -568,7 +549,7 @@ __global__ void k_fwd_from_inv(const uint32_t *inv, uint32_t *fwd, long long np)
 struct RhoJArgs {
   const int4 *pa, *pb;
   const float *g0;
-  const int32_t *off;
+  const int32_t *base, *cnt;
   SpeciesF32 s;
   int rel;
   long long ncell;
@@ -582,7 +563,7 @@ __global__ void __launch_bounds__(kBlock) k_deposit_rhoj_f32(RhoJArgs a) {
 #pragma unroll
   for (int q = 0; q < 20; q++) hi[q] = lo[q] = 0.f;
   if (c < a.ncell) {
-    const int b = a.off[c], e = a.off[c + 1];
+    const int b = a.base[c], e = b + a.cnt[c];
     for (int j = b + sub; j < e; j += 16) {
       const int4 A = a.pa[j], Bv = a.pb[j];
       const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
@@ -717,83 +698,120 @@ __global__ void __launch_bounds__(kBlock) k_permute_f32(PermF32Args a) {
   }
 }
 
+// The sorted loaded particles (dense, cell c at doff[c] .. doff[c] + cnt[c])
+// into the slot layout, with the stored gamma.
+struct ScatterArgs {
+  const int4 *pa, *pb;
+  const float *g0;
+  int4 *pa_out, *pb_out;
+  float *g0_out;
+  const int32_t *doff, *base;
+  long long np;
+};
+
+__global__ void __launch_bounds__(kBlock) k_scatter_slots_f32(ScatterArgs a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const int4 A = a.pa[i];
+  const int c = A.x & kCellMask;
+  const long long s = (long long)a.base[c] + (i - a.doff[c]);
+  a.pa_out[s] = A;
+  a.pb_out[s] = a.pb[i];
+  if (a.g0) a.g0_out[s] = a.g0[i];
+}
+
+// Per-cell iteration over the particles in array order (cell by cell, in
+// ascending cell order): 16 lanes per cell, particle k of cell c at slot
+// base[c] + k and array index doff[c] + k.
 struct OutArgs {
   Geo g;
   const int4 *pa, *pb;
   const float *g0;
   const float *gam_id;  // non-relativistic: the loaded gamma by id, or null
   const float4 *cv;     // with fld: the current grid's gather vectors
+  const int32_t *base, *cnt, *doff;
   int rel;
   double *x[3], *v[3], *gam;
   double *fld[6];       // the gathered E, B per particle (Field_part), or null
   int32_t *sp;
   int32_t *id;
+  int32_t *cell;
   long long np;
 };
 
 __global__ void __launch_bounds__(kBlock) k_output_f32(OutArgs a) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
+  const long long c = (long long)blockIdx.x * kTileCells + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & (kCellLanes - 1);
   const Geo &g = a.g;
-  const int4 A = a.pa[i], Bv = a.pb[i];
-  int cell = A.x & kCellMask;
+  if (c >= g.ncell) return;
+  const int n = a.cnt[c];
+  if (n == 0) return;
   int ic[3];
-  decode_cell(g, cell, &ic[0], &ic[1], &ic[2]);
-  const double uu[3] = {as_f(Bv.x), as_f(Bv.y), as_f(Bv.z)};
-  const double ff[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
-  double gam = 1.0;
-  if (a.rel) gam = a.g0 ? (double)a.g0[i] : sqrt(1.0 + uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2]);
-  for (int d = 0; d < 3; d++) {
-    if (a.x[d]) a.x[d][i] = g.x0[d] + g.d[d] * ((double)(ic[d] - 1) + ff[d]);
-    if (a.v[d]) a.v[d][i] = a.rel ? uu[d] / gam : uu[d];
-  }
-  if (!a.rel && a.gam_id && Bv.w >= 0 && Bv.w < a.np) gam = a.gam_id[Bv.w];
-  if (a.gam) a.gam[i] = gam;
-  if (a.fld[0]) {
-    const float4 *cv = a.cv + (long long)cell * kCellVec;
-    const CellVec V{cv[0], cv[1], cv[2], cv[3], cv[4]};
-    float e[3], b[3];
-    gather_cell(V, as_f(A.y), as_f(A.z), as_f(A.w), e, b);
+  decode_cell(g, (int)c, &ic[0], &ic[1], &ic[2]);
+  for (int k = sub; k < n; k += kCellLanes) {
+    const long long s = (long long)a.base[c] + k, i = (long long)a.doff[c] + k;
+    if (i >= a.np) break;  // (only if particles were lost, which is flagged)
+    const int4 A = a.pa[s], Bv = a.pb[s];
+    if (a.cell) a.cell[i] = (int32_t)c;
+    const double uu[3] = {as_f(Bv.x), as_f(Bv.y), as_f(Bv.z)};
+    const double ff[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
+    double gam = 1.0;
+    if (a.rel) gam = a.g0 ? (double)a.g0[s] : sqrt(1.0 + uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2]);
     for (int d = 0; d < 3; d++) {
-      a.fld[d][i] = e[d];
-      a.fld[3 + d][i] = b[d];
+      if (a.x[d]) a.x[d][i] = g.x0[d] + g.d[d] * ((double)(ic[d] - 1) + ff[d]);
+      if (a.v[d]) a.v[d][i] = a.rel ? uu[d] / gam : uu[d];
     }
+    if (!a.rel && a.gam_id && Bv.w >= 0 && Bv.w < a.np) gam = a.gam_id[Bv.w];
+    if (a.gam) a.gam[i] = gam;
+    if (a.fld[0]) {
+      const float4 *cv = a.cv + c * kCellVec;
+      const CellVec V{cv[0], cv[1], cv[2], cv[3], cv[4]};
+      float e[3], b[3];
+      gather_cell(V, as_f(A.y), as_f(A.z), as_f(A.w), e, b);
+      for (int d = 0; d < 3; d++) {
+        a.fld[d][i] = e[d];
+        a.fld[3 + d][i] = b[d];
+      }
+    }
+    if (a.sp) a.sp[i] = (int)((uint32_t)A.x >> kSpeciesShift);
+    if (a.id) a.id[i] = Bv.w;
   }
-  if (a.sp) a.sp[i] = (int)((uint32_t)A.x >> kSpeciesShift);
-  if (a.id) a.id[i] = Bv.w;
-}
-
-__global__ void k_cells_f32(const int4 *pa, int32_t *out, long long np) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < np) out[i] = pa[i].x & kCellMask;
 }
 
 struct DiagF32Args {
   const int4 *pa, *pb;
   const float *g0;
+  const int32_t *base, *cnt;
   int rel;
   SpeciesParams s;
-  long long np;
+  long long ncell;
   double *partials;
 };
 
+// FP64 sums over the particles (OutputBoxQuantities::output, src/IO/output.cpp:
+// 76-88), cell by cell, 16 lanes per cell, a fixed tree per block
 __global__ void __launch_bounds__(kBlock) k_diag_particles_f32(DiagF32Args a) {
   __shared__ double sh[5][kBlock];
   double acc[5] = {0, 0, 0, 0, 0};
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.np;
-       i += (long long)gridDim.x * blockDim.x) {
-    int sp = (int)((uint32_t)a.pa[i].x >> kSpeciesShift);
-    const int4 Bv = a.pb[i];
-    double m = a.s.m[sp];
-    double ux = as_f(Bv.x), uy = as_f(Bv.y), uz = as_f(Bv.z);
-    double ig = 1.0;
-    if (a.rel) ig = 1.0 / (a.g0 ? (double)a.g0[i] : sqrt(1.0 + ux * ux + uy * uy + uz * uz));
-    double vx = ux * ig, vy = uy * ig, vz = uz * ig;
-    acc[0] += m * vx;
-    acc[1] += m * vy;
-    acc[2] += m * vz;
-    acc[3] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
-    acc[4] += a.s.q[sp];
+  const int sub = threadIdx.x & (kCellLanes - 1);
+  for (long long c = (long long)blockIdx.x * kTileCells + (threadIdx.x >> 4); c < a.ncell;
+       c += (long long)gridDim.x * kTileCells) {
+    const int b = a.base[c], n = a.cnt[c];
+    for (int k = sub; k < n; k += kCellLanes) {
+      const int i = b + k;
+      const int sp = (int)((uint32_t)a.pa[i].x >> kSpeciesShift);
+      const int4 Bv = a.pb[i];
+      const double m = a.s.m[sp];
+      const double ux = as_f(Bv.x), uy = as_f(Bv.y), uz = as_f(Bv.z);
+      double ig = 1.0;
+      if (a.rel) ig = 1.0 / (a.g0 ? (double)a.g0[i] : sqrt(1.0 + ux * ux + uy * uy + uz * uz));
+      const double vx = ux * ig, vy = uy * ig, vz = uz * ig;
+      acc[0] += m * vx;
+      acc[1] += m * vy;
+      acc[2] += m * vz;
+      acc[3] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+      acc[4] += a.s.q[sp];
+    }
   }
   for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = acc[q];
   __syncthreads();
@@ -804,6 +822,18 @@ __global__ void __launch_bounds__(kBlock) k_diag_particles_f32(DiagF32Args a) {
   }
   if (threadIdx.x == 0)
     for (int q = 0; q < 5; q++) a.partials[5 * blockIdx.x + q] = sh[q][0];
+}
+
+// counts per cell -> slot ranges (before the scan into bases)
+__global__ void k_slot_caps(const int32_t *count, int32_t *caps, long long ncell, int tight) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncell) caps[c] = (int32_t)slot_cap(count[c], tight != 0);
+  if (c == ncell) caps[c] = 0;
+}
+
+__global__ void k_copy_i32(const int32_t *src, int32_t *dst, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 int set_err(emo_ctx *c, int code, const std::string &m) {
@@ -833,28 +863,70 @@ int check_flag(emo_ctx *c) {
   if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, hipGetErrorString(e));
   if (c->hflag[0] & kFlagOutOfGrid)
     return set_err(c, EMO_ERR_INVARIANT, "particle outside the padded grid");
-  if (c->hflag[0] & kFlagFarMovers)
-    return set_err(c, EMO_ERR_NUMERIC,
-                   "too many particles moved more than one cell in a step (|v| dt > dx): reduce dt");
-  if (c->hflag[0] & kFlagMigrate) return set_err(c, EMO_ERR_INVARIANT, "cell migration lost particles");
+  if (c->hflag[0] & kFlagMigrate)
+    return set_err(c, EMO_ERR_INVARIANT, "a moving particle found no slot in its new cell");
   return EMO_OK;
 }
 
-// Sorts freshly loaded particles (main arrays, any order) by cell, stably.
+// exclusive scan of n int32 values (device) into out
+int scan_i32(emo_ctx *c, const int32_t *in, int32_t *out, long long n) {
+  ChosenState *s = c->ch;
+  size_t bytes = s->scan_tmp_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(s->scan_tmp, bytes, in, out, (int)n, c->stream);
+  if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, std::string("scan: ") + hipGetErrorString(e));
+  return EMO_OK;
+}
+
+// the array index of each cell's first particle (readback in array order)
+int dense_offsets(emo_ctx *c) {
+  ChosenState *s = c->ch;
+  return scan_i32(c, s->cnt, s->doff, c->geo.ncell + 1);
+}
+
+// A layout for the counts count[0..ncell) (count[ncell] = 0): the generous
+// ranges if they fit the buffers, else the tight ones.  Writes s->base.
+int build_layout(emo_ctx *c, const int32_t *count) {
+  ChosenState *s = c->ch;
+  const long long nc = c->geo.ncell;
+  for (int tight = 0; tight < 2; tight++) {
+    hipLaunchKernelGGL(k_slot_caps, dim3(grid_for(nc + 1)), dim3(kBlock), 0, c->stream, count, s->base_new, nc, tight);
+    int rc = scan_i32(c, s->base_new, s->base, nc + 1);
+    if (rc) return rc;
+    int32_t total = 0;
+    EMO_HIP_CHECK_RET(c, hipMemcpyAsync(&total, s->base + nc, 4, hipMemcpyDeviceToHost, c->stream));
+    EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+    if (total >= 0 && total <= s->nslots) return EMO_OK;
+  }
+  return set_err(c, EMO_ERR_INVARIANT, "the slot layout does not fit the particle buffers");
+}
+
+// Lays out freshly loaded particles: s->pa / s->pb hold np records in any
+// order (and s->g0, when active, their stored gamma in the same order).
+// They are sorted by cell (radix sort of the cell ids, stable), then
+// scattered into the slot layout of their counts.
 int sort_loaded(emo_ctx *c) {
   ChosenState *s = c->ch;
   const long long np = c->np;
-  cs_fill_i32(c, s->srt.count, 0, c->geo.ncell + 1);
+  const long long nc = c->geo.ncell;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nc + 1)), dim3(kBlock), 0, c->stream, s->dcount, 0, nc + 1);
+  float *g0_dense = nullptr, *g0_slots = nullptr;
+  if (s->g0_active) {
+    int rc = dalloc(c, &g0_dense, std::max<long long>(np, 1));
+    if (!rc) rc = dalloc(c, &g0_slots, s->nslots);
+    if (rc) {
+      dfree(g0_dense);
+      return rc;
+    }
+  }
   if (np > 0) {
     hipLaunchKernelGGL(k_keys_from_pa, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, s->pa, s->key, s->perm, np);
     size_t bytes = s->sort_tmp_bytes;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->key, s->key_alt, s->perm,
                                                       s->perm_alt, (int)np, 0, s->sort_bits, c->stream);
-    if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, std::string("radix sort: ") + hipGetErrorString(e));
-    float *g0_new = nullptr;
-    if (s->g0_active) {
-      int rc = dalloc(c, &g0_new, np);
-      if (rc) return rc;
+    if (e != hipSuccess) {
+      dfree(g0_dense);
+      dfree(g0_slots);
+      return set_err(c, EMO_ERR_HIP, std::string("radix sort: ") + hipGetErrorString(e));
     }
     PermF32Args pa;
     pa.pa = s->pa;
@@ -862,23 +934,40 @@ int sort_loaded(emo_ctx *c) {
     pa.pa_out = s->pa_alt;
     pa.pb_out = s->pb_alt;
     pa.g0 = s->g0_active ? s->g0 : nullptr;
-    pa.g0_out = g0_new;
+    pa.g0_out = g0_dense;
     pa.perm = s->perm_alt;
-    pa.count = s->srt.count;
+    pa.count = s->dcount;
     pa.np = np;
     hipLaunchKernelGGL(k_permute_f32, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, pa);
-    std::swap(s->pa, s->pa_alt);
-    std::swap(s->pb, s->pb_alt);
-    if (g0_new) {
-      EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
-      dfree(s->g0);
-      s->g0 = g0_new;
-    }
   }
-  int rc = cellsort_scan(c, s->srt, s->srt.off, "scan_f32");
-  if (rc) return rc;
-  cs_fill_i32(c, s->srt.count, 0, c->geo.ncell + 1);
-  return EMO_OK;
+  int rc = scan_i32(c, s->dcount, s->doff, nc + 1);
+  if (!rc) rc = build_layout(c, s->dcount);
+  if (!rc && np > 0) {
+    ScatterArgs a;
+    a.pa = s->pa_alt;
+    a.pb = s->pb_alt;
+    a.g0 = g0_dense;
+    a.pa_out = s->pa;
+    a.pb_out = s->pb;
+    a.g0_out = g0_slots;
+    a.doff = s->doff;
+    a.base = s->base;
+    a.np = np;
+    hipLaunchKernelGGL(k_scatter_slots_f32, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, a);
+  }
+  if (!rc)
+    hipLaunchKernelGGL(k_copy_i32, dim3(grid_for(nc + 1)), dim3(kBlock), 0, c->stream, (const int32_t *)s->dcount,
+                       s->cnt, nc + 1);
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = set_err(c, EMO_ERR_HIP, "layout of the load failed");
+  dfree(g0_dense);
+  if (s->g0_active) {
+    if (s->g0_owned) dfree(s->g0);
+    s->g0 = g0_slots;
+    s->g0_owned = true;
+  } else {
+    dfree(g0_slots);
+  }
+  return rc;
 }
 
 PushK push_consts(emo_ctx *c, double dt) {
@@ -899,102 +988,127 @@ PushK push_consts(emo_ctx *c, double dt) {
   return k;
 }
 
-unsigned cell_wave_blocks(long long ncell, int cells_per_wave) {
-  const long long waves = (ncell + cells_per_wave - 1) / cells_per_wave;
-  return (unsigned)std::max<long long>((waves + kBlock / 64 - 1) / (kBlock / 64), 1);
+void launch_cellvec(emo_ctx *c) {
+  ChosenState *s = c->ch;
+  CellVecArgs a;
+  a.g = c->geo;
+  for (int d = 0; d < 3; d++) {
+    a.E[d] = s->fs.E[d];
+    a.B[d] = s->fs.B[d];
+    a.Bt[d] = s->fs.Bt[d];
+  }
+  a.cv = s->cv;
+  TimedLaunch t(c, "cellvec_f32");
+  hipLaunchKernelGGL(k_cellvec_f32, dim3(grid_for(c->geo.ncell)), dim3(kBlock), 0, c->stream, a);
 }
 
-// One step's particle work: push + ranks, far list, group offsets, new
-// ranges, inverse permutation, move + deposit, grid sum of the currents.
-int particle_step(emo_ctx *c, double dt) {
-  ChosenState *s = c->ch;
-  const Geo &g = c->geo;
-  const long long np = c->np;
-  {
-    CellVecArgs a;
-    a.g = g;
-    for (int d = 0; d < 3; d++) {
-      a.E[d] = s->fs.E[d];
-      a.B[d] = s->fs.B[d];
-      a.Bt[d] = s->fs.Bt[d];
-    }
-    a.cv = s->cv;
-    TimedLaunch t(c, "cellvec_f32");
-    hipLaunchKernelGGL(k_cellvec_f32, dim3(grid_for(g.ncell)), dim3(kBlock), 0, c->stream, a);
-  }
-  const PushK k = push_consts(c, dt);
-  const float *g0 = s->g0_active ? s->g0 : nullptr;
+template <bool REDO>
+void launch_push_deposit(emo_ctx *c, const PushDepArgs &a, dim3 grid) {
   bool all_per = true;
   for (int q = 0; q < 6; q++) all_per = all_per && c->cfg.parts_bound[q] == EMO_BC_PERIODIC;
+  if (c->cfg.relativity && all_per)
+    hipLaunchKernelGGL((k_push_deposit_f32<true, true, REDO>), grid, dim3(kBlock), 0, c->stream, a);
+  else if (c->cfg.relativity)
+    hipLaunchKernelGGL((k_push_deposit_f32<true, false, REDO>), grid, dim3(kBlock), 0, c->stream, a);
+  else if (all_per)
+    hipLaunchKernelGGL((k_push_deposit_f32<false, true, REDO>), grid, dim3(kBlock), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_push_deposit_f32<false, false, REDO>), grid, dim3(kBlock), 0, c->stream, a);
+}
+
+// One step's particle work, from the current record buffers into the
+// alternate ones (which then become current): gather vectors, push +
+// deposit, the layout check (and the rebuild it may need), mover placement,
+// arrivals, grid sum of the currents.  With fwd_out (a tape), the step's
+// map source slot -> new slot, and the movers' new records go to stage_a /
+// stage_b instead of over their source slots, which then stay untouched.
+int particle_step(emo_ctx *c, double dt, uint32_t *fwd_out, int4 *stage_a, int4 *stage_b) {
+  ChosenState *s = c->ch;
+  const Geo &g = c->geo;
+  launch_cellvec(c);
+  const long long ntiles = (g.ncell + kTileCells - 1) / kTileCells;
+  PushDepArgs a;
+  a.k = push_consts(c, dt);
+  a.pa = s->pa;
+  a.pb = s->pb;
+  a.pa_out = s->pa_alt;
+  a.pb_out = s->pb_alt;
+  a.ma = stage_a ? stage_a : s->pa;
+  a.mb = stage_b ? stage_b : s->pb;
+  a.mv = s->mv;
+  a.g0 = s->g0_active ? s->g0 : nullptr;
+  a.base = s->base;
+  a.cnt = s->cnt;
+  a.base_out = s->base;
+  a.ns = s->ns;
+  a.arr = s->arr;
+  a.list = s->list;
+  a.nmv = s->nmv;
+  a.cj = s->cj;
+  a.fwd = fwd_out;
+  a.lay = s->lay;
+  a.ncell = g.ncell;
+  a.ntiles = ntiles;
+  a.flag = c->dflag;
   {
-    PushRankArgs a;
-    a.k = k;
-    a.pa = s->pa;
-    a.pb = s->pb;
-    a.g0 = g0;
-    a.off = s->srt.off;
-    a.ncell = g.ncell;
-    a.code = s->srt.code;
-    a.rank = s->srt.rank;
-    a.nsk = s->srt.nsk;
-    a.fdst = s->fdst;
-    a.nfar = s->srt.misc;
-    a.flag = c->dflag;
-    TimedLaunch t(c, "push_rank_f32");
-    const dim3 grid(xcd_grid(cell_wave_blocks(g.ncell, kPushCells)));
-    if (c->cfg.relativity && all_per)
-      hipLaunchKernelGGL((k_push_rank_f32<true, true>), grid, dim3(kBlock), 0, c->stream, a);
-    else if (c->cfg.relativity)
-      hipLaunchKernelGGL((k_push_rank_f32<true, false>), grid, dim3(kBlock), 0, c->stream, a);
-    else if (all_per)
-      hipLaunchKernelGGL((k_push_rank_f32<false, true>), grid, dim3(kBlock), 0, c->stream, a);
-    else
-      hipLaunchKernelGGL((k_push_rank_f32<false, false>), grid, dim3(kBlock), 0, c->stream, a);
+    TimedLaunch t(c, "push_deposit_f32");
+    launch_push_deposit<false>(c, a, dim3(xcd_grid((unsigned)ntiles)));
   }
-  cellsort_far(c, s->srt, np, CellsF32{s->pa, s->fdst}, "far_f32");
-  int rc = cellsort_base_scan(c, s->srt, "base_f32", "scan_f32");
-  if (rc) return rc;
   {
-    InvArgs a;
-    a.g = g;
-    a.b = s->srt;
-    a.fdst = s->fdst;
-    cellsort_periodic(c, a.per);
-    a.inv = s->srt.perm;
-    a.flag = c->dflag;
-    TimedLaunch t(c, "inv_f32");
-    hipLaunchKernelGGL(k_inv_f32, dim3(xcd_grid(cell_wave_blocks(g.ncell, kCellsPerWave))), dim3(kBlock), 0,
-                       c->stream, a);
+    TimedLaunch t(c, "check_f32");
+    hipLaunchKernelGGL(k_check_f32, dim3(grid_for(g.ncell)), dim3(kBlock), 0, c->stream, (const int32_t *)s->ns,
+                       (const int32_t *)s->arr, (const int32_t *)s->base, g.ncell, s->lay);
   }
   {
-    MoveDepArgs a;
-    a.k = k;
-    a.pa = s->pa;
-    a.pb = s->pb;
-    a.g0 = g0;
-    a.pa_out = s->pa_alt;
-    a.pb_out = s->pb_alt;
-    a.inv = s->srt.perm;
-    a.off = s->srt.off_new;
-    a.ncell = g.ncell;
-    a.np = np;
-    a.cj = s->cj;
-    a.nfar = s->srt.misc;
-    a.flag = c->dflag;
-    const unsigned blocks = xcd_grid((unsigned)((g.ncell + (kBlock / kDepLanes) - 1) / (kBlock / kDepLanes)));
-    TimedLaunch t(c, "move_deposit_f32");
-    if (c->cfg.relativity && all_per)
-      hipLaunchKernelGGL((k_move_deposit_f32<true, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
-    else if (c->cfg.relativity)
-      hipLaunchKernelGGL((k_move_deposit_f32<true, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
-    else if (all_per)
-      hipLaunchKernelGGL((k_move_deposit_f32<false, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
-    else
-      hipLaunchKernelGGL((k_move_deposit_f32<false, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    TimedLaunch t(c, "rebase_f32");
+    hipLaunchKernelGGL(k_rebase_f32, dim3(1), dim3(kRebaseThreads), 0, c->stream, (const int32_t *)s->ns,
+                       (const int32_t *)s->arr, g.ncell, (long long)s->nslots, s->base_new, s->lay, c->dflag);
+  }
+  {
+    PushDepArgs r = a;
+    r.base_out = s->base_new;
+    TimedLaunch t(c, "push_deposit_redo_f32");
+    launch_push_deposit<true>(c, r, dim3((unsigned)std::min<long long>(ntiles, 1024)));
+  }
+  {
+    TimedLaunch t(c, "place_f32");
+    hipLaunchKernelGGL(k_place_f32, dim3(1024), dim3(kBlock), 0, c->stream, (const int32_t *)s->list,
+                       (const int32_t *)s->nmv, (const int4 *)a.ma, (const int32_t *)s->base,
+                       (const int32_t *)s->base_new, (const int32_t *)s->lay, (const int32_t *)s->ns, s->put, s->akey,
+                       c->dflag);
+  }
+  {
+    ArriveArgs r;
+    r.base = s->base;
+    r.base_new = s->base_new;
+    r.lay = s->lay;
+    r.ns = s->ns;
+    r.arr = s->arr;
+    r.put = s->put;
+    r.cnt = s->cnt;
+    r.nmv = s->nmv;
+    r.akey = s->akey;
+    r.akey2 = s->akey2;
+    r.ma = a.ma;
+    r.mb = a.mb;
+    r.mv = s->mv;
+    r.pa_out = s->pa_alt;
+    r.pb_out = s->pb_alt;
+    r.fwd = fwd_out;
+    r.cj = s->cj;
+    for (int q = 0; q < EMO_NSPEC; q++) r.jc[q] = a.k.s.test[q] ? 0.f : a.k.s.jc[q];
+    r.ncell = g.ncell;
+    r.flag = c->dflag;
+    TimedLaunch t(c, "arrive_f32");
+    hipLaunchKernelGGL(k_arrive_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
+  }
+  {
+    TimedLaunch t(c, "commit_f32");
+    hipLaunchKernelGGL(k_commit_f32, dim3(256), dim3(kBlock), 0, c->stream, s->base, (const int32_t *)s->base_new,
+                       (const int32_t *)s->lay, g.ncell);
   }
   std::swap(s->pa, s->pa_alt);
   std::swap(s->pb, s->pb_alt);
-  std::swap(s->srt.off, s->srt.off_new);
   DepGridArgs<float> b;
   b.g = g;
   b.cj = s->cj;
@@ -1016,16 +1130,32 @@ int fp32_create(emo_ctx *c) {
   const Geo &g = c->geo;
   if (g.ncell > kCellMask) return set_err(c, EMO_ERR_CONFIG, "grid too large for the FP32 cell word");
   int64_t cap = std::max<int64_t>(c->cfg.max_particles, 1);
-  if (cap > INT_MAX) return set_err(c, EMO_ERR_CONFIG, "the FP32 chosen form holds at most 2^31-1 particles");
+  const long long nslots = slots_for(cap, g.ncell);
+  if (nslots > INT_MAX)
+    return set_err(c, EMO_ERR_CONFIG, "the FP32 chosen form's slot layout needs int32 slot indices (fewer particles)");
   s->cap = cap;
+  s->nslots = nslots;
   c->cap = cap;
+  const size_t nc1 = (size_t)g.ncell + 1;
   int rc = 0;
-  if (!rc) rc = dalloc(c, &s->pa, cap);
-  if (!rc) rc = dalloc(c, &s->pa_alt, cap);
-  if (!rc) rc = dalloc(c, &s->pb, cap);
-  if (!rc) rc = dalloc(c, &s->pb_alt, cap);
-  if (!rc) rc = dalloc(c, &s->fdst, cap);
-  if (!rc) rc = cellsort_alloc(c, s->srt, cap, g.ncell);
+  if (!rc) rc = dalloc(c, &s->pa, nslots);
+  if (!rc) rc = dalloc(c, &s->pa_alt, nslots);
+  if (!rc) rc = dalloc(c, &s->pb, nslots);
+  if (!rc) rc = dalloc(c, &s->pb_alt, nslots);
+  if (!rc) rc = dalloc(c, &s->mv, nslots);
+  if (!rc) rc = dalloc(c, &s->akey, nslots);
+  if (!rc) rc = dalloc(c, &s->akey2, nslots);
+  if (!rc) rc = dalloc(c, &s->list, cap);
+  if (!rc) rc = dalloc(c, &s->base, nc1);
+  if (!rc) rc = dalloc(c, &s->base_new, nc1);
+  if (!rc) rc = dalloc(c, &s->cnt, nc1);
+  if (!rc) rc = dalloc(c, &s->ns, nc1);
+  if (!rc) rc = dalloc(c, &s->arr, nc1);
+  if (!rc) rc = dalloc(c, &s->put, nc1);
+  if (!rc) rc = dalloc(c, &s->dcount, nc1);
+  if (!rc) rc = dalloc(c, &s->doff, nc1);
+  if (!rc) rc = dalloc(c, &s->nmv, 4);
+  if (!rc) rc = dalloc(c, &s->lay, 4);
   if (!rc) rc = dalloc(c, &s->key, cap);
   if (!rc) rc = dalloc(c, &s->key_alt, cap);
   if (!rc) rc = dalloc(c, &s->perm, cap);
@@ -1048,6 +1178,15 @@ int fp32_create(emo_ctx *c) {
   float *all[13] = {s->fs.E[0], s->fs.E[1], s->fs.E[2], s->fs.Bfix[0], s->fs.Bfix[1], s->fs.Bfix[2],
                     s->Btfix[0], s->Btfix[1], s->Btfix[2], s->fs.J[0], s->fs.J[1], s->fs.J[2], s->fs.rho};
   for (int q = 0; q < 13; q++) EMO_HIP_CHECK_RET(c, hipMemset(all[q], 0, nt * sizeof(float)));
+  // per-step counters start (and end every step) at zero; an empty layout
+  int32_t *zero[] = {s->cnt, s->ns, s->arr, s->put, s->dcount};
+  for (int32_t *p : zero) EMO_HIP_CHECK_RET(c, hipMemset(p, 0, nc1 * 4));
+  EMO_HIP_CHECK_RET(c, hipMemset(s->nmv, 0, 16));
+  // every arrival key ever read is a slot number (also where a flagged error
+  // left a range short)
+  EMO_HIP_CHECK_RET(c, hipMemset(s->akey, 0, (size_t)nslots * 4));
+  EMO_HIP_CHECK_RET(c, hipMemset(s->akey2, 0, (size_t)nslots * 4));
+  EMO_HIP_CHECK_RET(c, hipMemset(s->lay, 0, 16));
   int bits = 1;
   while ((1ll << bits) < g.ncell) bits++;
   s->sort_bits = bits;
@@ -1056,44 +1195,47 @@ int fp32_create(emo_ctx *c) {
                                      (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)cap, 0, bits);
   if (hipMalloc(&s->sort_tmp, std::max<size_t>(s->sort_tmp_bytes, 16)) != hipSuccess)
     return set_err(c, EMO_ERR_ALLOC, "sort scratch");
-  return EMO_OK;
+  s->scan_tmp_bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, s->scan_tmp_bytes, (const int32_t *)nullptr, (int32_t *)nullptr,
+                                   (int)nc1);
+  if (hipMalloc(&s->scan_tmp, std::max<size_t>(s->scan_tmp_bytes, 16)) != hipSuccess)
+    return set_err(c, EMO_ERR_ALLOC, "scan scratch");
+  return build_layout(c, s->cnt);
 }
 
 void fp32_destroy(emo_ctx *c) {
   ChosenState *s = c->ch;
   if (!s) return;
-  dfree(s->pa);
-  dfree(s->pa_alt);
-  dfree(s->pb);
-  dfree(s->pb_alt);
-  dfree(s->fdst);
+  void *ps[] = {s->pa, s->pa_alt, s->pb, s->pb_alt, s->mv, s->akey, s->akey2, s->list, s->base, s->base_new, s->cnt,
+                s->ns, s->arr, s->put, s->dcount, s->doff, s->nmv, s->lay, s->key, s->key_alt, s->perm,
+                s->perm_alt, s->sort_tmp, s->scan_tmp, s->cj, s->cv, s->gam_id, s->fs.rho};
+  for (void *p : ps) dfree(p);
+  if (s->g0_owned) dfree(s->g0);
   for (int d = 0; d < 3; d++) {
     dfree(s->fs.E[d]);
     dfree(s->fs.Bfix[d]);
     dfree(s->Btfix[d]);
     dfree(s->fs.J[d]);
   }
-  dfree(s->fs.rho);
-  dfree(s->g0);
-  dfree(s->gam_id);
-  cellsort_free(s->srt);
-  dfree(s->key);
-  dfree(s->key_alt);
-  dfree(s->perm);
-  dfree(s->perm_alt);
-  dfree(s->sort_tmp);
-  dfree(s->cj);
-  dfree(s->cv);
   delete s;
   c->ch = nullptr;
 }
 
+// drops the stored gamma (and frees it when it is this state's own)
+void drop_g0(ChosenState *s) {
+  if (s->g0_owned) dfree(s->g0);
+  s->g0 = nullptr;
+  s->g0_active = false;
+  s->g0_owned = true;
+}
+
 // Converts FP64 particles to the chosen representation on the host (the same
-// cell-id arithmetic as Grid::getCellID) and sorts them by cell.
+// cell-id arithmetic as Grid::getCellID) and lays them out by cell.
 int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const double *gamma,
               const int32_t *species, const int64_t *id) {
   ChosenState *s = c->ch;
   const Geo &g = c->geo;
+  if (n > s->cap) return set_err(c, EMO_ERR_CONFIG, "more particles than max_particles");
   std::vector<int4> ha(n), hb(n);
   std::vector<float> g0;
   const bool rel = c->cfg.relativity != 0;
@@ -1113,6 +1255,8 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
       if (f < 0.f) f = 0.f;
       ff[d] = f;
     }
+    if (species[i] < 0 || species[i] >= std::max(c->cfg.nspecies, 1))
+      return set_err(c, EMO_ERR_CONFIG, "species index out of range");
     int32_t w = (species[i] << kSpeciesShift) | ((ic[0] * g.nc[1] + ic[1]) * g.nc[2] + ic[2]);
     double gm = (rel && gamma) ? gamma[i] : 1.0;
     float u[3];
@@ -1135,9 +1279,7 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
     EMO_HIP_CHECK_RET(c, hipMemcpy(s->pb, hb.data(), n * sizeof(int4), hipMemcpyHostToDevice));
   }
   s->have_id = id != nullptr;
-  dfree(s->g0);
-  s->g0 = nullptr;
-  s->g0_active = false;
+  drop_g0(s);
   dfree(s->gam_id);
   s->gam_id = nullptr;
   if (rel && n > 0) {
@@ -1163,8 +1305,29 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
   c->np = n;
   int rc = sort_loaded(c);
   if (rc) return rc;
-  EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
   return check_flag(c);
+}
+
+// output in array order (cell by cell); the caller's device buffers of np
+// entries, any of them null
+int launch_output(emo_ctx *c, OutArgs &a) {
+  ChosenState *s = c->ch;
+  int rc = dense_offsets(c);
+  if (rc) return rc;
+  a.g = c->geo;
+  a.pa = s->pa;
+  a.pb = s->pb;
+  a.g0 = s->g0_active ? s->g0 : nullptr;
+  a.gam_id = s->gam_id;
+  a.cv = s->cv;
+  a.base = s->base;
+  a.cnt = s->cnt;
+  a.doff = s->doff;
+  a.rel = c->cfg.relativity;
+  a.np = c->np;
+  hipLaunchKernelGGL(k_output_f32, dim3((unsigned)((c->geo.ncell + kTileCells - 1) / kTileCells)), dim3(kBlock), 0,
+                     c->stream, a);
+  return EMO_OK;
 }
 
 int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species, int64_t *id,
@@ -1180,41 +1343,26 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
     if (!rc) rc = dalloc(c, &dv[d], n);
   }
   for (int d = 0; d < 6 && !rc && field6; d++) rc = dalloc(c, &df[d], n);
-  if (!rc && field6) {
-    // the gather of the current grid (Particle_Handler::InterpolateEB, which
-    // ends the reference's step): the gather vectors of this grid first
-    CellVecArgs ca;
-    ca.g = c->geo;
-    for (int d = 0; d < 3; d++) {
-      ca.E[d] = s->fs.E[d];
-      ca.B[d] = s->fs.B[d];
-      ca.Bt[d] = s->fs.Bt[d];
-    }
-    ca.cv = s->cv;
-    hipLaunchKernelGGL(k_cellvec_f32, dim3(grid_for(c->geo.ncell)), dim3(kBlock), 0, c->stream, ca);
-  }
+  // the gather of the current grid (Particle_Handler::InterpolateEB, which
+  // ends the reference's step): the gather vectors of this grid first
+  if (!rc && field6) launch_cellvec(c);
   if (!rc) rc = dalloc(c, &dg, n);
   if (!rc) rc = dalloc(c, &dsp, n);
   if (!rc) rc = dalloc(c, &did, n);
   if (!rc) {
-    OutArgs a;
-    a.g = c->geo;
-    a.pa = s->pa;
-    a.pb = s->pb;
+    OutArgs a{};
     for (int d = 0; d < 3; d++) {
       a.x[d] = dx[d];
       a.v[d] = dv[d];
     }
-    a.g0 = s->g0_active ? s->g0 : nullptr;
-    a.gam_id = s->gam_id;
-    a.cv = s->cv;
     for (int d = 0; d < 6; d++) a.fld[d] = df[d];
-    a.rel = c->cfg.relativity;
     a.gam = dg;
     a.sp = dsp;
     a.id = did;
-    a.np = n;
-    hipLaunchKernelGGL(k_output_f32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);
+    a.cell = nullptr;
+    rc = launch_output(c, a);
+  }
+  if (!rc) {
     std::vector<double> buf(n);
     auto down = [&](double *dst, const double *src, int stride, int comp) -> int {
       EMO_HIP_CHECK_RET(c, hipMemcpyAsync(buf.data(), src, n * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1247,16 +1395,19 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
 }
 
 int fp32_get_cells(emo_ctx *c, int32_t *cells) {
-  ChosenState *s = c->ch;
   int64_t n = c->np;
   if (n == 0) return EMO_OK;
   int32_t *d = nullptr;
   int rc = dalloc(c, &d, n);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_cells_f32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, s->pa, d, (long long)n);
-  hipError_t e = hipMemcpyAsync(cells, d, n * 4, hipMemcpyDeviceToHost, c->stream);
+  OutArgs a{};
+  a.cell = d;
+  rc = launch_output(c, a);
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipMemcpyAsync(cells, d, n * 4, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   dfree(d);
+  if (rc) return rc;
   if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, hipGetErrorString(e));
   return EMO_OK;
 }
@@ -1287,7 +1438,8 @@ int fp32_deposit_rho(emo_ctx *c) {
     a.pa = s->pa;
     a.pb = s->pb;
     a.g0 = s->g0_active ? s->g0 : nullptr;
-    a.off = s->srt.off;
+    a.base = s->base;
+    a.cnt = s->cnt;
     a.s = species_f32(c, 0.0);
     a.rel = c->cfg.relativity;
     a.ncell = g.ncell;
@@ -1357,9 +1509,8 @@ int fp32_init_constant(emo_ctx *c) {
 int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n) {
   ChosenState *s = c->ch;
   const bool rel = c->cfg.relativity != 0;
-  dfree(s->g0);
-  s->g0 = nullptr;
-  s->g0_active = false;
+  if (n > s->cap) return set_err(c, EMO_ERR_CONFIG, "more particles than max_particles");
+  drop_g0(s);
   dfree(s->gam_id);
   s->gam_id = nullptr;
   if (rel && n > 0) {
@@ -1387,7 +1538,6 @@ int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n) {
   c->np = n;
   int rc = sort_loaded(c);
   if (rc) return rc;
-  EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
   return check_flag(c);
 }
 
@@ -1409,7 +1559,8 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
     a.pa = s->pa;
     a.pb = s->pb;
     a.g0 = s->g0_active ? s->g0 : nullptr;
-    a.off = s->srt.off;
+    a.base = s->base;
+    a.cnt = s->cnt;
     a.s = species_f32(c, 0.0);
     a.rel = c->cfg.relativity;
     a.ncell = g.ncell;
@@ -1513,23 +1664,15 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
 
 PushK fp32_push_consts(emo_ctx *c, double dt) { return push_consts(c, dt); }
 
-int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out) {
+int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out, int4 *stage_a, int4 *stage_b) {
   ChosenState *s = c->ch;
-  int rc = particle_step(c, dt);
+  int rc = particle_step(c, dt, fwd_out, stage_a, stage_b);
   if (rc) return rc;
-  if (fwd_out && c->np > 0)
-    hipLaunchKernelGGL(k_fwd_from_inv, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, (const uint32_t *)s->srt.perm,
-                       fwd_out, (long long)c->np);
   if (s->g0_active) {
     // the stored gamma was used by this first push only (Q3): from now on
     // gamma = sqrt(1 + u^2)
-    if (s->g0_owned) {
-      EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
-      dfree(s->g0);
-    }
-    s->g0_active = false;
-    s->g0 = nullptr;
-    s->g0_owned = true;
+    if (s->g0_owned) EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+    drop_g0(s);
   }
   {
     FieldSet<float> fj = s->fs;
@@ -1559,18 +1702,33 @@ Fp32View fp32_view(emo_ctx *c) {
   v.g0_owned = &s->g0_owned;
   v.fs = &s->fs;
   for (int q = 0; q < 3; q++) v.Btfix[q] = s->Btfix[q];
-  v.off = &s->srt.off;
+  v.base = &s->base;
+  v.cnt = &s->cnt;
+  v.nslots = s->nslots;
+  v.lay = s->lay;
   return v;
 }
 
 int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
   double t = t0;
   for (int k = 0; k < nsteps; k++) {
-    int rc = fp32_step_one(c, t, dt, nullptr);
+    int rc = fp32_step_one(c, t, dt, nullptr, nullptr, nullptr);
     if (rc) return rc;
     t += dt;
   }
   return fp32_check(c);
+}
+
+int fp32_layout_info(emo_ctx *c, int64_t *out3) {
+  ChosenState *s = c->ch;
+  int32_t used = 0, lay[2] = {0, 0};
+  EMO_HIP_CHECK_RET(c, hipMemcpyAsync(&used, s->base + c->geo.ncell, 4, hipMemcpyDeviceToHost, c->stream));
+  EMO_HIP_CHECK_RET(c, hipMemcpyAsync(lay, s->lay, 8, hipMemcpyDeviceToHost, c->stream));
+  EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+  out3[0] = s->nslots;
+  out3[1] = used;
+  out3[2] = lay[1];
+  return EMO_OK;
 }
 
 int fp32_diag(emo_ctx *c, double *out8) {
@@ -1581,9 +1739,11 @@ int fp32_diag(emo_ctx *c, double *out8) {
   a.pa = s->pa;
   a.pb = s->pb;
   a.g0 = s->g0_active ? s->g0 : nullptr;
+  a.base = s->base;
+  a.cnt = s->cnt;
   a.rel = c->cfg.relativity;
   a.s = c->species;
-  a.np = c->np;
+  a.ncell = c->geo.ncell;
   a.partials = pp;
   hipLaunchKernelGGL(k_diag_particles_f32, dim3(nb), dim3(kBlock), 0, c->stream, a);
   launch_reduce_partials(c, pp, nb, 5, res);

@@ -835,6 +835,13 @@ int emo_diag(emo_ctx *c, double *out8) {
   return EMO_OK;
 }
 
+int emo_layout_info(emo_ctx *c, int64_t *out3) {
+  if (!c || !out3) return EMO_ERR_CONFIG;
+  if (c->cfg.precision != EMO_FP32_CHOSEN)
+    return set_err(c, EMO_ERR_UNSUPPORTED, "the slot layout belongs to the FP32 chosen form");
+  return fp32_layout_info(c, out3);
+}
+
 int emo_synchronize(emo_ctx *c) {
   if (!c) return EMO_ERR_CONFIG;
   EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));

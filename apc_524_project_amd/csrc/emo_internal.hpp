@@ -156,10 +156,11 @@ namespace emo {
 // error flag bits written by kernels into dflag[0]
 constexpr unsigned long long kFlagOutOfGrid = 1ull;
 constexpr unsigned long long kFlagNonFinite = 2ull;
-// FP32 migration sort: more particles left the one-cell neighbourhood of
+// FP64 migration sort: more particles left the one-cell neighbourhood of
 // their cell in one step than its side list holds
 constexpr unsigned long long kFlagFarMovers = 4ull;
-// FP32 migration sort: a cell's gathered particles disagree with its count
+// FP64 migration sort: a cell's gathered particles disagree with its count;
+// FP32 slot layout: a moving particle found no slot in its new cell
 constexpr unsigned long long kFlagMigrate = 8ull;
 
 // ---- plain FP64 particle kernels (plain_kernels.hip) ----
@@ -276,20 +277,25 @@ void poisson_ws_carve(const emo_ctx *c, void *base, PoissonWs &ws);
 int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps, double *resid,
                       const PoissonWs &ws);
 // The FP32 form's device state, by reference, for its adjoint (adjoint.hip):
-// the particle records (current and alternate buffers), the stored gamma of
-// the first push, the fields and the cell ranges of the current order.
+// the particle records (current and alternate buffers, nslots slots each),
+// the stored gamma of the first push (by slot), the fields and the slot
+// layout (cell c holds cnt[c] particles from slot base[c]).
 struct Fp32View {
   int4 **pa, **pb, **pa_alt, **pb_alt;
   float **g0;
   bool *g0_active, *g0_owned;
   FieldSet<float> *fs;
   float *Btfix[3];
-  int32_t **off;
+  int32_t **base, **cnt;
+  int64_t nslots;
+  int32_t *lay;
 };
 Fp32View fp32_view(emo_ctx *c);
 // one step of fp32_step without the flag readback; with fwd_out, the step's
-// sort as pre-push index -> new slot
-int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out);
+// map source slot -> new slot, and the movers' new records written to
+// stage_a / stage_b (nslots each) instead of over their source slots, which
+// then keep the step's input intact
+int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out, int4 *stage_a, int4 *stage_b);
 // the device error flags of the FP32 form (stream synchronisation)
 int fp32_check(emo_ctx *c);
 int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, int32_t K, double *energy,
@@ -299,6 +305,8 @@ int fp32_load_uniform(emo_ctx *c, const UniformGen &u, int64_t n);
 void launch_uniform_f64(emo_ctx *c, const UniformGen &u, int64_t n);
 int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt);
 int fp32_diag(emo_ctx *c, double *out8);
+// slots per record buffer, slots covered by the cells' ranges, rebuilds so far
+int fp32_layout_info(emo_ctx *c, int64_t *out3);
 
 // timing helpers (emo_api.cpp)
 struct TimedLaunch {

@@ -216,6 +216,14 @@ int emo_adjoint_field_energy(emo_ctx *ctx, int32_t nsteps, double t0, double dt,
  * No counterpart in the reference (the loader extension is this library's). */
 int emo_loader_dx_deps(const emo_loader *ld, const emo_config *cfg, double *dx_deps);
 
+/* No counterpart in the reference (whose particle vector has no layout).
+ * FP32 chosen form: out3 = {slots in each particle record buffer, slots
+ * covered by the cells' ranges, layout rebuilds so far}.  A cell owns a slot
+ * range of its count plus headroom; a step whose arrivals exceed a cell's
+ * headroom rebuilds the ranges on the device.  EMO_ERR_UNSUPPORTED in the
+ * FP64 plain form. */
+int emo_layout_info(emo_ctx *ctx, int64_t *out3);
+
 /* ---- stream / timing ---------------------------------------------------- */
 int emo_synchronize(emo_ctx *ctx);
 /* Record HIP events around every launch of the step's dominant kernels on
