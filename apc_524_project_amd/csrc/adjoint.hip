@@ -1176,6 +1176,7 @@ struct AdjRun32 {
   std::vector<FillPlane> planes;
   int4 *own[4] = {};  // the context's own record buffers (pa, pb, pa_alt, pb_alt)
   int4 *stage_a = nullptr, *stage_b = nullptr;  // the recomputed movers' new records
+  int32_t *mvsrc = nullptr;                       // and their source slots
 };
 
 void restore_own_buffers(AdjRun32 &R) {
@@ -1203,7 +1204,8 @@ int recompute32(AdjRun32 &R, int seg, int set) {
     snap32_take(c, tp[st - s0], t, false);
     *v.pa_alt = tp[st - s0 + 1].pa;
     *v.pb_alt = tp[st - s0 + 1].pb;
-    rc = fp32_step_one(c, t, R.dt, tp[st - s0].fwd, R.stage_a, R.stage_b);
+    const Fp32Tape tape{tp[st - s0].fwd, R.stage_a, R.stage_b, R.mvsrc};
+    rc = fp32_step_one(c, t, R.dt, &tape);
     t += R.dt;
   }
   restore_own_buffers(R);
@@ -1326,6 +1328,7 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
     P.take(&R.g0b, npa);
     P.take(&R.stage_a, npa);
     P.take(&R.stage_b, npa);
+    P.take(&R.mvsrc, npa);
   };
   {
     Pool dry;
@@ -1376,7 +1379,7 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
         cb.flush();
       }
     } else {
-      for (int st = s0; st < s1 && !rc; st++) rc = fp32_step_one(c, tstep[st], dt, nullptr, nullptr, nullptr);
+      for (int st = s0; st < s1 && !rc; st++) rc = fp32_step_one(c, tstep[st], dt, nullptr);
     }
     t = tstep[s1];
   }

@@ -83,12 +83,14 @@ struct ChosenState {
   float *gam_id = nullptr;
   // slot layout (ncell + 1 bases, ncell counts) and its rebuild target
   int32_t *base = nullptr, *base_new = nullptr, *cnt = nullptr;
-  // per step: stayers and arrivals per cell, placement counters, the
-  // movers' list and count, their velocities by source slot, the arrival
-  // ranges' provisional source slots; lay[0] = rebuild pending, lay[1] =
-  // rebuilds so far
-  int32_t *ns = nullptr, *arr = nullptr, *put = nullptr, *list = nullptr, *nmv = nullptr;
+  // per step: stayers, movers and arrivals per cell, placement counters,
+  // the movers' velocities by source slot; lay[0] = rebuild pending,
+  // lay[1] = rebuilds so far
+  int32_t *ns = nullptr, *nm = nullptr, *arr = nullptr, *put = nullptr;
+  long long *rpart = nullptr;  // the check's per-block sums of the new ranges (2 x nblk)
   float4 *mv = nullptr;
+  // akey: arrival ranges' provisional staging slots (destination layout);
+  // akey2: the sorted keys of crowded arrival ranges
   int32_t *akey = nullptr, *akey2 = nullptr;
   int32_t *lay = nullptr;
   // per-cell dense offsets (readback in array order) and scan scratch
@@ -120,21 +122,30 @@ __device__ inline unsigned xcd_tile() {
 }
 inline unsigned xcd_grid(unsigned blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
 
-// A cell's slot range for n particles: n, a quarter more and 8, rounded up
+// A cell's slot range for n particles: n plus 6 sqrt(n) plus 16, rounded up
 // to 8 slots (so every range starts on a 128-byte boundary of both record
-// arrays); the tight form (n rounded up, plus 8) when the generous layout
-// does not fit the buffers.  The buffers hold 1.25 cap + 16 ncell slots, so
-// the tight layout always fits.
+// arrays).  A cell's count fluctuates by about sqrt(n) around its mean, so
+// headroom of 6 sqrt(n) rarely fills (a 256k-cell thermal plasma at 64
+// particles per cell rebuilds about once per hundred steps, against one in
+// seven steps with n/4 + 8).  The tight form (n rounded up, plus 8) is used
+// when the generous layout does not fit the buffers; the buffers hold
+// cap + 6 sqrt(cap ncell) + 29 ncell slots, more than either needs
+// (sum sqrt(n) <= sqrt(cap ncell)).
+__host__ __device__ inline long long isqrt_ceil(long long n) {
+  long long r = (long long)sqrt((double)n);
+  while (r * r < n) r++;
+  while (r > 0 && (r - 1) * (r - 1) >= n) r--;
+  return r;
+}
 __host__ __device__ inline long long slot_cap(long long n, bool tight) {
-  return tight ? ((n + 7) & ~7ll) + 8 : (n + (n >> 2) + 8 + 7) & ~7ll;
+  return tight ? ((n + 7) & ~7ll) + 8 : (n + 6 * isqrt_ceil(n) + 16 + 7) & ~7ll;
 }
 inline long long slots_for(long long cap_particles, long long ncell) {
-  return ((cap_particles + (cap_particles >> 2) + 7) & ~7ll) + 16 * ncell + 64;
+  return cap_particles + 6 * isqrt_ceil(cap_particles * ncell) + 29 * ncell + 64;
 }
 
 constexpr int kCellLanes = 16;                   // lanes per cell
 constexpr int kTileCells = kBlock / kCellLanes;  // cells per block
-constexpr int kArrFast = 64;                     // arrivals a cell sorts in LDS
 
 // The gather vector of a cell (Grid::getFieldInterpolatorVec,
 // src/grid/oGrid.cpp:153-197, without the vertex): the 4 edge values of each
@@ -238,35 +249,41 @@ __device__ inline void deposit_j(float hi[12], float lo[12], const float f[3], c
 // ---------------------------------------------------------------------------
 // The step's particle pass.  16 lanes per cell, 16 cells per block; the
 // lanes take the cell's particles 16 at a time, in order, with the next 16
-// records in flight during a push.  The redo form (REDO) runs only after a
-// rebuild: it writes the stayers again, at the new layout's slots, from the
-// same inputs (the push is a deterministic function of them); it skips the
-// slots whose record a mover has already replaced (their cell is not the
-// slot's) and leaves currents, counts and the movers' data as they are.
+// records in flight during a push.  Stayers go to the output buffer at the
+// cell's next free slots.  Movers go, in order, to the front of the cell's
+// own range of the staging buffers (the input buffer itself, whose slots up
+// to the current particle have been read, or, with a tape, separate
+// arrays), with their velocity beside them; each counts itself among its
+// destination's arrivals and, while the destination's range has room, writes
+// its staging slot to the top of that range in the arrival keys (the order
+// the atomics give is provisional; arrive_f32 puts each range in a
+// canonical order).
 // ---------------------------------------------------------------------------
 struct PushDepArgs {
   PushK k;
   const int4 *pa, *pb;        // records before the step, source layout
   int4 *pa_out, *pb_out;      // records after the step (the stayers here)
-  int4 *ma, *mb;              // movers' new records, at their source slot
-  float4 *mv;                 // movers' velocities (deposit), at their source slot
+  int4 *ma, *mb;              // staging: cell c's movers' new records from slot base[c] on
+  float4 *mv;                 // staging: their velocities (deposit)
+  int32_t *mvsrc;             // with a tape: their source slots, or null
   const float *g0;            // the stored gamma of the first push, by source slot, or null
   const int32_t *base, *cnt;  // source layout
-  const int32_t *base_out;    // layout of the output (base, or the rebuilt one)
-  int32_t *ns;                // stayers per cell
+  int32_t *ns, *nm;           // stayers and movers per cell
   int32_t *arr;               // arrivals per cell (counted here, zero on entry)
-  int32_t *list, *nmv;        // movers' source slots, their number (zero on entry)
+  int32_t *akey;              // arrival keys: the staging slots arriving in each range, from its top
   float *cj;                  // 12 per cell: the stayers' currents
   uint32_t *fwd;              // with a tape: source slot -> new slot, or null
-  int32_t *lay;               // lay[0]: rebuild pending (cleared here; the redo runs only when set)
+  int32_t *lay;               // lay[0]: rebuild pending (cleared here)
   long long ncell, ntiles;
   unsigned long long *flag;
 };
 
-template <bool REL, bool PER, bool REDO>
+constexpr int kRedVals = 4;  // (hi, lo) pairs per lane per LDS round of the cell sums
+
+template <bool REL, bool PER>
 __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
-  if (REDO && a.lay[0] == 0) return;
   __shared__ float qp_s[EMO_NSPEC], jc_s[EMO_NSPEC];
+  __shared__ float2 red[kBlock * kRedVals];
   // species constants in LDS (a per-lane load from the argument block would
   // make every iteration wait for all loads in flight); test particles
   // deposit nothing (src/particles/particle_handler.cpp:205)
@@ -274,115 +291,151 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
     qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
     jc_s[threadIdx.x] = a.k.s.test[threadIdx.x] ? 0.f : a.k.s.jc[threadIdx.x];
   }
-  if (!REDO && blockIdx.x == 0 && threadIdx.x == 0) a.lay[0] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.lay[0] = 0;
   __syncthreads();
   const int sub = threadIdx.x & (kCellLanes - 1);
   const int lane = threadIdx.x & 63;
   const int gsh = lane & ~(kCellLanes - 1);  // the cell's first lane in the wavefront
+  const unsigned below = (1u << sub) - 1u;
   const bool use_g0 = a.g0 != nullptr;
-  const long long first = REDO ? (long long)blockIdx.x : (long long)xcd_tile();
-  const long long stride = REDO ? (long long)gridDim.x : a.ntiles;
-  for (long long tile = first; tile < a.ntiles; tile += stride) {
-    const long long c = tile * kTileCells + (threadIdx.x >> 4);
-    const bool in = c < a.ncell;
-    const int b = in ? a.base[c] : 0;
-    const int n = in ? a.cnt[c] : 0;
-    const int bo = in ? a.base_out[c] : 0;
-    // the wavefront runs to its largest cell (ballots need every lane)
-    int nmax = n;
-    nmax = max(nmax, __shfl_xor(nmax, 16, 64));
-    nmax = max(nmax, __shfl_xor(nmax, 32, 64));
-    CellVec V{};
-    if (n > 0) V = load_cellvec(a.k, (int)c);
-    float hi[12], lo[12];
+  const long long tile = xcd_tile();
+  if (tile >= a.ntiles) return;
+  const long long c = tile * kTileCells + (threadIdx.x >> 4);
+  const bool in = c < a.ncell;
+  const int b = in ? a.base[c] : 0;
+  const int n = in ? a.cnt[c] : 0;
+  // the wavefront runs to its largest cell (ballots need every lane)
+  int nmax = n;
+  nmax = max(nmax, __shfl_xor(nmax, 16, 64));
+  nmax = max(nmax, __shfl_xor(nmax, 32, 64));
+  CellVec V{};
+  if (n > 0) V = load_cellvec(a.k, (int)c);
+  float hi[12], lo[12];
 #pragma unroll
-    for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
-    int nstay = 0;  // the same in all lanes of the cell
-    // loads past the cell's end read its last particle (a slot of the cell's
-    // own range even when it is empty), so that they are unconditional
-    const int last = b + max(n - 1, 0);
-    int4 A = a.pa[min(b + sub, last)], Bv = a.pb[min(b + sub, last)];
-    float G = use_g0 ? a.g0[min(b + sub, last)] : 0.f;
-    for (int j0 = 0; j0 < nmax; j0 += kCellLanes) {
-      const int jn = min(b + j0 + kCellLanes + sub, last);
-      const int4 An = a.pa[jn], Bn = a.pb[jn];
-      const float Gn = use_g0 ? a.g0[jn] : 0.f;
-      const int src = b + j0 + sub;
-      const bool valid = j0 + sub < n;
-      const bool live = valid && (!REDO || (A.x & kCellMask) == (int)c);
-      const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
-      Pushed o;
-      push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
-                              qp_s[sp], use_g0, G, o, (!REDO && valid) ? a.flag : nullptr);
-      const bool stay = live && o.cell == (int)c;
-      const bool move = live && !stay;
-      const unsigned gm = (unsigned)(__ballot(stay) >> gsh) & 0xffffu;
-      if (stay) {
-        const int dst = bo + nstay + __popc(gm & ((1u << sub) - 1u));
-        a.pa_out[dst] = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)c), as_i(o.f[0]), as_i(o.f[1]),
-                                  as_i(o.f[2]));
-        a.pb_out[dst] = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
-        if (a.fwd) a.fwd[src] = (uint32_t)dst;
-      }
-      nstay += __popc(gm);
-      if (!REDO) {
-        if (stay) deposit_j(hi, lo, o.f, o.v, jc_s[sp]);
-        const unsigned long long mbal = __ballot(move);
-        if (mbal) {
-          int mb0 = 0;
-          if (lane == 0) mb0 = atomicAdd(a.nmv, __popcll(mbal));
-          mb0 = __shfl(mb0, 0, 64);
-          if (move) {
-            a.ma[src] = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)o.cell), as_i(o.f[0]),
-                                  as_i(o.f[1]), as_i(o.f[2]));
-            a.mb[src] = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
-            a.mv[src] = make_float4(o.v[0], o.v[1], o.v[2], 0.f);
-            a.list[mb0 + __popcll(mbal & lanes_below(lane))] = src;
-            atomicAdd(&a.arr[o.cell], 1);
-          }
-        }
-      }
-      A = An;
-      Bv = Bn;
-      G = Gn;
+  for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
+  int nstay = 0, nmove = 0;  // the same in all lanes of the cell
+  // loads past the cell's end read its last particle (a slot of the cell's
+  // own range even when it is empty), so that they are unconditional
+  const int last = b + max(n - 1, 0);
+  int4 A = a.pa[min(b + sub, last)], Bv = a.pb[min(b + sub, last)];
+  float G = use_g0 ? a.g0[min(b + sub, last)] : 0.f;
+  for (int j0 = 0; j0 < nmax; j0 += kCellLanes) {
+    const int jn = min(b + j0 + kCellLanes + sub, last);
+    const int4 An = a.pa[jn], Bn = a.pb[jn];
+    const float Gn = use_g0 ? a.g0[jn] : 0.f;
+    const int src = b + j0 + sub;
+    const bool valid = j0 + sub < n;
+    const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
+    Pushed o;
+    push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
+                            qp_s[sp], use_g0, G, o, valid ? a.flag : nullptr);
+    const bool stay = valid && o.cell == (int)c;
+    const bool move = valid && !stay;
+    const unsigned gm = (unsigned)(__ballot(stay) >> gsh) & 0xffffu;
+    const unsigned mm = (unsigned)(__ballot(move) >> gsh) & 0xffffu;
+    const int4 Ao = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)o.cell), as_i(o.f[0]),
+                              as_i(o.f[1]), as_i(o.f[2]));
+    const int4 Bo = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
+    if (stay) {
+      const int dst = b + nstay + __popc(gm & below);
+      a.pa_out[dst] = Ao;
+      a.pb_out[dst] = Bo;
+      if (a.fwd) a.fwd[src] = (uint32_t)dst;
+      deposit_j(hi, lo, o.f, o.v, jc_s[sp]);
     }
-    if (!REDO) {
-      // all 64 lanes take part (the DPP moves read every lane of the group)
-      static_assert(kCellLanes == 16, "the butterfly covers 16 lanes");
-      dep_reduce16(hi, lo);
-      if (sub == 0 && in) {
+    if (move) {
+      // (the staging slot is at most this particle's own: already read)
+      const int st = b + nmove + __popc(mm & below);
+      a.ma[st] = Ao;
+      a.mb[st] = Bo;
+      a.mv[st] = make_float4(o.v[0], o.v[1], o.v[2], 0.f);
+      if (a.mvsrc) a.mvsrc[st] = src;
+      const int k = atomicAdd(&a.arr[o.cell], 1);
+      const int e1 = a.base[o.cell + 1];
+      if (k < e1 - a.base[o.cell]) a.akey[e1 - 1 - k] = st;  // (else the layout is rebuilt)
+    }
+    nstay += __popc(gm);
+    nmove += __popc(mm);
+    A = An;
+    Bv = Bn;
+    G = Gn;
+  }
+  // the cell's sum over its 16 lanes in lane order, with TwoSum: the lanes'
+  // pairs through LDS kRedVals edges at a time, then one lane per (cell, edge)
+  float2 *rw = red + (threadIdx.x & ~63) * kRedVals;
 #pragma unroll
-        for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = hi[q] + lo[q];
-        a.ns[c] = nstay;
+  for (int q0 = 0; q0 < 12; q0 += kRedVals) {
+#pragma unroll
+    for (int q = 0; q < kRedVals; q++) rw[lane * kRedVals + q] = make_float2(hi[q0 + q], lo[q0 + q]);
+    wave_sync();
+    if (lane < 4 * kRedVals) {
+      const int grp = lane / kRedVals, q = lane - kRedVals * grp;
+      const long long cc = tile * kTileCells + (threadIdx.x >> 6) * 4 + grp;
+      float2 acc = rw[(grp * kCellLanes) * kRedVals + q];
+      for (int l = 1; l < kCellLanes; l++) {
+        const float2 p = rw[(grp * kCellLanes + l) * kRedVals + q];
+        float s, e;
+        two_sum(acc.x, p.x, s, e);
+        acc = make_float2(s, acc.y + p.y + e);
       }
+      if (cc < a.ncell) a.cj[(q0 + q) * a.ncell + cc] = acc.x + acc.y;
     }
+    wave_sync();
+  }
+  if (sub == 0 && in) {
+    a.ns[c] = nstay;
+    a.nm[c] = nmove;
   }
 }
 
 // Does every cell's range hold its stayers and arrivals?  If not, a rebuild
-// is pending for this step.
-__global__ void k_check_f32(const int32_t *ns, const int32_t *arr, const int32_t *base, long long ncell,
-                            int32_t *lay) {
+// is pending for this step.  Also the sums, per block of cells, of the
+// generous and of the tight ranges of the new counts (for a rebuild).
+constexpr int kRebaseThreads = 1024;
+__global__ void __launch_bounds__(kBlock) k_check_f32(const int32_t *ns, const int32_t *arr, const int32_t *base,
+                                                      long long ncell, long long *part, long long nblk,
+                                                      int32_t *lay) {
+  __shared__ long long sh[2][kBlock / 64];
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ncell && ns[c] + arr[c] > base[c + 1] - base[c]) atomicOr(&lay[0], 1);
+  long long g = 0, t = 0;
+  if (c < ncell) {
+    const int n = ns[c] + arr[c];
+    if (n > base[c + 1] - base[c]) atomicOr(&lay[0], 1);
+    g = slot_cap(n, false);
+    t = slot_cap(n, true);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    g += __shfl_xor(g, o, 64);
+    t += __shfl_xor(t, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = g;
+    sh[1][threadIdx.x >> 6] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[nblk + blockIdx.x] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
 }
 
-// The rebuilt layout from the new counts (one block; runs only when a
-// rebuild is pending): generous ranges if they fit the buffers, else tight
-// ones (which always fit).
-constexpr int kRebaseThreads = 1024;
+// The rebuilt layout (one block; runs only when a rebuild is pending): the
+// block sums scanned, generous ranges if they fit the buffers, else tight
+// ones (which always fit), then every cell's new base.
 __global__ void __launch_bounds__(kRebaseThreads) k_rebase_f32(const int32_t *ns, const int32_t *arr, long long ncell,
+                                                               const long long *part, long long nblk,
                                                                long long nslots, int32_t *base_new, int32_t *lay,
                                                                unsigned long long *flag) {
   if (lay[0] == 0) return;
   __shared__ long long sh[kRebaseThreads];
+  __shared__ int use_tight;
   const int t = threadIdx.x;
-  const long long per = (ncell + kRebaseThreads - 1) / kRebaseThreads;
-  const long long c0 = min((long long)t * per, ncell), c1 = min(c0 + per, ncell);
+  const long long per = (nblk + kRebaseThreads - 1) / kRebaseThreads;
+  const long long b0 = min((long long)t * per, nblk), b1 = min(b0 + per, nblk);
+  long long total = 0, s = 0;
   for (int pass = 0; pass < 2; pass++) {
-    const bool tight = pass == 1;
-    long long s = 0;
-    for (long long c = c0; c < c1; c++) s += slot_cap(ns[c] + arr[c], tight);
+    s = 0;
+    for (long long b = b0; b < b1; b++) s += part[pass * nblk + b];
     sh[t] = s;
     __syncthreads();
     for (int o = 1; o < kRebaseThreads; o <<= 1) {
@@ -391,64 +444,113 @@ __global__ void __launch_bounds__(kRebaseThreads) k_rebase_f32(const int32_t *ns
       sh[t] += v;
       __syncthreads();
     }
-    const long long total = sh[kRebaseThreads - 1];
-    if (total <= nslots) {
-      long long run = sh[t] - s;
-      for (long long c = c0; c < c1; c++) {
-        base_new[c] = (int32_t)run;
-        run += slot_cap(ns[c] + arr[c], tight);
-      }
-      if (t == kRebaseThreads - 1) {
-        base_new[ncell] = (int32_t)total;
-        lay[1] += 1;
-      }
-      return;
-    }
+    total = sh[kRebaseThreads - 1];
+    if (t == 0) use_tight = pass;
+    __syncthreads();
+    if (total <= nslots) break;
     __syncthreads();
   }
-  // (cannot happen: the buffers hold the tight layout of any counts) no
-  // rebuild, so that every later access stays in the current layout
-  if (t == 0) {
-    atomicOr(flag, kFlagMigrate);
-    lay[0] = 0;
-  }
-}
-
-// Each mover reserves a slot in its destination's arrival range, which
-// follows the destination's stayers.  The order the atomics give is only
-// provisional: arrive_f32 puts each range in a canonical order.
-__global__ void __launch_bounds__(kBlock) k_place_f32(const int32_t *list, const int32_t *nmv, const int4 *ma,
-                                                      const int32_t *base, const int32_t *base_new,
-                                                      const int32_t *lay, const int32_t *ns, int32_t *put,
-                                                      int32_t *akey, unsigned long long *flag) {
-  const int n = *nmv;
-  const int32_t *B = lay[0] ? base_new : base;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int src = list[i];
-    const int d = ma[src].x & kCellMask;
-    const int k = atomicAdd(&put[d], 1);
-    const int pos = B[d] + ns[d] + k;
-    if (pos < B[d + 1])
-      akey[pos] = src;
-    else
+  if (total > nslots) {
+    // (cannot happen: the buffers hold the tight layout of any counts) no
+    // rebuild, so that every later access stays in the current layout
+    if (t == 0) {
       atomicOr(flag, kFlagMigrate);
+      lay[0] = 0;
+    }
+    return;
+  }
+  const bool tight = use_tight != 0;
+  long long run = sh[t] - s;
+  const long long c0 = b0 * kBlock, c1 = min(b1 * kBlock, ncell);
+  for (long long c = c0; c < c1; c++) {
+    base_new[c] = (int32_t)run;
+    run += slot_cap(ns[c] + arr[c], tight);
+  }
+  if (t == kRebaseThreads - 1) {
+    base_new[ncell] = (int32_t)total;
+    lay[1] += 1;
   }
 }
 
-// Per cell (16 lanes): the arrivals in ascending order of their old slot --
-// ranked in LDS for up to kArrFast arrivals, else ranked from global memory
-// into akey2 -- the rank-r arrival taken by lane r % 16: its record written after the stayers,
-// its currents summed as in the particle pass and added to the stayers' sum.
-// Then the cell's new count; the per-step counters are reset for the next
-// step.
+// Rebuild, step 1 of 2 (runs only when a rebuild is pending): each cell's
+// stayers from the output buffer (old layout) to the staging buffers'
+// free slots behind its movers -- exactly as many as it has stayers, since
+// its range there held stayers + movers.
+struct RelayoutArgs {
+  const int32_t *base, *base_new, *cnt, *ns, *nm, *lay;
+  int4 *pa_out, *pb_out;  // the step's output buffer
+  int4 *ma, *mb;          // staging
+  uint32_t *fwd;          // with a tape: source slot -> new slot, or null
+  int32_t *put, *akey;
+  long long ncell;
+  unsigned long long *flag;
+};
+
+__global__ void __launch_bounds__(kBlock) k_restage_f32(RelayoutArgs a) {
+  if (a.lay[0] == 0) return;
+  const long long c = (long long)blockIdx.x * kTileCells + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & (kCellLanes - 1);
+  if (c >= a.ncell) return;
+  const int b = a.base[c], s = b + a.nm[c], n = a.ns[c];
+  for (int k = sub; k < n; k += kCellLanes) {
+    a.ma[s + k] = a.pa_out[b + k];
+    a.mb[s + k] = a.pb_out[b + k];
+  }
+}
+
+// Rebuild, step 2 of 2: the stayers to their new slots in the output buffer
+// (with a tape, their map entries moved along), and the cell's movers placed
+// in the new layout's arrival ranges.
+__global__ void __launch_bounds__(kBlock) k_relayout_f32(RelayoutArgs a) {
+  if (a.lay[0] == 0) return;
+  const long long c = (long long)blockIdx.x * kTileCells + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & (kCellLanes - 1);
+  if (c >= a.ncell) return;
+  const int b = a.base[c], m = a.nm[c], n = a.ns[c], bn = a.base_new[c];
+  for (int k = sub; k < n; k += kCellLanes) {
+    a.pa_out[bn + k] = a.ma[b + m + k];
+    a.pb_out[bn + k] = a.mb[b + m + k];
+  }
+  if (a.fwd) {
+    // stayers' entries point into [b, b + n); movers' entries are rewritten
+    // by arrive_f32
+    const int cn = a.cnt[c];
+    for (int j = sub; j < cn; j += kCellLanes) {
+      const uint32_t f = a.fwd[b + j];
+      if (f >= (uint32_t)b && f < (uint32_t)(b + n)) a.fwd[b + j] = f - (uint32_t)b + (uint32_t)bn;
+    }
+  }
+  for (int j = sub; j < m; j += kCellLanes) {
+    const int d = a.ma[b + j].x & kCellMask;
+    const int k = atomicAdd(&a.put[d], 1);
+    const int e1 = a.base_new[d + 1];
+    if (k < e1 - a.base_new[d])
+      a.akey[e1 - 1 - k] = b + j;
+    else
+      atomicOr(a.flag, kFlagMigrate);
+  }
+}
+
+// Per cell (4 lanes, 64 cells per block): the arrivals in ascending order of
+// their staging slot (source cell, then order within it: the previous array
+// order) -- ranked in LDS for up to kArrFast arrivals, else ranked from global
+// memory into akey2 -- the rank-r arrival taken by lane r % 4: its record
+// written after the stayers, its currents summed as in the particle pass
+// (lane order, TwoSum) and added to the stayers' sum.  Then the cell's new
+// count; the per-step counters are reset for the next step.
+constexpr int kArrLanes = 4;
+constexpr int kArrCells = kBlock / kArrLanes;
+constexpr int kArrFast = 32;
+
 struct ArriveArgs {
   const int32_t *base, *base_new, *lay;
   const int32_t *ns;
-  int32_t *arr, *put, *cnt, *nmv;
+  int32_t *arr, *put, *cnt;
   const int32_t *akey;
-  int32_t *akey2;  // the sorted keys of the cells with more than kArrFast arrivals
+  int32_t *akey2;  // the sorted keys of the ranges with more than kArrFast arrivals
   const int4 *ma, *mb;
   const float4 *mv;
+  const int32_t *mvsrc;  // with a tape
   int4 *pa_out, *pb_out;
   uint32_t *fwd;
   float *cj;
@@ -458,64 +560,66 @@ struct ArriveArgs {
 };
 
 __global__ void __launch_bounds__(kBlock) k_arrive_f32(ArriveArgs a) {
-  __shared__ int keys[kTileCells][kArrFast];
-  __shared__ int sorted[kTileCells][kArrFast];
+  __shared__ int keys[kArrCells][kArrFast];
+  __shared__ int sorted[kArrCells][kArrFast];
   __shared__ float jc_s[EMO_NSPEC];
   if (threadIdx.x < EMO_NSPEC) jc_s[threadIdx.x] = a.jc[threadIdx.x];
-  if (blockIdx.x == 0 && threadIdx.x == 0) *a.nmv = 0;
-  const int grp = threadIdx.x >> 4, sub = threadIdx.x & (kCellLanes - 1);
-  const long long c = (long long)blockIdx.x * kTileCells + grp;
+  const int grp = threadIdx.x / kArrLanes, sub = threadIdx.x & (kArrLanes - 1);
+  const long long c = (long long)xcd_tile() * kArrCells + grp;
   const bool in = c < a.ncell;
   const int32_t *B = a.lay[0] ? a.base_new : a.base;
   int na = in ? a.arr[c] : 0;
   const int s0 = in ? B[c] + a.ns[c] : 0;
-  if (in && s0 + na > B[c + 1]) {  // (place_f32 has flagged it)
+  const int e1 = in ? B[c + 1] : 0;
+  if (in && s0 + na > e1) {  // (the check has rebuilt the layout, so only after a flagged error)
     atomicOr(a.flag, kFlagMigrate);
-    na = max(B[c + 1] - s0, 0);
+    na = max(e1 - s0, 0);
   }
+  const int k0 = e1 - na;  // the range's keys
   const bool fast = na <= kArrFast;
   if (fast)
-    for (int i = sub; i < na; i += kCellLanes) keys[grp][i] = a.akey[s0 + i];
+    for (int i = sub; i < na; i += kArrLanes) keys[grp][i] = a.akey[k0 + i];
   __syncthreads();
   if (fast) {
-    for (int i = sub; i < na; i += kCellLanes) {
+    for (int i = sub; i < na; i += kArrLanes) {
       const int k = keys[grp][i];
       int r = 0;
       for (int j = 0; j < na; j++) r += keys[grp][j] < k ? 1 : 0;
       sorted[grp][r] = k;
     }
   } else {
-    for (int i = sub; i < na; i += kCellLanes) {
-      const int k = a.akey[s0 + i];
+    for (int i = sub; i < na; i += kArrLanes) {
+      const int k = a.akey[k0 + i];
       int r = 0;
-      for (int j = 0; j < na; j++) r += a.akey[s0 + j] < k ? 1 : 0;
-      a.akey2[s0 + r] = k;
+      for (int j = 0; j < na; j++) r += a.akey[k0 + j] < k ? 1 : 0;
+      a.akey2[k0 + r] = k;
     }
   }
-  // the block's global stores complete, then every lane reads past its L1
-  __threadfence();
+  // (a work-group barrier orders the block's global stores before its loads)
   __syncthreads();
-  __threadfence();
   float hi[12], lo[12];
 #pragma unroll
   for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
-  auto take = [&](int src, int r) {
-    const int4 A = a.ma[src], Bv = a.mb[src];
-    const float4 v4 = a.mv[src];
+  auto take = [&](int st, int r) {
+    const int4 A = a.ma[st], Bv = a.mb[st];
+    const float4 v4 = a.mv[st];
     const int pos = s0 + r;
     a.pa_out[pos] = A;
     a.pb_out[pos] = Bv;
-    if (a.fwd) a.fwd[src] = (uint32_t)pos;
+    if (a.fwd) a.fwd[a.mvsrc[st]] = (uint32_t)pos;
     const float f[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
     const float v[3] = {v4.x, v4.y, v4.z};
     deposit_j(hi, lo, f, v, jc_s[(uint32_t)A.x >> kSpeciesShift]);
   };
   if (fast) {
-    for (int r = sub; r < na; r += kCellLanes) take(sorted[grp][r], r);
+    for (int r = sub; r < na; r += kArrLanes) take(sorted[grp][r], r);
   } else {
-    for (int r = sub; r < na; r += kCellLanes) take(a.akey2[s0 + r], r);
+    for (int r = sub; r < na; r += kArrLanes) take(a.akey2[k0 + r], r);
   }
-  dep_reduce16(hi, lo);
+  // the 4 lanes' sums in a fixed order (quad permutations)
+  static_assert(kArrLanes == 4, "the reduction covers 4 lanes");
+  dep_level<0x4E>(hi, lo);
+  dep_level<0xB1>(hi, lo);
   if (sub == 0 && in) {
     if (na > 0) {
 #pragma unroll
@@ -1002,80 +1106,90 @@ void launch_cellvec(emo_ctx *c) {
   hipLaunchKernelGGL(k_cellvec_f32, dim3(grid_for(c->geo.ncell)), dim3(kBlock), 0, c->stream, a);
 }
 
-template <bool REDO>
 void launch_push_deposit(emo_ctx *c, const PushDepArgs &a, dim3 grid) {
   bool all_per = true;
   for (int q = 0; q < 6; q++) all_per = all_per && c->cfg.parts_bound[q] == EMO_BC_PERIODIC;
   if (c->cfg.relativity && all_per)
-    hipLaunchKernelGGL((k_push_deposit_f32<true, true, REDO>), grid, dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_push_deposit_f32<true, true>), grid, dim3(kBlock), 0, c->stream, a);
   else if (c->cfg.relativity)
-    hipLaunchKernelGGL((k_push_deposit_f32<true, false, REDO>), grid, dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_push_deposit_f32<true, false>), grid, dim3(kBlock), 0, c->stream, a);
   else if (all_per)
-    hipLaunchKernelGGL((k_push_deposit_f32<false, true, REDO>), grid, dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_push_deposit_f32<false, true>), grid, dim3(kBlock), 0, c->stream, a);
   else
-    hipLaunchKernelGGL((k_push_deposit_f32<false, false, REDO>), grid, dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_push_deposit_f32<false, false>), grid, dim3(kBlock), 0, c->stream, a);
 }
 
 // One step's particle work, from the current record buffers into the
 // alternate ones (which then become current): gather vectors, push +
-// deposit, the layout check (and the rebuild it may need), mover placement,
-// arrivals, grid sum of the currents.  With fwd_out (a tape), the step's
-// map source slot -> new slot, and the movers' new records go to stage_a /
-// stage_b instead of over their source slots, which then stay untouched.
-int particle_step(emo_ctx *c, double dt, uint32_t *fwd_out, int4 *stage_a, int4 *stage_b) {
+// deposit, the layout check (and the rebuild it may need), arrivals, grid
+// sum of the currents.  With a tape, the step's map source slot -> new slot,
+// and the movers staged in the tape's arrays instead of the input buffer,
+// which then stays untouched.
+int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
   ChosenState *s = c->ch;
   const Geo &g = c->geo;
   launch_cellvec(c);
   const long long ntiles = (g.ncell + kTileCells - 1) / kTileCells;
+  uint32_t *fwd = tape ? tape->fwd : nullptr;
   PushDepArgs a;
   a.k = push_consts(c, dt);
   a.pa = s->pa;
   a.pb = s->pb;
   a.pa_out = s->pa_alt;
   a.pb_out = s->pb_alt;
-  a.ma = stage_a ? stage_a : s->pa;
-  a.mb = stage_b ? stage_b : s->pb;
+  a.ma = tape ? tape->stage_a : s->pa;
+  a.mb = tape ? tape->stage_b : s->pb;
   a.mv = s->mv;
+  a.mvsrc = tape ? tape->mvsrc : nullptr;
   a.g0 = s->g0_active ? s->g0 : nullptr;
   a.base = s->base;
   a.cnt = s->cnt;
-  a.base_out = s->base;
   a.ns = s->ns;
+  a.nm = s->nm;
   a.arr = s->arr;
-  a.list = s->list;
-  a.nmv = s->nmv;
+  a.akey = s->akey;
   a.cj = s->cj;
-  a.fwd = fwd_out;
+  a.fwd = fwd;
   a.lay = s->lay;
   a.ncell = g.ncell;
   a.ntiles = ntiles;
   a.flag = c->dflag;
   {
     TimedLaunch t(c, "push_deposit_f32");
-    launch_push_deposit<false>(c, a, dim3(xcd_grid((unsigned)ntiles)));
+    launch_push_deposit(c, a, dim3(xcd_grid((unsigned)ntiles)));
   }
+  const long long nblk = (g.ncell + kBlock - 1) / kBlock;
   {
     TimedLaunch t(c, "check_f32");
-    hipLaunchKernelGGL(k_check_f32, dim3(grid_for(g.ncell)), dim3(kBlock), 0, c->stream, (const int32_t *)s->ns,
-                       (const int32_t *)s->arr, (const int32_t *)s->base, g.ncell, s->lay);
+    hipLaunchKernelGGL(k_check_f32, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, (const int32_t *)s->ns,
+                       (const int32_t *)s->arr, (const int32_t *)s->base, g.ncell, s->rpart, nblk, s->lay);
   }
   {
     TimedLaunch t(c, "rebase_f32");
     hipLaunchKernelGGL(k_rebase_f32, dim3(1), dim3(kRebaseThreads), 0, c->stream, (const int32_t *)s->ns,
-                       (const int32_t *)s->arr, g.ncell, (long long)s->nslots, s->base_new, s->lay, c->dflag);
+                       (const int32_t *)s->arr, g.ncell, (const long long *)s->rpart, nblk, (long long)s->nslots,
+                       s->base_new, s->lay, c->dflag);
   }
   {
-    PushDepArgs r = a;
-    r.base_out = s->base_new;
-    TimedLaunch t(c, "push_deposit_redo_f32");
-    launch_push_deposit<true>(c, r, dim3((unsigned)std::min<long long>(ntiles, 1024)));
-  }
-  {
-    TimedLaunch t(c, "place_f32");
-    hipLaunchKernelGGL(k_place_f32, dim3(1024), dim3(kBlock), 0, c->stream, (const int32_t *)s->list,
-                       (const int32_t *)s->nmv, (const int4 *)a.ma, (const int32_t *)s->base,
-                       (const int32_t *)s->base_new, (const int32_t *)s->lay, (const int32_t *)s->ns, s->put, s->akey,
-                       c->dflag);
+    RelayoutArgs r;
+    r.base = s->base;
+    r.base_new = s->base_new;
+    r.cnt = s->cnt;
+    r.ns = s->ns;
+    r.nm = s->nm;
+    r.lay = s->lay;
+    r.pa_out = s->pa_alt;
+    r.pb_out = s->pb_alt;
+    r.ma = a.ma;
+    r.mb = a.mb;
+    r.fwd = fwd;
+    r.put = s->put;
+    r.akey = s->akey;
+    r.ncell = g.ncell;
+    r.flag = c->dflag;
+    TimedLaunch t(c, "relayout_f32");
+    hipLaunchKernelGGL(k_restage_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
+    hipLaunchKernelGGL(k_relayout_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
   }
   {
     ArriveArgs r;
@@ -1086,21 +1200,22 @@ int particle_step(emo_ctx *c, double dt, uint32_t *fwd_out, int4 *stage_a, int4 
     r.arr = s->arr;
     r.put = s->put;
     r.cnt = s->cnt;
-    r.nmv = s->nmv;
     r.akey = s->akey;
     r.akey2 = s->akey2;
     r.ma = a.ma;
     r.mb = a.mb;
     r.mv = s->mv;
+    r.mvsrc = a.mvsrc;
     r.pa_out = s->pa_alt;
     r.pb_out = s->pb_alt;
-    r.fwd = fwd_out;
+    r.fwd = fwd;
     r.cj = s->cj;
     for (int q = 0; q < EMO_NSPEC; q++) r.jc[q] = a.k.s.test[q] ? 0.f : a.k.s.jc[q];
     r.ncell = g.ncell;
     r.flag = c->dflag;
     TimedLaunch t(c, "arrive_f32");
-    hipLaunchKernelGGL(k_arrive_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
+    hipLaunchKernelGGL(k_arrive_f32, dim3(xcd_grid((unsigned)((g.ncell + kArrCells - 1) / kArrCells))), dim3(kBlock),
+                       0, c->stream, r);
   }
   {
     TimedLaunch t(c, "commit_f32");
@@ -1145,16 +1260,16 @@ int fp32_create(emo_ctx *c) {
   if (!rc) rc = dalloc(c, &s->mv, nslots);
   if (!rc) rc = dalloc(c, &s->akey, nslots);
   if (!rc) rc = dalloc(c, &s->akey2, nslots);
-  if (!rc) rc = dalloc(c, &s->list, cap);
   if (!rc) rc = dalloc(c, &s->base, nc1);
   if (!rc) rc = dalloc(c, &s->base_new, nc1);
   if (!rc) rc = dalloc(c, &s->cnt, nc1);
   if (!rc) rc = dalloc(c, &s->ns, nc1);
+  if (!rc) rc = dalloc(c, &s->nm, nc1);
+  if (!rc) rc = dalloc(c, &s->rpart, 2 * (nc1 / kBlock + 1));
   if (!rc) rc = dalloc(c, &s->arr, nc1);
   if (!rc) rc = dalloc(c, &s->put, nc1);
   if (!rc) rc = dalloc(c, &s->dcount, nc1);
   if (!rc) rc = dalloc(c, &s->doff, nc1);
-  if (!rc) rc = dalloc(c, &s->nmv, 4);
   if (!rc) rc = dalloc(c, &s->lay, 4);
   if (!rc) rc = dalloc(c, &s->key, cap);
   if (!rc) rc = dalloc(c, &s->key_alt, cap);
@@ -1179,9 +1294,8 @@ int fp32_create(emo_ctx *c) {
                     s->Btfix[0], s->Btfix[1], s->Btfix[2], s->fs.J[0], s->fs.J[1], s->fs.J[2], s->fs.rho};
   for (int q = 0; q < 13; q++) EMO_HIP_CHECK_RET(c, hipMemset(all[q], 0, nt * sizeof(float)));
   // per-step counters start (and end every step) at zero; an empty layout
-  int32_t *zero[] = {s->cnt, s->ns, s->arr, s->put, s->dcount};
+  int32_t *zero[] = {s->cnt, s->ns, s->nm, s->arr, s->put, s->dcount};
   for (int32_t *p : zero) EMO_HIP_CHECK_RET(c, hipMemset(p, 0, nc1 * 4));
-  EMO_HIP_CHECK_RET(c, hipMemset(s->nmv, 0, 16));
   // every arrival key ever read is a slot number (also where a flagged error
   // left a range short)
   EMO_HIP_CHECK_RET(c, hipMemset(s->akey, 0, (size_t)nslots * 4));
@@ -1206,8 +1320,8 @@ int fp32_create(emo_ctx *c) {
 void fp32_destroy(emo_ctx *c) {
   ChosenState *s = c->ch;
   if (!s) return;
-  void *ps[] = {s->pa, s->pa_alt, s->pb, s->pb_alt, s->mv, s->akey, s->akey2, s->list, s->base, s->base_new, s->cnt,
-                s->ns, s->arr, s->put, s->dcount, s->doff, s->nmv, s->lay, s->key, s->key_alt, s->perm,
+  void *ps[] = {s->pa, s->pa_alt, s->pb, s->pb_alt, s->mv, s->akey, s->akey2, s->base, s->base_new, s->cnt,
+                s->ns, s->nm, s->rpart, s->arr, s->put, s->dcount, s->doff, s->lay, s->key, s->key_alt, s->perm,
                 s->perm_alt, s->sort_tmp, s->scan_tmp, s->cj, s->cv, s->gam_id, s->fs.rho};
   for (void *p : ps) dfree(p);
   if (s->g0_owned) dfree(s->g0);
@@ -1664,9 +1778,9 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
 
 PushK fp32_push_consts(emo_ctx *c, double dt) { return push_consts(c, dt); }
 
-int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out, int4 *stage_a, int4 *stage_b) {
+int fp32_step_one(emo_ctx *c, double t, double dt, const Fp32Tape *tape) {
   ChosenState *s = c->ch;
-  int rc = particle_step(c, dt, fwd_out, stage_a, stage_b);
+  int rc = particle_step(c, dt, tape);
   if (rc) return rc;
   if (s->g0_active) {
     // the stored gamma was used by this first push only (Q3): from now on
@@ -1712,7 +1826,7 @@ Fp32View fp32_view(emo_ctx *c) {
 int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
   double t = t0;
   for (int k = 0; k < nsteps; k++) {
-    int rc = fp32_step_one(c, t, dt, nullptr, nullptr, nullptr);
+    int rc = fp32_step_one(c, t, dt, nullptr);
     if (rc) return rc;
     t += dt;
   }

@@ -291,11 +291,16 @@ struct Fp32View {
   int32_t *lay;
 };
 Fp32View fp32_view(emo_ctx *c);
-// one step of fp32_step without the flag readback; with fwd_out, the step's
-// map source slot -> new slot, and the movers' new records written to
-// stage_a / stage_b (nslots each) instead of over their source slots, which
-// then keep the step's input intact
-int fp32_step_one(emo_ctx *c, double t, double dt, uint32_t *fwd_out, int4 *stage_a, int4 *stage_b);
+// A recomputed step's outputs for the adjoint: the map source slot -> new
+// slot, and staging arrays (nslots each) for the movers' new records and
+// source slots, so that the step's input records stay intact.
+struct Fp32Tape {
+  uint32_t *fwd;
+  int4 *stage_a, *stage_b;
+  int32_t *mvsrc;
+};
+// one step of fp32_step without the flag readback, optionally taped
+int fp32_step_one(emo_ctx *c, double t, double dt, const Fp32Tape *tape);
 // the device error flags of the FP32 form (stream synchronisation)
 int fp32_check(emo_ctx *c);
 int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, int32_t K, double *energy,

@@ -53,10 +53,7 @@ def algorithmic_bytes(kernel: str, np_: int, ncell: int, relativity: int) -> flo
         "yee_E_f64": 96 * ncell,
         "yee_B_f64": 72 * ncell,
         # FP32 chosen form: 32-byte particles (cell word, offset[3], u[3], int32 id)
-        "push_rank_f32": 35 * np_ + 36 * ncell,
-        "inv_f32": 7 * np_ + 10 * ncell,
-        "move_deposit_f32": 68 * np_ + 88 * ncell,
-        "base_f32": 4 * 27 * ncell,
+        "push_deposit_f32": 64 * np_ + 140 * ncell,
         "jsum_f32": 48 * ncell + 12 * ng,
         "yee_E_f32": 48 * ncell,
         "yee_B_f32": 36 * ncell,
@@ -68,8 +65,8 @@ KERNELS = {
     "fp64": ["pgd_f64", "far_f64", "rank_f64", "base_f64", "scan_f64", "scatter_f64", "gather_f64", "push_f64",
              "particle_bc_f64", "sort_keys", "sort_radix", "sort_permute", "deposit_cells_f64", "deposit_grid_f64",
              "yee_E_f64", "yee_B_f64"],
-    "fp32": ["cellvec_f32", "push_rank_f32", "far_f32", "base_f32", "scan_f32", "inv_f32", "move_deposit_f32", "jsum_f32",
-             "yee_E_f32", "yee_B_f32"],
+    "fp32": ["cellvec_f32", "push_deposit_f32", "check_f32", "rebase_f32", "relayout_f32",
+             "arrive_f32", "commit_f32", "jsum_f32", "yee_E_f32", "yee_B_f32"],
 }
 
 
