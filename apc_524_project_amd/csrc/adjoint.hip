@@ -108,10 +108,8 @@ struct FillAdjArgs {
   Geo g;
   double *f;
   int dim, idx;
-  double waist;
-  double off[3];
-  double Lc[3];
-  double *out;  // the weighted plane sum
+  const double *prof;  // the plane's transverse profile (nt[u] x nt[w]), or null: uniform
+  double *out;         // the weighted plane sum
 };
 
 // ---------------------------------------------------------------------------
@@ -341,12 +339,7 @@ __global__ void __launch_bounds__(kBlock) k_plane_fill_adj_multi(FillAdjMulti m)
     ii[u] = pu;
     ii[w] = pw;
     long long p = gidx(g, ii[0], ii[1], ii[2]);
-    double prof = 1.0;
-    if (a.waist > 0) {
-      double ya = g.x0[u] + g.d[u] * (pu - 1) + a.off[u] - a.Lc[u];
-      double yb = g.x0[w] + g.d[w] * (pw - 1) + a.off[w] - a.Lc[w];
-      prof = exp(-(ya * ya + yb * yb) / (a.waist * a.waist));
-    }
+    const double prof = a.prof ? a.prof[t] : 1.0;
     s += a.f[p] * prof;
     a.f[p] = 0.0;
   }
@@ -826,11 +819,7 @@ void fields_step_adj(emo_ctx *c, hipStream_t as, AdjFields &af, const std::vecto
         fa.f = f.pass == 0 ? af.Eb[f.comp] : af.Ab[mnew][f.comp];
         fa.dim = f.dim;
         fa.idx = f.idx;
-        fa.waist = c->cfg.waist;
-        for (int q = 0; q < 3; q++) {
-          fa.off[q] = f.off[q];
-          fa.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
-        }
+        fa.prof = c->wprof[f.dim][f.pass == 0 ? f.comp : 3 + f.comp];
         fa.out = dsums_row + pi;
       }
       if (nb) hipLaunchKernelGGL(k_plane_fill_adj_multi, dim3(nb), dim3(kBlock), 0, as, fm);

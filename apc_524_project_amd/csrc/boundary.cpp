@@ -7,6 +7,9 @@
 // only fill and copy planes.
 #include <cmath>
 
+#include <cmath>
+#include <vector>
+
 #include "emo_internal.hpp"
 
 namespace emo {
@@ -77,13 +80,7 @@ void light_bc(emo_ctx *c, FieldSet<T> &fs, int side, double t, double dt) {
       pf.idx = edge_plane_index(c->geo, fid, dim, sign > 0);
       pf.base = pass == 0 ? cf.E0[eid] : cf.B0[bid];
       pf.wave = has ? (pass == 0 ? ewave : bwave) : 0.0;
-      pf.waist = has ? cf.waist : 0.0;
-      // staggering offsets of Grid::getGridPhys (src/grid/grid.cpp:415-440)
-      int comp = fid % 3;
-      for (int q = 0; q < 3; q++) {
-        if (fid < 3) pf.off[q] = q == comp ? 0.5 * c->geo.d[q] : 0.0;
-        else pf.off[q] = q == comp ? 0.0 : 0.5 * c->geo.d[q];
-      }
+      pf.prof = has ? c->wprof[dim][fid] : nullptr;
       fills[nf++] = pf;
     }
   }
@@ -132,11 +129,62 @@ void potential_bc(emo_ctx *c, T *u, const double *bound) {
       pf.idx = right ? 1 + c->geo.n[d] : 1;  // PoissonBC offset_ (poissonBC.cpp:15-20)
       pf.base = bound[side];
       pf.wave = 0.0;
-      pf.waist = 0.0;
-      pf.off[0] = pf.off[1] = pf.off[2] = 0.0;
+      pf.prof = nullptr;
       fk_plane_fill<T>(c, pf);
     }
   }
+}
+
+// The transverse profile exp(-(ya^2 + yb^2) / waist^2) of the wave part on
+// the planes normal to dim, for field fid (0..5 = Ex..Bz): ya, yb the
+// distances of the field's staggered points (Grid::getGridPhys,
+// src/grid/grid.cpp:415-440: E_i offset dx_i/2 along i, B_i offset d/2 along
+// the two other axes) from the box centre, over the full padded plane.
+// Computed on the host with the host's libm, once per context, so that both
+// precisions' plane fills and the adjoint's use the same values.
+int build_waist_profiles(emo_ctx *c) {
+  const emo_config &cf = c->cfg;
+  if (!(cf.waist > 0) || cf.nwaves <= 0) return EMO_OK;
+  const Geo &g = c->geo;
+  for (int dim = 0; dim < 3; dim++) {
+    if (cf.fields_bound[2 * dim] == EMO_BC_PERIODIC && cf.fields_bound[2 * dim + 1] == EMO_BC_PERIODIC) continue;
+    const int u = dim == 0 ? 1 : 0, w = dim == 2 ? 1 : 2;
+    const size_t np = (size_t)g.nt[u] * g.nt[w];
+    std::vector<double> h(np);
+    for (int fid = 0; fid < 6; fid++) {
+      const int comp = fid % 3;
+      double off[3];
+      for (int q = 0; q < 3; q++) {
+        if (fid < 3) off[q] = q == comp ? 0.5 * g.d[q] : 0.0;
+        else off[q] = q == comp ? 0.0 : 0.5 * g.d[q];
+      }
+      const double cu = cf.xyz0[u] + 0.5 * cf.Lxyz[u], cw = cf.xyz0[w] + 0.5 * cf.Lxyz[w];
+      for (int pu = 0; pu < g.nt[u]; pu++)
+        for (int pw = 0; pw < g.nt[w]; pw++) {
+          const double ya = g.x0[u] + g.d[u] * (pu - 1) + off[u] - cu;
+          const double yb = g.x0[w] + g.d[w] * (pw - 1) + off[w] - cw;
+          h[(size_t)pu * g.nt[w] + pw] = std::exp(-(ya * ya + yb * yb) / (cf.waist * cf.waist));
+        }
+      if (hipMalloc((void **)&c->wprof[dim][fid], np * sizeof(double)) != hipSuccess) {
+        c->wprof[dim][fid] = nullptr;
+        c->err = "hipMalloc failed for the pulse profile";
+        return EMO_ERR_ALLOC;
+      }
+      if (hipMemcpy(c->wprof[dim][fid], h.data(), np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        c->err = "hipMemcpy failed for the pulse profile";
+        return EMO_ERR_HIP;
+      }
+    }
+  }
+  return EMO_OK;
+}
+
+void free_waist_profiles(emo_ctx *c) {
+  for (auto &row : c->wprof)
+    for (double *&p : row) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+    }
 }
 
 template void refresh_fields<double>(emo_ctx *, FieldSet<double> &, double, double);

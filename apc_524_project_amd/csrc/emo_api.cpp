@@ -256,6 +256,7 @@ int emo_create(const emo_config *cfg, emo_ctx **out) {
   } else {
     rc = fp32_create(c);
   }
+  if (!rc) rc = build_waist_profiles(c);
   if (rc) {
     fprintf(stderr, "emo_create: %s\n", c->err.c_str());
     emo_destroy(c);
@@ -272,6 +273,7 @@ int emo_create(const emo_config *cfg, emo_ctx **out) {
 void emo_destroy(emo_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  free_waist_profiles(c);
   for (auto &kv : c->pending)
     for (auto &r : kv.second) {
       hipEventDestroy(r.a);

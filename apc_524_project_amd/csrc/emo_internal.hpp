@@ -30,8 +30,9 @@ struct PlaneFill {
   int idx;        // plane index along dim
   double base;    // E0/B0 component
   double wave;    // pulse value added on top
-  double waist;   // <=0: uniform
-  double off[3];  // staggering offsets of this field (Grid::getGridPhys)
+  // the wave part's transverse profile over the plane (nt[u] x nt[w], u < w
+  // the two other dims), device memory; null: uniform (the reference)
+  const double *prof;
 };
 
 // One precision's grid arrays.  Bfix are the fixed allocations that the
@@ -126,6 +127,11 @@ struct emo_ctx {
   void *adj_pool = nullptr;
   size_t adj_pool_bytes = 0;
   hipStream_t adj_stream = nullptr;  // the adjoint's backward sweep
+
+  // transverse Gaussian profiles of an injected pulse (emo_config.waist > 0)
+  // per plane normal and field id 0..5 (E, B), built on the host at
+  // creation (boundary.cpp)
+  double *wprof[3][6] = {};
 
   // the device this context was created on, and that device's copy of the
   // static workspace of emo_init_poisson_noalloc (emo_api.cpp)
@@ -241,6 +247,9 @@ template <typename T>
 void fk_subtract_mean(emo_ctx *c, T *f, double *partials, int nblocks, double *sum_out);
 
 // ---- boundary host logic (boundary.cpp) ----
+// the waist profiles of the LightBC planes (emo_config.waist > 0)
+int build_waist_profiles(emo_ctx *c);
+void free_waist_profiles(emo_ctx *c);
 // Grid::executeBC(-1): periodic ghost copies (through Bfix for B) and LightBC
 template <typename T>
 void refresh_fields(emo_ctx *c, FieldSet<T> &fs, double t, double dt);

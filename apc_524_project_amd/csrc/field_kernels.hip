@@ -159,7 +159,6 @@ struct FillMultiArgs {
   PlaneFill pf[6];
   int npf;
   long long start[7];
-  double Lc[3];
 };
 
 template <typename T>
@@ -181,10 +180,8 @@ __global__ void __launch_bounds__(kBlock) k_plane_fill_multi(FillMultiArgs a) {
   ii[u] = pu;
   ii[w] = pw;
   double v;
-  if (pf.waist > 0) {
-    double ya = g.x0[u] + g.d[u] * (pu - 1) + pf.off[u] - a.Lc[u];
-    double yb = g.x0[w] + g.d[w] * (pw - 1) + pf.off[w] - a.Lc[w];
-    v = pf.base + pf.wave * exp(-(ya * ya + yb * yb) / (pf.waist * pf.waist));
+  if (pf.prof) {
+    v = pf.base + pf.wave * pf.prof[(long long)pu * g.nt[w] + pw];
   } else {
     v = pf.base + pf.wave;
   }
@@ -246,13 +243,13 @@ __global__ void __launch_bounds__(kBlock) k_evolve_B(YeeArgs<T> a) {
 // ---------------------------------------------------------------------------
 // LightBC plane fill: Grid::setFieldInPlane over the full (n+3)^2 plane
 // (src/grid/oGrid.cpp:446-476) with the value of LightBC::applyBCs
-// (src/grid/lightBC.cpp:165-198).  With waist>0 the wave part is shaped by a
-// transverse Gaussian (an extension; the reference is plane-uniform).
+// (src/grid/lightBC.cpp:165-198).  With a profile table the wave part is
+// shaped by a transverse Gaussian (an extension; the reference is
+// plane-uniform).
 // ---------------------------------------------------------------------------
 struct FillArgs {
   Geo g;
   PlaneFill pf;
-  double Lc[3];  // box centre per dim
 };
 
 template <typename T>
@@ -271,10 +268,8 @@ __global__ void __launch_bounds__(kBlock) k_plane_fill(FillArgs a) {
   ii[u] = pu;
   ii[w] = pw;
   double v;
-  if (pf.waist > 0) {
-    double ya = g.x0[u] + g.d[u] * (pu - 1) + pf.off[u] - a.Lc[u];
-    double yb = g.x0[w] + g.d[w] * (pw - 1) + pf.off[w] - a.Lc[w];
-    v = pf.base + pf.wave * exp(-(ya * ya + yb * yb) / (pf.waist * pf.waist));
+  if (pf.prof) {
+    v = pf.base + pf.wave * pf.prof[(long long)pu * g.nt[w] + pw];
   } else {
     v = pf.base + pf.wave;
   }
@@ -532,7 +527,6 @@ void fk_plane_fill(emo_ctx *c, const PlaneFill &pf) {
   FillArgs a;
   a.g = c->geo;
   a.pf = pf;
-  for (int q = 0; q < 3; q++) a.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
   int d = pf.dim;
   int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
   hipLaunchKernelGGL(k_plane_fill<T>, dim3(grid_for((long long)c->geo.nt[u] * c->geo.nt[w])),
@@ -552,7 +546,6 @@ void fk_plane_fill_multi(emo_ctx *c, const PlaneFill *pf, int n) {
     const int u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
     a.start[s + 1] = a.start[s] + (long long)c->geo.nt[u] * c->geo.nt[w];
   }
-  for (int q = 0; q < 3; q++) a.Lc[q] = c->cfg.xyz0[q] + 0.5 * c->cfg.Lxyz[q];
   hipLaunchKernelGGL(k_plane_fill_multi<T>, dim3(grid_for(a.start[n])), dim3(kBlock), 0, c->stream, a);
 }
 

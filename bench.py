@@ -185,7 +185,28 @@ def measured_traffic(kernel: str, mode: str, scale: float):
     return int(e["hbm_bytes"]) if e else None
 
 
+def path_bytes(mode: str, np_: int, ncell: int, relativity: int) -> int:
+    """Algorithmic bytes of one pass of the path (push + gather + deposit)
+    over np_ particles and ncell cells: each particle's state read once and
+    written once, each cell's field block read once and its current written
+    once.
+      FP32 chosen form: 56 B per particle (a 28-byte state: int32 cell,
+      fp32 offset[3], fp32 u[3]) and 60 B per cell (E, B, B_tm1: 9 fp32
+      values read; J: 3 two-float values written).
+      FP64 plain form: 112 B per particle relativistic (x, v, gamma: 56 B),
+      96 B for Boris (x, v), and 96 B per cell (9 fp64 values read, 3
+      written)."""
+    if mode == "fp32":
+        return 56 * np_ + 60 * ncell
+    return (112 if relativity else 96) * np_ + 96 * ncell
+
+
 def roofline_for(res):
+    """The dominant hand-written kernel of the step (longest average
+    duration, HIP events around its launches on the library stream during
+    the timed steps) against HBM: achieved = the path's algorithmic bytes
+    per launch / its average duration.  Also the whole step against the same
+    bytes, and the kernel's own operand bytes as a secondary figure."""
     w = res["workload"]
     kt = res["ktimes"]
     own = {k: v for k, v in kt.items() if k not in LIBRARY_CALLS}
@@ -194,17 +215,50 @@ def roofline_for(res):
     dom = res.get("dominant") or max(own, key=lambda k: own[k][0])
     ms, n = kt[dom]
     avg_s = ms / n / 1e3
-    b = algorithmic_bytes(dom, w.nparticles, w.ncells, int(w.cfg.relativity))
-    if b is None:
-        return {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": None, "traffic": None, "avg_ms": ms / n}
+    rel = int(w.cfg.relativity)
+    b = path_bytes(res["mode"], w.nparticles, w.ncells, rel)
     ach = b / avg_s / 1e9
     traffic = measured_traffic(dom, res["mode"], res["scale"])
+    step_s = res["ms_per_step"] / 1e3
+    kown = algorithmic_bytes(dom, w.nparticles, w.ncells, rel)
     return {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "avg_ms": round(ms / n, 5),
             "algorithmic_bytes": int(b),
             "traffic_over_algorithmic": round(traffic / b, 3) if traffic else None,
-            "kernel_ms_per_step": {k: round(v[0] / v[1], 5) for k, v in sorted(kt.items())}}
+            "whole_step": {"ms": round(res["ms_per_step"], 5), "achieved": round(b / step_s / 1e9, 1),
+                           "frac": round(b / step_s / 1e9 / HBM_PEAK_GBS, 4)},
+            "kernel_own_bytes": int(kown) if kown else None,
+            "kernel_own_frac": round(kown / avg_s / 1e9 / HBM_PEAK_GBS, 4) if kown else None}
+
+
+def run_yee(mode: str, name: str, calls: int, scale: float = 1.0):
+    """The Yee update alone (emo_evolve_fields: the E and B stencil kernels)
+    on workload `name`'s grid: wall time of `calls` calls between stream
+    synchronisations.  Cell-updates/s = physical cells x calls / time; the
+    achieved bandwidth counts 84 B (FP32) or 168 B (FP64) per cell-update
+    (E: read E, B, J, write E; B: read B, E, write B)."""
+    import apc_524_project_amd as emo
+    from apc_524_project_amd import FP32_CHOSEN, FP64_PLAIN
+
+    w = emo.make_workload(name, precision=FP32_CHOSEN if mode == "fp32" else FP64_PLAIN, scale=scale)
+    cfg = w.cfg
+    cfg.max_particles = 1
+    ctx = emo.Context(cfg)
+    ctx.init_constant()
+    for _ in range(3):
+        ctx.evolve(w.dt)
+    ctx.synchronize()
+    a = time.perf_counter()
+    for _ in range(calls):
+        ctx.evolve(w.dt)
+    ctx.synchronize()
+    el = time.perf_counter() - a
+    ctx.close()
+    rate = w.ncells * calls / el
+    per = 84 if mode == "fp32" else 168
+    return {"grid": name, "cells": w.ncells, "calls": calls, "ms_per_update": round(1e3 * el / calls, 5),
+            "cell_updates_per_s": round(rate, 1), "achieved_GBs": round(rate * per / 1e9, 1),
+            "frac": round(rate * per / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict | None = None):
@@ -225,13 +279,7 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
     w = emo.make_workload("c4", precision=FP32_CHOSEN if fp32 else FP64_PLAIN)
     n = steps or w.steps
 
-    def ready(amp=None):
-        import copy
-
-        cfg = copy.deepcopy(w.cfg)
-        if amp is not None:
-            cfg.peakamps[0] = amp
-        c = emo.Context(cfg)
+    def reset(c):
         c.load_maxwellian(w.loader)
         c.init_constant()
         if not fp32:
@@ -239,8 +287,27 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
         c.synchronize()
         return c
 
-    def forward(amp=None):
-        c = ready(amp)
+    def ready(amp=None, phase=None, eps=None):
+        import copy
+
+        cfg = copy.deepcopy(w.cfg)
+        if amp is not None:
+            cfg.peakamps[0] = amp
+        if phase is not None:
+            cfg.phases[0] = phase
+        c = emo.Context(cfg)
+        if eps is None:
+            return reset(c)
+        ld = copy.deepcopy(w.loader)
+        ld.pert_eps[0] = eps
+        c.load_maxwellian(ld)
+        c.init_constant()
+        c.interpolate()
+        c.synchronize()
+        return c
+
+    def forward(amp=None, phase=None, eps=None):
+        c = ready(amp, phase, eps)
         if not fp32:
             c.sort()
         c.synchronize()
@@ -254,7 +321,12 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
 
     forward()  # warm-up (first launches, allocations)
     f0, t_fwd = forward()
+    # the first gradient call on a context allocates its checkpoint and tape
+    # pool, which later calls reuse: the timed call is the second one, from
+    # the same initial state (the context reloaded)
     c = ready()
+    c.adjoint_field_energy(n, 0.0, w.dt, k, particles=False)
+    reset(c)
     a = time.perf_counter()
     out = c.adjoint_field_energy(n, 0.0, w.dt, k, particles=False)
     c.synchronize()
@@ -280,6 +352,8 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
         ("dF_damp_fp64_central_diff" if fp32 else "dF_damp_central_diff"): float(fd) if fd else None,
         "fd_rel_err": float(abs(out["d_amp"][0] - fd) / abs(fd)) if fd else None,
     }
+    if not fp32 and n == w.steps:
+        res["fd_check"] = c4_fd_check(w, ready, forward)
     if fp32 and fp64_leg:
         res["dF_damp_rel_diff_vs_fp64_adjoint"] = float(abs(out["d_amp"][0] - fp64_leg["dF_damp"]) /
                                                         abs(fp64_leg["dF_damp"]))
@@ -287,6 +361,51 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
           f"{(t_adj - t_fwd) / t_fwd:.2f}, gradient/forward {t_adj / t_fwd:.2f}, fd rel err {res['fd_rel_err']}",
           file=sys.stderr, flush=True)
     return res
+
+
+def c4_fd_check(w, ready, forward):
+    """The c4 gradient (amplitude, phase per radian, eps) against central
+    differences of the FP64 forward at the configuration's step h = 1e-4 and
+    at a step small enough that no particle changes cell (1e-9 relative for
+    the amplitude, 1e-9 rad, 1e-10 for eps).  At h = 1e-4 particles cross
+    cell faces, where the derivative of the linear weights jumps, so the
+    quotient there is not the derivative the adjoint computes; both are
+    reported.  `resolution` is ulp(F)/(2h) relative to the derivative: the
+    finest the small-step quotient can resolve.  Untimed."""
+    import numpy as np
+
+    from apc_524_project_amd._lib import loader_dx_deps
+
+    c = ready()
+    out = c.adjoint_field_energy(w.steps, 0.0, w.dt, 20)
+    p = c.particles(fields=False)
+    c.close()
+    species = np.empty(len(p["id"]), np.int32)
+    species[p["id"]] = p["species"]
+    d_eps = float((out["xbar0"][:, 0] * loader_dx_deps(w.loader, w.cfg))[species == 0].sum())
+    F = out["energy"]
+    amp = float(w.cfg.peakamps[0])
+    ph0 = float(w.cfg.phases[0])
+    eps0 = float(w.loader.pert_eps[0])
+    adj = {"amp": float(out["d_amp"][0]), "phase_per_rad": float(out["d_phase"][0]), "eps": d_eps}
+    rows = {}
+    for name, small in (("amp", 1e-9 * amp), ("phase_per_rad", 1e-9), ("eps", 1e-10)):
+        r = {"adjoint": adj[name]}
+        for tag, h in (("h_small", small), ("h_1e-4", 1e-4)):
+            if name == "amp":
+                fd = (forward(amp=amp + h)[0] - forward(amp=amp - h)[0]) / (2 * h)
+            elif name == "phase_per_rad":
+                hd = h * 180.0 / math.pi  # phases are given in degrees
+                fd = (forward(phase=ph0 + hd)[0] - forward(phase=ph0 - hd)[0]) / (2 * h)
+            else:
+                fd = (forward(eps=eps0 + h)[0] - forward(eps=eps0 - h)[0]) / (2 * h)
+            r[tag] = {"h": h, "central_diff": float(fd), "rel_diff": float(abs(fd - adj[name]) / abs(fd))}
+        r["h_small"]["resolution"] = float(math.ulp(F) / (2 * small) / abs(adj[name]))
+        rows[name] = r
+        print(f"[bench] c4 fd {name}: adjoint {adj[name]:.8e}, small h {r['h_small']['central_diff']:.8e} "
+              f"(rel {r['h_small']['rel_diff']:.2e}), h=1e-4 {r['h_1e-4']['central_diff']:.8e} "
+              f"(rel {r['h_1e-4']['rel_diff']:.2e})", file=sys.stderr, flush=True)
+    return rows
 
 
 def run_adjoint_c5s(steps: int, k: int):
@@ -310,14 +429,16 @@ def run_adjoint_c5s(steps: int, k: int):
     for prec, tag in ((FP64_PLAIN, "fp64"), (FP32_CHOSEN, "fp32")):
         w = emo.make_workload("c5s", precision=prec)
 
-        def ready():
-            c = emo.Context(w.cfg)
+        def reset(c):
             load(c, w)
             init_fields(c, w)
             if prec == FP64_PLAIN:
                 c.interpolate()
             c.synchronize()
             return c
+
+        def ready():
+            return reset(emo.Context(w.cfg))
 
         c = ready()
         if prec == FP64_PLAIN:
@@ -336,14 +457,17 @@ def run_adjoint_c5s(steps: int, k: int):
         f_fwd = d[4] + d[5]
         c.close()
         # timed without the per-particle outputs (by-id copies to the host of
-        # ~1 GB at this size); the gradient itself comes from a second call
+        # ~1 GB at this size), as the second gradient call on a context (the
+        # first allocates the checkpoint and tape pool, which later calls
+        # reuse); the gradient itself comes from a third call
         c = ready()
+        c.adjoint_field_energy(steps, 0.0, w.dt, k, particles=False)
+        reset(c)
         a = time.perf_counter()
         c.adjoint_field_energy(steps, 0.0, w.dt, k, particles=False)
         c.synchronize()
         t_grad = time.perf_counter() - a
-        c.close()
-        c = ready()
+        reset(c)
         g = c.adjoint_field_energy(steps, 0.0, w.dt, k)
         p = c.particles(fields=False)
         c.close()
@@ -509,10 +633,16 @@ def run_c5(steps: int, warmup: int):
     ctx.load_uniform(w.loader)
     ctx.synchronize()
     t_load = time.perf_counter() - t0
+    ctx.timing(True)
+    ctx.timing_select("jacobi_f32")
+    ctx.timing_reset()
     a = time.perf_counter()
     pois = init_fields(ctx, w)
     ctx.synchronize()
     t_pois = time.perf_counter() - a
+    jac_ms, jac_n = ctx.kernel_time("jacobi_f32")
+    ctx.timing(False)
+    ctx.timing_select(None)
     d0 = ctx.diag()
     t = 0.0
     ctx.step(warmup, t, w.dt)
@@ -533,12 +663,51 @@ def run_c5(steps: int, warmup: int):
                         "device loader (seed -5)",
             "particle_steps_per_s": round(rate, 1), "ms_per_step": round(1e3 * el / steps, 4), "steps": steps,
             "warmup": warmup, "load_s": round(t_load, 2), "poisson_s": round(t_pois, 3), "poisson_init": pois,
+            "jacobi_sweep": jacobi_roof(jac_ms, jac_n, w.ncells),
             "diag_initial": dict(zip(keys, map(float, d0))), "diag_final": dict(zip(keys, map(float, d1)))}
 
 
-def cpu_baseline(sample_cells: int = 64, sample_steps: int = 10):
+def host_cpu():
+    """The host CPU's model name, family/model/stepping and clock from
+    /proc/cpuinfo, and the logical CPUs this process may use."""
+    info = {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                    info[k] = v
+            if not line.strip() and info:
+                break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        ncpu = os.cpu_count()
+    return (f"{info.get('model name', '?')} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}, {info.get('cpu MHz', '?')} MHz), {ncpu} logical CPUs available")
+
+
+def jacobi_roof(ms, n, ncells):
+    """One FP32 Jacobi sweep (HIP events around its launches): 12 B per
+    physical point (the iterate and the source read, the new iterate
+    written)."""
+    if not n:
+        return None
+    avg = ms / n / 1e3
+    gbs = 12 * ncells / avg / 1e9
+    return {"sweeps": n, "avg_ms": round(ms / n, 5), "achieved_GBs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def cpu_baseline(sample_cells: int = 64, sample_steps: int = 4, runs: int = 3):
     """The reference's serial loop (oracle/_ref) on one core, c2 physics on a
-    sample_cells^3 sub-box at the same density (32 ppc/species)."""
+    sample_cells^3 box at the same density (32 ppc/species).  The loop's
+    first step (which sorts the freshly loaded particles: the reference sorts
+    when ti % 100 == 0) is timed apart; then `runs` runs of `sample_steps`
+    steps (no sort among them), whose median gives the rate.  The rate with
+    the sort amortised over its 100-step period is given beside it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes as oc
 
@@ -547,41 +716,45 @@ def cpu_baseline(sample_cells: int = 64, sample_steps: int = 10):
     case = oc.Case(n=(n, n, n), L=(float(n),) * 3, nparticles=2 * 32 * n ** 3, relativity=1,
                    temp=((0.1 / oc.UNIT_VTH) ** 2, (0.1 / oc.UNIT_VTH) ** 2), dens_phys=0.01 / wpe2,
                    zero_avg=0)
+    cpu = host_cpu()
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except (AttributeError, OSError):
         pass
     dt = 0.5 / math.sqrt(3.0)
+    runs_s = []
     if oc.ref_available():
         r = oc.Reference(case, nstep_sort=100)
         r.initialize()
-        el = r.time_steps(sample_steps, dt)
+        first = r.time_steps_from(0, 1, dt)
+        for k in range(runs):
+            runs_s.append(r.time_steps_from(1 + k * sample_steps, sample_steps, dt))
         kind = "reference"
     else:
         o = oc.Oracle(case)
         o.load()
         o.initialize()
-        a = time.perf_counter()
         t = 0.0
-        for _ in range(sample_steps):
-            o.step(t, dt)
-            t += dt
-        el = time.perf_counter() - a
+        a = time.perf_counter()
+        o.step(t, dt)
+        first = time.perf_counter() - a
+        t += dt
+        for _ in range(runs):
+            a = time.perf_counter()
+            for _ in range(sample_steps):
+                o.step(t, dt)
+                t += dt
+            runs_s.append(time.perf_counter() - a)
         kind = "port"
-    cpu = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {"value": case.nparticles * sample_steps / el, "unit": "particle-steps/s", "cores": 1,
-            "kind": kind,
+    med = sorted(runs_s)[len(runs_s) // 2] / sample_steps
+    sort_s = max(first - med, 0.0)
+    return {"value": case.nparticles / med, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+            "value_with_sort_amortised": case.nparticles / (med + sort_s / 100.0),
             "sample": f"c2 physics on a {n}^3 box{' (the full c2 box)' if n == 64 else ''} "
-                      f"(32 ppc/species, {case.nparticles} particles), "
-                      f"{sample_steps} steps of the reference serial loop (nstep_sort=100), "
-                      f"1 thread pinned; host CPU: {cpu}; {el:.2f}s"}
+                      f"(32 ppc/species, {case.nparticles} particles), reference serial loop (nstep_sort=100): "
+                      f"first step (with the sort of the loaded particles) {first:.2f}s, then {runs} runs of "
+                      f"{sample_steps} steps: {', '.join(f'{x:.2f}s' for x in runs_s)} (median used); "
+                      f"1 thread pinned to one core; host CPU: {cpu}"}
 
 
 def main():
@@ -605,7 +778,7 @@ def main():
                     help="steps of the c5s FP32-vs-FP64 adjoint leg (0: skip it)")
     ap.add_argument("--c5s-adjoint-only", action="store_true", help="only the c5s adjoint leg")
     ap.add_argument("--cpu-cells", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=4, help="steps per timed run of the CPU baseline (3 runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the multi-GPU decomposition is a design, "
@@ -672,11 +845,16 @@ def main():
         },
         "roofline": roofline_for(r),
     }
+    if head == "fp32":
+        out["config"]["rho"] = ("the FP32 step deposits J only; rho, which no phase of the step reads, is "
+                                "deposited from the current particles when it is read (emo_get_field)")
     for m, rr in results.items():
         out[f"{m}_particle_steps_per_s"] = round(rr["rate"], 1)
         out[f"{m}_kernel_ms_per_step"] = {k: round(v[0] / v[1], 5) for k, v in sorted(rr["ktimes"].items())}
         out[f"{m}_ms_per_step_events_every_launch"] = round(rr["ms_per_step_all_events"], 4)
-        out[f"{m}_yee_cell_updates_per_s"] = round(rr["workload"].ncells * args.steps / rr["elapsed"], 1)
+        out[f"{m}_yee_c2"] = run_yee(m, "c2", 200, args.scale)
+    if "fp64" in results:
+        out["fp64_roofline"] = roofline_for(results["fp64"])
     if "fp32" in results and "fp64" in results:
         out["fp32_over_fp64"] = round(results["fp32"]["rate"] / results["fp64"]["rate"], 3)
         rate_pp, ms_pp = run_fp64_per_phase(max(args.steps // 5, 5), 2, args.scale)
@@ -689,6 +867,8 @@ def main():
                                 for nm in names}
     if args.c5_steps > 0 and "fp32" in results:
         out["c5"] = run_c5(args.c5_steps, 2)
+        out["c5"]["yee_fp32"] = run_yee("fp32", "c5", 50)
+        out["c5"]["yee_fp64"] = run_yee("fp64", "c5", 50)
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint_c4(args.adjoint_steps or None, args.checkpoint_every)
         if "fp32" in results:

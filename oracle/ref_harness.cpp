@@ -453,4 +453,18 @@ double rh_time_steps(void *hp, int nsteps, double dt) {
   return (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
 }
 
+/* steps ti0 .. ti0 + nsteps - 1 of the loop (the reference sorts when
+ * ti % nstep_sort == 0), wall time */
+double rh_time_steps_from(void *hp, int ti0, int nsteps, double dt) {
+  struct timespec a, b;
+  clock_gettime(CLOCK_MONOTONIC, &a);
+  double t = ti0 * dt;
+  for (int ti = ti0; ti < ti0 + nsteps; ti++) {
+    rh_step(hp, ti, t, dt);
+    t += dt;
+  }
+  clock_gettime(CLOCK_MONOTONIC, &b);
+  return (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
+}
+
 }  // extern "C"

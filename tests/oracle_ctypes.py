@@ -274,6 +274,8 @@ def ref_lib():
         lib.rh_gaussian_pulses.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_double]
         lib.rh_time_steps.restype = C.c_double
         lib.rh_time_steps.argtypes = [vp, C.c_int, C.c_double]
+        lib.rh_time_steps_from.restype = C.c_double
+        lib.rh_time_steps_from.argtypes = [vp, C.c_int, C.c_int, C.c_double]
         _ref = lib
     return _ref
 
@@ -528,6 +530,10 @@ class Reference:
 
     def time_steps(self, nsteps, dt):
         return self.lib.rh_time_steps(self.h, nsteps, dt)
+
+    def time_steps_from(self, ti0, nsteps, dt):
+        """Wall time of the loop's steps ti0 .. ti0 + nsteps - 1."""
+        return self.lib.rh_time_steps_from(self.h, ti0, nsteps, dt)
 
 
 def ref_rng_uniform(seed, n):
