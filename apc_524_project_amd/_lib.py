@@ -31,7 +31,7 @@ EXPORTS = (
     "emo_deposit", "emo_fold", "emo_evolve_fields", "emo_refresh", "emo_interpolate",
     "emo_step", "emo_diag", "emo_synchronize", "emo_timing_enable", "emo_timing_select", "emo_kernel_time",
     "emo_timing_reset", "emo_adjoint_field_energy", "emo_loader_dx_deps", "emo_init_poisson_noalloc",
-    "emo_layout_info",
+    "emo_layout_info", "emo_get_tracked",
 )
 
 
@@ -136,6 +136,7 @@ def lib():
     L.emo_num_particles.restype = i64
     L.emo_get_particles.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.emo_get_cell_ids.argtypes = [vp, vp]
+    L.emo_get_tracked.argtypes = [vp, i64, vp, vp, vp]
     L.emo_field_points.argtypes = [vp]
     L.emo_field_points.restype = i64
     L.emo_get_field.argtypes = [vp, C.c_int, vp]
@@ -236,6 +237,15 @@ class Context:
         out = np.zeros(3, np.int64)
         self._chk(self.L.emo_layout_info(self.h, _p(out)))
         return {"slots": int(out[0]), "used": int(out[1]), "rebuilds": int(out[2])}
+
+    def tracked(self, nid):
+        """x, v of the particles with id < nid, by id, and whether each id exists
+        (the rows of the reference's particle track files)."""
+        x = np.zeros((nid, 3))
+        v = np.zeros((nid, 3))
+        found = np.zeros(nid, np.int32)
+        self._chk(self.L.emo_get_tracked(self.h, nid, _p(x), _p(v), _p(found)))
+        return x, v, found.astype(bool)
 
     def cell_ids(self):
         out = np.zeros(self.num_particles, np.int32)

@@ -86,9 +86,9 @@ def build_host() -> str:
     reference's main loop through it (host/run_main.cpp), linked against
     libemopic.so; host code only, so the host compiler builds it."""
     tmp = RUN_MAIN + ".tmp"
-    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter",
+    cmd = [CXX, "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra", "-Wno-unused-parameter",
            "-I", os.path.join(ROOT, "include"),
-           os.path.join(HOST, "emopic.cpp"), os.path.join(HOST, "run_main.cpp"),
+           os.path.join(HOST, "emopic.cpp"), os.path.join(HOST, "readinfo.cpp"), os.path.join(HOST, "run_main.cpp"),
            "-L", PKG, "-lemopic", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:

@@ -840,6 +840,8 @@ struct OutArgs {
   int32_t *sp;
   int32_t *id;
   int32_t *cell;
+  double *trk;          // the particles with id < ntrk by id: 7 per id (x, v, found), or null
+  long long ntrk;
   long long np;
 };
 
@@ -861,9 +863,20 @@ __global__ void __launch_bounds__(kBlock) k_output_f32(OutArgs a) {
     const double ff[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
     double gam = 1.0;
     if (a.rel) gam = a.g0 ? (double)a.g0[s] : sqrt(1.0 + uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2]);
+    double xo[3], vo[3];
     for (int d = 0; d < 3; d++) {
-      if (a.x[d]) a.x[d][i] = g.x0[d] + g.d[d] * ((double)(ic[d] - 1) + ff[d]);
-      if (a.v[d]) a.v[d][i] = a.rel ? uu[d] / gam : uu[d];
+      xo[d] = g.x0[d] + g.d[d] * ((double)(ic[d] - 1) + ff[d]);
+      vo[d] = a.rel ? uu[d] / gam : uu[d];
+      if (a.x[d]) a.x[d][i] = xo[d];
+      if (a.v[d]) a.v[d][i] = vo[d];
+    }
+    if (a.trk && Bv.w >= 0 && Bv.w < a.ntrk) {
+      double *o = a.trk + 7 * (long long)Bv.w;
+      for (int d = 0; d < 3; d++) {
+        o[d] = xo[d];
+        o[3 + d] = vo[d];
+      }
+      o[6] = 1.0;
     }
     if (!a.rel && a.gam_id && Bv.w >= 0 && Bv.w < a.np) gam = a.gam_id[Bv.w];
     if (a.gam) a.gam[i] = gam;
@@ -1442,6 +1455,16 @@ int launch_output(emo_ctx *c, OutArgs &a) {
   hipLaunchKernelGGL(k_output_f32, dim3((unsigned)((c->geo.ncell + kTileCells - 1) / kTileCells)), dim3(kBlock), 0,
                      c->stream, a);
   return EMO_OK;
+}
+
+// the particles with id < nid by id into out (7 per id: x, v, found)
+int fp32_tracked(emo_ctx *c, long long nid, double *out) {
+  launch_const_fill_f64(c, out, 0.0, 7 * nid);
+  if (c->np == 0 || !c->ch->have_id) return EMO_OK;
+  OutArgs a{};
+  a.trk = out;
+  a.ntrk = nid;
+  return launch_output(c, a);
 }
 
 int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species, int64_t *id,

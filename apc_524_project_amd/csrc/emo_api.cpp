@@ -437,6 +437,34 @@ int emo_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t
   return EMO_OK;
 }
 
+int emo_get_tracked(emo_ctx *c, int64_t nid, double *x3, double *v3, int32_t *found) {
+  if (!c || nid < 0) return EMO_ERR_CONFIG;
+  if (nid == 0) return EMO_OK;
+  if (!x3 || !v3 || !found) return set_err(c, EMO_ERR_CONFIG, "emo_get_tracked needs x3, v3 and found");
+  double *d = nullptr;
+  int rc = dalloc(c, &d, (size_t)(7 * nid));
+  if (rc) return rc;
+  if (c->cfg.precision != EMO_FP64_PLAIN) rc = fp32_tracked(c, nid, d);
+  else launch_tracked_f64(c, nid, d);
+  std::vector<double> h(rc ? 0 : (size_t)(7 * nid));
+  hipError_t e = hipSuccess;
+  if (!rc) {
+    e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  dfree(d);
+  if (rc) return rc;
+  if (e != hipSuccess) return set_err(c, EMO_ERR_HIP, hipGetErrorString(e));
+  for (int64_t k = 0; k < nid; k++) {
+    for (int q = 0; q < 3; q++) {
+      x3[3 * k + q] = h[7 * k + q];
+      v3[3 * k + q] = h[7 * k + 3 + q];
+    }
+    found[k] = h[7 * k + 6] != 0.0;
+  }
+  return EMO_OK;
+}
+
 int emo_get_cell_ids(emo_ctx *c, int32_t *cells) {
   if (!c || !cells) return EMO_ERR_CONFIG;
   if (c->cfg.precision != EMO_FP64_PLAIN) {

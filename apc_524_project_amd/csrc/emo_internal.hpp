@@ -187,6 +187,7 @@ void fld_materialize(emo_ctx *c);
 void fld_invalidate(emo_ctx *c);
 void launch_deposit_f64(emo_ctx *c, int rho);
 void launch_cell_ids(emo_ctx *c, int32_t *out);
+void launch_tracked_f64(emo_ctx *c, long long nid, double *out);
 size_t sort_tmp_bytes_for(int64_t n, int bits);
 
 // ---- field kernels (field_kernels.hip) ----
@@ -266,6 +267,7 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
 int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t *species, int64_t *id,
                        double *field6);
 int fp32_get_cells(emo_ctx *c, int32_t *cells);
+int fp32_tracked(emo_ctx *c, long long nid, double *out);
 int fp32_get_field(emo_ctx *c, int fid, double *out);
 int fp32_set_field(emo_ctx *c, int fid, const double *in);
 int fp32_init_constant(emo_ctx *c);

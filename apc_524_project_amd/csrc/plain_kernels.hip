@@ -604,6 +604,29 @@ void launch_uniform_f64(emo_ctx *c, const UniformGen &u, int64_t n) {
   hipLaunchKernelGGL(k_uniform_f64, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);
 }
 
+// The particles with id < nid, by id (Particle_Handler::outputParticles,
+// src/particles/particle_handler.cpp:308-365, writes these rows): out[7 id +
+// 0..2] = x, [3..5] = v, [6] = 1 (found); the caller zeroes out.
+__global__ void __launch_bounds__(kBlock) k_tracked_f64(ParticlesF64 p, long long np, long long nid, double *out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const long long k = p.id[i];
+  if (k < 0 || k >= nid) return;
+  double *o = out + 7 * k;
+  for (int d = 0; d < 3; d++) {
+    o[d] = p.x[d][i];
+    o[3 + d] = p.v[d][i];
+  }
+  o[6] = 1.0;
+}
+
+void launch_tracked_f64(emo_ctx *c, long long nid, double *out) {
+  launch_const_fill_f64(c, out, 0.0, 7 * nid);
+  if (c->np == 0) return;
+  hipLaunchKernelGGL(k_tracked_f64, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, c->p, (long long)c->np, nid,
+                     out);
+}
+
 void launch_cell_ids(emo_ctx *c, int32_t *out) {
   if (c->np == 0) return;
   hipLaunchKernelGGL(k_cell_ids, dim3(grid_for(c->np)), dim3(kBlock), 0, c->stream, c->geo,

@@ -1,10 +1,14 @@
 // emopic.cpp -- the reference's class surface over libemopic (see emopic.hpp).
 #include "emopic.hpp"
 
+#include <math.h>  // sqrt for the shared per-particle arithmetic
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+
+#include "../csrc/pic_math.hpp"
 
 namespace emopic {
 
@@ -200,6 +204,45 @@ int Relativistic_Boris::Step(Particle *part, Field_part *field, double dt) {
   part->x[2] += dt * part->v[2];
   part->gamma = g;
   return 1;
+}
+
+// ---- single host particle: gather and deposit ---------------------------
+namespace {
+// the cell lengths are the only geometry the per-particle formulas read
+emo::Geo cell_geo(const double *lcell) {
+  emo::Geo g{};
+  for (int d = 0; d < 3; d++) {
+    g.d[d] = lcell[d];
+    g.id[d] = 1.0 / lcell[d];
+  }
+  return g;
+}
+}  // namespace
+
+void Interpolator::interpolate_fields(double *pos, double *lcell, double *cellvars, Field_part *field) {
+  const emo::Geo g = cell_geo(lcell);
+  double f[6];
+  emo::interp_fields(g, cellvars, pos, f);
+  field->e1 = f[0];
+  field->e2 = f[1];
+  field->e3 = f[2];
+  field->b1 = f[3];
+  field->b2 = f[4];
+  field->b3 = f[5];
+}
+
+void Depositor::deposit_particle_J(Particle *part, double *lcell, double *cellverts, double *JObj) {
+  const emo::Geo g = cell_geo(lcell);
+  double r[20];
+  emo::deposit_row(g, cellverts, part->x, part->v, part->q, false, r);
+  for (int q = 0; q < 12; q++) JObj[q] += r[q];
+}
+
+void Depositor::deposit_particle_Rho(Particle *part, double *lcell, double *cellverts, double *RhoObj) {
+  const emo::Geo g = cell_geo(lcell);
+  double r[20];
+  emo::deposit_row(g, cellverts, part->x, part->v, part->q, true, r);
+  for (int q = 0; q < 8; q++) RhoObj[q] += r[12 + q];
 }
 
 // ---- factories ------------------------------------------------------------
@@ -493,6 +536,52 @@ void Particle_Handler::getParticles(std::vector<Particle> &out) {
       p.field.b2 = f[6 * i + 4];
       p.field.b3 = f[6 * i + 5];
     }
+  }
+}
+
+void Particle_Handler::outputParticles(const char *basename, long step, Input_Info_t *input_info) {
+  const double t = time_phys;
+  dstep_ = input_info->nstep_parts;
+  outputCount_ = input_info->output_pCount;
+  bool needs = false;
+  if (dT_ > 0 && t >= nextT_) {
+    needs = true;
+    nextT_ += dT_;
+  }
+  if (dstep_ > 0 && step >= nextStep_) {
+    needs = true;
+    nextStep_ += dstep_;
+  }
+  if (!tracks_init_ && !needs) return;
+  if (!dev_) throw std::runtime_error("particle tracks before the particles met a grid");
+  // loaded ids are 0..N-1, so no tracked id is at or above the count
+  const int64_t nid = std::max<int64_t>(0, std::min<int64_t>(outputCount_, emo_num_particles(dev_->ctx())));
+  std::vector<double> x(3 * nid), v(3 * nid);
+  std::vector<int32_t> found(nid);
+  if (nid > 0) dev_->check(emo_get_tracked(dev_->ctx(), nid, x.data(), v.data(), found.data()));
+  auto name = [&](int64_t id) {
+    char fname[4096];
+    snprintf(fname, sizeof(fname), "%strack_%d_%ld.dat", basename, 0, (long)id);
+    return std::string(fname);
+  };
+  if (tracks_init_) {
+    tracks_init_ = false;
+    for (int64_t k = 0; k < nid; k++) {
+      if (!found[k]) continue;
+      FILE *fp = fopen(name(k).c_str(), "w");
+      if (!fp) throw std::runtime_error("cannot open " + name(k));
+      fprintf(fp, "[1] time [2] x [3] y [4] z  [5] vx   [6] vy   [7] vz\n");
+      fclose(fp);
+    }
+  }
+  if (!needs) return;
+  for (int64_t k = 0; k < nid; k++) {
+    if (!found[k]) continue;
+    FILE *fp = fopen(name(k).c_str(), "a");
+    if (!fp) throw std::runtime_error("cannot open " + name(k));
+    fprintf(fp, "%e %.15e %.15e %.15e %.15e %.15e %.15e\n", t, x[3 * k], x[3 * k + 1], x[3 * k + 2], v[3 * k],
+            v[3 * k + 1], v[3 * k + 2]);
+    fclose(fp);
   }
 }
 

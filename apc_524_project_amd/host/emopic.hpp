@@ -14,8 +14,10 @@
 //     getParticles(std::vector<Particle>&), a copy in the current array order.
 //   - SortParticles is a stable counting sort by cell every step; the
 //     comparator argument is accepted for source compatibility.
-//   - Pusher::Step on one host particle is kept for single-particle tests
-//     (the FP64 formulas of src/pusher/boris.cpp, relativisticBoris.cpp).
+//   - Pusher::Step, Interpolator and Depositor on one host particle are kept
+//     for single-particle use and tests (the FP64 formulas of
+//     src/pusher/boris.cpp, relativisticBoris.cpp,
+//     src/particles/interpolate.cpp, deposit.cpp).
 //   - Errors: the reference asserts or throws; the facade throws
 //     emopic::Error (derived from std::runtime_error) with libemopic's message.
 #pragma once
@@ -81,6 +83,7 @@ struct Input_Info_t {
   double waist = 0.0;              // transverse Gaussian waist of injected pulses (0: uniform)
   int poisson_iter_max = 10000;    // ITER_MAX (src/poisson/poisson.cpp:14)
   double poisson_tol = -1.0;       // < 0: the reference's tolerance rule
+  double dt = -1.0;                // < 0: min(dx)/100, as src/main.cpp:213-214 sets it
 };
 
 class Error : public std::runtime_error {
@@ -97,6 +100,9 @@ class Error : public std::runtime_error {
 class Input {
  public:
   Input() = default;
+  // readinfo (src/IO/readinfo.cpp:18-733): the reference's libconfig input
+  // file; 0 on success, EXIT_FAILURE after a message on stderr
+  int readinfo(const char *fname);
   int ProcessInfo();
   Input_Info_t *getinfo() { return &info_; }
 
@@ -136,6 +142,30 @@ struct Particle {
   short isGhost = 0;
   short isTestParticle = 0;
   Field_part field;
+};
+
+// Interpolator (src/particles/interpolate.hpp:6-11): the E and B of one host
+// particle from its cell's 21 values (the least vertex, 4 E edges per
+// component, 2 (B + B_tm1)/2 faces per component, as
+// Grid::getFieldInterpolatorVec orders them); FP64, the operation order of
+// src/particles/interpolate.cpp:11-47, which the device gather also follows.
+class Interpolator {
+ public:
+  Interpolator() = default;
+  ~Interpolator() = default;
+  void interpolate_fields(double *pos, double *lcell, double *cellvars, Field_part *field);
+};
+
+// Depositor (src/particles/deposit.hpp:6-14): one host particle's current
+// on its cell's 12 edges and charge on its 8 vertices, added into JObj and
+// RhoObj; FP64, the operation order of src/particles/deposit.cpp:12-62,
+// which the device deposit also follows.
+class Depositor {
+ public:
+  Depositor() = default;
+  ~Depositor() = default;
+  void deposit_particle_J(Particle *part, double *lcell, double *cellverts, double *JObj);
+  void deposit_particle_Rho(Particle *part, double *lcell, double *cellverts, double *RhoObj);
 };
 
 // Pushers (src/pusher/pusher.hpp:6-10): select the device kernel; Step on a
@@ -280,6 +310,12 @@ class Particle_Handler {
   void setParticleBoundaries(BC_Particle **bc) { boundaries_ = bc; }
   void executeParticleBoundaryConditions();
   void getParticles(std::vector<Particle> &out);
+  // outputParticles (src/particles/particle_handler.cpp:308-365): at the
+  // first call, a file <basename>track_0_<id>.dat with the reference's header
+  // for each particle with my_id < output_pCount; then, every nstep_parts
+  // steps, one row "%e %.15e x6" of time_phys, x, v appended to each.  The
+  // tracked particles are selected on the device (emo_get_tracked).
+  void outputParticles(const char *basename, long step, Input_Info_t *input_info);
   // attaches to the grid's device context and loads the particles (first use)
   void bind(Grid *grid);
 
@@ -290,6 +326,11 @@ class Particle_Handler {
   Input_Info_t *info_ = nullptr;
   Domain *domain_ = nullptr;
   bool loaded_ = false;
+  // track-file cadence (src/particles/particle_handler.cpp:24-27, 311-341)
+  bool tracks_init_ = true;
+  double dT_ = -1.0, nextT_ = 0.0;
+  long dstep_ = -1, nextStep_ = 0;
+  long outputCount_ = 0;
 };
 
 // OutputBoxQuantities (src/IO/output.hpp): history.dat every dStep_ = 10

@@ -1,20 +1,24 @@
 // run_main.cpp -- the reference's serial driver loop (src/main.cpp:114-287)
 // written against the emopic facade, as a caller of the drop-in would.
 //
-//   emo_run_main <settings.txt> <outdir> [--fused]
+//   emo_run_main <input.txt> [outdir] [--fused | --print-input]
 //
-// settings.txt holds one Input_Info_t field per line, `name = v1 v2 ...`
-// (# starts a comment; strings unquoted), using the field names of
-// src/IO/input.hpp:15-126 plus `dt` (default min(dx)/100, src/main.cpp:213-214)
-// and the facade additions precision / seed / waist / poisson_iter_max /
-// poisson_tol.  The libconfig front end of the reference (src/IO/readinfo.cpp)
-// is not part of this library.
+// input.txt is the reference's libconfig input file (Input::readinfo,
+// src/IO/readinfo.cpp), read unchanged; an optional group `emopic` holds the
+// facade's additions (precision, seed, waist, dt, poisson_iter_max,
+// poisson_tol).  dt defaults to min(dx)/100 (src/main.cpp:213-214).
 //
-// Writes <outdir>/history.dat (OutputBoxQuantities) and <outdir>/particles.bin
+// Outputs go to outdir, or, as in the reference (src/main.cpp:182-185), to
+// the input file's directory: history.dat (OutputBoxQuantities), the particle
+// tracks track_0_<id>.dat (Particle_Handler::outputParticles, written at step
+// 0 and after every step at the nstep_parts cadence), and particles.bin
 // (int64 n; then x[3n], v[3n], gamma[n] as doubles and id[n] as int64, in
-// ascending id order) after the last step.  --fused runs the loop body as one
-// device call per step (required for the FP32 chosen form); the default calls
-// the phases one by one in the reference's order.
+// ascending id order) after the last step.  The HDF5 field output
+// (src/IO/hdf5io.cpp) is not written: no HDF5 library is available.
+//
+// --fused runs the loop body as one device call per step (required for the
+// FP32 chosen form); the default calls the phases one by one in the
+// reference's order.  --print-input prints what readinfo read and exits.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -31,91 +35,6 @@
 using namespace emopic;
 
 namespace {
-
-template <class T>
-void put(std::istringstream &is, T *dst, int cap, const std::string &key) {
-  int k = 0;
-  T v;
-  while (is >> v) {
-    if (k >= cap) throw std::runtime_error("too many values for " + key);
-    dst[k++] = v;
-  }
-  if (k == 0) throw std::runtime_error("no value for " + key);
-  if (cap == 3 && k == 1) dst[1] = dst[2] = dst[0];  // readinfo's single-value rule
-}
-
-void put_str(std::istringstream &is, char (*dst)[NCHAR], int cap, const std::string &key) {
-  int k = 0;
-  std::string v;
-  while (is >> v) {
-    if (k >= cap || v.size() >= NCHAR) throw std::runtime_error("bad value for " + key);
-    strcpy(dst[k++], v.c_str());
-  }
-  if (k != cap) throw std::runtime_error(key + " needs " + std::to_string(cap) + " values");
-}
-
-void read_settings(const char *fname, Input_Info_t *in, double *dt) {
-  std::ifstream f(fname);
-  if (!f) throw std::runtime_error(std::string("cannot open ") + fname);
-  std::string line;
-  while (std::getline(f, line)) {
-    auto h = line.find('#');
-    if (h != std::string::npos) line.resize(h);
-    auto eq = line.find('=');
-    if (eq == std::string::npos) {
-      if (line.find_first_not_of(" \t\r") != std::string::npos)
-        throw std::runtime_error("expected name = value: " + line);
-      continue;
-    }
-    std::string key = line.substr(0, eq);
-    key.erase(std::remove_if(key.begin(), key.end(), ::isspace), key.end());
-    std::istringstream is(line.substr(eq + 1));
-    if (key == "nCell") put(is, in->nCell, 3, key);
-    else if (key == "xyz0") put(is, in->xyz0, 3, key);
-    else if (key == "Lxyz") put(is, in->Lxyz, 3, key);
-    else if (key == "nt" || key == "nTimesteps") put(is, &in->nt, 1, key);
-    else if (key == "t0" || key == "startTime") put(is, &in->t0, 1, key);
-    else if (key == "debug") put(is, &in->debug, 1, key);
-    else if (key == "relativity") put(is, &in->relativity, 1, key);
-    else if (key == "nspecies") put(is, &in->nspecies, 1, key);
-    else if (key == "nstep_sort") put(is, &in->nstep_sort, 1, key);
-    else if (key == "nparticles_tot" || key == "nParticles_tot") put(is, &in->nparticles_tot, 1, key);
-    else if (key == "dens_phys") put(is, &in->dens_phys, 1, key);
-    else if (key == "mass_ratio") put(is, in->mass_ratio, NSPEC, key);
-    else if (key == "charge_ratio") put(is, in->charge_ratio, NSPEC, key);
-    else if (key == "dens_frac") put(is, in->dens_frac, NSPEC, key);
-    else if (key == "temp" || key == "temperature") put(is, in->temp, NSPEC, key);
-    else if (key == "isTestParticle") put(is, in->isTestParticle, NSPEC, key);
-    else if (key == "E0") put(is, in->E0, 3, key);
-    else if (key == "B0") put(is, in->B0, 3, key);
-    else if (key == "fields_init" || key == "init") {
-      std::string v;
-      is >> v;
-      if (v.empty() || v.size() >= NCHAR) throw std::runtime_error("bad fields_init");
-      strcpy(in->fields_init, v.c_str());
-    } else if (key == "parts_bound") put_str(is, in->parts_bound, 6, key);
-    else if (key == "fields_bound") put_str(is, in->fields_bound, 6, key);
-    else if (key == "nwaves") put(is, &in->nwaves, 1, key);
-    else if (key == "inSide") put(is, in->inSide, NWAVE, key);
-    else if (key == "inPolE") put(is, in->inPolE, NWAVE, key);
-    else if (key == "peakamps") put(is, in->peakamps, NWAVE, key);
-    else if (key == "omegas") put(is, in->omegas, NWAVE, key);
-    else if (key == "phases") put(is, in->phases, NWAVE, key);
-    else if (key == "invWidths") put(is, in->invWidths, NWAVE, key);
-    else if (key == "delays") put(is, in->delays, NWAVE, key);
-    else if (key == "bound_phi") put(is, in->bound_phi, 6, key);
-    else if (key == "bound_Ax") put(is, in->bound_Ax, 6, key);
-    else if (key == "bound_Ay") put(is, in->bound_Ay, 6, key);
-    else if (key == "bound_Az") put(is, in->bound_Az, 6, key);
-    else if (key == "precision") put(is, &in->precision, 1, key);
-    else if (key == "seed") put(is, &in->seed, 1, key);
-    else if (key == "waist") put(is, &in->waist, 1, key);
-    else if (key == "poisson_iter_max") put(is, &in->poisson_iter_max, 1, key);
-    else if (key == "poisson_tol") put(is, &in->poisson_tol, 1, key);
-    else if (key == "dt") put(is, dt, 1, key);
-    else throw std::runtime_error("unknown setting " + key);
-  }
-}
 
 void write_particles(const std::string &path, Particle_Handler *ph) {
   std::vector<Particle> parts;
@@ -142,22 +61,108 @@ void write_particles(const std::string &path, Particle_Handler *ph) {
   if (!ok) throw std::runtime_error("short write to " + path);
 }
 
+// --print-input: the Input_Info_t that readinfo produced, one `name = values`
+// line per field, before ProcessInfo scales it; no device call
+void print_input(const Input_Info_t &in) {
+  auto ints = [](const char *name, const int *v, int n) {
+    printf("%s =", name);
+    for (int i = 0; i < n; i++) printf(" %d", v[i]);
+    printf("\n");
+  };
+  auto dbls = [](const char *name, const double *v, int n) {
+    printf("%s =", name);
+    for (int i = 0; i < n; i++) printf(" %.17g", v[i]);
+    printf("\n");
+  };
+  auto strs = [](const char *name, const char (*v)[NCHAR], int n) {
+    printf("%s =", name);
+    for (int i = 0; i < n; i++) printf(" %s", v[i]);
+    printf("\n");
+  };
+  const int ns = std::max(0, std::min(in.nspecies, NSPEC)), nw = std::max(0, std::min(in.nwaves, NWAVE));
+  ints("nCell", in.nCell, 3);
+  ints("nProc", in.nProc, 3);
+  dbls("xyz0", in.xyz0, 3);
+  dbls("Lxyz", in.Lxyz, 3);
+  ints("nt", &in.nt, 1);
+  dbls("t0", &in.t0, 1);
+  ints("debug", &in.debug, 1);
+  ints("nstep_sort", &in.nstep_sort, 1);
+  ints("restart", &in.restart, 1);
+  dbls("dens_phys", &in.dens_phys, 1);
+  ints("relativity", &in.relativity, 1);
+  ints("nspecies", &in.nspecies, 1);
+  dbls("mass_ratio", in.mass_ratio, ns);
+  dbls("charge_ratio", in.charge_ratio, ns);
+  dbls("temp", in.temp, ns);
+  dbls("dens_frac", in.dens_frac, ns);
+  ints("isTestParticle", in.isTestParticle, ns);
+  printf("nparticles_tot = %ld\n", in.nparticles_tot);
+  strs("parts_bound", in.parts_bound, 6);
+  ints("electrostatic", &in.electrostatic, 1);
+  printf("fields_init = %s\n", in.fields_init);
+  dbls("B0", in.B0, 3);
+  dbls("E0", in.E0, 3);
+  strs("fields_bound", in.fields_bound, 6);
+  ints("nwaves", &in.nwaves, 1);
+  ints("inSide", in.inSide, nw);
+  ints("inPolE", in.inPolE, nw);
+  dbls("peakamps", in.peakamps, nw);
+  dbls("omegas", in.omegas, nw);
+  dbls("phases", in.phases, nw);
+  dbls("invWidths", in.invWidths, nw);
+  dbls("delays", in.delays, nw);
+  dbls("bound_phi", in.bound_phi, 6);
+  dbls("bound_Ax", in.bound_Ax, 6);
+  dbls("bound_Ay", in.bound_Ay, 6);
+  dbls("bound_Az", in.bound_Az, 6);
+  ints("nstep_fields", &in.nstep_fields, 1);
+  ints("nstep_parts", &in.nstep_parts, 1);
+  ints("nstep_restart", &in.nstep_restart, 1);
+  ints("output_pCount", &in.output_pCount, 1);
+  ints("which_fields", &in.which_fields, 1);
+  ints("precision", &in.precision, 1);
+  printf("seed = %ld\n", in.seed);
+  dbls("waist", &in.waist, 1);
+  dbls("dt", &in.dt, 1);
+  ints("poisson_iter_max", &in.poisson_iter_max, 1);
+  dbls("poisson_tol", &in.poisson_tol, 1);
+}
+
 int run(int argc, char **argv) {
-  if (argc < 3) {
-    fprintf(stderr, "usage: %s <settings.txt> <outdir> [--fused]\n", argv[0]);
+  std::vector<std::string> pos;
+  bool fused = false, print_only = false;
+  for (int k = 1; k < argc; k++) {
+    if (std::string(argv[k]) == "--fused") fused = true;
+    else if (std::string(argv[k]) == "--print-input") print_only = true;
+    else pos.push_back(argv[k]);
+  }
+  if (pos.empty() || pos.size() > 2) {
+    fprintf(stderr, "usage: %s <input.txt> [outdir] [--fused | --print-input]\n", argv[0]);
     return 2;
   }
-  bool fused = argc > 3 && std::string(argv[3]) == "--fused";
-  std::string dir = std::string(argv[2]) + "/";
+  const std::string inputname = pos[0];
+  std::string dir = pos.size() > 1 ? pos[1] + "/" : inputname.substr(0, inputname.find_last_of('/') + 1);
 
   Input *input = new Input();
   Input_Info_t *input_info = input->getinfo();
-  double dt = -1.0;
-  read_settings(argv[1], input_info, &dt);
+  if (input->readinfo(inputname.c_str()) != 0) {
+    fprintf(stderr, "input file %s could not be read\n", inputname.c_str());
+    return 1;
+  }
+  if (print_only) {
+    print_input(*input_info);
+    return 0;
+  }
+  if (input_info->electrostatic != 0) {
+    fprintf(stderr, "electrostatic field evolution (Grid::evolveFieldsES) is not part of this library\n");
+    return 1;
+  }
   if (input->ProcessInfo() != 0) {
     fprintf(stderr, "input check failed\n");
     return 1;
   }
+  const double dt = input_info->dt;
   if (input_info->precision != EMO_FP64_PLAIN) fused = true;
   bool poisson = strcmp(input_info->fields_init, "poisson") == 0;
 
@@ -182,6 +187,7 @@ int run(int argc, char **argv) {
   part_handler->InterpolateEB(grids);
   OutputBoxQuantities *boxOutput =
       new OutputBoxQuantities((dir + "history.dat").c_str(), grids, part_handler, input_info);
+  part_handler->outputParticles(dir.c_str(), 0, input_info);
 
   int nt = input_info->nt;
   time_phys = input_info->t0;
@@ -192,6 +198,7 @@ int run(int argc, char **argv) {
     if (fused) {
       boxOutput->output(time_phys, ti);
       advance(grids, part_handler, 1);
+      part_handler->outputParticles(dir.c_str(), ti + 1, input_info);
       continue;
     }
     if (input_info->nstep_sort > 0 && ti % input_info->nstep_sort == 0) part_handler->SortParticles(compare);
@@ -205,6 +212,7 @@ int run(int argc, char **argv) {
     grids->executeBC(-1);
     part_handler->InterpolateEB(grids);
     time_phys += dt_phys;
+    part_handler->outputParticles(dir.c_str(), ti + 1, input_info);
   }
   write_particles(dir + "particles.bin", part_handler);
 

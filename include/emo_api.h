@@ -134,6 +134,12 @@ int emo_get_particles(emo_ctx *ctx, double *x3, double *v3, double *gamma, int32
                       int64_t *id, double *field6);
 /* per-particle cell ids (Grid::getCellID, src/grid/oGrid.cpp:208-222) in array order */
 int emo_get_cell_ids(emo_ctx *ctx, int32_t *cells);
+/* The rows of the particle track files (Particle_Handler::outputParticles,
+ * src/particles/particle_handler.cpp:308-365: x and v of each particle with
+ * my_id < output_pCount): x3[3 id + d], v3[3 id + d] for id < nid, found[id]
+ * = 1 where a particle with that id exists.  Selected on the device, so only
+ * 7 doubles per tracked id cross to the host. */
+int emo_get_tracked(emo_ctx *ctx, int64_t nid, double *x3, double *v3, int32_t *found);
 
 /* ---- fields ------------------------------------------------------------- */
 int64_t emo_field_points(const emo_ctx *ctx);
