@@ -245,6 +245,16 @@ def test_reference_readinput_unittest():
     assert v["nparticles_tot"] == ["16"]
 
 
+def test_reader_on_deposit_test_input():
+    """test/test_data/depositJ_unittest_data.txt, the input of the reference's
+    deposit test, which needs exactly two particles of opposite charge
+    (test/test_src/depositJ_unittests.cc:72-77) in 5x5x5 cells over 5x1x1."""
+    v, _ = read_input(os.path.join(INPUTS, "depositJ_unittest_data.txt"))
+    assert v["nCell"] == ["5", "5", "5"] and [float(x) for x in v["Lxyz"]] == [5, 1, 1]
+    assert v["nparticles_tot"] == ["2"] and [float(x) for x in v["charge_ratio"]] == [-1, 1]
+    assert v["fields_init"] == ["poisson"]
+
+
 def test_reader_on_unitest_input():
     """Every value of data/unitest/input.txt as the file states it; its three
     waves sit under periodic field boundaries, so readinfo does not read them
@@ -454,7 +464,7 @@ def test_depositor_two_particles_reference_case(fc):
     """sumOverJandRho (test/test_src/depositJ_unittests.cc:64-125): two
     particles of opposite charge in one cell; |sum J| = UNIT_RHOJ |q1 v1 + q2
     v2| / V and sum rho = UNIT_RHOJ (q1 + q2) / V to 1e-15."""
-    lcell = np.array([1.0, 0.2, 0.2])  # depositJ_unittest_data.txt: L = (5,1,1), n = (5,5,5)
+    lcell = np.array([1.0, 0.2, 0.2])  # depositJ_unittest_data.txt: L = (5,1,1), n = (5,5,5), as read above
     verts = np.array([2.0, 0.4, 0.6])
     rng = np.random.default_rng(7)
     J = np.zeros(12)
