@@ -88,7 +88,7 @@ struct ChosenState {
   // lay[1] = rebuilds so far
   int32_t *ns = nullptr, *nm = nullptr, *arr = nullptr, *put = nullptr;
   long long *rpart = nullptr;  // the check's per-block sums of the new ranges (2 x nblk)
-  float4 *mv = nullptr;
+  float *mv = nullptr;
   // akey: arrival ranges' provisional staging slots (destination layout);
   // akey2: the sorted keys of crowded arrival ranges
   int32_t *akey = nullptr, *akey2 = nullptr;
@@ -98,6 +98,8 @@ struct ChosenState {
   void *scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   // initial sort of loaded particles (arbitrary order)
+  // the load's radix sort (keys, permutation, scratch): allocated by
+  // sort_loaded for the sort and freed after it
   uint32_t *key = nullptr, *key_alt = nullptr, *perm = nullptr, *perm_alt = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
@@ -264,7 +266,7 @@ struct PushDepArgs {
   const int4 *pa, *pb;        // records before the step, source layout
   int4 *pa_out, *pb_out;      // records after the step (the stayers here)
   int4 *ma, *mb;              // staging: cell c's movers' new records from slot base[c] on
-  float4 *mv;                 // staging: their velocities (deposit)
+  float *mv;                  // staging: their 1/gamma (v = u / gamma for the deposit)
   int32_t *mvsrc;             // with a tape: their source slots, or null
   const float *g0;            // the stored gamma of the first push, by source slot, or null
   const int32_t *base, *cnt;  // source layout
@@ -348,7 +350,7 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
       const int st = b + nmove + __popc(mm & below);
       a.ma[st] = Ao;
       a.mb[st] = Bo;
-      a.mv[st] = make_float4(o.v[0], o.v[1], o.v[2], 0.f);
+      a.mv[st] = o.ig;
       if (a.mvsrc) a.mvsrc[st] = src;
       const int k = atomicAdd(&a.arr[o.cell], 1);
       const int e1 = a.base[o.cell + 1];
@@ -549,7 +551,7 @@ struct ArriveArgs {
   const int32_t *akey;
   int32_t *akey2;  // the sorted keys of the ranges with more than kArrFast arrivals
   const int4 *ma, *mb;
-  const float4 *mv;
+  const float *mv;
   const int32_t *mvsrc;  // with a tape
   int4 *pa_out, *pb_out;
   uint32_t *fwd;
@@ -602,13 +604,14 @@ __global__ void __launch_bounds__(kBlock) k_arrive_f32(ArriveArgs a) {
   for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
   auto take = [&](int st, int r) {
     const int4 A = a.ma[st], Bv = a.mb[st];
-    const float4 v4 = a.mv[st];
+    const float ig = a.mv[st];
     const int pos = s0 + r;
     a.pa_out[pos] = A;
     a.pb_out[pos] = Bv;
     if (a.fwd) a.fwd[a.mvsrc[st]] = (uint32_t)pos;
     const float f[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
-    const float v[3] = {v4.x, v4.y, v4.z};
+    // the push's v = u * (1/gamma), the same product of the same floats
+    const float v[3] = {as_f(Bv.x) * ig, as_f(Bv.y) * ig, as_f(Bv.z) * ig};
     deposit_j(hi, lo, f, v, jc_s[(uint32_t)A.x >> kSpeciesShift]);
   };
   if (fast) {
@@ -1035,12 +1038,40 @@ int sort_loaded(emo_ctx *c) {
       return rc;
     }
   }
+  auto free_sort = [&]() {
+    hipStreamSynchronize(c->stream);
+    void *ps[] = {s->key, s->key_alt, s->perm, s->perm_alt, s->sort_tmp};
+    for (void *q : ps) dfree(q);
+    s->key = s->key_alt = s->perm = s->perm_alt = nullptr;
+    s->sort_tmp = nullptr;
+    s->sort_tmp_bytes = 0;
+  };
   if (np > 0) {
+    int rc = dalloc(c, &s->key, np);
+    if (!rc) rc = dalloc(c, &s->key_alt, np);
+    if (!rc) rc = dalloc(c, &s->perm, np);
+    if (!rc) rc = dalloc(c, &s->perm_alt, np);
+    if (!rc) {
+      s->sort_tmp_bytes = 0;
+      hipcub::DeviceRadixSort::SortPairs(nullptr, s->sort_tmp_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                         (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)np, 0, s->sort_bits);
+      if (hipMalloc(&s->sort_tmp, std::max<size_t>(s->sort_tmp_bytes, 16)) != hipSuccess) {
+        s->sort_tmp = nullptr;
+        rc = set_err(c, EMO_ERR_ALLOC, "sort scratch");
+      }
+    }
+    if (rc) {
+      free_sort();
+      dfree(g0_dense);
+      dfree(g0_slots);
+      return rc;
+    }
     hipLaunchKernelGGL(k_keys_from_pa, dim3(grid_for(np)), dim3(kBlock), 0, c->stream, s->pa, s->key, s->perm, np);
     size_t bytes = s->sort_tmp_bytes;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->key, s->key_alt, s->perm,
                                                       s->perm_alt, (int)np, 0, s->sort_bits, c->stream);
     if (e != hipSuccess) {
+      free_sort();
       dfree(g0_dense);
       dfree(g0_slots);
       return set_err(c, EMO_ERR_HIP, std::string("radix sort: ") + hipGetErrorString(e));
@@ -1076,6 +1107,7 @@ int sort_loaded(emo_ctx *c) {
     hipLaunchKernelGGL(k_copy_i32, dim3(grid_for(nc + 1)), dim3(kBlock), 0, c->stream, (const int32_t *)s->dcount,
                        s->cnt, nc + 1);
   if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = set_err(c, EMO_ERR_HIP, "layout of the load failed");
+  free_sort();
   dfree(g0_dense);
   if (s->g0_active) {
     if (s->g0_owned) dfree(s->g0);
@@ -1284,10 +1316,6 @@ int fp32_create(emo_ctx *c) {
   if (!rc) rc = dalloc(c, &s->dcount, nc1);
   if (!rc) rc = dalloc(c, &s->doff, nc1);
   if (!rc) rc = dalloc(c, &s->lay, 4);
-  if (!rc) rc = dalloc(c, &s->key, cap);
-  if (!rc) rc = dalloc(c, &s->key_alt, cap);
-  if (!rc) rc = dalloc(c, &s->perm, cap);
-  if (!rc) rc = dalloc(c, &s->perm_alt, cap);
   if (!rc) rc = dalloc(c, &s->cj, 12 * g.ncell);
   if (!rc) rc = dalloc(c, &s->cv, (size_t)kCellVec * g.ncell);
   size_t nt = (size_t)g.ntot;
@@ -1317,11 +1345,6 @@ int fp32_create(emo_ctx *c) {
   int bits = 1;
   while ((1ll << bits) < g.ncell) bits++;
   s->sort_bits = bits;
-  s->sort_tmp_bytes = 0;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, s->sort_tmp_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                     (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)cap, 0, bits);
-  if (hipMalloc(&s->sort_tmp, std::max<size_t>(s->sort_tmp_bytes, 16)) != hipSuccess)
-    return set_err(c, EMO_ERR_ALLOC, "sort scratch");
   s->scan_tmp_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, s->scan_tmp_bytes, (const int32_t *)nullptr, (int32_t *)nullptr,
                                    (int)nc1);

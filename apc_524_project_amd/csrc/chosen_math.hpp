@@ -68,7 +68,8 @@ struct Pushed {
   bool flip[3];  // reflected (u flipped) in dim d
   float f[3];   // new offset
   float u[3];   // new u (after the BCs)
-  float v[3];   // new velocity (deposit)
+  float v[3];   // new velocity (deposit): u[d] * ig
+  float ig;     // 1 / gamma of the new u (1 non-relativistic)
 };
 
 
@@ -241,6 +242,7 @@ __device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, 
   o.v[0] = ux * ig_new;
   o.v[1] = uy * ig_new;
   o.v[2] = uz * ig_new;
+  o.ig = ig_new;
 }
 
 
