@@ -31,7 +31,7 @@ EXPORTS = (
     "emo_deposit", "emo_fold", "emo_evolve_fields", "emo_refresh", "emo_interpolate",
     "emo_step", "emo_diag", "emo_synchronize", "emo_timing_enable", "emo_timing_select", "emo_kernel_time",
     "emo_timing_reset", "emo_adjoint_field_energy", "emo_loader_dx_deps", "emo_init_poisson_noalloc",
-    "emo_layout_info", "emo_get_tracked",
+    "emo_layout_info", "emo_get_tracked", "emo_adjoint_stats",
 )
 
 
@@ -137,6 +137,7 @@ def lib():
     L.emo_get_particles.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.emo_get_cell_ids.argtypes = [vp, vp]
     L.emo_get_tracked.argtypes = [vp, i64, vp, vp, vp]
+    L.emo_adjoint_stats.argtypes = [vp, vp, vp]
     L.emo_field_points.argtypes = [vp]
     L.emo_field_points.restype = i64
     L.emo_get_field.argtypes = [vp, C.c_int, vp]
@@ -349,6 +350,17 @@ class Context:
             _p(xb) if particles else None, _p(vb) if particles else None, _p(gb) if particles else None))
         return dict(energy=float(e[0]), d_amp=da[:nw], d_phase=da[:0] if nw == 0 else dp[:nw],
                     xbar0=xb, vbar0=vb, gbar0=gb)
+
+    def adjoint_stats(self):
+        """The last gradient call: checkpointing ("segments" or "binomial"),
+        snapshots held, forward steps run, restores, and per species the sums
+        of xbar0[3], vbar0[3], gbar0 (an array nspecies x 7)."""
+        info = np.zeros(4, np.int64)
+        ns = max(int(self.cfg.nspecies), 1)
+        sums = np.zeros(7 * ns)
+        self._chk(self.L.emo_adjoint_stats(self.h, _p(info), _p(sums)))
+        return dict(checkpointing="binomial" if info[0] == 1 else "segments", snapshots=int(info[1]),
+                    forward_steps=int(info[2]), restores=int(info[3]), sums=sums.reshape(ns, 7))
 
 
 def loader_dx_deps(ld: EmoLoader, cfg: EmoConfig) -> np.ndarray:

@@ -975,9 +975,13 @@ struct CellStepAdj32Args {
   double *cg;                   // 18 per cell, component-major
   long long ncell;
   uint32_t nslots;
+  // BYID: the adjoints by particle id, 6 per id {f[3], u[3]}, updated in
+  // place (no fwd map); g0b by id
+  float *adj;
+  long long np;
 };
 
-template <bool REL, bool PER>
+template <bool REL, bool PER, bool BYID>
 __global__ void __launch_bounds__(kBlock) k_cell_step_adj32(CellStepAdj32Args a) {
   __shared__ float qp_s[EMO_NSPEC], jc_s[EMO_NSPEC];
   if (threadIdx.x < EMO_NSPEC) {
@@ -1011,27 +1015,55 @@ __global__ void __launch_bounds__(kBlock) k_cell_step_adj32(CellStepAdj32Args a)
       const int last = e - 1;
       int i = b + l;
       int4 A = a.pa[min(i, last)], Bv = a.pb[min(i, last)];
-      // (a slot the step did not map -- only after a flagged error -- reads slot 0)
-      uint32_t j = a.fwd[min(i, last)];
-      j = j < a.nslots ? j : 0u;
+      // (a slot the step did not map -- only after a flagged error -- reads
+      // slot 0; by id, an id outside [0, np) reads and writes nothing)
+      uint32_t j = BYID ? (uint32_t)Bv.w : a.fwd[min(i, last)];
+      if (!BYID) j = j < a.nslots ? j : 0u;
       for (; i < e; i += 16) {
         const int inx = min(i + 16, last);
         const int4 An = a.pa[inx], Bn = a.pb[inx];
-        uint32_t jn = a.fwd[inx];
-        jn = jn < a.nslots ? jn : 0u;
+        uint32_t jn = BYID ? (uint32_t)Bn.w : a.fwd[inx];
+        if (!BYID) jn = jn < a.nslots ? jn : 0u;
+        const bool ok = !BYID || (long long)j < a.np;
         const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
         const float fin[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
         const float uin[3] = {as_f(Bv.x), as_f(Bv.y), as_f(Bv.z)};
-        const float fbo[3] = {a.in.f[0][j], a.in.f[1][j], a.in.f[2][j]};
-        const float ubo[3] = {a.in.u[0][j], a.in.u[1][j], a.in.u[2][j]};
+        float fbo[3] = {0.f, 0.f, 0.f}, ubo[3] = {0.f, 0.f, 0.f};
+        if (BYID) {
+          if (ok) {
+            const float2 *q = reinterpret_cast<const float2 *>(a.adj + 6 * (long long)j);
+            const float2 q0 = q[0], q1 = q[1], q2 = q[2];
+            fbo[0] = q0.x;
+            fbo[1] = q0.y;
+            fbo[2] = q1.x;
+            ubo[0] = q1.y;
+            ubo[1] = q2.x;
+            ubo[2] = q2.y;
+          }
+        } else {
+          for (int d = 0; d < 3; d++) {
+            fbo[d] = a.in.f[d][j];
+            ubo[d] = a.in.u[d][j];
+          }
+        }
         float fb[3], ub[3], g0b, cvb[18];
         push_deposit_vjp<REL, PER>(a.k, V, A.x, fin, uin, qp_s[sp], a.g0 != nullptr, a.g0 ? a.g0[i] : 0.f,
                                    jc_s[sp], a.Jb, fbo, ubo, fb, ub, g0b, cvb);
-        for (int d = 0; d < 3; d++) {
-          a.out.f[d][i] = fb[d];
-          a.out.u[d][i] = ub[d];
+        if (BYID) {
+          if (ok) {
+            float2 *q = reinterpret_cast<float2 *>(a.adj + 6 * (long long)j);
+            q[0] = make_float2(fb[0], fb[1]);
+            q[1] = make_float2(fb[2], ub[0]);
+            q[2] = make_float2(ub[1], ub[2]);
+            if (a.g0b) a.g0b[j] = g0b;
+          }
+        } else {
+          for (int d = 0; d < 3; d++) {
+            a.out.f[d][i] = fb[d];
+            a.out.u[d][i] = ub[d];
+          }
+          if (a.g0b) a.g0b[i] = g0b;
         }
-        if (a.g0b) a.g0b[i] = g0b;
 #pragma unroll
         for (int q = 0; q < 18; q++) acc[q] += (double)cvb[q];
         A = An;
@@ -1204,6 +1236,24 @@ int recompute32(AdjRun32 &R, int seg, int set) {
   return rc;
 }
 
+void launch_cell_step_adj32(emo_ctx *c, hipStream_t as, const CellStepAdj32Args &ca, bool all_per, unsigned blocks,
+                            bool byid) {
+  const bool rel = c->cfg.relativity != 0;
+#define EMO_ADJ32(R, P, I) hipLaunchKernelGGL((k_cell_step_adj32<R, P, I>), dim3(blocks), dim3(kBlock), 0, as, ca)
+  if (byid) {
+    if (rel && all_per) EMO_ADJ32(true, true, true);
+    else if (rel) EMO_ADJ32(true, false, true);
+    else if (all_per) EMO_ADJ32(false, true, true);
+    else EMO_ADJ32(false, false, true);
+  } else {
+    if (rel && all_per) EMO_ADJ32(true, true, false);
+    else if (rel) EMO_ADJ32(true, false, false);
+    else if (all_per) EMO_ADJ32(false, true, false);
+    else EMO_ADJ32(false, false, false);
+  }
+#undef EMO_ADJ32
+}
+
 void backward32(AdjRun32 &R, int seg, int set) {
   emo_ctx *c = R.c;
   const Geo &g = c->geo;
@@ -1239,17 +1289,12 @@ void backward32(AdjRun32 &R, int seg, int set) {
     ca.cg = R.af.cg;
     ca.ncell = g.ncell;
     ca.nslots = (uint32_t)fp32_view(c).nslots;
+    ca.adj = nullptr;
+    ca.np = 0;
     const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
     {
       TimedLaunch tl(c, "adj_cells_f32", as);
-      if (c->cfg.relativity && all_per)
-        hipLaunchKernelGGL((k_cell_step_adj32<true, true>), dim3(blocks), dim3(kBlock), 0, as, ca);
-      else if (c->cfg.relativity)
-        hipLaunchKernelGGL((k_cell_step_adj32<true, false>), dim3(blocks), dim3(kBlock), 0, as, ca);
-      else if (all_per)
-        hipLaunchKernelGGL((k_cell_step_adj32<false, true>), dim3(blocks), dim3(kBlock), 0, as, ca);
-      else
-        hipLaunchKernelGGL((k_cell_step_adj32<false, false>), dim3(blocks), dim3(kBlock), 0, as, ca);
+      launch_cell_step_adj32(c, as, ca, all_per, blocks, false);
     }
     gather_adj_grid_roles(c, as, pre.b_is_a0, R.af);
     std::swap(R.pa, R.pa2);
@@ -1261,7 +1306,706 @@ __global__ void k_zero_f(float *p, long long n) {
   if (i < n) p[i] = 0.f;
 }
 
+
+// ---------------------------------------------------------------------------
+// Per-species sums of the initial-state gradients (emo_adjoint_stats): the
+// sums of what the by-id outputs hold, reduced on the device.
+// ---------------------------------------------------------------------------
+struct SumsArgs32 {
+  const int4 *ra, *rb;        // the initial state's records
+  const int32_t *rbase;       // cell c's records from ra[rbase[c]] (slots, or dense offsets)
+  const int32_t *sbase;       // cell c's slots (the stored gamma's layout)
+  const int32_t *cnt;
+  const float *g0;            // the stored gamma by slot, or null
+  const float *adj;           // by id: 6 per id {f[3], u[3]}; else null
+  PartAdj32 pa;               // by slot (adj null)
+  const float *g0b;           // by id (adj) or slot, or null
+  double idx[3];              // 1 / dx
+  int rel, ns;
+  long long ncell, np;
+  double *part;               // [gridDim.x][ns * 7]
+};
+
+__global__ void __launch_bounds__(kBlock) k_species_sums32(SumsArgs32 a) {
+  __shared__ double sh[7][kBlock];
+  const long long c = (long long)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4);
+  const int l = threadIdx.x & 15;
+  for (int q = 0; q < a.ns; q++) {
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (c < a.ncell) {
+      const int n = a.cnt[c];
+      for (int k = l; k < n; k += 16) {
+        const long long r = (long long)a.rbase[c] + k, s = (long long)a.sbase[c] + k;
+        const int4 A = a.ra[r], B = a.rb[r];
+        if ((int)((uint32_t)A.x >> kSpeciesShift) != q) continue;
+        const long long id = B.w;
+        float fb[3], ub[3], gb0 = 0.f;
+        if (a.adj) {
+          if (id < 0 || id >= a.np) continue;
+          const float *p = a.adj + 6 * id;
+          for (int d = 0; d < 3; d++) {
+            fb[d] = p[d];
+            ub[d] = p[3 + d];
+          }
+          if (a.g0b) gb0 = a.g0b[id];
+        } else {
+          for (int d = 0; d < 3; d++) {
+            fb[d] = a.pa.f[d][s];
+            ub[d] = a.pa.u[d][s];
+          }
+          if (a.g0b) gb0 = a.g0b[s];
+        }
+        const double gm = a.g0 ? (double)a.g0[s] : 1.0;
+        const float u[3] = {as_f(B.x), as_f(B.y), as_f(B.z)};
+        double gb = a.g0 ? (double)gb0 : 0.0;
+        for (int d = 0; d < 3; d++) {
+          acc[d] += (double)fb[d] * a.idx[d];
+          acc[3 + d] += (double)ub[d] * (a.rel ? gm : 1.0);
+          gb += (double)ub[d] * ((double)u[d] / gm);
+        }
+        acc[6] += a.rel ? gb : 0.0;
+      }
+    }
+    for (int j = 0; j < 7; j++) sh[j][threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w)
+        for (int j = 0; j < 7; j++) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x < 7) a.part[((long long)blockIdx.x * a.ns + q) * 7 + threadIdx.x] = sh[threadIdx.x][0];
+    __syncthreads();
+  }
+}
+
+// device buffers outside the pool (hipMalloc, hipFree)
+template <typename T>
+int adj_alloc(emo_ctx *c, T **p, size_t n) {
+  if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    c->err = "hipMalloc failed in the adjoint";
+    return EMO_ERR_ALLOC;
+  }
+  return EMO_OK;
+}
+void adj_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+// FP64 plain form: the particle adjoints by position i of the initial order
+// (ids id[i], species sp[i])
+struct SumsArgs64 {
+  PartAdj pb;
+  const int32_t *sp;
+  const int64_t *id;
+  int ns;
+  long long np;
+  double *part;  // [gridDim.x][ns * 7]
+};
+
+__global__ void __launch_bounds__(kBlock) k_species_sums64(SumsArgs64 a) {
+  __shared__ double sh[7][kBlock];
+  for (int q = 0; q < a.ns; q++) {
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < a.np; i += (long long)gridDim.x * kBlock) {
+      if (a.sp[i] != q || a.id[i] < 0 || a.id[i] >= a.np) continue;
+      for (int d = 0; d < 3; d++) {
+        acc[d] += a.pb.x[d][i];
+        acc[3 + d] += a.pb.v[d][i];
+      }
+      acc[6] += a.pb.gam[i];
+    }
+    for (int j = 0; j < 7; j++) sh[j][threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w)
+        for (int j = 0; j < 7; j++) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x < 7) a.part[((long long)blockIdx.x * a.ns + q) * 7 + threadIdx.x] = sh[threadIdx.x][0];
+    __syncthreads();
+  }
+}
+
+// block partials [nb][ns * 7] -> c->adj_sums, summed in block order
+int finish_species_sums(emo_ctx *c, double *part, long long nb, int ns) {
+  std::vector<double> h((size_t)(nb * ns * 7));
+  hipError_t e = hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  adj_free(part);
+  if (e != hipSuccess) {
+    c->err = hipGetErrorString(e);
+    return EMO_ERR_HIP;
+  }
+  std::fill(c->adj_sums, c->adj_sums + 7 * EMO_NSPEC, 0.0);
+  for (long long b = 0; b < nb; b++)
+    for (int q = 0; q < ns * 7; q++) c->adj_sums[q] += h[(size_t)(b * ns * 7 + q)];
+  return EMO_OK;
+}
+
+int species_sums64(emo_ctx *c, SumsArgs64 a) {
+  const long long nb = 1024;
+  const int ns = std::max(1, (int)c->cfg.nspecies);
+  double *part = nullptr;
+  int rc = adj_alloc(c, &part, (size_t)(nb * ns * 7));
+  if (rc) return rc;
+  a.ns = ns;
+  a.np = c->np;
+  a.part = part;
+  hipLaunchKernelGGL(k_species_sums64, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, a);
+  return finish_species_sums(c, part, nb, ns);
+}
+
+// the sums into c->adj_sums (partials on the device, summed in block order)
+int species_sums32(emo_ctx *c, SumsArgs32 a) {
+  const long long nb = (c->geo.ncell + (kBlock / 16) - 1) / (kBlock / 16);
+  const int ns = std::max(1, (int)c->cfg.nspecies);
+  double *part = nullptr;
+  int rc = adj_alloc(c, &part, (size_t)(nb * ns * 7));
+  if (rc) return rc;
+  for (int d = 0; d < 3; d++) a.idx[d] = c->geo.id[d];
+  a.rel = c->cfg.relativity;
+  a.ns = ns;
+  a.ncell = c->geo.ncell;
+  a.np = c->np;
+  a.part = part;
+  hipLaunchKernelGGL(k_species_sums32, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, a);
+  return finish_species_sums(c, part, nb, ns);
+}
+
+// ---------------------------------------------------------------------------
+// Binomial checkpointing: the low-memory gradient of the FP32 form
+// ---------------------------------------------------------------------------
+// Snapshots hold the state densely (the records in cell order, 32 bytes a
+// particle, with the fields and the slot layout); the particle adjoints are
+// indexed by particle id and updated in place; no step keeps a tape.  The
+// backward step of step s runs on the live state at s, which a restore and a
+// re-run reach.  The split points minimise the re-run steps for the number
+// of snapshots that fit in device memory (binomial checkpointing, Griewank
+// and Walther, "Algorithm 799: revolve", ACM TOMS 26, 2000), found by
+// dynamic programming over (steps, free snapshots).
+
+struct Dense32 {
+  int4 *ra = nullptr, *rb = nullptr;
+  float *E[3] = {}, *A[2][3] = {};  // A[0] = Bfix, A[1] = Btfix
+  int32_t *base = nullptr, *cnt = nullptr, *doff = nullptr;
+  bool b_is_a0 = true;
+  int step = -1;                    // -1: free
+};
+
+void dense32_alloc(Pool &P, const emo_ctx *c, Dense32 &s) {
+  const size_t n = (size_t)std::max<int64_t>(c->np, 1), nt = (size_t)c->geo.ntot;
+  const size_t nc1 = (size_t)c->geo.ncell + 1;
+  P.take(&s.ra, n);
+  P.take(&s.rb, n);
+  for (int d = 0; d < 3; d++) {
+    P.take(&s.E[d], nt);
+    P.take(&s.A[0][d], nt);
+    P.take(&s.A[1][d], nt);
+  }
+  P.take(&s.base, nc1);
+  P.take(&s.cnt, nc1);
+  P.take(&s.doff, nc1);
+}
+
+struct PackArgs {
+  int4 *pa, *pb;             // the slot layout
+  int4 *ra, *rb;             // dense, cell order
+  const int32_t *base, *cnt, *doff;
+  long long ncell, np;
+};
+
+// cell c's cnt[c] records between the slots from base[c] and the dense
+// positions from doff[c]; 16 lanes per cell
+template <bool UNPACK>
+__global__ void __launch_bounds__(kBlock) k_pack_dense32(PackArgs a) {
+  const long long c = (long long)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4);
+  const int l = threadIdx.x & 15;
+  if (c >= a.ncell) return;
+  const int n = a.cnt[c];
+  const long long b = a.base[c], d = a.doff[c];
+  for (int k = l; k < n; k += 16) {
+    if (d + k >= a.np) break;  // (only where particles were lost, which is flagged)
+    if (UNPACK) {
+      a.pa[b + k] = a.ra[d + k];
+      a.pb[b + k] = a.rb[d + k];
+    } else {
+      a.ra[d + k] = a.pa[b + k];
+      a.rb[d + k] = a.pb[b + k];
+    }
+  }
+}
+
+// Split points of binomial checkpointing: arg(n, s) for reversing n steps
+// with s free snapshots, minimising re-run steps plus kRestoreCost per
+// restore (a restore moves about as many bytes as half a step).
+struct BinomialPlan {
+  int N = 0, S = 0;
+  std::vector<double> cost;
+  std::vector<int> arg;
+  static constexpr double kRestoreCost = 0.5;
+  double &C(int n, int s) { return cost[(size_t)s * (N + 1) + n]; }
+  int &A(int n, int s) { return arg[(size_t)s * (N + 1) + n]; }
+  void build(int n_, int s_) {
+    N = n_;
+    S = s_;
+    cost.assign((size_t)(S + 1) * (N + 1), 0.0);
+    arg.assign((size_t)(S + 1) * (N + 1), 1);
+    for (int n = 2; n <= N; n++) C(n, 0) = C(n - 1, 0) + (n - 1) + kRestoreCost;
+    const bool exact = (double)N * N * S <= 2e8;
+    for (int s = 1; s <= S; s++) {
+      C(1, s) = 0.0;
+      for (int n = 2; n <= N; n++) {
+        if (!exact) {  // long runs: equal parts
+          const int k = std::max(1, n / (s + 1));
+          A(n, s) = k;
+          C(n, s) = k + C(n - k, s - 1) + kRestoreCost + C(k, s);
+          continue;
+        }
+        double best = -1.0;
+        int bk = 1;
+        for (int k = 1; k < n; k++) {
+          const double v = k + C(n - k, s - 1) + kRestoreCost + C(k, s);
+          if (best < 0 || v < best) {
+            best = v;
+            bk = k;
+          }
+        }
+        C(n, s) = best;
+        A(n, s) = bk;
+      }
+    }
+  }
+};
+
+struct Lean32 {
+  emo_ctx *c = nullptr;
+  double dt = 0.0;
+  int N = 0, S = 0;
+  bool has_g0 = false, all_per = true;
+  std::vector<double> tstep;
+  std::vector<Dense32> slot;     // slot[0]: the initial state
+  std::vector<int> free_slots;
+  std::vector<int> stored;       // step -> slot holding it, or -1
+  int live = 0;                  // the step of the live state
+  float *g0slots = nullptr;      // the stored gamma of the initial layout (owned here)
+  float *adj = nullptr;          // 6 per particle id
+  float *g0b = nullptr;          // by id (first step)
+  float *Jb32[3] = {};
+  AdjFields af;
+  double *dsums = nullptr;
+  std::vector<FillPlane> planes;
+  BinomialPlan plan;
+  int64_t nfwd = 0, nrestore = 0;
+  int rc = 0;
+};
+
+void dense32_take(Lean32 &L, int sl, int step) {
+  emo_ctx *c = L.c;
+  Dense32 &s = L.slot[sl];
+  Fp32View v = fp32_view(c);
+  const size_t nc1 = (size_t)c->geo.ncell + 1, nt4 = (size_t)c->geo.ntot * 4;
+  CopyBatch cb(c);
+  cb.add(s.base, *v.base, nc1 * 4);
+  cb.add(s.cnt, *v.cnt, nc1 * 4);
+  for (int d = 0; d < 3; d++) {
+    cb.add(s.E[d], v.fs->E[d], nt4);
+    cb.add(s.A[0][d], v.fs->Bfix[d], nt4);
+    cb.add(s.A[1][d], v.Btfix[d], nt4);
+  }
+  cb.flush();
+  if (!L.rc) L.rc = fp32_scan_i32(c, s.cnt, s.doff, (long long)nc1);
+  PackArgs pk{*v.pa, *v.pb, s.ra, s.rb, s.base, s.cnt, s.doff, c->geo.ncell, c->np};
+  const unsigned blocks = (unsigned)((c->geo.ncell + (kBlock / 16) - 1) / (kBlock / 16));
+  hipLaunchKernelGGL(k_pack_dense32<false>, dim3(blocks), dim3(kBlock), 0, c->stream, pk);
+  s.b_is_a0 = v.fs->B[0] == v.fs->Bfix[0];
+  s.step = step;
+  L.stored[step] = sl;
+}
+
+void dense32_restore(Lean32 &L, int sl) {
+  emo_ctx *c = L.c;
+  const Dense32 &s = L.slot[sl];
+  Fp32View v = fp32_view(c);
+  const size_t nc1 = (size_t)c->geo.ncell + 1, nt4 = (size_t)c->geo.ntot * 4;
+  CopyBatch cb(c);
+  cb.add(*v.base, s.base, nc1 * 4);
+  cb.add(*v.cnt, s.cnt, nc1 * 4);
+  for (int d = 0; d < 3; d++) {
+    cb.add(v.fs->E[d], s.E[d], nt4);
+    cb.add(v.fs->Bfix[d], s.A[0][d], nt4);
+    cb.add(v.Btfix[d], s.A[1][d], nt4);
+  }
+  cb.flush();
+  PackArgs pk{*v.pa, *v.pb, s.ra, s.rb, s.base, s.cnt, s.doff, c->geo.ncell, c->np};
+  const unsigned blocks = (unsigned)((c->geo.ncell + (kBlock / 16) - 1) / (kBlock / 16));
+  hipLaunchKernelGGL(k_pack_dense32<true>, dim3(blocks), dim3(kBlock), 0, c->stream, pk);
+  for (int q = 0; q < 3; q++) {
+    v.fs->B[q] = s.b_is_a0 ? v.fs->Bfix[q] : v.Btfix[q];
+    v.fs->Bt[q] = s.b_is_a0 ? v.Btfix[q] : v.fs->Bfix[q];
+  }
+  const bool g0 = s.step == 0 && L.has_g0;
+  *v.g0_active = g0;
+  *v.g0 = g0 ? L.g0slots : nullptr;
+  *v.g0_owned = false;
+  L.live = s.step;
+  L.nrestore++;
+}
+
+void lean_advance(Lean32 &L, int to) {
+  while (!L.rc && L.live < to) {
+    L.rc = fp32_step_one(L.c, L.tstep[L.live], L.dt, nullptr);
+    L.live++;
+    L.nfwd++;
+  }
+}
+
+// the live state at step `to` (a <= to), from the snapshot sa of step a
+void lean_reach(Lean32 &L, int sa, int to) {
+  if (L.rc) return;
+  const int a = L.slot[sa].step;
+  if (L.live > to || L.live < a) dense32_restore(L, sa);
+  lean_advance(L, to);
+}
+
+int lean_store(Lean32 &L, int step) {
+  const int sl = L.free_slots.back();
+  L.free_slots.pop_back();
+  dense32_take(L, sl, step);
+  return sl;
+}
+
+// the backward step of step st on the live state (at st)
+void lean_backward(Lean32 &L, int st) {
+  if (L.rc) return;
+  emo_ctx *c = L.c;
+  const Geo &g = c->geo;
+  Fp32View v = fp32_view(c);
+  const bool b_is_a0 = v.fs->B[0] == v.fs->Bfix[0];
+  const int nplanes = (int)L.planes.size();
+  fields_step_adj(c, c->stream, L.af, L.planes, L.dsums + (size_t)st * nplanes, L.dt, b_is_a0);
+  hipLaunchKernelGGL(k_round_f32, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, (const double *)L.af.Jb[0],
+                     (const double *)L.af.Jb[1], (const double *)L.af.Jb[2], L.Jb32[0], L.Jb32[1], L.Jb32[2],
+                     (long long)g.ntot);
+  CellStepAdj32Args ca{};
+  ca.k = fp32_push_consts(c, L.dt);
+  ca.pa = *v.pa;
+  ca.pb = *v.pb;
+  ca.g0 = (st == 0 && L.has_g0) ? L.g0slots : nullptr;
+  ca.base = *v.base;
+  ca.cnt = *v.cnt;
+  ca.fwd = nullptr;
+  for (int d = 0; d < 3; d++) {
+    ca.E[d] = v.fs->E[d];
+    ca.B[d] = b_is_a0 ? v.fs->Bfix[d] : v.Btfix[d];
+    ca.Bt[d] = b_is_a0 ? v.Btfix[d] : v.fs->Bfix[d];
+    ca.Jb[d] = L.Jb32[d];
+  }
+  ca.g0b = ca.g0 ? L.g0b : nullptr;
+  if (ca.g0b) {
+    // (g0b may share a snapshot's memory, free from this step on: cleared here)
+    hipLaunchKernelGGL(k_zero_f, dim3(grid_for(std::max<int64_t>(c->np, 1))), dim3(kBlock), 0, c->stream, L.g0b,
+                       (long long)c->np);
+  }
+  ca.cg = L.af.cg;
+  ca.ncell = g.ncell;
+  ca.nslots = (uint32_t)v.nslots;
+  ca.adj = L.adj;
+  ca.np = c->np;
+  const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
+  {
+    TimedLaunch tl(c, "adj_cells_f32", c->stream);
+    launch_cell_step_adj32(c, c->stream, ca, L.all_per, blocks, true);
+  }
+  gather_adj_grid_roles(c, c->stream, b_is_a0, L.af);
+}
+
+// reverses steps [a, b) with s free snapshots; snapshot sa holds step a
+void lean_reverse(Lean32 &L, int a, int b, int s, int sa) {
+  if (L.rc) return;
+  if (b - a == 1) {
+    lean_reach(L, sa, a);
+    lean_backward(L, a);
+    return;
+  }
+  const int m = s > 0 ? a + L.plan.A(b - a, s) : a;
+  // no free snapshot (or, defensively, none left to store the split point in):
+  // every backward step re-runs from a
+  if (s == 0 || (L.stored[m] < 0 && L.free_slots.empty())) {
+    for (int st = b - 1; st >= a && !L.rc; st--) {
+      lean_reach(L, sa, st);
+      lean_backward(L, st);
+    }
+    return;
+  }
+  int sm = L.stored[m];
+  if (sm < 0) {
+    lean_reach(L, sa, m);
+    if (L.rc) return;
+    sm = lean_store(L, m);
+  }
+  lean_reverse(L, m, b, s - 1, sm);
+  L.stored[m] = -1;
+  L.slot[sm].step = -1;
+  L.free_slots.push_back(sm);
+  lean_reverse(L, a, m, s, sa);
+}
+
 }  // namespace
+
+// the most snapshots binomial checkpointing takes when it picks the number
+constexpr int kMaxBinomialSnapshots = 32;
+
+// The gradient call of the FP32 chosen form with binomial checkpointing:
+// snapshots in device memory, as many as fit (at most smax besides the
+// initial state), particle adjoints by id, no tape.  For runs whose K-step
+// segments do not fit (checkpoint_every < 0 asks for it).
+int fp32_adjoint_binomial(emo_ctx *c, int32_t nsteps, double t0, double dt, int smax, double *energy, double *d_amp,
+                          double *d_phase, double *xbar0, double *vbar0, double *gbar0) {
+  const Geo &g = c->geo;
+  const int64_t np = c->np;
+  const int nw = c->cfg.nwaves;
+  const bool rel = c->cfg.relativity != 0;
+  Fp32View v = fp32_view(c);
+  Lean32 L;
+  L.c = c;
+  L.dt = dt;
+  L.N = nsteps;
+  L.planes = fill_planes(c);
+  L.has_g0 = *v.g0_active && rel;
+  for (int q = 0; q < 6; q++) L.all_per = L.all_per && c->cfg.parts_bound[q] == EMO_BC_PERIODIC;
+  // the times of the steps, accumulated as emo_step advances them
+  L.tstep.resize(nsteps + 1);
+  L.tstep[0] = t0;
+  for (int s = 0; s < nsteps; s++) L.tstep[s + 1] = L.tstep[s] + dt;
+  const size_t nt = (size_t)g.ntot;
+  const int nplanes = (int)L.planes.size();
+  auto layout = [&](Pool &P, int S) {
+    L.slot.assign(S + 1, Dense32());
+    P.take(&L.adj, 6 * (size_t)std::max<int64_t>(np, 1));
+    for (int d = 0; d < 3; d++) {
+      P.take(&L.af.Eb[d], nt);
+      P.take(&L.af.Ab[0][d], nt);
+      P.take(&L.af.Ab[1][d], nt);
+      P.take(&L.af.Jb[d], nt);
+      P.take(&L.Jb32[d], nt);
+    }
+    P.take(&L.af.cg, 18 * (size_t)g.ncell);
+    P.take(&L.dsums, (size_t)std::max(1, nsteps * nplanes));
+    // with a free snapshot the stored gamma's adjoint (written by the first
+    // step's backward step, when every other snapshot is free) uses one
+    if (L.has_g0 && S == 0) P.take(&L.g0b, (size_t)std::max<int64_t>(np, 1));
+    for (int k = 0; k <= S; k++) dense32_alloc(P, c, L.slot[k]);
+  };
+  // the snapshots that fit in what the device has free (the held pool counts)
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    c->err = "hipMemGetInfo failed";
+    return EMO_ERR_HIP;
+  }
+  const size_t avail = fr + c->adj_pool_bytes, margin = (size_t)1 << 30;
+  const int scap = std::max(0, std::min(smax, nsteps - 1));
+  int S = 0;
+  {
+    Pool d0;
+    layout(d0, 0);
+    if (d0.off + margin > avail) {
+      c->err = "not enough device memory for the adjoint (particle adjoints and one snapshot)";
+      return EMO_ERR_ALLOC;
+    }
+    while (S < scap) {
+      Pool dp;
+      layout(dp, S + 1);
+      if (dp.off + margin > avail) break;
+      S++;
+    }
+  }
+  for (;;) {
+    Pool dry;
+    layout(dry, S);
+    if (dry.off <= c->adj_pool_bytes) break;
+    if (c->adj_pool) hipFree(c->adj_pool);
+    c->adj_pool = nullptr;
+    c->adj_pool_bytes = 0;
+    if (hipMalloc(&c->adj_pool, dry.off) == hipSuccess) {
+      c->adj_pool_bytes = dry.off;
+      break;
+    }
+    c->adj_pool = nullptr;
+    (void)hipGetLastError();
+    if (S == 0) {
+      c->err = "hipMalloc failed for the adjoint's snapshots";
+      return EMO_ERR_ALLOC;
+    }
+    S--;
+  }
+  {
+    Pool real;
+    real.base = (char *)c->adj_pool;
+    layout(real, S);
+  }
+  L.S = S;
+  for (int k = S; k >= 1; k--) L.free_slots.push_back(k);
+  L.stored.assign((size_t)nsteps + 1, -1);
+  if (nsteps > 0) L.plan.build(nsteps, S);
+  if (L.has_g0 && S >= 1) L.g0b = reinterpret_cast<float *>(L.slot[1].ra);
+  // the stored gamma of the initial layout, kept here for the re-runs of the
+  // first step (the context's copy, or a copy of a borrowed one)
+  if (L.has_g0) {
+    if (*v.g0_owned) {
+      L.g0slots = *v.g0;
+    } else {
+      if (adj_alloc(c, &L.g0slots, (size_t)v.nslots)) return EMO_ERR_ALLOC;
+      CopyBatch cb(c);
+      cb.add(L.g0slots, *v.g0, (size_t)v.nslots * 4);
+      cb.flush();
+    }
+    *v.g0 = L.g0slots;
+    *v.g0_owned = false;
+  }
+  // ---- first pass: the initial snapshot, the rightmost split points, F ----
+  L.live = 0;
+  dense32_take(L, 0, 0);
+  {
+    int a = 0, b = nsteps, s = S;
+    while (!L.rc && s > 0 && b - a > 1) {
+      const int m = a + L.plan.A(b - a, s);
+      lean_advance(L, m);
+      if (L.rc) break;
+      lean_store(L, m);
+      a = m;
+      s--;
+    }
+  }
+  lean_advance(L, nsteps);
+  if (!L.rc) L.rc = fp32_check(c);
+  double F = 0.0;
+  if (!L.rc) {
+    double out8[8];
+    L.rc = emo_diag(c, out8);
+    F = out8[4] + out8[5];
+  }
+  // ---- seed and reverse ----------------------------------------------------
+  if (!L.rc) {
+    for (int d = 0; d < 3; d++) {
+      launch_zero(c->stream, L.af.Eb[d], nt);
+      launch_zero(c->stream, L.af.Ab[0][d], nt);
+      launch_zero(c->stream, L.af.Ab[1][d], nt);
+      launch_zero(c->stream, L.af.Jb[d], nt);
+    }
+    hipLaunchKernelGGL(k_zero_f, dim3(grid_for(6 * std::max<int64_t>(np, 1))), dim3(kBlock), 0, c->stream, L.adj,
+                       (long long)(6 * np));
+    SeedArgs<float> sa;
+    sa.g = g;
+    for (int d = 0; d < 3; d++) {
+      sa.E[d] = v.fs->E[d];
+      sa.B[d] = v.fs->Bfix[d];
+      sa.Eb[d] = L.af.Eb[d];
+      sa.Bb[d] = L.af.Ab[0][d];
+    }
+    hipLaunchKernelGGL(k_seed_energy<float>, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, sa);
+  }
+  if (!L.rc && nsteps > 0) lean_reverse(L, 0, nsteps, S, 0);
+  if (!L.rc) L.rc = fp32_check(c);
+  int rc = L.rc;
+  const Dense32 &s0 = L.slot[0];
+  // ---- outputs ---------------------------------------------------------------
+  if (!rc) {
+    if (energy) *energy = F;
+    std::vector<double> sums((size_t)std::max(1, nsteps * nplanes));
+    EMO_HIP_CHECK_RET(c, hipMemcpyAsync(sums.data(), L.dsums, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
+    std::vector<double> da(nw, 0.0), dp(nw, 0.0), ta, tp;
+    for (int s = 0; s < nsteps; s++)
+      for (int k = 0; k < nplanes; k++) {
+        pulse_derivs(c, L.planes[k], L.tstep[s], dt, ta, tp);
+        for (int w = 0; w < nw; w++) {
+          da[w] += sums[(size_t)s * nplanes + k] * ta[w];
+          dp[w] += sums[(size_t)s * nplanes + k] * tp[w];
+        }
+      }
+    for (int w = 0; w < nw; w++) {
+      if (d_amp) d_amp[w] = da[w];
+      if (d_phase) d_phase[w] = dp[w];
+    }
+    SumsArgs32 sa{};
+    sa.ra = s0.ra;
+    sa.rb = s0.rb;
+    sa.rbase = s0.doff;
+    sa.sbase = s0.base;
+    sa.cnt = s0.cnt;
+    sa.g0 = L.has_g0 ? L.g0slots : nullptr;
+    sa.adj = L.adj;
+    sa.g0b = (L.has_g0 && nsteps > 0) ? L.g0b : nullptr;
+    rc = species_sums32(c, sa);
+    // particle adjoints of the initial state, by id (the offsets f = (x -
+    // x0) / dx - ix and, relativistic, u = gamma0 v with the stored gamma0)
+    if (!rc && (xbar0 || vbar0 || gbar0) && np > 0) {
+      const size_t nc1 = (size_t)g.ncell + 1;
+      std::vector<int32_t> hbase(nc1), hcnt(nc1), hdoff(nc1);
+      std::vector<float> adj((size_t)(6 * np)), g0h, g0bh;
+      std::vector<int4> rb((size_t)np);
+      EMO_HIP_CHECK_RET(c, hipMemcpy(hbase.data(), s0.base, nc1 * 4, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, hipMemcpy(hcnt.data(), s0.cnt, nc1 * 4, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, hipMemcpy(hdoff.data(), s0.doff, nc1 * 4, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, hipMemcpy(rb.data(), s0.rb, (size_t)np * 16, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, hipMemcpy(adj.data(), L.adj, (size_t)np * 24, hipMemcpyDeviceToHost));
+      if (L.has_g0) {
+        g0h.resize((size_t)v.nslots);
+        g0bh.assign((size_t)np, 0.f);
+        EMO_HIP_CHECK_RET(c, hipMemcpy(g0h.data(), L.g0slots, (size_t)v.nslots * 4, hipMemcpyDeviceToHost));
+        if (nsteps > 0) EMO_HIP_CHECK_RET(c, hipMemcpy(g0bh.data(), L.g0b, (size_t)np * 4, hipMemcpyDeviceToHost));
+      }
+      for (long long cell = 0; cell < g.ncell; cell++)
+        for (int j = 0; j < hcnt[cell]; j++) {
+          const int64_t r = (int64_t)hdoff[cell] + j, sl = (int64_t)hbase[cell] + j;
+          if (r >= np) break;
+          const int64_t k = rb[r].w;
+          if (k < 0 || k >= np) continue;
+          float u[3];
+          std::memcpy(u, &rb[r], 12);
+          const double gm = L.has_g0 ? (double)g0h[sl] : 1.0;
+          double gb = L.has_g0 ? (double)g0bh[k] : 0.0;
+          for (int d = 0; d < 3; d++) {
+            const double fb = adj[6 * k + d], ub = adj[6 * k + 3 + d];
+            if (xbar0) xbar0[3 * k + d] = fb * g.id[d];
+            if (vbar0) vbar0[3 * k + d] = ub * (rel ? gm : 1.0);
+            gb += ub * ((double)u[d] / gm);
+          }
+          if (gbar0) gbar0[k] = rel ? gb : 0.0;
+        }
+    }
+  }
+  // ---- the final state: re-run from the initial snapshot --------------------
+  if (nsteps > 0 && !rc) {
+    lean_reach(L, 0, nsteps);
+    if (!L.rc) L.rc = fp32_check(c);
+    rc = L.rc;
+  }
+  if (L.has_g0) {
+    if (nsteps == 0) {
+      // the state is still the initial one: its stored gamma stays with it
+      *v.g0 = L.g0slots;
+      *v.g0_active = true;
+      *v.g0_owned = true;
+    } else {
+      // past the first step the stored gamma is no longer part of the state
+      (void)hipStreamSynchronize(c->stream);
+      if (*v.g0 == L.g0slots) {
+        *v.g0 = nullptr;
+        *v.g0_active = false;
+        *v.g0_owned = true;
+      }
+      adj_free(L.g0slots);
+    }
+  }
+  c->adj_info[0] = 1;
+  c->adj_info[1] = S + 1;
+  c->adj_info[2] = L.nfwd;
+  c->adj_info[3] = L.nrestore;
+  return rc;
+}
 
 // The gradient call of the FP32 chosen form: forward with checkpoints every
 // K steps (the last segment taped on the way), then segment by segment
@@ -1269,7 +2013,9 @@ __global__ void k_zero_f(float *p, long long n) {
 // adjoint of the segment after it (adjoint stream).
 int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, int32_t K, double *energy,
                               double *d_amp, double *d_phase, double *xbar0, double *vbar0, double *gbar0) {
-  if (K <= 0) K = std::max(nsteps, 1);
+  if (K < 0)
+    return fp32_adjoint_binomial(c, nsteps, t0, dt, -K, energy, d_amp, d_phase, xbar0, vbar0, gbar0);
+  if (K == 0) K = std::max(nsteps, 1);
   const Geo &g = c->geo;
   const int64_t np = c->np;
   const int nw = c->cfg.nwaves;
@@ -1323,6 +2069,12 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
     Pool dry;
     layout(dry);
     if (dry.off > c->adj_pool_bytes) {
+      // segments and tape that do not fit in device memory: binomial
+      // checkpointing, with what fits
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess && dry.off + ((size_t)1 << 30) > fr + c->adj_pool_bytes)
+        return fp32_adjoint_binomial(c, nsteps, t0, dt, kMaxBinomialSnapshots, energy, d_amp, d_phase, xbar0, vbar0,
+                                     gbar0);
       if (c->adj_pool) hipFree(c->adj_pool);
       c->adj_pool = nullptr;
       c->adj_pool_bytes = 0;
@@ -1339,8 +2091,10 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
   }
   // ---- forward with checkpoints -------------------------------------------
   double t = t0;
+  // the times of the steps, accumulated as emo_step advances them
   std::vector<double> tstep(nsteps + 1);
-  for (int s = 0; s <= nsteps; s++) tstep[s] = t0 + s * dt;
+  tstep[0] = t0;
+  for (int s = 0; s < nsteps; s++) tstep[s + 1] = tstep[s] + dt;
   const int last_set = R.nseg > 0 ? (R.nseg - 1) % nsets : 0;
   for (int seg = 0; seg < R.nseg && !rc; seg++) {
     snap32_take(c, R.ck[seg], t, true);
@@ -1466,9 +2220,22 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
       if (d_amp) d_amp[w] = da[w];
       if (d_phase) d_phase[w] = dp[w];
     }
+    {
+      SumsArgs32 sa{};
+      sa.ra = R.ck[0].pa;
+      sa.rb = R.ck[0].pb;
+      sa.rbase = R.ck[0].base;
+      sa.sbase = R.ck[0].base;
+      sa.cnt = R.ck[0].cnt;
+      sa.g0 = has_g0 ? R.ck[0].g0 : nullptr;
+      sa.adj = nullptr;
+      sa.pa = R.pa;
+      sa.g0b = has_g0 ? R.g0b : nullptr;
+      rc = species_sums32(c, sa);
+    }
     // particle adjoints of the initial state, by id: the offsets f = (x -
     // x0) / dx - ix and, relativistic, u = gamma0 v with the stored gamma0
-    if ((xbar0 || vbar0 || gbar0) && np > 0) {
+    if (!rc && (xbar0 || vbar0 || gbar0) && np > 0) {
       // the initial state's slots: cell c holds cnt[c] particles from base[c]
       const size_t nc1 = (size_t)g.ncell + 1;
       std::vector<int32_t> hbase(nc1), hcnt(nc1);
@@ -1508,6 +2275,11 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
         }
     }
   }
+  // checkpointing of this call: K-step segments, each re-run once
+  c->adj_info[0] = 0;
+  c->adj_info[1] = R.nseg;
+  c->adj_info[2] = (int64_t)nsteps + (int64_t)std::max(R.nseg - 1, 0) * R.K;
+  c->adj_info[3] = R.nseg;
   // ---- the final state back (the records stayed in the own buffers) --------
   hipStreamSynchronize(R.as);
   if (R.nseg > 0) restore_own_buffers(R);
@@ -1741,8 +2513,17 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
       if (d_amp) d_amp[w] = da[w];
       if (d_phase) d_phase[w] = dp[w];
     }
+    if (np > 0) {
+      SumsArgs64 sa{};
+      sa.pb = R.pb;
+      sa.sp = R.ck[0].sp;
+      sa.id = R.ck[0].id;
+      rc = species_sums64(c, sa);
+    } else {
+      std::fill(c->adj_sums, c->adj_sums + 7 * EMO_NSPEC, 0.0);
+    }
     // particle adjoints of the initial state, by id
-    if ((xbar0 || vbar0 || gbar0) && np > 0) {
+    if (!rc && (xbar0 || vbar0 || gbar0) && np > 0) {
       std::vector<int64_t> ids(np);
       std::vector<double> h(np);
       EMO_HIP_CHECK_RET(c, hipMemcpy(ids.data(), R.ck[0].id, np * 8, hipMemcpyDeviceToHost));
@@ -1761,6 +2542,11 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
       if (gbar0 && !rc) rc = put(gbar0, R.pb.gam, 1, 0);
     }
   }
+  // checkpointing of this call: K-step segments, each re-run once
+  c->adj_info[0] = 0;
+  c->adj_info[1] = R.nseg;
+  c->adj_info[2] = (int64_t)nsteps + (int64_t)std::max(R.nseg - 1, 0) * R.K;
+  c->adj_info[3] = R.nseg;
   // ---- restore the final state (the buffers stay with the context) --------
   hipStreamSynchronize(R.as);
   if (nsteps > 0 && fin_taken) {
@@ -1776,6 +2562,19 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   }
   for (hipEvent_t e : ev) hipEventDestroy(e);
   return rc;
+}
+
+extern "C" int emo_adjoint_stats(emo_ctx *c, int64_t *info4, double *sums) {
+  if (!c) return EMO_ERR_CONFIG;
+  if (c->adj_info[0] < 0) {
+    c->err = "no gradient call has run on this context";
+    return EMO_ERR_CONFIG;
+  }
+  if (info4)
+    for (int k = 0; k < 4; k++) info4[k] = c->adj_info[k];
+  if (sums)
+    for (int k = 0; k < 7 * std::max(1, (int)c->cfg.nspecies) && k < 7 * EMO_NSPEC; k++) sums[k] = c->adj_sums[k];
+  return EMO_OK;
 }
 
 extern "C" int emo_loader_dx_deps(const emo_loader *ld, const emo_config *cfg, double *dx_deps) {

@@ -1879,6 +1879,8 @@ int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
   return fp32_check(c);
 }
 
+int fp32_scan_i32(emo_ctx *c, const int32_t *in, int32_t *out, long long n) { return scan_i32(c, in, out, n); }
+
 int fp32_layout_info(emo_ctx *c, int64_t *out3) {
   ChosenState *s = c->ch;
   int32_t used = 0, lay[2] = {0, 0};

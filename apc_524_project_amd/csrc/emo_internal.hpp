@@ -127,6 +127,11 @@ struct emo_ctx {
   void *adj_pool = nullptr;
   size_t adj_pool_bytes = 0;
   hipStream_t adj_stream = nullptr;  // the adjoint's backward sweep
+  // the last gradient call (emo_adjoint_stats): {checkpointing (0: K-step
+  // segments, 1: binomial), snapshots, forward steps run, restores}, and the
+  // per-species sums of its initial-state gradients (7 per species)
+  int64_t adj_info[4] = {-1, 0, 0, 0};
+  double adj_sums[7 * EMO_NSPEC] = {};
 
   // transverse Gaussian profiles of an injected pulse (emo_config.waist > 0)
   // per plane normal and field id 0..5 (E, B), built on the host at
@@ -314,6 +319,8 @@ struct Fp32Tape {
 int fp32_step_one(emo_ctx *c, double t, double dt, const Fp32Tape *tape);
 // the device error flags of the FP32 form (stream synchronisation)
 int fp32_check(emo_ctx *c);
+// exclusive prefix sum of n int32 values on the context stream (n <= ncell + 1)
+int fp32_scan_i32(emo_ctx *c, const int32_t *in, int32_t *out, long long n);
 int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, int32_t K, double *energy,
                               double *d_amp, double *d_phase, double *xbar0, double *vbar0, double *gbar0);
 struct UniformGen;  // uniform_load.hpp

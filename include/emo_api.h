@@ -213,10 +213,28 @@ int emo_diag(emo_ctx *ctx, double *out8);
  *   d_phase[nw]   dF/d phases[w], per radian
  *   xbar0[n][3], vbar0[n][3], gbar0[n]   dF/d the initial positions,
  *                 velocities and stored gamma, in particle-id order
- * The context is left at the final state. */
+ * The context is left at the final state.
+ * Checkpointing: checkpoint_every = K > 0 keeps a checkpoint every K steps
+ * and tapes one K-step segment at a time (each segment re-run once); K = 0
+ * is one segment.  FP32 chosen form: K < 0 asks for binomial checkpointing
+ * with at most -K snapshots besides the initial state (particle adjoints by
+ * id, no tape: the backward step of step s runs on the state at s, reached
+ * by a restore and a re-run), which the call also takes by itself, with as
+ * many snapshots as fit, when the segments do not fit in free device memory.
+ * Both give the same bits; emo_adjoint_stats reports which ran. */
 int emo_adjoint_field_energy(emo_ctx *ctx, int32_t nsteps, double t0, double dt, int32_t checkpoint_every,
                              double *energy, double *d_amp, double *d_phase, double *xbar0, double *vbar0,
                              double *gbar0);
+/* The last gradient call on ctx (no counterpart in the reference):
+ *   info4 = {checkpointing (0: K-step segments, 1: binomial), snapshots
+ *            held (with the initial state), forward steps run (the first
+ *            pass and every re-run), snapshot restores}
+ *   sums[7 * nspecies] = per species the sums over its particles of
+ *            xbar0[3], vbar0[3] and gbar0 (the by-id outputs), reduced on
+ *            the device, so that runs too large for per-particle outputs
+ *            still report ensemble gradients (a beam's dF/d drift).
+ * Either pointer may be NULL.  EMO_ERR_CONFIG before any gradient call. */
+int emo_adjoint_stats(emo_ctx *ctx, int64_t *info4, double *sums);
 /* d x0 / d pert_eps[species] of emo_load_maxwellian's displacement loader for
  * every particle (load = id order; 0 for unperturbed species), n doubles.
  * No counterpart in the reference (the loader extension is this library's). */
