@@ -479,6 +479,9 @@ def run_adjoint_c5s(steps: int, k: int):
                     "gradient_over_forward": round(t_grad / t_fwd, 3),
                     "energy_matches_forward": bool(g["energy"] == f_fwd), "energy": g["energy"],
                     "dF_dvx_beam1": float(g["vbar0"][sp == 0, 0].sum())}
+        out[tag]["dF_dvx_beam1_over_F"] = out[tag]["dF_dvx_beam1"] / g["energy"]
+        out[tag]["energy_per_cell"] = g["energy"] / w.ncells
+        out[tag]["dF_dvx_beam1_per_cell"] = out[tag]["dF_dvx_beam1"] / w.ncells
         print(f"[bench] adjoint c5s {tag}: forward {t_fwd:.3f}s, gradient {t_grad:.3f}s, adjoint/forward "
               f"{(t_grad - t_fwd) / t_fwd:.2f}, dF/dvx(beam 1) {out[tag]['dF_dvx_beam1']:.6e}", file=sys.stderr,
               flush=True)
@@ -492,6 +495,64 @@ def run_adjoint_c5s(steps: int, k: int):
                               np.linalg.norm(grads["fp64"]["vbar0"])),
     }
     print(f"[bench] adjoint c5s fp32 vs fp64: {out['fp32_vs_fp64']}", file=sys.stderr, flush=True)
+    return out
+
+
+def run_adjoint_c5(steps: int, k: int):
+    """Workload c5 at full size (1.07e9 particles) in the FP32 chosen form:
+    `steps` forward steps timed alone, then the gradient call from the same
+    initial state (reloaded) with checkpoint_every = k.  The K-step segments
+    and their tape do not fit beside the state in device memory, so the call
+    takes binomial checkpointing with the snapshots that fit; the schedule it
+    ran (snapshots, forward steps, restores) is reported with the times.  The
+    time of the call includes allocating its snapshot pool (the first call on
+    the context).  Reported gradient: dF/d(a uniform shift of beam 1's v_x),
+    the sum of its particles' x-velocity adjoints (stored gamma held fixed),
+    reduced on the device (emo_adjoint_stats); (1/F) dF/dv is intensive and
+    is set beside c5s's (the same physics on 1/64 of the volume)."""
+    import apc_524_project_amd as emo
+    from apc_524_project_amd import FP32_CHOSEN
+
+    w = emo.make_workload("c5", precision=FP32_CHOSEN)
+    ctx = emo.Context(w.cfg)
+
+    def reset():
+        ctx.load_uniform(w.loader)
+        init_fields(ctx, w)
+        ctx.synchronize()
+
+    reset()
+    ctx.step(2, 0.0, w.dt)  # first launches
+    reset()
+    a = time.perf_counter()
+    ctx.step(steps, 0.0, w.dt)
+    ctx.synchronize()
+    t_fwd = time.perf_counter() - a
+    d = ctx.diag()
+    f_fwd = d[4] + d[5]
+    print(f"[bench] adjoint c5: forward {steps} steps {t_fwd:.2f}s; gradient call running", file=sys.stderr,
+          flush=True)
+    reset()
+    a = time.perf_counter()
+    g = ctx.adjoint_field_energy(steps, 0.0, w.dt, k, particles=False)
+    ctx.synchronize()
+    t_grad = time.perf_counter() - a
+    st = ctx.adjoint_stats()
+    ctx.close()
+    dvx = float(st["sums"][0, 3])
+    out = {"workload": f"c5: 512x128x128 periodic, two electron beams +-0.2c (sigma 0.01c), 64 ppc per beam "
+                       f"({w.nparticles} particles), FP32 chosen form, {steps} steps",
+           "forward_s": round(t_fwd, 3), "gradient_s": round(t_grad, 3),
+           "gradient_over_forward": round(t_grad / t_fwd, 3),
+           "adjoint_over_forward": round((t_grad - t_fwd) / t_fwd, 3),
+           "checkpointing": st["checkpointing"], "snapshots": st["snapshots"],
+           "forward_steps_run": st["forward_steps"], "restores": st["restores"],
+           "energy": g["energy"], "energy_matches_forward": bool(g["energy"] == f_fwd),
+           "dF_dvx_beam1": dvx, "dF_dvx_beam1_over_F": dvx / g["energy"],
+           "energy_per_cell": g["energy"] / w.ncells, "dF_dvx_beam1_per_cell": dvx / w.ncells}
+    print(f"[bench] adjoint c5: gradient {t_grad:.2f}s ({out['gradient_over_forward']}x forward), "
+          f"{st['checkpointing']} with {st['snapshots']} snapshots, {st['forward_steps']} forward steps, "
+          f"{st['restores']} restores; dF/dvx(beam 1) {dvx:.6e}", file=sys.stderr, flush=True)
     return out
 
 
@@ -777,6 +838,9 @@ def main():
     ap.add_argument("--c5s-adjoint-steps", type=int, default=200,
                     help="steps of the c5s FP32-vs-FP64 adjoint leg (0: skip it)")
     ap.add_argument("--c5s-adjoint-only", action="store_true", help="only the c5s adjoint leg")
+    ap.add_argument("--c5-adjoint-steps", type=int, default=200,
+                    help="steps of the full-size c5 FP32 adjoint leg (0: skip it)")
+    ap.add_argument("--c5-adjoint-only", action="store_true", help="only the c5 adjoint leg")
     ap.add_argument("--cpu-cells", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps per timed run of the CPU baseline (3 runs)")
     args = ap.parse_args()
@@ -802,6 +866,10 @@ def main():
         return
     if args.c5s_adjoint_only:
         print(json.dumps({"adjoint_c5s": run_adjoint_c5s(args.c5s_adjoint_steps or 200, args.checkpoint_every)}),
+              flush=True)
+        return
+    if args.c5_adjoint_only:
+        print(json.dumps({"adjoint_c5": run_adjoint_c5(args.c5_adjoint_steps or 200, args.checkpoint_every)}),
               flush=True)
         return
 
@@ -876,6 +944,16 @@ def main():
                                                  out["adjoint"])
             if args.c5s_adjoint_steps > 0:
                 out["adjoint_c5s"] = run_adjoint_c5s(args.c5s_adjoint_steps, args.checkpoint_every)
+            if args.c5_adjoint_steps > 0:
+                a5 = run_adjoint_c5(args.c5_adjoint_steps, args.checkpoint_every)
+                if "adjoint_c5s" in out and args.c5s_adjoint_steps == args.c5_adjoint_steps:
+                    # per cell (and per unit of F) the figures are intensive: c5 beside
+                    # c5s's FP64 and FP32 adjoints (c5's longer box holds longer modes,
+                    # so they agree to the growth those modes add, not exactly)
+                    for tag in ("fp64", "fp32"):
+                        for key in ("dF_dvx_beam1_per_cell", "energy_per_cell", "dF_dvx_beam1_over_F"):
+                            a5[f"c5s_{tag}_{key}"] = out["adjoint_c5s"][tag][key]
+                out["adjoint_c5"] = a5
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_cells, args.cpu_steps)
