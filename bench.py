@@ -596,10 +596,10 @@ def run_accuracy(name, checkpoints, control=False, scale: float = 1.0):
     checkpoint.
 
     With control, the same errors are given for an FP64 plain run started
-    from the FP32 form's initial particles (positions and velocities as the
-    FP32 store holds them): it isolates how fast the loop itself amplifies a
+    from the FP32 form's initial state (its particles and fields as the FP32
+    store holds them): it isolates how fast the loop itself amplifies a
     perturbation of FP32 representation size, independent of FP32
-    arithmetic."""
+    arithmetic during the steps."""
     import numpy as np
 
     import apc_524_project_amd as emo
@@ -615,11 +615,16 @@ def run_accuracy(name, checkpoints, control=False, scale: float = 1.0):
         poisson[form] = init_fields(c, w)
         ctxs[form] = c
     if control:
+        # the FP32 form's initial state (particles and fields, as the FP32
+        # store holds them) stepped in FP64
         p32 = ctxs["fp32"].particles(fields=False)
         w64 = emo.make_workload(name, precision=FP64_PLAIN, scale=scale)
         c = emo.Context(w64.cfg)
         c.load_particles(p32["x"], p32["v"], p32["gamma"], p32["species"], p32["id"])
-        init_fields(c, w64)
+        c.init_constant()
+        for fid in (L_.EX, L_.EY, L_.EZ, L_.BX, L_.BY, L_.BZ, L_.BX_TM1, L_.BY_TM1, L_.BZ_TM1):
+            c.set_field(fid, ctxs["fp32"].field(fid))
+        c.interpolate()
         ctxs["fp64_from_fp32_state"] = c
         del p32
     n = w.cfg.nCell
@@ -678,6 +683,20 @@ def run_accuracy(name, checkpoints, control=False, scale: float = 1.0):
             "positions_rel_l2": float(np.linalg.norm(dx) / np.linalg.norm(parts["fp64"][0])),
             "velocities_rel_l2": rel(parts[form][1], parts["fp64"][1]),
         }
+    if "fp64_from_fp32_state" in rows:
+        # FP32 error / control error per checkpoint: about 1 where the loop's
+        # amplification of the initial rounding explains the FP32 error,
+        # above 1 where FP32 arithmetic adds its own
+        ratios = []
+        for r32, rc in zip(rows["fp32"], rows["fp64_from_fp32_state"]):
+            row = {"steps": r32["steps"]}
+            for k in ("E", "B", "J", "field_energy", "ex_mode_k1"):
+                row[k] = (r32[k] / rc[k]) if (r32[k] is not None and rc[k]) else None
+            ratios.append(row)
+        res["fp32_over_control"] = ratios
+        print(f"[bench] accuracy {name} fp32/control: " +
+              "; ".join(f"{r['steps']}: E {r['E']:.3g} J {r['J']:.3g}" for r in ratios
+                        if r["E"] is not None and r["J"] is not None), file=sys.stderr, flush=True)
     return res
 
 
@@ -841,6 +860,7 @@ def main():
     ap.add_argument("--c5-adjoint-steps", type=int, default=200,
                     help="steps of the full-size c5 FP32 adjoint leg (0: skip it)")
     ap.add_argument("--c5-adjoint-only", action="store_true", help="only the c5 adjoint leg")
+    ap.add_argument("--accuracy-only", action="store_true", help="only the FP32-vs-FP64 error leg")
     ap.add_argument("--cpu-cells", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps per timed run of the CPU baseline (3 runs)")
     args = ap.parse_args()
@@ -867,6 +887,11 @@ def main():
     if args.c5s_adjoint_only:
         print(json.dumps({"adjoint_c5s": run_adjoint_c5s(args.c5s_adjoint_steps or 200, args.checkpoint_every)}),
               flush=True)
+        return
+    if args.accuracy_only:
+        names = list(ACCURACY_CHECKPOINTS) if args.accuracy in ("all", "none") else args.accuracy.split(",")
+        print(json.dumps({"fp32_accuracy": {nm: run_accuracy(nm, ACCURACY_CHECKPOINTS[nm], control=True,
+                                                             scale=args.scale) for nm in names}}), flush=True)
         return
     if args.c5_adjoint_only:
         print(json.dumps({"adjoint_c5": run_adjoint_c5(args.c5_adjoint_steps or 200, args.checkpoint_every)}),
@@ -931,7 +956,7 @@ def main():
         out["fp32_over_fp64_per_phase"] = round(results["fp32"]["rate"] / rate_pp, 3)
     if args.accuracy != "none" and "fp32" in results:
         names = list(ACCURACY_CHECKPOINTS) if args.accuracy == "all" else args.accuracy.split(",")
-        out["fp32_accuracy"] = {nm: run_accuracy(nm, ACCURACY_CHECKPOINTS[nm], control=(nm == "c2"), scale=args.scale)
+        out["fp32_accuracy"] = {nm: run_accuracy(nm, ACCURACY_CHECKPOINTS[nm], control=True, scale=args.scale)
                                 for nm in names}
     if args.c5_steps > 0 and "fp32" in results:
         out["c5"] = run_c5(args.c5_steps, 2)
