@@ -115,14 +115,22 @@ void refresh_fields(emo_ctx *c, FieldSet<T> &fs, double t, double dt) {
 
 template <typename T>
 void potential_bc(emo_ctx *c, T *u, const double *bound) {
+  // runs of consecutive periodic sides in one launch (their sources are
+  // physical points, their destinations ghost points); an external side's
+  // fill keeps its place in the side order (it writes physical plane 1 on
+  // the left, which later copies read)
+  T *f[1] = {u};
+  int ns = 0, ds[6], srcs[6], dsts[6];
   for (int side = 0; side < 6; side++) {
     int d = side / 2, right = side & 1;
     if (c->cfg.fields_bound[side] == EMO_BC_PERIODIC) {
-      int src = right ? 1 : c->geo.n[d];
-      int dst = right ? c->geo.n[d] + 1 : 0;
-      T *f[1] = {u};
-      fk_plane_copy<T>(c, f, 1, d, src, dst, 0);
+      ds[ns] = d;
+      srcs[ns] = right ? 1 : c->geo.n[d];
+      dsts[ns] = right ? c->geo.n[d] + 1 : 0;
+      ns++;
     } else {
+      fk_plane_copy_sides<T>(c, f, 1, ns, ds, srcs, dsts);
+      ns = 0;
       PlaneFill pf;
       pf.f = u;
       pf.dim = d;
@@ -133,6 +141,7 @@ void potential_bc(emo_ctx *c, T *u, const double *bound) {
       fk_plane_fill<T>(c, pf);
     }
   }
+  fk_plane_copy_sides<T>(c, f, 1, ns, ds, srcs, dsts);
 }
 
 // The transverse profile exp(-(ya^2 + yb^2) / waist^2) of the wave part on

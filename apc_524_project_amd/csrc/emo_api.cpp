@@ -711,7 +711,7 @@ int init_poisson_f64(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
   const double *srcs[4] = {c->rho, c->J[0], c->J[1], c->J[2]};
   double resid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int32_t sweeps[4] = {0, 0, 0, 0};
-  double hpart[2 * kMaxResidualBlocks];
+  double hpart[2];
   for (int s = 0; s < 4 && !rc; s++) {
     double *u1 = u[2 * s], *u2 = u[2 * s + 1];
     double tl = s == 0 ? tphi : tA;
@@ -722,22 +722,18 @@ int init_poisson_f64(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_o
       iternum++;
       const double *src = (iternum % 2 == 0) ? u1 : u2;
       double *dst = (iternum % 2 == 0) ? u2 : u1;
-      launch_jacobi_f64(c, src, dst, srcs[s], ax, ay, az, af, sourceMult);
+      // the sweep with its residual (max|du|, sum du^2 over the physical
+      // points, which the ghost refresh does not touch)
+      launch_jacobi_resid_f64(c, src, dst, srcs[s], ax, ay, az, af, sourceMult, partials);
       potential_bc<double>(c, dst, bounds[s]);
-      launch_residual_f64(c, u1, u2, partials, nb);
-      // per-block max|du| and sum du^2, combined on the host in block order
-      hipError_t e = hipMemcpyAsync(hpart, partials, 2 * nb * sizeof(double),
+      hipError_t e = hipMemcpyAsync(hpart, partials + 2 * kMaxResidualBlocks, 2 * sizeof(double),
                                     hipMemcpyDeviceToHost, c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       if (e != hipSuccess) {
         rc = set_err(c, EMO_ERR_HIP, std::string("poisson residual: ") + hipGetErrorString(e));
         break;
       }
-      double maxDiff = 0.0, l2 = 0.0;
-      for (int b = 0; b < nb; b++) {
-        maxDiff = std::max(maxDiff, hpart[2 * b]);
-        l2 += hpart[2 * b + 1];
-      }
+      const double maxDiff = hpart[0], l2 = hpart[1];
       resid[2 * s] = maxDiff;
       resid[2 * s + 1] = std::sqrt(l2);
       if (maxDiff < tl) converged = true;

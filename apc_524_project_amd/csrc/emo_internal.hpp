@@ -204,6 +204,8 @@ void launch_plane_fill_f64(emo_ctx *c, const PlaneFill &pf);
 void launch_const_fill_f64(emo_ctx *c, double *f, double v, long long n);
 void launch_jacobi_f64(emo_ctx *c, const double *src, double *dst, const double *R, double ax,
                        double ay, double az, double af, double sourceMult);
+void launch_jacobi_resid_f64(emo_ctx *c, const double *src, double *dst, const double *R, double ax, double ay,
+                             double az, double af, double sourceMult, double *partials);
 void launch_residual_f64(emo_ctx *c, const double *u1, const double *u2, double *partials,
                          int nblocks);
 int residual_blocks(emo_ctx *c);
@@ -243,6 +245,11 @@ void fk_jacobi(emo_ctx *c, const T *src, T *dst, const T *R, double ax, double a
                double sourceMult);
 template <typename T>
 void fk_residual(emo_ctx *c, const T *u1, const T *u2, double *partials, int nblocks);
+// one Jacobi sweep with its residual: max|dst - src| and sum (dst - src)^2
+// over the physical points land in partials[2 * kMaxResidualBlocks + 0, 1]
+template <typename T>
+void fk_jacobi_resid(emo_ctx *c, const T *src, T *dst, const T *R, double ax, double ay, double az, double af,
+                     double sourceMult, double *partials);
 template <typename T>
 void fk_phi_to_e(emo_ctx *c, const T *phi, T *const *E);
 template <typename T>
@@ -287,7 +294,7 @@ struct PoissonWs {
   double *partials = nullptr;  // 2 * kMaxResidualBlocks + 2 doubles
   float *cr = nullptr;         // FP32 form: 8 per cell
 };
-constexpr int kMaxResidualBlocks = 1024;
+constexpr int kMaxResidualBlocks = 4096;
 size_t poisson_ws_bytes(const emo_ctx *c);
 void poisson_ws_carve(const emo_ctx *c, void *base, PoissonWs &ws);
 int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps, double *resid,

@@ -308,6 +308,131 @@ __global__ void __launch_bounds__(kBlock) k_jacobi(JacobiArgs<T> a) {
              a.az * (u[c - 1] + u[c + 1]) - a.af * a.R[c] * a.sourceMult;
 }
 
+// The sweep fused with its residual, tiled for reuse: a block holds a
+// kJacTK x kJacTJ (z, y) tile and marches over `ni` x planes; each thread
+// keeps its column's x neighbours in registers and the tile of the current
+// plane with its y/z halo in LDS, so each value of the iterate comes from
+// HBM about once.  The arithmetic is k_jacobi's, in the same order.  Per
+// block: max|dst - src| and sum (dst - src)^2 over its points in FP64
+// (partials[2b], [2b + 1]), the residual of the sweep (the ghost refresh
+// after it touches no physical point).
+constexpr int kJacTK = 32, kJacTJ = kBlock / kJacTK;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_jacobi_tiled(JacobiArgs<T> a, int ni, int nb, double *partials) {
+  __shared__ T L[kJacTJ + 2][kJacTK + 2];
+  __shared__ double smax[kBlock], ssum[kBlock];
+  const Geo &g = a.g;
+  const int tk = threadIdx.x % kJacTK, tj = threadIdx.x / kJacTK;
+  const int tilesK = (g.n[2] + kJacTK - 1) / kJacTK, tilesJ = (g.n[1] + kJacTJ - 1) / kJacTJ;
+  // the grid is a multiple of the 8 XCDs, which take blocks round-robin:
+  // XCD x runs the x-th eighth of the tiles (whole runs of neighbouring
+  // tiles and x chunks), so halo and chunk-edge planes are hits in its L2
+  const int per = (int)gridDim.x / 8;
+  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (t >= nb) return;
+  int b = t;
+  const int bK = b % tilesK;
+  b /= tilesK;
+  const int bJ = b % tilesJ;
+  const int chunk = b / tilesJ;
+  const int k0 = 1 + bK * kJacTK, j0 = 1 + bJ * kJacTJ;
+  const int kB = min(k0 + kJacTK - 1, g.n[2]), jB = min(j0 + kJacTJ - 1, g.n[1]);
+  const int k = k0 + tk, j = j0 + tj;
+  const bool in = k <= kB && j <= jB;
+  const int i0 = 1 + chunk * ni, i1 = min(g.n[0], i0 + ni - 1);
+  const long long sx = g.sx, sy = g.sy;
+  // the halo values this thread loads for the tile (at most one per side)
+  const bool hN = in && j == j0, hS = in && j == jB, hW = in && k == k0, hE = in && k == kB;
+  long long c = gidx(g, i0, in ? j : j0, in ? k : k0);
+  const T *u = a.src;
+  // registers: plane i - 1 and i of this column, plane i's halo, plane i's
+  // R; the next plane's values load one iteration ahead
+  T up = in ? u[c - sx] : (T)0, uc = in ? u[c] : (T)0;
+  T n0 = hN ? u[c - sy] : (T)0, s0 = hS ? u[c + sy] : (T)0, w0 = hW ? u[c - 1] : (T)0, e0 = hE ? u[c + 1] : (T)0;
+  T rr = in ? a.R[c] : (T)0;
+  T un = in ? u[c + sx] : (T)0;
+  double mx = 0.0, sm = 0.0;
+  for (int i = i0; i <= i1; i++) {
+    // prefetch the next plane (its ghost plane at the chunk's end is valid)
+    const long long cn = c + sx;
+    T un2 = (T)0, n1 = (T)0, s1 = (T)0, w1 = (T)0, e1 = (T)0, rr1 = (T)0;
+    if (i < i1 && in) {
+      un2 = u[cn + sx];
+      rr1 = a.R[cn];
+      if (hN) n1 = u[cn - sy];
+      if (hS) s1 = u[cn + sy];
+      if (hW) w1 = u[cn - 1];
+      if (hE) e1 = u[cn + 1];
+    }
+    __syncthreads();  // the previous plane's LDS reads are done
+    if (in) {
+      L[tj + 1][tk + 1] = uc;
+      if (hN) L[0][tk + 1] = n0;
+      if (hS) L[tj + 2][tk + 1] = s0;
+      if (hW) L[tj + 1][0] = w0;
+      if (hE) L[tj + 1][tk + 2] = e0;
+    }
+    __syncthreads();
+    if (in) {
+      const T v = a.ax * (up + un) + a.ay * (L[tj][tk + 1] + L[tj + 2][tk + 1]) +
+                  a.az * (L[tj + 1][tk] + L[tj + 1][tk + 2]) - a.af * rr * a.sourceMult;
+      a.dst[c] = v;
+      const double dd = (double)v - (double)uc;
+      const double ad = fabs(dd);
+      mx = ad > mx ? ad : mx;
+      sm += dd * dd;
+    }
+    up = uc;
+    uc = un;
+    un = un2;
+    n0 = n1;
+    s0 = s1;
+    w0 = w1;
+    e0 = e1;
+    rr = rr1;
+    c = cn;
+  }
+  smax[threadIdx.x] = mx;
+  ssum[threadIdx.x] = sm;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      ssum[threadIdx.x] += ssum[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * t] = smax[0];
+    partials[2 * t + 1] = ssum[0];
+  }
+}
+
+// the block partials of a residual in block order: out[0] = max, out[1] = sum
+__global__ void __launch_bounds__(kBlock) k_reduce_resid(const double *partials, int nb, double *out) {
+  __shared__ double smax[kBlock], ssum[kBlock];
+  double mx = 0.0, sm = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+    mx = fmax(mx, partials[2 * b]);
+    sm += partials[2 * b + 1];
+  }
+  smax[threadIdx.x] = mx;
+  ssum[threadIdx.x] = sm;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      ssum[threadIdx.x] += ssum[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = smax[0];
+    out[1] = ssum[0];
+  }
+}
+
 // max|u2-u1| (:250-260) and sum (u2-u1)^2 per block in FP64 (for either
 // precision of u): partials[2*b], [2*b+1].  The max is exact in any order;
 // the L2 partials are reduced in a fixed order by launch_reduce_partials.
@@ -632,6 +757,47 @@ void fk_jacobi(emo_ctx *c, const T *src, T *dst, const T *R, double ax, double a
   hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
 }
 
+// One sweep src -> dst and its residual (max|dst - src|, sum (dst - src)^2
+// over the physical points) in partials[2 * kMaxResidualBlocks + 0, 1];
+// partials holds 2 * kMaxResidualBlocks + 2 doubles.
+template <typename T>
+void fk_jacobi_resid(emo_ctx *c, const T *src, T *dst, const T *R, double ax, double ay, double az, double af,
+                     double sourceMult, double *partials) {
+  const Geo &g = c->geo;
+  const long long tiles = (long long)((g.n[2] + kJacTK - 1) / kJacTK) * ((g.n[1] + kJacTJ - 1) / kJacTJ);
+  double *out = partials + 2 * kMaxResidualBlocks;
+  if (tiles <= kMaxResidualBlocks) {
+    JacobiArgs<T> a;
+    a.g = g;
+    a.src = src;
+    a.dst = dst;
+    a.R = R;
+    a.ax = (T)ax;
+    a.ay = (T)ay;
+    a.az = (T)az;
+    a.af = (T)af;
+    a.sourceMult = (T)sourceMult;
+    // x planes per block: at least 8, at most kMaxResidualBlocks blocks
+    const long long chunks_max = std::max<long long>(1, kMaxResidualBlocks / tiles);
+    const int ni = (int)std::max<long long>(8, (g.n[0] + chunks_max - 1) / chunks_max);
+    const int chunks = (g.n[0] + ni - 1) / ni;
+    const int nb = (int)(tiles * chunks);
+    {
+      TimedLaunch t(c, sizeof(T) == 8 ? "jacobi_f64" : "jacobi_f32");
+      hipLaunchKernelGGL(k_jacobi_tiled<T>, dim3((nb + 7) / 8 * 8), dim3(kBlock), 0, c->stream, a, ni, nb,
+                         partials);
+    }
+    hipLaunchKernelGGL(k_reduce_resid, dim3(1), dim3(kBlock), 0, c->stream, (const double *)partials, nb, out);
+    return;
+  }
+  // very wide transverse grids: the sweep, then the residual pass
+  fk_jacobi<T>(c, src, dst, R, ax, ay, az, af, sourceMult);
+  const int nb = residual_blocks(c);
+  hipLaunchKernelGGL(k_residual<T>, dim3(nb), dim3(kBlock), 0, c->stream, g, (const T *)src, (const T *)dst,
+                     partials);
+  hipLaunchKernelGGL(k_reduce_resid, dim3(1), dim3(kBlock), 0, c->stream, (const double *)partials, nb, out);
+}
+
 template <typename T>
 void fk_residual(emo_ctx *c, const T *u1, const T *u2, double *partials, int nblocks) {
   hipLaunchKernelGGL(k_residual<T>, dim3(nblocks), dim3(kBlock), 0, c->stream, c->geo, u1, u2, partials);
@@ -684,6 +850,8 @@ void fk_diag_fields(emo_ctx *c, const FieldSet<T> &fs, double *partials, int nbl
   template void fk_diag_fields<T>(emo_ctx *, const FieldSet<T> &, double *, int);                \
   template void fk_jacobi<T>(emo_ctx *, const T *, T *, const T *, double, double, double, double, double); \
   template void fk_residual<T>(emo_ctx *, const T *, const T *, double *, int);                  \
+  template void fk_jacobi_resid<T>(emo_ctx *, const T *, T *, const T *, double, double, double, double, double, \
+                                   double *);                                                    \
   template void fk_phi_to_e<T>(emo_ctx *, const T *, T *const *);                                \
   template void fk_a_to_b<T>(emo_ctx *, const T *, const T *, const T *, T *const *);            \
   template void fk_subtract_mean<T>(emo_ctx *, T *, double *, int, double *);
@@ -732,6 +900,11 @@ void launch_const_fill_f64(emo_ctx *c, double *f, double v, long long n) {
 void launch_jacobi_f64(emo_ctx *c, const double *src, double *dst, const double *R, double ax,
                        double ay, double az, double af, double sourceMult) {
   fk_jacobi<double>(c, src, dst, R, ax, ay, az, af, sourceMult);
+}
+
+void launch_jacobi_resid_f64(emo_ctx *c, const double *src, double *dst, const double *R, double ax, double ay,
+                             double az, double af, double sourceMult, double *partials) {
+  fk_jacobi_resid<double>(c, src, dst, R, ax, ay, az, af, sourceMult, partials);
 }
 
 int residual_blocks(emo_ctx *c) {

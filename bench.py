@@ -747,6 +747,35 @@ def run_c5(steps: int, warmup: int):
             "diag_initial": dict(zip(keys, map(float, d0))), "diag_final": dict(zip(keys, map(float, d1)))}
 
 
+def run_fields_c5(yee_calls: int = 50):
+    """The field kernels at c5 size (512x128x128): the FP32 Poisson
+    initialisation of the c5 load (Jacobi sweeps timed with events around
+    their launches) and the Yee update alone in both precisions."""
+    import apc_524_project_amd as emo
+    from apc_524_project_amd import FP32_CHOSEN
+
+    w = emo.make_workload("c5", precision=FP32_CHOSEN)
+    ctx = emo.Context(w.cfg)
+    ctx.load_uniform(w.loader)
+    ctx.synchronize()
+    ctx.timing(True)
+    ctx.timing_select("jacobi_f32")
+    ctx.timing_reset()
+    a = time.perf_counter()
+    pois = init_fields(ctx, w)
+    ctx.synchronize()
+    t_pois = time.perf_counter() - a
+    jac_ms, jac_n = ctx.kernel_time("jacobi_f32")
+    ctx.timing(False)
+    ctx.timing_select(None)
+    ctx.close()
+    out = {"poisson_s": round(t_pois, 3), "poisson_init": pois, "jacobi_sweep": jacobi_roof(jac_ms, jac_n, w.ncells),
+           "yee_fp32": run_yee("fp32", "c5", yee_calls), "yee_fp64": run_yee("fp64", "c5", yee_calls)}
+    print(f"[bench] fields c5: jacobi {out['jacobi_sweep']}, yee fp32 {out['yee_fp32']}, yee fp64 {out['yee_fp64']}",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def host_cpu():
     """The host CPU's model name, family/model/stepping and clock from
     /proc/cpuinfo, and the logical CPUs this process may use."""
@@ -861,6 +890,7 @@ def main():
                     help="steps of the full-size c5 FP32 adjoint leg (0: skip it)")
     ap.add_argument("--c5-adjoint-only", action="store_true", help="only the c5 adjoint leg")
     ap.add_argument("--accuracy-only", action="store_true", help="only the FP32-vs-FP64 error leg")
+    ap.add_argument("--fields-only", action="store_true", help="only the c5 field kernels (Jacobi, Yee)")
     ap.add_argument("--cpu-cells", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps per timed run of the CPU baseline (3 runs)")
     args = ap.parse_args()
@@ -887,6 +917,9 @@ def main():
     if args.c5s_adjoint_only:
         print(json.dumps({"adjoint_c5s": run_adjoint_c5s(args.c5s_adjoint_steps or 200, args.checkpoint_every)}),
               flush=True)
+        return
+    if args.fields_only:
+        print(json.dumps({"fields_c5": run_fields_c5()}), flush=True)
         return
     if args.accuracy_only:
         names = list(ACCURACY_CHECKPOINTS) if args.accuracy in ("all", "none") else args.accuracy.split(",")
