@@ -194,6 +194,16 @@ __global__ void __launch_bounds__(kBlock) k_plane_fill_multi(FillMultiArgs a) {
 // kernels: the B kernel must see every new E, including the stale ghost
 // planes that are only refreshed afterwards.
 // ---------------------------------------------------------------------------
+// The Yee grids are multiples of the 8 XCDs, which take blocks round-robin:
+// XCD x runs the x-th eighth of the points (a contiguous run of rows and
+// planes), so the neighbouring rows and planes a stencil reads are hits in
+// its own L2.
+__device__ inline long long yee_block() {
+  const unsigned per = gridDim.x / 8;
+  return (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+inline unsigned yee_grid(long long n) { return (unsigned)((grid_for(n) + 7) / 8 * 8); }
+
 template <typename T>
 struct YeeArgs {
   Geo g;
@@ -207,7 +217,7 @@ struct YeeArgs {
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_evolve_E(YeeArgs<T> a) {
   const Geo &g = a.g;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long t = (long long)yee_block() * blockDim.x + threadIdx.x;
   if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
   int i, j, k;
   phys_ijk(g, t, &i, &j, &k);
@@ -227,7 +237,7 @@ __global__ void __launch_bounds__(kBlock) k_evolve_E(YeeArgs<T> a) {
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_evolve_B(YeeArgs<T> a) {
   const Geo &g = a.g;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long t = (long long)yee_block() * blockDim.x + threadIdx.x;
   if (t >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
   int i, j, k;
   phys_ijk(g, t, &i, &j, &k);
@@ -733,10 +743,10 @@ void fk_evolve(emo_ctx *c, FieldSet<T> &fs, double dt) {
   const bool f64 = sizeof(T) == 8;
   {
     TimedLaunch t(c, f64 ? "yee_E_f64" : "yee_E_f32");
-    hipLaunchKernelGGL(k_evolve_E<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_evolve_E<T>, dim3(yee_grid(nphys)), dim3(kBlock), 0, c->stream, a);
   }
   TimedLaunch t(c, f64 ? "yee_B_f64" : "yee_B_f32");
-  hipLaunchKernelGGL(k_evolve_B<T>, dim3(grid_for(nphys)), dim3(kBlock), 0, c->stream, a);
+  hipLaunchKernelGGL(k_evolve_B<T>, dim3(yee_grid(nphys)), dim3(kBlock), 0, c->stream, a);
 }
 
 template <typename T>
