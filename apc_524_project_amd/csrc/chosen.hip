@@ -12,8 +12,8 @@
 // Slot layout: cell c owns the slots [base[c], base[c+1]) -- its count plus
 // headroom, each range starting on a 128-byte boundary -- and holds its
 // cnt[c] particles in [base[c], base[c] + cnt[c]), in ascending cell order
-// overall.  The layout is rebuilt only when a cell's arrivals exceed its
-// headroom.
+// overall.  The layout is rebuilt on the device, within the step, only when
+// a cell's arrivals exceed its headroom.
 //
 // The algorithm is the reference's (same step order, same defects); only the
 // representation and precision change.  With x = x0 + dx (ix - 1 + f) the
@@ -29,26 +29,32 @@
 //                     cell (~93% at 64 particles per cell) is written to D at
 //                     the cell's next free slot (in order) and its 12 edge
 //                     currents are summed with compensated (two-float)
-//                     accumulation, combined over the lanes by a fixed
-//                     butterfly.  A particle that leaves writes its new
-//                     record over its own slot of S, its velocity beside it,
-//                     and its slot to the movers' list; it counts itself in
-//                     its destination's arrivals.
+//                     accumulation, combined over the lanes in lane order.  A
+//                     particle that leaves is staged in S at the front of its
+//                     source range (slots this pass has already read): its
+//                     new record and its 1/gamma.  It takes an arrival number
+//                     in its destination cell with an integer atomic and
+//                     writes its staging slot as an arrival key at the end of
+//                     the destination's key range (the order of the keys is
+//                     provisional; their set is not).
 //   check_f32         does every cell's stayers + arrivals fit its range?
-//   rebase_f32,       only when one does not: a new layout from the new
-//   push_deposit_redo counts, the stayers written again at their new slots
-//   place_f32         each mover reserves a slot in its destination's
-//                     arrival range (integer atomics: provisional order)
-//   arrive_f32        per cell, the arrivals in ascending order of their old
-//                     slot (a canonical order, independent of the atomics):
-//                     records to D, their currents added to the cell's sum
-//   commit_f32        the rebuilt layout becomes current
+//   rebase_f32,       only when one does not (early exits otherwise): a new
+//   restage_f32,      layout from the new counts, the stayers and the staged
+//   relayout_f32      movers copied to it, the arrival keys re-placed
+//   arrive_f32        4 lanes per cell: the cell's arrivals in ascending order
+//                     of their staging slot (a canonical order, independent
+//                     of the atomics; ranked in LDS for up to 32 arrivals,
+//                     in global memory beyond), records to D after the
+//                     stayers, their currents (v = u * 1/gamma, the push's own
+//                     product) added in that order
+//   commit_f32        the new counts (and a rebuilt layout) become current
 //   jsum_f32          per grid point, the fixed-order sum of the cell vectors
 //   fold, Yee E/B, ghost refresh   shared FP32 instances of the grid kernels
 // Within a cell the new order is the stayers in their previous order, then
-// the arrivals in their previous order: deterministic, with no atomics on
-// floating-point data.  Particle traffic is 32 B read and 32 B written per
-// particle, plus about 90 B per mover.
+// the arrivals by the slot they were staged in (source cells in ascending
+// order, each in its previous order): deterministic, independent of the
+// headroom, with no atomics on floating-point data.  Particle traffic is
+// 32 B read and 32 B written per particle, plus about 40 B per mover.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
