@@ -28,15 +28,17 @@
 //                     every particle, once.  A particle that stays in its
 //                     cell (~93% at 64 particles per cell) is written to D at
 //                     the cell's next free slot (in order) and its 12 edge
-//                     currents are summed with compensated (two-float)
-//                     accumulation, combined over the lanes in lane order.  A
+//                     currents (fp32 products) are summed in FP64, combined
+//                     over the lanes in lane order.  A
 //                     particle that leaves is staged in S at the front of its
 //                     source range (slots this pass has already read): its
-//                     new record and its 1/gamma.  It takes an arrival number
-//                     in its destination cell with an integer atomic and
-//                     writes its staging slot as an arrival key at the end of
-//                     the destination's key range (the order of the keys is
-//                     provisional; their set is not).
+//                     new record and its 1/gamma, and is listed in LDS with
+//                     its destination.  At the block's end each group of
+//                     listed movers with one source cell and destination
+//                     takes its arrival numbers there with one integer atomic,
+//                     and each mover writes its staging slot as an arrival
+//                     key at the end of the destination's key range (the
+//                     order of the keys is provisional; their set is not).
 //   check_f32         does every cell's stayers + arrivals fit its range?
 //   rebase_f32,       only when one does not (early exits otherwise): a new
 //   restage_f32,      layout from the new counts, the stayers and the staged
@@ -197,45 +199,12 @@ __global__ void __launch_bounds__(kBlock) k_cellvec_f32(CellVecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Compensated deposit helpers
+// Deposit helpers
 // ---------------------------------------------------------------------------
-__device__ inline void two_sum(float a, float b, float &s, float &e) {
-  s = a + b;
-  float bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
-
-// One level of the deposit butterfly: the partner's (hi, lo) by a DPP move
-// (CTRL), added with TwoSum.  Levels, within each group of 16 lanes:
-// row_mirror pairs lane i with 15 - i, row_half_mirror lane i with 7 - i
-// within each 8, quad_perm [2,3,0,1] and [1,0,3,2] the quad halves and
-// neighbours.  Each level pairs lanes whose partial sums cover disjoint
-// lane sets, so after the last level every lane of the group holds the sum
-// of all of them, combined in a fixed order.
-template <int CTRL>
-__device__ inline void dep_level(float hi[12], float lo[12]) {
-#pragma unroll
-  for (int q = 0; q < 12; q++) {
-    const float oh = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(hi[q]), CTRL, 0xF, 0xF, false));
-    const float ol = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(lo[q]), CTRL, 0xF, 0xF, false));
-    float s, err;
-    two_sum(hi[q], oh, s, err);
-    hi[q] = s;
-    lo[q] = lo[q] + ol + err;
-  }
-}
-
-__device__ inline void dep_reduce16(float hi[12], float lo[12]) {
-  dep_level<0x140>(hi, lo);
-  dep_level<0x141>(hi, lo);
-  dep_level<0x4E>(hi, lo);
-  dep_level<0xB1>(hi, lo);
-}
-
 // the 12 edge currents of one particle (Depositor::deposit_particle_J,
-// src/particles/deposit.cpp:12-40, weights 2 - f and f - 1), accumulated
-// with TwoSum into (hi, lo)
-__device__ inline void deposit_j(float hi[12], float lo[12], const float f[3], const float v[3], float jc) {
+// src/particles/deposit.cpp:12-40, weights 2 - f and f - 1) as fp32
+// products: 4 per component, the x edges first
+__device__ inline void edge_terms(const float f[3], const float v[3], float jc, float w[12]) {
   const float A[3] = {2.f - f[0], 2.f - f[1], 2.f - f[2]};
   const float Bw[3] = {f[0] - 1.f, f[1] - 1.f, f[2] - 1.f};
 #pragma unroll
@@ -243,14 +212,34 @@ __device__ inline void deposit_j(float hi[12], float lo[12], const float f[3], c
     const int j = (q + 1) % 3, k = (q + 2) % 3;
     const float cq = jc * v[q];
     const float ca = cq * A[j], cb = cq * Bw[j];
-    const float w4[4] = {ca * A[k], cb * A[k], cb * Bw[k], ca * Bw[k]};
+    w[4 * q + 0] = ca * A[k];
+    w[4 * q + 1] = cb * A[k];
+    w[4 * q + 2] = cb * Bw[k];
+    w[4 * q + 3] = ca * Bw[k];
+  }
+}
+
+// the same, accumulated in FP64: each fp32 term is exact in a double, and a
+// lane's few terms per cell sum without loss (v_cvt_f64_f32 and v_add_f64
+// issue at the full vector rate: two instructions a term, where a
+// compensated fp32 sum takes six)
+__device__ inline void deposit_j64(double acc[12], const float f[3], const float v[3], float jc) {
+  float w[12];
+  edge_terms(f, v, jc, w);
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      float s, err;
-      two_sum(hi[4 * q + r], w4[r], s, err);
-      hi[4 * q + r] = s;
-      lo[4 * q + r] += err;
-    }
+  for (int q = 0; q < 12; q++) acc[q] += (double)w[q];
+}
+
+// one level of a fixed-order butterfly on FP64 values within quads (DPP
+// quad_perm CTRL on the two halves)
+template <int CTRL>
+__device__ inline void dep_level64(double acc[12]) {
+#pragma unroll
+  for (int q = 0; q < 12; q++) {
+    const long long bits = __double_as_longlong(acc[q]);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+    acc[q] += __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
   }
 }
 
@@ -261,11 +250,12 @@ __device__ inline void deposit_j(float hi[12], float lo[12], const float f[3], c
 // cell's next free slots.  Movers go, in order, to the front of the cell's
 // own range of the staging buffers (the input buffer itself, whose slots up
 // to the current particle have been read, or, with a tape, separate
-// arrays), with their velocity beside them; each counts itself among its
-// destination's arrivals and, while the destination's range has room, writes
-// its staging slot to the top of that range in the arrival keys (the order
-// the atomics give is provisional; arrive_f32 puts each range in a
-// canonical order).
+// arrays), with their 1/gamma beside them, and each is listed in LDS under
+// its (source cell, destination) group.  At the end the block files the
+// groups (one global atomic each), so that no round of the particle loop
+// waits for an atomic's return; a mover that finds no room is filed at
+// once.  The cell's gather vector is read from LDS every round rather than
+// held in registers, and the lanes' currents are summed in FP64.
 // ---------------------------------------------------------------------------
 struct PushDepArgs {
   PushK k;
@@ -286,18 +276,54 @@ struct PushDepArgs {
   unsigned long long *flag;
 };
 
-constexpr int kRedVals = 4;  // (hi, lo) pairs per lane per LDS round of the cell sums
+// A staged mover filed under its destination d: its arrival number there
+// (an integer atomic) and, while d's range has room, its staging slot as an
+// arrival key at the top of that range (else the check rebuilds the layout).
+__device__ inline void file_mover(int32_t *arr, int32_t *akey, const int32_t *base, int d, int st) {
+  const int k = atomicAdd(&arr[d], 1);
+  const int e0 = base[d], e1 = base[d + 1];
+  if (k < e1 - e0) akey[e1 - 1 - k] = st;
+}
+
+// The movers of a block are listed in LDS and filed at its end, grouped by
+// (source cell, destination): one atomic per group reserves the group's
+// arrival numbers, and each mover takes its rank within the group.  A
+// drifting beam sends all of a cell's movers to one neighbour; filed one by
+// one, their atomics would hit one address from many lanes of one
+// instruction, which serialises them.
+constexpr int kMoveList = 384;                        // list entries per block
+constexpr int kWaveList = kMoveList / (kBlock / 64);  // per wavefront
+constexpr int kGroupSlots = 8;                        // destinations per source cell
+
+// The cell sums of the particle pass, 4 edges a round: each lane's FP64 edge
+// sums in LDS, lane-major, a group of 16 lanes (one cell) padded so that the
+// 16 reading lanes (4 cells x 4 edges) fall on different banks.
+constexpr int kRedQ = 4;                             // edges per round
+constexpr int kRedGroup = kCellLanes * kRedQ + 4;    // doubles per cell group
+constexpr int kRedWave = 4 * kRedGroup;              // doubles per wavefront
+
 
 template <bool REL, bool PER>
 __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
   __shared__ float qp_s[EMO_NSPEC], jc_s[EMO_NSPEC];
-  __shared__ float2 red[kBlock * kRedVals];
+  __shared__ double red[(kBlock / 64) * kRedWave];
+  __shared__ float4 cvs[kTileCells][kCellVec];  // the cells' gather vectors
+  __shared__ int2 mlist[kMoveList];  // (staging slot, group << 16 | rank), per wavefront
+  // (source cell, destination) groups: destination, movers, first arrival
+  // number, destination range
+  __shared__ int gkey[kTileCells * kGroupSlots], gcnt[kTileCells * kGroupSlots];
+  __shared__ int gk0[kTileCells * kGroupSlots], ge0[kTileCells * kGroupSlots], ge1[kTileCells * kGroupSlots];
+  __shared__ int wcnt[kBlock / 64];
   // species constants in LDS (a per-lane load from the argument block would
   // make every iteration wait for all loads in flight); test particles
   // deposit nothing (src/particles/particle_handler.cpp:205)
   if (threadIdx.x < EMO_NSPEC) {
     qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
     jc_s[threadIdx.x] = a.k.s.test[threadIdx.x] ? 0.f : a.k.s.jc[threadIdx.x];
+  }
+  if (threadIdx.x < kTileCells * kGroupSlots) {
+    gkey[threadIdx.x] = -1;
+    gcnt[threadIdx.x] = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.lay[0] = 0;
   __syncthreads();
@@ -316,18 +342,27 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
   int nmax = n;
   nmax = max(nmax, __shfl_xor(nmax, 16, 64));
   nmax = max(nmax, __shfl_xor(nmax, 32, 64));
-  CellVec V{};
-  if (n > 0) V = load_cellvec(a.k, (int)c);
-  float hi[12], lo[12];
+  // the cell's gather vector in LDS, read every round rather than held in
+  // 20 registers (fewer registers, more wavefronts in flight)
+  if (sub < kCellVec) cvs[threadIdx.x >> 4][sub] = n > 0 ? a.k.cv[c * kCellVec + sub] : make_float4(0.f, 0.f, 0.f, 0.f);
+  wave_sync();
+  const float4 *cvl = cvs[threadIdx.x >> 4];
+  double acc[12];
 #pragma unroll
-  for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
+  for (int q = 0; q < 12; q++) acc[q] = 0.0;
   int nstay = 0, nmove = 0;  // the same in all lanes of the cell
+  int wl = 0;                // the wavefront's listed movers
+  const unsigned long long below64 = (1ull << lane) - 1ull;
+  int2 *wlist = mlist + (threadIdx.x >> 6) * kWaveList;
   // loads past the cell's end read its last particle (a slot of the cell's
   // own range even when it is empty), so that they are unconditional
   const int last = b + max(n - 1, 0);
   int4 A = a.pa[min(b + sub, last)], Bv = a.pb[min(b + sub, last)];
   float G = use_g0 ? a.g0[min(b + sub, last)] : 0.f;
   for (int j0 = 0; j0 < nmax; j0 += kCellLanes) {
+    // (keeps the gather vector's LDS reads in the loop)
+    asm volatile("" ::: "memory");
+    const CellVec V{cvl[0], cvl[1], cvl[2], cvl[3], cvl[4]};
     const int jn = min(b + j0 + kCellLanes + sub, last);
     const int4 An = a.pa[jn], Bn = a.pb[jn];
     const float Gn = use_g0 ? a.g0[jn] : 0.f;
@@ -349,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
       a.pa_out[dst] = Ao;
       a.pb_out[dst] = Bo;
       if (a.fwd) a.fwd[src] = (uint32_t)dst;
-      deposit_j(hi, lo, o.f, o.v, jc_s[sp]);
+      deposit_j64(acc, o.f, o.v, jc_s[sp]);
     }
     if (move) {
       // (the staging slot is at most this particle's own: already read)
@@ -358,9 +393,33 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
       a.mb[st] = Bo;
       a.mv[st] = o.ig;
       if (a.mvsrc) a.mvsrc[st] = src;
-      const int k = atomicAdd(&a.arr[o.cell], 1);
-      const int e1 = a.base[o.cell + 1];
-      if (k < e1 - a.base[o.cell]) a.akey[e1 - 1 - k] = st;  // (else the layout is rebuilt)
+    }
+    {
+      // the mover's group: its destination's slot in the cell's table (a
+      // mover that finds no free slot, or no room in the wavefront's list,
+      // is filed at once)
+      int gs = -1;
+      if (move) {
+        int *keys = gkey + (threadIdx.x >> 4) * kGroupSlots;
+        const int h = (o.cell * 5) & (kGroupSlots - 1);
+        for (int t = 0; t < kGroupSlots; t++) {
+          const int sl = (h + t) & (kGroupSlots - 1);
+          const int old = atomicCAS(&keys[sl], -1, o.cell);
+          if (old == -1 || old == o.cell) {
+            gs = (threadIdx.x >> 4) * kGroupSlots + sl;
+            break;
+          }
+        }
+      }
+      const unsigned long long lm = __ballot(gs >= 0);
+      const int pos = wl + __popcll(lm & below64);
+      wl += __popcll(lm);
+      if (gs >= 0 && pos < kWaveList) {
+        const int rank = atomicAdd(&gcnt[gs], 1);
+        wlist[pos] = make_int2(b + nmove + __popc(mm & below), (gs << 16) | rank);
+      } else if (move) {
+        file_mover(a.arr, a.akey, a.base, o.cell, b + nmove + __popc(mm & below));
+      }
     }
     nstay += __popc(gm);
     nmove += __popc(mm);
@@ -368,31 +427,49 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
     Bv = Bn;
     G = Gn;
   }
-  // the cell's sum over its 16 lanes in lane order, with TwoSum: the lanes'
-  // pairs through LDS kRedVals edges at a time, then one lane per (cell, edge)
-  float2 *rw = red + (threadIdx.x & ~63) * kRedVals;
+  // the cell's sum over its 16 lanes in lane order, in FP64: the lanes' sums
+  // through LDS 4 edges at a time, then one lane per (cell, edge)
+  double *rw = red + (threadIdx.x >> 6) * kRedWave;
 #pragma unroll
-  for (int q0 = 0; q0 < 12; q0 += kRedVals) {
+  for (int q0 = 0; q0 < 12; q0 += kRedQ) {
+    double *mine = rw + (lane >> 4) * kRedGroup + (lane & 15) * kRedQ;
 #pragma unroll
-    for (int q = 0; q < kRedVals; q++) rw[lane * kRedVals + q] = make_float2(hi[q0 + q], lo[q0 + q]);
+    for (int q = 0; q < kRedQ; q++) mine[q] = acc[q0 + q];
     wave_sync();
-    if (lane < 4 * kRedVals) {
-      const int grp = lane / kRedVals, q = lane - kRedVals * grp;
+    if (lane < 4 * kRedQ) {
+      const int grp = lane / kRedQ, q = lane - kRedQ * grp;
       const long long cc = tile * kTileCells + (threadIdx.x >> 6) * 4 + grp;
-      float2 acc = rw[(grp * kCellLanes) * kRedVals + q];
-      for (int l = 1; l < kCellLanes; l++) {
-        const float2 p = rw[(grp * kCellLanes + l) * kRedVals + q];
-        float s, e;
-        two_sum(acc.x, p.x, s, e);
-        acc = make_float2(s, acc.y + p.y + e);
-      }
-      if (cc < a.ncell) a.cj[(q0 + q) * a.ncell + cc] = acc.x + acc.y;
+      const double *col = rw + grp * kRedGroup + q;
+      double sum = col[0];
+#pragma unroll
+      for (int l = 1; l < kCellLanes; l++) sum += col[kRedQ * l];
+      if (cc < a.ncell) a.cj[(q0 + q) * a.ncell + cc] = (float)sum;
     }
     wave_sync();
   }
   if (sub == 0 && in) {
     a.ns[c] = nstay;
     a.nm[c] = nmove;
+  }
+  // the groups filed: one atomic each, for all its movers; then every listed
+  // mover's key, at its group's first arrival number plus its rank (the
+  // order the ranges' keys take is provisional: arrive_f32 sorts them)
+  if (lane == 0) wcnt[threadIdx.x >> 6] = min(wl, kWaveList);
+  __syncthreads();
+  if (threadIdx.x < kTileCells * kGroupSlots) {
+    const int g = threadIdx.x, d = gkey[g], m = gcnt[g];
+    if (d >= 0 && m > 0) {
+      gk0[g] = atomicAdd(&a.arr[d], m);
+      ge0[g] = a.base[d];
+      ge1[g] = a.base[d + 1];
+    }
+  }
+  __syncthreads();
+  const int nl = wcnt[threadIdx.x >> 6];
+  for (int i = lane; i < nl; i += 64) {
+    const int2 e = wlist[i];
+    const int g = e.y >> 16, k = gk0[g] + (e.y & 0xffff);
+    if (k < ge1[g] - ge0[g]) a.akey[ge1[g] - 1 - k] = e.x;  // (else the layout is rebuilt)
   }
 }
 
@@ -544,7 +621,7 @@ __global__ void __launch_bounds__(kBlock) k_relayout_f32(RelayoutArgs a) {
 // order) -- ranked in LDS for up to kArrFast arrivals, else ranked from global
 // memory into akey2 -- the rank-r arrival taken by lane r % 4: its record
 // written after the stayers, its currents summed as in the particle pass
-// (lane order, TwoSum) and added to the stayers' sum.  Then the cell's new
+// (FP64, lane order) and added to the stayers' sum.  Then the cell's new
 // count; the per-step counters are reset for the next step.
 constexpr int kArrLanes = 4;
 constexpr int kArrCells = kBlock / kArrLanes;
@@ -605,9 +682,9 @@ __global__ void __launch_bounds__(kBlock) k_arrive_f32(ArriveArgs a) {
   }
   // (a work-group barrier orders the block's global stores before its loads)
   __syncthreads();
-  float hi[12], lo[12];
+  double acc[12];
 #pragma unroll
-  for (int q = 0; q < 12; q++) hi[q] = lo[q] = 0.f;
+  for (int q = 0; q < 12; q++) acc[q] = 0.0;
   auto take = [&](int st, int r) {
     const int4 A = a.ma[st], Bv = a.mb[st];
     const float ig = a.mv[st];
@@ -618,25 +695,21 @@ __global__ void __launch_bounds__(kBlock) k_arrive_f32(ArriveArgs a) {
     const float f[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
     // the push's v = u * (1/gamma), the same product of the same floats
     const float v[3] = {as_f(Bv.x) * ig, as_f(Bv.y) * ig, as_f(Bv.z) * ig};
-    deposit_j(hi, lo, f, v, jc_s[(uint32_t)A.x >> kSpeciesShift]);
+    deposit_j64(acc, f, v, jc_s[(uint32_t)A.x >> kSpeciesShift]);
   };
   if (fast) {
     for (int r = sub; r < na; r += kArrLanes) take(sorted[grp][r], r);
   } else {
     for (int r = sub; r < na; r += kArrLanes) take(a.akey2[k0 + r], r);
   }
-  // the 4 lanes' sums in a fixed order (quad permutations)
+  // the 4 lanes' sums in a fixed order (quad permutations), in FP64
   static_assert(kArrLanes == 4, "the reduction covers 4 lanes");
-  dep_level<0x4E>(hi, lo);
-  dep_level<0xB1>(hi, lo);
+  dep_level64<0x4E>(acc);
+  dep_level64<0xB1>(acc);
   if (sub == 0 && in) {
     if (na > 0) {
 #pragma unroll
-      for (int q = 0; q < 12; q++) {
-        float s, e;
-        two_sum(a.cj[q * a.ncell + c], hi[q], s, e);
-        a.cj[q * a.ncell + c] = s + (e + lo[q]);
-      }
+      for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = (float)((double)a.cj[q * a.ncell + c] + acc[q]);
     }
     a.cnt[c] = a.ns[c] + na;
     a.arr[c] = 0;
@@ -656,8 +729,8 @@ __global__ void k_commit_f32(int32_t *base, const int32_t *base_new, const int32
 // rho + J of the current state, for the Poisson initialisation
 // (src/main.cpp:160-166): the 12 edge currents and the 8 vertex charges of
 // Depositor::deposit_particle_J / _Rho (src/particles/deposit.cpp:12-62)
-// with the weights 2 - f and f - 1, 16 lanes per cell, compensated sums and a
-// fixed butterfly as in the step's deposit.
+// with the weights 2 - f and f - 1, 16 lanes per cell, FP64 sums and a
+// fixed butterfly.
 // ---------------------------------------------------------------------------
 struct RhoJArgs {
   const int4 *pa, *pb;
@@ -672,9 +745,9 @@ struct RhoJArgs {
 __global__ void __launch_bounds__(kBlock) k_deposit_rhoj_f32(RhoJArgs a) {
   const int sub = threadIdx.x & 15;
   const long long c = (long long)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4);
-  float hi[20], lo[20];
+  double acc[20];
 #pragma unroll
-  for (int q = 0; q < 20; q++) hi[q] = lo[q] = 0.f;
+  for (int q = 0; q < 20; q++) acc[q] = 0.0;
   if (c < a.ncell) {
     const int b = a.base[c], e = b + a.cnt[c];
     for (int j = b + sub; j < e; j += 16) {
@@ -687,7 +760,7 @@ __global__ void __launch_bounds__(kBlock) k_deposit_rhoj_f32(RhoJArgs a) {
       if (a.rel) ig = a.g0 ? __builtin_amdgcn_rcpf(a.g0[j]) : __builtin_amdgcn_rsqf(1.f + ux * ux + uy * uy + uz * uz);
       const float v[3] = {ux * ig, uy * ig, uz * ig};
       const float jc = a.s.jc[sp];
-      deposit_j(hi, lo, f, v, jc);
+      deposit_j64(acc, f, v, jc);
       const float Aw[3] = {2.f - f[0], 2.f - f[1], 2.f - f[2]};
       const float Bw[3] = {f[0] - 1.f, f[1] - 1.f, f[2] - 1.f};
       // vertex order of Depositor::deposit_particle_Rho
@@ -695,31 +768,20 @@ __global__ void __launch_bounds__(kBlock) k_deposit_rhoj_f32(RhoJArgs a) {
                            Aw[0] * Bw[1] * Aw[2], Aw[0] * Bw[1] * Bw[2], Bw[0] * Bw[1] * Bw[2],
                            Bw[0] * Aw[1] * Bw[2], Aw[0] * Aw[1] * Bw[2]};
 #pragma unroll
-      for (int r = 0; r < 8; r++) {
-        float s2, err;
-        two_sum(hi[12 + r], jc * w8[r], s2, err);
-        hi[12 + r] = s2;
-        lo[12 + r] += err;
-      }
+      for (int r = 0; r < 8; r++) acc[12 + r] += (double)(jc * w8[r]);
     }
   }
+  // the 16 lanes' sums in a fixed order (xor butterfly), in FP64
 #pragma unroll
   for (int off = 8; off > 0; off >>= 1) {
 #pragma unroll
-    for (int q = 0; q < 20; q++) {
-      float oh = __shfl_xor(hi[q], off, 64);
-      float ol = __shfl_xor(lo[q], off, 64);
-      float s2, err;
-      two_sum(hi[q], oh, s2, err);
-      hi[q] = s2;
-      lo[q] = lo[q] + ol + err;
-    }
+    for (int q = 0; q < 20; q++) acc[q] += __shfl_xor(acc[q], off, 64);
   }
   if (sub == 0 && c < a.ncell) {
 #pragma unroll
-    for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = hi[q] + lo[q];
+    for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = (float)acc[q];
 #pragma unroll
-    for (int q = 0; q < 8; q++) a.cr[q * a.ncell + c] = hi[12 + q] + lo[12 + q];
+    for (int q = 0; q < 8; q++) a.cr[q * a.ncell + c] = (float)acc[12 + q];
   }
 }
 
