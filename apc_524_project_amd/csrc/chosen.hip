@@ -101,6 +101,9 @@ struct ChosenState {
   // akey2: the sorted keys of crowded arrival ranges
   int32_t *akey = nullptr, *akey2 = nullptr;
   int32_t *lay = nullptr;
+  // where the particle pass's lanes without a particle send their stores
+  // (written, never read), so that every round issues the same stores
+  int4 *sink = nullptr;
   // per-cell dense offsets (readback in array order) and scan scratch
   int32_t *dcount = nullptr, *doff = nullptr;
   void *scan_tmp = nullptr;
@@ -155,6 +158,7 @@ inline long long slots_for(long long cap_particles, long long ncell) {
 }
 
 constexpr int kCellLanes = 16;                   // lanes per cell
+constexpr int kSinkVecs = 4;                     // the particle pass's sink: 2 records and a 1/gamma
 constexpr int kTileCells = kBlock / kCellLanes;  // cells per block
 
 // The gather vector of a cell (Grid::getFieldInterpolatorVec,
@@ -272,6 +276,7 @@ struct PushDepArgs {
   float *cj;                  // 12 per cell: the stayers' currents
   uint32_t *fwd;              // with a tape: source slot -> new slot, or null
   int32_t *lay;               // lay[0]: rebuild pending (cleared here)
+  int4 *sink;                 // kSinkVecs vectors, written (never read) by lanes without a particle
   long long ncell, ntiles;
   unsigned long long *flag;
 };
@@ -303,8 +308,14 @@ constexpr int kRedGroup = kCellLanes * kRedQ + 4;    // doubles per cell group
 constexpr int kRedWave = 4 * kRedGroup;              // doubles per wavefront
 
 
-template <bool REL, bool PER>
-__global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
+// G0: the first relativistic push after a load, which reads the stored
+// gamma; TAPE: a recomputed step writing the adjoint's map and mover
+// sources.  Without either, every round issues the same global memory
+// instructions whatever its particles do (stores of lanes without a
+// particle go to the sink), so that the compiler's wait for the next
+// round's records counts exactly and never waits for this round's stores.
+template <bool REL, bool PER, bool G0, bool TAPE>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_push_deposit_f32(PushDepArgs a) {
   __shared__ float qp_s[EMO_NSPEC], jc_s[EMO_NSPEC];
   __shared__ double red[(kBlock / 64) * kRedWave];
   __shared__ float4 cvs[kTileCells][kCellVec];  // the cells' gather vectors
@@ -314,6 +325,7 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
   __shared__ int gkey[kTileCells * kGroupSlots], gcnt[kTileCells * kGroupSlots];
   __shared__ int gk0[kTileCells * kGroupSlots], ge0[kTileCells * kGroupSlots], ge1[kTileCells * kGroupSlots];
   __shared__ int wcnt[kBlock / 64];
+  __shared__ int overflow;  // a mover found no group slot or list entry
   // species constants in LDS (a per-lane load from the argument block would
   // make every iteration wait for all loads in flight); test particles
   // deposit nothing (src/particles/particle_handler.cpp:205)
@@ -325,13 +337,13 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
     gkey[threadIdx.x] = -1;
     gcnt[threadIdx.x] = 0;
   }
+  if (threadIdx.x == 0) overflow = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.lay[0] = 0;
   __syncthreads();
   const int sub = threadIdx.x & (kCellLanes - 1);
   const int lane = threadIdx.x & 63;
   const int gsh = lane & ~(kCellLanes - 1);  // the cell's first lane in the wavefront
   const unsigned below = (1u << sub) - 1u;
-  const bool use_g0 = a.g0 != nullptr;
   const long long tile = xcd_tile();
   if (tile >= a.ntiles) return;
   const long long c = tile * kTileCells + (threadIdx.x >> 4);
@@ -352,26 +364,36 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
   for (int q = 0; q < 12; q++) acc[q] = 0.0;
   int nstay = 0, nmove = 0;  // the same in all lanes of the cell
   int wl = 0;                // the wavefront's listed movers
+  bool lost = false, outside = false;
   const unsigned long long below64 = (1ull << lane) - 1ull;
   int2 *wlist = mlist + (threadIdx.x >> 6) * kWaveList;
+  // (one sink address for all lanes: wavefront-uniform, so it stays in
+  // scalar registers)
+  int4 *const sink_a = a.sink, *const sink_b = a.sink + 1;
+  float *const sink_v = reinterpret_cast<float *>(a.sink + 2);
   // loads past the cell's end read its last particle (a slot of the cell's
   // own range even when it is empty), so that they are unconditional
   const int last = b + max(n - 1, 0);
   int4 A = a.pa[min(b + sub, last)], Bv = a.pb[min(b + sub, last)];
-  float G = use_g0 ? a.g0[min(b + sub, last)] : 0.f;
+  float G = G0 ? a.g0[min(b + sub, last)] : 0.f;
+  // the first records in before the loop: the compiler's waits in the loop
+  // then count only the loop's own loads (otherwise every round would wait
+  // as the first one must)
+  __builtin_amdgcn_s_waitcnt(0);
   for (int j0 = 0; j0 < nmax; j0 += kCellLanes) {
     // (keeps the gather vector's LDS reads in the loop)
     asm volatile("" ::: "memory");
     const CellVec V{cvl[0], cvl[1], cvl[2], cvl[3], cvl[4]};
     const int jn = min(b + j0 + kCellLanes + sub, last);
     const int4 An = a.pa[jn], Bn = a.pb[jn];
-    const float Gn = use_g0 ? a.g0[jn] : 0.f;
+    const float Gn = G0 ? a.g0[jn] : 0.f;
     const int src = b + j0 + sub;
     const bool valid = j0 + sub < n;
     const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
     Pushed o;
     push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
-                            qp_s[sp], use_g0, G, o, valid ? a.flag : nullptr);
+                            qp_s[sp], G0, G, o, nullptr);
+    outside = outside || (valid && o.outside);
     const bool stay = valid && o.cell == (int)c;
     const bool move = valid && !stay;
     const unsigned gm = (unsigned)(__ballot(stay) >> gsh) & 0xffffu;
@@ -379,25 +401,23 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
     const int4 Ao = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)o.cell), as_i(o.f[0]),
                               as_i(o.f[1]), as_i(o.f[2]));
     const int4 Bo = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
-    if (stay) {
-      const int dst = b + nstay + __popc(gm & below);
-      a.pa_out[dst] = Ao;
-      a.pb_out[dst] = Bo;
-      if (a.fwd) a.fwd[src] = (uint32_t)dst;
-      deposit_j64(acc, o.f, o.v, jc_s[sp]);
+    // a stayer to the output at the cell's next free slot, a mover to its
+    // staging slot (at most this particle's own: already read) with its
+    // 1/gamma; lanes without a particle to the sink
+    const int rs = stay ? b + nstay + __popc(gm & below) : b + nmove + __popc(mm & below);
+    int4 *const da = valid ? (stay ? a.pa_out : a.ma) + rs : sink_a;
+    int4 *const db = valid ? (stay ? a.pb_out : a.mb) + rs : sink_b;
+    float *const dv = move ? a.mv + rs : sink_v;
+    *da = Ao;
+    *db = Bo;
+    *dv = o.ig;
+    if (TAPE) {
+      if (stay) a.fwd[src] = (uint32_t)rs;
+      if (move) a.mvsrc[rs] = src;
     }
-    if (move) {
-      // (the staging slot is at most this particle's own: already read)
-      const int st = b + nmove + __popc(mm & below);
-      a.ma[st] = Ao;
-      a.mb[st] = Bo;
-      a.mv[st] = o.ig;
-      if (a.mvsrc) a.mvsrc[st] = src;
-    }
+    if (stay) deposit_j64(acc, o.f, o.v, jc_s[sp]);
     {
-      // the mover's group: its destination's slot in the cell's table (a
-      // mover that finds no free slot, or no room in the wavefront's list,
-      // is filed at once)
+      // the mover's group: its destination's slot in the cell's table
       int gs = -1;
       if (move) {
         int *keys = gkey + (threadIdx.x >> 4) * kGroupSlots;
@@ -416,9 +436,9 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
       wl += __popcll(lm);
       if (gs >= 0 && pos < kWaveList) {
         const int rank = atomicAdd(&gcnt[gs], 1);
-        wlist[pos] = make_int2(b + nmove + __popc(mm & below), (gs << 16) | rank);
-      } else if (move) {
-        file_mover(a.arr, a.akey, a.base, o.cell, b + nmove + __popc(mm & below));
+        wlist[pos] = make_int2(rs, (gs << 16) | rank);
+      } else {
+        lost = lost || move;
       }
     }
     nstay += __popc(gm);
@@ -427,6 +447,8 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
     Bv = Bn;
     G = Gn;
   }
+  if (__ballot(outside) != 0ull && lane == 0) atomicOr(a.flag, kFlagOutOfGrid);
+  if (__ballot(lost) != 0ull && lane == 0) overflow = 1;
   // the cell's sum over its 16 lanes in lane order, in FP64: the lanes' sums
   // through LDS 4 edges at a time, then one lane per (cell, edge)
   double *rw = red + (threadIdx.x >> 6) * kRedWave;
@@ -451,11 +473,19 @@ __global__ void __launch_bounds__(kBlock) k_push_deposit_f32(PushDepArgs a) {
     a.ns[c] = nstay;
     a.nm[c] = nmove;
   }
+  if (lane == 0) wcnt[threadIdx.x >> 6] = min(wl, kWaveList);
+  // (the barrier also orders the block's staging stores before the loads
+  // below)
+  __syncthreads();
+  if (overflow) {
+    // a mover found no room in its cell's group table or its wavefront's
+    // list: the block's movers filed one by one, from the staged records
+    for (int j = sub; j < nmove; j += kCellLanes) file_mover(a.arr, a.akey, a.base, a.ma[b + j].x & kCellMask, b + j);
+    return;
+  }
   // the groups filed: one atomic each, for all its movers; then every listed
   // mover's key, at its group's first arrival number plus its rank (the
   // order the ranges' keys take is provisional: arrive_f32 sorts them)
-  if (lane == 0) wcnt[threadIdx.x >> 6] = min(wl, kWaveList);
-  __syncthreads();
   if (threadIdx.x < kTileCells * kGroupSlots) {
     const int g = threadIdx.x, d = gkey[g], m = gcnt[g];
     if (d >= 0 && m > 0) {
@@ -1219,17 +1249,30 @@ void launch_cellvec(emo_ctx *c) {
   hipLaunchKernelGGL(k_cellvec_f32, dim3(grid_for(c->geo.ncell)), dim3(kBlock), 0, c->stream, a);
 }
 
+template <bool REL, bool PER>
+void launch_push_deposit_bc(emo_ctx *c, const PushDepArgs &a, dim3 grid) {
+  const bool g0 = a.g0 != nullptr, tape = a.fwd != nullptr;
+  if (g0 && tape)
+    hipLaunchKernelGGL((k_push_deposit_f32<REL, PER, true, true>), grid, dim3(kBlock), 0, c->stream, a);
+  else if (g0)
+    hipLaunchKernelGGL((k_push_deposit_f32<REL, PER, true, false>), grid, dim3(kBlock), 0, c->stream, a);
+  else if (tape)
+    hipLaunchKernelGGL((k_push_deposit_f32<REL, PER, false, true>), grid, dim3(kBlock), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_push_deposit_f32<REL, PER, false, false>), grid, dim3(kBlock), 0, c->stream, a);
+}
+
 void launch_push_deposit(emo_ctx *c, const PushDepArgs &a, dim3 grid) {
   bool all_per = true;
   for (int q = 0; q < 6; q++) all_per = all_per && c->cfg.parts_bound[q] == EMO_BC_PERIODIC;
   if (c->cfg.relativity && all_per)
-    hipLaunchKernelGGL((k_push_deposit_f32<true, true>), grid, dim3(kBlock), 0, c->stream, a);
+    launch_push_deposit_bc<true, true>(c, a, grid);
   else if (c->cfg.relativity)
-    hipLaunchKernelGGL((k_push_deposit_f32<true, false>), grid, dim3(kBlock), 0, c->stream, a);
+    launch_push_deposit_bc<true, false>(c, a, grid);
   else if (all_per)
-    hipLaunchKernelGGL((k_push_deposit_f32<false, true>), grid, dim3(kBlock), 0, c->stream, a);
+    launch_push_deposit_bc<false, true>(c, a, grid);
   else
-    hipLaunchKernelGGL((k_push_deposit_f32<false, false>), grid, dim3(kBlock), 0, c->stream, a);
+    launch_push_deposit_bc<false, false>(c, a, grid);
 }
 
 // One step's particle work, from the current record buffers into the
@@ -1264,6 +1307,7 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
   a.cj = s->cj;
   a.fwd = fwd;
   a.lay = s->lay;
+  a.sink = s->sink;
   a.ncell = g.ncell;
   a.ntiles = ntiles;
   a.flag = c->dflag;
@@ -1384,6 +1428,7 @@ int fp32_create(emo_ctx *c) {
   if (!rc) rc = dalloc(c, &s->dcount, nc1);
   if (!rc) rc = dalloc(c, &s->doff, nc1);
   if (!rc) rc = dalloc(c, &s->lay, 4);
+  if (!rc) rc = dalloc(c, &s->sink, kSinkVecs);
   if (!rc) rc = dalloc(c, &s->cj, 12 * g.ncell);
   if (!rc) rc = dalloc(c, &s->cv, (size_t)kCellVec * g.ncell);
   size_t nt = (size_t)g.ntot;
@@ -1425,7 +1470,7 @@ void fp32_destroy(emo_ctx *c) {
   ChosenState *s = c->ch;
   if (!s) return;
   void *ps[] = {s->pa, s->pa_alt, s->pb, s->pb_alt, s->mv, s->akey, s->akey2, s->base, s->base_new, s->cnt,
-                s->ns, s->nm, s->rpart, s->arr, s->put, s->dcount, s->doff, s->lay, s->key, s->key_alt, s->perm,
+                s->ns, s->nm, s->rpart, s->arr, s->put, s->dcount, s->doff, s->lay, s->sink, s->key, s->key_alt, s->perm,
                 s->perm_alt, s->sort_tmp, s->scan_tmp, s->cj, s->cv, s->gam_id, s->fs.rho};
   for (void *p : ps) dfree(p);
   if (s->g0_owned) dfree(s->g0);

@@ -70,6 +70,7 @@ struct Pushed {
   float u[3];   // new u (after the BCs)
   float v[3];   // new velocity (deposit): u[d] * ig
   float ig;     // 1 / gamma of the new u (1 non-relativistic)
+  bool outside;  // the new position left the padded grid (then clamped into it)
 };
 
 
@@ -225,7 +226,8 @@ __device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, 
     move_dim(fmaf(a.dtid[1], uy * ig_new, fy), iy, g.n[1], a.lper[1], a.rper[1], jy, o.f[1], uy, o.flip[1]);
     move_dim(fmaf(a.dtid[2], uz * ig_new, fz), iz, g.n[2], a.lper[2], a.rper[2], jz, o.f[2], uz, o.flip[2]);
   }
-  if (jx < 0 || jx >= g.nc[0] || jy < 0 || jy >= g.nc[1] || jz < 0 || jz >= g.nc[2]) {
+  o.outside = jx < 0 || jx >= g.nc[0] || jy < 0 || jy >= g.nc[1] || jz < 0 || jz >= g.nc[2];
+  if (o.outside) {
     if (flag) atomicOr(flag, kFlagOutOfGrid);
     jx = min(max(jx, 1), g.n[0]);
     jy = min(max(jy, 1), g.n[1]);
