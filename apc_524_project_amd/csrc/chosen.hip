@@ -27,9 +27,9 @@
 //   push_deposit_f32  16 lanes per cell: gather + push + particle BCs of
 //                     every particle, once.  A particle that stays in its
 //                     cell (~93% at 64 particles per cell) is written to D at
-//                     the cell's next free slot (in order) and its 12 edge
-//                     currents (fp32 products) are summed in FP64, combined
-//                     over the lanes in lane order.  A
+//                     the cell's next free slot (in order) and its current
+//                     is summed as 12 FP64 moments (deposit_moments),
+//                     combined over the lanes in lane order.  A
 //                     particle that leaves is staged in S at the front of its
 //                     source range (slots this pass has already read): its
 //                     new record and its 1/gamma, and is listed in LDS with
@@ -91,14 +91,15 @@ struct ChosenState {
   float *gam_id = nullptr;
   // slot layout (ncell + 1 bases, ncell counts) and its rebuild target
   int32_t *base = nullptr, *base_new = nullptr, *cnt = nullptr;
-  // per step: stayers, movers and arrivals per cell, placement counters,
-  // the movers' velocities by source slot; lay[0] = rebuild pending,
-  // lay[1] = rebuilds so far
-  int32_t *ns = nullptr, *nm = nullptr, *arr = nullptr, *put = nullptr;
+  // per step: stayers, movers, arrivals and the arrivals' direction mask
+  // per cell; lay[0] = rebuild pending, lay[1] = rebuilds so far
+  int32_t *ns = nullptr, *nm = nullptr, *arr = nullptr, *arrm = nullptr;
   long long *rpart = nullptr;  // the check's per-block sums of the new ranges (2 x nblk)
-  float *mv = nullptr;
-  // akey: arrival ranges' provisional staging slots (destination layout);
-  // akey2: the sorted keys of crowded arrival ranges
+  // movers further than one cell: (destination, staging slot), and their count
+  int2 *far = nullptr;
+  int32_t *nfar = nullptr;
+  // the arrival slots of crowded cells and their sorted order, at the top of
+  // each range
   int32_t *akey = nullptr, *akey2 = nullptr;
   int32_t *lay = nullptr;
   // where the particle pass's lanes without a particle send their stores
@@ -115,7 +116,8 @@ struct ChosenState {
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   int sort_bits = 0;
-  float *cj = nullptr;  // 12 per cell, component-major
+  float *cj = nullptr;  // 12 per cell, component-major with stride cstride (a multiple of 32)
+  long long cstride = 0;
   float4 *cv = nullptr;  // gather vectors, kCellVec per cell
   FieldSet<float> fs;
   float *Btfix[3] = {};
@@ -158,7 +160,8 @@ inline long long slots_for(long long cap_particles, long long ncell) {
 }
 
 constexpr int kCellLanes = 16;                   // lanes per cell
-constexpr int kSinkVecs = 4;                     // the particle pass's sink: 2 records and a 1/gamma
+constexpr int kSinkVecs = 2;                     // the particle pass's sink: 2 records
+constexpr long long kFarListCap = 1 << 20;       // movers further than one cell, per step
 constexpr int kTileCells = kBlock / kCellLanes;  // cells per block
 
 // The gather vector of a cell (Grid::getFieldInterpolatorVec,
@@ -234,6 +237,35 @@ __device__ inline void deposit_j64(double acc[12], const float f[3], const float
   for (int q = 0; q < 12; q++) acc[q] += (double)w[q];
 }
 
+// The step's deposit as moments.  With g = f - 1 the weights are A = 2 - f
+// = 1 - g and B = f - 1 = g, so the four edge currents of component q (j, k
+// the other two dimensions; edge order A_j A_k, B_j A_k, B_j B_k, A_j B_k)
+// are linear in four moments of c = jc v_q over the cell's particles:
+// sum c, sum c g_j, sum c g_k and sum c g_j g_k.  The moments are summed in
+// FP64: g = f - 1 is exact in a double, c is the fp32 product jc v_q, and
+// each term is rounded once (1e-16 relative), so a cell's edge currents,
+// formed from its moment sums by edges_from_moments, carry no rounding of the
+// fp32 weight products.  27 vector instructions a particle (FP64 fused
+// multiply-adds issue at the full vector rate).
+__device__ inline void deposit_moments(double m[12], const float f[3], const float v[3], float jc) {
+  const double g[3] = {(double)f[0] - 1.0, (double)f[1] - 1.0, (double)f[2] - 1.0};
+  const double gp[3] = {g[1] * g[2], g[2] * g[0], g[0] * g[1]};  // g_j g_k of component q
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int j = (q + 1) % 3, k = (q + 2) % 3;
+    const double cq = (double)(jc * v[q]);
+    m[4 * q + 0] += cq;
+    m[4 * q + 1] = fma(cq, g[j], m[4 * q + 1]);
+    m[4 * q + 2] = fma(cq, g[k], m[4 * q + 2]);
+    m[4 * q + 3] = fma(cq, gp[q], m[4 * q + 3]);
+  }
+}
+
+// edge e (0..3) of a component from its moments (M0, Mj, Mk, Mjk)
+__device__ inline double edge_from_moments(int e, double M0, double Mj, double Mk, double Mjk) {
+  return e == 0 ? ((M0 - Mj) - Mk) + Mjk : e == 1 ? Mj - Mjk : e == 2 ? Mjk : Mk - Mjk;
+}
+
 // one level of a fixed-order butterfly on FP64 values within quads (DPP
 // quad_perm CTRL on the two halves)
 template <int CTRL>
@@ -248,84 +280,89 @@ __device__ inline void dep_level64(double acc[12]) {
 }
 
 // ---------------------------------------------------------------------------
-// The step's particle pass.  16 lanes per cell, 16 cells per block; the
-// lanes take the cell's particles 16 at a time, in order, with the next 16
-// records in flight during a push.  Stayers go to the output buffer at the
-// cell's next free slots.  Movers go, in order, to the front of the cell's
-// own range of the staging buffers (the input buffer itself, whose slots up
-// to the current particle have been read, or, with a tape, separate
-// arrays), with their 1/gamma beside them, and each is listed in LDS under
-// its (source cell, destination) group.  At the end the block files the
-// groups (one global atomic each), so that no round of the particle loop
-// waits for an atomic's return; a mover that finds no room is filed at
-// once.  The cell's gather vector is read from LDS every round rather than
-// held in registers, and the lanes' currents are summed in FP64.
+// The step's particle pass.  16 lanes per cell, 16 cells per tile, two tiles
+// per block (32 cells); the lanes take a cell's particles 16 at a time, in
+// order, with the records of the next two rounds in flight during a push.
+// Stayers go to the output buffer at the cell's next free slots.  Movers go,
+// in order, to the front of the cell's own range of the staging buffers (the
+// input buffer itself, whose slots up to the current particle have been
+// read, or, with a tape, separate arrays).  Each mover adds one to its
+// destination's arrival count and sets the bit of its direction in the
+// destination's arrival mask -- atomics whose results are never read, so no
+// round waits for them; a mover that is not a neighbour of its source cell
+// (far) is also listed, with its staging slot, in a global list.  The
+// arrival pass finds a cell's arrivals in the staging ranges of the
+// neighbours its mask names.
+//
+// Every 128-byte line the pass writes is written whole, because a partial
+// line costs the memory a read of the line: the stayers' and the movers'
+// runs are padded with zeros to the next 8-slot boundary (slots the pass has
+// read, or headroom of the cell's own range), and the per-cell outputs (the
+// stayer and mover counts and the stayers' current vector) are gathered in
+// LDS and stored 32 cells at a time.  The cell's gather vector is read from
+// LDS every round rather than held in registers, and the lanes' currents are
+// summed as FP64 moments.
 // ---------------------------------------------------------------------------
+constexpr int kTilesPerBlock = 2;
+constexpr int kBlockCells = kTilesPerBlock * kTileCells;  // 32: a 128-byte line of 32-bit per-cell values
+// a cell's arrival mask: bit k = 9 (dx + 1) + 3 (dy + 1) + (dz + 1) for a
+// neighbour step (dx, dy, dz) (bit 13, no step, is never set), kFarBit for a
+// mover from further away
+constexpr int kFarBit = 27;
+
 struct PushDepArgs {
   PushK k;
   const int4 *pa, *pb;        // records before the step, source layout
   int4 *pa_out, *pb_out;      // records after the step (the stayers here)
   int4 *ma, *mb;              // staging: cell c's movers' new records from slot base[c] on
-  float *mv;                  // staging: their 1/gamma (v = u / gamma for the deposit)
   int32_t *mvsrc;             // with a tape: their source slots, or null
   const float *g0;            // the stored gamma of the first push, by source slot, or null
   const int32_t *base, *cnt;  // source layout
   int32_t *ns, *nm;           // stayers and movers per cell
-  int32_t *arr;               // arrivals per cell (counted here, zero on entry)
-  int32_t *akey;              // arrival keys: the staging slots arriving in each range, from its top
-  float *cj;                  // 12 per cell: the stayers' currents
+  int32_t *arr, *arrm;        // arrivals per cell and their direction mask (zero on entry)
+  int2 *far;                  // far movers: (destination, staging slot)
+  int32_t *nfar;              // far movers listed this step (zero on entry)
+  long long far_cap;
+  float *cj;                  // 12 per cell, component-major with stride cstride: the stayers' currents
+  long long cstride;
   uint32_t *fwd;              // with a tape: source slot -> new slot, or null
   int32_t *lay;               // lay[0]: rebuild pending (cleared here)
   int4 *sink;                 // kSinkVecs vectors, written (never read) by lanes without a particle
-  long long ncell, ntiles;
+  long long ncell, nblocks;   // cells; blocks of kBlockCells cells
   unsigned long long *flag;
 };
 
-// A staged mover filed under its destination d: its arrival number there
-// (an integer atomic) and, while d's range has room, its staging slot as an
-// arrival key at the top of that range (else the check rebuilds the layout).
-__device__ inline void file_mover(int32_t *arr, int32_t *akey, const int32_t *base, int d, int st) {
-  const int k = atomicAdd(&arr[d], 1);
-  const int e0 = base[d], e1 = base[d + 1];
-  if (k < e1 - e0) akey[e1 - 1 - k] = st;
+// A change d of a cell index in a dimension of n cells as a neighbour step:
+// -1, 0 or 1, with the periodic wrap (from index 1 to index n is a step of
+// -1) where the dimension's particle BCs are periodic; 2 for a larger move.
+__device__ inline int neighbour_step(int d, int n, bool per) {
+  if (per && n > 2) d = d == n - 1 ? -1 : (d == 1 - n ? 1 : d);
+  return (d >= -1 && d <= 1) ? d : 2;
 }
 
-// The movers of a block are listed in LDS and filed at its end, grouped by
-// (source cell, destination): one atomic per group reserves the group's
-// arrival numbers, and each mover takes its rank within the group.  A
-// drifting beam sends all of a cell's movers to one neighbour; filed one by
-// one, their atomics would hit one address from many lanes of one
-// instruction, which serialises them.
-constexpr int kMoveList = 384;                        // list entries per block
-constexpr int kWaveList = kMoveList / (kBlock / 64);  // per wavefront
-constexpr int kGroupSlots = 8;                        // destinations per source cell
-
-// The cell sums of the particle pass, 4 edges a round: each lane's FP64 edge
-// sums in LDS, lane-major, a group of 16 lanes (one cell) padded so that the
-// 16 reading lanes (4 cells x 4 edges) fall on different banks.
-constexpr int kRedQ = 4;                             // edges per round
+// The cell sums of the particle pass, one component (4 moments) a round:
+// each lane's FP64 moment sums in LDS, lane-major, a group of 16 lanes (one
+// cell) padded so that the 16 reading lanes (4 cells x 4 moments) fall on
+// different banks.
+constexpr int kRedQ = 4;                             // moments per round
 constexpr int kRedGroup = kCellLanes * kRedQ + 4;    // doubles per cell group
 constexpr int kRedWave = 4 * kRedGroup;              // doubles per wavefront
 
 
 // G0: the first relativistic push after a load, which reads the stored
 // gamma; TAPE: a recomputed step writing the adjoint's map and mover
-// sources.  Without either, every round issues the same global memory
-// instructions whatever its particles do (stores of lanes without a
-// particle go to the sink), so that the compiler's wait for the next
-// round's records counts exactly and never waits for this round's stores.
+// sources.  Without either, every round issues the same stores whatever its
+// particles do (stores of lanes without a particle go to the sink), so that
+// the compiler's wait for the next rounds' records counts exactly.  Four
+// wavefronts per SIMD (104 registers: the two rounds of prefetched records
+// do not fit in the 102 of five without spilling).
 template <bool REL, bool PER, bool G0, bool TAPE>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_push_deposit_f32(PushDepArgs a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_push_deposit_f32(PushDepArgs a) {
   __shared__ float qp_s[EMO_NSPEC], jc_s[EMO_NSPEC];
   __shared__ double red[(kBlock / 64) * kRedWave];
-  __shared__ float4 cvs[kTileCells][kCellVec];  // the cells' gather vectors
-  __shared__ int2 mlist[kMoveList];  // (staging slot, group << 16 | rank), per wavefront
-  // (source cell, destination) groups: destination, movers, first arrival
-  // number, destination range
-  __shared__ int gkey[kTileCells * kGroupSlots], gcnt[kTileCells * kGroupSlots];
-  __shared__ int gk0[kTileCells * kGroupSlots], ge0[kTileCells * kGroupSlots], ge1[kTileCells * kGroupSlots];
-  __shared__ int wcnt[kBlock / 64];
-  __shared__ int overflow;  // a mover found no group slot or list entry
+  __shared__ float4 cvs[kTileCells][kCellVec];  // the tile's gather vectors
+  __shared__ float cj_s[12][kBlockCells];       // the block's per-cell outputs
+  __shared__ int ns_s[kBlockCells], nm_s[kBlockCells];
   // species constants in LDS (a per-lane load from the argument block would
   // make every iteration wait for all loads in flight); test particles
   // deposit nothing (src/particles/particle_handler.cpp:205)
@@ -333,173 +370,174 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
     jc_s[threadIdx.x] = a.k.s.test[threadIdx.x] ? 0.f : a.k.s.jc[threadIdx.x];
   }
-  if (threadIdx.x < kTileCells * kGroupSlots) {
-    gkey[threadIdx.x] = -1;
-    gcnt[threadIdx.x] = 0;
-  }
-  if (threadIdx.x == 0) overflow = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.lay[0] = 0;
   __syncthreads();
+  const long long blk = xcd_tile();
+  if (blk >= a.nblocks) return;
   const int sub = threadIdx.x & (kCellLanes - 1);
   const int lane = threadIdx.x & 63;
   const int gsh = lane & ~(kCellLanes - 1);  // the cell's first lane in the wavefront
   const unsigned below = (1u << sub) - 1u;
-  const long long tile = xcd_tile();
-  if (tile >= a.ntiles) return;
-  const long long c = tile * kTileCells + (threadIdx.x >> 4);
-  const bool in = c < a.ncell;
-  const int b = in ? a.base[c] : 0;
-  const int n = in ? a.cnt[c] : 0;
-  // the wavefront runs to its largest cell (ballots need every lane)
-  int nmax = n;
-  nmax = max(nmax, __shfl_xor(nmax, 16, 64));
-  nmax = max(nmax, __shfl_xor(nmax, 32, 64));
-  // the cell's gather vector in LDS, read every round rather than held in
-  // 20 registers (fewer registers, more wavefronts in flight)
-  if (sub < kCellVec) cvs[threadIdx.x >> 4][sub] = n > 0 ? a.k.cv[c * kCellVec + sub] : make_float4(0.f, 0.f, 0.f, 0.f);
-  wave_sync();
-  const float4 *cvl = cvs[threadIdx.x >> 4];
-  double acc[12];
-#pragma unroll
-  for (int q = 0; q < 12; q++) acc[q] = 0.0;
-  int nstay = 0, nmove = 0;  // the same in all lanes of the cell
-  int wl = 0;                // the wavefront's listed movers
-  bool lost = false, outside = false;
-  const unsigned long long below64 = (1ull << lane) - 1ull;
-  int2 *wlist = mlist + (threadIdx.x >> 6) * kWaveList;
+  const bool per[3] = {PER || a.k.per[0] != 0, PER || a.k.per[1] != 0, PER || a.k.per[2] != 0};
+  bool outside = false;
   // (one sink address for all lanes: wavefront-uniform, so it stays in
   // scalar registers)
   int4 *const sink_a = a.sink, *const sink_b = a.sink + 1;
-  float *const sink_v = reinterpret_cast<float *>(a.sink + 2);
-  // loads past the cell's end read its last particle (a slot of the cell's
-  // own range even when it is empty), so that they are unconditional
-  const int last = b + max(n - 1, 0);
-  int4 A = a.pa[min(b + sub, last)], Bv = a.pb[min(b + sub, last)];
-  float G = G0 ? a.g0[min(b + sub, last)] : 0.f;
-  // the first records in before the loop: the compiler's waits in the loop
-  // then count only the loop's own loads (otherwise every round would wait
-  // as the first one must)
-  __builtin_amdgcn_s_waitcnt(0);
-  for (int j0 = 0; j0 < nmax; j0 += kCellLanes) {
-    // (keeps the gather vector's LDS reads in the loop)
-    asm volatile("" ::: "memory");
-    const CellVec V{cvl[0], cvl[1], cvl[2], cvl[3], cvl[4]};
-    const int jn = min(b + j0 + kCellLanes + sub, last);
-    const int4 An = a.pa[jn], Bn = a.pb[jn];
-    const float Gn = G0 ? a.g0[jn] : 0.f;
-    const int src = b + j0 + sub;
-    const bool valid = j0 + sub < n;
-    const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
-    Pushed o;
-    push_particle<REL, PER>(a.k, V, A.x, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y), as_f(Bv.z),
-                            qp_s[sp], G0, G, o, nullptr);
-    outside = outside || (valid && o.outside);
-    const bool stay = valid && o.cell == (int)c;
-    const bool move = valid && !stay;
-    const unsigned gm = (unsigned)(__ballot(stay) >> gsh) & 0xffffu;
-    const unsigned mm = (unsigned)(__ballot(move) >> gsh) & 0xffffu;
-    const int4 Ao = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)o.cell), as_i(o.f[0]),
-                              as_i(o.f[1]), as_i(o.f[2]));
-    const int4 Bo = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
-    // a stayer to the output at the cell's next free slot, a mover to its
-    // staging slot (at most this particle's own: already read) with its
-    // 1/gamma; lanes without a particle to the sink
-    const int rs = stay ? b + nstay + __popc(gm & below) : b + nmove + __popc(mm & below);
-    int4 *const da = valid ? (stay ? a.pa_out : a.ma) + rs : sink_a;
-    int4 *const db = valid ? (stay ? a.pb_out : a.mb) + rs : sink_b;
-    float *const dv = move ? a.mv + rs : sink_v;
-    *da = Ao;
-    *db = Bo;
-    *dv = o.ig;
-    if (TAPE) {
-      if (stay) a.fwd[src] = (uint32_t)rs;
-      if (move) a.mvsrc[rs] = src;
-    }
-    if (stay) deposit_j64(acc, o.f, o.v, jc_s[sp]);
-    {
-      // the mover's group: its destination's slot in the cell's table
-      int gs = -1;
+  const int4 zero4 = make_int4(0, 0, 0, 0);
+  for (int t = 0; t < kTilesPerBlock; t++) {
+    const int lc = t * kTileCells + (threadIdx.x >> 4);  // the cell among the block's 32
+    const long long c = blk * kBlockCells + lc;
+    const bool in = c < a.ncell;
+    // every particle of the cell has its padded indices
+    int cix, ciy, ciz;
+    decode_fast(a.k, (int)c, cix, ciy, ciz);
+    const int b = in ? a.base[c] : 0;
+    const int n = in ? a.cnt[c] : 0;
+    // the wavefront runs to its largest cell (ballots need every lane)
+    int nmax = n;
+    nmax = max(nmax, __shfl_xor(nmax, 16, 64));
+    nmax = max(nmax, __shfl_xor(nmax, 32, 64));
+    // the cell's gather vector in LDS, read every round rather than held in
+    // 20 registers (fewer registers, more wavefronts in flight); the
+    // wavefront's reads of the previous tile's vectors are done (LDS
+    // operations of a wavefront complete in order)
+    if (sub < kCellVec)
+      cvs[threadIdx.x >> 4][sub] = n > 0 ? a.k.cv[c * kCellVec + sub] : make_float4(0.f, 0.f, 0.f, 0.f);
+    wave_sync();
+    const float4 *cvl = cvs[threadIdx.x >> 4];
+    double acc[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) acc[q] = 0.0;
+    int nstay = 0, nmove = 0;  // the same in all lanes of the cell
+    // loads past the cell's end read its last particle (a slot of the cell's
+    // own range even when it is empty), so that they are unconditional
+    const int last = b + max(n - 1, 0);
+    int4 A = a.pa[min(b + sub, last)], Bv = a.pb[min(b + sub, last)];
+    float G = G0 ? a.g0[min(b + sub, last)] : 0.f;
+    // the records two rounds ahead are in flight during a round (a staging
+    // store of this round goes to a slot of this round or an earlier one)
+    int4 A1 = a.pa[min(b + kCellLanes + sub, last)], B1 = a.pb[min(b + kCellLanes + sub, last)];
+    float G1 = G0 ? a.g0[min(b + kCellLanes + sub, last)] : 0.f;
+    // the first records in before the loop: the compiler's waits in the loop
+    // then count only the loop's own loads (otherwise every round would wait
+    // as the first one must)
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int j0 = 0; j0 < nmax; j0 += kCellLanes) {
+      // (keeps the gather vector's LDS reads in the loop)
+      asm volatile("" ::: "memory");
+      const CellVec V{cvl[0], cvl[1], cvl[2], cvl[3], cvl[4]};
+      const int jn = min(b + j0 + 2 * kCellLanes + sub, last);
+      const int4 An = a.pa[jn], Bn = a.pb[jn];
+      const float Gn = G0 ? a.g0[jn] : 0.f;
+      const int src = b + j0 + sub;
+      const bool valid = j0 + sub < n;
+      const int sp = (int)((uint32_t)A.x >> kSpeciesShift);
+      Pushed o;
+      push_particle<REL, PER>(a.k, V, cix, ciy, ciz, as_f(A.y), as_f(A.z), as_f(A.w), as_f(Bv.x), as_f(Bv.y),
+                              as_f(Bv.z), qp_s[sp], G0, G, o, nullptr);
+      outside = outside || (valid && o.outside);
+      const bool stay = valid && o.cell == (int)c;
+      const bool move = valid && !stay;
+      const unsigned gm = (unsigned)(__ballot(stay) >> gsh) & 0xffffu;
+      const unsigned mm = (unsigned)(__ballot(move) >> gsh) & 0xffffu;
+      const int4 Ao = make_int4((int)(((uint32_t)sp << kSpeciesShift) | (uint32_t)o.cell), as_i(o.f[0]),
+                                as_i(o.f[1]), as_i(o.f[2]));
+      const int4 Bo = make_int4(as_i(o.u[0]), as_i(o.u[1]), as_i(o.u[2]), Bv.w);
+      // a stayer to the output at the cell's next free slot, a mover to its
+      // staging slot (at most this particle's own: already read); lanes
+      // without a particle to the sink
+      const int rs = stay ? b + nstay + __popc(gm & below) : b + nmove + __popc(mm & below);
+      int4 *const da = valid ? (stay ? a.pa_out : a.ma) + rs : sink_a;
+      int4 *const db = valid ? (stay ? a.pb_out : a.mb) + rs : sink_b;
+      *da = Ao;
+      *db = Bo;
+      if (TAPE) {
+        if (stay) a.fwd[src] = (uint32_t)rs;
+        if (move) a.mvsrc[rs] = src;
+      }
+      if (stay) deposit_moments(acc, o.f, o.v, jc_s[sp]);
       if (move) {
-        int *keys = gkey + (threadIdx.x >> 4) * kGroupSlots;
-        const int h = (o.cell * 5) & (kGroupSlots - 1);
-        for (int t = 0; t < kGroupSlots; t++) {
-          const int sl = (h + t) & (kGroupSlots - 1);
-          const int old = atomicCAS(&keys[sl], -1, o.cell);
-          if (old == -1 || old == o.cell) {
-            gs = (threadIdx.x >> 4) * kGroupSlots + sl;
-            break;
-          }
+        // the destination's arrival count and mask (results not read)
+        const int sx = neighbour_step(o.ijk[0] - cix, a.k.g.n[0], per[0]);
+        const int sy = neighbour_step(o.ijk[1] - ciy, a.k.g.n[1], per[1]);
+        const int sz = neighbour_step(o.ijk[2] - ciz, a.k.g.n[2], per[2]);
+        const bool far = sx == 2 || sy == 2 || sz == 2;
+        atomicAdd(&a.arr[o.cell], 1);
+        atomicOr(&a.arrm[o.cell], far ? 1 << kFarBit : 1 << (9 * (sx + 1) + 3 * (sy + 1) + (sz + 1)));
+        if (far) {
+          // (only a particle moving more than a cell in one step)
+          const int k = atomicAdd(a.nfar, 1);
+          if (k < a.far_cap)
+            a.far[k] = make_int2(o.cell, rs);
+          else
+            atomicOr(a.flag, kFlagMigrate);
         }
       }
-      const unsigned long long lm = __ballot(gs >= 0);
-      const int pos = wl + __popcll(lm & below64);
-      wl += __popcll(lm);
-      if (gs >= 0 && pos < kWaveList) {
-        const int rank = atomicAdd(&gcnt[gs], 1);
-        wlist[pos] = make_int2(rs, (gs << 16) | rank);
-      } else {
-        lost = lost || move;
+      nstay += __popc(gm);
+      nmove += __popc(mm);
+      A = A1;
+      Bv = B1;
+      G = G1;
+      A1 = An;
+      B1 = Bn;
+      G1 = Gn;
+    }
+    // zeros to the next 8-slot boundary after the stayers (output) and the
+    // staged movers: whole 128-byte lines (every range starts on an 8-slot
+    // boundary and holds at least round8(n) slots)
+    {
+      const int es = b + nstay, pe = (es + 7) & ~7;
+      if (in && es + sub < pe) {
+        a.pa_out[es + sub] = zero4;
+        a.pb_out[es + sub] = zero4;
+      }
+      const int em = b + nmove, pm = (em + 7) & ~7;
+      if (in && em + sub < pm) {
+        a.ma[em + sub] = zero4;
+        a.mb[em + sub] = zero4;
       }
     }
-    nstay += __popc(gm);
-    nmove += __popc(mm);
-    A = An;
-    Bv = Bn;
-    G = Gn;
+    // the cell's sum over its 16 lanes in lane order, in FP64: the lanes' sums
+    // through LDS one component (4 moments) at a time, then one lane per
+    // (cell, moment), and the quad of a cell's 4 moments forms its 4 edges
+    static_assert(kRedQ == 4, "a reduction round holds one component's 4 moments");
+    double *rw = red + (threadIdx.x >> 6) * kRedWave;
+#pragma unroll
+    for (int q0 = 0; q0 < 12; q0 += kRedQ) {
+      double *mine = rw + (lane >> 4) * kRedGroup + (lane & 15) * kRedQ;
+#pragma unroll
+      for (int q = 0; q < kRedQ; q++) mine[q] = acc[q0 + q];
+      wave_sync();
+      if (lane < 4 * kRedQ) {
+        const int grp = lane / kRedQ, q = lane - kRedQ * grp;
+        const double *col = rw + grp * kRedGroup + q;
+        double sum = col[0];
+#pragma unroll
+        for (int l = 1; l < kCellLanes; l++) sum += col[kRedQ * l];
+        const int qb = lane & ~3;
+        const double M0 = __shfl(sum, qb, 64), Mj = __shfl(sum, qb + 1, 64);
+        const double Mk = __shfl(sum, qb + 2, 64), Mjk = __shfl(sum, qb + 3, 64);
+        cj_s[q0 + q][t * kTileCells + (threadIdx.x >> 6) * 4 + grp] = (float)edge_from_moments(q, M0, Mj, Mk, Mjk);
+      }
+      wave_sync();
+    }
+    if (sub == 0) {
+      ns_s[lc] = nstay;
+      nm_s[lc] = nmove;
+    }
   }
   if (__ballot(outside) != 0ull && lane == 0) atomicOr(a.flag, kFlagOutOfGrid);
-  if (__ballot(lost) != 0ull && lane == 0) overflow = 1;
-  // the cell's sum over its 16 lanes in lane order, in FP64: the lanes' sums
-  // through LDS 4 edges at a time, then one lane per (cell, edge)
-  double *rw = red + (threadIdx.x >> 6) * kRedWave;
-#pragma unroll
-  for (int q0 = 0; q0 < 12; q0 += kRedQ) {
-    double *mine = rw + (lane >> 4) * kRedGroup + (lane & 15) * kRedQ;
-#pragma unroll
-    for (int q = 0; q < kRedQ; q++) mine[q] = acc[q0 + q];
-    wave_sync();
-    if (lane < 4 * kRedQ) {
-      const int grp = lane / kRedQ, q = lane - kRedQ * grp;
-      const long long cc = tile * kTileCells + (threadIdx.x >> 6) * 4 + grp;
-      const double *col = rw + grp * kRedGroup + q;
-      double sum = col[0];
-#pragma unroll
-      for (int l = 1; l < kCellLanes; l++) sum += col[kRedQ * l];
-      if (cc < a.ncell) a.cj[(q0 + q) * a.ncell + cc] = (float)sum;
-    }
-    wave_sync();
-  }
-  if (sub == 0 && in) {
-    a.ns[c] = nstay;
-    a.nm[c] = nmove;
-  }
-  if (lane == 0) wcnt[threadIdx.x >> 6] = min(wl, kWaveList);
-  // (the barrier also orders the block's staging stores before the loads
-  // below)
   __syncthreads();
-  if (overflow) {
-    // a mover found no room in its cell's group table or its wavefront's
-    // list: the block's movers filed one by one, from the staged records
-    for (int j = sub; j < nmove; j += kCellLanes) file_mover(a.arr, a.akey, a.base, a.ma[b + j].x & kCellMask, b + j);
-    return;
+  // the block's per-cell outputs, 32 cells (128 bytes) per array
+  const long long c0 = blk * kBlockCells;
+  const int ncb = (int)min((long long)kBlockCells, a.ncell - c0);
+  for (int i = threadIdx.x; i < 12 * kBlockCells; i += kBlock) {
+    const int q = i / kBlockCells, l = i - q * kBlockCells;
+    if (l < ncb) a.cj[q * a.cstride + c0 + l] = cj_s[q][l];
   }
-  // the groups filed: one atomic each, for all its movers; then every listed
-  // mover's key, at its group's first arrival number plus its rank (the
-  // order the ranges' keys take is provisional: arrive_f32 sorts them)
-  if (threadIdx.x < kTileCells * kGroupSlots) {
-    const int g = threadIdx.x, d = gkey[g], m = gcnt[g];
-    if (d >= 0 && m > 0) {
-      gk0[g] = atomicAdd(&a.arr[d], m);
-      ge0[g] = a.base[d];
-      ge1[g] = a.base[d + 1];
-    }
-  }
-  __syncthreads();
-  const int nl = wcnt[threadIdx.x >> 6];
-  for (int i = lane; i < nl; i += 64) {
-    const int2 e = wlist[i];
-    const int g = e.y >> 16, k = gk0[g] + (e.y & 0xffff);
-    if (k < ge1[g] - ge0[g]) a.akey[ge1[g] - 1 - k] = e.x;  // (else the layout is rebuilt)
+  if (threadIdx.x < ncb) {
+    a.ns[c0 + threadIdx.x] = ns_s[threadIdx.x];
+    a.nm[c0 + threadIdx.x] = nm_s[threadIdx.x];
   }
 }
 
@@ -596,9 +634,7 @@ struct RelayoutArgs {
   int4 *pa_out, *pb_out;  // the step's output buffer
   int4 *ma, *mb;          // staging
   uint32_t *fwd;          // with a tape: source slot -> new slot, or null
-  int32_t *put, *akey;
   long long ncell;
-  unsigned long long *flag;
 };
 
 __global__ void __launch_bounds__(kBlock) k_restage_f32(RelayoutArgs a) {
@@ -614,8 +650,8 @@ __global__ void __launch_bounds__(kBlock) k_restage_f32(RelayoutArgs a) {
 }
 
 // Rebuild, step 2 of 2: the stayers to their new slots in the output buffer
-// (with a tape, their map entries moved along), and the cell's movers placed
-// in the new layout's arrival ranges.
+// (with a tape, their map entries moved along).  The movers stay where they
+// are staged (the old layout's ranges), where the arrival pass finds them.
 __global__ void __launch_bounds__(kBlock) k_relayout_f32(RelayoutArgs a) {
   if (a.lay[0] == 0) return;
   const long long c = (long long)blockIdx.x * kTileCells + (threadIdx.x >> 4);
@@ -635,120 +671,191 @@ __global__ void __launch_bounds__(kBlock) k_relayout_f32(RelayoutArgs a) {
       if (f >= (uint32_t)b && f < (uint32_t)(b + n)) a.fwd[b + j] = f - (uint32_t)b + (uint32_t)bn;
     }
   }
-  for (int j = sub; j < m; j += kCellLanes) {
-    const int d = a.ma[b + j].x & kCellMask;
-    const int k = atomicAdd(&a.put[d], 1);
-    const int e1 = a.base_new[d + 1];
-    if (k < e1 - a.base_new[d])
-      a.akey[e1 - 1 - k] = b + j;
-    else
-      atomicOr(a.flag, kFlagMigrate);
-  }
 }
 
-// Per cell (4 lanes, 64 cells per block): the arrivals in ascending order of
-// their staging slot (source cell, then order within it: the previous array
-// order) -- ranked in LDS for up to kArrFast arrivals, else ranked from global
-// memory into akey2 -- the rank-r arrival taken by lane r % 4: its record
-// written after the stayers, its currents summed as in the particle pass
-// (FP64, lane order) and added to the stayers' sum.  Then the cell's new
-// count; the per-step counters are reset for the next step.
-constexpr int kArrLanes = 4;
-constexpr int kArrCells = kBlock / kArrLanes;
-constexpr int kArrFast = 32;
+// Per cell (one lane, 128 cells per block, XCD-aware order): the cell's
+// arrivals are the movers staged in the ranges of the neighbours its arrival
+// mask names (bit k: the neighbour one step of -(dx, dy, dz) away, the
+// periodic wrap applied where the particle BCs are periodic) whose new cell
+// is this one, and, when a far mover was flagged, the far list's entries for
+// this cell.  They are taken in ascending order of their staging slot
+// (source cell, then order within it: the previous array order), sorted by
+// insertion in LDS for up to kArrFast arrivals, ranked in global memory
+// beyond.  The rank-r arrival is written after the stayers and its current
+// (v = u / gamma, gamma from u as the next push computes it) added to the
+// cell's moments in that order; the last line is padded with zeros.  Then
+// the cell's current vector (stayers + arrivals) and new count, every cell
+// of the block (whole lines); the per-step counters are reset for the next
+// step.
+constexpr int kArrBlock = 128;
+constexpr int kArrFast = 16;
 
 struct ArriveArgs {
+  Geo g;
+  int per[3];
+  int rel;
+  int dedupe;  // a periodic dimension of at most 2 cells: two steps can reach one neighbour
   const int32_t *base, *base_new, *lay;
-  const int32_t *ns;
-  int32_t *arr, *put, *cnt;
-  const int32_t *akey;
-  int32_t *akey2;  // the sorted keys of the ranges with more than kArrFast arrivals
+  const int32_t *ns, *nm;
+  int32_t *arr, *arrm, *cnt;
+  int32_t *akey, *akey2;  // more than kArrFast arrivals: their slots, and sorted, at the top of the range
+  const int2 *far;
+  const int32_t *nfar;
+  long long far_cap;
   const int4 *ma, *mb;
-  const float *mv;
   const int32_t *mvsrc;  // with a tape
   int4 *pa_out, *pb_out;
   uint32_t *fwd;
   float *cj;
+  long long cstride;
   float jc[EMO_NSPEC];  // 0 for test particles
   long long ncell;
   unsigned long long *flag;
 };
 
-__global__ void __launch_bounds__(kBlock) k_arrive_f32(ArriveArgs a) {
-  __shared__ int keys[kArrCells][kArrFast];
-  __shared__ int sorted[kArrCells][kArrFast];
+// the padded index of the neighbour one step of -s away in a dimension (the
+// source of an arrival that took step s), wrapped where periodic (the
+// caller checks it against the padded grid)
+__device__ inline int source_index(int i, int s, int n, bool per) {
+  int j = i - s;
+  if (per) j = j < 1 ? j + n : (j > n ? j - n : j);
+  return j;
+}
+
+__global__ void __launch_bounds__(kArrBlock) k_arrive_f32(ArriveArgs a) {
+  __shared__ int cand[kArrBlock][kArrFast + 1];
   __shared__ float jc_s[EMO_NSPEC];
   if (threadIdx.x < EMO_NSPEC) jc_s[threadIdx.x] = a.jc[threadIdx.x];
-  const int grp = threadIdx.x / kArrLanes, sub = threadIdx.x & (kArrLanes - 1);
-  const long long c = (long long)xcd_tile() * kArrCells + grp;
+  __syncthreads();
+  const long long c = (long long)xcd_tile() * kArrBlock + threadIdx.x;
   const bool in = c < a.ncell;
-  const int32_t *B = a.lay[0] ? a.base_new : a.base;
+  // the loads that do not depend on each other issued together: both
+  // layouts' bounds (the rebuilt one is current after a rebuild), the counts,
+  // and the stayers' currents this pass adds to
+  const bool rebuilt = a.lay[0] != 0;
+  const int bo0 = in ? a.base[c] : 0, bo1 = in ? a.base[c + 1] : 0;
+  const int bn0 = in ? a.base_new[c] : 0, bn1 = in ? a.base_new[c + 1] : 0;
   int na = in ? a.arr[c] : 0;
-  const int s0 = in ? B[c] + a.ns[c] : 0;
-  const int e1 = in ? B[c + 1] : 0;
+  const int mask = in ? a.arrm[c] : 0;
+  const int nsc = in ? a.ns[c] : 0;
+  float cj0[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) cj0[q] = in ? a.cj[q * a.cstride + c] : 0.f;
+  const int s0 = (rebuilt ? bn0 : bo0) + nsc;
+  const int e1 = rebuilt ? bn1 : bo1;
   if (in && s0 + na > e1) {  // (the check has rebuilt the layout, so only after a flagged error)
     atomicOr(a.flag, kFlagMigrate);
     na = max(e1 - s0, 0);
   }
-  const int k0 = e1 - na;  // the range's keys
-  const bool fast = na <= kArrFast;
-  if (fast)
-    for (int i = sub; i < na; i += kArrLanes) keys[grp][i] = a.akey[k0 + i];
-  __syncthreads();
-  if (fast) {
-    for (int i = sub; i < na; i += kArrLanes) {
-      const int k = keys[grp][i];
-      int r = 0;
-      for (int j = 0; j < na; j++) r += keys[grp][j] < k ? 1 : 0;
-      sorted[grp][r] = k;
-    }
-  } else {
-    for (int i = sub; i < na; i += kArrLanes) {
-      const int k = a.akey[k0 + i];
-      int r = 0;
-      for (int j = 0; j < na; j++) r += a.akey[k0 + j] < k ? 1 : 0;
-      a.akey2[k0 + r] = k;
-    }
-  }
-  // (a work-group barrier orders the block's global stores before its loads)
-  __syncthreads();
   double acc[12];
 #pragma unroll
   for (int q = 0; q < 12; q++) acc[q] = 0.0;
-  auto take = [&](int st, int r) {
-    const int4 A = a.ma[st], Bv = a.mb[st];
-    const float ig = a.mv[st];
-    const int pos = s0 + r;
-    a.pa_out[pos] = A;
-    a.pb_out[pos] = Bv;
-    if (a.fwd) a.fwd[a.mvsrc[st]] = (uint32_t)pos;
-    const float f[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
-    // the push's v = u * (1/gamma), the same product of the same floats
-    const float v[3] = {as_f(Bv.x) * ig, as_f(Bv.y) * ig, as_f(Bv.z) * ig};
-    deposit_j64(acc, f, v, jc_s[(uint32_t)A.x >> kSpeciesShift]);
-  };
-  if (fast) {
-    for (int r = sub; r < na; r += kArrLanes) take(sorted[grp][r], r);
-  } else {
-    for (int r = sub; r < na; r += kArrLanes) take(a.akey2[k0 + r], r);
-  }
-  // the 4 lanes' sums in a fixed order (quad permutations), in FP64
-  static_assert(kArrLanes == 4, "the reduction covers 4 lanes");
-  dep_level64<0x4E>(acc);
-  dep_level64<0xB1>(acc);
-  if (sub == 0 && in) {
-    if (na > 0) {
-#pragma unroll
-      for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = (float)((double)a.cj[q * a.ncell + c] + acc[q]);
+  if (na > 0) {
+    const bool fast = na <= kArrFast;
+    const int k0 = e1 - na;  // (slow path) the range's top na slots of akey and akey2
+    int *cl = cand[threadIdx.x];
+    int nc = 0;
+    // a candidate's staging slot, kept in ascending order (insertion) in LDS
+    auto add = [&](int slot) {
+      if (nc < na) {
+        if (fast) {
+          int i = nc;
+          while (i > 0 && cl[i - 1] > slot) {
+            cl[i] = cl[i - 1];
+            i--;
+          }
+          cl[i] = slot;
+        } else {
+          a.akey[k0 + nc] = slot;
+        }
+      }
+      nc++;
+    };
+    int ix, iy, iz;
+    decode_cell(a.g, (int)c, &ix, &iy, &iz);
+    for (int m = mask & ((1 << kFarBit) - 1); m != 0; m &= m - 1) {
+      const int k = __ffs(m) - 1;
+      const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+      const int sx = source_index(ix, dx, a.g.n[0], a.per[0]);
+      const int sy = source_index(iy, dy, a.g.n[1], a.per[1]);
+      const int sz = source_index(iz, dz, a.g.n[2], a.per[2]);
+      if (sx < 0 || sy < 0 || sz < 0 || sx >= a.g.nc[0] || sy >= a.g.nc[1] || sz >= a.g.nc[2]) continue;
+      const int s = (sx * a.g.nc[1] + sy) * a.g.nc[2] + sz;
+      if (a.dedupe) {
+        // another step of the mask that reaches the same source was scanned
+        bool seen = false;
+        for (int m2 = mask & ((1 << k) - 1); m2 != 0 && !seen; m2 &= m2 - 1) {
+          const int k2 = __ffs(m2) - 1;
+          const int tx = source_index(ix, k2 / 9 - 1, a.g.n[0], a.per[0]);
+          const int ty = source_index(iy, (k2 / 3) % 3 - 1, a.g.n[1], a.per[1]);
+          const int tz = source_index(iz, k2 % 3 - 1, a.g.n[2], a.per[2]);
+          seen = tx == sx && ty == sy && tz == sz;
+        }
+        if (seen) continue;
+      }
+      const int bs = a.base[s], ms = a.nm[s];
+      for (int j = 0; j < ms; j++)
+        if ((a.ma[bs + j].x & kCellMask) == (int)c) add(bs + j);
     }
-    a.cnt[c] = a.ns[c] + na;
+    if (mask & (1 << kFarBit)) {
+      const int nf = (int)min((long long)*a.nfar, a.far_cap);
+      for (int j = 0; j < nf; j++) {
+        const int2 e = a.far[j];
+        if (e.x == (int)c) add(e.y);
+      }
+    }
+    if (nc != na) {  // (the counts and the staged movers disagree: flagged)
+      atomicOr(a.flag, kFlagMigrate);
+      na = min(na, nc);
+    }
+    if (!fast) {
+      // more arrivals than the LDS list holds: ranked in global memory
+      for (int i = 0; i < na; i++) {
+        const int key = a.akey[k0 + i];
+        int r = 0;
+        for (int j = 0; j < na; j++) r += a.akey[k0 + j] < key ? 1 : 0;
+        a.akey2[k0 + r] = key;
+      }
+    }
+    const int jcr = a.rel;
+    for (int r = 0; r < na; r++) {
+      const int st = fast ? cl[r] : a.akey2[k0 + r];
+      const int4 A = a.ma[st], Bv = a.mb[st];
+      const int pos = s0 + r;
+      a.pa_out[pos] = A;
+      a.pb_out[pos] = Bv;
+      if (a.fwd) a.fwd[a.mvsrc[st]] = (uint32_t)pos;
+      const float f[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
+      const float ux = as_f(Bv.x), uy = as_f(Bv.y), uz = as_f(Bv.z);
+      const float ig = jcr ? __builtin_amdgcn_rsqf(fmaf(ux, ux, fmaf(uy, uy, fmaf(uz, uz, 1.f)))) : 1.f;
+      const float v[3] = {ux * ig, uy * ig, uz * ig};
+      deposit_moments(acc, f, v, jc_s[(uint32_t)A.x >> kSpeciesShift]);
+    }
+    // zeros to the next 8-slot boundary: a whole last line
+    const int4 zero4 = make_int4(0, 0, 0, 0);
+    for (int p = s0 + na; p & 7; p++) {
+      a.pa_out[p] = zero4;
+      a.pb_out[p] = zero4;
+    }
+  }
+  if (in) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+      const int q0 = q & ~3;
+      const double w = edge_from_moments(q & 3, acc[q0], acc[q0 + 1], acc[q0 + 2], acc[q0 + 3]);
+      a.cj[q * a.cstride + c] = (float)((double)cj0[q] + w);
+    }
+    a.cnt[c] = nsc + na;
     a.arr[c] = 0;
-    a.put[c] = 0;
+    a.arrm[c] = 0;
   }
 }
 
-// The rebuilt layout becomes current (only after a rebuild).
-__global__ void k_commit_f32(int32_t *base, const int32_t *base_new, const int32_t *lay, long long ncell) {
+// The rebuilt layout becomes current (only after a rebuild); the far list
+// is emptied for the next step.
+__global__ void k_commit_f32(int32_t *base, const int32_t *base_new, const int32_t *lay, long long ncell,
+                             int32_t *nfar) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nfar = 0;
   if (lay[0] == 0) return;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c <= ncell;
        c += (long long)gridDim.x * blockDim.x)
@@ -770,6 +877,7 @@ struct RhoJArgs {
   int rel;
   long long ncell;
   float *cj, *cr;
+  long long sj;  // component stride of cj (cr: ncell)
 };
 
 __global__ void __launch_bounds__(kBlock) k_deposit_rhoj_f32(RhoJArgs a) {
@@ -809,7 +917,7 @@ __global__ void __launch_bounds__(kBlock) k_deposit_rhoj_f32(RhoJArgs a) {
   }
   if (sub == 0 && c < a.ncell) {
 #pragma unroll
-    for (int q = 0; q < 12; q++) a.cj[q * a.ncell + c] = (float)acc[q];
+    for (int q = 0; q < 12; q++) a.cj[q * a.sj + c] = (float)acc[q];
 #pragma unroll
     for (int q = 0; q < 8; q++) a.cr[q * a.ncell + c] = (float)acc[12 + q];
   }
@@ -1285,7 +1393,7 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
   ChosenState *s = c->ch;
   const Geo &g = c->geo;
   launch_cellvec(c);
-  const long long ntiles = (g.ncell + kTileCells - 1) / kTileCells;
+  const long long nblocks = (g.ncell + kBlockCells - 1) / kBlockCells;
   uint32_t *fwd = tape ? tape->fwd : nullptr;
   PushDepArgs a;
   a.k = push_consts(c, dt);
@@ -1295,7 +1403,6 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
   a.pb_out = s->pb_alt;
   a.ma = tape ? tape->stage_a : s->pa;
   a.mb = tape ? tape->stage_b : s->pb;
-  a.mv = s->mv;
   a.mvsrc = tape ? tape->mvsrc : nullptr;
   a.g0 = s->g0_active ? s->g0 : nullptr;
   a.base = s->base;
@@ -1303,18 +1410,23 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
   a.ns = s->ns;
   a.nm = s->nm;
   a.arr = s->arr;
-  a.akey = s->akey;
+  a.arrm = s->arrm;
+  a.far = s->far;
+  a.nfar = s->nfar;
+  a.far_cap = kFarListCap;
   a.cj = s->cj;
+  a.cstride = s->cstride;
   a.fwd = fwd;
   a.lay = s->lay;
   a.sink = s->sink;
   a.ncell = g.ncell;
-  a.ntiles = ntiles;
+  a.nblocks = nblocks;
   a.flag = c->dflag;
   {
     TimedLaunch t(c, "push_deposit_f32");
-    launch_push_deposit(c, a, dim3(xcd_grid((unsigned)ntiles)));
+    launch_push_deposit(c, a, dim3(xcd_grid((unsigned)nblocks)));
   }
+  const long long ntiles = (g.ncell + kTileCells - 1) / kTileCells;
   const long long nblk = (g.ncell + kBlock - 1) / kBlock;
   {
     TimedLaunch t(c, "check_f32");
@@ -1340,44 +1452,50 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
     r.ma = a.ma;
     r.mb = a.mb;
     r.fwd = fwd;
-    r.put = s->put;
-    r.akey = s->akey;
     r.ncell = g.ncell;
-    r.flag = c->dflag;
     TimedLaunch t(c, "relayout_f32");
     hipLaunchKernelGGL(k_restage_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
     hipLaunchKernelGGL(k_relayout_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
   }
   {
     ArriveArgs r;
+    r.g = g;
+    for (int d = 0; d < 3; d++) r.per[d] = a.k.per[d];
+    r.rel = c->cfg.relativity ? 1 : 0;
+    r.dedupe = 0;
+    for (int d = 0; d < 3; d++) r.dedupe |= (a.k.per[d] && g.n[d] <= 2) ? 1 : 0;
     r.base = s->base;
     r.base_new = s->base_new;
     r.lay = s->lay;
     r.ns = s->ns;
+    r.nm = s->nm;
     r.arr = s->arr;
-    r.put = s->put;
+    r.arrm = s->arrm;
     r.cnt = s->cnt;
     r.akey = s->akey;
     r.akey2 = s->akey2;
+    r.far = s->far;
+    r.nfar = s->nfar;
+    r.far_cap = kFarListCap;
     r.ma = a.ma;
     r.mb = a.mb;
-    r.mv = s->mv;
     r.mvsrc = a.mvsrc;
     r.pa_out = s->pa_alt;
     r.pb_out = s->pb_alt;
     r.fwd = fwd;
     r.cj = s->cj;
+    r.cstride = s->cstride;
     for (int q = 0; q < EMO_NSPEC; q++) r.jc[q] = a.k.s.test[q] ? 0.f : a.k.s.jc[q];
     r.ncell = g.ncell;
     r.flag = c->dflag;
     TimedLaunch t(c, "arrive_f32");
-    hipLaunchKernelGGL(k_arrive_f32, dim3(xcd_grid((unsigned)((g.ncell + kArrCells - 1) / kArrCells))), dim3(kBlock),
-                       0, c->stream, r);
+    hipLaunchKernelGGL(k_arrive_f32, dim3(xcd_grid((unsigned)((g.ncell + kArrBlock - 1) / kArrBlock))),
+                       dim3(kArrBlock), 0, c->stream, r);
   }
   {
     TimedLaunch t(c, "commit_f32");
     hipLaunchKernelGGL(k_commit_f32, dim3(256), dim3(kBlock), 0, c->stream, s->base, (const int32_t *)s->base_new,
-                       (const int32_t *)s->lay, g.ncell);
+                       (const int32_t *)s->lay, g.ncell, s->nfar);
   }
   std::swap(s->pa, s->pa_alt);
   std::swap(s->pb, s->pb_alt);
@@ -1385,6 +1503,8 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
   b.g = g;
   b.cj = s->cj;
   b.cr = nullptr;
+  b.sj = s->cstride;
+  b.sr = g.ncell;
   for (int d = 0; d < 3; d++) b.J[d] = s->fs.J[d];
   b.rho = s->fs.rho;
   b.do_rho = 0;
@@ -1414,7 +1534,6 @@ int fp32_create(emo_ctx *c) {
   if (!rc) rc = dalloc(c, &s->pa_alt, nslots);
   if (!rc) rc = dalloc(c, &s->pb, nslots);
   if (!rc) rc = dalloc(c, &s->pb_alt, nslots);
-  if (!rc) rc = dalloc(c, &s->mv, nslots);
   if (!rc) rc = dalloc(c, &s->akey, nslots);
   if (!rc) rc = dalloc(c, &s->akey2, nslots);
   if (!rc) rc = dalloc(c, &s->base, nc1);
@@ -1424,12 +1543,15 @@ int fp32_create(emo_ctx *c) {
   if (!rc) rc = dalloc(c, &s->nm, nc1);
   if (!rc) rc = dalloc(c, &s->rpart, 2 * (nc1 / kBlock + 1));
   if (!rc) rc = dalloc(c, &s->arr, nc1);
-  if (!rc) rc = dalloc(c, &s->put, nc1);
+  if (!rc) rc = dalloc(c, &s->arrm, nc1);
+  if (!rc) rc = dalloc(c, &s->far, kFarListCap);
+  if (!rc) rc = dalloc(c, &s->nfar, 1);
   if (!rc) rc = dalloc(c, &s->dcount, nc1);
   if (!rc) rc = dalloc(c, &s->doff, nc1);
   if (!rc) rc = dalloc(c, &s->lay, 4);
   if (!rc) rc = dalloc(c, &s->sink, kSinkVecs);
-  if (!rc) rc = dalloc(c, &s->cj, 12 * g.ncell);
+  s->cstride = (g.ncell + kBlockCells - 1) / kBlockCells * kBlockCells;
+  if (!rc) rc = dalloc(c, &s->cj, 12 * s->cstride);
   if (!rc) rc = dalloc(c, &s->cv, (size_t)kCellVec * g.ncell);
   size_t nt = (size_t)g.ntot;
   for (int q = 0; q < 3 && !rc; q++) {
@@ -1448,8 +1570,9 @@ int fp32_create(emo_ctx *c) {
                     s->Btfix[0], s->Btfix[1], s->Btfix[2], s->fs.J[0], s->fs.J[1], s->fs.J[2], s->fs.rho};
   for (int q = 0; q < 13; q++) EMO_HIP_CHECK_RET(c, hipMemset(all[q], 0, nt * sizeof(float)));
   // per-step counters start (and end every step) at zero; an empty layout
-  int32_t *zero[] = {s->cnt, s->ns, s->nm, s->arr, s->put, s->dcount};
+  int32_t *zero[] = {s->cnt, s->ns, s->nm, s->arr, s->arrm, s->dcount};
   for (int32_t *p : zero) EMO_HIP_CHECK_RET(c, hipMemset(p, 0, nc1 * 4));
+  EMO_HIP_CHECK_RET(c, hipMemset(s->nfar, 0, 4));
   // every arrival key ever read is a slot number (also where a flagged error
   // left a range short)
   EMO_HIP_CHECK_RET(c, hipMemset(s->akey, 0, (size_t)nslots * 4));
@@ -1469,8 +1592,8 @@ int fp32_create(emo_ctx *c) {
 void fp32_destroy(emo_ctx *c) {
   ChosenState *s = c->ch;
   if (!s) return;
-  void *ps[] = {s->pa, s->pa_alt, s->pb, s->pb_alt, s->mv, s->akey, s->akey2, s->base, s->base_new, s->cnt,
-                s->ns, s->nm, s->rpart, s->arr, s->put, s->dcount, s->doff, s->lay, s->sink, s->key, s->key_alt, s->perm,
+  void *ps[] = {s->pa, s->pa_alt, s->pb, s->pb_alt, s->akey, s->akey2, s->base, s->base_new, s->cnt,
+                s->ns, s->nm, s->rpart, s->arr, s->arrm, s->far, s->nfar, s->dcount, s->doff, s->lay, s->sink, s->key, s->key_alt, s->perm,
                 s->perm_alt, s->sort_tmp, s->scan_tmp, s->cj, s->cv, s->gam_id, s->fs.rho};
   for (void *p : ps) dfree(p);
   if (s->g0_owned) dfree(s->g0);
@@ -1718,12 +1841,14 @@ int fp32_deposit_rho(emo_ctx *c) {
     a.ncell = g.ncell;
     a.cj = cj;
     a.cr = cr;
+    a.sj = g.ncell;
     const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
     hipLaunchKernelGGL(k_deposit_rhoj_f32, dim3(blocks), dim3(kBlock), 0, c->stream, a);
     DepGridArgs<float> b;
     b.g = g;
     b.cj = cj;
     b.cr = cr;
+    b.sj = b.sr = g.ncell;
     for (int d = 0; d < 3; d++) b.J[d] = jt + d * g.ntot;  // discarded
     b.rho = s->fs.rho;
     b.do_rho = 1;
@@ -1839,12 +1964,15 @@ int fp32_init_poisson(emo_ctx *c, int32_t iter_max, double tol, int32_t *sweeps_
     a.ncell = g.ncell;
     a.cj = s->cj;
     a.cr = cr;
+    a.sj = s->cstride;
     const unsigned blocks = (unsigned)((g.ncell + (kBlock / 16) - 1) / (kBlock / 16));
     hipLaunchKernelGGL(k_deposit_rhoj_f32, dim3(blocks), dim3(kBlock), 0, c->stream, a);
     DepGridArgs<float> b;
     b.g = g;
     b.cj = s->cj;
     b.cr = cr;
+    b.sj = s->cstride;
+    b.sr = g.ncell;
     for (int d = 0; d < 3; d++) b.J[d] = s->fs.J[d];
     b.rho = s->fs.rho;
     b.do_rho = 1;

@@ -64,7 +64,7 @@ PushK fp32_push_consts(emo_ctx *c, double dt);
 
 struct Pushed {
   int cell;     // new cell id
-  int code;     // move code (kFarCode: further than one cell)
+  int ijk[3];   // its padded indices
   bool flip[3];  // reflected (u flipped) in dim d
   float f[3];   // new offset
   float u[3];   // new u (after the BCs)
@@ -163,15 +163,15 @@ __device__ __forceinline__ void gather_cell(const CellVec &V, float fx, float fy
   b[2] = fmaf(az, vBz.x, bz * vBz.y);
 }
 
-// q_p: ((0.5 dt q)/m) UNIT_ACC of the particle's species; g0: the stored
-// gamma for the first relativistic push (Q3), if use_g0
+// (ix, iy, iz): the padded indices of the particle's cell (every particle
+// of a cell shares them, so the caller decodes the cell id once); q_p:
+// ((0.5 dt q)/m) UNIT_ACC of the particle's species; g0: the stored gamma
+// for the first relativistic push (Q3), if use_g0
 template <bool REL, bool PER>
-__device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, int w, float fx, float fy,
-                                              float fz, float ux, float uy, float uz, float q_p, bool use_g0,
-                                              float g0, Pushed &o, unsigned long long *flag) {
+__device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, int ix, int iy, int iz, float fx,
+                                              float fy, float fz, float ux, float uy, float uz, float q_p,
+                                              bool use_g0, float g0, Pushed &o, unsigned long long *flag) {
   const Geo &g = a.g;
-  int ix, iy, iz;
-  decode_fast(a, w & kCellMask, ix, iy, iz);
   float eg[3], bg[3];
   gather_cell(V, fx, fy, fz, eg, bg);
   const float ex = eg[0], ey = eg[1], ez = eg[2];
@@ -234,10 +234,9 @@ __device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, 
     jz = min(max(jz, 1), g.n[2]);
   }
   o.cell = (jx * g.nc[1] + jy) * g.nc[2] + jz;
-  const int k0 = move_code(jx - ix, g.n[0], PER ? 1 : a.per[0]);
-  const int k1 = move_code(jy - iy, g.n[1], PER ? 1 : a.per[1]);
-  const int k2 = move_code(jz - iz, g.n[2], PER ? 1 : a.per[2]);
-  o.code = (k0 < 0 || k1 < 0 || k2 < 0) ? (int)kFarCode : (k0 * 6 + k1) * 6 + k2;
+  o.ijk[0] = jx;
+  o.ijk[1] = jy;
+  o.ijk[2] = jz;
   o.u[0] = ux;
   o.u[1] = uy;
   o.u[2] = uz;

@@ -4,7 +4,8 @@
 // (src/grid/oGrid.cpp:91-139) would add to it, in ascending cell-id order,
 // starting from zero (Grid::constJ / constRho).  Includes addJ's duplicate
 // Jx[ix][iy+1][iz] target (entries 1 and 3) and its never-written
-// Jx[ix][iy][iz+1].  Cell vectors are component-major: v[q * ncell + cell].
+// Jx[ix][iy][iz+1].  Cell vectors are component-major: v[q * stride + cell]
+// (sj for the J vectors, sr for the rho vectors, at least ncell).
 #pragma once
 
 #include "emo_internal.hpp"
@@ -15,6 +16,7 @@ template <typename T>
 struct DepGridArgs {
   Geo g;
   const T *cj, *cr;
+  long long sj, sr;  // component strides of cj and cr
   T *J[3], *rho;
   int do_rho;
 };
@@ -34,7 +36,7 @@ __global__ void __launch_bounds__(256) k_deposit_grid(DepGridArgs<T> a) {
   int k = (int)(p % g.nt[2]);
   int j = (int)((p / g.nt[2]) % g.nt[1]);
   int i = (int)(p / ((long long)g.nt[1] * g.nt[2]));
-  const long long N = g.ncell;
+  const long long N = a.sj;
   const T *cj = a.cj;
   T acc;
   // Jx(i,j,k) <- J2(i,j-1,k-1), J1(i,j-1,k), J3(i,j-1,k), J0(i,j,k)
@@ -64,6 +66,7 @@ __global__ void __launch_bounds__(256) k_deposit_grid(DepGridArgs<T> a) {
     // rho(i,j,k) <- R5(i-1,j-1,k-1), R2(i-1,j-1,k), R6(i-1,j,k-1), R1(i-1,j,k),
     //               R4(i,j-1,k-1), R3(i,j-1,k), R7(i,j,k-1), R0(i,j,k)
     const T *cr = a.cr;
+    const long long N = a.sr;
     acc = 0;
     if (dg_cell_ok(g, i - 1, j - 1, k - 1)) acc += cr[5 * N + dg_cid(g, i - 1, j - 1, k - 1)];
     if (dg_cell_ok(g, i - 1, j - 1, k)) acc += cr[2 * N + dg_cid(g, i - 1, j - 1, k)];

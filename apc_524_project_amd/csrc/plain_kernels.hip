@@ -565,6 +565,7 @@ void launch_deposit_f64(emo_ctx *c, int rho) {
   b.g = c->geo;
   b.cj = c->cj;
   b.cr = c->cr;
+  b.sj = b.sr = c->geo.ncell;
   for (int d = 0; d < 3; d++) b.J[d] = c->J[d];
   b.rho = c->rho;
   b.do_rho = rho;
