@@ -572,23 +572,54 @@ __global__ void __launch_bounds__(kBlock) k_check_f32(const int32_t *ns, const i
   }
 }
 
-// The rebuilt layout (one block; runs only when a rebuild is pending): the
+// Rebuild, step 1 of 2 (one launch; every block returns at once unless a
+// rebuild is pending).  Block 0 computes the rebuilt layout: the check's
 // block sums scanned, generous ranges if they fit the buffers, else tight
-// ones (which always fit), then every cell's new base.
-__global__ void __launch_bounds__(kRebaseThreads) k_rebase_f32(const int32_t *ns, const int32_t *arr, long long ncell,
-                                                               const long long *part, long long nblk,
-                                                               long long nslots, int32_t *base_new, int32_t *lay,
-                                                               unsigned long long *flag) {
-  if (lay[0] == 0) return;
+// ones (which always fit), then every cell's new base.  The other blocks,
+// 64 cells each, copy each cell's stayers from the output buffer (old
+// layout) to the staging buffers' free slots behind its movers -- exactly as
+// many as it has stayers, since its range there held stayers + movers.
+struct RelayoutArgs {
+  const int32_t *base, *base_new, *cnt, *ns, *nm, *lay;
+  int4 *pa_out, *pb_out;  // the step's output buffer
+  int4 *ma, *mb;          // staging
+  uint32_t *fwd;          // with a tape: source slot -> new slot, or null
+  long long ncell;
+};
+struct RebuildArgs {
+  RelayoutArgs r;
+  const int32_t *arr;
+  const long long *part;  // the check's per-block sums of the new ranges (generous, tight)
+  long long nblk, nslots;
+  int32_t *base_new, *lay;
+  unsigned long long *flag;
+};
+constexpr int kRestageCells = kRebaseThreads / kCellLanes;
+
+__global__ void __launch_bounds__(kRebaseThreads) k_rebuild1_f32(RebuildArgs a) {
+  if (a.lay[0] == 0) return;
+  if (blockIdx.x > 0) {
+    const RelayoutArgs &r = a.r;
+    const long long c = (long long)(blockIdx.x - 1) * kRestageCells + (threadIdx.x >> 4);
+    const int sub = threadIdx.x & (kCellLanes - 1);
+    if (c >= r.ncell) return;
+    const int b = r.base[c], sm = b + r.nm[c], n = r.ns[c];
+    for (int k = sub; k < n; k += kCellLanes) {
+      r.ma[sm + k] = r.pa_out[b + k];
+      r.mb[sm + k] = r.pb_out[b + k];
+    }
+    return;
+  }
   __shared__ long long sh[kRebaseThreads];
   __shared__ int use_tight;
+  const long long nblk = a.nblk, ncell = a.r.ncell;
   const int t = threadIdx.x;
   const long long per = (nblk + kRebaseThreads - 1) / kRebaseThreads;
   const long long b0 = min((long long)t * per, nblk), b1 = min(b0 + per, nblk);
   long long total = 0, s = 0;
   for (int pass = 0; pass < 2; pass++) {
     s = 0;
-    for (long long b = b0; b < b1; b++) s += part[pass * nblk + b];
+    for (long long b = b0; b < b1; b++) s += a.part[pass * nblk + b];
     sh[t] = s;
     __syncthreads();
     for (int o = 1; o < kRebaseThreads; o <<= 1) {
@@ -600,15 +631,16 @@ __global__ void __launch_bounds__(kRebaseThreads) k_rebase_f32(const int32_t *ns
     total = sh[kRebaseThreads - 1];
     if (t == 0) use_tight = pass;
     __syncthreads();
-    if (total <= nslots) break;
+    if (total <= a.nslots) break;
     __syncthreads();
   }
-  if (total > nslots) {
+  if (total > a.nslots) {
     // (cannot happen: the buffers hold the tight layout of any counts) no
-    // rebuild, so that every later access stays in the current layout
+    // rebuild, so that every later access stays in the current layout (the
+    // other blocks' copies went to free staging slots)
     if (t == 0) {
-      atomicOr(flag, kFlagMigrate);
-      lay[0] = 0;
+      atomicOr(a.flag, kFlagMigrate);
+      a.lay[0] = 0;
     }
     return;
   }
@@ -616,36 +648,12 @@ __global__ void __launch_bounds__(kRebaseThreads) k_rebase_f32(const int32_t *ns
   long long run = sh[t] - s;
   const long long c0 = b0 * kBlock, c1 = min(b1 * kBlock, ncell);
   for (long long c = c0; c < c1; c++) {
-    base_new[c] = (int32_t)run;
-    run += slot_cap(ns[c] + arr[c], tight);
+    a.base_new[c] = (int32_t)run;
+    run += slot_cap(a.r.ns[c] + a.arr[c], tight);
   }
   if (t == kRebaseThreads - 1) {
-    base_new[ncell] = (int32_t)total;
-    lay[1] += 1;
-  }
-}
-
-// Rebuild, step 1 of 2 (runs only when a rebuild is pending): each cell's
-// stayers from the output buffer (old layout) to the staging buffers'
-// free slots behind its movers -- exactly as many as it has stayers, since
-// its range there held stayers + movers.
-struct RelayoutArgs {
-  const int32_t *base, *base_new, *cnt, *ns, *nm, *lay;
-  int4 *pa_out, *pb_out;  // the step's output buffer
-  int4 *ma, *mb;          // staging
-  uint32_t *fwd;          // with a tape: source slot -> new slot, or null
-  long long ncell;
-};
-
-__global__ void __launch_bounds__(kBlock) k_restage_f32(RelayoutArgs a) {
-  if (a.lay[0] == 0) return;
-  const long long c = (long long)blockIdx.x * kTileCells + (threadIdx.x >> 4);
-  const int sub = threadIdx.x & (kCellLanes - 1);
-  if (c >= a.ncell) return;
-  const int b = a.base[c], s = b + a.nm[c], n = a.ns[c];
-  for (int k = sub; k < n; k += kCellLanes) {
-    a.ma[s + k] = a.pa_out[b + k];
-    a.mb[s + k] = a.pb_out[b + k];
+    a.base_new[ncell] = (int32_t)total;
+    a.lay[1] += 1;
   }
 }
 
@@ -849,17 +857,6 @@ __global__ void __launch_bounds__(kArrBlock) k_arrive_f32(ArriveArgs a) {
     a.arr[c] = 0;
     a.arrm[c] = 0;
   }
-}
-
-// The rebuilt layout becomes current (only after a rebuild); the far list
-// is emptied for the next step.
-__global__ void k_commit_f32(int32_t *base, const int32_t *base_new, const int32_t *lay, long long ncell,
-                             int32_t *nfar) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *nfar = 0;
-  if (lay[0] == 0) return;
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c <= ncell;
-       c += (long long)gridDim.x * blockDim.x)
-    base[c] = base_new[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -1434,13 +1431,8 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
                        (const int32_t *)s->arr, (const int32_t *)s->base, g.ncell, s->rpart, nblk, s->lay);
   }
   {
-    TimedLaunch t(c, "rebase_f32");
-    hipLaunchKernelGGL(k_rebase_f32, dim3(1), dim3(kRebaseThreads), 0, c->stream, (const int32_t *)s->ns,
-                       (const int32_t *)s->arr, g.ncell, (const long long *)s->rpart, nblk, (long long)s->nslots,
-                       s->base_new, s->lay, c->dflag);
-  }
-  {
-    RelayoutArgs r;
+    RebuildArgs rb;
+    RelayoutArgs &r = rb.r;
     r.base = s->base;
     r.base_new = s->base_new;
     r.cnt = s->cnt;
@@ -1453,8 +1445,16 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
     r.mb = a.mb;
     r.fwd = fwd;
     r.ncell = g.ncell;
+    rb.arr = s->arr;
+    rb.part = s->rpart;
+    rb.nblk = nblk;
+    rb.nslots = s->nslots;
+    rb.base_new = s->base_new;
+    rb.lay = s->lay;
+    rb.flag = c->dflag;
     TimedLaunch t(c, "relayout_f32");
-    hipLaunchKernelGGL(k_restage_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
+    hipLaunchKernelGGL(k_rebuild1_f32, dim3((unsigned)(1 + (g.ncell + kRestageCells - 1) / kRestageCells)),
+                       dim3(kRebaseThreads), 0, c->stream, rb);
     hipLaunchKernelGGL(k_relayout_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
   }
   {
@@ -1492,11 +1492,6 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
     hipLaunchKernelGGL(k_arrive_f32, dim3(xcd_grid((unsigned)((g.ncell + kArrBlock - 1) / kArrBlock))),
                        dim3(kArrBlock), 0, c->stream, r);
   }
-  {
-    TimedLaunch t(c, "commit_f32");
-    hipLaunchKernelGGL(k_commit_f32, dim3(256), dim3(kBlock), 0, c->stream, s->base, (const int32_t *)s->base_new,
-                       (const int32_t *)s->lay, g.ncell, s->nfar);
-  }
   std::swap(s->pa, s->pa_alt);
   std::swap(s->pb, s->pb_alt);
   DepGridArgs<float> b;
@@ -1508,6 +1503,12 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
   for (int d = 0; d < 3; d++) b.J[d] = s->fs.J[d];
   b.rho = s->fs.rho;
   b.do_rho = 0;
+  // the rebuilt layout becomes current (after a rebuild) and the far list
+  // is emptied, in this launch
+  b.commit_base = s->base;
+  b.commit_new = s->base_new;
+  b.commit_lay = s->lay;
+  b.nfar = s->nfar;
   TimedLaunch t(c, "jsum_f32");
   hipLaunchKernelGGL(k_deposit_grid<float>, dim3(grid_for(g.ntot)), dim3(kBlock), 0, c->stream, b);
   return EMO_OK;

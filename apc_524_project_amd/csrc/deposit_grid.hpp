@@ -19,6 +19,11 @@ struct DepGridArgs {
   long long sj, sr;  // component strides of cj and cr
   T *J[3], *rho;
   int do_rho;
+  // optional (FP32 step): base[c] = base_new[c] for c <= ncell when lay[0]
+  // is set, and *nfar = 0
+  int32_t *commit_base = nullptr;
+  const int32_t *commit_new = nullptr, *commit_lay = nullptr;
+  int32_t *nfar = nullptr;
 };
 
 __device__ inline bool dg_cell_ok(const Geo &g, int i, int j, int k) {
@@ -32,6 +37,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_deposit_grid(DepGridArgs<T> a) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const Geo &g = a.g;
+  if (a.commit_base && p <= g.ncell && a.commit_lay[0] != 0) a.commit_base[p] = a.commit_new[p];
+  if (a.nfar && p == 0) *a.nfar = 0;
   if (p >= g.ntot) return;
   int k = (int)(p % g.nt[2]);
   int j = (int)((p / g.nt[2]) % g.nt[1]);

@@ -65,8 +65,8 @@ KERNELS = {
     "fp64": ["pgd_f64", "far_f64", "rank_f64", "base_f64", "scan_f64", "scatter_f64", "gather_f64", "push_f64",
              "particle_bc_f64", "sort_keys", "sort_radix", "sort_permute", "deposit_cells_f64", "deposit_grid_f64",
              "yee_E_f64", "yee_B_f64"],
-    "fp32": ["cellvec_f32", "push_deposit_f32", "check_f32", "rebase_f32", "relayout_f32",
-             "arrive_f32", "commit_f32", "jsum_f32", "yee_E_f32", "yee_B_f32"],
+    "fp32": ["cellvec_f32", "push_deposit_f32", "check_f32", "relayout_f32", "arrive_f32", "jsum_f32",
+             "yee_E_f32", "yee_B_f32"],
 }
 
 
