@@ -24,39 +24,37 @@
 // Per step, from the record buffers S (before) to D (after):
 //   cellvec_f32       per cell, the 18 values the gather reads (E on the 4
 //                     edges of each component, (B + B_tm1)/2 on the 2 faces)
-//   push_deposit_f32  16 lanes per cell: gather + push + particle BCs of
-//                     every particle, once.  A particle that stays in its
-//                     cell (~93% at 64 particles per cell) is written to D at
-//                     the cell's next free slot (in order) and its current
-//                     is summed as 12 FP64 moments (deposit_moments),
-//                     combined over the lanes in lane order.  A
-//                     particle that leaves is staged in S at the front of its
-//                     source range (slots this pass has already read): its
-//                     new record and its 1/gamma, and is listed in LDS with
-//                     its destination.  At the block's end each group of
-//                     listed movers with one source cell and destination
-//                     takes its arrival numbers there with one integer atomic,
-//                     and each mover writes its staging slot as an arrival
-//                     key at the end of the destination's key range (the
-//                     order of the keys is provisional; their set is not).
+//   push_deposit_f32  16 lanes per cell, 32 cells per block: gather + push +
+//                     particle BCs of every particle, once.  A particle that
+//                     stays in its cell (~93% at 64 particles per cell) is
+//                     written to D at the cell's next free slot (in order)
+//                     and its current is summed as 12 FP64 moments
+//                     (deposit_moments), combined over the lanes in lane
+//                     order.  A particle that leaves is staged in S at the
+//                     front of its source range (slots this pass has already
+//                     read) and adds one to its destination's arrival count
+//                     and its step's bit to the destination's arrival mask.
+//                     Every line written is written whole (runs padded with
+//                     zeros, per-cell outputs gathered in LDS).
 //   check_f32         does every cell's stayers + arrivals fit its range?
-//   rebase_f32,       only when one does not (early exits otherwise): a new
-//   restage_f32,      layout from the new counts, the stayers and the staged
-//   relayout_f32      movers copied to it, the arrival keys re-placed
-//   arrive_f32        4 lanes per cell: the cell's arrivals in ascending order
-//                     of their staging slot (a canonical order, independent
-//                     of the atomics; ranked in LDS for up to 32 arrivals,
-//                     in global memory beyond), records to D after the
-//                     stayers, their currents (v = u * 1/gamma, the push's own
-//                     product) added in that order
-//   commit_f32        the new counts (and a rebuilt layout) become current
-//   jsum_f32          per grid point, the fixed-order sum of the cell vectors
+//   rebuild1_f32,     only when one does not (early exits otherwise): a new
+//   relayout_f32      layout from the new counts, the stayers copied to it
+//                     (the staged movers stay where they are)
+//   arrive_f32        one lane per cell: the cell's arrivals found in the
+//                     staging ranges of the neighbours its mask names, in
+//                     ascending order of their staging slot (a canonical
+//                     order, independent of the atomics; sorted in LDS for
+//                     up to 16 arrivals, in global memory beyond), records to
+//                     D after the stayers, their currents added in that order
+//   jsum_f32          per grid point, the fixed-order sum of the cell
+//                     vectors; a rebuilt layout becomes current
 //   fold, Yee E/B, ghost refresh   shared FP32 instances of the grid kernels
 // Within a cell the new order is the stayers in their previous order, then
 // the arrivals by the slot they were staged in (source cells in ascending
 // order, each in its previous order): deterministic, independent of the
 // headroom, with no atomics on floating-point data.  Particle traffic is
-// 32 B read and 32 B written per particle, plus about 40 B per mover.
+// 32 B read and 32 B written per particle, plus about 64 B per mover and the
+// padding of each run to a whole line.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
