@@ -330,11 +330,20 @@ struct PushDepArgs {
   unsigned long long *flag;
 };
 
-// A change d of a cell index in a dimension of n cells as a neighbour step:
-// -1, 0 or 1, with the periodic wrap (from index 1 to index n is a step of
-// -1) where the dimension's particle BCs are periodic; 2 for a larger move.
-__device__ inline int neighbour_step(int d, int n, bool per) {
-  if (per && n > 2) d = d == n - 1 ? -1 : (d == 1 - n ? 1 : d);
+// A move from cell index i to j in a dimension of n cells as a neighbour
+// step: -1, 0 or 1, with the periodic wrap (from index 1 to index n is a
+// step of -1) where the dimension's particle BCs are periodic; 2 for a
+// larger move.  Where the BCs are periodic, a move from or to a cell with
+// the upper ghost index n + 1 in that dimension is also 2: a particle
+// exactly at the upper boundary stays in that ghost cell (the reference's
+// periodic BC moves P > n, not P = n), and the arrival pass, which wraps
+// source indices into [1, n], would look for it in the cell at index 1.
+__device__ inline int neighbour_step(int i, int j, int n, bool per) {
+  int d = j - i;
+  if (per) {
+    if (i < 1 || i > n || j < 1 || j > n) return 2;
+    if (n > 2) d = d == n - 1 ? -1 : (d == 1 - n ? 1 : d);
+  }
   return (d >= -1 && d <= 1) ? d : 2;
 }
 
@@ -456,9 +465,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       if (stay) deposit_moments(acc, o.f, o.v, jc_s[sp]);
       if (move) {
         // the destination's arrival count and mask (results not read)
-        const int sx = neighbour_step(o.ijk[0] - cix, a.k.g.n[0], per[0]);
-        const int sy = neighbour_step(o.ijk[1] - ciy, a.k.g.n[1], per[1]);
-        const int sz = neighbour_step(o.ijk[2] - ciz, a.k.g.n[2], per[2]);
+        const int sx = neighbour_step(cix, o.ijk[0], a.k.g.n[0], per[0]);
+        const int sy = neighbour_step(ciy, o.ijk[1], a.k.g.n[1], per[1]);
+        const int sz = neighbour_step(ciz, o.ijk[2], a.k.g.n[2], per[2]);
         const bool far = sx == 2 || sy == 2 || sz == 2;
         atomicAdd(&a.arr[o.cell], 1);
         atomicOr(&a.arrm[o.cell], far ? 1 << kFarBit : 1 << (9 * (sx + 1) + 3 * (sy + 1) + (sz + 1)));
@@ -468,7 +477,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
           if (k < a.far_cap)
             a.far[k] = make_int2(o.cell, rs);
           else
-            atomicOr(a.flag, kFlagMigrate);
+            atomicOr(a.flag, kFlagFarList);
         }
       }
       nstay += __popc(gm);
@@ -637,7 +646,7 @@ __global__ void __launch_bounds__(kRebaseThreads) k_rebuild1_f32(RebuildArgs a) 
     // rebuild, so that every later access stays in the current layout (the
     // other blocks' copies went to free staging slots)
     if (t == 0) {
-      atomicOr(a.flag, kFlagMigrate);
+      atomicOr(a.flag, kFlagLayout);
       a.lay[0] = 0;
     }
     return;
@@ -811,7 +820,7 @@ __global__ void __launch_bounds__(kArrBlock) k_arrive_f32(ArriveArgs a) {
       }
     }
     if (nc != na) {  // (the counts and the staged movers disagree: flagged)
-      atomicOr(a.flag, kFlagMigrate);
+      atomicOr(a.flag, kFlagArrivals);
       na = min(na, nc);
     }
     if (!fast) {
@@ -1186,6 +1195,12 @@ int check_flag(emo_ctx *c) {
     return set_err(c, EMO_ERR_INVARIANT, "particle outside the padded grid");
   if (c->hflag[0] & kFlagMigrate)
     return set_err(c, EMO_ERR_INVARIANT, "a moving particle found no slot in its new cell");
+  if (c->hflag[0] & kFlagFarList)
+    return set_err(c, EMO_ERR_INVARIANT, "more particles moved further than one cell in a step than the far list holds");
+  if (c->hflag[0] & kFlagArrivals)
+    return set_err(c, EMO_ERR_INVARIANT, "a cell's arrivals found in its neighbours disagree with its arrival count");
+  if (c->hflag[0] & kFlagLayout)
+    return set_err(c, EMO_ERR_INVARIANT, "the rebuilt slot layout does not fit the particle buffers");
   return EMO_OK;
 }
 

@@ -173,6 +173,14 @@ constexpr unsigned long long kFlagFarMovers = 4ull;
 // FP64 migration sort: a cell's gathered particles disagree with its count;
 // FP32 slot layout: a moving particle found no slot in its new cell
 constexpr unsigned long long kFlagMigrate = 8ull;
+// FP32 step: more particles moved further than one cell than the far list
+// holds
+constexpr unsigned long long kFlagFarList = 16ull;
+// FP32 step: the movers found in a cell's neighbours disagree with the
+// cell's arrival count
+constexpr unsigned long long kFlagArrivals = 32ull;
+// FP32 step: a rebuilt slot layout does not fit the particle buffers
+constexpr unsigned long long kFlagLayout = 64ull;
 
 // ---- plain FP64 particle kernels (plain_kernels.hip) ----
 void launch_gather_f64(emo_ctx *c);

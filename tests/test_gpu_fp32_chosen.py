@@ -368,3 +368,50 @@ def test_fp32_layout_rebuild_when_a_cell_fills():
     assert rel(g["x"], p.x) < 1e-6
     for f in (JX, EX):
         assert rel(ctx2.field(f), o.field(f)) < 1e-3, f
+
+
+def test_fp32_particles_on_the_upper_periodic_boundary():
+    """Particles exactly at x = L, y = L or z = L (and at the corners) sit in
+    the upper ghost cell, since the reference's periodic BC moves only P > n;
+    they leave it inward and through the boundary in the first step: against
+    the oracle, in the documented order.  (Such a move has the step of a wrap
+    move from index 1, and the arrival pass must still find it.)"""
+    n = (6, 5, 4)
+    case = Case(n=n, L=(6.0, 5.0, 4.0), nparticles=640, temp=(0.0, 0.0), dens_phys=2.824e9, relativity=1)
+    rng = np.random.default_rng(11)
+    L = np.array([6.0, 5.0, 4.0])
+    x = rng.uniform(0.0, 1.0, (640, 3)) * L
+    # 16 particles on each upper face, 8 on each of the three upper edges, 8 at the corner
+    k = 0
+    for dims in ((0,), (1,), (2,), (0, 1), (0, 2), (1, 2), (0, 1, 2)):
+        m = 16 if len(dims) == 1 else 8
+        for d in dims:
+            x[k:k + m, d] = L[d]
+        k += m
+    v = rng.uniform(-0.5, 0.5, (640, 3))
+    species = np.repeat(np.array([0, 1], np.int32), 320)
+    gamma = 1.0 / np.sqrt(1.0 - (v ** 2).sum(axis=1))
+    ids = np.arange(640)
+    assert np.any(cell_ids_from_x(case, x[:k]) != cell_ids_from_x(case, x[:k] - 1e-9))
+
+    def make():
+        o = Oracle(case)
+        assert o.load() == 0
+        o.set_particles(x, v, gamma, species, ids)
+        ctx = Context(emo_config(case, precision=FP32_CHOSEN, q=o.q, m=o.m))
+        ctx.load_particles(x, v, gamma, species, ids)
+        ctx.init_constant()
+        o.initialize()
+        return ctx, o
+
+    dt = case.dt_cfl()
+    ctx, _ = make()
+    check_step_order(ctx, dt, 3)
+    ctx2, o = make()
+    run(ctx2, o, 3, dt)
+    g = by_id(ctx2)
+    p = o.particles().by_id()
+    assert rel(g["x"], p.x) < 1e-6
+    assert rel(g["v"], p.v) < 1e-4
+    for f in (JX, JY, JZ):
+        assert rel(ctx2.field(f), o.field(f)) < 1e-3, f
