@@ -92,6 +92,7 @@ struct ChosenState {
   // per step: stayers, movers, arrivals and the arrivals' direction mask
   // per cell; lay[0] = rebuild pending, lay[1] = rebuilds so far
   int32_t *ns = nullptr, *nm = nullptr, *arr = nullptr, *arrm = nullptr;
+  unsigned long long *mcode = nullptr;  // the movers' codes per cell
   long long *rpart = nullptr;  // the check's per-block sums of the new ranges (2 x nblk)
   // movers further than one cell: (destination, staging slot), and their count
   int2 *far = nullptr;
@@ -307,6 +308,13 @@ constexpr int kBlockCells = kTilesPerBlock * kTileCells;  // 32: a 128-byte line
 // neighbour step (dx, dy, dz) (bit 13, no step, is never set), kFarBit for a
 // mover from further away
 constexpr int kFarBit = 27;
+// A cell's mover codes: 3 bits for each of its first kCodedMovers staged
+// movers, in staging order -- 1..6 for the face steps (the mask bits of
+// kFaceBit, in that order), 7 for any other move -- so that the arrival
+// pass reads a face neighbour's movers for its cell without reading the
+// others.
+constexpr int kCodedMovers = 21;
+constexpr int kFaceBit[6] = {4, 10, 12, 14, 16, 22};  // steps -x, -y, -z, +z, +y, +x
 
 struct PushDepArgs {
   PushK k;
@@ -317,6 +325,7 @@ struct PushDepArgs {
   const float *g0;            // the stored gamma of the first push, by source slot, or null
   const int32_t *base, *cnt;  // source layout
   int32_t *ns, *nm;           // stayers and movers per cell
+  unsigned long long *mcode;  // the movers' codes per cell
   int32_t *arr, *arrm;        // arrivals per cell and their direction mask (zero on entry)
   int2 *far;                  // far movers: (destination, staging slot)
   int32_t *nfar;              // far movers listed this step (zero on entry)
@@ -370,6 +379,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   __shared__ float4 cvs[kTileCells][kCellVec];  // the tile's gather vectors
   __shared__ float cj_s[12][kBlockCells];       // the block's per-cell outputs
   __shared__ int ns_s[kBlockCells], nm_s[kBlockCells];
+  __shared__ unsigned long long mc_s[kBlockCells];
   // species constants in LDS (a per-lane load from the argument block would
   // make every iteration wait for all loads in flight); test particles
   // deposit nothing (src/particles/particle_handler.cpp:205)
@@ -377,6 +387,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     qp_s[threadIdx.x] = a.k.s.qp[threadIdx.x];
     jc_s[threadIdx.x] = a.k.s.test[threadIdx.x] ? 0.f : a.k.s.jc[threadIdx.x];
   }
+  if (threadIdx.x < kBlockCells) mc_s[threadIdx.x] = 0ull;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.lay[0] = 0;
   __syncthreads();
   const long long blk = xcd_tile();
@@ -471,6 +482,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
         const bool far = sx == 2 || sy == 2 || sz == 2;
         atomicAdd(&a.arr[o.cell], 1);
         atomicOr(&a.arrm[o.cell], far ? 1 << kFarBit : 1 << (9 * (sx + 1) + 3 * (sy + 1) + (sz + 1)));
+        // the mover's code in its cell's word (an LDS atomic, no wait)
+        const int mi = rs - b;
+        if (mi < kCodedMovers) {
+          const int nz = (sx != 0) + (sy != 0) + (sz != 0);
+          const int code = (far || nz != 1) ? 7 : (sx < 0 ? 1 : sy < 0 ? 2 : sz < 0 ? 3 : sz > 0 ? 4 : sy > 0 ? 5 : 6);
+          atomicOr(&mc_s[lc], (unsigned long long)code << (3 * mi));
+        }
         if (far) {
           // (only a particle moving more than a cell in one step)
           const int k = atomicAdd(a.nfar, 1);
@@ -545,6 +563,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   if (threadIdx.x < ncb) {
     a.ns[c0 + threadIdx.x] = ns_s[threadIdx.x];
     a.nm[c0 + threadIdx.x] = nm_s[threadIdx.x];
+    a.mcode[c0 + threadIdx.x] = mc_s[threadIdx.x];
   }
 }
 
@@ -712,6 +731,7 @@ struct ArriveArgs {
   int dedupe;  // a periodic dimension of at most 2 cells: two steps can reach one neighbour
   const int32_t *base, *base_new, *lay;
   const int32_t *ns, *nm;
+  const unsigned long long *mcode;
   int32_t *arr, *arrm, *cnt;
   int32_t *akey, *akey2;  // more than kArrFast arrivals: their slots, and sorted, at the top of the range
   const int2 *far;
@@ -809,7 +829,23 @@ __global__ void __launch_bounds__(kArrBlock) k_arrive_f32(ArriveArgs a) {
         if (seen) continue;
       }
       const int bs = a.base[s], ms = a.nm[s];
-      for (int j = 0; j < ms; j++)
+      int j0 = 0;
+      // a face step: the source's first kCodedMovers movers by their codes
+      // (distinct steps reach distinct sources when no dimension needs the
+      // dedupe), the rest by their new cell
+      int face = 0;
+#pragma unroll
+      for (int f = 0; f < 6; f++) face = k == kFaceBit[f] ? f + 1 : face;
+      if (face && !a.dedupe) {
+        constexpr unsigned long long kLow = 0x1249249249249249ull;  // bit 3i, i < 21
+        const unsigned long long x = a.mcode[s] ^ (kLow * (unsigned long long)face);
+        unsigned long long hit = ~(x | (x >> 1) | (x >> 2)) & kLow;  // groups equal to face
+        const int coded = min(ms, kCodedMovers);
+        hit &= coded >= kCodedMovers ? ~0ull : (1ull << (3 * coded)) - 1ull;
+        for (; hit != 0ull; hit &= hit - 1ull) add(bs + (__ffsll((long long)hit) - 1) / 3);
+        j0 = kCodedMovers;
+      }
+      for (int j = j0; j < ms; j++)
         if ((a.ma[bs + j].x & kCellMask) == (int)c) add(bs + j);
     }
     if (mask & (1 << kFarBit)) {
@@ -1419,6 +1455,7 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
   a.cnt = s->cnt;
   a.ns = s->ns;
   a.nm = s->nm;
+  a.mcode = s->mcode;
   a.arr = s->arr;
   a.arrm = s->arrm;
   a.far = s->far;
@@ -1482,6 +1519,7 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
     r.lay = s->lay;
     r.ns = s->ns;
     r.nm = s->nm;
+    r.mcode = s->mcode;
     r.arr = s->arr;
     r.arrm = s->arrm;
     r.cnt = s->cnt;
@@ -1555,6 +1593,7 @@ int fp32_create(emo_ctx *c) {
   if (!rc) rc = dalloc(c, &s->cnt, nc1);
   if (!rc) rc = dalloc(c, &s->ns, nc1);
   if (!rc) rc = dalloc(c, &s->nm, nc1);
+  if (!rc) rc = dalloc(c, &s->mcode, nc1);
   if (!rc) rc = dalloc(c, &s->rpart, 2 * (nc1 / kBlock + 1));
   if (!rc) rc = dalloc(c, &s->arr, nc1);
   if (!rc) rc = dalloc(c, &s->arrm, nc1);
@@ -1607,7 +1646,7 @@ void fp32_destroy(emo_ctx *c) {
   ChosenState *s = c->ch;
   if (!s) return;
   void *ps[] = {s->pa, s->pa_alt, s->pb, s->pb_alt, s->akey, s->akey2, s->base, s->base_new, s->cnt,
-                s->ns, s->nm, s->rpart, s->arr, s->arrm, s->far, s->nfar, s->dcount, s->doff, s->lay, s->sink, s->key, s->key_alt, s->perm,
+                s->ns, s->nm, s->mcode, s->rpart, s->arr, s->arrm, s->far, s->nfar, s->dcount, s->doff, s->lay, s->sink, s->key, s->key_alt, s->perm,
                 s->perm_alt, s->sort_tmp, s->scan_tmp, s->cj, s->cv, s->gam_id, s->fs.rho};
   for (void *p : ps) dfree(p);
   if (s->g0_owned) dfree(s->g0);
