@@ -786,91 +786,35 @@ __global__ void __launch_bounds__(kArrBlock) k_arrive_f32(ArriveArgs a) {
 #pragma unroll
   for (int q = 0; q < 12; q++) acc[q] = 0.0;
   if (na > 0) {
-    const bool fast = na <= kArrFast;
-    const int k0 = e1 - na;  // (slow path) the range's top na slots of akey and akey2
-    int *cl = cand[threadIdx.x];
-    int nc = 0;
-    // a candidate's staging slot, kept in ascending order (insertion) in LDS
-    auto add = [&](int slot) {
-      if (nc < na) {
-        if (fast) {
-          int i = nc;
-          while (i > 0 && cl[i - 1] > slot) {
-            cl[i] = cl[i - 1];
-            i--;
-          }
-          cl[i] = slot;
-        } else {
-          a.akey[k0 + nc] = slot;
-        }
-      }
-      nc++;
-    };
     int ix, iy, iz;
     decode_cell(a.g, (int)c, &ix, &iy, &iz);
-    for (int m = mask & ((1 << kFarBit) - 1); m != 0; m &= m - 1) {
-      const int k = __ffs(m) - 1;
-      const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
-      const int sx = source_index(ix, dx, a.g.n[0], a.per[0]);
-      const int sy = source_index(iy, dy, a.g.n[1], a.per[1]);
-      const int sz = source_index(iz, dz, a.g.n[2], a.per[2]);
-      if (sx < 0 || sy < 0 || sz < 0 || sx >= a.g.nc[0] || sy >= a.g.nc[1] || sz >= a.g.nc[2]) continue;
-      const int s = (sx * a.g.nc[1] + sy) * a.g.nc[2] + sz;
-      if (a.dedupe) {
-        // another step of the mask that reaches the same source was scanned
-        bool seen = false;
-        for (int m2 = mask & ((1 << k) - 1); m2 != 0 && !seen; m2 &= m2 - 1) {
-          const int k2 = __ffs(m2) - 1;
-          const int tx = source_index(ix, k2 / 9 - 1, a.g.n[0], a.per[0]);
-          const int ty = source_index(iy, (k2 / 3) % 3 - 1, a.g.n[1], a.per[1]);
-          const int tz = source_index(iz, k2 % 3 - 1, a.g.n[2], a.per[2]);
-          seen = tx == sx && ty == sy && tz == sz;
-        }
-        if (seen) continue;
-      }
-      const int bs = a.base[s], ms = a.nm[s];
-      int j0 = 0;
-      // a face step: the source's first kCodedMovers movers by their codes
-      // (distinct steps reach distinct sources when no dimension needs the
-      // dedupe), the rest by their new cell
+    // the source cell of step k, or -1 outside the padded grid
+    auto source_of = [&](int k) -> int {
+      const int sx = source_index(ix, k / 9 - 1, a.g.n[0], a.per[0] != 0);
+      const int sy = source_index(iy, (k / 3) % 3 - 1, a.g.n[1], a.per[1] != 0);
+      const int sz = source_index(iz, k % 3 - 1, a.g.n[2], a.per[2] != 0);
+      if (sx < 0 || sy < 0 || sz < 0 || sx >= a.g.nc[0] || sy >= a.g.nc[1] || sz >= a.g.nc[2]) return -1;
+      return (sx * a.g.nc[1] + sy) * a.g.nc[2] + sz;
+    };
+    // the face code (1..6) of step k, 0 for another step
+    auto face_of = [&](int k) -> int {
       int face = 0;
 #pragma unroll
       for (int f = 0; f < 6; f++) face = k == kFaceBit[f] ? f + 1 : face;
-      if (face && !a.dedupe) {
-        constexpr unsigned long long kLow = 0x1249249249249249ull;  // bit 3i, i < 21
-        const unsigned long long x = a.mcode[s] ^ (kLow * (unsigned long long)face);
-        unsigned long long hit = ~(x | (x >> 1) | (x >> 2)) & kLow;  // groups equal to face
-        const int coded = min(ms, kCodedMovers);
-        hit &= coded >= kCodedMovers ? ~0ull : (1ull << (3 * coded)) - 1ull;
-        for (; hit != 0ull; hit &= hit - 1ull) add(bs + (__ffsll((long long)hit) - 1) / 3);
-        j0 = kCodedMovers;
-      }
-      for (int j = j0; j < ms; j++)
-        if ((a.ma[bs + j].x & kCellMask) == (int)c) add(bs + j);
-    }
-    if (mask & (1 << kFarBit)) {
-      const int nf = (int)min((long long)*a.nfar, a.far_cap);
-      for (int j = 0; j < nf; j++) {
-        const int2 e = a.far[j];
-        if (e.x == (int)c) add(e.y);
-      }
-    }
-    if (nc != na) {  // (the counts and the staged movers disagree: flagged)
-      atomicOr(a.flag, kFlagArrivals);
-      na = min(na, nc);
-    }
-    if (!fast) {
-      // more arrivals than the LDS list holds: ranked in global memory
-      for (int i = 0; i < na; i++) {
-        const int key = a.akey[k0 + i];
-        int r = 0;
-        for (int j = 0; j < na; j++) r += a.akey[k0 + j] < key ? 1 : 0;
-        a.akey2[k0 + r] = key;
-      }
-    }
-    const int jcr = a.rel;
-    for (int r = 0; r < na; r++) {
-      const int st = fast ? cl[r] : a.akey2[k0 + r];
+      return face;
+    };
+    // bit 3i for each of source s's first kCodedMovers movers with code face
+    auto coded_hits = [&](int s, int ms, int face) -> unsigned long long {
+      constexpr unsigned long long kLow = 0x1249249249249249ull;  // bit 3i, i < 21
+      const unsigned long long x = a.mcode[s] ^ (kLow * (unsigned long long)face);
+      const unsigned long long hit = ~(x | (x >> 1) | (x >> 2)) & kLow;
+      const int coded = min(ms, kCodedMovers);
+      return hit & (coded >= kCodedMovers ? ~0ull : (1ull << (3 * coded)) - 1ull);
+    };
+    // the arrival of rank r: its staged record written after the stayers,
+    // its current (v = u / gamma, gamma from u as the next push computes
+    // it) added to the cell's moments
+    auto take = [&](int st, int r) {
       const int4 A = a.ma[st], Bv = a.mb[st];
       const int pos = s0 + r;
       a.pa_out[pos] = A;
@@ -878,9 +822,118 @@ __global__ void __launch_bounds__(kArrBlock) k_arrive_f32(ArriveArgs a) {
       if (a.fwd) a.fwd[a.mvsrc[st]] = (uint32_t)pos;
       const float f[3] = {as_f(A.y), as_f(A.z), as_f(A.w)};
       const float ux = as_f(Bv.x), uy = as_f(Bv.y), uz = as_f(Bv.z);
-      const float ig = jcr ? __builtin_amdgcn_rsqf(fmaf(ux, ux, fmaf(uy, uy, fmaf(uz, uz, 1.f)))) : 1.f;
+      const float ig = a.rel ? __builtin_amdgcn_rsqf(fmaf(ux, ux, fmaf(uy, uy, fmaf(uz, uz, 1.f)))) : 1.f;
       const float v[3] = {ux * ig, uy * ig, uz * ig};
       deposit_moments(acc, f, v, jc_s[(uint32_t)A.x >> kSpeciesShift]);
+    };
+    int nc = 0;
+    if (!a.dedupe && !(mask & (1 << kFarBit))) {
+      // The sources in ascending cell-id order -- per dimension the three
+      // steps ordered by their (wrapped) source index, the x steps outermost
+      // -- and each source's movers in staging order: the arrivals come out
+      // in ascending staging slot, the canonical order, and are placed as
+      // they are found (a face source's by their codes).
+      int ord[3][3];
+      const int idx[3] = {ix, iy, iz};
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const bool per = a.per[d] != 0;
+        const bool lo = per && idx[d] == 1, hi = per && idx[d] == a.g.n[d];
+        ord[d][0] = lo ? 0 : (hi ? -1 : 1);   // sources i - 1, i, i + 1 in ascending order,
+        ord[d][1] = lo ? -1 : (hi ? 1 : 0);   // the wrapped one (index n or 1) moved to its
+        ord[d][2] = lo ? 1 : (hi ? 0 : -1);   // place (every periodic dimension has n >= 3)
+      }
+      for (int ua = 0; ua < 3; ua++)
+        for (int ub = 0; ub < 3; ub++)
+          for (int uc = 0; uc < 3; uc++) {
+            const int k = 9 * (ord[0][ua] + 1) + 3 * (ord[1][ub] + 1) + (ord[2][uc] + 1);
+            if (!((mask >> k) & 1)) continue;
+            const int s = source_of(k);
+            if (s < 0) continue;
+            const int bs = a.base[s], ms = a.nm[s];
+            const int face = face_of(k);
+            int j0 = 0;
+            if (face) {
+              for (unsigned long long hit = coded_hits(s, ms, face); hit != 0ull; hit &= hit - 1ull) {
+                if (nc < na) take(bs + (__ffsll((long long)hit) - 1) / 3, nc);
+                nc++;
+              }
+              j0 = kCodedMovers;
+            }
+            for (int j = j0; j < ms; j++)
+              if ((a.ma[bs + j].x & kCellMask) == (int)c) {
+                if (nc < na) take(bs + j, nc);
+                nc++;
+              }
+          }
+      if (nc != na) {  // (the counts and the staged movers disagree: flagged)
+        atomicOr(a.flag, kFlagArrivals);
+        na = min(na, nc);
+      }
+    } else {
+      // A far mover (listed by slot) or a periodic dimension of at most 2
+      // cells (where two steps can reach one source): the candidates
+      // collected, sorted by insertion in LDS for up to kArrFast, ranked in
+      // global memory beyond, then taken in ascending order.
+      const bool fast = na <= kArrFast;
+      const int k0 = e1 - na;  // (slow path) the range's top na slots of akey and akey2
+      int *cl = cand[threadIdx.x];
+      auto add = [&](int slot) {
+        if (nc < na) {
+          if (fast) {
+            int i = nc;
+            while (i > 0 && cl[i - 1] > slot) {
+              cl[i] = cl[i - 1];
+              i--;
+            }
+            cl[i] = slot;
+          } else {
+            a.akey[k0 + nc] = slot;
+          }
+        }
+        nc++;
+      };
+      for (int m = mask & ((1 << kFarBit) - 1); m != 0; m &= m - 1) {
+        const int k = __ffs(m) - 1;
+        const int s = source_of(k);
+        if (s < 0) continue;
+        if (a.dedupe) {
+          // another step of the mask that reaches the same source was scanned
+          bool seen = false;
+          for (int m2 = mask & ((1 << k) - 1); m2 != 0 && !seen; m2 &= m2 - 1) seen = source_of(__ffs(m2) - 1) == s;
+          if (seen) continue;
+        }
+        const int bs = a.base[s], ms = a.nm[s];
+        const int face = a.dedupe ? 0 : face_of(k);
+        int j0 = 0;
+        if (face) {
+          for (unsigned long long hit = coded_hits(s, ms, face); hit != 0ull; hit &= hit - 1ull)
+            add(bs + (__ffsll((long long)hit) - 1) / 3);
+          j0 = kCodedMovers;
+        }
+        for (int j = j0; j < ms; j++)
+          if ((a.ma[bs + j].x & kCellMask) == (int)c) add(bs + j);
+      }
+      if (mask & (1 << kFarBit)) {
+        const int nf = (int)min((long long)*a.nfar, a.far_cap);
+        for (int j = 0; j < nf; j++) {
+          const int2 e = a.far[j];
+          if (e.x == (int)c) add(e.y);
+        }
+      }
+      if (nc != na) {  // (the counts and the staged movers disagree: flagged)
+        atomicOr(a.flag, kFlagArrivals);
+        na = min(na, nc);
+      }
+      if (!fast) {
+        for (int i = 0; i < na; i++) {
+          const int key = a.akey[k0 + i];
+          int r = 0;
+          for (int jj = 0; jj < na; jj++) r += a.akey[k0 + jj] < key ? 1 : 0;
+          a.akey2[k0 + r] = key;
+        }
+      }
+      for (int r = 0; r < na; r++) take(fast ? cl[r] : a.akey2[k0 + r], r);
     }
     // zeros to the next 8-slot boundary: a whole last line
     const int4 zero4 = make_int4(0, 0, 0, 0);
