@@ -119,8 +119,41 @@ def run_gpu(mode: str, steps: int, warmup: int, scale: float):
     print(f"[bench] {mode}: {steps} steps in {el:.3f}s -> {rate:.4g} particle-steps/s "
           f"({1e3 * el_ev / nb:.4f} ms/step with events around every launch)", file=sys.stderr, flush=True)
     ctx.close()
+    floor = copy_floor(w.nparticles, 32) if mode == "fp32" else None
     return dict(rate=rate, elapsed=el, ms_per_step=1e3 * el / steps, ktimes=ktimes, workload=w, mode=mode,
-                scale=scale, dominant=dom, ms_per_step_all_events=1e3 * el_ev / nb)
+                scale=scale, dominant=dom, ms_per_step_all_events=1e3 * el_ev / nb, copy_floor=floor)
+
+
+def copy_floor(np_: int, record_bytes: int, reps: int = 20):
+    """A plain device copy of the particle records' bytes (np_ records of
+    record_bytes, as two arrays of 16-byte vectors, read once and written
+    once) timed with events on torch's stream, right after the timed steps:
+    the time a pure stream of the particle pass's own particle bytes takes on
+    this GPU in its current clock state."""
+    import torch
+
+    n = np_ * record_bytes // 16
+    half = n // 2
+    src = [torch.empty((half, 4), dtype=torch.int32, device="cuda") for _ in range(2)]
+    dst = [torch.empty((half, 4), dtype=torch.int32, device="cuda") for _ in range(2)]
+    for t in src:
+        t.fill_(1)
+    for d, s_ in zip(dst, src):
+        d.copy_(s_)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        for d, s_ in zip(dst, src):
+            d.copy_(s_)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 2 * 2 * half * 16
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"what": f"device copy of {np_} x {record_bytes} B records (read + written)", "bytes": int(nbytes),
+            "ms": round(ms, 5), "GBs": round(nbytes / ms / 1e6, 1)}
 
 
 def run_fp64_per_phase(steps: int, warmup: int, scale: float):
@@ -228,7 +261,8 @@ def roofline_for(res):
             "whole_step": {"ms": round(res["ms_per_step"], 5), "achieved": round(b / step_s / 1e9, 1),
                            "frac": round(b / step_s / 1e9 / HBM_PEAK_GBS, 4)},
             "kernel_own_bytes": int(kown) if kown else None,
-            "kernel_own_frac": round(kown / avg_s / 1e9 / HBM_PEAK_GBS, 4) if kown else None}
+            "kernel_own_frac": round(kown / avg_s / 1e9 / HBM_PEAK_GBS, 4) if kown else None,
+            "copy_floor": res.get("copy_floor")}
 
 
 def run_yee(mode: str, name: str, calls: int, scale: float = 1.0):
