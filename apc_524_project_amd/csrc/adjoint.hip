@@ -1946,16 +1946,16 @@ int fp32_adjoint_binomial(emo_ctx *c, int32_t nsteps, double t0, double dt, int 
       std::vector<int32_t> hbase(nc1), hcnt(nc1), hdoff(nc1);
       std::vector<float> adj((size_t)(6 * np)), g0h, g0bh;
       std::vector<int4> rb((size_t)np);
-      EMO_HIP_CHECK_RET(c, hipMemcpy(hbase.data(), s0.base, nc1 * 4, hipMemcpyDeviceToHost));
-      EMO_HIP_CHECK_RET(c, hipMemcpy(hcnt.data(), s0.cnt, nc1 * 4, hipMemcpyDeviceToHost));
-      EMO_HIP_CHECK_RET(c, hipMemcpy(hdoff.data(), s0.doff, nc1 * 4, hipMemcpyDeviceToHost));
-      EMO_HIP_CHECK_RET(c, hipMemcpy(rb.data(), s0.rb, (size_t)np * 16, hipMemcpyDeviceToHost));
-      EMO_HIP_CHECK_RET(c, hipMemcpy(adj.data(), L.adj, (size_t)np * 24, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, hbase.data(), s0.base, nc1 * 4));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, hcnt.data(), s0.cnt, nc1 * 4));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, hdoff.data(), s0.doff, nc1 * 4));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, rb.data(), s0.rb, (size_t)np * 16));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, adj.data(), L.adj, (size_t)np * 24));
       if (L.has_g0) {
         g0h.resize((size_t)v.nslots);
         g0bh.assign((size_t)np, 0.f);
-        EMO_HIP_CHECK_RET(c, hipMemcpy(g0h.data(), L.g0slots, (size_t)v.nslots * 4, hipMemcpyDeviceToHost));
-        if (nsteps > 0) EMO_HIP_CHECK_RET(c, hipMemcpy(g0bh.data(), L.g0b, (size_t)np * 4, hipMemcpyDeviceToHost));
+        EMO_HIP_CHECK_RET(c, ctx_d2h(c, g0h.data(), L.g0slots, (size_t)v.nslots * 4));
+        if (nsteps > 0) EMO_HIP_CHECK_RET(c, ctx_d2h(c, g0bh.data(), L.g0b, (size_t)np * 4));
       }
       for (long long cell = 0; cell < g.ncell; cell++)
         for (int j = 0; j < hcnt[cell]; j++) {
@@ -2239,23 +2239,23 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
       // the initial state's slots: cell c holds cnt[c] particles from base[c]
       const size_t nc1 = (size_t)g.ncell + 1;
       std::vector<int32_t> hbase(nc1), hcnt(nc1);
-      EMO_HIP_CHECK_RET(c, hipMemcpy(hbase.data(), R.ck[0].base, nc1 * 4, hipMemcpyDeviceToHost));
-      EMO_HIP_CHECK_RET(c, hipMemcpy(hcnt.data(), R.ck[0].cnt, nc1 * 4, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, hbase.data(), R.ck[0].base, nc1 * 4));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, hcnt.data(), R.ck[0].cnt, nc1 * 4));
       std::vector<int4> rb(npa);
       std::vector<float> g0h;
       std::vector<float> fb[3], ub[3], g0bh;
-      EMO_HIP_CHECK_RET(c, hipMemcpy(rb.data(), R.ck[0].pb, npa * 16, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, rb.data(), R.ck[0].pb, npa * 16));
       for (int d = 0; d < 3; d++) {
         fb[d].resize(npa);
         ub[d].resize(npa);
-        EMO_HIP_CHECK_RET(c, hipMemcpy(fb[d].data(), R.pa.f[d], npa * 4, hipMemcpyDeviceToHost));
-        EMO_HIP_CHECK_RET(c, hipMemcpy(ub[d].data(), R.pa.u[d], npa * 4, hipMemcpyDeviceToHost));
+        EMO_HIP_CHECK_RET(c, ctx_d2h(c, fb[d].data(), R.pa.f[d], npa * 4));
+        EMO_HIP_CHECK_RET(c, ctx_d2h(c, ub[d].data(), R.pa.u[d], npa * 4));
       }
       if (has_g0) {
         g0h.resize(npa);
         g0bh.resize(npa);
-        EMO_HIP_CHECK_RET(c, hipMemcpy(g0h.data(), R.ck[0].g0, npa * 4, hipMemcpyDeviceToHost));
-        EMO_HIP_CHECK_RET(c, hipMemcpy(g0bh.data(), R.g0b, npa * 4, hipMemcpyDeviceToHost));
+        EMO_HIP_CHECK_RET(c, ctx_d2h(c, g0h.data(), R.ck[0].g0, npa * 4));
+        EMO_HIP_CHECK_RET(c, ctx_d2h(c, g0bh.data(), R.g0b, npa * 4));
       }
       for (long long cell = 0; cell < g.ncell; cell++)
         for (int j = 0; j < hcnt[cell]; j++) {
@@ -2526,9 +2526,9 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     if (!rc && (xbar0 || vbar0 || gbar0) && np > 0) {
       std::vector<int64_t> ids(np);
       std::vector<double> h(np);
-      EMO_HIP_CHECK_RET(c, hipMemcpy(ids.data(), R.ck[0].id, np * 8, hipMemcpyDeviceToHost));
+      EMO_HIP_CHECK_RET(c, ctx_d2h(c, ids.data(), R.ck[0].id, np * 8));
       auto put = [&](double *dst, const double *src, int stride, int comp) -> int {
-        EMO_HIP_CHECK_RET(c, hipMemcpy(h.data(), src, np * 8, hipMemcpyDeviceToHost));
+        EMO_HIP_CHECK_RET(c, ctx_d2h(c, h.data(), src, np * 8));
         for (int64_t i = 0; i < np; i++) {
           int64_t k = ids[i];
           if (k >= 0 && k < np) dst[stride * k + comp] = h[i];

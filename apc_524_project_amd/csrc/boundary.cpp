@@ -179,7 +179,7 @@ int build_waist_profiles(emo_ctx *c) {
         c->err = "hipMalloc failed for the pulse profile";
         return EMO_ERR_ALLOC;
       }
-      if (hipMemcpy(c->wprof[dim][fid], h.data(), np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      if (ctx_h2d(c, c->wprof[dim][fid], h.data(), np * sizeof(double)) != hipSuccess) {
         c->err = "hipMemcpy failed for the pulse profile";
         return EMO_ERR_HIP;
       }

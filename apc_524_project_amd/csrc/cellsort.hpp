@@ -352,9 +352,9 @@ inline int cellsort_alloc(emo_ctx *c, CellSortBufs &b, int64_t cap, long long nc
   if (!rc) rc = al((void **)&b.far_pos, (size_t)kFarCap * 4);
   if (!rc) rc = al((void **)&b.perm, (size_t)cap * 4);
   if (rc) return rc;
-  if (hipMemset(b.nsk, 0, (size_t)kCodes * ncell * 2) != hipSuccess ||
-      hipMemset(b.count, 0, (size_t)(ncell + 1) * 4) != hipSuccess ||
-      hipMemset(b.off, 0, (size_t)(ncell + 1) * 4) != hipSuccess || hipMemset(b.misc, 0, 16) != hipSuccess) {
+  if (ctx_zero(c, b.nsk, (size_t)kCodes * ncell * 2) != hipSuccess ||
+      ctx_zero(c, b.count, (size_t)(ncell + 1) * 4) != hipSuccess ||
+      ctx_zero(c, b.off, (size_t)(ncell + 1) * 4) != hipSuccess || ctx_zero(c, b.misc, 16) != hipSuccess) {
     c->err = "hipMemset failed (cell sort buffers)";
     return EMO_ERR_HIP;
   }

@@ -38,14 +38,17 @@
 //                     zeros, per-cell outputs gathered in LDS).
 //   check_f32         does every cell's stayers + arrivals fit its range?
 //   rebuild1_f32,     only when one does not (early exits otherwise): a new
-//   relayout_f32      layout from the new counts, the stayers copied to it
-//                     (the staged movers stay where they are)
+//   relayout_f32      layout from the new counts (a scan of the check's
+//                     block sums, then one per block of cells), the stayers
+//                     copied to it (the staged movers stay where they are)
 //   arrive_f32        one lane per cell: the cell's arrivals found in the
 //                     staging ranges of the neighbours its mask names, in
 //                     ascending order of their staging slot (a canonical
-//                     order, independent of the atomics; sorted in LDS for
-//                     up to 16 arrivals, in global memory beyond), records to
-//                     D after the stayers, their currents added in that order
+//                     order, independent of the atomics: the sources visited
+//                     in ascending cell id, each source's movers in staging
+//                     order; far movers and periodic dimensions of at most 2
+//                     cells collected and sorted instead), records to D
+//                     after the stayers, their currents added in that order
 //   jsum_f32          per grid point, the fixed-order sum of the cell
 //                     vectors; a rebuilt layout becomes current
 //   fold, Yee E/B, ghost refresh   shared FP32 instances of the grid kernels
@@ -93,7 +96,7 @@ struct ChosenState {
   // per cell; lay[0] = rebuild pending, lay[1] = rebuilds so far
   int32_t *ns = nullptr, *nm = nullptr, *arr = nullptr, *arrm = nullptr;
   unsigned long long *mcode = nullptr;  // the movers' codes per cell
-  long long *rpart = nullptr;  // the check's per-block sums of the new ranges (2 x nblk)
+  long long *rpart = nullptr;  // the check's per-block sums of the new ranges (2 x nblk), then the rebuild's block offsets
   // movers further than one cell: (destination, staging slot), and their count
   int2 *far = nullptr;
   int32_t *nfar = nullptr;
@@ -470,7 +473,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       *da = Ao;
       *db = Bo;
       if (TAPE) {
-        if (stay) a.fwd[src] = (uint32_t)rs;
+        // (a mover's map entry is written by the arrival pass; this one
+        // only completes the line)
+        if (valid) a.fwd[src] = stay ? (uint32_t)rs : 0xffffffffu;
         if (move) a.mvsrc[rs] = src;
       }
       if (stay) deposit_moments(acc, o.f, o.v, jc_s[sp]);
@@ -520,6 +525,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       if (in && em + sub < pm) {
         a.ma[em + sub] = zero4;
         a.mb[em + sub] = zero4;
+      }
+      if (TAPE) {
+        // the map and the movers' sources, to whole 32-byte sectors
+        const int en = b + n, pn = (en + 7) & ~7;
+        if (in && en + sub < pn) a.fwd[en + sub] = 0xffffffffu;
+        if (in && em + sub < pm) a.mvsrc[em + sub] = -1;
       }
     }
     // the cell's sum over its 16 lanes in lane order, in FP64: the lanes' sums
@@ -599,14 +610,16 @@ __global__ void __launch_bounds__(kBlock) k_check_f32(const int32_t *ns, const i
 }
 
 // Rebuild, step 1 of 2 (one launch; every block returns at once unless a
-// rebuild is pending).  Block 0 computes the rebuilt layout: the check's
-// block sums scanned, generous ranges if they fit the buffers, else tight
-// ones (which always fit), then every cell's new base.  The other blocks,
-// 64 cells each, copy each cell's stayers from the output buffer (old
+// rebuild is pending).  Block 0 scans the check's block sums -- generous
+// ranges if they fit the buffers, else tight ones (which always fit) -- into
+// the first new slot of each check block's 256 cells.  The other blocks, 64
+// cells at a time, copy each cell's stayers from the output buffer (old
 // layout) to the staging buffers' free slots behind its movers -- exactly as
 // many as it has stayers, since its range there held stayers + movers.
 struct RelayoutArgs {
-  const int32_t *base, *base_new, *cnt, *ns, *nm, *lay;
+  const int32_t *base, *cnt, *ns, *nm, *arr, *lay;
+  int32_t *base_new;
+  const long long *boff;  // the first new slot of each check block's cells
   int4 *pa_out, *pb_out;  // the step's output buffer
   int4 *ma, *mb;          // staging
   uint32_t *fwd;          // with a tape: source slot -> new slot, or null
@@ -614,25 +627,26 @@ struct RelayoutArgs {
 };
 struct RebuildArgs {
   RelayoutArgs r;
-  const int32_t *arr;
-  const long long *part;  // the check's per-block sums of the new ranges (generous, tight)
+  long long *part;  // the check's per-block sums of the new ranges (generous, tight); then the blocks' offsets
   long long nblk, nslots;
-  int32_t *base_new, *lay;
+  int32_t *lay;     // lay[2]: the rebuilt ranges are tight
   unsigned long long *flag;
 };
 constexpr int kRestageCells = kRebaseThreads / kCellLanes;
+constexpr unsigned kRestageBlocks = 2048;  // at most (the copy loops over the cells)
 
 __global__ void __launch_bounds__(kRebaseThreads) k_rebuild1_f32(RebuildArgs a) {
   if (a.lay[0] == 0) return;
   if (blockIdx.x > 0) {
     const RelayoutArgs &r = a.r;
-    const long long c = (long long)(blockIdx.x - 1) * kRestageCells + (threadIdx.x >> 4);
     const int sub = threadIdx.x & (kCellLanes - 1);
-    if (c >= r.ncell) return;
-    const int b = r.base[c], sm = b + r.nm[c], n = r.ns[c];
-    for (int k = sub; k < n; k += kCellLanes) {
-      r.ma[sm + k] = r.pa_out[b + k];
-      r.mb[sm + k] = r.pb_out[b + k];
+    const long long step = (long long)(gridDim.x - 1) * kRestageCells;
+    for (long long c = (long long)(blockIdx.x - 1) * kRestageCells + (threadIdx.x >> 4); c < r.ncell; c += step) {
+      const int b = r.base[c], sm = b + r.nm[c], n = r.ns[c];
+      for (int k = sub; k < n; k += kCellLanes) {
+        r.ma[sm + k] = r.pa_out[b + k];
+        r.mb[sm + k] = r.pb_out[b + k];
+      }
     }
     return;
   }
@@ -643,7 +657,8 @@ __global__ void __launch_bounds__(kRebaseThreads) k_rebuild1_f32(RebuildArgs a) 
   const long long per = (nblk + kRebaseThreads - 1) / kRebaseThreads;
   const long long b0 = min((long long)t * per, nblk), b1 = min(b0 + per, nblk);
   long long total = 0, s = 0;
-  for (int pass = 0; pass < 2; pass++) {
+  int pass = 0;
+  for (; pass < 2; pass++) {
     s = 0;
     for (long long b = b0; b < b1; b++) s += a.part[pass * nblk + b];
     sh[t] = s;
@@ -670,39 +685,65 @@ __global__ void __launch_bounds__(kRebaseThreads) k_rebuild1_f32(RebuildArgs a) 
     }
     return;
   }
-  const bool tight = use_tight != 0;
+  const int used = use_tight;
   long long run = sh[t] - s;
-  const long long c0 = b0 * kBlock, c1 = min(b1 * kBlock, ncell);
-  for (long long c = c0; c < c1; c++) {
-    a.base_new[c] = (int32_t)run;
-    run += slot_cap(a.r.ns[c] + a.arr[c], tight);
+  for (long long b = b0; b < b1; b++) {
+    a.part[2 * nblk + b] = run;
+    run += a.part[used * nblk + b];
   }
   if (t == kRebaseThreads - 1) {
-    a.base_new[ncell] = (int32_t)total;
+    a.r.base_new[ncell] = (int32_t)total;
     a.lay[1] += 1;
+    a.lay[2] = used;
   }
 }
 
-// Rebuild, step 2 of 2: the stayers to their new slots in the output buffer
-// (with a tape, their map entries moved along).  The movers stay where they
-// are staged (the old layout's ranges), where the arrival pass finds them.
+// Rebuild, step 2 of 2, per check block of 256 cells: each cell's new base
+// (the block's offset plus a scan of its cells' new ranges), then the
+// stayers to their new slots in the output buffer (with a tape, their map
+// entries moved along).  The movers stay where they are staged (the old
+// layout's ranges), where the arrival pass finds them.
 __global__ void __launch_bounds__(kBlock) k_relayout_f32(RelayoutArgs a) {
   if (a.lay[0] == 0) return;
-  const long long c = (long long)blockIdx.x * kTileCells + (threadIdx.x >> 4);
-  const int sub = threadIdx.x & (kCellLanes - 1);
-  if (c >= a.ncell) return;
-  const int b = a.base[c], m = a.nm[c], n = a.ns[c], bn = a.base_new[c];
-  for (int k = sub; k < n; k += kCellLanes) {
-    a.pa_out[bn + k] = a.ma[b + m + k];
-    a.pb_out[bn + k] = a.mb[b + m + k];
+  __shared__ int bn_s[kBlock];
+  __shared__ long long wsum[kBlock / 64];
+  const bool tight = a.lay[2] != 0;
+  const long long c0 = (long long)blockIdx.x * kBlock;
+  {
+    const long long c = c0 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const long long cap = c < a.ncell ? slot_cap(a.ns[c] + a.arr[c], tight) : 0;
+    long long x = cap;  // inclusive scan over the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    long long off = a.boff[blockIdx.x];
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) off += wsum[w];
+    const int bn = (int)(off + x - cap);
+    bn_s[threadIdx.x] = bn;
+    if (c < a.ncell) a.base_new[c] = bn;
+    __syncthreads();
   }
-  if (a.fwd) {
-    // stayers' entries point into [b, b + n); movers' entries are rewritten
-    // by arrive_f32
-    const int cn = a.cnt[c];
-    for (int j = sub; j < cn; j += kCellLanes) {
-      const uint32_t f = a.fwd[b + j];
-      if (f >= (uint32_t)b && f < (uint32_t)(b + n)) a.fwd[b + j] = f - (uint32_t)b + (uint32_t)bn;
+  const int sub = threadIdx.x & (kCellLanes - 1);
+  for (int lc = threadIdx.x >> 4; lc < kBlock; lc += kBlock / kCellLanes) {
+    const long long c = c0 + lc;
+    if (c >= a.ncell) break;
+    const int b = a.base[c], m = a.nm[c], n = a.ns[c], bn = bn_s[lc];
+    for (int k = sub; k < n; k += kCellLanes) {
+      a.pa_out[bn + k] = a.ma[b + m + k];
+      a.pb_out[bn + k] = a.mb[b + m + k];
+    }
+    if (a.fwd) {
+      // stayers' entries point into [b, b + n); movers' entries are
+      // rewritten by arrive_f32
+      const int cn = a.cnt[c];
+      for (int j = sub; j < cn; j += kCellLanes) {
+        const uint32_t f = a.fwd[b + j];
+        if (f >= (uint32_t)b && f < (uint32_t)(b + n)) a.fwd[b + j] = f - (uint32_t)b + (uint32_t)bn;
+      }
     }
   }
 }
@@ -1526,7 +1567,6 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
     TimedLaunch t(c, "push_deposit_f32");
     launch_push_deposit(c, a, dim3(xcd_grid((unsigned)nblocks)));
   }
-  const long long ntiles = (g.ncell + kTileCells - 1) / kTileCells;
   const long long nblk = (g.ncell + kBlock - 1) / kBlock;
   {
     TimedLaunch t(c, "check_f32");
@@ -1537,28 +1577,29 @@ int particle_step(emo_ctx *c, double dt, const Fp32Tape *tape) {
     RebuildArgs rb;
     RelayoutArgs &r = rb.r;
     r.base = s->base;
-    r.base_new = s->base_new;
     r.cnt = s->cnt;
     r.ns = s->ns;
     r.nm = s->nm;
+    r.arr = s->arr;
     r.lay = s->lay;
+    r.base_new = s->base_new;
+    r.boff = s->rpart + 2 * nblk;
     r.pa_out = s->pa_alt;
     r.pb_out = s->pb_alt;
     r.ma = a.ma;
     r.mb = a.mb;
     r.fwd = fwd;
     r.ncell = g.ncell;
-    rb.arr = s->arr;
     rb.part = s->rpart;
     rb.nblk = nblk;
     rb.nslots = s->nslots;
-    rb.base_new = s->base_new;
     rb.lay = s->lay;
     rb.flag = c->dflag;
+    const long long nrestage = (g.ncell + kRestageCells - 1) / kRestageCells;
     TimedLaunch t(c, "relayout_f32");
-    hipLaunchKernelGGL(k_rebuild1_f32, dim3((unsigned)(1 + (g.ncell + kRestageCells - 1) / kRestageCells)),
+    hipLaunchKernelGGL(k_rebuild1_f32, dim3(1u + (unsigned)std::min<long long>(nrestage, kRestageBlocks)),
                        dim3(kRebaseThreads), 0, c->stream, rb);
-    hipLaunchKernelGGL(k_relayout_f32, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, r);
+    hipLaunchKernelGGL(k_relayout_f32, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, r);
   }
   {
     ArriveArgs r;
@@ -1647,7 +1688,7 @@ int fp32_create(emo_ctx *c) {
   if (!rc) rc = dalloc(c, &s->ns, nc1);
   if (!rc) rc = dalloc(c, &s->nm, nc1);
   if (!rc) rc = dalloc(c, &s->mcode, nc1);
-  if (!rc) rc = dalloc(c, &s->rpart, 2 * (nc1 / kBlock + 1));
+  if (!rc) rc = dalloc(c, &s->rpart, 3 * (nc1 / kBlock + 1));
   if (!rc) rc = dalloc(c, &s->arr, nc1);
   if (!rc) rc = dalloc(c, &s->arrm, nc1);
   if (!rc) rc = dalloc(c, &s->far, kFarListCap);
@@ -1674,16 +1715,16 @@ int fp32_create(emo_ctx *c) {
   }
   float *all[13] = {s->fs.E[0], s->fs.E[1], s->fs.E[2], s->fs.Bfix[0], s->fs.Bfix[1], s->fs.Bfix[2],
                     s->Btfix[0], s->Btfix[1], s->Btfix[2], s->fs.J[0], s->fs.J[1], s->fs.J[2], s->fs.rho};
-  for (int q = 0; q < 13; q++) EMO_HIP_CHECK_RET(c, hipMemset(all[q], 0, nt * sizeof(float)));
+  for (int q = 0; q < 13; q++) EMO_HIP_CHECK_RET(c, ctx_zero(c, all[q], nt * sizeof(float)));
   // per-step counters start (and end every step) at zero; an empty layout
   int32_t *zero[] = {s->cnt, s->ns, s->nm, s->arr, s->arrm, s->dcount};
-  for (int32_t *p : zero) EMO_HIP_CHECK_RET(c, hipMemset(p, 0, nc1 * 4));
-  EMO_HIP_CHECK_RET(c, hipMemset(s->nfar, 0, 4));
+  for (int32_t *p : zero) EMO_HIP_CHECK_RET(c, ctx_zero(c, p, nc1 * 4));
+  EMO_HIP_CHECK_RET(c, ctx_zero(c, s->nfar, 4));
   // every arrival key ever read is a slot number (also where a flagged error
   // left a range short)
-  EMO_HIP_CHECK_RET(c, hipMemset(s->akey, 0, (size_t)nslots * 4));
-  EMO_HIP_CHECK_RET(c, hipMemset(s->akey2, 0, (size_t)nslots * 4));
-  EMO_HIP_CHECK_RET(c, hipMemset(s->lay, 0, 16));
+  EMO_HIP_CHECK_RET(c, ctx_zero(c, s->akey, (size_t)nslots * 4));
+  EMO_HIP_CHECK_RET(c, ctx_zero(c, s->akey2, (size_t)nslots * 4));
+  EMO_HIP_CHECK_RET(c, ctx_zero(c, s->lay, 16));
   int bits = 1;
   while ((1ll << bits) < g.ncell) bits++;
   s->sort_bits = bits;
@@ -1767,8 +1808,8 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
     hb[i] = make_int4(ub[0], ub[1], ub[2], pid);
   }
   if (n > 0) {
-    EMO_HIP_CHECK_RET(c, hipMemcpy(s->pa, ha.data(), n * sizeof(int4), hipMemcpyHostToDevice));
-    EMO_HIP_CHECK_RET(c, hipMemcpy(s->pb, hb.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+    EMO_HIP_CHECK_RET(c, ctx_h2d(c, s->pa, ha.data(), n * sizeof(int4)));
+    EMO_HIP_CHECK_RET(c, ctx_h2d(c, s->pb, hb.data(), n * sizeof(int4)));
   }
   s->have_id = id != nullptr;
   drop_g0(s);
@@ -1777,7 +1818,7 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
   if (rel && n > 0) {
     int rc = dalloc(c, &s->g0, n);
     if (rc) return rc;
-    EMO_HIP_CHECK_RET(c, hipMemcpy(s->g0, g0.data(), n * 4, hipMemcpyHostToDevice));
+    EMO_HIP_CHECK_RET(c, ctx_h2d(c, s->g0, g0.data(), n * 4));
     s->g0_active = true;
   }
   if (!rel && gamma && id && n > 0) {
@@ -1791,7 +1832,7 @@ int fp32_load(emo_ctx *c, int64_t n, const double *x3, const double *v3, const d
     if (ok) {
       int rc = dalloc(c, &s->gam_id, n);
       if (rc) return rc;
-      EMO_HIP_CHECK_RET(c, hipMemcpy(s->gam_id, gi.data(), n * 4, hipMemcpyHostToDevice));
+      EMO_HIP_CHECK_RET(c, ctx_h2d(c, s->gam_id, gi.data(), n * 4));
     }
   }
   c->np = n;
@@ -1878,10 +1919,10 @@ int fp32_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_
     }
     if (gamma && !rc) rc = down(gamma, dg, 1, 0);
     for (int d = 0; d < 6 && field6 && !rc; d++) rc = down(field6, df[d], 6, d);
-    if (species && !rc && hipMemcpy(species, dsp, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
+    if (species && !rc && ctx_d2h(c, species, dsp, n * 4) != hipSuccess) rc = EMO_ERR_HIP;
     if (id && !rc) {
       std::vector<int32_t> h(n);
-      if (hipMemcpy(h.data(), did, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMO_ERR_HIP;
+      if (ctx_d2h(c, h.data(), did, n * 4) != hipSuccess) rc = EMO_ERR_HIP;
       for (int64_t i = 0; i < n && !rc; i++) id[i] = s->have_id ? h[i] : -1;
     }
   }
@@ -1991,7 +2032,7 @@ int fp32_set_field(emo_ctx *c, int fid, const double *in) {
   if (fid == EMO_RHO) c->ch->rho_stale = false;
   std::vector<float> h(c->geo.ntot);
   for (long long i = 0; i < c->geo.ntot; i++) h[i] = (float)in[i];
-  EMO_HIP_CHECK_RET(c, hipMemcpy(p, h.data(), c->geo.ntot * 4, hipMemcpyHostToDevice));
+  EMO_HIP_CHECK_RET(c, ctx_h2d(c, p, h.data(), c->geo.ntot * 4));
   return EMO_OK;
 }
 

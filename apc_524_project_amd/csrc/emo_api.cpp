@@ -216,7 +216,7 @@ int emo_create(const emo_config *cfg, emo_ctx **out) {
   rc = 0;
   if (!rc) rc = dalloc(c, &c->dflag, 8);
   if (!rc && hipHostMalloc((void **)&c->hflag, 8 * sizeof(unsigned long long)) != hipSuccess) rc = EMO_ERR_ALLOC;
-  if (!rc) rc = hipMemset(c->dflag, 0, 8 * sizeof(unsigned long long)) == hipSuccess ? 0 : EMO_ERR_HIP;
+  if (!rc) rc = ctx_zero(c, c->dflag, 8 * sizeof(unsigned long long)) == hipSuccess ? 0 : EMO_ERR_HIP;
   c->red_len = 16 * 1024;
   if (!rc) rc = dalloc(c, &c->red, c->red_len);
   if (cfg->precision == EMO_FP64_PLAIN) {
@@ -234,7 +234,7 @@ int emo_create(const emo_config *cfg, emo_ctx **out) {
     double *all[13] = {c->E[0], c->E[1], c->E[2], c->Bfix[0], c->Bfix[1], c->Bfix[2],
                        c->Btfix[0], c->Btfix[1], c->Btfix[2], c->J[0], c->J[1], c->J[2], c->rho};
     for (int q = 0; q < 13 && !rc; q++)
-      if (hipMemset(all[q], 0, nt * sizeof(double)) != hipSuccess) rc = EMO_ERR_HIP;
+      if (ctx_zero(c, all[q], nt * sizeof(double)) != hipSuccess) rc = EMO_ERR_HIP;
     int64_t cap = std::max<int64_t>(cfg->max_particles, 1);
     c->cap = cap;
     if (!rc) rc = alloc_particles(c, c->p, cap);
@@ -334,11 +334,11 @@ int emo_load_particles(emo_ctx *c, int64_t n, const double *x3, const double *v3
   for (int d = 0; d < 3 && !rc; d++) rc = up(c->p.v[d], v3, 3, d);
   if (!rc) rc = up(c->p.gam, gamma, 1, 0);
   if (rc) return rc;
-  EMO_HIP_CHECK_RET(c, hipMemcpy(c->p.sp, species, n * sizeof(int32_t), hipMemcpyHostToDevice));
+  EMO_HIP_CHECK_RET(c, ctx_h2d(c, c->p.sp, species, n * sizeof(int32_t)));
   std::vector<int64_t> ids(n);
   for (int64_t i = 0; i < n; i++) ids[i] = id ? id[i] : i;
-  EMO_HIP_CHECK_RET(c, hipMemcpy(c->p.id, ids.data(), n * sizeof(int64_t), hipMemcpyHostToDevice));
-  for (int d = 0; d < 6; d++) EMO_HIP_CHECK_RET(c, hipMemset(c->p.fld[d], 0, std::max<int64_t>(n, 1) * sizeof(double)));
+  EMO_HIP_CHECK_RET(c, ctx_h2d(c, c->p.id, ids.data(), n * sizeof(int64_t)));
+  for (int d = 0; d < 6; d++) EMO_HIP_CHECK_RET(c, ctx_zero(c, c->p.fld[d], std::max<int64_t>(n, 1) * sizeof(double)));
   c->np = n;
   c->sorted = false;
   c->fld_state = kFldStale;  // zero until the first gather
@@ -432,8 +432,8 @@ int emo_get_particles(emo_ctx *c, double *x3, double *v3, double *gamma, int32_t
     for (int d = 0; d < 6 && !rc; d++) rc = down(field6, c->p.fld[d], 6, d);
   if (rc) return rc;
   EMO_HIP_CHECK_RET(c, hipStreamSynchronize(c->stream));
-  if (species && n) EMO_HIP_CHECK_RET(c, hipMemcpy(species, c->p.sp, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  if (id && n) EMO_HIP_CHECK_RET(c, hipMemcpy(id, c->p.id, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (species && n) EMO_HIP_CHECK_RET(c, ctx_d2h(c, species, c->p.sp, n * sizeof(int32_t)));
+  if (id && n) EMO_HIP_CHECK_RET(c, ctx_d2h(c, id, c->p.id, n * sizeof(int64_t)));
   return EMO_OK;
 }
 

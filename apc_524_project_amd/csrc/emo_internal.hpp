@@ -358,6 +358,22 @@ struct TimedLaunch {
 
 }  // namespace emo
 
+namespace emo {
+// Host copies and fills of context memory, ordered on the context's stream.
+// That stream is non-blocking: a hipMemcpy or hipMemset on the null stream
+// is not ordered with the kernels the context launches (a fill could land
+// after the kernel that reads it).  The copies return once they are done.
+inline hipError_t ctx_h2d(emo_ctx *c, void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+  return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
+inline hipError_t ctx_d2h(emo_ctx *c, void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+  return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
+inline hipError_t ctx_zero(emo_ctx *c, void *p, size_t bytes) { return hipMemsetAsync(p, 0, bytes, c->stream); }
+}  // namespace emo
+
 #define EMO_HIP_CHECK_RET(ctx, expr)                                              \
   do {                                                                            \
     hipError_t e_ = (expr);                                                       \
