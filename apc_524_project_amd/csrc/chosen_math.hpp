@@ -43,11 +43,11 @@ __device__ inline float as_f(int v) { return __int_as_float(v); }
 __device__ inline int as_i(float v) { return __float_as_int(v); }
 
 // ---------------------------------------------------------------------------
-// One particle's gather + push + particle BCs.  Both particle passes call
-// this on the same inputs; the file is compiled without floating-point
-// contraction (every fused multiply-add is an explicit fmaf), so the two calls
-// give the same result bit for bit.  Branch-free: a wavefront's lanes take
-// different sides of the boundary tests.
+// One particle's gather + push + particle BCs.  The file is compiled without
+// floating-point contraction (every fused multiply-add is an explicit fmaf),
+// so the result does not depend on where the function is inlined.
+// Branch-free: a wavefront's lanes take different sides of the boundary
+// tests.
 // ---------------------------------------------------------------------------
 struct PushK {
   Geo g;
@@ -255,10 +255,11 @@ __device__ __forceinline__ void push_particle(const PushK &a, const CellVec &V, 
 // to, the adjoints of its inputs f, u (and of the stored gamma g0 of the
 // first relativistic push) and of the 18 gather-vector entries it read
 // (cvb, the order of CellVec).  The forward is recomputed from the inputs
-// with the same expressions (without the fused multiply-adds, which change
-// only rounding).  Cell changes are piecewise constant: this is the
-// derivative away from cell faces.  jc = 0 for particles that deposit
-// nothing (test particles).
+// with the same expressions and the same single-instruction reciprocals and
+// square roots, with each expression's multiply-adds fused (which changes
+// only rounding, and the same way wherever the function is inlined).  Cell changes are piecewise constant: this is the derivative
+// away from cell faces.  jc = 0 for particles that deposit nothing (test
+// particles).
 // ---------------------------------------------------------------------------
 __device__ inline void cross3(const float a[3], const float b[3], float c[3]) {
   c[0] = a[1] * b[2] - a[2] * b[1];
@@ -272,6 +273,7 @@ __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w,
                                         const float uin[3], float q_p, bool use_g0, float g0, float jc,
                                         const float *const Jb[3], const float fbo[3], const float ubo[3],
                                         float fb[3], float ub[3], float &g0b, float cvb[18]) {
+#pragma clang fp contract(on)
   const Geo &g = a.g;
   int ic[3];
   decode_fast(a, w & kCellMask, ic[0], ic[1], ic[2]);
@@ -290,7 +292,7 @@ __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w,
   float ig = 1.f, v0[3], cr[3], up[3], tau[3], tau2 = 0.f, sigma = 0.f, ustar = 0.f, root = 1.f, igi = 1.f;
   float t[3], s = 1.f, udott = 0.f, wv[3], un[3], sv[3], vm[3], vi[3], D = 1.f, tsqr = 0.f;
   if (REL) {
-    ig = use_g0 ? 1.f / g0 : 1.f / sqrtf(1.f + dot3(u, u));
+    ig = use_g0 ? __builtin_amdgcn_rcpf(g0) : __builtin_amdgcn_rsqf(1.f + dot3(u, u));
     for (int d = 0; d < 3; d++) v0[d] = u[d] * ig;
     cross3(v0, b, cr);
     for (int d = 0; d < 3; d++) {
@@ -300,11 +302,11 @@ __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w,
     tau2 = dot3(tau, tau);
     sigma = 1.f + dot3(up, up) - tau2;
     ustar = dot3(up, tau);
-    root = sqrtf(sigma * sigma + 4.f * (ustar * ustar + tau2));
-    igi = 1.f / sqrtf(0.5f * (sigma + root));
+    root = __builtin_amdgcn_sqrtf(sigma * sigma + 4.f * (ustar * ustar + tau2));
+    igi = __builtin_amdgcn_rsqf(0.5f * (sigma + root));
     for (int d = 0; d < 3; d++) t[d] = tau[d] * igi;
     D = 1.f + dot3(t, t);
-    s = 1.f / D;
+    s = __builtin_amdgcn_rcpf(D);
     udott = ustar * igi;
     float uxt[3];
     cross3(up, t, uxt);
@@ -315,7 +317,7 @@ __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w,
   } else {
     for (int d = 0; d < 3; d++) t[d] = q_p * b[d];
     D = 1.f + q_p * q_p * dot3(b, b);
-    tsqr = 2.f / D;
+    tsqr = 2.f * __builtin_amdgcn_rcpf(D);
     for (int d = 0; d < 3; d++) {
       sv[d] = tsqr * t[d];
       vm[d] = u[d] + q_p * e[d];
@@ -419,7 +421,7 @@ __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w,
     float sigmab = 0.5f * hb;
     const float rootb = 0.5f * hb;
     // root = sqrt(R),  R = sigma^2 + 4 (ustar^2 + tau2)
-    const float Rb = rootb * 0.5f / root;
+    const float Rb = rootb * 0.5f * __builtin_amdgcn_rcpf(root);
     sigmab += 2.f * sigma * Rb;
     ustarb += 8.f * ustar * Rb;
     float tau2b = 4.f * Rb;
@@ -483,7 +485,7 @@ __device__ inline void push_deposit_vjp(const PushK &a, const CellVec &V, int w,
     // sv = tsqr t,  tsqr = 2 / D,  D = 1 + q_p^2 b.b,  t = q_p b
     const float tsqrb = dot3(svb, t);
     for (int d = 0; d < 3; d++) tb[d] += tsqr * svb[d];
-    const float Db = -tsqr / D * tsqrb;
+    const float Db = -tsqr * __builtin_amdgcn_rcpf(D) * tsqrb;
     for (int d = 0; d < 3; d++) bb[d] += 2.f * q_p * q_p * b[d] * Db + q_p * tb[d];
   }
   // ---- gather^T ----
