@@ -2095,7 +2095,9 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
   std::vector<double> tstep(nsteps + 1);
   tstep[0] = t0;
   for (int s = 0; s < nsteps; s++) tstep[s + 1] = tstep[s] + dt;
-  const int last_set = R.nseg > 0 ? (R.nseg - 1) % nsets : 0;
+  // The last two segments (one with a single tape set) are taped on the way,
+  // into the two tape sets, so that the backward recomputes only the others.
+  const int taped_from = std::max(R.nseg - nsets, 0);
   for (int seg = 0; seg < R.nseg && !rc; seg++) {
     snap32_take(c, R.ck[seg], t, true);
     if (seg == 0 && has_g0) {
@@ -2108,17 +2110,18 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
       *v.g0_owned = false;
     }
     const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
-    if (seg == R.nseg - 1) {
-      // the own buffers of the final state's records
+    if (seg >= taped_from) {
+      // the context's own record buffers (the segment's end state is copied
+      // back into them from the tape)
       R.own[0] = *v.pa;
       R.own[1] = *v.pb;
       R.own[2] = *v.pa_alt;
       R.own[3] = *v.pb_alt;
-      rc = recompute32(R, seg, last_set);
+      rc = recompute32(R, seg, seg % nsets);
       if (!rc) {
         CopyBatch cb(c);
-        cb.add(R.own[0], R.tape[last_set][s1 - s0].pa, npa * 16);
-        cb.add(R.own[1], R.tape[last_set][s1 - s0].pb, npa * 16);
+        cb.add(R.own[0], R.tape[seg % nsets][s1 - s0].pa, npa * 16);
+        cb.add(R.own[1], R.tape[seg % nsets][s1 - s0].pb, npa * 16);
         cb.flush();
       }
     } else {
@@ -2184,12 +2187,14 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
   if (!rc && nsteps > 0) {
     std::vector<hipEvent_t> done_r(R.nseg, nullptr), done_a(R.nseg, nullptr);
     done_r[R.nseg - 1] = record(c->stream);
+    for (int seg = taped_from; seg < R.nseg - 1; seg++) done_r[seg] = done_r[R.nseg - 1];
     for (int seg = R.nseg - 1; seg >= 0 && !rc; seg--) {
       rc = wait(R.as, done_r[seg]);
       if (rc) break;
       backward32(R, seg, seg % nsets);
       done_a[seg] = record(R.as);
-      if (seg > 0) {
+      if (seg > 0 && seg - 1 < taped_from) {
+        // into the set the backward of segment seg + 1 read
         if (seg + 1 < R.nseg) rc = wait(c->stream, done_a[seg + 1]);
         if (!rc) rc = recompute32(R, seg - 1, (seg - 1) % nsets);
         if (!rc) done_r[seg - 1] = record(c->stream);
@@ -2402,16 +2407,21 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   // ---- forward with checkpoints -------------------------------------------
   double t = t0;
   std::vector<double> tstep(nsteps + 1);
-  // the last segment is taped on the way (it needs no recomputation)
+  // The last two segments (one with a single tape set) are taped on the way,
+  // into the two tape sets: the backward recomputes only the others.
   const int last0 = nsteps > 0 ? (R.nseg - 1) * K : 0;
   const int last_set = nsteps > 0 ? (R.nseg - 1) % nsets : 0;
+  const int taped_from = std::max(R.nseg - nsets, 0);
   for (int s = 0; s < nsteps; s++) {
+    const int seg = s / K;
     if (s % K == 0) {
-      rc = snap_take(c, R.ck[s / K], t);
+      // the end state of the previous segment, when that one is taped
+      if (seg > 0 && seg - 1 >= taped_from) rc = snap_take(c, R.tape[(seg - 1) % nsets][K], t);
+      if (!rc) rc = snap_take(c, R.ck[seg], t);
       if (rc) break;
     }
     tstep[s] = t;
-    rc = forward_step(c, t, dt, s >= last0 ? &R.tape[last_set][s - last0] : nullptr);
+    rc = forward_step(c, t, dt, seg >= taped_from ? &R.tape[seg % nsets][s - seg * K] : nullptr);
     if (rc) break;
     t += dt;
   }
@@ -2477,12 +2487,13 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   if (!rc && nsteps > 0) {
     std::vector<hipEvent_t> done_r(R.nseg, nullptr), done_a(R.nseg, nullptr);
     done_r[R.nseg - 1] = record(c->stream);  // taped by the forward pass
+    for (int seg = taped_from; seg < R.nseg - 1; seg++) done_r[seg] = done_r[R.nseg - 1];
     for (int seg = R.nseg - 1; seg >= 0 && !rc; seg--) {
       rc = wait(R.as, done_r[seg]);
       if (rc) break;
       backward_segment(R, seg, seg % nsets);
       done_a[seg] = record(R.as);
-      if (seg > 0) {
+      if (seg > 0 && seg - 1 < taped_from) {
         // the set seg - 1 tapes into was last read by segment seg + 1's adjoint
         if (seg + 1 < R.nseg) rc = wait(c->stream, done_a[seg + 1]);
         if (!rc) rc = recompute_segment(R, seg - 1, (seg - 1) % nsets);
