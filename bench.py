@@ -298,8 +298,9 @@ def run_yee(mode: str, name: str, calls: int, scale: float = 1.0):
 def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict | None = None):
     """Workload c4 (64x32x32, x: LightBC pulse + reflecting walls, 8 ppc/species):
     wall time of the forward steps alone vs. the gradient call (the same
-    forward with checkpoints, then the adjoint: the backward sweep with every
-    segment re-run from its checkpoint), and a central-difference check of
+    forward with checkpoints, then the adjoint: the backward sweep with the
+    segments the forward did not tape re-run from their checkpoints), each
+    the median of three alternating runs, and a central-difference check of
     dF/d(pulse amplitude).  adjoint_s = gradient_s - forward_s is the time the
     derivative adds to a forward run.  FP64 plain form, or the FP32 chosen
     form (FP32 particle adjoint, FP64 field adjoints).  The FP32 gradient is
@@ -354,18 +355,22 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
         return d[4] + d[5], el
 
     forward()  # warm-up (first launches, allocations)
-    f0, t_fwd = forward()
     # the first gradient call on a context allocates its checkpoint and tape
-    # pool, which later calls reuse: the timed call is the second one, from
-    # the same initial state (the context reloaded)
+    # pool, which later calls reuse; then three forward runs and three
+    # gradient calls from the same initial state, alternating: the medians
     c = ready()
     c.adjoint_field_energy(n, 0.0, w.dt, k, particles=False)
-    reset(c)
-    a = time.perf_counter()
-    out = c.adjoint_field_energy(n, 0.0, w.dt, k, particles=False)
-    c.synchronize()
-    t_adj = time.perf_counter() - a
+    t_f, t_g = [], []
+    for _ in range(3):
+        f0, el = forward()
+        t_f.append(el)
+        reset(c)
+        a = time.perf_counter()
+        out = c.adjoint_field_energy(n, 0.0, w.dt, k, particles=False)
+        c.synchronize()
+        t_g.append(time.perf_counter() - a)
     c.close()
+    t_fwd, t_adj = sorted(t_f)[1], sorted(t_g)[1]
     amp = float(w.cfg.peakamps[0])
     if fp32:
         fd = fp64_leg["dF_damp_central_diff"] if fp64_leg else None
@@ -378,6 +383,8 @@ def run_adjoint_c4(steps: int | None, k: int, fp32: bool = False, fp64_leg: dict
                     f"{n} steps, checkpoint every {k}",
         "forward_s": round(t_fwd, 4),
         "gradient_s": round(t_adj, 4),
+        "forward_runs_s": [round(t, 4) for t in t_f],
+        "gradient_runs_s": [round(t, 4) for t in t_g],
         "adjoint_s": round(t_adj - t_fwd, 4),
         "adjoint_over_forward": round((t_adj - t_fwd) / t_fwd, 3),
         "gradient_over_forward": round(t_adj / t_fwd, 3),
@@ -479,28 +486,34 @@ def run_adjoint_c5s(steps: int, k: int):
             c.sort()
         c.step(5, 0.0, w.dt)  # warm-up (first launches, allocations)
         c.close()
-        c = ready()
-        if prec == FP64_PLAIN:
-            c.sort()
-        c.synchronize()
-        a = time.perf_counter()
-        c.step(steps, 0.0, w.dt)
-        c.synchronize()
-        t_fwd = time.perf_counter() - a
-        d = c.diag()
-        f_fwd = d[4] + d[5]
-        c.close()
-        # timed without the per-particle outputs (by-id copies to the host of
-        # ~1 GB at this size), as the second gradient call on a context (the
-        # first allocates the checkpoint and tape pool, which later calls
-        # reuse); the gradient itself comes from a third call
-        c = ready()
-        c.adjoint_field_energy(steps, 0.0, w.dt, k, particles=False)
-        reset(c)
-        a = time.perf_counter()
-        c.adjoint_field_energy(steps, 0.0, w.dt, k, particles=False)
-        c.synchronize()
-        t_grad = time.perf_counter() - a
+        # the gradient calls are timed without the per-particle outputs
+        # (by-id copies to the host of ~1 GB at this size) on one context
+        # (its first call allocates the checkpoint and tape pool, which later
+        # calls reuse): three forward runs and three gradient calls from the
+        # same initial state, alternating, and their medians; the gradient
+        # itself comes from a last call
+        cg = ready()
+        cg.adjoint_field_energy(steps, 0.0, w.dt, k, particles=False)
+        t_f, t_g = [], []
+        for _ in range(3):
+            c = ready()
+            if prec == FP64_PLAIN:
+                c.sort()
+            c.synchronize()
+            a = time.perf_counter()
+            c.step(steps, 0.0, w.dt)
+            c.synchronize()
+            t_f.append(time.perf_counter() - a)
+            d = c.diag()
+            f_fwd = d[4] + d[5]
+            c.close()
+            reset(cg)
+            a = time.perf_counter()
+            cg.adjoint_field_energy(steps, 0.0, w.dt, k, particles=False)
+            cg.synchronize()
+            t_g.append(time.perf_counter() - a)
+        t_fwd, t_grad = sorted(t_f)[1], sorted(t_g)[1]
+        c = cg
         reset(c)
         g = c.adjoint_field_energy(steps, 0.0, w.dt, k)
         p = c.particles(fields=False)
@@ -509,6 +522,7 @@ def run_adjoint_c5s(steps: int, k: int):
         sp[p["id"]] = p["species"]
         grads[tag] = g
         out[tag] = {"forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+                    "forward_runs_s": [round(t, 4) for t in t_f], "gradient_runs_s": [round(t, 4) for t in t_g],
                     "adjoint_over_forward": round((t_grad - t_fwd) / t_fwd, 3),
                     "gradient_over_forward": round(t_grad / t_fwd, 3),
                     "energy_matches_forward": bool(g["energy"] == f_fwd), "energy": g["energy"],
