@@ -2254,10 +2254,16 @@ Fp32View fp32_view(emo_ctx *c) {
   return v;
 }
 
+// The error flags are read every kStepCheck steps (one stream
+// synchronisation each), so a call stops within that many steps of an error
+// instead of stepping an invalid state to its end.
+constexpr int kStepCheck = 64;
+
 int fp32_step(emo_ctx *c, int32_t nsteps, double t0, double dt) {
   double t = t0;
   for (int k = 0; k < nsteps; k++) {
     int rc = fp32_step_one(c, t, dt, nullptr);
+    if (!rc && (k + 1) % kStepCheck == 0 && k + 1 < nsteps) rc = fp32_check(c);
     if (rc) return rc;
     t += dt;
   }
