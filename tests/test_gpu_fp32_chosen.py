@@ -370,6 +370,46 @@ def test_fp32_layout_rebuild_when_a_cell_fills():
         assert rel(ctx2.field(f), o.field(f)) < 1e-3, f
 
 
+def test_fp32_layout_rebuild_on_a_grid_of_many_blocks():
+    """The rebuild's scan at a size where each of its 1024 threads sums
+    several of the check's 256-cell blocks (1102 x 18 x 18 padded cells,
+    1395 blocks): a beam slab fills empty cells, the layout is rebuilt, and
+    the particles and fields match the oracle, in the documented order."""
+    n = (1100, 16, 16)
+    npart = 2 * 64 * n[1] * n[2]
+    case = Case(n=n, L=(1100.0, 16.0, 16.0), nparticles=npart, temp=(0.0, 0.0), dens_phys=2.824e9,
+                relativity=0)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.0, 1.0, (npart, 3)) * np.array([1.0, 16.0, 16.0])
+    v = np.zeros((npart, 3))
+    v[:npart // 2, 0] = 0.9
+    species = np.repeat(np.array([0, 1], np.int32), npart // 2)
+    gamma = np.sqrt(1 - (v ** 2).sum(axis=1))
+    ids = np.arange(npart)
+
+    def make():
+        o = Oracle(case)
+        assert o.load() == 0
+        o.set_particles(x, v, gamma, species, ids)
+        ctx = Context(emo_config(case, precision=FP32_CHOSEN, q=o.q, m=o.m))
+        ctx.load_particles(x, v, gamma, species, ids)
+        ctx.init_constant()
+        o.initialize()
+        return ctx, o
+
+    dt = case.dt_cfl()
+    ctx, _ = make()
+    check_step_order(ctx, dt, 8)
+    assert ctx.layout_info()["rebuilds"] > 0
+    ctx2, o = make()
+    run(ctx2, o, 8, dt)
+    g = by_id(ctx2)
+    p = o.particles().by_id()
+    assert rel(g["x"], p.x) < 1e-6
+    for f in (JX, EX):
+        assert rel(ctx2.field(f), o.field(f)) < 1e-3, f
+
+
 def test_fp32_particles_on_the_upper_periodic_boundary():
     """Particles exactly at x = L, y = L or z = L (and at the corners) sit in
     the upper ghost cell, since the reference's periodic BC moves only P > n;
