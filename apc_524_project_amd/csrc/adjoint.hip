@@ -861,12 +861,31 @@ void fields_step_adj(emo_ctx *c, hipStream_t as, AdjFields &af, const std::vecto
 }
 
 // Everything one gradient call needs besides the context.
+
+// Tape sets of the K-step adjoint: two let the backward of one segment run
+// beside the recomputation of the next, and each set holds one segment the
+// forward pass tapes on its way (so the backward does not recompute it).
+// Beyond two, as many as fit in free device memory with 1 GiB to spare, at
+// most kMaxTapeSets.
+constexpr int kMaxTapeSets = 4;
+template <class Layout>
+int more_tape_sets(emo_ctx *c, int nseg, int nsets, Layout &&layout_of) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return nsets;
+  for (int ns = std::min(nseg, kMaxTapeSets); ns > nsets; ns--) {
+    Pool d;
+    layout_of(d, ns);
+    if (d.off + ((size_t)1 << 30) <= fr + c->adj_pool_bytes) return ns;
+  }
+  return nsets;
+}
+
 struct AdjRun {
   emo_ctx *c;
   hipStream_t as;  // the backward sweep's stream; the context's stream recomputes
   double dt;
   int nsteps, K, nseg;
-  std::vector<Snap> ck, tape[2];
+  std::vector<Snap> ck, tape[kMaxTapeSets];
   Snap fin;
   AdjFields af;
   double *dsums = nullptr;  // nsteps x nplanes
@@ -1187,7 +1206,7 @@ struct AdjRun32 {
   hipStream_t as;
   double dt;
   int nsteps, K, nseg;
-  std::vector<Snap32> ck, tape[2];
+  std::vector<Snap32> ck, tape[kMaxTapeSets];
   Snap32 fin;
   AdjFields af;
   double *dsums = nullptr;
@@ -2036,8 +2055,8 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
   R.nseg = (nsteps + K - 1) / K;
   R.planes = fill_planes(c);
   R.ck.resize(std::max(R.nseg, 1));
-  const int nsets = R.nseg > 1 ? 2 : 1;
-  for (int st = 0; st < nsets; st++) R.tape[st].resize(K + 1);
+  int nsets = R.nseg > 1 ? 2 : 1;
+  for (int st = 0; st < kMaxTapeSets; st++) R.tape[st].resize(K + 1);
   const size_t nt = (size_t)g.ntot;
   const size_t npa = (size_t)v.nslots;  // particle arrays are by slot
   const int nplanes = (int)R.planes.size();
@@ -2065,6 +2084,12 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
     P.take(&R.stage_b, npa);
     P.take(&R.mvsrc, npa);
   };
+  nsets = more_tape_sets(c, R.nseg, nsets, [&](Pool &P, int ns) {
+    const int keep = nsets;
+    nsets = ns;
+    layout(P);
+    nsets = keep;
+  });
   {
     Pool dry;
     layout(dry);
@@ -2095,8 +2120,8 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
   std::vector<double> tstep(nsteps + 1);
   tstep[0] = t0;
   for (int s = 0; s < nsteps; s++) tstep[s + 1] = tstep[s] + dt;
-  // The last two segments (one with a single tape set) are taped on the way,
-  // into the two tape sets, so that the backward recomputes only the others.
+  // The last nsets segments are taped on the way, one into each tape set, so
+  // that the backward recomputes only the others.
   const int taped_from = std::max(R.nseg - nsets, 0);
   for (int seg = 0; seg < R.nseg && !rc; seg++) {
     snap32_take(c, R.ck[seg], t, true);
@@ -2194,8 +2219,8 @@ int fp32_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, double dt, 
       backward32(R, seg, seg % nsets);
       done_a[seg] = record(R.as);
       if (seg > 0 && seg - 1 < taped_from) {
-        // into the set the backward of segment seg + 1 read
-        if (seg + 1 < R.nseg) rc = wait(c->stream, done_a[seg + 1]);
+        // into the set the backward of segment seg - 1 + nsets read
+        if (seg - 1 + nsets < R.nseg) rc = wait(c->stream, done_a[seg - 1 + nsets]);
         if (!rc) rc = recompute32(R, seg - 1, (seg - 1) % nsets);
         if (!rc) done_r[seg - 1] = record(c->stream);
       }
@@ -2361,11 +2386,10 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   R.nseg = (nsteps + K - 1) / K;
   R.planes = fill_planes(c);
   R.ck.resize(std::max(R.nseg, 1));
-  R.tape[0].resize(K + 1);
-  R.tape[1].resize(K + 1);
+  for (int st = 0; st < kMaxTapeSets; st++) R.tape[st].resize(K + 1);
   const size_t nt = (size_t)g.ntot;
   const int nplanes = (int)R.planes.size();
-  const int nsets = R.nseg > 1 ? 2 : 1;
+  int nsets = R.nseg > 1 ? 2 : 1;
   auto layout = [&](Pool &P) {
     for (auto &sn : R.ck) snap_alloc(P, c, sn, true, false);
     snap_alloc(P, c, R.fin, true, false, true);
@@ -2386,6 +2410,12 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
     P.take(&R.pb.gam, (size_t)np);
     P.take(&R.pb2.gam, (size_t)np);
   };
+  nsets = more_tape_sets(c, R.nseg, nsets, [&](Pool &P, int ns) {
+    const int keep = nsets;
+    nsets = ns;
+    layout(P);
+    nsets = keep;
+  });
   {
     Pool dry;
     layout(dry);
@@ -2407,8 +2437,8 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
   // ---- forward with checkpoints -------------------------------------------
   double t = t0;
   std::vector<double> tstep(nsteps + 1);
-  // The last two segments (one with a single tape set) are taped on the way,
-  // into the two tape sets: the backward recomputes only the others.
+  // The last nsets segments are taped on the way, one into each tape set:
+  // the backward recomputes only the others.
   const int last0 = nsteps > 0 ? (R.nseg - 1) * K : 0;
   const int last_set = nsteps > 0 ? (R.nseg - 1) % nsets : 0;
   const int taped_from = std::max(R.nseg - nsets, 0);
@@ -2494,8 +2524,8 @@ extern "C" int emo_adjoint_field_energy(emo_ctx *c, int32_t nsteps, double t0, d
       backward_segment(R, seg, seg % nsets);
       done_a[seg] = record(R.as);
       if (seg > 0 && seg - 1 < taped_from) {
-        // the set seg - 1 tapes into was last read by segment seg + 1's adjoint
-        if (seg + 1 < R.nseg) rc = wait(c->stream, done_a[seg + 1]);
+        // the set seg - 1 tapes into was last read by segment seg - 1 + nsets's adjoint
+        if (seg - 1 + nsets < R.nseg) rc = wait(c->stream, done_a[seg - 1 + nsets]);
         if (!rc) rc = recompute_segment(R, seg - 1, (seg - 1) % nsets);
         if (!rc) done_r[seg - 1] = record(c->stream);
       }
