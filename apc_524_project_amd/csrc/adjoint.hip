@@ -867,7 +867,7 @@ void fields_step_adj(emo_ctx *c, hipStream_t as, AdjFields &af, const std::vecto
 // forward pass tapes on its way (so the backward does not recompute it).
 // Beyond two, as many as fit in free device memory with 1 GiB to spare, at
 // most kMaxTapeSets.
-constexpr int kMaxTapeSets = 4;
+constexpr int kMaxTapeSets = 6;
 template <class Layout>
 int more_tape_sets(emo_ctx *c, int nseg, int nsets, Layout &&layout_of) {
   size_t fr = 0, tot = 0;
